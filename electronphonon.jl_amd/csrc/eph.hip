@@ -1,0 +1,256 @@
+// Energy-window filter, phonon post-processing, k-point index maps and the e-ph rotations.
+// These are bandwidth-bound: one thread (or one wave) per point, coalesced along the fastest index.
+#include "common.hpp"
+
+namespace ephip {
+
+struct c32 {
+    float re, im;
+};
+struct c64 {
+    double re, im;
+};
+
+// inside_window + _filter_kpoints (src/filter.jl:19, :40-100): E is (nw, nk) ascending per k.
+// Band indices are 1-based; a k with no in-window band gets keep = 0 and bmin = bmax = 0.
+__global__ void window_filter_kernel(int nw, int64_t nk, const double *__restrict__ E, double wmin, double wmax,
+                                     uint8_t *__restrict__ keep, int64_t *__restrict__ bmin,
+                                     int64_t *__restrict__ bmax, int64_t *__restrict__ nbelow) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    const double *e = E + (int64_t)nw * k;
+    int first = nw + 1, last = 0;
+    for (int i = 0; i < nw; i++)
+        if (e[i] >= wmin) {
+            first = i + 1;
+            break;
+        }
+    for (int i = nw - 1; i >= 0; i--)
+        if (e[i] <= wmax) {
+            last = i + 1;
+            break;
+        }
+    const bool in = first <= last;
+    keep[k] = in ? 1 : 0;
+    bmin[k] = in ? first : 0;
+    bmax[k] = in ? last : 0;
+    if (nbelow) nbelow[k] = first - 1;
+}
+
+// D(q)[i, j] /= sqrt(m_i); /= sqrt(m_j)  (src/compute_states.jl:295-297, src/wannier_to_bloch.jl:199-202)
+__global__ void ph_mass_scale_kernel(int nm, int64_t nq, c64 *__restrict__ D, const double *__restrict__ mass) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nm * nm * nq) return;
+    const int i = (int)(e % nm), j = (int)((e / nm) % nm);
+    c64 v = D[e];
+    const double si = sqrt(mass[i]), sj = sqrt(mass[j]);
+    v.re = v.re / si / sj;
+    v.im = v.im / si / sj;
+    D[e] = v;
+}
+
+// omega = sign(w2) sqrt(|w2|); u[i, :] /= sqrt(m_i)  (src/diagonalize.jl:118-126, src/compute_states.jl:299-300)
+__global__ void ph_finalize_kernel(int nm, int64_t nq, const double *__restrict__ w2, double *__restrict__ omega,
+                                   c32 *__restrict__ U, const double *__restrict__ mass) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < (int64_t)nm * nq) {
+        const double w = w2[e];
+        omega[e] = w >= 0 ? sqrt(w) : -sqrt(-w);
+    }
+    if (U && e < (int64_t)nm * nm * nq) {
+        const int i = (int)(e % nm);
+        const float s = static_cast<float>(sqrt(mass[i]));
+        c32 u = U[e];
+        u.re /= s;
+        u.im /= s;
+        U[e] = u;
+    }
+}
+
+// _grid_coords_reduced + _hash_xk + dense-table _ik_from_hash (src/common/kpoints.jl:614-656):
+// c_d = mod(round(Int, (x_d + dx_d - s_d) n_d), n_d) (ties to even), hash = (c1 n2 + c2) n3 + c3,
+// idx = table[hash] - 1 (table holds 1-based indices, 0 = not a point) or hash itself without a table.
+__global__ void kq_index_kernel(int64_t nk, const double *__restrict__ xk, double dx0, double dx1, double dx2,
+                                int n1, int n2, int n3, double s0, double s1, double s2,
+                                const int64_t *__restrict__ table, int64_t *__restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    auto red = [](double x, double dx, double s, int n) {
+        int64_t c = (int64_t)rint((x + dx - s) * n) % n;
+        return c < 0 ? c + n : c;
+    };
+    const int64_t c1 = red(xk[3 * k], dx0, s0, n1), c2 = red(xk[3 * k + 1], dx1, s1, n2),
+                  c3 = red(xk[3 * k + 2], dx2, s2, n3);
+    const int64_t h = (c1 * n2 + c2) * n3 + c3;
+    out[k] = table ? table[h] - 1 : h;
+}
+
+// RR->Rq phonon-mode rotation (src/wannier_to_bloch.jl:292-297) for a batch of q:
+// out[b, nu', Re, q] = sum_nu g[b, nu, Re, q] uph[nu, nu', q]. One thread per (b, Re, q).
+__global__ void rotate_modes_kernel(int64_t nbas, int nm, int nr_el, int nq, const c32 *__restrict__ g, int64_t ldg,
+                                    const c32 *__restrict__ uph, c32 *__restrict__ out, int64_t ldo) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per_q = nbas * nr_el;
+    if (e >= per_q * nq) return;
+    const int64_t q = e / per_q, rem = e % per_q;
+    const int64_t b = rem % nbas, re = rem / nbas;
+    const c32 *gi = g + ldg * q + b + nbas * nm * re;
+    c32 *go = out + ldo * q + b + nbas * nm * re;
+    const c32 *u = uph + (int64_t)nm * nm * q;
+    c32 in[48];
+    for (int nu = 0; nu < nm; nu++) in[nu] = gi[nbas * nu];
+    for (int nu2 = 0; nu2 < nm; nu2++) {
+        float sr = 0.f, si = 0.f;
+        for (int nu = 0; nu < nm; nu++) {
+            const c32 a = in[nu], w = u[nu + nm * nu2];
+            sr += a.re * w.re - a.im * w.im;
+            si += a.re * w.im + a.im * w.re;
+        }
+        go[nbas * nu2] = c32{sr, si};
+    }
+}
+
+// eph_apply_rotations_rqkq! + g2 fold (src/wannier_to_bloch_batched.jl:360-377, :451-453;
+// CUDA kernel ext/ElectronPhononCUDAExt.jl:218-239) restricted to the energy window, as the outer-q
+// loop's column masks make it (src/calculator/run_eph_over_q_and_k.jl:555-585):
+//   ep[m, n, nu] = sum_ij conj(ukq[i, m]) g[i, j, nu] uk[j, n],  m, n in the in-window bands,
+//   g2 = |ep|^2 / (2 omega_nu).
+// One wave per batch column t; output slots [m - mlo, n - nlo, nu] in an (nbw, nbw, nm) block,
+// zero outside the window.
+__global__ __launch_bounds__(256) void rotate_rqkq_g2_kernel(
+    int nw, int nm, int64_t ncol, const c32 *__restrict__ g, int64_t ldg, const int64_t *__restrict__ ik,
+    const int64_t *__restrict__ ikq, const c32 *__restrict__ U, const int64_t *__restrict__ bmin,
+    const int64_t *__restrict__ bmax, const double *__restrict__ omega, int nbw, float *__restrict__ g2,
+    c32 *__restrict__ ep_out) {
+    extern __shared__ c32 tsh[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t t = (int64_t)blockIdx.x * 4 + wave;
+    if (t >= ncol) return;
+    c32 *T1 = tsh + (int64_t)wave * nw * nbw * nm;  // T1[i, n', nu]
+    const int64_t k = ik[t], kq = ikq[t];
+    const int nlo = (int)bmin[k] - 1, nbk = (int)(bmax[k] - bmin[k] + 1) * (bmin[k] > 0);
+    const int mlo = (int)bmin[kq] - 1, nbkq = (int)(bmax[kq] - bmin[kq] + 1) * (bmin[kq] > 0);
+    // Slots cover nbw bands; a wider window (a caller error) is truncated, never written past LDS.
+    const int nbk_s = min(nbk, nbw), nbkq_s = min(nbkq, nbw);
+    const c32 *gt = g + ldg * t;
+    const c32 *uk = U + (int64_t)nw * nw * k;
+    const c32 *ukq = U + (int64_t)nw * nw * kq;
+    if (nbk_s > 0 && nbkq_s > 0) {
+        for (int p = lane; p < nw * nm; p += 64) {
+            const int i = p % nw, nu = p / nw;
+            for (int n = 0; n < nbk_s; n++) {
+                float sr = 0.f, si = 0.f;
+                for (int j = 0; j < nw; j++) {
+                    const c32 a = gt[i + (int64_t)nw * (j + (int64_t)nw * nu)];
+                    const c32 u = uk[j + (int64_t)nw * (nlo + n)];
+                    sr += a.re * u.re - a.im * u.im;
+                    si += a.re * u.im + a.im * u.re;
+                }
+                T1[i + nw * (n + nbw * nu)] = c32{sr, si};
+            }
+        }
+    }
+    wave_lds_sync();
+    const int nslot = nbw * nbw * nm;
+    for (int p = lane; p < nslot; p += 64) {
+        const int ms = p % nbw, ns = (p / nbw) % nbw, nu = p / (nbw * nbw);
+        float er = 0.f, ei = 0.f;
+        if (ms < nbkq_s && ns < nbk_s) {
+            for (int i = 0; i < nw; i++) {
+                const c32 u = ukq[i + (int64_t)nw * (mlo + ms)];
+                const c32 a = T1[i + nw * (ns + nbw * nu)];
+                er += u.re * a.re + u.im * a.im;
+                ei += u.re * a.im - u.im * a.re;
+            }
+        }
+        const float w = static_cast<float>(omega[nu]);
+        if (g2) g2[t * nslot + p] = (er * er + ei * ei) / (2.f * w);
+        if (ep_out) ep_out[t * nslot + p] = c32{er, ei};
+    }
+}
+
+}  // namespace ephip
+
+using namespace ephip;
+
+static inline unsigned blocks_for(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+extern "C" int ephip_window_filter(int32_t nw, int64_t nk, const double *E, double wmin, double wmax, uint8_t *keep,
+                                   int64_t *bmin, int64_t *bmax, int64_t *nbelow, void *stream) {
+    EPHIP_REQUIRE(nw >= 1 && nk >= 0, "window_filter: bad sizes nw=%d nk=%lld", nw, (long long)nk);
+    if (nk == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(E && keep && bmin && bmax, "window_filter: null device pointer");
+    window_filter_kernel<<<blocks_for(nk), 256, 0, as_stream(stream)>>>(nw, nk, E, wmin, wmax, keep, bmin, bmax,
+                                                                         nbelow);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_ph_mass_scale(int32_t nm, int64_t nq, void *D, const double *mass, void *stream) {
+    EPHIP_REQUIRE(nm >= 1 && nq >= 0, "ph_mass_scale: bad sizes");
+    if (nq == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(D && mass, "ph_mass_scale: null device pointer");
+    ph_mass_scale_kernel<<<blocks_for((int64_t)nm * nm * nq), 256, 0, as_stream(stream)>>>(
+        nm, nq, static_cast<c64 *>(D), mass);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_ph_finalize(int32_t nm, int64_t nq, const double *w2, double *omega, void *U,
+                                 const double *mass, void *stream) {
+    EPHIP_REQUIRE(nm >= 1 && nq >= 0, "ph_finalize: bad sizes");
+    if (nq == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(w2 && omega && mass, "ph_finalize: null device pointer");
+    ph_finalize_kernel<<<blocks_for((int64_t)nm * nm * nq), 256, 0, as_stream(stream)>>>(
+        nm, nq, w2, omega, static_cast<c32 *>(U), mass);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_kq_index(int64_t nk, const double *xk, const double *dx, const int32_t *ngrid,
+                              const double *shift, const int64_t *table, int64_t *out, void *stream) {
+    EPHIP_REQUIRE(nk >= 0, "kq_index: negative size");
+    EPHIP_REQUIRE(ngrid && ngrid[0] > 0 && ngrid[1] > 0 && ngrid[2] > 0, "kq_index: bad grid");
+    if (nk == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(xk && out, "kq_index: null device pointer");
+    const double z[3] = {0.0, 0.0, 0.0};
+    const double *d = dx ? dx : z, *s = shift ? shift : z;
+    kq_index_kernel<<<blocks_for(nk), 256, 0, as_stream(stream)>>>(nk, xk, d[0], d[1], d[2], ngrid[0], ngrid[1],
+                                                                   ngrid[2], s[0], s[1], s[2], table, out);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_eph_rotate_modes(int64_t nbasis, int32_t nm, int32_t nr_el, int32_t nq, const void *g,
+                                      int64_t ldg, const void *uph, void *out, int64_t ldout, void *stream) {
+    EPHIP_REQUIRE(nbasis >= 1 && nm >= 1 && nm <= 48 && nr_el >= 1 && nq >= 0,
+                  "eph_rotate_modes: bad sizes (nm must be <= 48)");
+    EPHIP_REQUIRE(ldg >= nbasis * nm * nr_el && ldout >= nbasis * nm * nr_el, "eph_rotate_modes: bad ld");
+    if (nq == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(g && uph && out && g != out, "eph_rotate_modes: null or aliased device pointer");
+    rotate_modes_kernel<<<blocks_for(nbasis * nr_el * nq), 256, 0, as_stream(stream)>>>(
+        nbasis, nm, nr_el, nq, static_cast<const c32 *>(g), ldg, static_cast<const c32 *>(uph),
+        static_cast<c32 *>(out), ldout);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_eph_rotate_rqkq_g2(int32_t nw, int32_t nm, int64_t ncol, const void *g, int64_t ldg,
+                                        const int64_t *ik, const int64_t *ikq, const void *U, const int64_t *bmin,
+                                        const int64_t *bmax, const double *omega, int32_t nbw, float *g2,
+                                        void *ep, void *stream) {
+    EPHIP_REQUIRE(nw >= 1 && nw <= 64 && nm >= 1 && ncol >= 0,
+                  "eph_rotate_rqkq_g2: bad sizes nw=%d nm=%d ncol=%lld", nw, nm, (long long)ncol);
+    EPHIP_REQUIRE(nbw >= 1 && nbw <= nw, "eph_rotate_rqkq_g2: nbw=%d must be in 1..nw=%d", nbw, nw);
+    EPHIP_REQUIRE(ldg >= (int64_t)nw * nw * nm, "eph_rotate_rqkq_g2: ldg too small");
+    const size_t lds = 4 * sizeof(c32) * (size_t)nw * nbw * nm;
+    EPHIP_REQUIRE(lds <= 64 * 1024, "eph_rotate_rqkq_g2: nw*nbw*nm too large for LDS");
+    if (ncol == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(g && ik && ikq && U && bmin && bmax && omega && (g2 || ep),
+                  "eph_rotate_rqkq_g2: null device pointer");
+    rotate_rqkq_g2_kernel<<<blocks_for(ncol, 4), 256, lds, as_stream(stream)>>>(
+        nw, nm, ncol, static_cast<const c32 *>(g), ldg, ik, ikq, static_cast<const c32 *>(U), bmin, bmax, omega,
+        nbw, g2, static_cast<c32 *>(ep));
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}

@@ -1,0 +1,340 @@
+// Phase-sum Fourier transform on MFMA:  out[d, j] = sum_r op[d, r] * exp(2 pi i R_r . x_j).
+//
+// This is the contraction of the reference's fourier_batched! (src/wannier/batched_fourier_core.jl:
+// 117-139, a ZGEMM of op_r against a phase matrix built by fourier_phase!, :24-30). Here the phase
+// matrix is never stored: each workgroup generates its (R x column) phase tile in LDS from the
+// FP64 R.x argument reduced mod 1, and the complex product runs as a real GEMM on the matrix cores:
+//
+//   [out_re]   [op_re  -op_im] [P_re]        rows (d, re|im) interleaved, K = (r, re|im)
+//   [out_im] = [op_im   op_re] [P_im]
+//
+// so the real A' operand (2 ndata x 2 nr) is formed on the fly from the complex op tile and
+// no flop is wasted (8 real flops per complex multiply-add, as in ZGEMM).
+//
+// Tiles: 256 threads = 4 waves in a 2 x 2 arrangement; a workgroup owns TM = 64 complex rows and
+// TN columns; K advances KC = 16 complex R per LDS stage. FP32 and FP64 use the 16x16x4 MFMA
+// (lane l holds A[l%16][l/16], B[l/16][l%16], C[4(l/16)+r][l%16]); the split-BF16 form uses
+// 16x16x32 bf16 MFMAs with hi/lo operand splitting and three products per real product.
+#include "common.hpp"
+
+namespace ephip {
+
+template <class T>
+struct cplx {
+    T re, im;
+};
+
+template <class T>
+__device__ __forceinline__ typename std::conditional<sizeof(T) == 4, f32x4, f64x4>::type
+mfma16x16x4(T a, T b, typename std::conditional<sizeof(T) == 4, f32x4, f64x4>::type c) {
+    if constexpr (sizeof(T) == 4)
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+constexpr int FT_TM = 64;  // complex rows per workgroup tile
+constexpr int FT_KC = 16;  // complex R per LDS stage
+constexpr int FT_APAD = 8;
+constexpr int FT_BPAD = 16;
+
+template <class T, int TN>
+__global__ __launch_bounds__(256) void ft_mfma_kernel(int64_t ndata, int nr, const cplx<T> *__restrict__ op,
+                                                      int64_t ldop, const int *__restrict__ irvec, int nk,
+                                                      const double *__restrict__ xk, cplx<T> *__restrict__ out,
+                                                      int64_t ldout) {
+    using acc_t = typename std::conditional<sizeof(T) == 4, f32x4, f64x4>::type;
+    constexpr int NCB = TN / 32;  // 16-column blocks per wave
+    __shared__ cplx<T> As[FT_KC][FT_TM + FT_APAD];
+    __shared__ T Bs[FT_KC][2][TN + FT_BPAD];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int64_t d0 = (int64_t)blockIdx.y * FT_TM;
+    const int64_t j0 = (int64_t)blockIdx.x * TN;
+
+    // The phase column this thread generates is fixed for the whole kernel.
+    const int pj = tid % TN;
+    const int prl0 = tid / TN;
+    constexpr int PSTEP = 256 / TN;
+    double x0 = 0, x1 = 0, x2 = 0;
+    const bool pvalid = (j0 + pj) < nk;
+    if (pvalid) {
+        x0 = xk[3 * (j0 + pj)];
+        x1 = xk[3 * (j0 + pj) + 1];
+        x2 = xk[3 * (j0 + pj) + 2];
+    }
+
+    acc_t acc[4][NCB];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < NCB; j++) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    for (int r0 = 0; r0 < nr; r0 += FT_KC) {
+        // op tile: FT_TM x FT_KC complex, coalesced along d
+        for (int e = tid; e < FT_TM * FT_KC; e += 256) {
+            const int dl = e % FT_TM, rl = e / FT_TM;
+            const int64_t d = d0 + dl;
+            const int r = r0 + rl;
+            cplx<T> v{0, 0};
+            if (d < ndata && r < nr) v = op[d + ldop * r];
+            As[rl][dl] = v;
+        }
+        // phase tile: FT_KC x TN
+        for (int rl = prl0; rl < FT_KC; rl += PSTEP) {
+            const int r = r0 + rl;
+            T c = 0, s = 0;
+            if (pvalid && r < nr) {
+                const double t = irvec[3 * r] * x0 + irvec[3 * r + 1] * x1 + irvec[3 * r + 2] * x2;
+                cis2pi(t, c, s);
+            }
+            Bs[rl][0][pj] = c;
+            Bs[rl][1][pj] = s;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < FT_KC / 2; ks++) {
+            const int kk = lane >> 4;
+            const int rl = 2 * ks + (kk >> 1);
+            const int cc = kk & 1;
+            T a[4], b[NCB];
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++) {
+                const int rr = wm * 64 + 16 * rb + (lane & 15);
+                const cplx<T> v = As[rl][rr >> 1];
+                a[rb] = (rr & 1) == 0 ? (cc == 0 ? v.re : -v.im) : (cc == 0 ? v.im : v.re);
+            }
+#pragma unroll
+            for (int cb = 0; cb < NCB; cb++) b[cb] = Bs[rl][cc][wn * (TN / 2) + 16 * cb + (lane & 15)];
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+                for (int cb = 0; cb < NCB; cb++) acc[rb][cb] = mfma16x16x4<T>(a[rb], b[cb], acc[rb][cb]);
+        }
+        __syncthreads();
+    }
+
+    const int q = lane >> 4;
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++) {
+            const int64_t d = d0 + (wm * 64 + 16 * rb) / 2 + 2 * q;
+            const int64_t j = j0 + wn * (TN / 2) + 16 * cb + (lane & 15);
+            if (j < nk) {
+                if (d < ndata) out[d + ldout * j] = cplx<T>{acc[rb][cb][0], acc[rb][cb][1]};
+                if (d + 1 < ndata) out[d + 1 + ldout * j] = cplx<T>{acc[rb][cb][2], acc[rb][cb][3]};
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Split-BF16 form. Every real operand x is written as x_hi + x_lo with x_hi = bf16(x) and
+// x_lo = bf16(x - x_hi); the product keeps hi*hi + hi*lo + lo*hi (the lo*lo term is below FP32
+// resolution of the sum) and accumulates in FP32 on the 16x16x32 bf16 MFMA. One K step covers
+// 32 real K = 16 complex R, i.e. one whole LDS stage. Lane l holds A[l%16][8(l/16)+e] and
+// B[8(l/16)+e][l%16], e = 0..7.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
+    hi = static_cast<__bf16>(x);
+    lo = static_cast<__bf16>(x - static_cast<float>(hi));
+}
+
+template <int TN>
+__global__ __launch_bounds__(256) void ft_bf16x3_kernel(int64_t ndata, int nr, const cplx<float> *__restrict__ op,
+                                                        int64_t ldop, const int *__restrict__ irvec, int nk,
+                                                        const double *__restrict__ xk, cplx<float> *__restrict__ out,
+                                                        int64_t ldout) {
+    constexpr int NCB = TN / 32;
+    constexpr int KR = 2 * FT_KC;  // real K per stage = 32
+    // Pre-split operands, stored row-major along K so each lane reads its 8 K values as 16 bytes.
+    __shared__ __attribute__((aligned(16))) __bf16 Ahi[2 * FT_TM][KR + 8];
+    __shared__ __attribute__((aligned(16))) __bf16 Alo[2 * FT_TM][KR + 8];
+    __shared__ __attribute__((aligned(16))) __bf16 Bhi[TN][KR + 8];
+    __shared__ __attribute__((aligned(16))) __bf16 Blo[TN][KR + 8];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int64_t d0 = (int64_t)blockIdx.y * FT_TM;
+    const int64_t j0 = (int64_t)blockIdx.x * TN;
+
+    const int pj = tid % TN;
+    const int prl0 = tid / TN;
+    constexpr int PSTEP = 256 / TN;
+    double x0 = 0, x1 = 0, x2 = 0;
+    const bool pvalid = (j0 + pj) < nk;
+    if (pvalid) {
+        x0 = xk[3 * (j0 + pj)];
+        x1 = xk[3 * (j0 + pj) + 1];
+        x2 = xk[3 * (j0 + pj) + 2];
+    }
+
+    f32x4 acc[4][NCB];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < NCB; j++) acc[i][j] = f32x4{0, 0, 0, 0};
+
+    for (int r0 = 0; r0 < nr; r0 += FT_KC) {
+        // A' rows (d, re|im), K = (r, re|im): [re -im; im re]
+        for (int e = tid; e < FT_TM * FT_KC; e += 256) {
+            const int dl = e % FT_TM, rl = e / FT_TM;
+            const int64_t d = d0 + dl;
+            const int r = r0 + rl;
+            cplx<float> v{0.f, 0.f};
+            if (d < ndata && r < nr) v = op[d + ldop * r];
+            __bf16 h, l;
+            split_bf16(v.re, h, l);
+            Ahi[2 * dl][2 * rl] = h;     Alo[2 * dl][2 * rl] = l;
+            Ahi[2 * dl + 1][2 * rl + 1] = h; Alo[2 * dl + 1][2 * rl + 1] = l;
+            split_bf16(v.im, h, l);
+            Ahi[2 * dl + 1][2 * rl] = h; Alo[2 * dl + 1][2 * rl] = l;
+            split_bf16(-v.im, h, l);
+            Ahi[2 * dl][2 * rl + 1] = h; Alo[2 * dl][2 * rl + 1] = l;
+        }
+        for (int rl = prl0; rl < FT_KC; rl += PSTEP) {
+            const int r = r0 + rl;
+            float c = 0.f, s = 0.f;
+            if (pvalid && r < nr) {
+                const double t = irvec[3 * r] * x0 + irvec[3 * r + 1] * x1 + irvec[3 * r + 2] * x2;
+                cis2pi(t, c, s);
+            }
+            __bf16 h, l;
+            split_bf16(c, h, l);
+            Bhi[pj][2 * rl] = h; Blo[pj][2 * rl] = l;
+            split_bf16(s, h, l);
+            Bhi[pj][2 * rl + 1] = h; Blo[pj][2 * rl + 1] = l;
+        }
+        __syncthreads();
+        const int kb = 8 * (lane >> 4);
+        bf16x8 ah[4], al[4];
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++) {
+            const int rr = wm * 64 + 16 * rb + (lane & 15);
+            ah[rb] = *reinterpret_cast<const bf16x8 *>(&Ahi[rr][kb]);
+            al[rb] = *reinterpret_cast<const bf16x8 *>(&Alo[rr][kb]);
+        }
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++) {
+            const int jl = wn * (TN / 2) + 16 * cb + (lane & 15);
+            const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&Bhi[jl][kb]);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&Blo[jl][kb]);
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++) {
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rb], bh, acc[rb][cb], 0, 0, 0);
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], bl, acc[rb][cb], 0, 0, 0);
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], bh, acc[rb][cb], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+
+    const int q = lane >> 4;
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++) {
+            const int64_t d = d0 + (wm * 64 + 16 * rb) / 2 + 2 * q;
+            const int64_t j = j0 + wn * (TN / 2) + 16 * cb + (lane & 15);
+            if (j < nk) {
+                if (d < ndata) out[d + ldout * j] = cplx<float>{acc[rb][cb][0], acc[rb][cb][1]};
+                if (d + 1 < ndata) out[d + 1 + ldout * j] = cplx<float>{acc[rb][cb][2], acc[rb][cb][3]};
+            }
+        }
+}
+
+template <class T>
+__global__ void phase_kernel(int nr, const int *__restrict__ irvec, int nk, const double *__restrict__ xk,
+                             cplx<T> *__restrict__ phase) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nr * nk) return;
+    const int r = (int)(e % nr);
+    const int64_t j = e / nr;
+    const double t = irvec[3 * r] * xk[3 * j] + irvec[3 * r + 1] * xk[3 * j + 1] + irvec[3 * r + 2] * xk[3 * j + 2];
+    T c, s;
+    cis2pi(t, c, s);
+    phase[e] = cplx<T>{c, s};
+}
+
+static int check_ft_args(int64_t ndata, int nr, const void *op, int64_t ldop, const int *irvec, int nk,
+                         const double *xk, void *out, int64_t ldout) {
+    EPHIP_REQUIRE(ndata >= 0 && nr >= 0 && nk >= 0, "fourier: negative size (ndata=%lld nr=%d nk=%d)",
+                  (long long)ndata, nr, nk);
+    EPHIP_REQUIRE(ldop >= ndata && ldout >= ndata, "fourier: leading dimension smaller than ndata");
+    EPHIP_REQUIRE(ndata == 0 || nk == 0 || (op && irvec && xk && out) || nr == 0,
+                  "fourier: null device pointer");
+    EPHIP_REQUIRE((ndata + FT_TM - 1) / FT_TM <= 65535, "fourier: ndata=%lld exceeds %lld rows per call",
+                  (long long)ndata, (long long)FT_TM * 65535);
+    return EPHIP_OK;
+}
+
+template <class T>
+static int zero_out(int64_t ndata, int nk, void *out, int64_t ldout, hipStream_t st) {
+    EPHIP_TRY_HIP(hipMemset2DAsync(out, ldout * sizeof(cplx<T>), 0, ndata * sizeof(cplx<T>), nk, st));
+    return EPHIP_OK;
+}
+
+}  // namespace ephip
+
+using namespace ephip;
+
+extern "C" int ephip_fourier_c32(int64_t ndata, int32_t nr, const void *op, int64_t ldop, const int32_t *irvec,
+                                 int32_t nk, const double *xk, void *out, int64_t ldout, int32_t precision,
+                                 void *stream) {
+    int st = check_ft_args(ndata, nr, op, ldop, irvec, nk, xk, out, ldout);
+    if (st) return st;
+    EPHIP_REQUIRE(precision == EPHIP_PREC_FP32 || precision == EPHIP_PREC_BF16X3,
+                  "fourier: unknown precision %d", precision);
+    if (ndata == 0 || nk == 0) return EPHIP_OK;
+    hipStream_t s = as_stream(stream);
+    if (nr == 0) return zero_out<float>(ndata, nk, out, ldout, s);
+    const unsigned gy = (unsigned)((ndata + FT_TM - 1) / FT_TM);
+    auto o = static_cast<const cplx<float> *>(op);
+    auto c = static_cast<cplx<float> *>(out);
+    if (precision == EPHIP_PREC_FP32) {
+        if (nk <= 64) {
+            dim3 g((nk + 63) / 64, gy);
+            ft_mfma_kernel<float, 64><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk, xk, c, ldout);
+        } else {
+            dim3 g((nk + 127) / 128, gy);
+            ft_mfma_kernel<float, 128><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk, xk, c, ldout);
+        }
+    } else {
+        if (nk <= 64) {
+            dim3 g((nk + 63) / 64, gy);
+            ft_bf16x3_kernel<64><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk, xk, c, ldout);
+        } else {
+            dim3 g((nk + 127) / 128, gy);
+            ft_bf16x3_kernel<128><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk, xk, c, ldout);
+        }
+    }
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_fourier_c64(int64_t ndata, int32_t nr, const void *op, int64_t ldop, const int32_t *irvec,
+                                 int32_t nk, const double *xk, void *out, int64_t ldout, void *stream) {
+    int st = check_ft_args(ndata, nr, op, ldop, irvec, nk, xk, out, ldout);
+    if (st) return st;
+    if (ndata == 0 || nk == 0) return EPHIP_OK;
+    hipStream_t s = as_stream(stream);
+    if (nr == 0) return zero_out<double>(ndata, nk, out, ldout, s);
+    dim3 g((nk + 63) / 64, (unsigned)((ndata + FT_TM - 1) / FT_TM));
+    ft_mfma_kernel<double, 64><<<g, 256, 0, s>>>(ndata, nr, static_cast<const cplx<double> *>(op), ldop, irvec,
+                                                 nk, xk, static_cast<cplx<double> *>(out), ldout);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_fourier_phase_c64(int32_t nr, const int32_t *irvec, int32_t nk, const double *xk, void *phase,
+                                       void *stream) {
+    EPHIP_REQUIRE(nr >= 0 && nk >= 0, "fourier_phase: negative size");
+    const int64_t n = (int64_t)nr * nk;
+    if (n == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(irvec && xk && phase, "fourier_phase: null device pointer");
+    phase_kernel<double><<<(unsigned)((n + 255) / 256), 256, 0, as_stream(stream)>>>(
+        nr, irvec, nk, xk, static_cast<cplx<double> *>(phase));
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}

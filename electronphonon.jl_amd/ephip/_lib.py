@@ -1,0 +1,90 @@
+"""ctypes binding of libephip.so (the C-ABI declared in include/ephip.h).
+
+The library is loaded from the package directory (``electronphonon.jl_amd/libephip.so``). There is
+no fallback: if it is missing, every device entry point raises ``EphipError``. Device buffers are
+torch CUDA tensors; their raw pointers and the current stream handle are passed through.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "libephip.so")
+
+EPHIP_OK, EPHIP_ERR_ARG, EPHIP_ERR_HIP, EPHIP_ERR_NOCONV = 0, 1, 2, 3
+PREC_FP32, PREC_BF16X3 = 0, 1
+
+_vp, _i32, _i64, _d = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+
+# name -> argtypes (restype is always int status, except ephip_last_error / ephip_version)
+SIGNATURES = {
+    "ephip_free_bytes": [_vp, _vp],
+    "ephip_synchronize": [_vp],
+    "ephip_fourier_phase_c64": [_i32, _vp, _i32, _vp, _vp, _vp],
+    "ephip_fourier_c32": [_i64, _i32, _vp, _i64, _vp, _i32, _vp, _vp, _i64, _i32, _vp],
+    "ephip_fourier_c64": [_i64, _i32, _vp, _i64, _vp, _i32, _vp, _vp, _i64, _vp],
+    "ephip_heev_c32": [_i32, _i64, _vp, _i32, _vp, _vp, _vp, _vp],
+    "ephip_heev_refine": [_i32, _i64, _vp, _vp, _vp, _vp, _d, _vp],
+    "ephip_window_filter": [_i32, _i64, _vp, _d, _d, _vp, _vp, _vp, _vp, _vp],
+    "ephip_ph_mass_scale": [_i32, _i64, _vp, _vp, _vp],
+    "ephip_ph_finalize": [_i32, _i64, _vp, _vp, _vp, _vp, _vp],
+    "ephip_kq_index": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "ephip_eph_rotate_modes": [_i64, _i32, _i32, _i32, _vp, _i64, _vp, _vp, _i64, _vp],
+    "ephip_eph_rotate_rqkq_g2": [_i32, _i32, _i64, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp,
+                                 _vp, _vp],
+}
+
+
+class EphipError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def load():
+    """Load libephip.so (once). Raises EphipError if the library has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise EphipError(f"libephip.so not found at {LIB_PATH}; build it with __graft_entry__.build() "
+                         "(there is no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    lib.ephip_last_error.restype = C.c_char_p
+    lib.ephip_last_error.argtypes = []
+    lib.ephip_version.restype = C.c_int
+    lib.ephip_version.argtypes = []
+    for name, args in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    _LIB = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    return ["ephip_last_error", "ephip_version", *SIGNATURES]
+
+
+def call(name: str, *args) -> None:
+    lib = load()
+    st = getattr(lib, name)(*args)
+    if st != EPHIP_OK:
+        msg = lib.ephip_last_error().decode(errors="replace")
+        raise EphipError(f"{name}: {msg} (status {st})")
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a torch tensor (None for None)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise EphipError("expected a device (CUDA/HIP) tensor")
+    return t.data_ptr()
+
+
+def stream_handle() -> int | None:
+    import torch
+    return torch.cuda.current_stream().cuda_stream or None

@@ -1,0 +1,271 @@
+"""Device-side mirror of ElectronPhonon.jl's batched Wannier->Bloch API, over libephip.
+
+Names follow the reference (``src/common/gpu_utils.jl``, ``src/wannier/batched_fourier_core.jl``,
+``src/wannier_to_bloch_batched.jl``, ``src/filter.jl``). Device arrays are torch CUDA tensors; every
+computation runs in libephip's HIP kernels (torch only allocates memory and orders the stream).
+
+Layout convention: a column-major array of reference shape ``(a, b, c)`` is stored as a C-ordered
+tensor of shape ``(c, b, a)`` -- the same memory. So ``op_r (ndata x nr)`` is a ``(nr, ndata)``
+tensor, a stack of Hamiltonians ``(nw, nw, nk)`` is ``(nk, nw, nw)`` holding the transposes, etc.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from . import _lib as L
+
+# Degeneracy threshold of the reference, 0.01 meV (src/common/constants.jl:6), in eV.
+ELECTRON_DEGEN_CUTOFF_EV = 0.01e-3
+# Phonon modes below this are skipped by consumers (src/common/constants.jl:5), in eV.
+OMEGA_ACOUSTIC_EV = 0.01e-3
+# Eigenvalue gap below which the FP64 refinement treats eigenpairs as one cluster, in eV.
+CLUSTER_TOL_EV = 1e-4
+
+
+def _torch():
+    import torch
+    return torch
+
+
+# ------------------------------------------------------------------------------------------------
+# Backend primitives (src/common/gpu_utils.jl:16-78)
+# ------------------------------------------------------------------------------------------------
+@dataclasses.dataclass(frozen=True)
+class GPUBackend:
+    device: int = 0
+
+    @property
+    def torch_device(self):
+        return _torch().device("cuda", self.device)
+
+
+def gpu_backend(device: int = 0) -> GPUBackend:
+    """gpu_backend() (src/common/gpu_utils.jl:29; ext/ElectronPhononCUDAExt.jl:34)."""
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise L.EphipError("no GPU visible: the HIP path has no CPU fallback")
+    L.load()
+    return GPUBackend(device)
+
+
+def free_bytes(backend: GPUBackend) -> int:
+    """free_bytes (src/common/gpu_utils.jl:51)."""
+    import ctypes as C
+    f, t = C.c_int64(), C.c_int64()
+    L.call("ephip_free_bytes", C.addressof(f), C.addressof(t))
+    return f.value
+
+
+def synchronize(backend: GPUBackend | None = None) -> None:
+    """synchronize (src/common/gpu_utils.jl:60): drain the current stream, surface async faults."""
+    L.call("ephip_synchronize", L.stream_handle())
+
+
+def alloc(backend: GPUBackend, dtype, *dims_colmajor):
+    """alloc(backend, T, dims...) (src/common/gpu_utils.jl:41-42) for column-major dims."""
+    torch = _torch()
+    return torch.empty(tuple(reversed(dims_colmajor)), dtype=dtype, device=backend.torch_device)
+
+
+def to_device(backend: GPUBackend, x):
+    """to_device (src/common/gpu_utils.jl:39; ext/ElectronPhononCUDAExt.jl:51-54)."""
+    torch = _torch()
+    if isinstance(x, WannierObject):
+        return x.to_device(backend)
+    if isinstance(x, np.ndarray):
+        return torch.from_numpy(np.ascontiguousarray(x)).to(backend.torch_device)
+    return x.to(backend.torch_device)
+
+
+# ------------------------------------------------------------------------------------------------
+# Wannier objects and the phase-sum Fourier transform
+# ------------------------------------------------------------------------------------------------
+class WannierObject:
+    """WannierObject (src/wannier/WannierObject.jl:22-38): ``op_r`` (ndata x nr) with sorted irvec.
+
+    ``op_r`` is a ``(nr, ld)`` array (numpy on the host, torch on the device); only the first
+    ``ndata`` rows of each column are transformed. ``irvec`` is a host ``(nr, 3)`` int array."""
+
+    def __init__(self, irvec, op_r, ndata: int | None = None, irvec_next=None, check: bool = True):
+        irvec = np.ascontiguousarray(irvec, dtype=np.int32)
+        if check:
+            if op_r.shape[0] != irvec.shape[0]:
+                raise ValueError(f"op_r has {op_r.shape[0]} columns but irvec has {irvec.shape[0]} vectors")
+            key = irvec[:, ::-1]
+            if any(tuple(key[i]) > tuple(key[i + 1]) for i in range(len(key) - 1)):
+                raise ValueError("irvec must be sorted by reversed R (check_wannierobject)")
+        self.irvec = irvec
+        self.op_r = op_r
+        self.nr = irvec.shape[0]
+        self.ndata = op_r.shape[1] if ndata is None else int(ndata)
+        self.irvec_next = irvec_next
+        self._irvec_dev = None
+
+    @property
+    def on_device(self) -> bool:
+        return not isinstance(self.op_r, np.ndarray)
+
+    def irvec_dev(self):
+        if self._irvec_dev is None:
+            torch = _torch()
+            self._irvec_dev = torch.from_numpy(self.irvec).to(self.op_r.device)
+        return self._irvec_dev
+
+    def to_device(self, backend: GPUBackend, dtype=None) -> "WannierObject":
+        torch = _torch()
+        op = self.op_r
+        if isinstance(op, np.ndarray):
+            op = torch.from_numpy(np.ascontiguousarray(op))
+        if dtype is not None:
+            op = op.to(dtype)
+        op = op.to(backend.torch_device)
+        return WannierObject(self.irvec, op, self.ndata, self.irvec_next, check=False)
+
+
+def fourier_phase(irvec_dev, xk_dev):
+    """fourier_phase! (src/wannier/batched_fourier_core.jl:24-30): (nk, nr) complex128 tensor
+    ``[j, r] = exp(2 pi i R_r . x_j)``."""
+    torch = _torch()
+    nr, nk = irvec_dev.shape[0], xk_dev.shape[0]
+    out = torch.empty((nk, nr), dtype=torch.complex128, device=xk_dev.device)
+    L.call("ephip_fourier_phase_c64", nr, L.ptr(irvec_dev), nk, L.ptr(xk_dev), L.ptr(out), L.stream_handle())
+    return out
+
+
+def fourier_batched(parent: WannierObject, xk_dev, out=None, precision: int = L.PREC_FP32):
+    """fourier_batched! (src/wannier/batched_fourier_core.jl:117-139): ``(nk, ndata)`` tensor with
+    ``out[j] = op_r[1:ndata, :] * phase[:, j]``. complex64 op -> FP32/split-BF16 MFMA,
+    complex128 op -> FP64 MFMA."""
+    torch = _torch()
+    op = parent.op_r
+    nk = xk_dev.shape[0]
+    if xk_dev.dtype != torch.float64 or not xk_dev.is_contiguous():
+        raise L.EphipError("k-points must be a contiguous float64 (nk, 3) device tensor")
+    if out is None:
+        out = torch.empty((nk, parent.ndata), dtype=op.dtype, device=op.device)
+    ldout = out.stride(0) if nk > 0 else parent.ndata
+    if out.dim() != 2 or out.stride(1) != 1 or out.shape[0] < nk or out.shape[1] < parent.ndata:
+        raise L.EphipError("fourier_batched: out must be (nk, >=ndata) with unit inner stride")
+    if op.dtype == torch.complex64:
+        L.call("ephip_fourier_c32", parent.ndata, parent.nr, L.ptr(op), op.stride(0), L.ptr(parent.irvec_dev()),
+               nk, L.ptr(xk_dev), L.ptr(out), ldout, precision, L.stream_handle())
+    elif op.dtype == torch.complex128:
+        L.call("ephip_fourier_c64", parent.ndata, parent.nr, L.ptr(op), op.stride(0), L.ptr(parent.irvec_dev()),
+               nk, L.ptr(xk_dev), L.ptr(out), ldout, L.stream_handle())
+    else:
+        raise L.EphipError(f"unsupported op_r dtype {op.dtype}")
+    return out
+
+
+class BatchedWannierInterpolator:
+    """BatchedWannierInterpolator (src/wannier/batched_interpolator.jl:39-87): chunks the batched
+    Fourier transform at ``batch_size`` points per call."""
+
+    def __init__(self, parent: WannierObject, batch_size: int = 32768, precision: int = L.PREC_FP32):
+        if not parent.on_device:
+            raise L.EphipError("BatchedWannierInterpolator needs a device WannierObject (use to_device)")
+        self.parent = parent
+        self.batch_size = int(batch_size)
+        self.precision = precision
+
+
+def get_fourier_batched(out, itp: BatchedWannierInterpolator, xk_dev):
+    """get_fourier_batched! (src/wannier/batched_interpolator.jl:200-214)."""
+    nk = xk_dev.shape[0]
+    for s in range(0, nk, itp.batch_size):
+        e = min(nk, s + itp.batch_size)
+        fourier_batched(itp.parent, xk_dev[s:e], out=out[s:e], precision=itp.precision)
+    return out
+
+
+# ------------------------------------------------------------------------------------------------
+# Batched Hermitian eigensolves (src/wannier_to_bloch_batched.jl:30-65)
+# ------------------------------------------------------------------------------------------------
+def _as_stack(Hk):
+    if Hk.dim() == 2:
+        n = int(round(Hk.shape[1] ** 0.5))
+        Hk = Hk.view(Hk.shape[0], n, n)
+    return Hk
+
+
+def eigvals_batched(Hk):
+    """eigvals_batched: ``Hk`` (nk, n, n) (column-major blocks) -> E (nk, n) float32 ascending."""
+    return _heev(Hk, vectors=False)[0]
+
+
+def eigen_batched(Hk):
+    """eigen_batched: -> (E (nk, n) float32, U (nk, n, n) complex64 column-major blocks)."""
+    return _heev(Hk, vectors=True)
+
+
+def _heev(Hk, vectors: bool):
+    torch = _torch()
+    Hk = _as_stack(Hk).contiguous()
+    nb, n = Hk.shape[0], Hk.shape[1]
+    E = torch.empty((nb, n), dtype=torch.float32, device=Hk.device)
+    U = torch.empty((nb, n, n), dtype=torch.complex64, device=Hk.device) if vectors else None
+    info = torch.zeros(nb, dtype=torch.int32, device=Hk.device)
+    is64 = Hk.dtype == torch.complex128
+    if not is64 and Hk.dtype != torch.complex64:
+        raise L.EphipError(f"eigen: unsupported dtype {Hk.dtype}")
+    L.call("ephip_heev_c32", n, nb, L.ptr(Hk), int(is64), L.ptr(U), L.ptr(E), L.ptr(info), L.stream_handle())
+    if nb and int(info.max()) != 0:
+        raise L.EphipError(f"eigen: {int((info != 0).sum())} matrices did not converge")
+    return E, U
+
+
+def eigen_batched_refined(H64, cluster_tol: float = CLUSTER_TOL_EV, vectors: bool = True):
+    """FP32 eigensolve of the FP64 stack ``H64`` followed by the FP64 refinement. Returns
+    (E64 (nk, n) float64, U (nk, n, n) complex64)."""
+    torch = _torch()
+    H64 = _as_stack(H64).contiguous()
+    if H64.dtype != torch.complex128:
+        raise L.EphipError("eigen_batched_refined needs complex128 matrices")
+    E32, U = _heev(H64, vectors=True)
+    nb, n = E32.shape
+    E64 = torch.empty((nb, n), dtype=torch.float64, device=H64.device)
+    L.call("ephip_heev_refine", n, nb, L.ptr(H64), L.ptr(U), L.ptr(E32), L.ptr(E64), float(cluster_tol),
+           L.stream_handle())
+    return E64, U
+
+
+# ------------------------------------------------------------------------------------------------
+# Window filter (src/filter.jl:19, 40-100, 195-225)
+# ------------------------------------------------------------------------------------------------
+@dataclasses.dataclass
+class FilterResult:
+    keep: object        # (nk,) uint8
+    band_min: object    # (nk,) int64, 1-based, 0 if dropped
+    band_max: object
+    nbelow: object      # (nk,) int64 = searchsortedfirst - 1
+
+    def nelec_below_window(self, weights) -> float:
+        return float((self.nbelow.double() * weights).sum())
+
+
+def window_filter(E64, wmin: float, wmax: float) -> FilterResult:
+    torch = _torch()
+    E64 = E64.contiguous()
+    nk, nw = E64.shape
+    dev = E64.device
+    keep = torch.empty(nk, dtype=torch.uint8, device=dev)
+    bmin, bmax, nb = (torch.empty(nk, dtype=torch.int64, device=dev) for _ in range(3))
+    L.call("ephip_window_filter", nw, nk, L.ptr(E64), float(wmin), float(wmax), L.ptr(keep), L.ptr(bmin),
+           L.ptr(bmax), L.ptr(nb), L.stream_handle())
+    return FilterResult(keep, bmin, bmax, nb)
+
+
+def kq_index(xk_dev, dx, ngrid, shift=(0.0, 0.0, 0.0), table=None):
+    """Grid index of x_k + dx (``_hash_xk`` / ``xk_to_ik``, src/common/kpoints.jl:614-656)."""
+    import ctypes as C
+    torch = _torch()
+    nk = xk_dev.shape[0]
+    out = torch.empty(nk, dtype=torch.int64, device=xk_dev.device)
+    d = (C.c_double * 3)(*map(float, dx))
+    g = (C.c_int32 * 3)(*map(int, ngrid))
+    s = (C.c_double * 3)(*map(float, shift))
+    L.call("ephip_kq_index", nk, L.ptr(xk_dev), C.addressof(d), C.addressof(g), C.addressof(s), L.ptr(table),
+           L.ptr(out), L.stream_handle())
+    return out

@@ -1,0 +1,296 @@
+"""Electron/phonon states and the outer-q e-ph loop on the device.
+
+Mirrors the reference's batched outer-q driver ``run_eph_over_q_and_k`` with ``EPDataKBatched``
+payloads (src/calculator/run_eph_over_q_and_k.jl:28-54, :428-613) and the device states branches
+(src/compute_states.jl:135-215, :285-311). The k+q eigenpairs are looked up in the fine-grid states
+(the q-grid divides the k-grid, so every k+q is a fine-grid point) instead of being re-solved per
+(q, k-batch); this gives the same eigen-decomposition as the reference's per-batch solve (its
+gauge is arbitrary too), and makes k and k+q states of one grid point identical.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from . import _lib as L
+from . import core
+from .core import WannierObject
+
+
+def _torch():
+    import torch
+    return torch
+
+
+@dataclasses.dataclass
+class ElectronStates:
+    """Full-band states of a k list: E (nk, nw) float64 ascending (FP64-refined), U (nk, nw, nw)
+    complex64 column-major eigenvector blocks, and the window filter."""
+    xk: object
+    E: object
+    U: object
+    filt: core.FilterResult | None
+    window: tuple[float, float] | None
+
+    @property
+    def nw(self) -> int:
+        return self.E.shape[1]
+
+
+def compute_electron_states(ham_dev: WannierObject, xk_dev, window=None, chunk: int = 32768) -> ElectronStates:
+    """H(R) -> H(k) in FP64 on the matrix cores, FP32 eigensolve, FP64 refinement, window filter.
+    Device counterpart of compute_electron_states + filter_electron_states
+    (src/compute_states.jl:135-178, src/filter.jl:46-72)."""
+    torch = _torch()
+    nk = xk_dev.shape[0]
+    nw = int(round(math.sqrt(ham_dev.ndata)))
+    if nw * nw != ham_dev.ndata:
+        raise L.EphipError(f"ndata={ham_dev.ndata} is not a perfect square")
+    dev = xk_dev.device
+    E = torch.empty((nk, nw), dtype=torch.float64, device=dev)
+    U = torch.empty((nk, nw, nw), dtype=torch.complex64, device=dev)
+    H64 = torch.empty((min(chunk, max(nk, 1)), nw * nw), dtype=torch.complex128, device=dev)
+    for s in range(0, nk, chunk):
+        e = min(nk, s + chunk)
+        h = H64[: e - s]
+        core.fourier_batched(ham_dev, xk_dev[s:e], out=h)
+        Ec, Uc = core.eigen_batched_refined(h)
+        E[s:e] = Ec
+        U[s:e] = Uc
+    filt = core.window_filter(E, window[0], window[1]) if window is not None else None
+    return ElectronStates(xk_dev, E, U, filt, window)
+
+
+@dataclasses.dataclass
+class PhononStates:
+    xq: object
+    omega: object   # (nq, nm) float64, sign(w2) sqrt(|w2|)
+    u: object       # (nq, nm, nm) complex64 column-major, rows divided by sqrt(mass)
+
+
+def compute_phonon_states(dyn_dev: WannierObject, mass, xq_dev) -> PhononStates:
+    """Device phonon states (src/compute_states.jl:285-311): D(q) in FP64, mass scaling, eigensolve
+    with FP64 refinement, omega = sign(w2) sqrt(|w2|), u /= sqrt(m)."""
+    torch = _torch()
+    nq = xq_dev.shape[0]
+    nm = int(round(math.sqrt(dyn_dev.ndata)))
+    dev = xq_dev.device
+    mass_d = torch.as_tensor(np.asarray(mass, dtype=np.float64), device=dev)
+    D = core.fourier_batched(dyn_dev, xq_dev)
+    L.call("ephip_ph_mass_scale", nm, nq, L.ptr(D), L.ptr(mass_d), L.stream_handle())
+    w2, U = core.eigen_batched_refined(D, cluster_tol=1e-12)
+    omega = torch.empty_like(w2)
+    L.call("ephip_ph_finalize", nm, nq, L.ptr(w2), L.ptr(omega), L.ptr(U), L.ptr(mass_d), L.stream_handle())
+    return PhononStates(xq_dev, omega, U)
+
+
+def get_eph_RR_to_Rq_batched(epmat_dev: WannierObject, nw: int, nm: int, xq_dev, uph, precision=L.PREC_FP32,
+                             work=None, out=None):
+    """get_eph_RR_to_Rq! (src/wannier_to_bloch.jl:277-300) for a batch of q: Fourier over Rp of the
+    "ph"-layout tensor (rows (iw, jw, nu, Re)), then the phonon-mode rotation by u_ph(q).
+    Returns (nq, nw*nw*nm*nr_el) complex64: per q, the op_r of the electron-Wannier /
+    phonon-Bloch object, column Re of nw*nw*nm rows."""
+    torch = _torch()
+    nq = xq_dev.shape[0]
+    nd = epmat_dev.ndata
+    nr_el = nd // (nw * nw * nm)
+    if work is None:
+        work = torch.empty((nq, nd), dtype=torch.complex64, device=xq_dev.device)
+    if out is None:
+        out = torch.empty((nq, nd), dtype=torch.complex64, device=xq_dev.device)
+    core.fourier_batched(epmat_dev, xq_dev, out=work[:nq], precision=precision)
+    uph = uph.contiguous()
+    L.call("ephip_eph_rotate_modes", nw * nw, nm, nr_el, nq, L.ptr(work), work.stride(0), L.ptr(uph),
+           L.ptr(out), out.stride(0), L.stream_handle())
+    return out[:nq]
+
+
+def eph_rotate_rqkq_g2(g, ik, ikq, states: ElectronStates, omega_q, nbw: int, g2=None, ep=None):
+    """eph_apply_rotations_rqkq! + g2 (src/wannier_to_bloch_batched.jl:360-377, :451-453) on the
+    in-window bands; returns g2 (ncol, nm, nbw, nbw) float32 (column-major (nbw, nbw, nm) blocks)."""
+    torch = _torch()
+    ncol = ik.shape[0]
+    nw = states.nw
+    nm = g.shape[1] // (nw * nw)
+    if g2 is None:
+        g2 = torch.empty((ncol, nm, nbw, nbw), dtype=torch.float32, device=g.device)
+    L.call("ephip_eph_rotate_rqkq_g2", nw, nm, ncol, L.ptr(g), g.stride(0), L.ptr(ik), L.ptr(ikq),
+           L.ptr(states.U), L.ptr(states.filt.band_min), L.ptr(states.filt.band_max), L.ptr(omega_q), nbw,
+           L.ptr(g2), L.ptr(ep), L.stream_handle())
+    return g2
+
+
+# ------------------------------------------------------------------------------------------------
+# Calculators (payload side of src/calculator/AbstractCalculator.jl:137-223)
+# ------------------------------------------------------------------------------------------------
+@dataclasses.dataclass
+class EPDataKWindowed:
+    """Device payload of one (q, k-batch): the windowed counterpart of EPDataKBatched
+    (src/EPData.jl:105-114). g2[t] is the (nbw, nbw, nm) window block of pair (ik[t], iq);
+    slot (ms, ns) is band (bmin[ikq[t]] - 1 + ms, bmin[ik[t]] - 1 + ns)."""
+    iq: int
+    ik: object
+    ikq: object
+    g2: object
+    ep: object | None
+    nbw: int
+
+
+class AbstractCalculator:
+    def setup(self, ctx):
+        pass
+
+    def begin_q(self, iq: int):
+        pass
+
+    def run(self, payload: EPDataKWindowed):
+        raise NotImplementedError
+
+    def end_q(self, iq: int):
+        pass
+
+    def postprocess(self):
+        pass
+
+
+class G2Recorder(AbstractCalculator):
+    """Records every payload to the host (test helper; the reference's _RecordCalcOuterQ pattern,
+    test/test_gpu.jl:600-654)."""
+
+    def __init__(self, keep_ep: bool = False):
+        self.blocks = []
+        self.keep_ep = keep_ep
+
+    def run(self, p: EPDataKWindowed):
+        self.blocks.append((p.iq, p.ik.cpu().numpy(), p.ikq.cpu().numpy(), p.g2.cpu().numpy(),
+                            None if p.ep is None else p.ep.cpu().numpy()))
+
+
+class G2Sum(AbstractCalculator):
+    """Gauge-invariant per-q scalar sum_k sum |g|^2 / (2w) on the device (modes above the acoustic
+    cutoff only) -- the _RecordCalcOuterQ reduction of test/test_gpu.jl:600-654 without weights."""
+
+    def __init__(self, nq: int, device):
+        torch = _torch()
+        self.acc = torch.zeros(nq, dtype=torch.float64, device=device)
+        self.mode_mask = None
+
+    def run(self, p: EPDataKWindowed):
+        g2 = p.g2
+        if self.mode_mask is not None:
+            g2 = g2 * self.mode_mask[p.iq].view(1, -1, 1, 1)
+        self.acc[p.iq] += g2.sum(dtype=_torch().float64)
+
+
+# ------------------------------------------------------------------------------------------------
+# Outer-q loop
+# ------------------------------------------------------------------------------------------------
+@dataclasses.dataclass
+class OuterQSetup:
+    el: ElectronStates
+    ph: PhononStates
+    ksurv: object          # (nks,) int64 fine-grid indices of k with an in-window band
+    nbw: int               # max in-window band count
+    epmat: WannierObject   # device, "ph" layout, complex64
+    irvec_el_dev: object
+    nw: int
+    nm: int
+    kgrid: tuple
+    qgrid: tuple
+
+
+def setup_outer_q(model, kgrid, qgrid, window, backend: core.GPUBackend, epmat_dev=None,
+                  el_chunk: int = 32768) -> OuterQSetup:
+    """Setup of run_eph_over_q_and_k (src/calculator/run_eph_over_q_and_k.jl:144-277): fine-grid
+    electron states + window filter, phonon states on the q-grid, e-ph tensor on the device."""
+    torch = _torch()
+    from .model import kpoints_grid
+    dev = backend.torch_device
+    nw, nm = model.nw, model.nmodes
+    ham = WannierObject(model.irvec_h, model.ham_r).to_device(backend, dtype=torch.complex128)
+    dyn = WannierObject(model.irvec_d, model.dyn_r).to_device(backend, dtype=torch.complex128)
+    xk = torch.from_numpy(kpoints_grid(kgrid)).to(dev)
+    xq = torch.from_numpy(kpoints_grid(qgrid)).to(dev)
+    el = compute_electron_states(ham, xk, window, chunk=el_chunk)
+    ph = compute_phonon_states(dyn, model.mass, xq)
+    ksurv = torch.nonzero(el.filt.keep, as_tuple=False).view(-1)
+    width = (el.filt.band_max - el.filt.band_min + 1) * (el.filt.band_min > 0)
+    nbw = max(1, int(width.max()))
+    if epmat_dev is None:
+        epmat_dev = WannierObject(model.irvec_ep, model.epmat_ph, check=False).to_device(backend,
+                                                                                          dtype=torch.complex64)
+    irvec_el_dev = torch.from_numpy(np.ascontiguousarray(model.irvec_el, dtype=np.int32)).to(dev)
+    return OuterQSetup(el, ph, ksurv, nbw, epmat_dev, irvec_el_dev, nw, nm, tuple(kgrid), tuple(qgrid))
+
+
+class OuterQLoop:
+    """The per-q body of _loop_eph_over_q_and_k_batched (src/calculator/run_eph_over_q_and_k.jl:
+    526-610): RR->Rq for a tile of q, then for each q and k-batch: Fourier over Re at the surviving
+    k, k+q lookup, windowed rotations and g2. Buffers are allocated once."""
+
+    def __init__(self, st: OuterQSetup, nk_batch: int = 8192, nq_tile: int = 16,
+                 precision: int = L.PREC_FP32, keep_ep: bool = False):
+        torch = _torch()
+        self.st = st
+        self.nk_batch = int(nk_batch)
+        self.nq_tile = int(nq_tile)
+        self.precision = precision
+        self.keep_ep = keep_ep
+        dev = st.el.xk.device
+        nd_q = st.nw * st.nw * st.nm
+        self.nr_el = st.epmat.ndata // nd_q
+        self.gq_work = torch.empty((self.nq_tile, st.epmat.ndata), dtype=torch.complex64, device=dev)
+        self.eRpq = torch.empty((self.nq_tile, st.epmat.ndata), dtype=torch.complex64, device=dev)
+        self.gk = torch.empty((self.nk_batch, nd_q), dtype=torch.complex64, device=dev)
+        self.g2 = torch.empty((self.nk_batch, st.nm, st.nbw, st.nbw), dtype=torch.float32, device=dev)
+        self.ep = (torch.empty((self.nk_batch, st.nm, st.nbw, st.nbw), dtype=torch.complex64, device=dev)
+                   if keep_ep else None)
+        self.xk_surv = st.el.xk.index_select(0, st.ksurv).contiguous()
+        self.xq_host = st.ph.xq.cpu().numpy()
+
+    def run_qtile(self, iq0: int, nq: int, calculators=()):
+        st = self.st
+        xq = st.ph.xq[iq0:iq0 + nq]
+        uph = st.ph.u[iq0:iq0 + nq]
+        get_eph_RR_to_Rq_batched(st.epmat, st.nw, st.nm, xq, uph, self.precision, work=self.gq_work,
+                                 out=self.eRpq)
+        nd_q = st.nw * st.nw * st.nm
+        nks = st.ksurv.shape[0]
+        for t in range(nq):
+            iq = iq0 + t
+            obj = _RqObject(self.eRpq[t].view(self.nr_el, nd_q), st.irvec_el_dev, nd_q)
+            for c in calculators:
+                c.begin_q(iq)
+            for s in range(0, nks, self.nk_batch):
+                e = min(nks, s + self.nk_batch)
+                ik = st.ksurv[s:e]
+                ikq = core.kq_index(self.xk_surv[s:e], self.xq_host[iq], st.kgrid)
+                core.fourier_batched(obj, self.xk_surv[s:e], out=self.gk[: e - s], precision=self.precision)
+                g2 = eph_rotate_rqkq_g2(self.gk[: e - s], ik, ikq, st.el, st.ph.omega[iq], st.nbw,
+                                        g2=self.g2[: e - s], ep=None if self.ep is None else self.ep[: e - s])
+                payload = EPDataKWindowed(iq, ik, ikq, g2, None if self.ep is None else self.ep[: e - s], st.nbw)
+                for c in calculators:
+                    c.run(payload)
+            for c in calculators:
+                c.end_q(iq)
+
+    def run(self, iq_start: int = 0, iq_stop: int | None = None, calculators=()):
+        nq = self.st.ph.xq.shape[0]
+        iq_stop = nq if iq_stop is None else min(nq, iq_stop)
+        for iq0 in range(iq_start, iq_stop, self.nq_tile):
+            self.run_qtile(iq0, min(self.nq_tile, iq_stop - iq0), calculators)
+
+
+class _RqObject(WannierObject):
+    """Device electron-Wannier / phonon-Bloch object at one q (op_r (nw*nw*nm) x nr_el); irvec lives
+    on the device already."""
+
+    def __init__(self, op_r, irvec_dev, ndata):
+        self.op_r = op_r
+        self.nr = op_r.shape[0]
+        self.ndata = ndata
+        self.irvec_next = None
+        self._irvec_dev = irvec_dev
+        self.irvec = None

@@ -1,0 +1,138 @@
+/*
+ * ephip.h -- C-ABI of libephip, the MI355X (gfx950) Wannier->Bloch engine.
+ *
+ * Drop-in boundary for ElectronPhonon.jl's device method table (src/common/gpu_utils.jl:1-78 and the
+ * CUDA extension ext/ElectronPhononCUDAExt.jl). Each entry names the reference interface it replaces
+ * (file:line in the ElectronPhonon.jl tree). A binding (Julia ccall, Python ctypes) passes plain
+ * device pointers and sizes; no runtime types cross the boundary.
+ *
+ * Conventions
+ *   - Every array argument is a DEVICE pointer unless marked (host). Arrays are column-major with the
+ *     reference's index order; complex values are interleaved (re, im) pairs: "c32" = 2 x float,
+ *     "c64" = 2 x double. Leading dimensions are in elements.
+ *   - Indices are 0-based, except window band indices (bmin/bmax), which are 1-based like the
+ *     reference's band ranges, with 0 meaning "no band in the window".
+ *   - O(k) = sum_R exp(+2 pi i k.R) O(R), k in crystal coordinates (3 doubles per point), R integer
+ *     (3 int32 per vector) -- src/wannier/WannierInterpolator.jl:127,135.
+ *   - `stream` is a hipStream_t (NULL = default stream). All calls are asynchronous on `stream`
+ *     unless stated otherwise; errors in argument checks are reported synchronously.
+ *   - Return value: EPHIP_OK or an error code; ephip_last_error() returns the message of the last
+ *     failing call on the calling thread.
+ */
+#ifndef EPHIP_H
+#define EPHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EPHIP_OK 0
+#define EPHIP_ERR_ARG 1
+#define EPHIP_ERR_HIP 2
+#define EPHIP_ERR_NOCONV 3
+
+/* Precision of a Fourier contraction (inputs and outputs are c32 in both). */
+#define EPHIP_PREC_FP32 0   /* FP32 MFMA, FP32 accumulation */
+#define EPHIP_PREC_BF16X3 1 /* split-BF16 hi+lo operands, three BF16 MFMA products, FP32 accumulation */
+
+/* Message of the last failing call on this thread ("" if none). Host-side, synchronous. */
+const char *ephip_last_error(void);
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int ephip_version(void);
+
+/* free_bytes(::GPUBackend) -- src/common/gpu_utils.jl:51, ext/ElectronPhononCUDAExt.jl:36.
+ * (host) free/total device memory in bytes of the current device. Synchronous. */
+int ephip_free_bytes(int64_t *free_bytes, int64_t *total_bytes);
+
+/* synchronize(::GPUBackend) -- src/common/gpu_utils.jl:60, ext/ElectronPhononCUDAExt.jl:37.
+ * Blocks until `stream` is drained, and reports any asynchronous kernel fault. */
+int ephip_synchronize(void *stream);
+
+/* fourier_phase! -- src/wannier/batched_fourier_core.jl:24-30.
+ * phase[r + nr*j] (c64) = exp(2 pi i R_r . x_j) for irvec (3 x nr int32) and xk (3 x nk double). */
+int ephip_fourier_phase_c64(int32_t nr, const int32_t *irvec, int32_t nk, const double *xk, void *phase,
+                            void *stream);
+
+/* fourier_batched! / get_fourier_batched! -- src/wannier/batched_fourier_core.jl:117-139,
+ * src/wannier/batched_interpolator.jl:200-214 (the ZGEMM at :136 of batched_fourier_core.jl).
+ * out[d + ldout*j] = sum_r op[d + ldop*r] * exp(2 pi i R_r . x_j), d < ndata, j < nk.
+ * op: c32 (ldop x nr), out: c32 (ldout x nk). The phase tile is generated on chip from the FP64
+ * argument R.x reduced mod 1; precision = EPHIP_PREC_FP32 or EPHIP_PREC_BF16X3. */
+int ephip_fourier_c32(int64_t ndata, int32_t nr, const void *op, int64_t ldop, const int32_t *irvec,
+                      int32_t nk, const double *xk, void *out, int64_t ldout, int32_t precision,
+                      void *stream);
+
+/* Same contraction in FP64 (c64 in and out), FP64 MFMA. Used where the reference's FP64 result is
+ * needed on the device (band energies for the window filter, dynamical matrices). */
+int ephip_fourier_c64(int64_t ndata, int32_t nr, const void *op, int64_t ldop, const int32_t *irvec,
+                      int32_t nk, const double *xk, void *out, int64_t ldout, void *stream);
+
+/* eigvals_batched / eigen_batched -- src/wannier_to_bloch_batched.jl:30-65,
+ * ext/ElectronPhononCUDAExt.jl:80-115. Hermitian eigensolve of nb matrices H (n x n x nb, upper
+ * triangle read), n <= 32, one wavefront per matrix, FP32 arithmetic. H is c64 if h_is_c64 else c32
+ * (not modified). E: float (n x nb) ascending. U: c32 (n x n x nb) eigenvectors, or NULL for values
+ * only (U may alias a c32 H). info: int32 per matrix (0 = converged) or NULL. */
+int ephip_heev_c32(int32_t n, int64_t nb, const void *H, int32_t h_is_c64, void *U, float *E,
+                   int32_t *info, void *stream);
+
+/* FP64 refinement of ephip_heev_c32 results against the FP64 matrices H64 (c64, n x n x nb):
+ * Rayleigh quotients for all vectors; for runs of eigenvalues closer than cluster_tol, a Rayleigh-Ritz
+ * step (FP64 orthonormalisation, projection, Jacobi) that also re-orthogonalises their vectors.
+ * U (c32) is updated in place; E64: double (n x nb) ascending. n <= 31. Replaces the FP64 accuracy of the
+ * reference's zheev-based eigenvalues (src/common/AllocatedLAPACK.jl:44-108) on the device. */
+int ephip_heev_refine(int32_t n, int64_t nb, const void *H64, void *U, const float *E32, double *E64,
+                      double cluster_tol, void *stream);
+
+/* _filter_kpoints / inside_window -- src/filter.jl:19, :40-100 (device branch :46-72).
+ * E: double (nw x nk) ascending. keep[k] = any band in [wmin, wmax]; bmin/bmax: 1-based
+ * searchsortedfirst(E, wmin) / searchsortedlast(E, wmax), 0 when keep = 0; nbelow[k] = bmin - 1
+ * counted as searchsortedfirst - 1 (may be NULL). */
+int ephip_window_filter(int32_t nw, int64_t nk, const double *E, double wmin, double wmax, uint8_t *keep,
+                        int64_t *bmin, int64_t *bmax, int64_t *nbelow, void *stream);
+
+/* Phonon dynamical-matrix mass scaling -- src/compute_states.jl:295-297 (device phonon states),
+ * src/wannier_to_bloch.jl:199-202: D (c64, nm x nm x nq) /= sqrt(m_i) sqrt(m_j); mass: double (nm). */
+int ephip_ph_mass_scale(int32_t nm, int64_t nq, void *D, const double *mass, void *stream);
+
+/* Phonon frequencies and eigenvector mass factor -- src/diagonalize.jl:118-126,
+ * src/compute_states.jl:298-300: omega = sign(w2) sqrt(|w2|) (double, nm x nq); U (c32, may be NULL)
+ * rows divided by sqrt(m_i). */
+int ephip_ph_finalize(int32_t nm, int64_t nq, const double *w2, double *omega, void *U, const double *mass,
+                      void *stream);
+
+/* k-point index map -- _grid_coords_reduced / _hash_xk / _ik_from_hash / xk_to_ik
+ * (src/common/kpoints.jl:614-656) as used by _fill_iqs! (src/calculator/run_eph_over_k_and_kq.jl:565-582).
+ * For each point x_k (xk, 3 x nk double) computes the grid hash of x_k + dx on the (host) ngrid[3]
+ * grid shifted by (host) shift[3] (NULL = 0), then out[k] = table[hash] - 1 with table a dense
+ * int64 map hash -> 1-based index (0 = absent, so out = -1), or out[k] = hash if table is NULL.
+ * dx: (host) 3 doubles or NULL. */
+int ephip_kq_index(int64_t nk, const double *xk, const double *dx, const int32_t *ngrid, const double *shift,
+                   const int64_t *table, int64_t *out, void *stream);
+
+/* Phonon-mode rotation of get_eph_RR_to_Rq! -- src/wannier_to_bloch.jl:292-297, batched over nq:
+ * out[b + nb*(nu' + nm*Re) + ldout*q] = sum_nu g[b + nb*(nu + nm*Re) + ldg*q] * uph[nu + nm*nu' + nm*nm*q]
+ * with nb = nbasis (= nw*nw), all c32; nm <= 48. g and out must not alias. */
+int ephip_eph_rotate_modes(int64_t nbasis, int32_t nm, int32_t nr_el, int32_t nq, const void *g, int64_t ldg,
+                           const void *uph, void *out, int64_t ldout, void *stream);
+
+/* eph_apply_rotations_rqkq! + g2 fold -- src/wannier_to_bloch_batched.jl:360-377 and :451-453,
+ * ext/ElectronPhononCUDAExt.jl:214-258, with the outer-q loop's window masks
+ * (src/calculator/run_eph_over_q_and_k.jl:555-585). For batch column t < ncol:
+ *   g column t: c32 (nw, nw, nm) at g + ldg*t, index legend g[i, j, nu] (i: k+q side, j: k side);
+ *   k = ik[t], k+q = ikq[t] index the states: U c32 (nw, nw, nstates), bmin/bmax int64 (nstates);
+ *   ep[m, n, nu] = sum_ij conj(U_kq[i, m]) g[i, j, nu] U_k[j, n] for m in the k+q window, n in the k
+ *   window; g2 = |ep|^2 / (2 omega[nu]) (omega: double nm, no cutoff, as the reference).
+ * Output block of column t: slot [ms + nbw*(ns + nbw*nu)], ms = m - (bmin[kq]-1), ns = n - (bmin[k]-1),
+ * zero outside the windows; g2 (float) and/or ep (c32) may be NULL. nbw must be >= every window width. */
+int ephip_eph_rotate_rqkq_g2(int32_t nw, int32_t nm, int64_t ncol, const void *g, int64_t ldg, const int64_t *ik,
+                             const int64_t *ikq, const void *U, const int64_t *bmin, const int64_t *bmax,
+                             const double *omega, int32_t nbw, float *g2, void *ep, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EPHIP_H */

@@ -1,0 +1,586 @@
+/*
+ * ephor.c -- FP64 CPU restatement of ElectronPhonon.jl's Wannier->Bloch path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the oracle the HIP library is checked against. Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the product path
+ * never calls it and has no CPU fallback.
+ *
+ * Every routine cites the reference function whose arithmetic it restates (paths relative to the
+ * ElectronPhonon.jl source tree). Conventions shared by all routines:
+ *   - complex numbers are C99 `double complex` (interleaved re/im, numpy complex128);
+ *   - matrices are column-major with explicit leading dimensions; indices are 0-based, except the
+ *     band indices returned by the window filter, which are 1-based like the reference's;
+ *   - O(k) = sum_R exp(+2 pi i k.R) O(R), k in crystal coordinates, R integer
+ *     (src/wannier/WannierInterpolator.jl:127,135).
+ *
+ * How each routine is pinned (beyond restating the reference) is listed in oracle/README.md and
+ * in the tests that exercise it.
+ */
+#define _GNU_SOURCE
+#include <complex.h>
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex zc;
+
+/* exp(2 pi i t) with t reduced to [-1/2, 1/2] first, like cispi(2t)
+ * (src/wannier/batched_fourier_core.jl:24-30). */
+static inline zc cis2pi(double t) {
+    t -= rint(t);
+    double s, c;
+    sincos(2.0 * M_PI * t, &s, &c);
+    return c + I * s;
+}
+
+/* fourier_phase! (src/wannier/batched_fourier_core.jl:24-30): phase[ir + nr*j] = exp(2 pi i R_ir.x_j) */
+void ephor_fourier_phase(int nr, const int32_t *irvec, int nk, const double *xk, zc *phase) {
+    for (int j = 0; j < nk; j++)
+        for (int r = 0; r < nr; r++) {
+            double t = irvec[3 * r] * xk[3 * j] + irvec[3 * r + 1] * xk[3 * j + 1] +
+                       irvec[3 * r + 2] * xk[3 * j + 2];
+            phase[r + (int64_t)nr * j] = cis2pi(t);
+        }
+}
+
+/* fourier_batched! / get_fourier! normal mode (src/wannier/batched_fourier_core.jl:117-139,
+ * src/wannier/WannierInterpolator.jl:121-139): out[:, j] = op_r[1:ndata, :] * phase[:, j]. */
+void ephor_fourier(int64_t ndata, int nr, const zc *op, int64_t ldop, const int32_t *irvec, int nk,
+                   const double *xk, zc *out, int64_t ldout) {
+#pragma omp parallel
+    {
+        zc *ph = (zc *)malloc(sizeof(zc) * nr);
+#pragma omp for schedule(static)
+        for (int j = 0; j < nk; j++) {
+            ephor_fourier_phase(nr, irvec, 1, xk + 3 * j, ph);
+            double *o = (double *)(out + ldout * j);
+            for (int64_t d = 0; d < ndata; d++) o[2 * d] = o[2 * d + 1] = 0.0;
+            for (int r = 0; r < nr; r++) {
+                const double *a = (const double *)(op + ldop * r);
+                double pr = creal(ph[r]), pi = cimag(ph[r]);
+                for (int64_t d = 0; d < ndata; d++) {
+                    double ar = a[2 * d], ai = a[2 * d + 1];
+                    o[2 * d] += ar * pr - ai * pi;
+                    o[2 * d + 1] += ar * pi + ai * pr;
+                }
+            }
+        }
+        free(ph);
+    }
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Hermitian eigensolver. The reference calls LAPACK zheev ('V'/'N', 'U')
+ * (src/common/AllocatedLAPACK.jl:44-108 via src/wannier_to_bloch_batched.jl:30-65,
+ * src/diagonalize.jl:35-47). Restated from the published method: Householder reduction of the
+ * Hermitian matrix to a real symmetric tridiagonal (Golub & Van Loan, Matrix Computations 4th ed.
+ * sec. 8.3.1, with the complex phase normalisation of the off-diagonal), implicit QL iterations
+ * with Wilkinson-type shifts (the EISPACK tql2 scheme), and back-transformation of the
+ * eigenvectors. Eigenvalues are returned ascending; eigenvectors are orthonormal columns. The
+ * upper triangle of A is read ('U'), A is destroyed.
+ * ------------------------------------------------------------------------------------------- */
+
+/* Implicit QL on the real symmetric tridiagonal (d, e), e[i] = T[i+1, i], e[n-1] unused.
+ * Accumulates rotations into the real n x n matrix z (column-major) if z != NULL. */
+static int ql_implicit(int n, double *d, double *e, double *z) {
+    if (n > 0) e[n - 1] = 0.0;
+    for (int l = 0; l < n; l++) {
+        int iter = 0, m;
+        do {
+            for (m = l; m < n - 1; m++) {
+                double dd = fabs(d[m]) + fabs(d[m + 1]);
+                if (fabs(e[m]) <= DBL_EPSILON * dd) break;
+            }
+            if (m != l) {
+                if (iter++ == 100) return -1;
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = hypot(g, 1.0);
+                g = d[m] - d[l] + e[l] / (g + copysign(r, g));
+                double s = 1.0, c = 1.0, p = 0.0;
+                int i;
+                for (i = m - 1; i >= l; i--) {
+                    double f = s * e[i], b = c * e[i];
+                    r = hypot(f, g);
+                    e[i + 1] = r;
+                    if (r == 0.0) {
+                        d[i + 1] -= p;
+                        e[m] = 0.0;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * b;
+                    p = s * r;
+                    d[i + 1] = g + p;
+                    g = c * r - b;
+                    if (z)
+                        for (int k = 0; k < n; k++) {
+                            double zf = z[k + (int64_t)n * (i + 1)];
+                            z[k + (int64_t)n * (i + 1)] = s * z[k + (int64_t)n * i] + c * zf;
+                            z[k + (int64_t)n * i] = c * z[k + (int64_t)n * i] - s * zf;
+                        }
+                }
+                if (r == 0.0 && i >= l) continue;
+                d[l] -= p;
+                e[l] = g;
+                e[m] = 0.0;
+            }
+        } while (m != l);
+    }
+    return 0;
+}
+
+/* Returns 0 on success. A: n x n (lda), w: n eigenvalues ascending, V: n x n (ldv) or NULL. */
+int ephor_heev(int n, zc *A, int lda, double *w, zc *V, int ldv) {
+    if (n <= 0) return 0;
+    zc *a = (zc *)malloc(sizeof(zc) * n * n);
+    /* Hermitian matrix from the upper triangle ('U'). */
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            zc v = (i <= j) ? A[i + (int64_t)lda * j] : conj(A[j + (int64_t)lda * i]);
+            if (i == j) v = creal(v);
+            a[i + n * j] = v;
+        }
+    zc *hv = (zc *)calloc((size_t)n * n, sizeof(zc)); /* Householder vector of step j in column j */
+    zc *ec = (zc *)calloc(n, sizeof(zc));
+    zc *p = (zc *)malloc(sizeof(zc) * n);
+    double *d = (double *)malloc(sizeof(double) * n), *e = (double *)calloc(n, sizeof(double));
+    for (int j = 0; j + 2 < n; j++) {
+        int m = n - j - 1;
+        zc *x = a + (j + 1) + n * j;
+        double xn = 0.0;
+        for (int i = 0; i < m; i++) xn += creal(x[i] * conj(x[i]));
+        xn = sqrt(xn);
+        if (xn == 0.0) {
+            ec[j] = 0.0;
+            continue;
+        }
+        double ax0 = cabs(x[0]);
+        zc ph0 = ax0 > 0.0 ? x[0] / ax0 : 1.0;
+        zc alpha = -ph0 * xn;
+        zc *v = hv + (j + 1) + n * j;
+        for (int i = 0; i < m; i++) v[i] = x[i];
+        v[0] -= alpha;
+        double vn = 0.0;
+        for (int i = 0; i < m; i++) vn += creal(v[i] * conj(v[i]));
+        vn = sqrt(vn);
+        for (int i = 0; i < m; i++) v[i] /= vn;
+        /* p = A22 v ; K = v^H p ; w = p - K v ; A22 -= 2 (v w^H + w v^H) */
+        zc *a22 = a + (j + 1) + n * (j + 1);
+        for (int i = 0; i < m; i++) {
+            zc s = 0.0;
+            for (int k = 0; k < m; k++) s += a22[i + n * k] * v[k];
+            p[i] = s;
+        }
+        double K = 0.0;
+        for (int i = 0; i < m; i++) K += creal(conj(v[i]) * p[i]);
+        for (int i = 0; i < m; i++) p[i] -= K * v[i];
+        for (int k = 0; k < m; k++)
+            for (int i = 0; i < m; i++)
+                a22[i + n * k] -= 2.0 * (v[i] * conj(p[k]) + p[i] * conj(v[k]));
+        ec[j] = alpha;
+        for (int i = 1; i < m; i++) x[i] = 0.0;
+        x[0] = alpha;
+    }
+    if (n >= 2) ec[n - 2] = a[(n - 1) + n * (n - 2)];
+    for (int i = 0; i < n; i++) d[i] = creal(a[i + n * i]);
+    /* phase normalisation: T = D T_r D^H with D = diag(tau), T_r real with e = |ec| */
+    zc *tau = (zc *)malloc(sizeof(zc) * n);
+    tau[0] = 1.0;
+    for (int j = 0; j + 1 < n; j++) {
+        double ae = cabs(ec[j]);
+        e[j] = ae;
+        tau[j + 1] = ae > 0.0 ? tau[j] * ec[j] / ae : tau[j];
+    }
+    double *z = NULL;
+    if (V) {
+        z = (double *)calloc((size_t)n * n, sizeof(double));
+        for (int i = 0; i < n; i++) z[i + n * i] = 1.0;
+    }
+    int st = ql_implicit(n, d, e, z);
+    /* ascending order (selection sort keeps it simple for small n) */
+    for (int i = 0; i < n - 1; i++) {
+        int kmin = i;
+        for (int k = i + 1; k < n; k++)
+            if (d[k] < d[kmin]) kmin = k;
+        if (kmin != i) {
+            double t = d[i];
+            d[i] = d[kmin];
+            d[kmin] = t;
+            if (z)
+                for (int r = 0; r < n; r++) {
+                    double tz = z[r + n * i];
+                    z[r + n * i] = z[r + n * kmin];
+                    z[r + n * kmin] = tz;
+                }
+        }
+    }
+    for (int i = 0; i < n; i++) w[i] = d[i];
+    if (V) {
+        /* Y = D Z, then V = H_0 H_1 ... H_{n-3} Y */
+        for (int c = 0; c < n; c++)
+            for (int r = 0; r < n; r++) V[r + (int64_t)ldv * c] = tau[r] * z[r + n * c];
+        for (int j = n - 3; j >= 0; j--) {
+            zc *v = hv + (j + 1) + n * j;
+            int m = n - j - 1;
+            for (int c = 0; c < n; c++) {
+                zc *y = V + (j + 1) + (int64_t)ldv * c;
+                zc s = 0.0;
+                for (int i = 0; i < m; i++) s += conj(v[i]) * y[i];
+                for (int i = 0; i < m; i++) y[i] -= 2.0 * v[i] * s;
+            }
+        }
+        free(z);
+    }
+    free(a);
+    free(hv);
+    free(ec);
+    free(p);
+    free(d);
+    free(e);
+    free(tau);
+    return st;
+}
+
+/* eigen_batched / eigvals_batched CPU methods (src/wannier_to_bloch_batched.jl:30-65): loop of
+ * Hermitian eigensolves over a (n, n, nb) stack. V may be NULL (values only). */
+int ephor_heev_batched(int n, int64_t nb, zc *A, double *w, zc *V) {
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int64_t b = 0; b < nb; b++)
+        bad |= ephor_heev(n, A + (int64_t)n * n * b, n, w + (int64_t)n * b,
+                          V ? V + (int64_t)n * n * b : NULL, n) != 0;
+    return bad ? -1 : 0;
+}
+
+/* Phonon eigenproblem at one q (src/wannier_to_bloch.jl:186-205, src/diagonalize.jl:104-127):
+ * D(q)[i,j] /= sqrt(m_i m_j); eigen; omega = sign(w) sqrt(|w|); u[i,:] /= sqrt(m_i). */
+int ephor_ph_eigen(int nm, zc *D, const double *mass, double *omega, zc *U) {
+    for (int j = 0; j < nm; j++)
+        for (int i = 0; i < nm; i++) D[i + nm * j] /= sqrt(mass[i]) * sqrt(mass[j]);
+    int st = ephor_heev(nm, D, nm, omega, U, nm);
+    for (int i = 0; i < nm; i++) omega[i] = omega[i] >= 0 ? sqrt(omega[i]) : -sqrt(-omega[i]);
+    if (U)
+        for (int c = 0; c < nm; c++)
+            for (int i = 0; i < nm; i++) U[i + nm * c] /= sqrt(mass[i]);
+    return st;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Energy-window filter (src/filter.jl:19 inside_window, :40-100 _filter_kpoints). E is (nw, nk)
+ * ascending per k. Band indices are 1-based: bmin = searchsortedfirst(E, wmin),
+ * bmax = searchsortedlast(E, wmax); a k is kept iff bmin <= bmax, and for a dropped k the
+ * reference leaves band_min = band_max = 0. nbelow[k] = bmin - 1 (the reference multiplies it
+ * by the k weight and sums).
+ * ------------------------------------------------------------------------------------------- */
+void ephor_window_filter(int nw, int64_t nk, const double *E, double wmin, double wmax,
+                         uint8_t *keep, int64_t *bmin, int64_t *bmax, int64_t *nbelow) {
+    for (int64_t k = 0; k < nk; k++) {
+        const double *e = E + (int64_t)nw * k;
+        int first = nw + 1, last = 0;
+        for (int i = 0; i < nw; i++)
+            if (e[i] >= wmin) {
+                first = i + 1;
+                break;
+            }
+        for (int i = nw - 1; i >= 0; i--)
+            if (e[i] <= wmax) {
+                last = i + 1;
+                break;
+            }
+        nbelow[k] = first - 1;
+        if (first <= last) {
+            keep[k] = 1;
+            bmin[k] = first;
+            bmax[k] = last;
+        } else {
+            keep[k] = 0;
+            bmin[k] = 0;
+            bmax[k] = 0;
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * k-point grids and index maps (src/common/kpoints.jl).
+ * ------------------------------------------------------------------------------------------- */
+
+/* kpoints_grid_range (src/common/kpoints.jl:135-153): third index fastest. */
+void ephor_kpoints_grid(int n1, int n2, int n3, const double *shift, double *xk) {
+    int64_t ik = 0;
+    for (int i = 0; i < n1; i++)
+        for (int j = 0; j < n2; j++)
+            for (int k = 0; k < n3; k++, ik++) {
+                xk[3 * ik] = (double)i / n1 + shift[0];
+                xk[3 * ik + 1] = (double)j / n2 + shift[1];
+                xk[3 * ik + 2] = (double)k / n3 + shift[2];
+            }
+}
+
+static inline int64_t floormod(int64_t a, int64_t n) {
+    int64_t r = a % n;
+    return r < 0 ? r + n : r;
+}
+
+/* _grid_coords_reduced + _hash_xk (src/common/kpoints.jl:614-651):
+ * c_d = mod(round(Int, (x_d - s_d) * n_d), n_d) with round-half-even; hash = (c1 n2 + c2) n3 + c3. */
+void ephor_hash_xk(int64_t nk, const double *xk, const int32_t *ngrid, const double *shift,
+                   int64_t *hash) {
+    for (int64_t k = 0; k < nk; k++) {
+        int64_t c[3];
+        for (int d = 0; d < 3; d++)
+            c[d] = floormod((int64_t)llrint((xk[3 * k + d] - shift[d]) * ngrid[d]), ngrid[d]);
+        hash[k] = (c[0] * ngrid[1] + c[1]) * ngrid[2] + c[2];
+    }
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Electron-phonon stages (src/wannier_to_bloch.jl:277-443).
+ * Index legend: i = k+q-side Wannier, j = k-side Wannier, m = k+q band, n = k band, nu = mode.
+ * ------------------------------------------------------------------------------------------- */
+
+/* get_eph_RR_to_Rq! (src/wannier_to_bloch.jl:277-300). epmat "ph" layout op_r is
+ * (nw*nw*nm*nr_el) x nr_ep; output (nw*nw*nm) x nr_el: out[b, nu', Re] = sum_nu g(q)[b,nu,Re] u_ph[nu,nu']. */
+void ephor_eph_RR_to_Rq(int nw, int nm, int nr_el, int nr_ep, const zc *epmat, const int32_t *irvec_ep,
+                        const double *xq, const zc *uph, zc *out) {
+    int64_t nb = (int64_t)nw * nw;
+    int64_t ndata = nb * nm * nr_el;
+    zc *g = (zc *)malloc(sizeof(zc) * ndata);
+    ephor_fourier(ndata, nr_ep, epmat, ndata, irvec_ep, 1, xq, g, ndata);
+#pragma omp parallel for schedule(static)
+    for (int re = 0; re < nr_el; re++)
+        for (int nu2 = 0; nu2 < nm; nu2++)
+            for (int64_t b = 0; b < nb; b++) {
+                zc s = 0.0;
+                for (int nu = 0; nu < nm; nu++) s += g[b + nb * (nu + (int64_t)nm * re)] * uph[nu + nm * nu2];
+                out[b + nb * (nu2 + (int64_t)nm * re)] = s;
+            }
+    free(g);
+}
+
+/* Rotation shared by get_eph_Rq_to_kq! (src/wannier_to_bloch.jl:326-350):
+ * ep[m,n,nu] = sum_ij conj(ukq[i,m]) g[i,j,nu] uk[j,n]. g is (nw, nw, nm). */
+static void rotate_rqkq(int nw, int nm, const zc *g, const zc *uk, int nbk, const zc *ukq, int nbkq, zc *ep) {
+    zc *t = (zc *)malloc(sizeof(zc) * nbkq * nw);
+    for (int nu = 0; nu < nm; nu++) {
+        const zc *gn = g + (int64_t)nw * nw * nu;
+        for (int j = 0; j < nw; j++)
+            for (int m = 0; m < nbkq; m++) {
+                zc s = 0.0;
+                for (int i = 0; i < nw; i++) s += conj(ukq[i + nw * m]) * gn[i + nw * j];
+                t[m + nbkq * j] = s;
+            }
+        for (int n = 0; n < nbk; n++)
+            for (int m = 0; m < nbkq; m++) {
+                zc s = 0.0;
+                for (int j = 0; j < nw; j++) s += t[m + nbkq * j] * uk[j + nw * n];
+                ep[m + nbkq * (n + (int64_t)nbk * nu)] = s;
+            }
+    }
+    free(t);
+}
+
+/* get_eph_Rq_to_kq! (src/wannier_to_bloch.jl:326-350) for one k. eRpq: (nw*nw*nm) x nr_el. */
+void ephor_eph_Rq_to_kq(int nw, int nm, int nr_el, const zc *eRpq, const int32_t *irvec_el, const double *xk,
+                        const zc *uk, int nbk, const zc *ukq, int nbkq, zc *ep) {
+    int64_t nd = (int64_t)nw * nw * nm;
+    zc *g = (zc *)malloc(sizeof(zc) * nd);
+    ephor_fourier(nd, nr_el, eRpq, nd, irvec_el, 1, xk, g, nd);
+    rotate_rqkq(nw, nm, g, uk, nbk, ukq, nbkq, ep);
+    free(g);
+}
+
+/* get_eph_RR_to_kR! (src/wannier_to_bloch.jl:365-393). epmat "el" layout op_r is
+ * (nw*nw*nm*nr_ep) x nr_el; out (nw*nbk*nm) x nr_ep: out[i,n,nu,Rp] = sum_j g(k)[i,j,nu,Rp] uk[j,n]. */
+void ephor_eph_RR_to_kR(int nw, int nm, int nr_el, int nr_ep, const zc *epmat_el, const int32_t *irvec_el,
+                        const double *xk, const zc *uk, int nbk, zc *out) {
+    int64_t nd = (int64_t)nw * nw * nm * nr_ep;
+    zc *g = (zc *)malloc(sizeof(zc) * nd);
+    ephor_fourier(nd, nr_el, epmat_el, nd, irvec_el, 1, xk, g, nd);
+    for (int rp = 0; rp < nr_ep; rp++)
+        for (int nu = 0; nu < nm; nu++) {
+            const zc *gb = g + (int64_t)nw * nw * (nu + (int64_t)nm * rp);
+            zc *o = out + (int64_t)nw * nbk * (nu + (int64_t)nm * rp);
+            for (int n = 0; n < nbk; n++)
+                for (int i = 0; i < nw; i++) {
+                    zc s = 0.0;
+                    for (int j = 0; j < nw; j++) s += gb[i + nw * j] * uk[j + nw * n];
+                    o[i + nw * n] = s;
+                }
+        }
+    free(g);
+}
+
+/* get_eph_kR_to_kq! (src/wannier_to_bloch.jl:409-443): FT over Rp at q, then
+ * ep[m,n,nu'] = sum_i sum_nu conj(ukq[i,m]) g[i,n,nu] u_ph[nu,nu']. ekpR: (nw*nbk*nm) x nr_ep. */
+void ephor_eph_kR_to_kq(int nw, int nbk, int nm, int nr_ep, const zc *ekpR, const int32_t *irvec_ep,
+                        const double *xq, const zc *uph, const zc *ukq, int nbkq, zc *ep) {
+    int64_t nd = (int64_t)nw * nbk * nm;
+    zc *g = (zc *)malloc(sizeof(zc) * nd);
+    zc *t = (zc *)malloc(sizeof(zc) * nbkq * nbk * nm);
+    ephor_fourier(nd, nr_ep, ekpR, nd, irvec_ep, 1, xq, g, nd);
+    for (int nu = 0; nu < nm; nu++)
+        for (int n = 0; n < nbk; n++)
+            for (int m = 0; m < nbkq; m++) {
+                zc s = 0.0;
+                for (int i = 0; i < nw; i++) s += conj(ukq[i + nw * m]) * g[i + nw * (n + (int64_t)nbk * nu)];
+                t[m + nbkq * (n + nbk * nu)] = s;
+            }
+    for (int nu2 = 0; nu2 < nm; nu2++)
+        for (int mn = 0; mn < nbkq * nbk; mn++) {
+            zc s = 0.0;
+            for (int nu = 0; nu < nm; nu++) s += t[mn + nbkq * nbk * nu] * uph[nu + nm * nu2];
+            ep[mn + (int64_t)nbkq * nbk * nu2] = s;
+        }
+    free(g);
+    free(t);
+}
+
+/* epstate_set_g2! (src/EPState.jl:73-81): g2 = |ep|^2 / (2 omega_nu), no omega cutoff. */
+void ephor_g2(int64_t nmn, int nm, const zc *ep, const double *omega, double *g2) {
+    for (int nu = 0; nu < nm; nu++) {
+        double inv = 1.0 / (2.0 * omega[nu]);
+        for (int64_t i = 0; i < nmn; i++) {
+            zc v = ep[i + nmn * nu];
+            g2[i + nmn * nu] = (creal(v) * creal(v) + cimag(v) * cimag(v)) * inv;
+        }
+    }
+}
+
+/* epstate_g2_degenerate_average! (src/EPState.jl:205-238): average g2 (nbkq, nbk, nm) over
+ * degenerate bands (|dE| <= cutoff) first at k, then at k+q. ek: nbk, ekq: nbkq. */
+void ephor_g2_degenerate_average(int nbkq, int nbk, int nm, double *g2, const double *ekq, const double *ek,
+                                 double cutoff) {
+    int64_t sz = (int64_t)nbkq * nbk * nm;
+    double *avg = (double *)malloc(sizeof(double) * sz);
+    memset(avg, 0, sizeof(double) * sz);
+    for (int ib = 0; ib < nbk; ib++) {
+        int nd = 0;
+        for (int jb = 0; jb < nbk; jb++)
+            if (fabs(ek[ib] - ek[jb]) <= cutoff) {
+                for (int nu = 0; nu < nm; nu++)
+                    for (int m = 0; m < nbkq; m++)
+                        avg[m + nbkq * (ib + nbk * nu)] += g2[m + nbkq * (jb + nbk * nu)];
+                nd++;
+            }
+        for (int nu = 0; nu < nm; nu++)
+            for (int m = 0; m < nbkq; m++) avg[m + nbkq * (ib + nbk * nu)] /= nd;
+    }
+    memcpy(g2, avg, sizeof(double) * sz);
+    memset(avg, 0, sizeof(double) * sz);
+    for (int ib = 0; ib < nbkq; ib++) {
+        int nd = 0;
+        for (int jb = 0; jb < nbkq; jb++)
+            if (fabs(ekq[ib] - ekq[jb]) <= cutoff) {
+                for (int nu = 0; nu < nm; nu++)
+                    for (int n = 0; n < nbk; n++) avg[ib + nbkq * (n + nbk * nu)] += g2[jb + nbkq * (n + nbk * nu)];
+                nd++;
+            }
+        for (int nu = 0; nu < nm; nu++)
+            for (int n = 0; n < nbk; n++) avg[ib + nbkq * (n + nbk * nu)] /= nd;
+    }
+    memcpy(g2, avg, sizeof(double) * sz);
+    free(avg);
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Outer-q loop body for one q over a list of k (src/calculator/run_eph_over_q_and_k.jl:526-610,
+ * with the per-k window of src/electron_state.jl:118-152 and the k+q column mask of :584-585).
+ * eRpq is the get_eph_RR_to_Rq! output at this q. For each listed k (index ik into the fine-grid
+ * states) and its k+q partner ikq, writes ep restricted to the in-window bands:
+ *   ep_out[(m - mlo) + nbw*((n - nlo) + nbw*(nu + nm*t))], m in [mlo, mhi), n in [nlo, nhi),
+ * with (nlo, nhi) = (bmin[ik]-1, bmax[ik]) and likewise at ikq. Slots beyond the window are 0.
+ * U is (nw, nw, nk_grid) full-band eigenvectors, xk is (3, nk_grid).
+ * ------------------------------------------------------------------------------------------- */
+void ephor_outer_q_block(int nw, int nm, int nr_el, const zc *eRpq, const int32_t *irvec_el, const double *xk,
+                         const zc *U, const int64_t *bmin, const int64_t *bmax, int64_t nlist, const int64_t *ik_list,
+                         const int64_t *ikq_list, int nbw, zc *ep_out) {
+    int64_t nd = (int64_t)nw * nw * nm;
+#pragma omp parallel
+    {
+        zc *g = (zc *)malloc(sizeof(zc) * nd);
+        zc *ep = (zc *)malloc(sizeof(zc) * nw * nw * nm);
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t t = 0; t < nlist; t++) {
+            int64_t ik = ik_list[t], ikq = ikq_list[t];
+            zc *o = ep_out + (int64_t)nbw * nbw * nm * t;
+            for (int64_t i = 0; i < (int64_t)nbw * nbw * nm; i++) o[i] = 0.0;
+            if (bmin[ik] <= 0 || bmin[ikq] <= 0) continue; /* no in-window band at k or k+q */
+            int nlo = (int)bmin[ik] - 1, nhi = (int)bmax[ik];
+            int mlo = (int)bmin[ikq] - 1, mhi = (int)bmax[ikq];
+            ephor_fourier(nd, nr_el, eRpq, nd, irvec_el, 1, xk + 3 * ik, g, nd);
+            int nbk = nhi - nlo, nbkq = mhi - mlo;
+            rotate_rqkq(nw, nm, g, U + (int64_t)nw * nw * ik + (int64_t)nw * nlo, nbk,
+                        U + (int64_t)nw * nw * ikq + (int64_t)nw * mlo, nbkq, ep);
+            for (int nu = 0; nu < nm; nu++)
+                for (int n = 0; n < nbk; n++)
+                    for (int m = 0; m < nbkq; m++)
+                        o[m + nbw * (n + (int64_t)nbw * nu)] = ep[m + nbkq * (n + (int64_t)nbk * nu)];
+        }
+        free(g);
+        free(ep);
+    }
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * BTE scattering increment (src/boltzmann/bte_scattering_core.jl:33-84) with the smeared delta
+ * of src/common/smearing.jl (1 = Gaussian exp(-(x/eta)^2)/(sqrt(pi) eta), 2 = Lorentzian).
+ * Writes (s_out, s_in).
+ * ------------------------------------------------------------------------------------------- */
+static inline double occ_fd(double e, double T) { return 1.0 / (exp(e / T) + 1.0); }
+static inline double occ_fd_deriv(double e, double T) { return -1.0 / (2.0 + exp(e / T) + exp(-e / T)) / T; }
+static inline double occ_boson(double e, double T) { return e == 0.0 ? 0.0 : 1.0 / expm1(e / T); }
+static inline double smear(int method, double eta, double x) {
+    if (method == 1) return (1.0 / sqrt(M_PI)) / eta * exp(-(x / eta) * (x / eta));
+    return (eta / M_PI) / (x * x + eta * eta);
+}
+
+void ephor_bte_increments(int method, double ek, double ekq, double wq, double g2, double wtq, double mu,
+                          double T, int smearing, double eta, double *out) {
+    double d1 = smear(smearing, eta, ek - ekq + wq);
+    double d2 = smear(smearing, eta, ek - ekq - wq);
+    out[0] = out[1] = 0.0;
+    if (d1 == 0.0 && d2 == 0.0) return;
+    double nq = occ_boson(wq, T), fk = occ_fd(ek - mu, T), fkq = occ_fd(ekq - mu, T);
+    double f1o, f2o, f1i, f2i;
+    if (method == 1) {
+        f1o = nq + fkq; f2o = nq + 1 - fkq;
+        f1i = nq + fk;  f2i = nq + 1 - fk;
+    } else if (method == 2) {
+        double preo = fkq * (1 - fkq) / (fk * (1 - fk));
+        f1o = preo * (nq + 1 - fk); f2o = preo * (nq + fk);
+        double prei = fk * (1 - fk) / (fkq * (1 - fkq));
+        f1i = prei * (nq + 1 - fkq); f2i = prei * (nq + fkq);
+    } else if (method == 3) {
+        double preo = (1 - fkq) / (1 - fk); f1o = preo * nq; f2o = preo * (nq + 1);
+        double prei = (1 - fk) / (1 - fkq); f1i = prei * nq; f2i = prei * (nq + 1);
+    } else if (method == 4) {
+        double preo = fkq / fk; f1o = preo * (nq + 1); f2o = preo * nq;
+        double prei = fk / fkq; f1i = prei * (nq + 1); f2i = prei * nq;
+    } else if (method == 5) {
+        double dk = occ_fd_deriv(ek - mu, T), dkq = occ_fd_deriv(ekq - mu, T);
+        f1o = sqrt(nq * (nq + 1) * dkq / dk); f2o = f1o;
+        f1i = sqrt(nq * (nq + 1) * dk / dkq); f2i = f1i;
+    } else {
+        double no = occ_boson(ekq - ek, T); f1o = no + fkq; f2o = -(no + fkq);
+        double ni = occ_boson(ek - ekq, T); f1i = ni + fk; f2i = -(ni + fk);
+    }
+    double pref = 2.0 * M_PI * wtq * g2;
+    out[0] = ((d1 == 0.0 ? 0.0 : d1 * f1o) + (d2 == 0.0 ? 0.0 : d2 * f2o)) * pref;
+    out[1] = ((d2 == 0.0 ? 0.0 : d2 * f1i) + (d1 == 0.0 ? 0.0 : d1 * f2i)) * pref;
+}
+
+int ephor_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

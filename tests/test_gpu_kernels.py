@@ -1,0 +1,207 @@
+"""GPU parity of each libephip kernel against the FP64 oracle (called through the C-ABI).
+
+Error metric for floating point: norm-wise max|got - ref| / max|ref| (the reference's own test
+style, test/test_gpu.jl:508,516). Tolerances are written in each test."""
+import numpy as np
+import pytest
+
+import oracle as O
+from ephip import model as M
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _rel(got, ref):
+    return float(np.abs(got - ref).max() / np.abs(ref).max())
+
+
+def _dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+# ------------------------------------------------------------------------------ Fourier
+@pytest.mark.parametrize("ndata,box,nk", [(1, 3, 1), (77, 5, 130), (256, 9, 257), (1000, 7, 64)])
+def test_fourier_parity(backend, ndata, box, nk):
+    from ephip import core
+    from ephip._lib import PREC_BF16X3, PREC_FP32
+    rng = np.random.default_rng(ndata + nk)
+    irvec = M.rbox(box)
+    nr = len(irvec)
+    decay = np.exp(-np.linalg.norm(irvec, axis=1) / 1.5)[:, None]
+    op = (rng.standard_normal((nr, ndata)) + 1j * rng.standard_normal((nr, ndata))) * decay
+    xk = rng.random((nk, 3)) * 2 - 0.5
+    ref = O.fourier(op, irvec, xk)
+    xk_d = _dev(xk)
+    obj64 = core.WannierObject(irvec, _dev(op))
+    got64 = core.fourier_batched(obj64, xk_d).cpu().numpy()
+    assert _rel(got64, ref) < 1e-13, "fp64"
+    obj32 = core.WannierObject(irvec, _dev(op.astype(np.complex64)))
+    got32 = core.fourier_batched(obj32, xk_d, precision=PREC_FP32).cpu().numpy()
+    assert _rel(got32, ref) < 2e-6, "fp32"
+    gotb = core.fourier_batched(obj32, xk_d, precision=PREC_BF16X3).cpu().numpy()
+    assert _rel(gotb, ref) < 2e-5, "split-bf16"
+
+
+def test_fourier_leading_dimensions(backend):
+    from ephip import core
+    rng = np.random.default_rng(3)
+    irvec = M.rbox(3)
+    op = rng.standard_normal((27, 40)) + 1j * rng.standard_normal((27, 40))
+    xk = rng.random((70, 3))
+    ref = O.fourier(op, irvec, xk, ndata=33)
+    obj = core.WannierObject(irvec, _dev(op.astype(np.complex64)), ndata=33)
+    out = torch.full((70, 50), 7.0 + 7j, dtype=torch.complex64, device="cuda")
+    core.fourier_batched(obj, _dev(xk), out=out)
+    o = out.cpu().numpy()
+    assert _rel(o[:, :33], ref) < 2e-6
+    assert np.all(o[:, 33:] == 7.0 + 7j)  # padding beyond ndata untouched
+
+
+def test_fourier_phase_parity(backend):
+    from ephip import core
+    rng = np.random.default_rng(4)
+    irvec = M.rbox(5)
+    xk = rng.random((33, 3))
+    got = core.fourier_phase(_dev(irvec), _dev(xk)).cpu().numpy()
+    assert np.abs(got - O.fourier_phase(irvec, xk)).max() < 1e-14
+
+
+# ------------------------------------------------------------------------------ eigensolver
+def _planted(rng, n, nb, degenerate_frac=0.3, split=0.0):
+    hs = []
+    for b in range(nb):
+        q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+        w = np.sort(rng.uniform(-5, 5, n))
+        if n >= 3 and b < degenerate_frac * nb:
+            w[1] = w[2] = w[1] if split == 0 else w[1]
+            w[2] = w[1] + split
+            w = np.sort(w)
+        hs.append((q * w) @ q.conj().T)
+    return np.stack(hs)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 6, 8, 12, 16, 17, 24, 32])
+def test_heev_c32_parity(backend, n):
+    from ephip import core
+    rng = np.random.default_rng(100 + n)
+    nb = 203
+    a = rng.standard_normal((nb, n, n)) + 1j * rng.standard_normal((nb, n, n))
+    h = a + np.conj(np.transpose(a, (0, 2, 1)))
+    hc = np.ascontiguousarray(np.transpose(h, (0, 2, 1)))  # column-major blocks
+    E, U = core.eigen_batched(_dev(hc.astype(np.complex64)))
+    E, U = E.cpu().numpy(), U.cpu().numpy()
+    Ev = core.eigvals_batched(_dev(hc)).cpu().numpy()  # c64 input, values only
+    for b in range(nb):
+        wl = np.linalg.eigvalsh(h[b])
+        scale = np.abs(wl).max()
+        assert np.abs(E[b] - wl).max() < 3e-6 * scale
+        assert np.abs(Ev[b] - wl).max() < 3e-6 * scale
+        V = U[b].T  # column-major block -> matrix
+        assert np.abs(h[b] @ V - V * E[b]).max() < 5e-6 * scale
+        assert np.abs(V.conj().T @ V - np.eye(n)).max() < 5e-6
+
+
+@pytest.mark.parametrize("n,split", [(8, 0.0), (16, 0.0), (16, 3e-6), (12, 5e-5)])
+def test_heev_refine_fp64(backend, n, split):
+    """FP32 solve + FP64 refinement reaches FP64-level eigenvalues, also on exact and
+    near-degenerate clusters, and re-orthogonalises the cluster vectors."""
+    from ephip import core
+    rng = np.random.default_rng(7 + n)
+    h = _planted(rng, n, 150, degenerate_frac=0.4, split=split)
+    hc = np.ascontiguousarray(np.transpose(h, (0, 2, 1)))
+    E, U = core.eigen_batched_refined(_dev(hc))
+    E, U = E.cpu().numpy(), U.cpu().numpy()
+    for b in range(len(h)):
+        wl = np.linalg.eigvalsh(h[b])
+        assert np.abs(E[b] - wl).max() < 1e-11 * np.abs(wl).max()
+        V = U[b].T.astype(np.complex128)
+        assert np.abs(V.conj().T @ V - np.eye(n)).max() < 3e-6
+        assert np.abs(h[b] @ V - V * E[b]).max() < 3e-5
+
+
+# ------------------------------------------------------------------------------ states + filter
+def test_electron_states_and_filter_bit_exact(backend):
+    from ephip import core, eph
+    m, g = M.make_config_model("cfg1", with_g=False)
+    xk = M.kpoints_grid(g["kgrid"])
+    Hk = O.fourier(m.ham_r, m.irvec_h, xk)
+    E_ref, _ = O.heev_batched(Hk)
+    w = g["window"]
+    keep_r, bmin_r, bmax_r, nb_r = O.window_filter(E_ref, -w, w)
+    ham = core.WannierObject(m.irvec_h, m.ham_r).to_device(backend)
+    st = eph.compute_electron_states(ham, _dev(xk), (-w, w), chunk=200)
+    E = st.E.cpu().numpy()
+    assert np.abs(E - E_ref).max() < 1e-6  # eV; FP64 refinement gives far less
+    assert np.abs(E - E_ref).max() < 1e-11 * np.abs(E_ref).max()
+    f = st.filt
+    assert np.array_equal(f.keep.cpu().numpy(), keep_r)
+    assert np.array_equal(f.band_min.cpu().numpy(), bmin_r)
+    assert np.array_equal(f.band_max.cpu().numpy(), bmax_r)
+    assert np.array_equal(f.nbelow.cpu().numpy(), nb_r)
+    # the window kernel itself on identical input is bit-exact
+    keep2 = core.window_filter(_dev(E_ref), -w, w)
+    assert np.array_equal(keep2.band_min.cpu().numpy(), bmin_r)
+
+
+def test_window_filter_edges(backend):
+    from ephip import core
+    E = np.array([[-1.0, -0.5, -0.5, 0.0, 0.5, 0.5, 1.0, 2.0],
+                  [-3, -2, -1, -0.6, 0.6, 1, 2, 3],
+                  [0.5, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.1]], dtype=np.float64)
+    f = core.window_filter(_dev(E), -0.5, 0.5)
+    ref = O.window_filter(E, -0.5, 0.5)
+    for got, r in zip((f.keep, f.band_min, f.band_max, f.nbelow), ref):
+        assert np.array_equal(got.cpu().numpy(), r)
+
+
+def test_kq_index_bit_exact(backend):
+    from ephip import core
+    for ng in [(4, 4, 4), (3, 4, 5), (48, 48, 48)]:
+        xk = M.kpoints_grid(ng)
+        rng = np.random.default_rng(sum(ng))
+        sel = rng.choice(len(xk), size=min(len(xk), 5000), replace=False)
+        for dq in [(0.0, 0.0, 0.0), (1 / ng[0], 2 / ng[1], -3 / ng[2]), (0.5, -0.25, 0.75)]:
+            got = core.kq_index(_dev(xk[sel]), dq, ng).cpu().numpy()
+            ref = O.hash_xk(xk[sel] + np.array(dq), ng)
+            assert np.array_equal(got, ref)
+
+
+def test_phonon_states_parity(backend):
+    from ephip import core, eph
+    m, g = M.make_config_model("cfg1", with_g=False)
+    xq = M.kpoints_grid(g["qgrid"])
+    Dq = O.fourier(m.dyn_r, m.irvec_d, xq)
+    om_ref = np.array([O.ph_eigen(Dq[i], m.mass)[0] for i in range(len(xq))])
+    dyn = core.WannierObject(m.irvec_d, m.dyn_r).to_device(backend)
+    ph = eph.compute_phonon_states(dyn, m.mass, _dev(xq))
+    om = ph.omega.cpu().numpy()
+    sel = np.abs(om_ref) > 1e-5  # acoustic modes at Gamma are ~0 and sign-ambiguous
+    assert np.abs(om - om_ref)[sel].max() < 1e-9
+    # eigenvectors: D(q) u = w^2 M u  with u rows scaled by 1/sqrt(m)
+    U = ph.u.cpu().numpy()
+    for i in range(0, len(xq), 37):
+        d = Dq[i].reshape(m.nmodes, m.nmodes, order="F")
+        u = U[i].T.astype(np.complex128)
+        lhs = d @ u
+        rhs = u * (np.sign(om_ref[i]) * om_ref[i] ** 2)
+        assert np.abs(lhs - rhs).max() < 1e-6 * np.abs(d).max()
+
+
+def test_rotate_modes_parity(backend):
+    from ephip import _lib as L
+    rng = np.random.default_rng(9)
+    nw, nm, nr_el, nq = 3, 6, 27, 5
+    nb = nw * nw
+    g = rng.standard_normal((nq, nb * nm * nr_el)) + 1j * rng.standard_normal((nq, nb * nm * nr_el))
+    u = rng.standard_normal((nq, nm, nm)) + 1j * rng.standard_normal((nq, nm, nm))  # column-major blocks
+    gd = _dev(g.astype(np.complex64))
+    ud = _dev(u.astype(np.complex64))
+    out = torch.empty_like(gd)
+    L.call("ephip_eph_rotate_modes", nb, nm, nr_el, nq, L.ptr(gd), gd.stride(0), L.ptr(ud), L.ptr(out),
+           out.stride(0), L.stream_handle())
+    got = out.cpu().numpy().reshape(nq, nr_el, nm, nb)
+    gg = g.reshape(nq, nr_el, nm, nb)
+    ref = np.einsum("qrnb,qmn->qrmb", gg, u)  # u[q] stored transposed: u[q][nu', nu] = uph[nu, nu']
+    assert _rel(got, ref) < 1e-6

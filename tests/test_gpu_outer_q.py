@@ -1,0 +1,79 @@
+"""End-to-end parity of the device outer-q e-ph loop against the FP64 oracle on a full small model
+(the plain method of the reference's CPU path, cfg1 shape: nw 8, 2 atoms, 5^3 boxes, 8^3 grids).
+
+Gauge-invariant comparison: g2 summed over degenerate subspaces (epstate_g2_degenerate_average!,
+src/EPState.jl:205-238) with the 0.01 meV cutoff, modes below the acoustic cutoff excluded (the
+reference's consumers skip them, src/common/constants.jl:5). Tolerance: norm-wise relative error
+of |g|^2 <= 1e-4 for FP32 and split-BF16."""
+import numpy as np
+import pytest
+
+import oracle as O
+from ephip import model as M
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    m, g = M.make_config_model("cfg1")
+    return m, g
+
+
+def _oracle_states(m, g):
+    xk = M.kpoints_grid(g["kgrid"])
+    E, U = O.heev_batched(O.fourier(m.ham_r, m.irvec_h, xk))
+    keep, bmin, bmax, _ = O.window_filter(E, -g["window"], g["window"])
+    xq = M.kpoints_grid(g["qgrid"])
+    Dq = O.fourier(m.dyn_r, m.irvec_d, xq)
+    ph = [O.ph_eigen(Dq[i], m.mass) for i in range(len(xq))]
+    return xk, E, U, keep, bmin, bmax, xq, ph
+
+
+@pytest.mark.parametrize("prec_name", ["fp32", "bf16x3"])
+def test_outer_q_loop_g2_parity(backend, cfg1, prec_name):
+    from ephip import _lib as L
+    from ephip import eph
+    m, g = cfg1
+    prec = L.PREC_FP32 if prec_name == "fp32" else L.PREC_BF16X3
+    w = g["window"]
+    st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
+    xk, E, U, keep, bmin, bmax, xq, ph = _oracle_states(m, g)
+    assert np.array_equal(st.el.filt.keep.cpu().numpy(), keep)
+    assert np.array_equal(st.ksurv.cpu().numpy(), np.flatnonzero(keep))
+    loop = eph.OuterQLoop(st, nk_batch=100, nq_tile=7, precision=prec)
+    rec = eph.G2Recorder()
+    iqs = [0, 1, 9, 77, 300, 511]
+    for iq in iqs:
+        loop.run_qtile(iq, 1, [rec])
+    nw, nm, nbw = m.nw, m.nmodes, st.nbw
+    cutoff = 1e-5  # 0.01 meV in eV
+    errs = []
+    for iq in iqs:
+        om, uph = ph[iq]
+        eRpq = O.eph_RR_to_Rq(nw, nm, m.epmat_ph, m.irvec_el, m.irvec_ep, xq[iq], uph)
+        blocks = [b for b in rec.blocks if b[0] == iq]
+        ik = np.concatenate([b[1] for b in blocks])
+        ikq = np.concatenate([b[2] for b in blocks])
+        g2 = np.concatenate([b[3] for b in blocks])  # (nl, nm, nbw, nbw) = [nu, n, m]
+        assert np.array_equal(ik, np.flatnonzero(keep))
+        assert np.array_equal(ikq, O.hash_xk(xk[ik] + xq[iq], g["kgrid"]))
+        ep_ref = O.outer_q_block(nw, nm, eRpq, m.irvec_el, xk, U, bmin, bmax, ik, ikq, nbw)
+        ok = np.abs(om) >= 1e-5
+        for t in range(len(ik)):
+            a, b = ik[t], ikq[t]
+            if not keep[b]:
+                assert not g2[t].any()
+                continue
+            nbk, nbkq = bmax[a] - bmin[a] + 1, bmax[b] - bmin[b] + 1
+            ek, ekq = E[a, bmin[a] - 1:bmax[a]], E[b, bmin[b] - 1:bmax[b]]
+            r = O.g2(ep_ref[t][:, :nbk, :nbkq], om)
+            gg = g2[t][:, :nbk, :nbkq].astype(np.float64)
+            r = O.g2_degenerate_average(r, ekq, ek, cutoff)[ok]
+            gg = O.g2_degenerate_average(gg, ekq, ek, cutoff)[ok]
+            assert not g2[t][:, nbk:, :].any() and not g2[t][:, :, nbkq:].any()
+            errs.append((np.abs(gg - r).max(), np.abs(r).max()))
+    e = np.array(errs)
+    rel = e[:, 0].max() / e[:, 1].max()
+    assert rel < 1e-4, rel

@@ -1,0 +1,66 @@
+"""C-ABI checks that need no GPU: the library loads, exports exactly what include/ephip.h declares,
+and argument errors are reported synchronously with a message (no kernel is launched)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from ephip import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "ephip.h")).read()
+    return set(re.findall(r"^(?:const char \*|int )(ephip_\w+)\(", src, flags=re.M))
+
+
+def test_header_and_binding_agree():
+    assert _declared() == set(L.exported_symbols())
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(L.LIB_PATH)
+    for name in _declared():
+        assert hasattr(lib, name), name
+
+
+def test_version_and_empty_error():
+    lib = L.load()
+    assert lib.ephip_version() >= 100
+
+
+@pytest.mark.parametrize("n", [0, 33, 64])
+def test_heev_rejects_bad_n(n):
+    with pytest.raises(L.EphipError, match="outside 1..32"):
+        L.call("ephip_heev_c32", n, 4, 1, 0, 1, 1, None, None)
+
+
+def test_fourier_rejects_bad_leading_dimension():
+    with pytest.raises(L.EphipError, match="leading dimension"):
+        L.call("ephip_fourier_c32", 100, 7, 1, 50, 1, 3, 1, 1, 100, L.PREC_FP32, None)
+
+
+def test_fourier_rejects_unknown_precision():
+    with pytest.raises(L.EphipError, match="precision"):
+        L.call("ephip_fourier_c32", 10, 7, 1, 10, 1, 3, 1, 1, 10, 9, None)
+
+
+def test_zero_sized_calls_are_noops():
+    # nothing to do -> OK without touching the device
+    L.call("ephip_fourier_c32", 0, 7, None, 0, None, 3, None, None, 0, L.PREC_FP32, None)
+    L.call("ephip_heev_c32", 8, 0, None, 0, None, None, None, None)
+    L.call("ephip_window_filter", 8, 0, None, 0.0, 1.0, None, None, None, None, None)
+
+
+def test_rotation_rejects_oversized_window_block():
+    with pytest.raises(L.EphipError, match="nbw"):
+        L.call("ephip_eph_rotate_rqkq_g2", 16, 12, 4, 1, 16 * 16 * 12, 1, 1, 1, 1, 1, 1, 17, 1, None, None)
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(L, "_LIB", None)
+    monkeypatch.setattr(L, "LIB_PATH", str(tmp_path / "libephip.so"))
+    with pytest.raises(L.EphipError, match="no CPU fallback"):
+        L.load()

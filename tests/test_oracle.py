@@ -1,0 +1,294 @@
+"""Pins the FP64 oracle (oracle/ephor.c) against things that do not share its code:
+numpy's FFT and LAPACK, closed forms, the reference's golden values, and identities that
+two different oracle code paths must satisfy (the reference's own test strategy,
+test/test_gpu.jl:44-211, test/test_epmat.jl:46-77, test/test_iq_build.jl:34-128)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from ephip import model as M
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _rand_herm(rng, n):
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    return a + a.conj().T
+
+
+# ---------------------------------------------------------------- Fourier (A1, A2)
+def test_fourier_phase_matches_cispi():
+    # reference bound: 4 eps vs a scalar cispi (test/test_gpu.jl:127-144)
+    rng = np.random.default_rng(1)
+    irvec = rng.integers(-3, 4, size=(37, 3)).astype(np.int32)
+    xk = rng.random((11, 3))
+    ph = O.fourier_phase(irvec, xk)
+    # extended-precision reference (x87 long double: 64-bit mantissa) without argument reduction
+    pi_l = np.arctan(np.longdouble(1)) * 4
+    arg = 2 * pi_l * (xk.astype(np.longdouble) @ irvec.T.astype(np.longdouble))
+    ref = np.cos(arg) + 1j * np.sin(arg)
+    # The FP64 argument R.x itself carries ~|R.x| eps of rounding (as in the reference's cispi(2 R.x));
+    # beyond that the reduction and sincos may add a few eps.
+    eps = np.finfo(float).eps
+    tmax = float(np.abs(xk @ irvec.T).max())
+    assert np.abs(ph - ref.astype(np.complex128)).max() <= 2 * np.pi * 2 * eps * tmax + 4 * eps
+
+
+@pytest.mark.parametrize("box,m", [(3, 4), (5, 6), (5, 8)])
+def test_fourier_equals_fft_on_full_box(box, m):
+    """On a complete R-box and an m-grid (m >= box), sum_R O(R) e^{2 pi i k.R} is an inverse DFT:
+    numpy.fft is an independent implementation of the same sum."""
+    rng = np.random.default_rng(box * 10 + m)
+    irvec = M.rbox(box)
+    ndata = 5
+    op = rng.standard_normal((len(irvec), ndata)) + 1j * rng.standard_normal((len(irvec), ndata))
+    xk = M.kpoints_grid((m, m, m))
+    got = O.fourier(op, irvec, xk)
+    A = np.zeros((m, m, m, ndata), complex)
+    for r, R in enumerate(irvec):
+        A[R[0] % m, R[1] % m, R[2] % m] += op[r]
+    ref = np.fft.ifftn(A, axes=(0, 1, 2)) * m ** 3
+    assert np.allclose(got, ref.reshape(m ** 3, ndata), rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+def test_fourier_hermitian_and_periodic():
+    m, _ = M.make_config_model("cfg1", with_g=False)
+    rng = np.random.default_rng(3)
+    xk = rng.random((7, 3))
+    hk = O.fourier(m.ham_r, m.irvec_h, xk)
+    hk2 = O.fourier(m.ham_r, m.irvec_h, xk + np.array([1.0, -2.0, 3.0]))
+    assert np.abs(hk - hk2).max() < 1e-11
+    for j in range(len(xk)):
+        h = hk[j].reshape(m.nw, m.nw, order="F")
+        assert np.abs(h - h.conj().T).max() < 1e-12
+
+
+def test_fourier_ndata_and_ld():
+    rng = np.random.default_rng(4)
+    irvec = M.rbox(3)
+    op = rng.standard_normal((27, 9)) + 1j * rng.standard_normal((27, 9))
+    xk = rng.random((4, 3))
+    full = O.fourier(op, irvec, xk)
+    part = O.fourier(op, irvec, xk, ndata=5)
+    assert np.array_equal(part, full[:, :5])
+
+
+# ---------------------------------------------------------------- eigensolver (A5, A7)
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 7, 8, 12, 16, 31, 32])
+def test_heev_vs_lapack(n):
+    rng = np.random.default_rng(n)
+    a = _rand_herm(rng, n)
+    w, v = O.heev(a)
+    assert np.all(np.diff(w) >= 0)
+    wl = np.linalg.eigvalsh(a)  # LAPACK zheevd via numpy: an independent implementation
+    assert np.abs(w - wl).max() <= 1e-13 * max(1.0, np.abs(wl).max())
+    assert np.abs(a @ v - v * w).max() <= 1e-12 * np.abs(a).max()
+    assert np.abs(v.conj().T @ v - np.eye(n)).max() <= 1e-13
+
+
+def test_heev_closed_forms():
+    # diagonal: eigenvalues are the sorted diagonal
+    d = np.array([3.0, -1.0, 2.0, 0.5])
+    w, v = O.heev(np.diag(d).astype(complex))
+    assert np.array_equal(w, np.sort(d))
+    # 2x2 [[a, c], [conj c, b]]: (a+b)/2 +- sqrt(((a-b)/2)^2 + |c|^2)
+    a, b, c = 0.3, -0.7, 0.2 - 0.4j
+    w = O.heev(np.array([[a, c], [np.conj(c), b]]), vectors=False)
+    r = np.sqrt(((a - b) / 2) ** 2 + abs(c) ** 2)
+    assert np.allclose(w, [(a + b) / 2 - r, (a + b) / 2 + r], atol=1e-15)
+    # exact degeneracy: U diag(1,1,2,5) U^H
+    rng = np.random.default_rng(5)
+    q, _ = np.linalg.qr(rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4)))
+    h = q @ np.diag([1.0, 1.0, 2.0, 5.0]) @ q.conj().T
+    w, v = O.heev(h)
+    assert np.allclose(w, [1, 1, 2, 5], atol=1e-14)
+    assert np.abs(h @ v - v * w).max() < 1e-13
+
+
+def test_heev_reads_upper_triangle():
+    rng = np.random.default_rng(6)
+    a = _rand_herm(rng, 6)
+    b = a.copy()
+    b[np.tril_indices(6, -1)] = 99.0  # garbage below the diagonal
+    assert np.allclose(O.heev(b, vectors=False), np.linalg.eigvalsh(a), atol=1e-13)
+
+
+def test_ph_eigen_closed_form_and_mass():
+    # D diagonal with masses: omega_i = sign(d_i/m_i) sqrt(|d_i/m_i|), u = e_i / sqrt(m_i)
+    d = np.array([4.0, -9.0, 1.0])
+    mass = np.array([1.0, 4.0, 0.25])
+    om, u = O.ph_eigen(np.diag(d).astype(complex).ravel(order="F"), mass)
+    expect = sorted(np.sign(d / mass) * np.sqrt(np.abs(d / mass)), key=lambda x: np.sign(x) * x * x)
+    assert np.allclose(om, expect, atol=1e-14)
+    U = u.reshape(3, 3, order="F")
+    # columns are unit eigenvectors scaled by 1/sqrt(m) row-wise: U^H diag(m) U = I
+    assert np.allclose(U.conj().T @ np.diag(mass) @ U, np.eye(3), atol=1e-13)
+
+
+# ---------------------------------------------------------------- window filter (A9)
+def test_window_filter_vs_searchsorted():
+    rng = np.random.default_rng(7)
+    E = np.sort(rng.uniform(-3, 3, size=(500, 8)), axis=1)
+    E[0] = [-1.0, -0.5, -0.5, 0.0, 0.5, 0.5, 1.0, 2.0]  # ties exactly on the edges
+    wmin, wmax = -0.5, 0.5
+    keep, bmin, bmax, nb = O.window_filter(E, wmin, wmax)
+    for k in range(len(E)):
+        first = np.searchsorted(E[k], wmin, side="left") + 1   # searchsortedfirst
+        last = np.searchsorted(E[k], wmax, side="right")       # searchsortedlast
+        assert nb[k] == first - 1
+        if first <= last:
+            assert keep[k] == 1 and bmin[k] == first and bmax[k] == last
+        else:
+            assert keep[k] == 0 and bmin[k] == 0 and bmax[k] == 0
+    assert (bmin[0], bmax[0]) == (2, 6)
+
+
+# ---------------------------------------------------------------- k-point maps (A10)
+def test_kpoints_grid_order():
+    x = O.kpoints_grid((2, 3, 4))
+    assert np.array_equal(x, M.kpoints_grid((2, 3, 4)))
+    assert np.allclose(x[1], [0, 0, 0.25]) and np.allclose(x[4], [0, 1 / 3, 0])  # third index fastest
+
+
+def test_hash_is_grid_index_and_xk_to_ik_oracle():
+    # On a complete unshifted grid in kpoints_grid order, hash == 0-based index; for k+q the hash is
+    # the index of the folded sum, checked by brute-force nearest-node search on a non-cubic grid
+    # (a transposed packing cannot pass, cf. test/test_iq_build.jl:55-61).
+    ng = (3, 4, 5)
+    xk = M.kpoints_grid(ng)
+    assert np.array_equal(O.hash_xk(xk, ng), np.arange(len(xk)))
+    rng = np.random.default_rng(8)
+    for _ in range(200):
+        a, b = rng.integers(len(xk), size=2)
+        s = xk[a] + xk[b] + rng.integers(-2, 3, size=3)
+        h = O.hash_xk(s, ng)[0]
+        frac = np.mod(s, 1.0)
+        d = np.abs(((xk - frac + 0.5) % 1.0) - 0.5).sum(axis=1)
+        assert h == int(np.argmin(d))
+
+
+def test_hash_rounds_half_to_even():
+    # round(Int, x) in the reference is RoundNearest (ties to even)
+    ng = np.array([4, 4, 4], np.int32)
+    assert O.hash_xk(np.array([0.125, 0.375, 0.0]), ng)[0] == (0 * 4 + 2) * 4 + 0
+    assert O.hash_xk(np.array([-0.125, 0.0, 0.0]), ng)[0] == 0
+
+
+# ---------------------------------------------------------------- e-ph stages (A11-A17)
+def _small_eph(seed=11, nw=3, natoms=1, box=3):
+    m = M.make_model(nw=nw, natoms=natoms, box_hd=box, box_g=box, seed=seed)
+    return m
+
+
+def _el_layout(m):
+    """"el" layout (iw, jw, nu, Rp) x Re from the "ph" layout (iw, jw, nu, Re) x Rp."""
+    nb = m.nw * m.nw * m.nmodes
+    nre, nrp = len(m.irvec_el), len(m.irvec_ep)
+    g = m.epmat_ph.reshape(nrp, nre, nb)       # [Rp, Re, b]
+    return np.ascontiguousarray(g.transpose(1, 0, 2).reshape(nre, nrp * nb))  # [Re, (b, Rp)]
+
+
+def test_stage_composition_ph_equals_el():
+    """RR->Rq->kq == RR->kR->kq, the reference's test/test_epmat.jl:46-77 across outer momenta."""
+    m = _small_eph()
+    rng = np.random.default_rng(12)
+    nw, nm = m.nw, m.nmodes
+    xk, xq = rng.random(3), rng.random(3)
+    uk = np.linalg.qr(rng.standard_normal((nw, nw)) + 1j * rng.standard_normal((nw, nw)))[0][:, :2]
+    ukq = np.linalg.qr(rng.standard_normal((nw, nw)) + 1j * rng.standard_normal((nw, nw)))[0][:, 1:]
+    uph = rng.standard_normal((nm, nm)) + 1j * rng.standard_normal((nm, nm))
+    uph_c = uph.ravel(order="F")
+    eRpq = O.eph_RR_to_Rq(nw, nm, m.epmat_ph, m.irvec_el, m.irvec_ep, xq, uph_c)
+    ep1 = O.eph_Rq_to_kq(nw, nm, eRpq, m.irvec_el, xk, uk, ukq)
+    ekpR = O.eph_RR_to_kR(nw, nm, _el_layout(m), m.irvec_el, m.irvec_ep, xk, uk)
+    ep2 = O.eph_kR_to_kq(nw, 2, nm, ekpR, m.irvec_ep, xq, uph_c, ukq)
+    assert np.allclose(ep1, ep2, rtol=0, atol=1e-12 * np.abs(ep1).max())
+    # brute-force double phase sum with einsum (different contraction order and code)
+    nb = nw * nw * nm
+    g = m.epmat_ph.reshape(len(m.irvec_ep), len(m.irvec_el), nm, nw, nw)  # [Rp, Re, nu, j, i]
+    pe = np.exp(2j * np.pi * (m.irvec_el @ xk))
+    pp = np.exp(2j * np.pi * (m.irvec_ep @ xq))
+    gkq = np.einsum("p,e,penji->ijn", pp, pe, g)
+    ep3 = np.einsum("im,ijn,jb,nv->vbm", ukq.conj(), gkq, uk, uph)
+    assert np.allclose(ep1, ep3, rtol=0, atol=1e-12 * np.abs(ep1).max())
+    assert nb == m.epmat_ph.shape[1] // len(m.irvec_el)
+
+
+def test_kq_convention_identity():
+    """exp(2 pi i Rp.q) = exp(2 pi i Rp.x_{k+q}) conj(exp(2 pi i Rp.x_k)) for integer Rp, also when
+    q needs a mod-G fold (test/test_gpu.jl:156-211)."""
+    rng = np.random.default_rng(13)
+    irvec = M.rbox(5)
+    xk = rng.random(3)
+    xq = rng.random(3)
+    xkq = np.mod(xk + xq, 1.0) - np.array([1.0, 0.0, 1.0])
+    pq = O.fourier_phase(irvec, xq[None])[0]
+    pkq = O.fourier_phase(irvec, xkq[None])[0]
+    pk = O.fourier_phase(irvec, xk[None])[0]
+    assert np.abs(pq - pkq * pk.conj()).max() < 1e-13
+
+
+def test_g2_and_degenerate_average_closed_form():
+    rng = np.random.default_rng(14)
+    ep = rng.standard_normal((2, 3, 4)) + 1j * rng.standard_normal((2, 3, 4))  # (nm, nbk, nbkq)
+    om = np.array([0.02, 0.05])
+    g2 = O.g2(ep, om)
+    assert np.allclose(g2, np.abs(ep) ** 2 / (2 * om[:, None, None]))
+    ek = np.array([0.1, 0.1 + 5e-6, 0.3])          # bands 0,1 degenerate at k (cutoff 1e-5)
+    ekq = np.array([-0.2, 0.0, 0.0 + 2e-6, 0.4])    # bands 1,2 degenerate at k+q
+    avg = O.g2_degenerate_average(g2, ekq, ek, 1e-5)
+    ref = g2.copy()
+    ref[:, [0, 1], :] = ref[:, [0, 1], :].mean(axis=1, keepdims=True)
+    ref[:, :, [1, 2]] = ref[:, :, [1, 2]].mean(axis=2, keepdims=True)
+    assert np.allclose(avg, ref, rtol=1e-14)
+    # gauge invariance: a unitary mix of the degenerate k bands leaves the averaged g2 unchanged
+    th = 0.7
+    rot = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]]) * np.exp(0.3j)
+    ep_r = ep.copy()
+    ep_r[:, [0, 1], :] = np.einsum("vbm,bc->vcm", ep[:, [0, 1], :], rot)
+    avg_r = O.g2_degenerate_average(O.g2(ep_r, om), ekq, ek, 1e-5)
+    assert np.allclose(avg_r, avg, rtol=1e-12)
+
+
+def test_outer_q_block_matches_stagewise():
+    m = _small_eph(nw=4)
+    nw, nm = m.nw, m.nmodes
+    rng = np.random.default_rng(15)
+    xk = M.kpoints_grid((4, 4, 4))
+    Hk = O.fourier(m.ham_r, m.irvec_h, xk)
+    E, U = O.heev_batched(Hk)
+    keep, bmin, bmax, _ = O.window_filter(E, -1.0, 1.0)
+    uph = rng.standard_normal((nm, nm)) + 1j * rng.standard_normal((nm, nm))
+    xq = np.array([0.25, 0.5, 0.0])
+    eRpq = O.eph_RR_to_Rq(nw, nm, m.epmat_ph, m.irvec_el, m.irvec_ep, xq, uph.ravel(order="F"))
+    ik = np.flatnonzero(keep)
+    ikq = O.hash_xk(xk[ik] + xq, (4, 4, 4))
+    nbw = int((bmax - bmin + 1)[keep == 1].max())
+    blk = O.outer_q_block(nw, nm, eRpq, m.irvec_el, xk, U, bmin, bmax, ik, ikq, nbw)
+    for t in range(0, len(ik), 5):
+        a, b = ik[t], ikq[t]
+        if not keep[b]:
+            assert not blk[t].any()
+            continue
+        Ua = U[a].reshape(nw, nw, order="F")[:, bmin[a] - 1:bmax[a]]
+        Ub = U[b].reshape(nw, nw, order="F")[:, bmin[b] - 1:bmax[b]]
+        ep = O.eph_Rq_to_kq(nw, nm, eRpq, m.irvec_el, xk[a], Ua, Ub)
+        assert np.allclose(blk[t][:, :Ua.shape[1], :Ub.shape[1]], ep, atol=1e-12)
+
+
+# ---------------------------------------------------------------- BTE golden vectors (next row)
+def test_bte_increments_golden():
+    with open(os.path.join(HERE, "golden", "bte_increments.json")) as f:
+        gold = json.load(f)
+    i = gold["input"]
+    for meth, (so, si) in gold["expected"].items():
+        got = O.bte_increments(int(meth), i["ek"], i["ekq"], i["wq"], i["g2"], i["wtq"], i["mu"], i["T"], 1,
+                               i["eta"])
+        assert got[0] == pytest.approx(so, rel=gold["rtol"])
+        assert got[1] == pytest.approx(si, rel=gold["rtol"])
+    u = gold["underflow_case"]
+    got = O.bte_increments(u["method"], u["ek"], u["ekq"], u["wq"], u["g2"], u["wtq"], u["mu"], u["T"], 1, u["eta"])
+    assert got == tuple(u["expected"])
