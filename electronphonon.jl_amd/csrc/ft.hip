@@ -9,11 +9,13 @@
 //   [out_im] = [op_im   op_re] [P_im]
 //
 // so the real A' operand (2 ndata x 2 nr) is formed on the fly from the complex op tile and
-// no flop is wasted (8 real flops per complex multiply-add, as in ZGEMM).
+// no flop is wasted (8 real flops per complex multiply-add, as in ZGEMM). The MFMA lane layouts
+// used here were checked on gfx950: f32 16x16x4 and bf16 16x16x32 return C[4(l/16)+r][l%16] in
+// lane l, register r; f64 16x16x4 returns C[4r+l/16][l%16].
 //
 // Tiles: 256 threads = 4 waves in a 2 x 2 arrangement; a workgroup owns TM = 64 complex rows and
 // TN columns; K advances KC = 16 complex R per LDS stage. FP32 and FP64 use the 16x16x4 MFMA
-// (lane l holds A[l%16][l/16], B[l/16][l%16], C[4(l/16)+r][l%16]); the split-BF16 form uses
+// (lane l holds A[l%16][l/16], B[l/16][l%16]); the split-BF16 form uses
 // 16x16x32 bf16 MFMAs with hi/lo operand splitting and three products per real product.
 #include "common.hpp"
 
@@ -93,6 +95,11 @@ __global__ __launch_bounds__(256) void ft_mfma_kernel(int64_t ndata, int nr, con
             Bs[rl][1][pj] = s;
         }
         __syncthreads();
+        // The f32 MFMA returns row 4(l/16)+r of the 16x16 block in lane l, register r; the f64 MFMA
+        // returns row 4r + l/16. Feeding the f64 A operand with rows permuted by
+        // i -> 4(i%4) + i/4 makes both land in the same (lane, register) slots.
+        const int il = lane & 15;
+        const int irow = (sizeof(T) == 8) ? (4 * (il & 3) + (il >> 2)) : il;
 #pragma unroll
         for (int ks = 0; ks < FT_KC / 2; ks++) {
             const int kk = lane >> 4;
@@ -101,7 +108,7 @@ __global__ __launch_bounds__(256) void ft_mfma_kernel(int64_t ndata, int nr, con
             T a[4], b[NCB];
 #pragma unroll
             for (int rb = 0; rb < 4; rb++) {
-                const int rr = wm * 64 + 16 * rb + (lane & 15);
+                const int rr = wm * 64 + 16 * rb + irow;
                 const cplx<T> v = As[rl][rr >> 1];
                 a[rb] = (rr & 1) == 0 ? (cc == 0 ? v.re : -v.im) : (cc == 0 ? v.im : v.re);
             }

@@ -249,6 +249,9 @@ class OuterQLoop:
                    if keep_ep else None)
         self.xk_surv = st.el.xk.index_select(0, st.ksurv).contiguous()
         self.xq_host = st.ph.xq.cpu().numpy()
+        # optional list collecting (start event, end event, columns) around each Rq->kq Fourier launch;
+        # the events are recorded on the current stream, which is the stream the kernels run on
+        self.ft_timer = None
 
     def run_qtile(self, iq0: int, nq: int, calculators=()):
         st = self.st
@@ -267,7 +270,15 @@ class OuterQLoop:
                 e = min(nks, s + self.nk_batch)
                 ik = st.ksurv[s:e]
                 ikq = core.kq_index(self.xk_surv[s:e], self.xq_host[iq], st.kgrid)
+                if self.ft_timer is not None:
+                    torch = _torch()
+                    ev0 = torch.cuda.Event(enable_timing=True)
+                    ev1 = torch.cuda.Event(enable_timing=True)
+                    ev0.record()
                 core.fourier_batched(obj, self.xk_surv[s:e], out=self.gk[: e - s], precision=self.precision)
+                if self.ft_timer is not None:
+                    ev1.record()
+                    self.ft_timer.append((ev0, ev1, e - s))
                 g2 = eph_rotate_rqkq_g2(self.gk[: e - s], ik, ikq, st.el, st.ph.omega[iq], st.nbw,
                                         g2=self.g2[: e - s], ep=None if self.ep is None else self.ep[: e - s])
                 payload = EPDataKWindowed(iq, ik, ikq, g2, None if self.ep is None else self.ep[: e - s], st.nbw)
