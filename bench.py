@@ -88,7 +88,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--qtile", type=int, default=8)
-    ap.add_argument("--kbatch", type=int, default=16384)
+    ap.add_argument("--kbatch", type=int, default=16384, help="k batch of the plain outer-q path")
+    ap.add_argument("--kbatch-k", type=int, default=128, help="k batch (one step) of the fast outer-k path")
+    ap.add_argument("--path", choices=["fast", "plain", "both"], default="both")
     ap.add_argument("--precision", choices=["fp32", "bf16x3"], default="fp32")
     ap.add_argument("--cpu-sample", type=int, default=3000, help="k-points in the CPU baseline slice (0 = skip)")
     ap.add_argument("--config", default="cfg2")
@@ -120,31 +122,92 @@ def main():
     log(f"[bench] setup {time.perf_counter() - t_setup:.1f} s: |K_surv| = {nks} of {nk_total}, nbw = {st.nbw}")
 
     prec = L.PREC_FP32 if args.precision == "fp32" else L.PREC_BF16X3
-    loop = eph.OuterQLoop(st, nk_batch=args.kbatch, nq_tile=args.qtile, precision=prec)
-    ft_events = []
-    loop.ft_timer = ft_events
-    for s in range(args.warmup):
-        loop.run_qtile((s * args.qtile) % nq_total, args.qtile)
-    torch.cuda.synchronize()
-    ft_events.clear()
-    iq_base = args.warmup * args.qtile
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        loop.run_qtile((iq_base + s * args.qtile) % (nq_total - args.qtile + 1), args.qtile)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    pairs = nks * args.qtile * args.steps
-    value = pairs / elapsed
-    # dominant kernel: the Fourier over Re (Rq->kq), timed with HIP events on its own stream
-    durs = [a.elapsed_time(b) * 1e-3 for a, b, _ in ft_events]
-    ncols = [n for _, _, n in ft_events]
-    nd = m.nw * m.nw * m.nmodes
-    nr_el = len(m.irvec_el)
-    flops = [8.0 * nd * nr_el * n for n in ncols]
-    achieved = sum(flops) / sum(durs) / 1e12
-    avg_ms = 1e3 * sum(durs) / len(durs)
-    log(f"[bench] {args.steps} steps in {elapsed:.3f} s; FT kernel avg {avg_ms:.3f} ms "
-        f"({len(durs)} launches), {achieved:.1f} TFLOP/s")
+    nq_grid = nq_total
+    kgrid = grids["kgrid"]
+
+    def kept_pairs(ks):
+        """Number of (unit, q) columns the outer-k path computes for the surviving-k positions ks
+        (host arithmetic on the filter: q kept iff k+q has an in-window band)."""
+        bmin = st.el.filt.band_min.cpu().numpy()
+        bmax = st.el.filt.band_max.cpu().numpy()
+        ksurv = st.ksurv.cpu().numpy()
+        xk = M.kpoints_grid(kgrid)
+        xq = M.kpoints_grid(grids["qgrid"])
+        tot = 0
+        for p in ks:
+            k = ksurv[p]
+            h = O_hash(xk[k] + xq, kgrid)
+            tot += int((bmin[h] > 0).sum()) * int(bmax[k] - bmin[k] + 1)
+        return tot
+
+    def O_hash(x, ng):
+        c = np.mod(np.rint(x * np.asarray(ng)).astype(np.int64), np.asarray(ng))
+        return (c[:, 0] * ng[1] + c[:, 1]) * ng[2] + c[:, 2]
+
+    results = {}
+    # ---- plain outer-q path (reference CS2 shape: separate Fourier / rotate kernels, full K_surv per q)
+    if args.path in ("plain", "both"):
+        loop = eph.OuterQLoop(st, nk_batch=args.kbatch, nq_tile=args.qtile, precision=L.PREC_FP32)
+        ev = []
+        loop.ft_timer = ev
+        for s in range(args.warmup):
+            loop.run_qtile((s * args.qtile) % nq_total, args.qtile)
+        torch.cuda.synchronize()
+        ev.clear()
+        steps = args.steps if args.path == "plain" else max(2, args.steps // 4)
+        iq_base = args.warmup * args.qtile
+        t0 = time.perf_counter()
+        for s in range(steps):
+            loop.run_qtile((iq_base + s * args.qtile) % (nq_total - args.qtile + 1), args.qtile)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        durs = [a.elapsed_time(b) * 1e-3 for a, b, _ in ev]
+        nd = m.nw * m.nw * m.nmodes
+        flops = [8.0 * nd * len(m.irvec_el) * n for _, _, n in ev]
+        results["plain"] = dict(value=nks * args.qtile * steps / el, ms_per_step=1e3 * el / steps, steps=steps,
+                                ft_tflops=sum(flops) / sum(durs) / 1e12, ft_avg_ms=1e3 * sum(durs) / len(durs),
+                                ft_flop_per_launch=sum(flops) / len(flops),
+                                kernel="ft_mfma_kernel<float,128> (Rq->kq Fourier over Re)")
+        log(f"[bench] plain outer-q fp32: {results['plain']['value']:.3e} pairs/s, FT "
+            f"{results['plain']['ft_tflops']:.1f} TFLOP/s")
+        del loop
+    # ---- fast outer-k path (k-side projection, k+q window skip)
+    if args.path in ("fast", "both"):
+        loop = eph.OuterKLoop(st, nk_batch=args.kbatch_k, precision=prec)
+        ev = []
+        loop.ft_timer = ev
+        nks_steps = args.kbatch_k
+        for s in range(args.warmup):
+            loop.run_kbatch(s * nks_steps, nks_steps)
+        torch.cuda.synchronize()
+        ev.clear()
+        k_base = args.warmup * nks_steps
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            loop.run_kbatch((k_base + s * nks_steps) % max(1, nks - nks_steps), nks_steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        ks_done = [((k_base + s * nks_steps) % max(1, nks - nks_steps)) + i for s in range(args.steps)
+                   for i in range(nks_steps)]
+        cols = kept_pairs(ks_done)
+        nd2 = m.nw * m.nmodes
+        nrp = len(m.irvec_ep)
+        kin = {"RR->kR": [], "kR->kq": []}
+        for a, b, lab, fl in ev:
+            kin[lab].append(a.elapsed_time(b) * 1e-3)
+        f_rrkr = 8.0 * m.nw * m.nw * m.nmodes * nrp * len(m.irvec_el) * nks_steps
+        f_krkq_total = 8.0 * nd2 * nrp * cols
+        t_rrkr, t_krkq = sum(kin["RR->kR"]), sum(kin["kR->kq"])
+        results["fast"] = dict(value=nks_steps * nq_grid * args.steps / el, ms_per_step=1e3 * el / args.steps,
+                               steps=args.steps, computed_columns=cols,
+                               rrkr_tflops=f_rrkr * len(kin["RR->kR"]) / t_rrkr / 1e12,
+                               rrkr_avg_ms=1e3 * t_rrkr / len(kin["RR->kR"]),
+                               krkq_tflops=f_krkq_total / t_krkq / 1e12,
+                               krkq_avg_ms=1e3 * t_krkq / len(kin["kR->kq"]),
+                               rrkr_flop_per_launch=f_rrkr, krkq_flop_per_launch=f_krkq_total / len(kin["kR->kq"]))
+        log(f"[bench] fast outer-k {args.precision}: {results['fast']['value']:.3e} pairs/s; RR->kR "
+            f"{results['fast']['rrkr_tflops']:.1f} TFLOP/s ({results['fast']['rrkr_avg_ms']:.2f} ms), kR->kq "
+            f"{results['fast']['krkq_tflops']:.1f} TFLOP/s ({results['fast']['krkq_avg_ms']:.2f} ms)")
 
     cpu = None
     if args.cpu_sample > 0:
@@ -154,30 +217,48 @@ def main():
         cpu = cpu_baseline(O, m, st, epmat, 0, args.cpu_sample, grids["kgrid"])
         log(f"[bench] cpu baseline {cpu['value']:.3e} pairs/s ({cpu['seconds']:.1f} s)")
 
+    if "fast" in results:
+        r = results["fast"]
+        value, ms, steps = r["value"], r["ms_per_step"], r["steps"]
+        # dominant kernel: the larger of the two Fourier launches per step
+        if r["rrkr_avg_ms"] >= r["krkq_avg_ms"]:
+            dom = ("ft kernel, RR->kR Fourier over Re (ndata = nw^2 nm nr_ep, nr = nr_el, nk = k batch)",
+                   r["rrkr_tflops"], r["rrkr_avg_ms"], r["rrkr_flop_per_launch"])
+        else:
+            dom = ("batched ft kernel, kR->kq Fourier over Rp (units x kept q)", r["krkq_tflops"],
+                   r["krkq_avg_ms"], r["krkq_flop_per_launch"])
+        path = "outer-k, k-side window projection, k+q window skip"
+        pairs_step = args.kbatch_k * nq_total
+    else:
+        r = results["plain"]
+        value, ms, steps = r["value"], r["ms_per_step"], r["steps"]
+        dom = (r["kernel"], r["ft_tflops"], r["ft_avg_ms"], r["ft_flop_per_launch"])
+        path = "outer-q plain (separate Fourier / rotate kernels)"
+        pairs_step = nks * args.qtile
+    peak = PEAK[args.precision if "fast" in results else "fp32"]
     out = {
         "metric": METRIC,
         "value": value,
         "unit": "pairs/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": args.precision,
+        "dtype": args.precision if "fast" in results else "fp32",
         "data": "synthetic (seeded model generator, no checkpoint)",
         "config": {"workload": f"{args.config}: nw {m.nw}, nmodes {m.nmodes}, H/D 9^3, g 7^3 x 7^3 c64, "
                                f"k {grids['kgrid']}, q {grids['qgrid']}, window +-{w} eV",
-                   "path": "outer-q plain (separate Fourier / rotate kernels)",
-                   "K_surv": nks, "pairs_per_step": nks * args.qtile, "q_per_step": args.qtile,
-                   "k_batch": args.kbatch, "pre_filter_pairs": nk_total * nq_total},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK[args.precision], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK[args.precision], "traffic": None,
-                     "kernel": "ft_mfma_kernel<float,128> (Rq->kq Fourier over Re)" if args.precision == "fp32"
-                     else "ft_bf16x3_kernel<128> (Rq->kq Fourier over Re)",
-                     "avg_launch_ms": avg_ms, "flop_per_launch_avg": sum(flops) / len(flops)},
+                   "path": path, "K_surv": nks, "pairs_per_step": pairs_step,
+                   "pre_filter_pairs": nk_total * nq_total,
+                   "plain_outer_q_fp32_pairs_per_s": results.get("plain", {}).get("value")},
+        "roofline": {"bound": "mfma", "achieved": dom[1], "peak": peak, "unit": "TFLOP/s",
+                     "frac": dom[1] / peak, "traffic": None, "kernel": dom[0], "avg_launch_ms": dom[2],
+                     "flop_per_launch": dom[3]},
         "cpu_baseline": cpu,
+        "detail": results,
     }
     print(json.dumps(out))
 

@@ -12,9 +12,9 @@
 //      wave (lane i holds d_i, e_i; reads go through readlane), the rotations are applied to the rows
 //      of Z in parallel (lane = row).
 //   4. Eigenvectors V = H_0 ... H_{n-3} D Z, one column per lane.
-// A second kernel refines in FP64 against the FP64 matrix: Rayleigh quotients for every vector and a
-// Rayleigh-Ritz step (orthonormalise + project + Jacobi) for clusters whose eigenvalue gaps are below
-// a tolerance, which fixes near-degenerate eigenvalues and re-orthogonalises their vectors.
+// A second kernel refines in FP64 against the FP64 matrix: a full Rayleigh-Ritz step in the basis of
+// the FP32 eigenvectors (FP64 orthonormalisation, projection, parallel Jacobi), which gives FP64-grade
+// eigenvalues and orthonormal vectors also for exactly and nearly degenerate bands.
 #include "common.hpp"
 
 namespace ephip {
@@ -253,9 +253,19 @@ __global__ __launch_bounds__(256) void heev_c32_kernel(int n, int64_t nb, const 
 }
 
 // ---------------------------------------------------------------------------------------------
-// FP64 refinement. One wave per matrix. LDS: H (n*n c64) | V (n*n c64) | W (n*n c64) | M (n*n c64).
+// FP64 refinement, one wave per matrix, in the basis of the FP32 eigenvectors:
+//   1. Q = V orthonormalised in FP64 by two Loewdin steps Q <- Q (3/2 I - 1/2 Q^H Q)
+//      (V is orthonormal to ~1e-6, two steps reach ~1e-23);
+//   2. M = Q^H H Q is unitarily similar to H and nearly diagonal (off-diagonal ~ FP32 residual);
+//   3. cyclic Jacobi sweeps on M in FP64, the n/2 disjoint rotations of a round-robin round applied
+//      in parallel (columns: lane = row; rows: lane = column), accumulated into Q. Convergence is
+//      quadratic from a nearly diagonal start, and exact or close eigenvalue pairs need no special
+//      handling (a degenerate pair gets a well-defined 45-degree rotation).
+// Eigenvalues are diag(M) sorted ascending, eigenvectors the matching columns of Q (rounded to FP32).
+// LDS per wave: H | Q | W | M (n*n c64 each) + rotation table and ranks.
 // ---------------------------------------------------------------------------------------------
-__host__ __device__ constexpr int refine_lds_doubles(int n) { return 8 * n * n + 2 * n; }
+constexpr int REFINE_EXTRA_DOUBLES = 176;
+__host__ __device__ constexpr int refine_lds_doubles(int n) { return 8 * n * n + REFINE_EXTRA_DOUBLES; }
 
 __device__ __forceinline__ double rdl64(double v, int l) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
@@ -263,68 +273,54 @@ __device__ __forceinline__ double rdl64(double v, int l) {
     return __hiloint2double(hi, lo);
 }
 
-// Cyclic complex Jacobi on the s x s Hermitian M (column-major, ld s) run by one lane; X accumulates
-// the rotations (s x s). Eigenvalues end on the diagonal of M.
-__device__ void jacobi_herm(int s, c64 *M, c64 *X) {
-    for (int i = 0; i < s * s; i++) X[i] = c64{(i % s == i / s) ? 1.0 : 0.0, 0.0};
-    for (int sweep = 0; sweep < 50; sweep++) {
-        double off = 0.0, tot = 0.0;
-        for (int c = 0; c < s; c++)
-            for (int r = 0; r < s; r++) {
-                const double a2 = M[r + s * c].re * M[r + s * c].re + M[r + s * c].im * M[r + s * c].im;
-                tot += a2;
-                if (r != c) off += a2;
-            }
-        if (off <= 1e-30 * tot || off == 0.0) break;
-        for (int p = 0; p < s - 1; p++)
-            for (int q = p + 1; q < s; q++) {
-                const c64 mpq = M[p + s * q];
-                const double amp = sqrt(mpq.re * mpq.re + mpq.im * mpq.im);
-                if (amp == 0.0) continue;
-                const double er = mpq.re / amp, ei = mpq.im / amp;  // e^{i phi}
-                const double tau = (M[q + s * q].re - M[p + s * p].re) / (2.0 * amp);
-                const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                const double cs = 1.0 / sqrt(1.0 + t * t), sn = t * cs;
-                // G = diag(1, e^{-i phi}) [[cs, sn], [-sn, cs]]
-                const c64 gpp{cs, 0.0}, gpq{sn, 0.0};
-                const c64 gqp{-sn * er, sn * ei}, gqq{cs * er, -cs * ei};
-                auto mul = [](c64 a, c64 b) { return c64{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; };
-                auto cmul = [](c64 a, c64 b) { return c64{a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re}; };
-                for (int r = 0; r < s; r++) {  // M <- M G, X <- X G
-                    const c64 mp = M[r + s * p], mq = M[r + s * q];
-                    const c64 a = mul(mp, gpp), bq = mul(mq, gqp), c2 = mul(mp, gpq), d2 = mul(mq, gqq);
-                    M[r + s * p] = c64{a.re + bq.re, a.im + bq.im};
-                    M[r + s * q] = c64{c2.re + d2.re, c2.im + d2.im};
-                    const c64 xp = X[r + s * p], xq = X[r + s * q];
-                    const c64 xa = mul(xp, gpp), xb = mul(xq, gqp), xc = mul(xp, gpq), xd = mul(xq, gqq);
-                    X[r + s * p] = c64{xa.re + xb.re, xa.im + xb.im};
-                    X[r + s * q] = c64{xc.re + xd.re, xc.im + xd.im};
-                }
-                for (int c = 0; c < s; c++) {  // M <- G^H M
-                    const c64 mp = M[p + s * c], mq = M[q + s * c];
-                    const c64 a = cmul(gpp, mp), bq = cmul(gqp, mq), c2 = cmul(gpq, mp), d2 = cmul(gqq, mq);
-                    M[p + s * c] = c64{a.re + bq.re, a.im + bq.im};
-                    M[q + s * c] = c64{c2.re + d2.re, c2.im + d2.im};
-                }
-                M[p + s * q] = c64{0.0, 0.0};
-                M[q + s * p] = c64{0.0, 0.0};
-            }
+__device__ __forceinline__ c64 cmul(c64 a, c64 b) { return c64{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ c64 cmulc(c64 a, c64 b) {  // conj(a) * b
+    return c64{a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+}
+__device__ __forceinline__ c64 cadd(c64 a, c64 b) { return c64{a.re + b.re, a.im + b.im}; }
+
+struct JacobiRot {
+    int p, q;
+    c64 gpp, gpq, gqp, gqq;  // G = diag(1, e^{-i phi}) [[cs, sn], [-sn, cs]] on (p, q)
+};
+
+// out[r + n*j] = alpha * sum_k A[r + n*k] B[k + n*j] + beta * C[r + n*j] ; lanes over entries
+__device__ void lds_gemm_nn(int n, const c64 *A, const c64 *B, c64 *out, double alpha, double beta, const c64 *C,
+                            int lane) {
+    for (int e = lane; e < n * n; e += 64) {
+        const int r = e % n, j = e / n;
+        c64 s{0.0, 0.0};
+        for (int k = 0; k < n; k++) s = cadd(s, cmul(A[r + n * k], B[k + n * j]));
+        c64 v{alpha * s.re, alpha * s.im};
+        if (beta != 0.0) v = c64{v.re + beta * C[e].re, v.im + beta * C[e].im};
+        out[e] = v;
+    }
+}
+
+// out[i + n*j] = sum_r conj(A[r + n*i]) B[r + n*j]
+__device__ void lds_gemm_cn(int n, const c64 *A, const c64 *B, c64 *out, int lane) {
+    for (int e = lane; e < n * n; e += 64) {
+        const int i = e % n, j = e / n;
+        c64 s{0.0, 0.0};
+        for (int r = 0; r < n; r++) s = cadd(s, cmulc(A[r + n * i], B[r + n * j]));
+        out[e] = s;
     }
 }
 
 __global__ __launch_bounds__(256) void heev_refine_kernel(int n, int64_t nb, const c64 *__restrict__ H,
-                                                          c32 *__restrict__ U, const float *__restrict__ E32,
-                                                          double *__restrict__ E64, double tol) {
+                                                          c32 *__restrict__ U, double *__restrict__ E64,
+                                                          double tol) {
     extern __shared__ double lds64[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t b = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
     if (b >= nb) return;
     const int nn = n * n;
     c64 *Hs = reinterpret_cast<c64 *>(lds64 + (int64_t)wave * refine_lds_doubles(n));
-    c64 *V = Hs + nn;
-    c64 *M = V + nn;
-    c64 *X = M + nn;
-    double *lam_s = reinterpret_cast<double *>(X + nn);
+    c64 *Q = Hs + nn;
+    c64 *W = Q + nn;
+    c64 *M = W + nn;
+    JacobiRot *rot = reinterpret_cast<JacobiRot *>(M + nn);  // 16 x 72 bytes
+    int *rank_s = reinterpret_cast<int *>(rot + 16);           // n ints
     for (int e = lane; e < nn; e += 64) {
         const int i = e % n, j = e / n;
         c64 h = (i <= j) ? H[b * nn + i + (int64_t)n * j] : H[b * nn + j + (int64_t)n * i];
@@ -332,110 +328,114 @@ __global__ __launch_bounds__(256) void heev_refine_kernel(int n, int64_t nb, con
         if (i == j) h.im = 0.0;
         Hs[e] = h;
         const c32 u = U[b * nn + e];
-        V[e] = c64{u.re, u.im};
+        Q[e] = c64{u.re, u.im};
     }
     wave_lds_sync();
-    // Rayleigh quotient of column `lane`
-    double lam = 0.0;
+    // 1. two Loewdin steps: S = Q^H Q (in W), M = 3/2 Q - 1/2 Q S, Q <- M
+    for (int it = 0; it < 2; it++) {
+        lds_gemm_cn(n, Q, Q, W, lane);
+        wave_lds_sync();
+        lds_gemm_nn(n, Q, W, M, -0.5, 1.5, Q, lane);
+        wave_lds_sync();
+        for (int e = lane; e < nn; e += 64) Q[e] = M[e];
+        wave_lds_sync();
+    }
+    // 2. M = Q^H (H Q)
+    lds_gemm_nn(n, Hs, Q, W, 1.0, 0.0, nullptr, lane);
+    wave_lds_sync();
+    lds_gemm_cn(n, Q, W, M, lane);
+    wave_lds_sync();
+    // 3. parallel cyclic Jacobi
+    const int N = n + (n & 1);
+    const int npairs = N / 2;
+    for (int sweep = 0; sweep < 12; sweep++) {
+        double off = 0.0, tot = 0.0;
+        for (int e = lane; e < nn; e += 64) {
+            const double a2 = M[e].re * M[e].re + M[e].im * M[e].im;
+            tot += a2;
+            if (e % n != e / n) off += a2;
+        }
+        off = wave_sum(off);
+        tot = wave_sum(tot);
+        if (off <= tol * tol * tot) break;
+        for (int rd = 0; rd < N - 1; rd++) {
+            if (lane < npairs) {
+                int a, c;
+                if (lane == 0) {
+                    a = N - 1;
+                    c = rd;
+                } else {
+                    a = (rd + lane) % (N - 1);
+                    c = (rd - lane + (N - 1)) % (N - 1);
+                }
+                JacobiRot g;
+                g.p = min(a, c);
+                g.q = max(a, c);
+                g.gpp = c64{1.0, 0.0};
+                g.gpq = c64{0.0, 0.0};
+                g.gqp = c64{0.0, 0.0};
+                g.gqq = c64{1.0, 0.0};
+                if (g.q < n) {
+                    const c64 mpq = M[g.p + n * g.q];
+                    const double amp = sqrt(mpq.re * mpq.re + mpq.im * mpq.im);
+                    if (amp > 0.0) {
+                        const double er = mpq.re / amp, ei = mpq.im / amp;
+                        const double tau = (M[g.q + n * g.q].re - M[g.p + n * g.p].re) / (2.0 * amp);
+                        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        const double cs = 1.0 / sqrt(1.0 + t * t), sn = t * cs;
+                        g.gpp = c64{cs, 0.0};
+                        g.gpq = c64{sn, 0.0};
+                        g.gqp = c64{-sn * er, sn * ei};
+                        g.gqq = c64{cs * er, -cs * ei};
+                    }
+                } else {
+                    g.q = g.p;  // dummy partner of an odd n: identity on a single index
+                }
+                rot[lane] = g;
+            }
+            wave_lds_sync();
+            if (lane < n) {  // M <- M G, Q <- Q G (lane = row)
+                for (int k = 0; k < npairs; k++) {
+                    const JacobiRot g = rot[k];
+                    if (g.p == g.q) continue;
+                    const c64 mp = M[lane + n * g.p], mq = M[lane + n * g.q];
+                    M[lane + n * g.p] = cadd(cmul(mp, g.gpp), cmul(mq, g.gqp));
+                    M[lane + n * g.q] = cadd(cmul(mp, g.gpq), cmul(mq, g.gqq));
+                    const c64 qp = Q[lane + n * g.p], qq = Q[lane + n * g.q];
+                    Q[lane + n * g.p] = cadd(cmul(qp, g.gpp), cmul(qq, g.gqp));
+                    Q[lane + n * g.q] = cadd(cmul(qp, g.gpq), cmul(qq, g.gqq));
+                }
+            }
+            wave_lds_sync();
+            if (lane < n) {  // M <- G^H M (lane = column)
+                for (int k = 0; k < npairs; k++) {
+                    const JacobiRot g = rot[k];
+                    if (g.p == g.q) continue;
+                    const c64 mp = M[g.p + n * lane], mq = M[g.q + n * lane];
+                    M[g.p + n * lane] = cadd(cmulc(g.gpp, mp), cmulc(g.gqp, mq));
+                    M[g.q + n * lane] = cadd(cmulc(g.gpq, mp), cmulc(g.gqq, mq));
+                }
+            }
+            wave_lds_sync();
+        }
+    }
+    // 4. ascending order
+    double d = 0.0;
+    if (lane < n) d = M[lane + n * lane].re;
+    int rank = 0;
+    for (int k = 0; k < n; k++) {
+        const double dk = rdl64(d, k);
+        rank += (dk < d || (dk == d && k < lane)) ? 1 : 0;
+    }
     if (lane < n) {
-        double num = 0.0, den = 0.0;
-        for (int r = 0; r < n; r++) {
-            double wr = 0.0, wi = 0.0;
-            for (int k = 0; k < n; k++) {
-                const c64 h = Hs[r + n * k], u = V[k + n * lane];
-                wr += h.re * u.re - h.im * u.im;
-                wi += h.re * u.im + h.im * u.re;
-            }
-            const c64 ur = V[r + n * lane];
-            num += ur.re * wr + ur.im * wi;
-            den += ur.re * ur.re + ur.im * ur.im;
-        }
-        lam = num / den;
+        E64[b * n + rank] = d;
+        rank_s[lane] = rank;
     }
-    // clusters of consecutive eigenvalues closer than tol (in the FP32 ascending order)
-    int start = 0;
-    for (int c = 1; c <= n; c++) {
-        const bool brk = (c == n) || (rdl64(lam, c) - rdl64(lam, c - 1) > tol);
-        if (!brk) continue;
-        const int s = c - start;
-        if (s >= 2) {
-            // modified Gram-Schmidt on V[:, start:c] (lanes over rows)
-            for (int a = start; a < c; a++) {
-                c64 va{0.0, 0.0};
-                if (lane < n) va = V[lane + n * a];
-                for (int a2 = start; a2 < a; a2++) {
-                    c64 vb{0.0, 0.0};
-                    if (lane < n) vb = V[lane + n * a2];
-                    const double pr = wave_sum(vb.re * va.re + vb.im * va.im);
-                    const double pi = wave_sum(vb.re * va.im - vb.im * va.re);
-                    va.re -= pr * vb.re - pi * vb.im;
-                    va.im -= pr * vb.im + pi * vb.re;
-                }
-                const double nrm = sqrt(wave_sum(va.re * va.re + va.im * va.im));
-                if (lane < n) V[lane + n * a] = c64{va.re / nrm, va.im / nrm};
-                wave_lds_sync();
-            }
-            // M = V_c^H H V_c (s x s): lane handles one entry at a time
-            for (int e = lane; e < s * s; e += 64) {
-                const int p = start + e % s, q = start + e / s;
-                double mr = 0.0, mi = 0.0;
-                for (int r = 0; r < n; r++) {
-                    double wr = 0.0, wi = 0.0;
-                    for (int k = 0; k < n; k++) {
-                        const c64 h = Hs[r + n * k], u = V[k + n * q];
-                        wr += h.re * u.re - h.im * u.im;
-                        wi += h.re * u.im + h.im * u.re;
-                    }
-                    const c64 up = V[r + n * p];
-                    mr += up.re * wr + up.im * wi;
-                    mi += up.re * wi - up.im * wr;
-                }
-                M[e] = c64{mr, mi};
-            }
-            wave_lds_sync();
-            if (lane == 0) {
-                jacobi_herm(s, M, X);
-                // ascending order inside the cluster
-                for (int i = 0; i < s; i++) lam_s[i] = M[i + s * i].re;
-                for (int i = 0; i < s - 1; i++) {
-                    int km = i;
-                    for (int k = i + 1; k < s; k++)
-                        if (lam_s[k] < lam_s[km]) km = k;
-                    if (km != i) {
-                        const double t = lam_s[i];
-                        lam_s[i] = lam_s[km];
-                        lam_s[km] = t;
-                        for (int r = 0; r < s; r++) {
-                            const c64 tx = X[r + s * i];
-                            X[r + s * i] = X[r + s * km];
-                            X[r + s * km] = tx;
-                        }
-                    }
-                }
-            }
-            wave_lds_sync();
-            // V_c <- V_c X ; lanes over rows
-            if (lane < n) {
-                c64 row[32];
-                for (int a = 0; a < s; a++) row[a] = V[lane + n * (start + a)];
-                for (int a = 0; a < s; a++) {
-                    double vr = 0.0, vi = 0.0;
-                    for (int a2 = 0; a2 < s; a2++) {
-                        const c64 x = X[a2 + s * a];
-                        vr += row[a2].re * x.re - row[a2].im * x.im;
-                        vi += row[a2].re * x.im + row[a2].im * x.re;
-                    }
-                    V[lane + n * (start + a)] = c64{vr, vi};
-                }
-            }
-            if (lane >= start && lane < c) lam = lam_s[lane - start];
-            wave_lds_sync();
-        }
-        start = c;
+    wave_lds_sync();
+    for (int e = lane; e < nn; e += 64) {
+        const int r = e % n, j = e / n;
+        U[b * nn + r + (int64_t)n * rank_s[j]] = c32{static_cast<float>(Q[e].re), static_cast<float>(Q[e].im)};
     }
-    if (lane < n) E64[b * n + lane] = lam;
-    for (int e = lane; e < nn; e += 64) U[b * nn + e] = c32{static_cast<float>(V[e].re), static_cast<float>(V[e].im)};
-    (void)E32;
 }
 
 }  // namespace ephip
@@ -474,17 +474,18 @@ extern "C" int ephip_heev_c32(int32_t n, int64_t nb, const void *H, int32_t h_is
 }
 
 extern "C" int ephip_heev_refine(int32_t n, int64_t nb, const void *H64, void *U, const float *E32, double *E64,
-                                 double cluster_tol, void *stream) {
+                                 double tol, void *stream) {
     int st = heev_launch_check(n, nb);
     if (st) return st;
     if (nb == 0) return EPHIP_OK;
     EPHIP_REQUIRE(n <= 31, "heev_refine: n=%d > 31 (the FP64 working set of one matrix exceeds 64 KiB of LDS)", n);
     EPHIP_REQUIRE(H64 && U && E64, "heev_refine: null device pointer");
-    EPHIP_REQUIRE(cluster_tol >= 0.0, "heev_refine: negative cluster tolerance");
+    EPHIP_REQUIRE(tol >= 0.0, "heev_refine: negative tolerance");
+    (void)E32;
     const int wpb = waves_per_block(sizeof(double) * refine_lds_doubles(n));
     const size_t lds = wpb * sizeof(double) * refine_lds_doubles(n);
     heev_refine_kernel<<<(unsigned)((nb + wpb - 1) / wpb), 64 * wpb, lds, as_stream(stream)>>>(
-        n, nb, static_cast<const c64 *>(H64), static_cast<c32 *>(U), E32, E64, cluster_tol);
+        n, nb, static_cast<const c64 *>(H64), static_cast<c32 *>(U), E64, tol > 0.0 ? tol : 1e-15);
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

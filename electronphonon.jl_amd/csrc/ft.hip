@@ -40,11 +40,22 @@ constexpr int FT_KC = 16;  // complex R per LDS stage
 constexpr int FT_APAD = 8;
 constexpr int FT_BPAD = 16;
 
+// Optional batch of independent contractions, one per grid z-slice u ("unit"): unit u reads its
+// operand at op + u*op_stride, writes at out + u*out_stride, has ncol[u] columns (or nk), and column
+// j uses point xk[col_index[u*col_stride + j]] (or xk[j]). A zero-initialised FtBatch is one plain call.
+struct FtBatch {
+    int64_t op_stride = 0;
+    int64_t out_stride = 0;
+    const int *col_index = nullptr;
+    int64_t col_stride = 0;
+    const int *ncol = nullptr;
+};
+
 template <class T, int TN>
 __global__ __launch_bounds__(256) void ft_mfma_kernel(int64_t ndata, int nr, const cplx<T> *__restrict__ op,
                                                       int64_t ldop, const int *__restrict__ irvec, int nk,
                                                       const double *__restrict__ xk, cplx<T> *__restrict__ out,
-                                                      int64_t ldout) {
+                                                      int64_t ldout, FtBatch bt) {
     using acc_t = typename std::conditional<sizeof(T) == 4, f32x4, f64x4>::type;
     constexpr int NCB = TN / 32;  // 16-column blocks per wave
     __shared__ cplx<T> As[FT_KC][FT_TM + FT_APAD];
@@ -54,6 +65,14 @@ __global__ __launch_bounds__(256) void ft_mfma_kernel(int64_t ndata, int nr, con
     const int wm = wave & 1, wn = wave >> 1;
     const int64_t d0 = (int64_t)blockIdx.y * FT_TM;
     const int64_t j0 = (int64_t)blockIdx.x * TN;
+    {
+        const int u = blockIdx.z;
+        if (bt.ncol) nk = bt.ncol[u];
+        if (j0 >= nk) return;  // uniform for the block: nothing of this unit in this column tile
+        op += u * bt.op_stride;
+        out += u * bt.out_stride;
+        if (bt.col_index) bt.col_index += u * bt.col_stride;
+    }
 
     // The phase column this thread generates is fixed for the whole kernel.
     const int pj = tid % TN;
@@ -62,9 +81,10 @@ __global__ __launch_bounds__(256) void ft_mfma_kernel(int64_t ndata, int nr, con
     double x0 = 0, x1 = 0, x2 = 0;
     const bool pvalid = (j0 + pj) < nk;
     if (pvalid) {
-        x0 = xk[3 * (j0 + pj)];
-        x1 = xk[3 * (j0 + pj) + 1];
-        x2 = xk[3 * (j0 + pj) + 2];
+        const int64_t pt = bt.col_index ? bt.col_index[j0 + pj] : j0 + pj;
+        x0 = xk[3 * pt];
+        x1 = xk[3 * pt + 1];
+        x2 = xk[3 * pt + 2];
     }
 
     acc_t acc[4][NCB];
@@ -152,7 +172,7 @@ template <int TN>
 __global__ __launch_bounds__(256) void ft_bf16x3_kernel(int64_t ndata, int nr, const cplx<float> *__restrict__ op,
                                                         int64_t ldop, const int *__restrict__ irvec, int nk,
                                                         const double *__restrict__ xk, cplx<float> *__restrict__ out,
-                                                        int64_t ldout) {
+                                                        int64_t ldout, FtBatch bt) {
     constexpr int NCB = TN / 32;
     constexpr int KR = 2 * FT_KC;  // real K per stage = 32
     // Pre-split operands, stored row-major along K so each lane reads its 8 K values as 16 bytes.
@@ -165,6 +185,14 @@ __global__ __launch_bounds__(256) void ft_bf16x3_kernel(int64_t ndata, int nr, c
     const int wm = wave & 1, wn = wave >> 1;
     const int64_t d0 = (int64_t)blockIdx.y * FT_TM;
     const int64_t j0 = (int64_t)blockIdx.x * TN;
+    {
+        const int u = blockIdx.z;
+        if (bt.ncol) nk = bt.ncol[u];
+        if (j0 >= nk) return;  // uniform for the block: nothing of this unit in this column tile
+        op += u * bt.op_stride;
+        out += u * bt.out_stride;
+        if (bt.col_index) bt.col_index += u * bt.col_stride;
+    }
 
     const int pj = tid % TN;
     const int prl0 = tid / TN;
@@ -172,9 +200,10 @@ __global__ __launch_bounds__(256) void ft_bf16x3_kernel(int64_t ndata, int nr, c
     double x0 = 0, x1 = 0, x2 = 0;
     const bool pvalid = (j0 + pj) < nk;
     if (pvalid) {
-        x0 = xk[3 * (j0 + pj)];
-        x1 = xk[3 * (j0 + pj) + 1];
-        x2 = xk[3 * (j0 + pj) + 2];
+        const int64_t pt = bt.col_index ? bt.col_index[j0 + pj] : j0 + pj;
+        x0 = xk[3 * pt];
+        x1 = xk[3 * pt + 1];
+        x2 = xk[3 * pt + 2];
     }
 
     f32x4 acc[4][NCB];
@@ -302,18 +331,18 @@ extern "C" int ephip_fourier_c32(int64_t ndata, int32_t nr, const void *op, int6
     if (precision == EPHIP_PREC_FP32) {
         if (nk <= 64) {
             dim3 g((nk + 63) / 64, gy);
-            ft_mfma_kernel<float, 64><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk, xk, c, ldout);
+            ft_mfma_kernel<float, 64><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk, xk, c, ldout, FtBatch{});
         } else {
             dim3 g((nk + 127) / 128, gy);
-            ft_mfma_kernel<float, 128><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk, xk, c, ldout);
+            ft_mfma_kernel<float, 128><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk, xk, c, ldout, FtBatch{});
         }
     } else {
         if (nk <= 64) {
             dim3 g((nk + 63) / 64, gy);
-            ft_bf16x3_kernel<64><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk, xk, c, ldout);
+            ft_bf16x3_kernel<64><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk, xk, c, ldout, FtBatch{});
         } else {
             dim3 g((nk + 127) / 128, gy);
-            ft_bf16x3_kernel<128><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk, xk, c, ldout);
+            ft_bf16x3_kernel<128><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk, xk, c, ldout, FtBatch{});
         }
     }
     EPHIP_LAUNCHED();
@@ -329,7 +358,7 @@ extern "C" int ephip_fourier_c64(int64_t ndata, int32_t nr, const void *op, int6
     if (nr == 0) return zero_out<double>(ndata, nk, out, ldout, s);
     dim3 g((nk + 63) / 64, (unsigned)((ndata + FT_TM - 1) / FT_TM));
     ft_mfma_kernel<double, 64><<<g, 256, 0, s>>>(ndata, nr, static_cast<const cplx<double> *>(op), ldop, irvec,
-                                                 nk, xk, static_cast<cplx<double> *>(out), ldout);
+                                                 nk, xk, static_cast<cplx<double> *>(out), ldout, FtBatch{});
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }
@@ -342,6 +371,46 @@ extern "C" int ephip_fourier_phase_c64(int32_t nr, const int32_t *irvec, int32_t
     EPHIP_REQUIRE(irvec && xk && phase, "fourier_phase: null device pointer");
     phase_kernel<double><<<(unsigned)((n + 255) / 256), 256, 0, as_stream(stream)>>>(
         nr, irvec, nk, xk, static_cast<cplx<double> *>(phase));
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_fourier_batched_c32(int64_t ndata, int32_t nr, const void *op, int64_t ldop, int64_t op_stride,
+                                         const int32_t *irvec, const double *xk, int32_t nunits, int32_t nk_max,
+                                         const int32_t *ncol, const int32_t *col_index, int64_t col_stride, void *out,
+                                         int64_t ldout, int64_t out_stride, int32_t precision, void *stream) {
+    int st = check_ft_args(ndata, nr, op, ldop, irvec, nk_max, xk, out, ldout);
+    if (st) return st;
+    EPHIP_REQUIRE(precision == EPHIP_PREC_FP32 || precision == EPHIP_PREC_BF16X3,
+                  "fourier_batched: unknown precision %d", precision);
+    EPHIP_REQUIRE(nunits >= 0 && nunits <= 65535, "fourier_batched: nunits=%d outside 0..65535", nunits);
+    EPHIP_REQUIRE(op_stride >= 0 && out_stride >= 0 && col_stride >= 0, "fourier_batched: negative stride");
+    EPHIP_REQUIRE(!col_index || col_stride >= nk_max, "fourier_batched: col_stride < nk_max");
+    if (ndata == 0 || nk_max == 0 || nunits == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(nr > 0, "fourier_batched: nr must be positive");
+    FtBatch bt;
+    bt.op_stride = op_stride;
+    bt.out_stride = out_stride;
+    bt.col_index = col_index;
+    bt.col_stride = col_stride;
+    bt.ncol = ncol;
+    hipStream_t s = as_stream(stream);
+    const unsigned gy = (unsigned)((ndata + FT_TM - 1) / FT_TM);
+    auto o = static_cast<const cplx<float> *>(op);
+    auto c = static_cast<cplx<float> *>(out);
+    const bool small = nk_max <= 64;
+    dim3 g(small ? (nk_max + 63) / 64 : (nk_max + 127) / 128, gy, nunits);
+    if (precision == EPHIP_PREC_FP32) {
+        if (small)
+            ft_mfma_kernel<float, 64><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk_max, xk, c, ldout, bt);
+        else
+            ft_mfma_kernel<float, 128><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk_max, xk, c, ldout, bt);
+    } else {
+        if (small)
+            ft_bf16x3_kernel<64><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk_max, xk, c, ldout, bt);
+        else
+            ft_bf16x3_kernel<128><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk_max, xk, c, ldout, bt);
+    }
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

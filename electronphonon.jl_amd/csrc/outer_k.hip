@@ -1,0 +1,221 @@
+// Outer-k e-ph path with the k-side window projection: the reference's list-batched RR->kR and
+// kR->kq drivers (src/wannier_to_bloch_batched.jl:178-273) and rotations (:442-455, CUDA
+// ext/ElectronPhononCUDAExt.jl:146-201), with the per-k window projection of the outer-k loop
+// (src/calculator/run_eph_over_k_and_kq.jl:657-659). A "unit" is one (k, in-window band n) pair;
+// each unit owns a projected operator g(k)[i, n, nu, Rp] and a compacted list of q whose k+q has an
+// in-window band (the other pairs are exactly zero, as the reference's k+q window leaves them).
+#include "common.hpp"
+
+namespace ephip {
+
+struct c32 {
+    float re, im;
+};
+
+// "ph" layout (b, Re) x Rp  ->  "el" layout (b, Rp) x Re, b in [0, B) with B = nw*nw*nm
+// (src/model_new.jl:551-565: the two outer-momentum layouts of the same tensor).
+__global__ void ph_to_el_kernel(int64_t B, int nre, int nrp, const c32 *__restrict__ x, c32 *__restrict__ y) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= B * nre * nrp) return;
+    const int64_t b = e % B, rest = e / B;
+    const int64_t rp = rest % nrp, re = rest / nrp;
+    y[e] = x[b + B * (re + (int64_t)nre * rp)];
+}
+
+// get_eph_RR_to_kR! projection (src/wannier_to_bloch.jl:381-388 per k; batched :193-197):
+// A_u[(i + nw*nu) + nw*nm*Rp] = sum_j G_col(u)[i + nw*(j + nw*(nu + nm*Rp))] * U_state(u)[j, band(u)].
+__global__ void project_kR_kernel(int nw, int nm, int nrp, int nunits, const c32 *__restrict__ G, int64_t ldg,
+                                  const int *__restrict__ ucol, const int64_t *__restrict__ ustate,
+                                  const int *__restrict__ uband, const c32 *__restrict__ U, c32 *__restrict__ A) {
+    const int64_t per = (int64_t)nw * nm * nrp;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per * nunits) return;
+    const int u = (int)(e / per);
+    const int64_t r = e % per;
+    const int i = (int)(r % nw);
+    const int64_t nurp = r / nw;  // nu + nm * Rp
+    const c32 *g = G + ldg * ucol[u] + i + (int64_t)nw * nw * nurp;
+    const c32 *uk = U + (int64_t)nw * nw * ustate[u] + (int64_t)nw * uband[u];
+    float sr = 0.f, si = 0.f;
+    for (int j = 0; j < nw; j++) {
+        const c32 a = g[(int64_t)nw * j], w = uk[j];
+        sr += a.re * w.re - a.im * w.im;
+        si += a.re * w.im + a.im * w.re;
+    }
+    A[e] = c32{sr, si};
+}
+
+// Per unit: the q of the list whose k+q (on the k grid) has an in-window band, in q order.
+// col_index[u*cs + c] = q, ikq[u*cs + c] = state index of k+q, count[u] = number kept.
+// One workgroup of 256 threads per unit; positions from wave ballots + a 4-wave prefix.
+__global__ __launch_bounds__(256) void kq_compact_kernel(int nunits, const int64_t *__restrict__ ustate,
+                                                        const double *__restrict__ xk, int nq,
+                                                        const double *__restrict__ xq, int n1, int n2, int n3,
+                                                        const int64_t *__restrict__ bmin, int64_t cs,
+                                                        int *__restrict__ col_index, int *__restrict__ ikq,
+                                                        int *__restrict__ count) {
+    __shared__ int wave_cnt[4];
+    const int u = blockIdx.x;
+    if (u >= nunits) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t s = ustate[u];
+    const double k0 = xk[3 * s], k1 = xk[3 * s + 1], k2 = xk[3 * s + 2];
+    auto red = [](double x, int n) {
+        int64_t c = (int64_t)rint(x * n) % n;
+        return c < 0 ? c + n : c;
+    };
+    int base = 0;
+    for (int c0 = 0; c0 < nq; c0 += 256) {
+        const int c = c0 + tid;
+        bool keep = false;
+        int h = 0;
+        if (c < nq) {
+            h = (int)((red(k0 + xq[3 * c], n1) * n2 + red(k1 + xq[3 * c + 1], n2)) * n3 + red(k2 + xq[3 * c + 2], n3));
+            keep = bmin[h] > 0;
+        }
+        const unsigned long long ball = __ballot(keep);
+        const int before = __popcll(ball & ((1ull << lane) - 1ull));
+        if (lane == 0) wave_cnt[wave] = __popcll(ball);
+        __syncthreads();
+        int off = base;
+        for (int w = 0; w < wave; w++) off += wave_cnt[w];
+        if (keep) {
+            col_index[u * cs + off + before] = c;
+            ikq[u * cs + off + before] = h;
+        }
+        base += wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
+        __syncthreads();
+    }
+    if (tid == 0) count[u] = base;
+}
+
+// kR->kq rotations + g2 fold for one (unit, kept q) per wave (eph_apply_rotations!,
+// src/wannier_to_bloch_batched.jl:442-455; src/wannier_to_bloch.jl:428-440):
+//   t[m, nu]   = sum_i conj(U_kq[i, mlo + m]) Gq[i + nw*nu]
+//   ep[m, nu'] = sum_nu t[m, nu] u_ph(q)[nu, nu'],   g2 = |ep|^2 / (2 omega(q)[nu'])
+// Output row (u*cs + c) holds nbw*nm values [m*nm + nu'], zero for m >= window width at k+q.
+__global__ __launch_bounds__(256) void rotate_kq_g2_kernel(int nw, int nm, int nbw, int nunits, int64_t cs,
+                                                          const int *__restrict__ count,
+                                                          const int *__restrict__ col_index,
+                                                          const int *__restrict__ ikq, const c32 *__restrict__ Gq,
+                                                          int64_t ldg, int64_t gstride, const c32 *__restrict__ U,
+                                                          const int64_t *__restrict__ bmin,
+                                                          const int64_t *__restrict__ bmax,
+                                                          const c32 *__restrict__ uph,
+                                                          const double *__restrict__ omega, float *__restrict__ g2,
+                                                          c32 *__restrict__ ep_out) {
+    extern __shared__ c32 tsh[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int u = blockIdx.y;
+    const int64_t c = (int64_t)blockIdx.x * 4 + wave;
+    if (u >= nunits || c >= count[u]) return;
+    c32 *T = tsh + wave * nbw * nm;
+    const int64_t row = (int64_t)u * cs + c;
+    const int iq = col_index[row];
+    const int64_t kq = ikq[row];
+    const int mlo = (int)bmin[kq] - 1;
+    int nbkq = bmin[kq] > 0 ? (int)(bmax[kq] - bmin[kq] + 1) : 0;  // no window band: all slots zero
+    nbkq = nbkq < nbw ? nbkq : nbw;
+    const c32 *g = Gq + gstride * u + ldg * c;
+    const c32 *uk = U + (int64_t)nw * nw * kq;
+    for (int p = lane; p < nbw * nm; p += 64) {
+        const int m = p % nbw, nu = p / nbw;
+        float sr = 0.f, si = 0.f;
+        if (m < nbkq) {
+            for (int i = 0; i < nw; i++) {
+                const c32 w = uk[i + (int64_t)nw * (mlo + m)], a = g[i + (int64_t)nw * nu];
+                sr += w.re * a.re + w.im * a.im;
+                si += w.re * a.im - w.im * a.re;
+            }
+        }
+        T[m + nbw * nu] = c32{sr, si};
+    }
+    wave_lds_sync();
+    const c32 *up = uph + (int64_t)nm * nm * iq;
+    const double *om = omega + (int64_t)nm * iq;
+    for (int p = lane; p < nbw * nm; p += 64) {
+        const int nu2 = p % nm, m = p / nm;
+        float er = 0.f, ei = 0.f;
+        if (m < nbkq) {
+            for (int nu = 0; nu < nm; nu++) {
+                const c32 t = T[m + nbw * nu], w = up[nu + nm * nu2];
+                er += t.re * w.re - t.im * w.im;
+                ei += t.re * w.im + t.im * w.re;
+            }
+        }
+        const int64_t o = row * nbw * nm + p;
+        if (g2) g2[o] = (er * er + ei * ei) / (2.f * static_cast<float>(om[nu2]));
+        if (ep_out) ep_out[o] = c32{er, ei};
+    }
+}
+
+}  // namespace ephip
+
+using namespace ephip;
+
+static inline unsigned nblocks(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+extern "C" int ephip_eph_ph_to_el(int64_t nbasis, int32_t nre, int32_t nrp, const void *ph, void *el, void *stream) {
+    EPHIP_REQUIRE(nbasis >= 1 && nre >= 1 && nrp >= 1, "eph_ph_to_el: bad sizes");
+    EPHIP_REQUIRE(ph && el && ph != el, "eph_ph_to_el: null or aliased device pointer");
+    const int64_t n = nbasis * nre * nrp;
+    EPHIP_REQUIRE(nblocks(n) < 0x7fffffffu, "eph_ph_to_el: tensor too large");
+    ph_to_el_kernel<<<nblocks(n), 256, 0, as_stream(stream)>>>(nbasis, nre, nrp, static_cast<const c32 *>(ph),
+                                                              static_cast<c32 *>(el));
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_eph_project_kR(int32_t nw, int32_t nm, int32_t nrp, int32_t nunits, const void *G, int64_t ldg,
+                                    const int32_t *ucol, const int64_t *ustate, const int32_t *uband, const void *U,
+                                    void *A, void *stream) {
+    EPHIP_REQUIRE(nw >= 1 && nm >= 1 && nrp >= 1 && nunits >= 0, "eph_project_kR: bad sizes");
+    EPHIP_REQUIRE(ldg >= (int64_t)nw * nw * nm * nrp, "eph_project_kR: ldg too small");
+    if (nunits == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(G && ucol && ustate && uband && U && A, "eph_project_kR: null device pointer");
+    const int64_t n = (int64_t)nw * nm * nrp * nunits;
+    project_kR_kernel<<<nblocks(n), 256, 0, as_stream(stream)>>>(nw, nm, nrp, nunits, static_cast<const c32 *>(G), ldg,
+                                                                ucol, ustate, uband, static_cast<const c32 *>(U),
+                                                                static_cast<c32 *>(A));
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_kq_compact(int32_t nunits, const int64_t *ustate, const double *xk, int32_t nq, const double *xq,
+                                const int32_t *ngrid, const int64_t *bmin, int64_t cs, int32_t *col_index, int32_t *ikq,
+                                int32_t *count, void *stream) {
+    EPHIP_REQUIRE(nunits >= 0 && nunits <= 0x7fffffff && nq >= 0, "kq_compact: bad sizes");
+    EPHIP_REQUIRE(ngrid && ngrid[0] > 0 && ngrid[1] > 0 && ngrid[2] > 0, "kq_compact: bad grid");
+    EPHIP_REQUIRE((int64_t)ngrid[0] * ngrid[1] * ngrid[2] < 0x7fffffffLL, "kq_compact: grid too large for int32 indices");
+    EPHIP_REQUIRE(cs >= nq, "kq_compact: column stride %lld < nq %d", (long long)cs, nq);
+    if (nunits == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(ustate && xk && xq && bmin && col_index && ikq && count, "kq_compact: null device pointer");
+    kq_compact_kernel<<<nunits, 256, 0, as_stream(stream)>>>(nunits, ustate, xk, nq, xq, ngrid[0], ngrid[1], ngrid[2],
+                                                           bmin, cs, col_index, ikq, count);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_eph_rotate_kq_g2(int32_t nw, int32_t nm, int32_t nbw, int32_t nunits, int64_t cs,
+                                      const int32_t *count, const int32_t *col_index, const int32_t *ikq,
+                                      const void *Gq, int64_t ldg, int64_t gstride, const void *U,
+                                      const int64_t *bmin, const int64_t *bmax, const void *uph, const double *omega,
+                                      float *g2, void *ep, void *stream) {
+    EPHIP_REQUIRE(nw >= 1 && nm >= 1 && nunits >= 0 && cs >= 0, "eph_rotate_kq_g2: bad sizes");
+    EPHIP_REQUIRE(nbw >= 1 && nbw <= nw, "eph_rotate_kq_g2: nbw=%d must be in 1..nw=%d", nbw, nw);
+    EPHIP_REQUIRE(ldg >= (int64_t)nw * nm && gstride >= ldg * cs, "eph_rotate_kq_g2: bad ldg/gstride");
+    const size_t lds = 4 * sizeof(c32) * (size_t)nbw * nm;
+    EPHIP_REQUIRE(lds <= 64 * 1024, "eph_rotate_kq_g2: nbw*nm too large");
+    EPHIP_REQUIRE(nunits <= 65535, "eph_rotate_kq_g2: nunits=%d > 65535", nunits);
+    if (nunits == 0 || cs == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(count && col_index && ikq && Gq && U && bmin && bmax && uph && omega && (g2 || ep),
+                  "eph_rotate_kq_g2: null device pointer");
+    dim3 grid((unsigned)((cs + 3) / 4), (unsigned)nunits);
+    rotate_kq_g2_kernel<<<grid, 256, lds, as_stream(stream)>>>(nw, nm, nbw, nunits, cs, count, col_index, ikq,
+                                                              static_cast<const c32 *>(Gq), ldg, gstride,
+                                                              static_cast<const c32 *>(U), bmin, bmax,
+                                                              static_cast<const c32 *>(uph), omega, g2,
+                                                              static_cast<c32 *>(ep));
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}

@@ -33,6 +33,13 @@ SIGNATURES = {
     "ephip_eph_rotate_modes": [_i64, _i32, _i32, _i32, _vp, _i64, _vp, _vp, _i64, _vp],
     "ephip_eph_rotate_rqkq_g2": [_i32, _i32, _i64, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp,
                                  _vp, _vp],
+    "ephip_fourier_batched_c32": [_i64, _i32, _vp, _i64, _i64, _vp, _vp, _i32, _i32, _vp, _vp, _i64, _vp,
+                                  _i64, _i64, _i32, _vp],
+    "ephip_eph_ph_to_el": [_i64, _i32, _i32, _vp, _vp, _vp],
+    "ephip_eph_project_kR": [_i32, _i32, _i32, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
+    "ephip_kq_compact": [_i32, _vp, _vp, _i32, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
+    "ephip_eph_rotate_kq_g2": [_i32, _i32, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp,
+                               _vp, _vp, _vp, _vp, _vp],
 }
 
 

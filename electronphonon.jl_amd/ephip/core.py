@@ -20,8 +20,6 @@ from . import _lib as L
 ELECTRON_DEGEN_CUTOFF_EV = 0.01e-3
 # Phonon modes below this are skipped by consumers (src/common/constants.jl:5), in eV.
 OMEGA_ACOUSTIC_EV = 0.01e-3
-# Eigenvalue gap below which the FP64 refinement treats eigenpairs as one cluster, in eV.
-CLUSTER_TOL_EV = 1e-4
 
 
 def _torch():
@@ -216,8 +214,9 @@ def _heev(Hk, vectors: bool):
     return E, U
 
 
-def eigen_batched_refined(H64, cluster_tol: float = CLUSTER_TOL_EV, vectors: bool = True):
-    """FP32 eigensolve of the FP64 stack ``H64`` followed by the FP64 refinement. Returns
+def eigen_batched_refined(H64, tol: float = 0.0):
+    """FP32 eigensolve of the FP64 stack ``H64`` followed by the FP64 Rayleigh-Ritz / Jacobi
+    refinement (``tol`` = 0: off-diagonal norm below 1e-15 of the norm). Returns
     (E64 (nk, n) float64, U (nk, n, n) complex64)."""
     torch = _torch()
     H64 = _as_stack(H64).contiguous()
@@ -226,7 +225,7 @@ def eigen_batched_refined(H64, cluster_tol: float = CLUSTER_TOL_EV, vectors: boo
     E32, U = _heev(H64, vectors=True)
     nb, n = E32.shape
     E64 = torch.empty((nb, n), dtype=torch.float64, device=H64.device)
-    L.call("ephip_heev_refine", n, nb, L.ptr(H64), L.ptr(U), L.ptr(E32), L.ptr(E64), float(cluster_tol),
+    L.call("ephip_heev_refine", n, nb, L.ptr(H64), L.ptr(U), L.ptr(E32), L.ptr(E64), float(tol),
            L.stream_handle())
     return E64, U
 

@@ -80,7 +80,7 @@ def compute_phonon_states(dyn_dev: WannierObject, mass, xq_dev) -> PhononStates:
     mass_d = torch.as_tensor(np.asarray(mass, dtype=np.float64), device=dev)
     D = core.fourier_batched(dyn_dev, xq_dev)
     L.call("ephip_ph_mass_scale", nm, nq, L.ptr(D), L.ptr(mass_d), L.stream_handle())
-    w2, U = core.eigen_batched_refined(D, cluster_tol=1e-12)
+    w2, U = core.eigen_batched_refined(D)
     omega = torch.empty_like(w2)
     L.call("ephip_ph_finalize", nm, nq, L.ptr(w2), L.ptr(omega), L.ptr(U), L.ptr(mass_d), L.stream_handle())
     return PhononStates(xq_dev, omega, U)
@@ -305,3 +305,153 @@ class _RqObject(WannierObject):
         self.irvec_next = None
         self._irvec_dev = irvec_dev
         self.irvec = None
+
+
+# ------------------------------------------------------------------------------------------------
+# Outer-k loop with the k-side window projection
+# ------------------------------------------------------------------------------------------------
+@dataclasses.dataclass
+class EPDataUnits:
+    """Device payload of one k batch of the outer-k loop: the windowed counterpart of
+    EPDataQBatched (src/EPData.jl:66-73) for a whole k batch. Unit u is the pair (k state
+    ``unit_state[u]``, in-window band ``unit_band[u]`` (0-based)); its kept q are
+    ``col_index[u, :count[u]]`` (q-grid indices) with k+q states ``ikq[u, :count[u]]``, and
+    ``g2[u, c]`` holds (nbw, nm) values [m, nu] for k+q window band bmin(k+q) - 1 + m. q not listed
+    for a unit have no in-window band at k+q: their g2 is exactly zero."""
+    k_range: tuple
+    unit_state: object
+    unit_band: object
+    count: object
+    col_index: object
+    ikq: object
+    g2: object
+    nbw: int
+
+
+class OuterKLoop:
+    """Outer-k e-ph loop on the device (the reference's _loop_eph_over_k_and_kq_batched,
+    src/calculator/run_eph_over_k_and_kq.jl:584-895, with the k-side window projection of :657-659,
+    and the q-grid as the inner list). Per k batch:
+      1. RR->kR: Fourier over Re of the "el"-layout tensor at the batch's k (matrix cores), then the
+         projection onto each k's in-window bands -> one operator g(k)[i, n, nu, Rp] per unit (k, n);
+      2. per unit, the q whose k+q has an in-window band (k+q looked up on the fine grid);
+      3. kR->kq: one batched Fourier over Rp for all units at their kept q (matrix cores);
+      4. k+q-window and phonon rotations, g2 = |ep|^2 / (2 omega).
+    Pairs (k, q) with no in-window band at k+q are exactly zero and are not computed."""
+
+    def __init__(self, st: OuterQSetup, nk_batch: int = 128, precision: int = L.PREC_FP32):
+        torch = _torch()
+        self.st = st
+        self.nk_batch = int(nk_batch)
+        self.precision = precision
+        dev = st.el.xk.device
+        nw, nm = st.nw, st.nm
+        self.nre = st.irvec_el_dev.shape[0]
+        self.nrp = st.epmat.nr
+        self.nd_el = nw * nw * nm * self.nrp
+        # "el" layout copy of the e-ph tensor: rows (i, j, nu, Rp), one column per Re
+        self.epmat_el = torch.empty((self.nre, self.nd_el), dtype=torch.complex64, device=dev)
+        L.call("ephip_eph_ph_to_el", nw * nw * nm, self.nre, self.nrp, L.ptr(st.epmat.op_r), L.ptr(self.epmat_el),
+               L.stream_handle())
+        self.el_obj = _RqObject(self.epmat_el, st.irvec_el_dev, self.nd_el)
+        self.irvec_ep_dev = st.epmat.irvec_dev()
+        # units of every surviving k, in k order
+        bmin = st.el.filt.band_min.cpu().numpy()
+        bmax = st.el.filt.band_max.cpu().numpy()
+        ks = st.ksurv.cpu().numpy()
+        nb = (bmax[ks] - bmin[ks] + 1).astype(np.int64)
+        self.k_unit_start = np.concatenate([[0], np.cumsum(nb)])
+        self.unit_kpos = torch.from_numpy(np.repeat(np.arange(len(ks)), nb).astype(np.int32)).to(dev)
+        self.unit_state = torch.from_numpy(np.repeat(ks, nb).astype(np.int64)).to(dev)
+        band = np.concatenate([np.arange(bmin[k] - 1, bmax[k]) for k in ks]) if len(ks) else np.zeros(0)
+        self.unit_band = torch.from_numpy(band.astype(np.int32)).to(dev)
+        self.xk_surv = st.el.xk.index_select(0, st.ksurv).contiguous()
+        self.nq = st.ph.xq.shape[0]
+        self.cs = self.nq
+        max_units = int(max(nb[i:i + self.nk_batch].sum() for i in range(0, max(len(ks), 1), self.nk_batch))) \
+            if len(ks) else 1
+        self.max_units = max_units
+        nd2 = nw * nm
+        self.G = torch.empty((self.nk_batch, self.nd_el), dtype=torch.complex64, device=dev)
+        self.A = torch.empty((max_units, nd2 * self.nrp), dtype=torch.complex64, device=dev)
+        self.col_index = torch.empty((max_units, self.cs), dtype=torch.int32, device=dev)
+        self.ikq = torch.empty((max_units, self.cs), dtype=torch.int32, device=dev)
+        self.count = torch.empty(max_units, dtype=torch.int32, device=dev)
+        self.Gq = torch.empty((max_units, self.cs, nd2), dtype=torch.complex64, device=dev)
+        self.g2 = torch.empty((max_units, self.cs, st.nbw * nm), dtype=torch.float32, device=dev)
+        self.uph = st.ph.u.contiguous()
+        self.omega = st.ph.omega.contiguous()
+        self.ft_timer = None   # optional list of (event, event, label, flops) around the two Fourier launches
+
+    def _timed(self, label, flops, fn):
+        if self.ft_timer is None:
+            return fn()
+        torch = _torch()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        r = fn()
+        b.record()
+        self.ft_timer.append((a, b, label, flops))
+        return r
+
+    def run_kbatch(self, i0: int, n: int, calculators=()):
+        import ctypes as C
+        st = self.st
+        nw, nm = st.nw, st.nm
+        n = min(n, st.ksurv.shape[0] - i0)
+        if n <= 0:
+            return None
+        u0, u1 = int(self.k_unit_start[i0]), int(self.k_unit_start[i0 + n])
+        nu = u1 - u0
+        stream = L.stream_handle()
+        # 1. RR->kR Fourier over Re for the batch
+        xk_b = self.xk_surv[i0:i0 + n]
+        self._timed("RR->kR", 8.0 * self.nd_el * self.nre * n,
+                    lambda: core.fourier_batched(self.el_obj, xk_b, out=self.G[:n], precision=self.precision))
+        # projection; the unit's column in G is its k position minus i0 (base pointer shifted by -i0 columns)
+        g_base = L.ptr(self.G) - i0 * self.G.stride(0) * 8
+        L.call("ephip_eph_project_kR", nw, nm, self.nrp, nu, g_base, self.G.stride(0), L.ptr(self.unit_kpos[u0:u1]),
+               L.ptr(self.unit_state[u0:u1]), L.ptr(self.unit_band[u0:u1]), L.ptr(st.el.U), L.ptr(self.A), stream)
+        # 2. kept q per unit
+        g = (C.c_int32 * 3)(*map(int, st.kgrid))
+        L.call("ephip_kq_compact", nu, L.ptr(self.unit_state[u0:u1]), L.ptr(st.el.xk), self.nq, L.ptr(st.ph.xq),
+               C.addressof(g), L.ptr(st.el.filt.band_min), self.cs, L.ptr(self.col_index), L.ptr(self.ikq),
+               L.ptr(self.count), stream)
+        # 3. kR->kq batched Fourier over Rp at the kept q
+        nd2 = nw * nm
+        self._timed("kR->kq", None, lambda: L.call(
+            "ephip_fourier_batched_c32", nd2, self.nrp, L.ptr(self.A), nd2, nd2 * self.nrp, L.ptr(self.irvec_ep_dev),
+            L.ptr(st.ph.xq), nu, self.nq, L.ptr(self.count), L.ptr(self.col_index), self.cs, L.ptr(self.Gq), nd2,
+            self.cs * nd2, self.precision, stream))
+        # 4. rotations + g2
+        L.call("ephip_eph_rotate_kq_g2", nw, nm, st.nbw, nu, self.cs, L.ptr(self.count), L.ptr(self.col_index),
+               L.ptr(self.ikq), L.ptr(self.Gq), nd2, self.cs * nd2, L.ptr(st.el.U), L.ptr(st.el.filt.band_min),
+               L.ptr(st.el.filt.band_max), L.ptr(self.uph), L.ptr(self.omega), L.ptr(self.g2), None, stream)
+        payload = EPDataUnits((i0, i0 + n), self.unit_state[u0:u1], self.unit_band[u0:u1], self.count[:nu],
+                              self.col_index[:nu], self.ikq[:nu], self.g2[:nu], st.nbw)
+        for c in calculators:
+            c.run(payload)
+        return payload
+
+    def run(self, k_start: int = 0, k_stop: int | None = None, calculators=()):
+        nks = self.st.ksurv.shape[0]
+        k_stop = nks if k_stop is None else min(nks, k_stop)
+        for i0 in range(k_start, k_stop, self.nk_batch):
+            self.run_kbatch(i0, min(self.nk_batch, k_stop - i0), calculators)
+
+
+class UnitRecorder(AbstractCalculator):
+    """Copies every outer-k payload to the host (test helper)."""
+
+    def __init__(self):
+        self.batches = []
+
+    def run(self, p: EPDataUnits):
+        cnt = p.count.cpu().numpy()
+        ci = p.col_index.cpu().numpy()
+        kq = p.ikq.cpu().numpy()
+        g2 = p.g2.cpu().numpy()
+        self.batches.append(dict(unit_state=p.unit_state.cpu().numpy(), unit_band=p.unit_band.cpu().numpy(),
+                                 cols=[ci[u, :cnt[u]] for u in range(len(cnt))],
+                                 ikq=[kq[u, :cnt[u]] for u in range(len(cnt))],
+                                 g2=[g2[u, :cnt[u]] for u in range(len(cnt))]))

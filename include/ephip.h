@@ -78,13 +78,15 @@ int ephip_fourier_c64(int64_t ndata, int32_t nr, const void *op, int64_t ldop, c
 int ephip_heev_c32(int32_t n, int64_t nb, const void *H, int32_t h_is_c64, void *U, float *E,
                    int32_t *info, void *stream);
 
-/* FP64 refinement of ephip_heev_c32 results against the FP64 matrices H64 (c64, n x n x nb):
- * Rayleigh quotients for all vectors; for runs of eigenvalues closer than cluster_tol, a Rayleigh-Ritz
- * step (FP64 orthonormalisation, projection, Jacobi) that also re-orthogonalises their vectors.
- * U (c32) is updated in place; E64: double (n x nb) ascending. n <= 31. Replaces the FP64 accuracy of the
- * reference's zheev-based eigenvalues (src/common/AllocatedLAPACK.jl:44-108) on the device. */
+/* FP64 refinement of ephip_heev_c32 results against the FP64 matrices H64 (c64, n x n x nb): the FP32
+ * eigenvectors U are orthonormalised in FP64, H64 is projected onto them (M = Q^H H Q, unitarily similar
+ * to H and nearly diagonal) and M is diagonalised by parallel cyclic Jacobi sweeps in FP64 until the
+ * off-diagonal Frobenius norm is <= tol x the Frobenius norm (tol = 0: 1e-15). U (c32) is overwritten
+ * with the refined vectors; E64: double (n x nb) ascending. n <= 31. E32 is not read. Gives the FP64
+ * accuracy of the reference's zheev eigenvalues (src/common/AllocatedLAPACK.jl:44-108), also for
+ * degenerate bands. */
 int ephip_heev_refine(int32_t n, int64_t nb, const void *H64, void *U, const float *E32, double *E64,
-                      double cluster_tol, void *stream);
+                      double tol, void *stream);
 
 /* _filter_kpoints / inside_window -- src/filter.jl:19, :40-100 (device branch :46-72).
  * E: double (nw x nk) ascending. keep[k] = any band in [wmin, wmax]; bmin/bmax: 1-based
@@ -130,6 +132,52 @@ int ephip_eph_rotate_modes(int64_t nbasis, int32_t nm, int32_t nr_el, int32_t nq
 int ephip_eph_rotate_rqkq_g2(int32_t nw, int32_t nm, int64_t ncol, const void *g, int64_t ldg, const int64_t *ik,
                              const int64_t *ikq, const void *U, const int64_t *bmin, const int64_t *bmax,
                              const double *omega, int32_t nbw, float *g2, void *ep, void *stream);
+
+/* Batched fourier_batched!: nunits independent contractions in one launch (the list-batched Fourier of
+ * get_eph_kR_to_kq_batched!, src/wannier_to_bloch_batched.jl:245-273, over many k at once). Unit u uses
+ * op + u*op_stride (c32, ldop x nr) and writes out + u*out_stride (c32, ldout x ncol[u]); it has
+ * ncol[u] columns (device int32, or nk_max for every unit if NULL); column j is the point
+ * xk[col_index[u*col_stride + j]] (device int32, or xk[j] if NULL). nunits <= 65535. */
+int ephip_fourier_batched_c32(int64_t ndata, int32_t nr, const void *op, int64_t ldop, int64_t op_stride,
+                              const int32_t *irvec, const double *xk, int32_t nunits, int32_t nk_max,
+                              const int32_t *ncol, const int32_t *col_index, int64_t col_stride, void *out,
+                              int64_t ldout, int64_t out_stride, int32_t precision, void *stream);
+
+/* The two outer-momentum layouts of the e-ph tensor (src/model_new.jl:551-565): "ph" (b, Re) x Rp to
+ * "el" (b, Rp) x Re, b < nbasis (= nw*nw*nmodes), c32. ph and el must not alias. */
+int ephip_eph_ph_to_el(int64_t nbasis, int32_t nre, int32_t nrp, const void *ph, void *el, void *stream);
+
+/* k-side projection of get_eph_RR_to_kR! (src/wannier_to_bloch.jl:381-388; batched
+ * src/wannier_to_bloch_batched.jl:190-197) onto single in-window bands, with the outer-k window
+ * projection of src/calculator/run_eph_over_k_and_kq.jl:657-659. G: Fourier output of the "el" tensor
+ * for a k batch, column t = g(k_t)[i + nw*(j + nw*(nu + nm*Rp))] (c32, ldg per column). Unit u:
+ * A[u*nw*nm*nrp + (i + nw*nu) + nw*nm*Rp] = sum_j G[ldg*ucol[u] + ...j...] * U[ustate[u]][j, uband[u]]
+ * with U c32 (nw, nw, nstates) eigenvectors and uband 0-based. */
+int ephip_eph_project_kR(int32_t nw, int32_t nm, int32_t nrp, int32_t nunits, const void *G, int64_t ldg,
+                         const int32_t *ucol, const int64_t *ustate, const int32_t *uband, const void *U, void *A,
+                         void *stream);
+
+/* k+q lookup and window skip for the outer-k loop (_fill_iqs!, src/calculator/run_eph_over_k_and_kq.jl:
+ * 565-582, with the hash of src/common/kpoints.jl:614-651 on the unshifted k grid ngrid[3] (host)).
+ * For unit u at k = xk[ustate[u]] and every q = xq[c], c < nq: k+q's grid index h; keeps the q with
+ * bmin[h] > 0 (an in-window band at k+q), in q order: col_index[u*cs + i] = c, ikq[u*cs + i] = h,
+ * count[u] = number kept. cs >= nq. Pairs dropped here are exactly zero in the reference. */
+int ephip_kq_compact(int32_t nunits, const int64_t *ustate, const double *xk, int32_t nq, const double *xq,
+                     const int32_t *ngrid, const int64_t *bmin, int64_t cs, int32_t *col_index, int32_t *ikq,
+                     int32_t *count, void *stream);
+
+/* kR->kq rotations + g2 fold (eph_apply_rotations!, src/wannier_to_bloch_batched.jl:442-455;
+ * src/wannier_to_bloch.jl:428-440; CUDA ext/ElectronPhononCUDAExt.jl:146-201) on the k+q window:
+ * for unit u and kept column c < count[u], with Gq column = Gq + u*gstride + c*ldg holding
+ * g[i + nw*nu] (c32), q = col_index[u*cs + c], k+q state = ikq[u*cs + c]:
+ *   ep[m, nu'] = sum_i sum_nu conj(U_kq[i, bmin-1+m]) g[i + nw*nu] uph(q)[nu, nu'],
+ *   g2 = |ep|^2 / (2 omega(q)[nu'])   (uph c32 (nm, nm, nq), omega double (nm, nq)).
+ * Output row u*cs + c holds nbw*nm floats [m*nm + nu'] (zero for m beyond the k+q window); g2 and/or
+ * ep (c32, same layout) may be NULL. nunits <= 65535. */
+int ephip_eph_rotate_kq_g2(int32_t nw, int32_t nm, int32_t nbw, int32_t nunits, int64_t cs, const int32_t *count,
+                           const int32_t *col_index, const int32_t *ikq, const void *Gq, int64_t ldg,
+                           int64_t gstride, const void *U, const int64_t *bmin, const int64_t *bmax, const void *uph,
+                           const double *omega, float *g2, void *ep, void *stream);
 
 #ifdef __cplusplus
 }

@@ -77,3 +77,58 @@ def test_outer_q_loop_g2_parity(backend, cfg1, prec_name):
     e = np.array(errs)
     rel = e[:, 0].max() / e[:, 1].max()
     assert rel < 1e-4, rel
+
+
+@pytest.mark.parametrize("prec_name", ["fp32", "bf16x3"])
+def test_outer_k_loop_g2_parity(backend, cfg1, prec_name):
+    """Outer-k loop with the k-side window projection (RR->kR->kq) against the oracle's RR->Rq->kq
+    on the same pairs: the kept-q lists are exact (q kept iff k+q has an in-window band), |g|^2 within
+    1e-4 norm-wise after degenerate averaging."""
+    from ephip import _lib as L
+    from ephip import eph
+    m, g = cfg1
+    prec = L.PREC_FP32 if prec_name == "fp32" else L.PREC_BF16X3
+    w = g["window"]
+    st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
+    xk, E, U, keep, bmin, bmax, xq, ph = _oracle_states(m, g)
+    loop = eph.OuterKLoop(st, nk_batch=37, precision=prec)
+    rec = eph.UnitRecorder()
+    loop.run_kbatch(0, 37, [rec])
+    loop.run_kbatch(200, 37, [rec])
+    nw, nm = m.nw, m.nmodes
+    per_kq = {}
+    for b in rec.batches:
+        for u in range(len(b["unit_state"])):
+            k, n = int(b["unit_state"][u]), int(b["unit_band"][u])
+            expect = [c for c in range(len(xq)) if keep[O.hash_xk(xk[k] + xq[c], g["kgrid"])[0]]]
+            assert list(b["cols"][u]) == expect
+            assert np.array_equal(b["ikq"][u], O.hash_xk(xk[k] + xq[b["cols"][u]], g["kgrid"]))
+            for c, q in enumerate(b["cols"][u]):
+                per_kq.setdefault((k, int(q)), {})[n] = b["g2"][u][c].reshape(st.nbw, nm)  # [m, nu]
+    qsel = set(range(0, len(xq), 23)) | {0, 1}
+    cutoff, errs = 1e-5, []
+    eRpq_cache = {}
+    for (k, q), bands in per_kq.items():
+        if q not in qsel:
+            continue
+        om, uph = ph[q]
+        if q not in eRpq_cache:
+            eRpq_cache[q] = O.eph_RR_to_Rq(nw, nm, m.epmat_ph, m.irvec_el, m.irvec_ep, xq[q], uph)
+        kq = O.hash_xk(xk[k] + xq[q], g["kgrid"])[0]
+        Uk = U[k].reshape(nw, nw, order="F")[:, bmin[k] - 1:bmax[k]]
+        Ukq = U[kq].reshape(nw, nw, order="F")[:, bmin[kq] - 1:bmax[kq]]
+        ep = O.eph_Rq_to_kq(nw, nm, eRpq_cache[q], m.irvec_el, xk[k], Uk, Ukq)  # [nu, n, m]
+        nbk, nbkq = Uk.shape[1], Ukq.shape[1]
+        assert sorted(bands) == list(range(bmin[k] - 1, bmax[k]))
+        got = np.stack([bands[n][:nbkq, :].T for n in range(bmin[k] - 1, bmax[k])], axis=1)  # [nu, n, m]
+        assert not np.any([bands[n][nbkq:, :] for n in bands])
+        ref = O.g2(ep, om)
+        ok = np.abs(om) >= 1e-5
+        ek, ekq = E[k, bmin[k] - 1:bmax[k]], E[kq, bmin[kq] - 1:bmax[kq]]
+        r = O.g2_degenerate_average(ref, ekq, ek, cutoff)[ok]
+        gg = O.g2_degenerate_average(got.astype(np.float64), ekq, ek, cutoff)[ok]
+        errs.append((np.abs(gg - r).max(), np.abs(r).max()))
+    assert len(errs) > 50
+    e = np.array(errs)
+    rel = e[:, 0].max() / e[:, 1].max()
+    assert rel < 1e-4, rel
