@@ -29,6 +29,10 @@ sys.path.insert(0, os.path.join(ROOT, "electronphonon.jl_amd"))
 METRIC = ("g(k,q) k·q pairs/s at |g|²≤1e-4 rel err vs FP64; plain vs "
           "mixed+batched+surrogate")
 PEAK = {"fp32": 157.3, "bf16x3": 2500.0}  # TFLOP/s, MI355X dense MFMA (fp32, bf16)
+# HBM bytes per launch of the RR->kR Fourier at k batch 128 on this workload, from the rocprofv3
+# FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00) passes calibrated on a 256 MiB copy; see
+# profiles/cfg2_hbm_counters.md. Counters cannot be read inside the timed run.
+RRKR_TRAFFIC_BYTES_KB128 = (1412486 * 2 + 1054128) * 1024
 
 
 def log(*a):
@@ -94,6 +98,8 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "bf16x3"], default="fp32")
     ap.add_argument("--cpu-sample", type=int, default=3000, help="k-points in the CPU baseline slice (0 = skip)")
     ap.add_argument("--config", default="cfg2")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one plain 256 MiB device-to-device copy kernel (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -108,6 +114,13 @@ def main():
 
     t_setup = time.perf_counter()
     backend = core.gpu_backend(0)
+    if args.calibrate_copy:
+        # a plain copy of known size: 256 MiB read + 256 MiB written, for scaling HBM byte counters
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=backend.torch_device)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     m, grids = M.make_config_model(args.config, with_g=False)
     log(f"[bench] generating e-ph tensor ({args.config})")
     epmat = make_epmat(M, m)
@@ -255,8 +268,11 @@ def main():
                    "pre_filter_pairs": nk_total * nq_total,
                    "plain_outer_q_fp32_pairs_per_s": results.get("plain", {}).get("value")},
         "roofline": {"bound": "mfma", "achieved": dom[1], "peak": peak, "unit": "TFLOP/s",
-                     "frac": dom[1] / peak, "traffic": None, "kernel": dom[0], "avg_launch_ms": dom[2],
-                     "flop_per_launch": dom[3]},
+                     "frac": dom[1] / peak,
+                     "traffic": (RRKR_TRAFFIC_BYTES_KB128 if ("fast" in results and dom[0].startswith("ft kernel, RR->kR")
+                                                              and args.kbatch_k == 128 and args.config == "cfg2")
+                                 else None),
+                     "kernel": dom[0], "avg_launch_ms": dom[2], "flop_per_launch": dom[3]},
         "cpu_baseline": cpu,
         "detail": results,
     }

@@ -156,6 +156,154 @@ __global__ __launch_bounds__(256) void ft_mfma_kernel(int64_t ndata, int nr, con
         }
 }
 
+// Double-buffered form of ft_mfma_kernel: two LDS stages; the next stage's operand loads are issued
+// before the current stage's MFMAs and its phase entries are generated one per K step between them,
+// so global latency and the sincos work overlap the matrix cores; one barrier per stage. KC (complex R
+// per stage) is a tuning parameter.
+template <class T, int TN, int KC>
+__global__ __launch_bounds__(256) void ft_mfma_db_kernel(int64_t ndata, int nr, const cplx<T> *__restrict__ op,
+                                                         int64_t ldop, const int *__restrict__ irvec, int nk,
+                                                         const double *__restrict__ xk, cplx<T> *__restrict__ out,
+                                                         int64_t ldout, FtBatch bt) {
+    using acc_t = typename std::conditional<sizeof(T) == 4, f32x4, f64x4>::type;
+    constexpr int NCB = TN / 32;
+    constexpr int AE = FT_TM * KC / 256;  // op elements staged per thread per stage
+    constexpr int PSTEP = 256 / TN;
+    constexpr int BE = KC / PSTEP;        // phase entries per thread per stage (<= KC/2)
+    static_assert(AE >= 1 && BE >= 1 && BE <= KC / 2, "tile parameters");
+    __shared__ cplx<T> As[2][KC][FT_TM + FT_APAD];
+    __shared__ T Bs[2][KC][2][TN + FT_BPAD];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int64_t d0 = (int64_t)blockIdx.y * FT_TM;
+    const int64_t j0 = (int64_t)blockIdx.x * TN;
+    {
+        const int u = blockIdx.z;
+        if (bt.ncol) nk = bt.ncol[u];
+        if (j0 >= nk) return;
+        op += u * bt.op_stride;
+        out += u * bt.out_stride;
+        if (bt.col_index) bt.col_index += u * bt.col_stride;
+    }
+    const int pj = tid % TN;
+    const int prl0 = tid / TN;
+    double x0 = 0, x1 = 0, x2 = 0;
+    const bool pvalid = (j0 + pj) < nk;
+    if (pvalid) {
+        const int64_t pt = bt.col_index ? bt.col_index[j0 + pj] : j0 + pj;
+        x0 = xk[3 * pt];
+        x1 = xk[3 * pt + 1];
+        x2 = xk[3 * pt + 2];
+    }
+    acc_t acc[4][NCB];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < NCB; j++) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    cplx<T> areg[AE];
+    T creg[BE], sreg[BE];
+    auto load_a = [&](int r0) {
+#pragma unroll
+        for (int i = 0; i < AE; i++) {
+            const int e = tid + 256 * i, dl = e % FT_TM, rl = e / FT_TM;
+            const int64_t d = d0 + dl;
+            const int r = r0 + rl;
+            areg[i] = (d < ndata && r < nr) ? op[d + ldop * r] : cplx<T>{0, 0};
+        }
+    };
+    auto phase_one = [&](int r0, int i) {
+        const int r = r0 + prl0 + PSTEP * i;
+        T c = 0, s = 0;
+        if (pvalid && r < nr) {
+            const double t = irvec[3 * r] * x0 + irvec[3 * r + 1] * x1 + irvec[3 * r + 2] * x2;
+            cis2pi(t, c, s);
+        }
+        creg[i] = c;
+        sreg[i] = s;
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < AE; i++) {
+            const int e = tid + 256 * i;
+            As[buf][e / FT_TM][e % FT_TM] = areg[i];
+        }
+#pragma unroll
+        for (int i = 0; i < BE; i++) {
+            Bs[buf][prl0 + PSTEP * i][0][pj] = creg[i];
+            Bs[buf][prl0 + PSTEP * i][1][pj] = sreg[i];
+        }
+    };
+
+    load_a(0);
+#pragma unroll
+    for (int i = 0; i < BE; i++) phase_one(0, i);
+    stash(0);
+    __syncthreads();
+
+    const int il = lane & 15;
+    const int irow = (sizeof(T) == 8) ? (4 * (il & 3) + (il >> 2)) : il;  // see ft_mfma_kernel
+    const int nchunks = (nr + KC - 1) / KC;
+    for (int ch = 0; ch < nchunks; ch++) {
+        const int buf = ch & 1;
+        const int rn = (ch + 1) * KC;
+        const bool more = ch + 1 < nchunks;
+        if (more) load_a(rn);
+#pragma unroll
+        for (int ks = 0; ks < KC / 2; ks++) {
+            if (ks < BE && more) phase_one(rn, ks);
+            const int kk = lane >> 4;
+            const int rl = 2 * ks + (kk >> 1);
+            const int cc = kk & 1;
+            T a[4], b[NCB];
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++) {
+                const int rr = wm * 64 + 16 * rb + irow;
+                const cplx<T> v = As[buf][rl][rr >> 1];
+                a[rb] = (rr & 1) == 0 ? (cc == 0 ? v.re : -v.im) : (cc == 0 ? v.im : v.re);
+            }
+#pragma unroll
+            for (int cb = 0; cb < NCB; cb++) b[cb] = Bs[buf][rl][cc][wn * (TN / 2) + 16 * cb + (lane & 15)];
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+                for (int cb = 0; cb < NCB; cb++) acc[rb][cb] = mfma16x16x4<T>(a[rb], b[cb], acc[rb][cb]);
+        }
+        if (more) stash(buf ^ 1);
+        __syncthreads();
+    }
+
+    const int q = lane >> 4;
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++) {
+            const int64_t d = d0 + (wm * 64 + 16 * rb) / 2 + 2 * q;
+            const int64_t j = j0 + wn * (TN / 2) + 16 * cb + (lane & 15);
+            if (j < nk) {
+                if (d < ndata) out[d + ldout * j] = cplx<T>{acc[rb][cb][0], acc[rb][cb][1]};
+                if (d + 1 < ndata) out[d + 1 + ldout * j] = cplx<T>{acc[rb][cb][2], acc[rb][cb][3]};
+            }
+        }
+}
+
+// Kernel variant per contraction kind, set by the tuner (ephip_ft_set_variant). Kinds:
+// 0 = FP32 wide (nk > 64), 1 = FP32 narrow (nk <= 64), 2 = FP64. Variants: 0 = single-buffered,
+// 1 = double-buffered KC 16, 2 = double-buffered KC 8.
+static int g_ft_variant[3] = {0, 0, 0};
+
+template <class T, int TN>
+static void launch_ft(int variant, dim3 g, hipStream_t s, int64_t ndata, int nr, const cplx<T> *op, int64_t ldop,
+                      const int *irvec, int nk, const double *xk, cplx<T> *out, int64_t ldout, FtBatch bt) {
+    if (variant == 1)
+        ft_mfma_db_kernel<T, TN, 16><<<g, 256, 0, s>>>(ndata, nr, op, ldop, irvec, nk, xk, out, ldout, bt);
+    else if (variant == 2)
+        ft_mfma_db_kernel<T, TN, 8><<<g, 256, 0, s>>>(ndata, nr, op, ldop, irvec, nk, xk, out, ldout, bt);
+    else
+        ft_mfma_kernel<T, TN><<<g, 256, 0, s>>>(ndata, nr, op, ldop, irvec, nk, xk, out, ldout, bt);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Split-BF16 form. Every real operand x is written as x_hi + x_lo with x_hi = bf16(x) and
 // x_lo = bf16(x - x_hi); the product keeps hi*hi + hi*lo + lo*hi (the lo*lo term is below FP32
@@ -331,10 +479,10 @@ extern "C" int ephip_fourier_c32(int64_t ndata, int32_t nr, const void *op, int6
     if (precision == EPHIP_PREC_FP32) {
         if (nk <= 64) {
             dim3 g((nk + 63) / 64, gy);
-            ft_mfma_kernel<float, 64><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk, xk, c, ldout, FtBatch{});
+            launch_ft<float, 64>(g_ft_variant[1], g, s, ndata, nr, o, ldop, irvec, nk, xk, c, ldout, FtBatch{});
         } else {
             dim3 g((nk + 127) / 128, gy);
-            ft_mfma_kernel<float, 128><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk, xk, c, ldout, FtBatch{});
+            launch_ft<float, 128>(g_ft_variant[0], g, s, ndata, nr, o, ldop, irvec, nk, xk, c, ldout, FtBatch{});
         }
     } else {
         if (nk <= 64) {
@@ -357,8 +505,8 @@ extern "C" int ephip_fourier_c64(int64_t ndata, int32_t nr, const void *op, int6
     hipStream_t s = as_stream(stream);
     if (nr == 0) return zero_out<double>(ndata, nk, out, ldout, s);
     dim3 g((nk + 63) / 64, (unsigned)((ndata + FT_TM - 1) / FT_TM));
-    ft_mfma_kernel<double, 64><<<g, 256, 0, s>>>(ndata, nr, static_cast<const cplx<double> *>(op), ldop, irvec,
-                                                 nk, xk, static_cast<cplx<double> *>(out), ldout, FtBatch{});
+    launch_ft<double, 64>(g_ft_variant[2], g, s, ndata, nr, static_cast<const cplx<double> *>(op), ldop, irvec, nk,
+                          xk, static_cast<cplx<double> *>(out), ldout, FtBatch{});
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }
@@ -402,9 +550,9 @@ extern "C" int ephip_fourier_batched_c32(int64_t ndata, int32_t nr, const void *
     dim3 g(small ? (nk_max + 63) / 64 : (nk_max + 127) / 128, gy, nunits);
     if (precision == EPHIP_PREC_FP32) {
         if (small)
-            ft_mfma_kernel<float, 64><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk_max, xk, c, ldout, bt);
+            launch_ft<float, 64>(g_ft_variant[1], g, s, ndata, nr, o, ldop, irvec, nk_max, xk, c, ldout, bt);
         else
-            ft_mfma_kernel<float, 128><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk_max, xk, c, ldout, bt);
+            launch_ft<float, 128>(g_ft_variant[0], g, s, ndata, nr, o, ldop, irvec, nk_max, xk, c, ldout, bt);
     } else {
         if (small)
             ft_bf16x3_kernel<64><<<g, 256, 0, s>>>(ndata, nr, o, ldop, irvec, nk_max, xk, c, ldout, bt);
@@ -414,3 +562,12 @@ extern "C" int ephip_fourier_batched_c32(int64_t ndata, int32_t nr, const void *
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }
+
+extern "C" int ephip_ft_set_variant(int32_t kind, int32_t variant) {
+    EPHIP_REQUIRE(kind >= 0 && kind < 3, "ft_set_variant: kind %d outside 0..2", kind);
+    EPHIP_REQUIRE(variant >= 0 && variant < 3, "ft_set_variant: variant %d outside 0..2", variant);
+    g_ft_variant[kind] = variant;
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_ft_get_variant(int32_t kind) { return (kind >= 0 && kind < 3) ? g_ft_variant[kind] : -1; }

@@ -38,6 +38,18 @@ SIGNATURES = {
     "ephip_eph_ph_to_el": [_i64, _i32, _i32, _vp, _vp, _vp],
     "ephip_eph_project_kR": [_i32, _i32, _i32, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
     "ephip_kq_compact": [_i32, _vp, _vp, _i32, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
+    "ephip_mlp_features": [_i64, _vp, _i32, _vp, _vp, _vp],
+    "ephip_gemm_f32": [_i32, _i32, _i32, _i32, _i32, _vp, _i64, _vp, _i64, _vp, _i64, _vp, C.c_float, _vp],
+    "ephip_silu_fwd": [_i64, _vp, _vp, _vp],
+    "ephip_silu_bwd": [_i64, _vp, _vp, _vp, _vp],
+    "ephip_mse_grad": [_i64, _vp, _vp, C.c_float, _vp, _vp, _vp],
+    "ephip_colsum": [_i32, _i32, _vp, _i64, _vp, _vp],
+    "ephip_adam_step": [_i64, _vp, _vp, _vp, _vp, C.c_float, C.c_float, C.c_float, C.c_float, _i32, _vp],
+    "ephip_f32_to_bf16": [_i64, _vp, _vp, _vp],
+    "ephip_mlp_bf16_prune": [_i64, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, C.c_float, C.c_float,
+                             C.c_float, _vp, _vp, _vp],
+    "ephip_ft_set_variant": [_i32, _i32],
+    "ephip_ft_get_variant": [_i32],
     "ephip_eph_rotate_kq_g2": [_i32, _i32, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp,
                                _vp, _vp, _vp, _vp, _vp],
 }

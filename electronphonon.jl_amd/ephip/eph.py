@@ -433,6 +433,14 @@ class OuterKLoop:
             c.run(payload)
         return payload
 
+    def tune(self, reps: int = 3) -> dict:
+        """Pick the FP32 Fourier kernel variant with the shortest time for one full k batch."""
+        from . import tune as T
+        if self.precision != L.PREC_FP32:
+            return {}
+        n = min(self.nk_batch, self.st.ksurv.shape[0])
+        return T.tune_fourier(T.KIND_FP32_WIDE, lambda: self.run_kbatch(0, n), reps=reps)
+
     def run(self, k_start: int = 0, k_stop: int | None = None, calculators=()):
         nks = self.st.ksurv.shape[0]
         k_stop = nks if k_stop is None else min(nks, k_stop)

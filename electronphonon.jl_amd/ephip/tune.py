@@ -1,0 +1,46 @@
+"""Autotuner for the Fourier-contraction kernels.
+
+The reference hard-codes its crossovers ("tuned on one GPU", src/wannier_to_bloch_batched.jl:397-406).
+Here each contraction kind has several kernel variants (LDS staging depth, single/double buffering);
+``tune_fourier`` times every variant on the caller's live shape with HIP events on the launch stream
+and installs the fastest through ``ephip_ft_set_variant``. Results are plain dicts, so a caller can
+log or cache them.
+"""
+from __future__ import annotations
+
+from . import _lib as L
+
+KIND_FP32_WIDE, KIND_FP32_NARROW, KIND_FP64 = 0, 1, 2
+VARIANTS = (0, 1, 2)
+VARIANT_NAMES = {0: "single-buffered KC16", 1: "double-buffered KC16", 2: "double-buffered KC8"}
+
+
+def get_variant(kind: int) -> int:
+    return L.load().ephip_ft_get_variant(kind)
+
+
+def set_variant(kind: int, variant: int) -> None:
+    L.call("ephip_ft_set_variant", kind, variant)
+
+
+def tune_fourier(kind: int, run, reps: int = 3, variants=VARIANTS) -> dict:
+    """Time ``run()`` (which must launch the contraction of this kind) under each variant; keep the
+    fastest. Returns {variant: median ms, "best": variant}."""
+    import torch
+    res = {}
+    for v in variants:
+        set_variant(kind, v)
+        run()  # warm-up (code object load)
+        times = []
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            run()
+            b.record()
+            b.synchronize()
+            times.append(a.elapsed_time(b))
+        res[v] = sorted(times)[len(times) // 2]
+    best = min((v for v in variants), key=lambda v: res[v])
+    set_variant(kind, best)
+    res["best"] = best
+    return res

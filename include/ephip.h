@@ -179,6 +179,53 @@ int ephip_eph_rotate_kq_g2(int32_t nw, int32_t nm, int32_t nbw, int32_t nunits, 
                            int64_t gstride, const void *U, const int64_t *bmin, const int64_t *bmax, const void *uph,
                            const double *omega, float *g2, void *ep, void *stream);
 
+/* Kernel variant of the Fourier contractions, chosen by the tuner (ephip/tune.py) from timings on the
+ * live shapes. kind: 0 = FP32 with nk > 64, 1 = FP32 with nk <= 64, 2 = FP64. variant: 0 = single-
+ * buffered LDS stages of 16 R, 1 = double-buffered stages of 16 R, 2 = double-buffered stages of 8 R.
+ * Process-wide setting (no counterpart in the reference, whose tuning constants are hard-coded:
+ * src/wannier_to_bloch_batched.jl:397-406). Host-side, synchronous. */
+int ephip_ft_set_variant(int32_t kind, int32_t variant);
+int ephip_ft_get_variant(int32_t kind); /* returns the variant, or -1 for a bad kind */
+
+/* ---- Band-energy surrogate (no counterpart in the reference: the engine's own learned pre-filter,
+ * used ahead of the exact window filter of src/filter.jl:40-100). All arrays row-major FP32 unless
+ * noted. ---- */
+
+/* Periodic Fourier features of k: f[n, j] = cos(2 pi m_j.k_n), f[n, nf2 + j] = sin(...);
+ * xk double (3 x npts), mvec int32 (3 x nf2), f float (2 nf2 x npts). */
+int ephip_mlp_features(int64_t npts, const double *xk, int32_t nf2, const int32_t *mvec, float *f, void *stream);
+
+/* C (M x N) = op(A) op(B) (+ bias[n]) (+ beta C), row-major with leading dimensions; ta/tb transpose. */
+int ephip_gemm_f32(int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, const float *A, int64_t lda,
+                   const float *B, int64_t ldb, float *C, int64_t ldc, const float *bias, float beta, void *stream);
+
+/* a = silu(z); gz = ga * silu'(z). */
+int ephip_silu_fwd(int64_t n, const float *z, float *a, void *stream);
+int ephip_silu_bwd(int64_t n, const float *z, const float *ga, float *gz, void *stream);
+
+/* gy = 2 (y - t) scale; partial[b] (double) = sum of (y - t)^2 over block b of 256 elements. */
+int ephip_mse_grad(int64_t n, const float *y, const float *t, float scale, float *gy, double *partial, void *stream);
+
+/* out[c] = sum over rows of x[r*ldx + c]. */
+int ephip_colsum(int32_t rows, int32_t cols, const float *x, int64_t ldx, float *out, void *stream);
+
+/* One Adam step (bias-corrected, step counts from 1) on n parameters. */
+int ephip_adam_step(int64_t n, float *p, const float *g, float *m, float *v, float lr, float beta1, float beta2,
+                    float eps, int32_t step, void *stream);
+
+/* y (bf16) = x (float) rounded to nearest-even. */
+int ephip_f32_to_bf16(int64_t n, const float *x, void *y, void *stream);
+
+/* Fused BF16 inference of the MLP (features -> W1,b1 -> SiLU -> W2,b2 -> SiLU -> W3,b3) on bf16 MFMAs
+ * with FP32 accumulation, and the pruning test: keep[p] = 1 if any predicted band lies in
+ * [wmin - margin, wmax + margin], else 0. W1 (hidden x 2nf2), W2 (hidden x hidden), W3 (nout x hidden)
+ * bf16 row-major; biases float; y_out (nout x npts float) may be NULL. 2 nf2 and hidden multiples of
+ * 32 up to 128, nout a multiple of 16 up to 128. */
+int ephip_mlp_bf16_prune(int64_t npts, const double *xk, int32_t nf2, const int32_t *mvec, int32_t hidden,
+                         int32_t nout, const void *W1, const float *b1, const void *W2, const float *b2,
+                         const void *W3, const float *b3, float wmin, float wmax, float margin, float *y_out,
+                         uint8_t *keep, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -206,3 +206,58 @@ def test_rotate_modes_parity(backend):
     gg = g.reshape(nq, nr_el, nm, nb)
     ref = np.einsum("qrnb,qmn->qrmb", gg, u)  # u[q] stored transposed: u[q][nu', nu] = uph[nu, nu']
     assert _rel(got, ref) < 1e-6
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_fourier_variants_parity(backend, variant):
+    """Every tuner variant of the FP32/FP64 kernels gives the same contraction (odd sizes, both tile
+    widths)."""
+    from ephip import core, tune
+    rng = np.random.default_rng(50 + variant)
+    irvec = M.rbox(7)
+    nr = len(irvec)
+    old = [tune.get_variant(k) for k in range(3)]
+    try:
+        for k in range(3):
+            tune.set_variant(k, variant)
+        for ndata, nk in [(333, 200), (70, 33)]:
+            op = rng.standard_normal((nr, ndata)) + 1j * rng.standard_normal((nr, ndata))
+            xk = rng.random((nk, 3))
+            ref = O.fourier(op, irvec, xk)
+            got64 = core.fourier_batched(core.WannierObject(irvec, _dev(op)), _dev(xk)).cpu().numpy()
+            assert _rel(got64, ref) < 1e-13
+            got32 = core.fourier_batched(core.WannierObject(irvec, _dev(op.astype(np.complex64))), _dev(xk))
+            assert _rel(got32.cpu().numpy(), ref) < 2e-6
+    finally:
+        for k in range(3):
+            tune.set_variant(k, old[k])
+
+
+@pytest.mark.parametrize("prec_name", ["fp32", "bf16x3"])
+def test_fourier_batched_units(backend, prec_name):
+    """Batched entry: per-unit operands, gathered column lists and per-unit counts; columns beyond a
+    unit's count are left untouched."""
+    from ephip import _lib as L
+    prec = L.PREC_FP32 if prec_name == "fp32" else L.PREC_BF16X3
+    rng = np.random.default_rng(60)
+    irvec = M.rbox(5)
+    nr, nd, nu, npts, cs = len(irvec), 48, 5, 300, 260
+    op = rng.standard_normal((nu, nr, nd)) + 1j * rng.standard_normal((nu, nr, nd))
+    xk = rng.random((npts, 3))
+    counts = np.array([260, 0, 1, 129, 77], np.int32)
+    cols = np.zeros((nu, cs), np.int32)
+    for u in range(nu):
+        cols[u, :counts[u]] = np.sort(rng.choice(npts, counts[u], replace=False))
+    out = torch.full((nu, cs, nd), 3.0 + 0j, dtype=torch.complex64, device="cuda")
+    opd, xkd, irv = _dev(op.astype(np.complex64)), _dev(xk), _dev(irvec)
+    cd, cnt = _dev(cols), _dev(counts)
+    L.call("ephip_fourier_batched_c32", nd, nr, L.ptr(opd), nd, nr * nd, L.ptr(irv), L.ptr(xkd), nu, cs,
+           L.ptr(cnt), L.ptr(cd), cs, L.ptr(out), nd, cs * nd, prec, L.stream_handle())
+    o = out.cpu().numpy()
+    tol = 2e-6 if prec_name == "fp32" else 2e-5
+    for u in range(nu):
+        c = counts[u]
+        if c:
+            ref = O.fourier(op[u], irvec, xk[cols[u, :c]])
+            assert _rel(o[u, :c], ref) < tol
+        assert np.all(o[u, c:] == 3.0)
