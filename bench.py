@@ -98,6 +98,7 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "bf16x3"], default="fp32")
     ap.add_argument("--cpu-sample", type=int, default=3000, help="k-points in the CPU baseline slice (0 = skip)")
     ap.add_argument("--config", default="cfg2")
+    ap.add_argument("--tune", action="store_true", help="autotune the Fourier kernel variant on the live batch first")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 256 MiB device-to-device copy kernel (counter calibration)")
     args = ap.parse_args()
@@ -187,6 +188,10 @@ def main():
     # ---- fast outer-k path (k-side projection, k+q window skip)
     if args.path in ("fast", "both"):
         loop = eph.OuterKLoop(st, nk_batch=args.kbatch_k, precision=prec)
+        if args.tune:
+            tuned = loop.tune()
+            log(f"[bench] tuner (ms per k batch by variant): {tuned}")
+            results["tuner"] = {str(k): v for k, v in tuned.items()}
         ev = []
         loop.ft_timer = ev
         nks_steps = args.kbatch_k

@@ -29,25 +29,30 @@ __global__ void mlp_features_kernel(int64_t n, const double *__restrict__ xk, in
 // C[m, n] = sum_k opA[m, k] opB[k, n] (+ bias[n]) (+ beta C[m, n]), row-major operands:
 // opA[m, k] = ta ? A[k*lda + m] : A[m*lda + k], opB[k, n] = tb ? B[n*ldb + k] : B[k*ldb + n].
 // 64 x 64 tile per 256-thread workgroup, 4 x 4 outputs per thread, K in stages of 16.
+// Split-K: grid z-slice z covers k in [z*kchunk, min(K, (z+1)*kchunk)) and writes C + z*cstride.
 __global__ __launch_bounds__(256) void gemm_f32_kernel(int ta, int tb, int M, int N, int K, const float *__restrict__ A,
                                                        int64_t lda, const float *__restrict__ B, int64_t ldb,
                                                        float *__restrict__ C, int64_t ldc,
-                                                       const float *__restrict__ bias, float beta) {
+                                                       const float *__restrict__ bias, float beta, int kchunk,
+                                                       int64_t cstride) {
     __shared__ float As[16][64 + 4];
     __shared__ float Bs[16][64 + 4];
     const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
     const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int kbeg = blockIdx.z * kchunk;
+    const int kend = min(K, kbeg + kchunk);
+    C += blockIdx.z * cstride;
     float acc[4][4] = {};
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    for (int k0 = kbeg; k0 < kend; k0 += 16) {
         for (int e = tid; e < 64 * 16; e += 256) {
             const int mm = e % 64, kk = e / 64;
             const int m = m0 + mm, k = k0 + kk;
             float v = 0.f;
-            if (m < M && k < K) v = ta ? A[(int64_t)k * lda + m] : A[(int64_t)m * lda + k];
+            if (m < M && k < kend) v = ta ? A[(int64_t)k * lda + m] : A[(int64_t)m * lda + k];
             As[kk][mm] = v;
             const int n = n0 + mm;
             float w = 0.f;
-            if (n < N && k < K) w = tb ? B[(int64_t)n * ldb + k] : B[(int64_t)k * ldb + n];
+            if (n < N && k < kend) w = tb ? B[(int64_t)n * ldb + k] : B[(int64_t)k * ldb + n];
             Bs[kk][mm] = w;
         }
         __syncthreads();
@@ -245,7 +250,24 @@ extern "C" int ephip_gemm_f32(int32_t ta, int32_t tb, int32_t M, int32_t N, int3
     EPHIP_REQUIRE(A && B && C, "gemm_f32: null device pointer");
     EPHIP_REQUIRE(lda >= (ta ? M : K) && ldb >= (tb ? K : N) && ldc >= N, "gemm_f32: leading dimension too small");
     dim3 g((N + 63) / 64, (M + 63) / 64);
-    gemm_f32_kernel<<<g, 256, 0, as_stream(stream)>>>(ta, tb, M, N, K, A, lda, B, ldb, C, ldc, bias, beta);
+    gemm_f32_kernel<<<g, 256, 0, as_stream(stream)>>>(ta, tb, M, N, K, A, lda, B, ldb, C, ldc, bias, beta,
+                                                      K > 0 ? K : 1, 0);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_gemm_f32_splitk(int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, const float *A,
+                                     int64_t lda, const float *B, int64_t ldb, float *Cpart, int64_t ldc,
+                                     int32_t nsplit, void *stream) {
+    EPHIP_REQUIRE(M >= 0 && N >= 0 && K >= 0 && nsplit >= 1 && nsplit <= 65535, "gemm_f32_splitk: bad sizes");
+    EPHIP_REQUIRE((M + 63) / 64 <= 65535, "gemm_f32_splitk: M too large");
+    if (M == 0 || N == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(A && B && Cpart, "gemm_f32_splitk: null device pointer");
+    EPHIP_REQUIRE(lda >= (ta ? M : K) && ldb >= (tb ? K : N) && ldc >= N, "gemm_f32_splitk: leading dimension too small");
+    const int kchunk = ((K + nsplit - 1) / nsplit + 15) / 16 * 16;
+    dim3 g((N + 63) / 64, (M + 63) / 64, nsplit);
+    gemm_f32_kernel<<<g, 256, 0, as_stream(stream)>>>(ta, tb, M, N, K, A, lda, B, ldb, Cpart, ldc, nullptr, 0.f,
+                                                      kchunk > 0 ? kchunk : 16, (int64_t)M * ldc);
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

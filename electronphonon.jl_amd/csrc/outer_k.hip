@@ -149,6 +149,110 @@ __global__ __launch_bounds__(256) void rotate_kq_g2_kernel(int nw, int nm, int n
     }
 }
 
+// Same contraction, 16 kept columns of one unit per 256-thread workgroup with every operand staged in
+// LDS: the columns' Fourier outputs (contiguous in memory, one coalesced sweep), their k+q window
+// eigenvector columns, phonon eigenvectors and frequencies. Then t = U_kq^H g and ep = t u_ph from LDS.
+constexpr int RKQ_COLS = 16;
+
+__host__ __device__ inline size_t rkq_lds_bytes(int nw, int nm, int nbw) {
+    return sizeof(c32) * (size_t)RKQ_COLS * ((size_t)nw * nm + (size_t)nm * nm + (size_t)nbw * nw + (size_t)nbw * nm) +
+           sizeof(float) * RKQ_COLS * nm + sizeof(int) * RKQ_COLS * 2;
+}
+
+__global__ __launch_bounds__(256) void rotate_kq_g2_lds_kernel(int nw, int nm, int nbw, int nunits, int64_t cs,
+                                                              const int *__restrict__ count,
+                                                              const int *__restrict__ col_index,
+                                                              const int *__restrict__ ikq, const c32 *__restrict__ Gq,
+                                                              int64_t ldg, int64_t gstride, const c32 *__restrict__ U,
+                                                              const int64_t *__restrict__ bmin,
+                                                              const int64_t *__restrict__ bmax,
+                                                              const c32 *__restrict__ uph,
+                                                              const double *__restrict__ omega,
+                                                              float *__restrict__ g2, c32 *__restrict__ ep_out) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char rkq_lds[];
+    const int u = blockIdx.y;
+    const int64_t c0 = (int64_t)blockIdx.x * RKQ_COLS;
+    if (u >= nunits) return;
+    const int cnt = count[u];
+    if (c0 >= cnt) return;  // uniform for the block
+    const int ncol = (int)min<int64_t>(RKQ_COLS, cnt - c0);
+    const int tid = threadIdx.x;
+    const int nd = nw * nm;
+    c32 *Gs = reinterpret_cast<c32 *>(rkq_lds);            // [c][nd]
+    c32 *Ps = Gs + RKQ_COLS * nd;                           // [c][nm*nm]
+    c32 *Us = Ps + RKQ_COLS * nm * nm;                      // [c][m][i]
+    c32 *Ts = Us + RKQ_COLS * nbw * nw;                     // [c][m][nu]
+    float *Ws = reinterpret_cast<float *>(Ts + RKQ_COLS * nbw * nm);  // [c][nu] omega
+    int *Ms = reinterpret_cast<int *>(Ws + RKQ_COLS * nm);             // [c] mlo, [c] nbkq
+    const int64_t row0 = (int64_t)u * cs + c0;
+    if (tid < RKQ_COLS) {
+        int mlo = 0, nb = 0;
+        if (tid < ncol) {
+            const int64_t kq = ikq[row0 + tid];
+            if (bmin[kq] > 0) {
+                mlo = (int)bmin[kq] - 1;
+                nb = (int)(bmax[kq] - bmin[kq] + 1);
+                nb = nb < nbw ? nb : nbw;
+            }
+        }
+        Ms[tid] = mlo;
+        Ms[RKQ_COLS + tid] = nb;
+    }
+    // Fourier outputs of the ncol columns are contiguous: ldg == nd is not assumed, copy column-wise
+    const c32 *gsrc = Gq + gstride * u + ldg * c0;
+    for (int e = tid; e < ncol * nd; e += 256) {
+        const int c = e / nd, d = e % nd;
+        Gs[e] = gsrc[ldg * c + d];
+    }
+    for (int e = tid; e < ncol * nm * nm; e += 256) {
+        const int c = e / (nm * nm), r = e % (nm * nm);
+        Ps[e] = uph[(int64_t)nm * nm * col_index[row0 + c] + r];
+    }
+    for (int e = tid; e < ncol * nm; e += 256) {
+        const int c = e / nm, r = e % nm;
+        Ws[e] = static_cast<float>(omega[(int64_t)nm * col_index[row0 + c] + r]);
+    }
+    __syncthreads();
+    for (int e = tid; e < ncol * nbw * nw; e += 256) {
+        const int c = e / (nbw * nw), r = e % (nbw * nw), m = r / nw, i = r % nw;
+        c32 v{0.f, 0.f};
+        if (m < Ms[RKQ_COLS + c]) v = U[(int64_t)nw * nw * ikq[row0 + c] + (int64_t)nw * (Ms[c] + m) + i];
+        Us[e] = v;
+    }
+    __syncthreads();
+    // t[c][m][nu] = sum_i conj(U[c][m][i]) g[c][i + nw nu]
+    for (int e = tid; e < ncol * nbw * nm; e += 256) {
+        const int c = e / (nbw * nm), r = e % (nbw * nm), m = r / nm, nu = r % nm;
+        float sr = 0.f, si = 0.f;
+        const c32 *uu = Us + (c * nbw + m) * nw;
+        const c32 *gg = Gs + c * nd + nw * nu;
+        for (int i = 0; i < nw; i++) {
+            const c32 w = uu[i], a = gg[i];
+            sr += w.re * a.re + w.im * a.im;
+            si += w.re * a.im - w.im * a.re;
+        }
+        Ts[e] = c32{sr, si};
+    }
+    __syncthreads();
+    // ep[c][m][nu'] = sum_nu t[c][m][nu] uph[c][nu + nm nu'], g2 = |ep|^2 / (2 omega)
+    for (int e = tid; e < ncol * nbw * nm; e += 256) {
+        const int c = e / (nbw * nm), r = e % (nbw * nm), m = r / nm, nu2 = r % nm;
+        float er = 0.f, ei = 0.f;
+        if (m < Ms[RKQ_COLS + c]) {
+            const c32 *tt = Ts + (c * nbw + m) * nm;
+            const c32 *pp = Ps + c * nm * nm + nm * nu2;
+            for (int nu = 0; nu < nm; nu++) {
+                const c32 t = tt[nu], w = pp[nu];
+                er += t.re * w.re - t.im * w.im;
+                ei += t.re * w.im + t.im * w.re;
+            }
+        }
+        const int64_t o = (row0 + c) * nbw * nm + r;
+        if (g2) g2[o] = (er * er + ei * ei) / (2.f * Ws[c * nm + nu2]);
+        if (ep_out) ep_out[o] = c32{er, ei};
+    }
+}
+
 }  // namespace ephip
 
 using namespace ephip;
@@ -210,12 +314,20 @@ extern "C" int ephip_eph_rotate_kq_g2(int32_t nw, int32_t nm, int32_t nbw, int32
     if (nunits == 0 || cs == 0) return EPHIP_OK;
     EPHIP_REQUIRE(count && col_index && ikq && Gq && U && bmin && bmax && uph && omega && (g2 || ep),
                   "eph_rotate_kq_g2: null device pointer");
-    dim3 grid((unsigned)((cs + 3) / 4), (unsigned)nunits);
-    rotate_kq_g2_kernel<<<grid, 256, lds, as_stream(stream)>>>(nw, nm, nbw, nunits, cs, count, col_index, ikq,
-                                                              static_cast<const c32 *>(Gq), ldg, gstride,
-                                                              static_cast<const c32 *>(U), bmin, bmax,
-                                                              static_cast<const c32 *>(uph), omega, g2,
-                                                              static_cast<c32 *>(ep));
+    const size_t lds2 = rkq_lds_bytes(nw, nm, nbw);
+    if (lds2 <= 64 * 1024) {
+        dim3 grid((unsigned)((cs + RKQ_COLS - 1) / RKQ_COLS), (unsigned)nunits);
+        rotate_kq_g2_lds_kernel<<<grid, 256, lds2, as_stream(stream)>>>(
+            nw, nm, nbw, nunits, cs, count, col_index, ikq, static_cast<const c32 *>(Gq), ldg, gstride,
+            static_cast<const c32 *>(U), bmin, bmax, static_cast<const c32 *>(uph), omega, g2, static_cast<c32 *>(ep));
+    } else {  // large nw*nm: one wave per column
+        dim3 grid((unsigned)((cs + 3) / 4), (unsigned)nunits);
+        rotate_kq_g2_kernel<<<grid, 256, lds, as_stream(stream)>>>(nw, nm, nbw, nunits, cs, count, col_index, ikq,
+                                                                  static_cast<const c32 *>(Gq), ldg, gstride,
+                                                                  static_cast<const c32 *>(U), bmin, bmax,
+                                                                  static_cast<const c32 *>(uph), omega, g2,
+                                                                  static_cast<c32 *>(ep));
+    }
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

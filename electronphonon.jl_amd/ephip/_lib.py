@@ -40,6 +40,7 @@ SIGNATURES = {
     "ephip_kq_compact": [_i32, _vp, _vp, _i32, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
     "ephip_mlp_features": [_i64, _vp, _i32, _vp, _vp, _vp],
     "ephip_gemm_f32": [_i32, _i32, _i32, _i32, _i32, _vp, _i64, _vp, _i64, _vp, _i64, _vp, C.c_float, _vp],
+    "ephip_gemm_f32_splitk": [_i32, _i32, _i32, _i32, _i32, _vp, _i64, _vp, _i64, _vp, _i64, _i32, _vp],
     "ephip_silu_fwd": [_i64, _vp, _vp, _vp],
     "ephip_silu_bwd": [_i64, _vp, _vp, _vp, _vp],
     "ephip_mse_grad": [_i64, _vp, _vp, C.c_float, _vp, _vp, _vp],

@@ -63,6 +63,35 @@ def compute_electron_states(ham_dev: WannierObject, xk_dev, window=None, chunk: 
     return ElectronStates(xk_dev, E, U, filt, window)
 
 
+def compute_electron_states_pruned(ham_dev: WannierObject, xk_dev, window, surrogate, chunk: int = 32768):
+    """compute_electron_states with the band-energy surrogate as a pre-filter: BF16 inference marks the
+    k-points whose predicted bands come within the surrogate's margin of the window; only those get the
+    exact FP64 Fourier + eigensolve + window filter. Returns (states, candidates): states are full-grid
+    arrays in which pruned points have keep = 0, band ranges 0, E = NaN, U = 0 and nbelow = -1 (not
+    computed). With zero misses the keep / band-range arrays equal the exact filter's."""
+    torch = _torch()
+    nk = xk_dev.shape[0]
+    nw = int(round(math.sqrt(ham_dev.ndata)))
+    dev = xk_dev.device
+    keep_s, _ = surrogate.infer_prune(xk_dev, window, surrogate.margin)
+    cand = torch.nonzero(keep_s, as_tuple=False).view(-1)
+    E = torch.full((nk, nw), float("nan"), dtype=torch.float64, device=dev)
+    U = torch.zeros((nk, nw, nw), dtype=torch.complex64, device=dev)
+    keep = torch.zeros(nk, dtype=torch.uint8, device=dev)
+    bmin = torch.zeros(nk, dtype=torch.int64, device=dev)
+    bmax = torch.zeros(nk, dtype=torch.int64, device=dev)
+    nbelow = torch.full((nk,), -1, dtype=torch.int64, device=dev)
+    if cand.numel():
+        sub = compute_electron_states(ham_dev, xk_dev.index_select(0, cand).contiguous(), window, chunk)
+        E.index_copy_(0, cand, sub.E)
+        U.index_copy_(0, cand, sub.U)
+        keep.index_copy_(0, cand, sub.filt.keep)
+        bmin.index_copy_(0, cand, sub.filt.band_min)
+        bmax.index_copy_(0, cand, sub.filt.band_max)
+        nbelow.index_copy_(0, cand, sub.filt.nbelow)
+    return ElectronStates(xk_dev, E, U, core.FilterResult(keep, bmin, bmax, nbelow), window), cand
+
+
 @dataclasses.dataclass
 class PhononStates:
     xq: object

@@ -199,6 +199,11 @@ int ephip_mlp_features(int64_t npts, const double *xk, int32_t nf2, const int32_
 int ephip_gemm_f32(int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, const float *A, int64_t lda,
                    const float *B, int64_t ldb, float *C, int64_t ldc, const float *bias, float beta, void *stream);
 
+/* Split-K form of ephip_gemm_f32 (no bias/beta): slice z of nsplit sums its share of K into
+ * Cpart + z*M*ldc; reduce the nsplit partials with ephip_colsum(nsplit, M*ldc, ...). */
+int ephip_gemm_f32_splitk(int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, const float *A, int64_t lda,
+                          const float *B, int64_t ldb, float *Cpart, int64_t ldc, int32_t nsplit, void *stream);
+
 /* a = silu(z); gz = ga * silu'(z). */
 int ephip_silu_fwd(int64_t n, const float *z, float *a, void *stream);
 int ephip_silu_bwd(int64_t n, const float *z, const float *ga, float *gz, void *stream);

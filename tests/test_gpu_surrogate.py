@@ -1,0 +1,94 @@
+"""GPU tests of the band-energy surrogate: HIP forward/backward against the FP64 reference, training,
+BF16 inference, and the zero-miss pruning guarantee against the exact window filter."""
+import numpy as np
+import pytest
+
+import mlp_ref as R
+from ephip import model as M
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _params_np(s):
+    p = s.p.cpu().numpy().astype(np.float64)
+    return {k: p[s.offs[k]:s.offs[k] + int(np.prod(sh))].reshape(sh) for k, sh in s.shapes.items()}
+
+
+def test_surrogate_gradients_vs_fp64_reference(backend):
+    from ephip import surrogate as S
+    s = S.BandSurrogate(16, backend.torch_device, nf2=32, hidden=64, seed=3)
+    # perturb biases so every term is exercised
+    s.p += 0.05 * torch.randn_like(s.p)
+    rng = np.random.default_rng(4)
+    n = 700
+    xk = rng.random((n, 3))
+    E = rng.standard_normal((n, 16))
+    xd = torch.from_numpy(xk).cuda()
+    tgt = s.targets(torch.from_numpy(E).cuda())
+    partial, acts, _ = s.loss_and_grad(xd, tgt)
+    loss = float(partial.sum()) / tgt.numel()
+    p = _params_np(s)
+    lref, gref = R.loss_and_grad(p, xk, tgt.cpu().numpy().astype(np.float64), R.fourier_vectors(32))
+    assert abs(loss - lref) <= 1e-5 * lref
+    gg = s.g.cpu().numpy()
+    for k in gref:
+        got = gg[s.offs[k]:s.offs[k] + gref[k].size].reshape(gref[k].shape)
+        err = np.abs(got - gref[k]).max() / np.abs(gref[k]).max()
+        assert err < 1e-4, (k, err)
+    yref, _ = R.forward(p, xk, R.fourier_vectors(32))
+    assert np.abs(acts["y"].cpu().numpy() - yref).max() < 1e-4 * np.abs(yref).max()
+
+
+def _exact_states(backend, m, xk):
+    from ephip import core, eph
+    ham = core.WannierObject(m.irvec_h, m.ham_r).to_device(backend)
+    return eph.compute_electron_states(ham, torch.from_numpy(xk).cuda(), None)
+
+
+def test_surrogate_train_bf16_and_zero_miss_prune(backend):
+    """Train on exact eigenvalues of a 24^3 coarse grid + 20000 random k, calibrate the margin on the
+    BF16 inference (2 x its max training error), prune a 48^3 fine grid, and check the pruned pipeline's
+    selection (keep flags and band ranges) is identical to the exact filter's on the full grid: zero
+    misses. Model: 16 bands, on-site energies in [-3, 3] eV and short-ranged hopping (decay 0.45), so the
+    +-0.3 eV window leaves sparse, uneven band counts. Also reports what the pruning saves."""
+    import time
+    from ephip import core, eph
+    from ephip import surrogate as S
+    m = M.make_model(nw=16, natoms=4, box_hd=9, box_g=3, seed=1004, onsite=(-3.0, 3.0), h_decay=0.45,
+                     with_g=False)
+    ham = core.WannierObject(m.irvec_h, m.ham_r).to_device(backend)
+    rng = np.random.default_rng(5)
+    xtr = np.vstack([M.kpoints_grid((24, 24, 24)), rng.random((20000, 3))])
+    xd = torch.from_numpy(xtr).cuda()
+    st = eph.compute_electron_states(ham, xd, None)
+    s = S.BandSurrogate(m.nw, backend.torch_device, nf2=32, seed=1)
+    t0 = time.perf_counter()
+    losses = s.train(xd, st.E, steps=3000)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    assert losses[-1][1] < 0.01 * losses[0][1]
+    e32, e16 = s.calibrate(xd, st.E)
+    assert abs(e16 - e32) < 0.2 * e32 + 0.01  # BF16 inference tracks the FP32 network
+    w = 0.3
+    xf = torch.from_numpy(M.kpoints_grid((48, 48, 48))).cuda()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    exact = eph.compute_electron_states(ham, xf, (-w, w))
+    torch.cuda.synchronize()
+    t_exact = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pruned, cand = eph.compute_electron_states_pruned(ham, xf, (-w, w), s)
+    torch.cuda.synchronize()
+    t_pruned = time.perf_counter() - t0
+    ke = exact.filt.keep.cpu().numpy().astype(bool)
+    kp = pruned.filt.keep.cpu().numpy().astype(bool)
+    misses = int((ke & ~kp).sum())
+    print(f"surrogate: train {t_train:.1f} s, max err fp32 {e32:.4f} eV bf16 {e16:.4f} eV, margin {s.margin:.4f} eV; "
+          f"candidates {cand.numel()} of {xf.shape[0]} ({cand.numel() / xf.shape[0]:.3f}), exact survivors "
+          f"{ke.sum()}; misses {misses}; states exact {t_exact * 1e3:.1f} ms, pruned {t_pruned * 1e3:.1f} ms")
+    assert misses == 0
+    assert np.array_equal(kp, ke)
+    assert np.array_equal(pruned.filt.band_min.cpu().numpy(), exact.filt.band_min.cpu().numpy())
+    assert np.array_equal(pruned.filt.band_max.cpu().numpy(), exact.filt.band_max.cpu().numpy())
+    assert cand.numel() < xf.shape[0]  # the network removes exact solves
