@@ -61,6 +61,30 @@ def synchronize(backend: GPUBackend | None = None) -> None:
     L.call("ephip_synchronize", L.stream_handle())
 
 
+def plan_batch(free: int | None, per_point: int, committed: int, cap: int, headroom=(7, 10), what: str = "") -> int:
+    """plan_batch (src/calculator/eph_device_staging.jl:29-47): batch width that fits
+    ``(free - committed) // den * num`` bytes at ``per_point`` bytes each, clamped to [1, cap]; ``free``
+    None means unbounded (batch = cap). Fails early if the whole-run commitments exceed free memory."""
+    num, den = headroom
+    if free is not None and committed > free:
+        raise L.EphipError(f"batched {what}: committed device memory ({committed / 1e9:.2f} GB, whole-run "
+                           f"stacks) exceeds free device memory ({free / 1e9:.2f} GB). Reduce the batch cap or "
+                           "the grid size.")
+    nb_mem = int(cap) if free is None else max(1, ((free - committed) // den * num) // per_point)
+    return min(int(cap), nb_mem)
+
+
+def outer_k_staging_bytes(nw: int, nm: int, nr_el: int, nr_ep: int, nk: int, nq: int, nbw: int) -> tuple[int, int]:
+    """Device bytes of the outer-k loop: (per k of a batch, whole-run committed). Per k: the RR->kR
+    Fourier output (8 nw^2 nm nr_ep) and, for each of up to nbw window bands, the projected operator
+    (8 nw nm nr_ep), kept-q lists (8 nq), the kR->kq Fourier output (8 nw nm nq) and the g2 block
+    (4 nbw nm nq). Committed: both e-ph tensor layouts, fine-grid states and phonons."""
+    per_k = 8 * nw * nw * nm * nr_ep + nbw * (8 * nw * nm * nr_ep + 8 * nq + 8 * nw * nm * nq + 4 * nbw * nm * nq + 4)
+    committed = (2 * 8 * nw * nw * nm * nr_el * nr_ep + (8 * nw + 8 * nw * nw + 49) * nk
+                 + (8 * nm + 8 * nm * nm + 24) * nq)
+    return per_k, committed
+
+
 def alloc(backend: GPUBackend, dtype, *dims_colmajor):
     """alloc(backend, T, dims...) (src/common/gpu_utils.jl:41-42) for column-major dims."""
     torch = _torch()

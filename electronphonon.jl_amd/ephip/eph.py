@@ -368,9 +368,15 @@ class OuterKLoop:
       4. k+q-window and phonon rotations, g2 = |ep|^2 / (2 omega).
     Pairs (k, q) with no in-window band at k+q are exactly zero and are not computed."""
 
-    def __init__(self, st: OuterQSetup, nk_batch: int = 128, precision: int = L.PREC_FP32):
+    def __init__(self, st: OuterQSetup, nk_batch: int | None = 128, precision: int = L.PREC_FP32,
+                 nk_batch_cap: int = 512):
         torch = _torch()
         self.st = st
+        if nk_batch is None:  # memory-adaptive width (plan_batch), the e-ph tensor already resident
+            per_k, _ = core.outer_k_staging_bytes(st.nw, st.nm, st.irvec_el_dev.shape[0], st.epmat.nr,
+                                                  st.el.xk.shape[0], st.ph.xq.shape[0], st.nbw)
+            nre_b = 8 * st.epmat.op_r.numel()  # the "el" copy is allocated below
+            nk_batch = core.plan_batch(core.free_bytes(None), per_k, nre_b, nk_batch_cap, what="outer-k")
         self.nk_batch = int(nk_batch)
         self.precision = precision
         dev = st.el.xk.device
