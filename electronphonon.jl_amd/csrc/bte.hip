@@ -1,0 +1,261 @@
+// Boltzmann transport on the outer-k payload (FP64): scattering increments, the scattering-out rates
+// and scattering-in matrix, band velocities, and the sparse product of the iterative solver.
+//
+// bte_scattering_increments restates src/boltzmann/bte_scattering_core.jl:33-84 (occupation
+// conventions 1..6, vanishing-delta guards) with the smeared delta of src/common/smearing.jl;
+// the accumulation follows the device kernel ext/ElectronPhononCUDAExt.jl:323-372 /
+// bte_window_accumulate! (src/boltzmann/boltzmann_calculator.jl:336-372): atomic S_out, plain S_in.
+// S_in is stored in the outer-k loop's own sparse layout: row (unit u, kept column c, k+q band slot m).
+#include "common.hpp"
+
+namespace ephip {
+
+__device__ __forceinline__ double occ_fd(double e, double T) { return 1.0 / (exp(e / T) + 1.0); }
+__device__ __forceinline__ double occ_fd_deriv(double e, double T) {
+    return -1.0 / (2.0 + exp(e / T) + exp(-e / T)) / T;
+}
+__device__ __forceinline__ double occ_boson(double e, double T) { return e == 0.0 ? 0.0 : 1.0 / expm1(e / T); }
+__device__ __forceinline__ double smear(int method, double eta, double x) {
+    if (method == 1) return (1.0 / sqrt(M_PI)) / eta * exp(-(x / eta) * (x / eta));
+    return (eta / M_PI) / (x * x + eta * eta);
+}
+
+__device__ void bte_increments(int method, double ek, double ekq, double wq, double g2, double wtq, double mu,
+                               double T, int smearing, double eta, double &so, double &si) {
+    const double d1 = smear(smearing, eta, ek - ekq + wq);
+    const double d2 = smear(smearing, eta, ek - ekq - wq);
+    so = 0.0;
+    si = 0.0;
+    if (d1 == 0.0 && d2 == 0.0) return;
+    const double nq = occ_boson(wq, T), fk = occ_fd(ek - mu, T), fkq = occ_fd(ekq - mu, T);
+    double f1o, f2o, f1i, f2i;
+    if (method == 1) {
+        f1o = nq + fkq; f2o = nq + 1 - fkq;
+        f1i = nq + fk;  f2i = nq + 1 - fk;
+    } else if (method == 2) {
+        const double preo = fkq * (1 - fkq) / (fk * (1 - fk));
+        f1o = preo * (nq + 1 - fk); f2o = preo * (nq + fk);
+        const double prei = fk * (1 - fk) / (fkq * (1 - fkq));
+        f1i = prei * (nq + 1 - fkq); f2i = prei * (nq + fkq);
+    } else if (method == 3) {
+        const double preo = (1 - fkq) / (1 - fk);
+        f1o = preo * nq; f2o = preo * (nq + 1);
+        const double prei = (1 - fk) / (1 - fkq);
+        f1i = prei * nq; f2i = prei * (nq + 1);
+    } else if (method == 4) {
+        const double preo = fkq / fk;
+        f1o = preo * (nq + 1); f2o = preo * nq;
+        const double prei = fk / fkq;
+        f1i = prei * (nq + 1); f2i = prei * nq;
+    } else if (method == 5) {
+        const double dk = occ_fd_deriv(ek - mu, T), dkq = occ_fd_deriv(ekq - mu, T);
+        f1o = sqrt(nq * (nq + 1) * dkq / dk); f2o = f1o;
+        f1i = sqrt(nq * (nq + 1) * dk / dkq); f2i = f1i;
+    } else {
+        const double no = occ_boson(ekq - ek, T);
+        f1o = no + fkq; f2o = -(no + fkq);
+        const double ni = occ_boson(ek - ekq, T);
+        f1i = ni + fk; f2i = -(ni + fk);
+    }
+    const double pref = 2.0 * M_PI * wtq * g2;
+    so = ((d1 == 0.0 ? 0.0 : d1 * f1o) + (d2 == 0.0 ? 0.0 : d2 * f2o)) * pref;
+    si = ((d2 == 0.0 ? 0.0 : d2 * f1i) + (d1 == 0.0 ? 0.0 : d1 * f2i)) * pref;
+}
+
+// in: (n x 8) rows {ek, ekq, wq, g2, wtq, mu, T, eta}; out: (n x 2) {s_out, s_in}
+__global__ void bte_increments_kernel(int64_t n, int method, int smearing, const double *__restrict__ in,
+                                      double *__restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const double *r = in + 8 * e;
+    double so, si;
+    bte_increments(method, r[0], r[1], r[2], r[3], r[4], r[5], r[6], smearing, r[7], so, si);
+    out[2 * e] = so;
+    out[2 * e + 1] = si;
+}
+
+// One thread per (unit u, kept column c, k+q slot m). i = unit_state_index[u] (initial state);
+// f = state_offset[ikq] + m (final state); energies e_state[i], e_state[f]; modes below w_cut skipped.
+// S_out[i + nstates*iT] += sum_nu s_out (atomic). S_in is compacted: the unit's kept columns are
+// rows rowptr[u] .. rowptr[u] + count[u] - 1, S_in[(row*nbw + m)*nT + iT] = sum_nu s_in, and
+// row_kq[row] keeps the k+q state of the row for the later products.
+__global__ void bte_accumulate_kernel(int nunits, int64_t cs, int nbw, int nm, const int *__restrict__ count,
+                                      const int *__restrict__ col_index, const int *__restrict__ ikq,
+                                      const float *__restrict__ g2, const int64_t *__restrict__ unit_state_index,
+                                      const int64_t *__restrict__ state_offset, const int64_t *__restrict__ bmin,
+                                      const int64_t *__restrict__ bmax, const double *__restrict__ e_state,
+                                      const double *__restrict__ omega, double wtq, int nT,
+                                      const double *__restrict__ mus, const double *__restrict__ Ts,
+                                      const double *__restrict__ etas, int method, int smearing, double w_cut,
+                                      int64_t nstates, const int64_t *__restrict__ rowptr, double *__restrict__ S_out,
+                                      double *__restrict__ S_in, int *__restrict__ row_kq) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int u = blockIdx.y;
+    if (u >= nunits || e >= cs * nbw) return;
+    const int64_t c = e / nbw;
+    const int m = (int)(e % nbw);
+    if (c >= count[u]) return;
+    const int64_t row = (int64_t)u * cs + c;      // payload row
+    const int64_t orow = rowptr[u] + c;           // compacted output row
+    const int64_t kq = ikq[row];
+    if (m == 0) row_kq[orow] = (int)kq;
+    const int nbkq = bmin[kq] > 0 ? (int)(bmax[kq] - bmin[kq] + 1) : 0;
+    double *sin_row = S_in + (orow * nbw + m) * nT;
+    if (m >= nbkq) {
+        for (int t = 0; t < nT; t++) sin_row[t] = 0.0;
+        return;
+    }
+    const int64_t i = unit_state_index[u];
+    const int64_t f = state_offset[kq] + m;
+    const double ek = e_state[i], ekq = e_state[f];
+    const int iq = col_index[row];
+    const float *gr = g2 + row * nbw * nm + (int64_t)m * nm;
+    for (int t = 0; t < nT; t++) {
+        double so = 0.0, si = 0.0;
+        for (int nu = 0; nu < nm; nu++) {
+            const double w = omega[(int64_t)nm * iq + nu];
+            if (w < w_cut) continue;
+            double a, b;
+            bte_increments(method, ek, ekq, w, (double)gr[nu], wtq, mus[t], Ts[t], smearing, etas[t], a, b);
+            so += a;
+            si += b;
+        }
+        atomicAdd(&S_out[i + nstates * t], so);
+        sin_row[t] = si;
+    }
+}
+
+// Diagonal band velocities: v[s][a] = Re(u^H dH_a u) for state s = (k = state_k[s], band = state_band[s]);
+// dH: per state-k column, 3 operators of nw*nw (c64), column index state_col[s].
+__global__ void band_velocity_kernel(int nw, int64_t nstates, const int64_t *__restrict__ state_col,
+                                     const int *__restrict__ state_band, const int64_t *__restrict__ state_k,
+                                     const double2 *__restrict__ dH, int64_t ld_dh,
+                                     const float2 *__restrict__ U, double *__restrict__ v) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nstates * 3) return;
+    const int64_t s = e / 3;
+    const int a = (int)(e % 3);
+    const double2 *M = dH + ld_dh * state_col[s] + (int64_t)a * nw * nw;
+    const float2 *u = U + (int64_t)nw * nw * state_k[s] + (int64_t)nw * state_band[s];
+    double acc = 0.0;
+    for (int j = 0; j < nw; j++) {
+        double wr = 0.0, wi = 0.0;
+        for (int i = 0; i < nw; i++) {
+            const double2 mij = M[i + (int64_t)nw * j];
+            const float2 ui = u[i];
+            // conj(u_i) M_ij
+            wr += ui.x * mij.x + ui.y * mij.y;
+            wi += ui.x * mij.y - ui.y * mij.x;
+        }
+        const float2 uj = u[j];
+        acc += wr * uj.x - wi * uj.y;
+    }
+    v[3 * s + a] = acc;
+}
+
+// y[i, :] = sum over the unit's rows and k+q slots S_in[row, m, iT] x[f, :] (3 components), one
+// wavefront per unit (i = unit_state_index[u]); lanes stride over the (row, slot) entries.
+__global__ void __launch_bounds__(256) bte_spmv_kernel(int nunits, int nbw, int nT, int iT,
+                                                      const int64_t *__restrict__ rowptr,
+                                                      const int *__restrict__ row_kq,
+                                                      const int64_t *__restrict__ unit_state_index,
+                                                      const int64_t *__restrict__ state_offset,
+                                                      const int64_t *__restrict__ bmin,
+                                                      const int64_t *__restrict__ bmax,
+                                                      const double *__restrict__ S_in,
+                                                      const double *__restrict__ x, double *__restrict__ y) {
+    const int lane = threadIdx.x & 63;
+    const int u = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (u >= nunits) return;  // whole wavefront exits together
+    const int64_t r0 = rowptr[u], r1 = rowptr[u + 1];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int64_t e = r0 * nbw + lane; e < r1 * nbw; e += 64) {
+        const int64_t row = e / nbw;
+        const int m = (int)(e % nbw);
+        const int64_t kq = row_kq[row];
+        if (bmin[kq] <= 0 || m > (int)(bmax[kq] - bmin[kq])) continue;
+        const double s = S_in[e * nT + iT];
+        const double *xf = x + 3 * (state_offset[kq] + m);
+        a0 += s * xf[0];
+        a1 += s * xf[1];
+        a2 += s * xf[2];
+    }
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    a2 = wave_sum(a2);
+    if (lane == 0) {
+        double *yi = y + 3 * unit_state_index[u];
+        yi[0] = a0;
+        yi[1] = a1;
+        yi[2] = a2;
+    }
+}
+
+}  // namespace ephip
+
+using namespace ephip;
+
+static inline unsigned nbk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+extern "C" int ephip_bte_increments(int64_t n, int32_t method, int32_t smearing, const double *in, double *out,
+                                    void *stream) {
+    EPHIP_REQUIRE(method >= 1 && method <= 6, "bte_increments: method %d outside 1..6", method);
+    EPHIP_REQUIRE(smearing == 1 || smearing == 2, "bte_increments: smearing must be 1 (Gaussian) or 2 (Lorentzian)");
+    if (n <= 0) return EPHIP_OK;
+    EPHIP_REQUIRE(in && out, "bte_increments: null device pointer");
+    bte_increments_kernel<<<nbk(n), 256, 0, as_stream(stream)>>>(n, method, smearing, in, out);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_bte_accumulate_units(int32_t nunits, int64_t cs, int32_t nbw, int32_t nm, const int32_t *count,
+                                          const int32_t *col_index, const int32_t *ikq, const float *g2,
+                                          const int64_t *unit_state_index, const int64_t *state_offset,
+                                          const int64_t *bmin, const int64_t *bmax, const double *e_state,
+                                          const double *omega, double wtq, int32_t nT, const double *mus,
+                                          const double *Ts, const double *etas, int32_t method, int32_t smearing,
+                                          double w_cut, int64_t nstates, const int64_t *rowptr, double *S_out,
+                                          double *S_in, int32_t *row_kq, void *stream) {
+    EPHIP_REQUIRE(nunits >= 0 && nunits <= 65535 && cs >= 0 && nbw >= 1 && nm >= 1 && nT >= 1,
+                  "bte_accumulate_units: bad sizes");
+    EPHIP_REQUIRE(method >= 1 && method <= 6 && (smearing == 1 || smearing == 2),
+                  "bte_accumulate_units: bad method/smearing");
+    if (nunits == 0 || cs == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(count && col_index && ikq && g2 && unit_state_index && state_offset && bmin && bmax && e_state &&
+                      omega && mus && Ts && etas && rowptr && S_out && S_in && row_kq,
+                  "bte_accumulate_units: null device pointer");
+    dim3 g(nbk(cs * nbw), (unsigned)nunits);
+    bte_accumulate_kernel<<<g, 256, 0, as_stream(stream)>>>(nunits, cs, nbw, nm, count, col_index, ikq, g2,
+                                                            unit_state_index, state_offset, bmin, bmax, e_state, omega,
+                                                            wtq, nT, mus, Ts, etas, method, smearing, w_cut, nstates,
+                                                            rowptr, S_out, S_in, row_kq);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_band_velocity(int32_t nw, int64_t nstates, const int64_t *state_col, const int32_t *state_band,
+                                   const int64_t *state_k, const void *dH, int64_t ld_dh, const void *U, double *v,
+                                   void *stream) {
+    EPHIP_REQUIRE(nw >= 1 && nstates >= 0 && ld_dh >= 3LL * nw * nw, "band_velocity: bad sizes");
+    if (nstates == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(state_col && state_band && state_k && dH && U && v, "band_velocity: null device pointer");
+    band_velocity_kernel<<<nbk(nstates * 3), 256, 0, as_stream(stream)>>>(
+        nw, nstates, state_col, state_band, state_k, static_cast<const double2 *>(dH), ld_dh,
+        static_cast<const float2 *>(U), v);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_bte_spmv(int32_t nunits, int32_t nbw, int32_t nT, int32_t iT, const int64_t *rowptr,
+                              const int32_t *row_kq, const int64_t *unit_state_index, const int64_t *state_offset,
+                              const int64_t *bmin, const int64_t *bmax, const double *S_in, const double *x, double *y,
+                              void *stream) {
+    EPHIP_REQUIRE(nunits >= 0 && nbw >= 1 && nT >= 1 && iT >= 0 && iT < nT, "bte_spmv: bad sizes");
+    if (nunits == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(rowptr && row_kq && unit_state_index && state_offset && bmin && bmax && S_in && x && y,
+                  "bte_spmv: null device pointer");
+    bte_spmv_kernel<<<(unsigned)((nunits + 3) / 4), 256, 0, as_stream(stream)>>>(
+        nunits, nbw, nT, iT, rowptr, row_kq, unit_state_index, state_offset, bmin, bmax, S_in, x, y);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}

@@ -151,12 +151,41 @@ class Model:
                           self.g_decay, dtype or np.complex64)
 
 
+def hermitian_eph(ep: np.ndarray, irvec: np.ndarray, nw: int, nm: int) -> np.ndarray:
+    """Symmetrise a "ph"-layout e-ph tensor (Re and Rp on the same box ``irvec``) so that the vertex is
+    the matrix element of a Hermitian perturbation: g_ij,nu(Re, Rp) = conj(g_ji,nu(-Re, Rp - Re)),
+    which makes g_mn,nu(k, q) = conj(g_nm,nu(k + q, -q)) in the Wannier gauge. Pairs whose partner
+    Rp - Re falls outside the box are set to zero. Returns a new array of ep's dtype."""
+    nr = irvec.shape[0]
+    index = {tuple(r): i for i, r in enumerate(irvec.tolist())}
+    neg = np.array([index[tuple(-x for x in r)] for r in irvec.tolist()])
+    # column layout (numpy C order of the Julia column (i, j, nu, Re)): (Re, nu, j, i)
+    g = ep.reshape(nr, nr, nm, nw, nw)
+    out = np.zeros_like(g)
+    for ip in range(nr):
+        part = [index.get(tuple(irvec[ip] - irvec[ie])) for ie in range(nr)]
+        ok = np.array([p is not None for p in part])
+        ies = np.flatnonzero(ok)
+        ips = np.array([part[ie] for ie in ies], dtype=np.int64)
+        partner = g[ips, neg[ies]]                      # (n, nm, j', i') at (-Re, Rp - Re)
+        out[ip, ies] = 0.5 * (g[ip, ies] + np.conj(np.swapaxes(partner, -1, -2)))
+    return out.reshape(ep.shape)
+
+
 def make_model(nw: int, natoms: int, box_hd: int, box_g: int, seed: int,
                onsite=(-2.0, 2.0), g_dtype=np.complex128, with_g: bool = True,
-               h_decay: float = 1.0, g_decay: float = 0.8) -> Model:
-    """Build a model. ``box_hd`` is the R-box edge of H and D, ``box_g`` that of both g boxes."""
+               h_decay: float = 1.0, g_decay: float = 0.8, hermitian_g: bool = False,
+               time_reversal: bool = False) -> Model:
+    """Build a model. ``box_hd`` is the R-box edge of H and D, ``box_g`` that of both g boxes.
+    ``hermitian_g`` symmetrises the e-ph tensor (``hermitian_eph``), which balances scattering in and
+    out in the linearised BTE. ``time_reversal`` keeps the real parts of H(R) and of the symmetrised
+    g (so E(k) = E(-k), v(-k) = -v(k)); the BTE right-hand side is then odd in k and the fixed-point
+    iteration converges."""
+    hermitian_g = hermitian_g or time_reversal
     irvec_h = rbox(box_hd)
     ham = hermitian_operator(seed, nw, irvec_h, h_decay, onsite, key=1)
+    if time_reversal:
+        ham = ham.real.astype(ham.dtype)
     dyn = dynamical_matrix(seed, natoms, irvec_h).astype(np.complex128)
     nm = 3 * natoms
     irvec_g = rbox(box_g)
@@ -165,6 +194,10 @@ def make_model(nw: int, natoms: int, box_hd: int, box_g: int, seed: int,
         ep = np.empty((irvec_g.shape[0], nw * nw * nm * irvec_g.shape[0]), dtype=g_dtype)
         for ip in range(irvec_g.shape[0]):
             ep[ip] = eph_column(seed, nw, nm, irvec_g, irvec_g[ip], ip, g_decay, g_dtype)
+        if hermitian_g:
+            ep = hermitian_eph(ep, irvec_g, nw, nm)
+        if time_reversal:
+            ep = ep.real.astype(ep.dtype)
     return Model(nw=nw, natoms=natoms, irvec_h=irvec_h, ham_r=ham, irvec_d=irvec_h.copy(),
                  dyn_r=dyn, mass=np.ones(nm), irvec_el=irvec_g, irvec_ep=irvec_g.copy(),
                  epmat_ph=ep, seed=seed, g_decay=g_decay)
@@ -182,8 +215,9 @@ CONFIGS = {
 }
 
 
-def make_config_model(name: str, with_g: bool = True) -> tuple[Model, dict]:
+def make_config_model(name: str, with_g: bool = True, **overrides) -> tuple[Model, dict]:
     c = dict(CONFIGS[name])
+    c.update(overrides)
     grids = {k: c.pop(k) for k in ("kgrid", "qgrid", "window")}
     return make_model(with_g=with_g, **c), grids
 

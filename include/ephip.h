@@ -231,6 +231,45 @@ int ephip_mlp_bf16_prune(int64_t npts, const double *xk, int32_t nf2, const int3
                          const void *W3, const float *b3, float wmin, float wmax, float margin, float *y_out,
                          uint8_t *keep, void *stream);
 
+/* ---- Boltzmann transport on the outer-k payload (FP64) ---- */
+
+/* Scattering increments for a list of n (electron k, electron k+q, phonon mode) triples: in is n rows
+ * {ek, ekq, wq, g2, wtq, mu, T, eta} (double), out is n rows {s_out, s_in}. method 1..6 selects the
+ * occupation convention and smearing 1 = Gaussian, 2 = Lorentzian delta. Replaces the per-triple
+ * bte_scattering_increments (src/boltzmann/bte_scattering_core.jl:33-84). */
+int ephip_bte_increments(int64_t n, int32_t method, int32_t smearing, const double *in, double *out, void *stream);
+
+/* Scattering-out rates and compacted scattering-in rows of one outer-k batch (the device work of
+ * bte_window_accumulate!, src/boltzmann/boltzmann_calculator.jl:336-365). Unit u (initial state
+ * unit_state_index[u]) has count[u] kept q (payload arrays col_index, ikq, g2 of ephip_eph_rotate_kq_g2,
+ * row stride cs, g2 rows of nbw x nm); final state f = state_offset[ikq] + m for k+q window slot m.
+ * For each temperature t: S_out[i + nstates*t] += sum over kept q, m, modes with omega >= w_cut
+ * (atomic), S_in[((rowptr[u] + c)*nbw + m)*nT + t] = sum over modes, row_kq[rowptr[u] + c] = ikq.
+ * e_state: per-state energies; omega (nm x nq); wtq: q weight; mus/Ts/etas: nT per-temperature values. */
+int ephip_bte_accumulate_units(int32_t nunits, int64_t cs, int32_t nbw, int32_t nm, const int32_t *count,
+                               const int32_t *col_index, const int32_t *ikq, const float *g2,
+                               const int64_t *unit_state_index, const int64_t *state_offset, const int64_t *bmin,
+                               const int64_t *bmax, const double *e_state, const double *omega, double wtq,
+                               int32_t nT, const double *mus, const double *Ts, const double *etas, int32_t method,
+                               int32_t smearing, double w_cut, int64_t nstates, const int64_t *rowptr, double *S_out,
+                               double *S_in, int32_t *row_kq, void *stream);
+
+/* Band-diagonal velocities v[3*s + a] = Re(u^H dH_a u) of nstates states (k block state_k[s], band
+ * state_band[s] of the nw x nw complex64 eigenvector blocks U); dH: complex128 operator columns of
+ * ld_dh >= 3 nw^2 entries (direction slowest), column state_col[s]. Replaces the per-k
+ * get_el_velocity_diag_berry_connection! (src/wannier_to_bloch.jl:70-81). */
+int ephip_band_velocity(int32_t nw, int64_t nstates, const int64_t *state_col, const int32_t *state_band,
+                        const int64_t *state_k, const void *dH, int64_t ld_dh, const void *U, double *v,
+                        void *stream);
+
+/* Scattering-in product of the iterative BTE for one temperature: y[3*i + a] = sum over the unit's
+ * rows rowptr[u]..rowptr[u+1]-1 and k+q slots m of S_in[(row*nbw + m)*nT + iT] x[3*f + a],
+ * f = state_offset[row_kq[row]] + m (the scat_mat * map_i_to_f * dF product of _solve_bte,
+ * src/boltzmann/electron_lbte.jl:116-150). */
+int ephip_bte_spmv(int32_t nunits, int32_t nbw, int32_t nT, int32_t iT, const int64_t *rowptr, const int32_t *row_kq,
+                   const int64_t *unit_state_index, const int64_t *state_offset, const int64_t *bmin,
+                   const int64_t *bmax, const double *S_in, const double *x, double *y, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -577,6 +577,45 @@ void ephor_bte_increments(int method, double ek, double ekq, double wq, double g
     out[1] = ((d2 == 0.0 ? 0.0 : d2 * f1i) + (d1 == 0.0 ? 0.0 : d1 * f2i)) * pref;
 }
 
+/* ---------------------------------------------------------------------------------------------
+ * Dense BTE scattering accumulation for the pairs of one q (bte_window_accumulate!,
+ * src/boltzmann/boltzmann_calculator.jl:336-365): g2 (nlist, nm, nbw, nbw) C-order [t][nu][n][m] for
+ * pair t = (ik[t], ikq[t]); initial state i = off[ik] + n, final state f = off[ikq] + m (in-window
+ * slots only); modes with omega_q[nu] < w_cut skipped. For each temperature t:
+ * S_out[t*ns + i] += sum_nu s_out, S_in[(t*ns + i)*ns + f] = sum_nu s_in.
+ * ------------------------------------------------------------------------------------------- */
+void ephor_bte_accumulate_q(int nm, int nbw, int64_t nlist, const int64_t *ik, const int64_t *ikq, const double *g2,
+                            const int64_t *bmin, const int64_t *bmax, const int64_t *off, const double *e_state,
+                            const double *omega_q, double wtq, int nT, const double *mus, const double *Ts,
+                            const double *etas, int method, int smearing, double w_cut, int64_t ns, double *S_out,
+                            double *S_in) {
+    double inc[2];
+    for (int64_t t = 0; t < nlist; t++) {
+        int64_t a = ik[t], b = ikq[t];
+        if (bmin[a] <= 0 || bmin[b] <= 0) continue;
+        int nbk = (int)(bmax[a] - bmin[a] + 1), nbkq = (int)(bmax[b] - bmin[b] + 1);
+        for (int n = 0; n < nbk; n++) {
+            int64_t i = off[a] + n;
+            for (int m = 0; m < nbkq; m++) {
+                int64_t f = off[b] + m;
+                for (int it = 0; it < nT; it++) {
+                    double so = 0.0, si = 0.0;
+                    for (int nu = 0; nu < nm; nu++) {
+                        if (omega_q[nu] < w_cut) continue;
+                        double gv = g2[((t * nm + nu) * nbw + n) * nbw + m];
+                        ephor_bte_increments(method, e_state[i], e_state[f], omega_q[nu], gv, wtq, mus[it], Ts[it],
+                                             smearing, etas[it], inc);
+                        so += inc[0];
+                        si += inc[1];
+                    }
+                    S_out[it * ns + i] += so;
+                    S_in[(it * ns + i) * ns + f] = si;
+                }
+            }
+        }
+    }
+}
+
 int ephor_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

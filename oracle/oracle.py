@@ -56,6 +56,8 @@ def lib():
             "ephor_g2_degenerate_average": [_i32, _i32, _i32, _dp, _dp, _dp, _d],
             "ephor_outer_q_block": [_i32, _i32, _i32, _zp, _ip, _dp, _zp, _lp, _lp, _i64, _lp, _lp,
                                     _i32, _zp],
+            "ephor_bte_accumulate_q": [_i32, _i32, _i64, _lp, _lp, _dp, _lp, _lp, _lp, _dp, _dp, _d, _i32, _dp,
+                                       _dp, _dp, _i32, _i32, _d, _i64, _dp, _dp],
             "ephor_bte_increments": [_i32, _d, _d, _d, _d, _d, _d, _d, _i32, _d, _dp],
             "ephor_num_threads": [],
         }
@@ -223,6 +225,17 @@ def bte_increments(method, ek, ekq, wq, g2v, wtq, mu, T, smearing, eta):
     out = np.empty(2)
     lib().ephor_bte_increments(method, ek, ekq, wq, g2v, wtq, mu, T, smearing, eta, out)
     return out[0], out[1]
+
+
+def bte_accumulate_q(nm, nbw, ik, ikq, g2v, bmin, bmax, off, e_state, omega_q, wtq, mus, Ts, etas, method,
+                     smearing, w_cut, S_out, S_in):
+    """Adds one q's pairs into S_out (nT, ns) and writes S_in (nT, ns, ns) in place (C-contiguous float64)."""
+    assert S_out.flags.c_contiguous and S_in.flags.c_contiguous and S_out.dtype == S_in.dtype == np.float64
+    ns = S_out.shape[1]
+    lib().ephor_bte_accumulate_q(nm, nbw, len(ik), _c(ik, np.int64), _c(ikq, np.int64), _c(g2v, np.float64),
+                                 _c(bmin, np.int64), _c(bmax, np.int64), _c(off, np.int64), _c(e_state, np.float64),
+                                 _c(omega_q, np.float64), wtq, len(mus), _c(mus, np.float64), _c(Ts, np.float64),
+                                 _c(etas, np.float64), method, smearing, w_cut, ns, S_out, S_in)
 
 
 def num_threads() -> int:

@@ -1,0 +1,166 @@
+"""Device Boltzmann transport against the FP64 oracle (oracle/bte_ref.py).
+
+* bte_scattering_increments on the device against the reference's golden vectors (rtol 1e-12);
+* on the cfg1 shape with a time-reversal-symmetric model (nw 8, 2 atoms, 8^3 grids, window +-1 eV,
+  Gaussian smearing 100 meV, occupation convention 5, two temperatures): in-window state set
+  (exact), band velocities, S_out, S_in, and the SERTA and iterative-BTE conductivities, all within
+  1e-5 relative (Frobenius norm) of the oracle; mobility = sigma / n inherits the bound;
+* the compressed scattering-in product against the dense matrix, and independence of the k batching.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import bte_ref as B
+import oracle as O
+from ephip import model as M
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+T_LIST = np.array([0.025852, 0.05])     # 300 K and 580 K in eV
+MU_LIST = np.array([0.0, 0.02])
+ETA = 0.1
+METHOD = 5
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def test_increments_golden(backend):
+    from ephip import _lib as L
+    with open(os.path.join(HERE, "golden", "bte_increments.json")) as f:
+        gold = json.load(f)
+    i = gold["input"]
+    row = [i["ek"], i["ekq"], i["wq"], i["g2"], i["wtq"], i["mu"], i["T"], i["eta"]]
+    u = gold["underflow_case"]
+    urow = [u["ek"], u["ekq"], u["wq"], u["g2"], u["wtq"], u["mu"], u["T"], u["eta"]]
+    dev = backend.torch_device
+    for meth, (so, si) in gold["expected"].items():
+        x = torch.tensor([row, row], dtype=torch.float64, device=dev)
+        out = torch.empty((2, 2), dtype=torch.float64, device=dev)
+        L.call("ephip_bte_increments", 2, int(meth), 1, L.ptr(x), L.ptr(out), L.stream_handle())
+        o = out.cpu().numpy()
+        assert o[0, 0] == pytest.approx(so, rel=gold["rtol"]) and o[0, 1] == pytest.approx(si, rel=gold["rtol"])
+        assert np.array_equal(o[0], o[1])
+    x = torch.tensor([urow], dtype=torch.float64, device=dev)
+    out = torch.empty((1, 2), dtype=torch.float64, device=dev)
+    L.call("ephip_bte_increments", 1, u["method"], 1, L.ptr(x), L.ptr(out), L.stream_handle())
+    assert tuple(out.cpu().numpy()[0]) == tuple(u["expected"])
+    # Lorentzian delta against the oracle on a spread of inputs, every method
+    rng = np.random.default_rng(4)
+    rows = np.column_stack([rng.uniform(-0.2, 0.2, 64), rng.uniform(-0.2, 0.2, 64), rng.uniform(0.01, 0.1, 64),
+                            rng.random(64), np.full(64, 0.01), np.zeros(64), np.full(64, 0.03),
+                            np.full(64, 0.02)])
+    x = torch.from_numpy(rows).to(dev)
+    out = torch.empty((64, 2), dtype=torch.float64, device=dev)
+    for meth in range(1, 7):
+        L.call("ephip_bte_increments", 64, meth, 2, L.ptr(x), L.ptr(out), L.stream_handle())
+        ref = np.array([O.bte_increments(meth, *r[:7], 2, r[7]) for r in rows])
+        assert np.allclose(out.cpu().numpy(), ref, rtol=1e-12, atol=0)
+    with pytest.raises(Exception):
+        L.call("ephip_bte_increments", 1, 7, 1, L.ptr(x), L.ptr(out), L.stream_handle())
+
+
+@pytest.fixture(scope="module")
+def transport(backend):
+    from ephip import _lib as L
+    from ephip import bte, core, eph
+    m, g = M.make_config_model("cfg1", time_reversal=True)
+    w = g["window"]
+    st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
+
+    def run(nk_batch):
+        loop = eph.OuterKLoop(st, nk_batch=nk_batch, precision=L.PREC_FP32)
+        params = bte.ElectronTransportParams(Tlist=T_LIST, nlist=[0.1, 0.1], volume=1.0, mulist=MU_LIST)
+        calc = bte.BoltzmannCalculator(params, smearing=("Gaussian", ETA), occupation_method=METHOD)
+        return bte.run_boltzmann(loop, calc)
+
+    calc = run(64)
+    vel = core.WannierObject(m.irvec_h, bte.velocity_operator(m.irvec_h, m.ham_r).op_r).to_device(
+        backend, dtype=torch.complex128)
+    bte.compute_velocities(vel, st.el, calc.bs)
+    res = bte.solve_electron_bte(calc, max_iter=600, rtol=1e-10)
+    # oracle
+    s = B.states(m, g["kgrid"], (-w, w))
+    v = B.velocities(m, s)
+    So, Si = B.scattering(m, s, g["qgrid"], g["kgrid"], T_LIST, MU_LIST, np.full(2, ETA), method=METHOD)
+    ss, sb, its = B.solve_bte(So, Si, s["es"], v, s["weight"], T_LIST, MU_LIST, max_iter=600, rtol=1e-10)
+    return dict(m=m, g=g, st=st, calc=calc, res=res, s=s, v=v, So=So, Si=Si, ss=ss, sb=sb, its=its, run=run)
+
+
+def test_states_and_velocities(transport):
+    t = transport
+    bs, s = t["calc"].bs, t["s"]
+    assert np.array_equal(bs.ks.cpu().numpy(), s["ks"])
+    assert np.array_equal(bs.ibands.cpu().numpy(), s["ib"])
+    assert np.abs(bs.es.cpu().numpy() - s["es"]).max() <= 1e-6
+    v = bs.vs.cpu().numpy()
+    err = np.abs(v - t["v"]).max() / np.abs(t["v"]).max()
+    print(f"velocity max rel err {err:.2e}")
+    assert err <= 1e-6
+
+
+def test_scattering_rates(transport):
+    t = transport
+    So_d = t["calc"].S_out.cpu().numpy()
+    for it in range(len(T_LIST)):
+        e_out = _rel(So_d[it], t["So"][it])
+        e_in = _rel(t["calc"].scattering_in_dense(it), t["Si"][it])
+        print(f"T={T_LIST[it]}: S_out rel {e_out:.2e}  S_in rel {e_in:.2e}")
+        assert e_out <= 1e-5 and e_in <= 1e-5
+
+
+def test_conductivity_and_mobility(transport):
+    from ephip import bte
+    t = transport
+    res = t["res"]
+    for it in range(len(T_LIST)):
+        es, eb = _rel(res.sigma_serta[it], t["ss"][it]), _rel(res.sigma[it], t["sb"][it])
+        print(f"T={T_LIST[it]}: sigma SERTA rel {es:.2e}, BTE rel {eb:.2e}, iterations {res.niter[it]} "
+              f"(oracle {t['its'][it]})")
+        assert res.converged[it]
+        assert es <= 1e-5 and eb <= 1e-5
+    p = t["calc"].params
+    mob = bte.transport_mobility(res.sigma, p)
+    ref = bte.transport_mobility(t["sb"], p)
+    assert _rel(mob, ref) <= 1e-5
+
+
+def test_spmv_matches_dense(transport):
+    t = transport
+    calc = t["calc"]
+    n = calc.bs.n
+    x = torch.randn((n, 3), dtype=torch.float64, device=calc.S_out.device)
+    y = torch.zeros_like(x)
+    for it in range(len(T_LIST)):
+        calc.spmv(it, x, y)
+        ref = calc.scattering_in_dense(it) @ x.cpu().numpy()
+        assert np.abs(y.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_batching_independent(transport):
+    t = transport
+    other = t["run"](200)
+    a, b = other.S_out.cpu().numpy(), t["calc"].S_out.cpu().numpy()
+    assert _rel(a, b) <= 1e-12
+    assert np.abs(other.scattering_in_dense(0) - t["calc"].scattering_in_dense(0)).max() <= \
+        1e-12 * np.abs(t["Si"][0]).max()
+
+
+def test_mu_from_density(transport, backend):
+    """Chemical potential from the carrier count on the device state energies agrees with the
+    oracle's bisection on the oracle energies."""
+    from ephip import bte
+    t = transport
+    bs = t["calc"].bs
+    s = t["s"]
+    base = s["nbelow"].sum() / len(s["xk"])
+    p = bte.ElectronTransportParams(Tlist=[0.025852], nlist=[2 * (0.5 - base)], volume=1.0)   # 0.5 in the window
+    bte.bte_compute_mu(p, bs)
+    ref = B.find_mu(0.5, 0.025852, s["es"], s["weight"])
+    assert abs(p.mulist[0] - ref) <= 1e-9
