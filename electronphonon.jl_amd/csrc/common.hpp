@@ -50,6 +50,7 @@ namespace ephip {
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 // Make this lane's LDS writes visible to the other lanes of its own wavefront and keep the
 // compiler from moving LDS accesses across the point. No cross-wave synchronisation.

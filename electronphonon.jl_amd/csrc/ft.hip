@@ -40,6 +40,91 @@ constexpr int FT_KC = 16;  // complex R per LDS stage
 constexpr int FT_APAD = 8;
 constexpr int FT_BPAD = 16;
 
+// Separable phases. exp(2 pi i R.x) = prod_a exp(2 pi i R_a x_a): a workgroup tabulates the three
+// per-axis factors for its columns once, for every integer R_a in the range irvec spans on axis a
+// (FP64 argument reduced mod 1, then sincospi), and a phase costs two FP32 complex products (a few
+// ulp) instead of an FP64 dot product and a sincos per (R, column). Used when the three spans add
+// up to at most FT_TAB_MAX rows; otherwise the phases are generated directly.
+constexpr int FT_TAB_MAX = 24;
+
+struct FtAxes {
+    int lo0, lo1, lo2;  // smallest R_a on each axis
+    int b1, b2;         // first table row of axes 1 and 2 (axis 0 starts at row 0)
+    int total;          // table rows in use
+};
+
+// Per-axis range of irvec[0..nr) (nr > 0), computed by the whole 256-thread workgroup.
+__device__ FtAxes ft_axes(const int *__restrict__ irvec, int nr, int *red) {
+    int mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    for (int r = threadIdx.x; r < nr; r += blockDim.x)
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const int v = irvec[3 * r + a];
+            mn[a] = min(mn[a], v);
+            mx[a] = max(mx[a], v);
+        }
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            mn[a] = min(mn[a], __shfl_xor(mn[a], off, 64));
+            mx[a] = max(mx[a], __shfl_xor(mx[a], off, 64));
+        }
+    const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            red[6 * wave + a] = mn[a];
+            red[6 * wave + 3 + a] = mx[a];
+        }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        mn[a] = red[a];
+        mx[a] = red[3 + a];
+        for (int w = 1; w < nw; w++) {
+            mn[a] = min(mn[a], red[6 * w + a]);
+            mx[a] = max(mx[a], red[6 * w + 3 + a]);
+        }
+    }
+    FtAxes ax;
+    ax.lo0 = mn[0];
+    ax.lo1 = mn[1];
+    ax.lo2 = mn[2];
+    ax.b1 = mx[0] - mn[0] + 1;
+    ax.b2 = ax.b1 + mx[1] - mn[1] + 1;
+    ax.total = ax.b2 + mx[2] - mn[2] + 1;
+    return ax;
+}
+
+// tab[row][j] for the workgroup's TN columns (zero for columns past nk).
+template <int TN>
+__device__ void ft_fill_table(float2 (*tab)[TN], const FtAxes &ax, const double *__restrict__ xk,
+                              const int *__restrict__ col_index, int64_t j0, int nk) {
+    for (int e = threadIdx.x; e < ax.total * TN; e += blockDim.x) {
+        const int row = e / TN, j = e % TN;
+        const int a = row < ax.b1 ? 0 : (row < ax.b2 ? 1 : 2);
+        const int ra = a == 0 ? ax.lo0 + row : (a == 1 ? ax.lo1 + row - ax.b1 : ax.lo2 + row - ax.b2);
+        float c = 0.f, s = 0.f;
+        if (j0 + j < nk) {
+            const int64_t pt = col_index ? col_index[j0 + j] : j0 + j;
+            cis2pi((double)ra * xk[3 * pt + a], c, s);
+        }
+        tab[row][j] = make_float2(c, s);
+    }
+}
+
+// Phase of R = (R[0], R[1], R[2]) for table column j (tab: rows of ld columns).
+__device__ __forceinline__ void ft_table_phase(const float2 *tab, int ld, const FtAxes &ax,
+                                               const int *__restrict__ R, int j, float &c, float &s) {
+    const float2 p0 = tab[(R[0] - ax.lo0) * ld + j];
+    const float2 p1 = tab[(ax.b1 + R[1] - ax.lo1) * ld + j];
+    const float2 p2 = tab[(ax.b2 + R[2] - ax.lo2) * ld + j];
+    const float r01 = p0.x * p1.x - p0.y * p1.y, i01 = p0.x * p1.y + p0.y * p1.x;
+    c = r01 * p2.x - i01 * p2.y;
+    s = r01 * p2.y + i01 * p2.x;
+}
+
 // Optional batch of independent contractions, one per grid z-slice u ("unit"): unit u reads its
 // operand at op + u*op_stride, writes at out + u*out_stride, has ncol[u] columns (or nk), and column
 // j uses point xk[col_index[u*col_stride + j]] (or xk[j]). A zero-initialised FtBatch is one plain call.
@@ -60,6 +145,9 @@ __global__ __launch_bounds__(256) void ft_mfma_kernel(int64_t ndata, int nr, con
     constexpr int NCB = TN / 32;  // 16-column blocks per wave
     __shared__ cplx<T> As[FT_KC][FT_TM + FT_APAD];
     __shared__ T Bs[FT_KC][2][TN + FT_BPAD];
+    constexpr bool kSep = sizeof(T) == 4;  // separable phases for FP32 (FP64 keeps direct sincos)
+    __shared__ float2 tab[kSep ? FT_TAB_MAX : 1][kSep ? TN : 1];
+    __shared__ int red[24];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -73,10 +161,18 @@ __global__ __launch_bounds__(256) void ft_mfma_kernel(int64_t ndata, int nr, con
         out += u * bt.out_stride;
         if (bt.col_index) bt.col_index += u * bt.col_stride;
     }
+    bool use_tab = false;
+    FtAxes ax{};
+    if constexpr (kSep) {
+        ax = ft_axes(irvec, nr, red);
+        use_tab = ax.total <= FT_TAB_MAX;
+        if (use_tab) ft_fill_table<TN>(tab, ax, xk, bt.col_index, j0, nk);
+    }
 
-    // The phase column this thread generates is fixed for the whole kernel.
+    // The phase column this thread generates is fixed for the whole kernel; its R rows are uniform
+    // per wavefront (TN is a multiple of 64).
     const int pj = tid % TN;
-    const int prl0 = tid / TN;
+    const int prl0 = __builtin_amdgcn_readfirstlane(tid / TN);
     constexpr int PSTEP = 256 / TN;
     double x0 = 0, x1 = 0, x2 = 0;
     const bool pvalid = (j0 + pj) < nk;
@@ -104,15 +200,25 @@ __global__ __launch_bounds__(256) void ft_mfma_kernel(int64_t ndata, int nr, con
             As[rl][dl] = v;
         }
         // phase tile: FT_KC x TN
-        for (int rl = prl0; rl < FT_KC; rl += PSTEP) {
-            const int r = r0 + rl;
-            T c = 0, s = 0;
-            if (pvalid && r < nr) {
-                const double t = irvec[3 * r] * x0 + irvec[3 * r + 1] * x1 + irvec[3 * r + 2] * x2;
-                cis2pi(t, c, s);
+        if (use_tab) {
+            for (int rl = prl0; rl < FT_KC; rl += PSTEP) {
+                const int r = r0 + rl;
+                float c = 0.f, s = 0.f;
+                if (r < nr) ft_table_phase(&tab[0][0], TN, ax, irvec + 3 * r, pj, c, s);
+                Bs[rl][0][pj] = c;
+                Bs[rl][1][pj] = s;
             }
-            Bs[rl][0][pj] = c;
-            Bs[rl][1][pj] = s;
+        } else {
+            for (int rl = prl0; rl < FT_KC; rl += PSTEP) {
+                const int r = r0 + rl;
+                T c = 0, s = 0;
+                if (pvalid && r < nr) {
+                    const double t = irvec[3 * r] * x0 + irvec[3 * r + 1] * x1 + irvec[3 * r + 2] * x2;
+                    cis2pi(t, c, s);
+                }
+                Bs[rl][0][pj] = c;
+                Bs[rl][1][pj] = s;
+            }
         }
         __syncthreads();
         // The f32 MFMA returns row 4(l/16)+r of the 16x16 block in lane l, register r; the f64 MFMA
@@ -328,6 +434,8 @@ __global__ __launch_bounds__(256) void ft_bf16x3_kernel(int64_t ndata, int nr, c
     __shared__ __attribute__((aligned(16))) __bf16 Alo[2 * FT_TM][KR + 8];
     __shared__ __attribute__((aligned(16))) __bf16 Bhi[TN][KR + 8];
     __shared__ __attribute__((aligned(16))) __bf16 Blo[TN][KR + 8];
+    __shared__ float2 tab[FT_TAB_MAX][TN];
+    __shared__ int red[24];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -341,9 +449,12 @@ __global__ __launch_bounds__(256) void ft_bf16x3_kernel(int64_t ndata, int nr, c
         out += u * bt.out_stride;
         if (bt.col_index) bt.col_index += u * bt.col_stride;
     }
+    const FtAxes ax = ft_axes(irvec, nr, red);
+    const bool use_tab = ax.total <= FT_TAB_MAX;
+    if (use_tab) ft_fill_table<TN>(tab, ax, xk, bt.col_index, j0, nk);
 
     const int pj = tid % TN;
-    const int prl0 = tid / TN;
+    const int prl0 = __builtin_amdgcn_readfirstlane(tid / TN);
     constexpr int PSTEP = 256 / TN;
     double x0 = 0, x1 = 0, x2 = 0;
     const bool pvalid = (j0 + pj) < nk;
@@ -368,27 +479,32 @@ __global__ __launch_bounds__(256) void ft_bf16x3_kernel(int64_t ndata, int nr, c
             const int r = r0 + rl;
             cplx<float> v{0.f, 0.f};
             if (d < ndata && r < nr) v = op[d + ldop * r];
-            __bf16 h, l;
-            split_bf16(v.re, h, l);
-            Ahi[2 * dl][2 * rl] = h;     Alo[2 * dl][2 * rl] = l;
-            Ahi[2 * dl + 1][2 * rl + 1] = h; Alo[2 * dl + 1][2 * rl + 1] = l;
-            split_bf16(v.im, h, l);
-            Ahi[2 * dl + 1][2 * rl] = h; Alo[2 * dl + 1][2 * rl] = l;
-            split_bf16(-v.im, h, l);
-            Ahi[2 * dl][2 * rl + 1] = h; Alo[2 * dl][2 * rl + 1] = l;
+            __bf16 hr, lr, hi, li;
+            split_bf16(v.re, hr, lr);
+            split_bf16(v.im, hi, li);
+            // rows (d, re) = [re, -im] and (d, im) = [im, re] along K = (r, re|im); negation is exact
+            *reinterpret_cast<bf16x2 *>(&Ahi[2 * dl][2 * rl]) = bf16x2{hr, -hi};
+            *reinterpret_cast<bf16x2 *>(&Alo[2 * dl][2 * rl]) = bf16x2{lr, -li};
+            *reinterpret_cast<bf16x2 *>(&Ahi[2 * dl + 1][2 * rl]) = bf16x2{hi, hr};
+            *reinterpret_cast<bf16x2 *>(&Alo[2 * dl + 1][2 * rl]) = bf16x2{li, lr};
         }
         for (int rl = prl0; rl < FT_KC; rl += PSTEP) {
             const int r = r0 + rl;
             float c = 0.f, s = 0.f;
-            if (pvalid && r < nr) {
-                const double t = irvec[3 * r] * x0 + irvec[3 * r + 1] * x1 + irvec[3 * r + 2] * x2;
-                cis2pi(t, c, s);
+            if (r < nr) {
+                if (use_tab) {
+                    ft_table_phase(&tab[0][0], TN, ax, irvec + 3 * r, pj, c, s);
+                } else if (pvalid) {
+                    const double t = irvec[3 * r] * x0 + irvec[3 * r + 1] * x1 + irvec[3 * r + 2] * x2;
+                    cis2pi(t, c, s);
+                }
             }
-            __bf16 h, l;
-            split_bf16(c, h, l);
-            Bhi[pj][2 * rl] = h; Blo[pj][2 * rl] = l;
-            split_bf16(s, h, l);
-            Bhi[pj][2 * rl + 1] = h; Blo[pj][2 * rl + 1] = l;
+            __bf16 hc, lc, hs, ls;
+            split_bf16(c, hc, lc);
+            split_bf16(s, hs, ls);
+            // (cos, sin) pairs are adjacent along K: one 32-bit LDS store per hi and lo
+            *reinterpret_cast<bf16x2 *>(&Bhi[pj][2 * rl]) = bf16x2{hc, hs};
+            *reinterpret_cast<bf16x2 *>(&Blo[pj][2 * rl]) = bf16x2{lc, ls};
         }
         __syncthreads();
         const int kb = 8 * (lane >> 4);

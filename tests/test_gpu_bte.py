@@ -160,7 +160,7 @@ def test_mu_from_density(transport, backend):
     bs = t["calc"].bs
     s = t["s"]
     base = s["nbelow"].sum() / len(s["xk"])
-    p = bte.ElectronTransportParams(Tlist=[0.025852], nlist=[2 * (0.5 - base)], volume=1.0)   # 0.5 in the window
+    p = bte.ElectronTransportParams(Tlist=[0.025852], nlist=[2 * (0.5 + base)], volume=1.0)   # 0.5 in the window
     bte.bte_compute_mu(p, bs)
     ref = B.find_mu(0.5, 0.025852, s["es"], s["weight"])
     assert abs(p.mulist[0] - ref) <= 1e-9

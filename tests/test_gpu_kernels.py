@@ -44,6 +44,23 @@ def test_fourier_parity(backend, ndata, box, nk):
     assert _rel(gotb, ref) < 2e-5, "split-bf16"
 
 
+@pytest.mark.parametrize("shift", [(1, -2, 0), (-3, 0, 4)])
+def test_fourier_offset_rvectors(backend, shift):
+    """R ranges that are not centred on 0 (per-axis phase tables start at the smallest R_a), with
+    k-points far outside the first zone."""
+    from ephip import core
+    from ephip._lib import PREC_BF16X3, PREC_FP32
+    rng = np.random.default_rng(sum(shift) + 50)
+    irvec = M.rbox(5) + np.array(shift, dtype=np.int32)
+    nr, ndata, nk = len(irvec), 90, 150
+    op = rng.standard_normal((nr, ndata)) + 1j * rng.standard_normal((nr, ndata))
+    xk = rng.random((nk, 3)) * 40 - 20
+    ref = O.fourier(op, irvec, xk)
+    obj32 = core.WannierObject(irvec, _dev(op.astype(np.complex64)))
+    assert _rel(core.fourier_batched(obj32, _dev(xk), precision=PREC_FP32).cpu().numpy(), ref) < 2e-6
+    assert _rel(core.fourier_batched(obj32, _dev(xk), precision=PREC_BF16X3).cpu().numpy(), ref) < 2e-5
+
+
 def test_fourier_leading_dimensions(backend):
     from ephip import core
     rng = np.random.default_rng(3)
