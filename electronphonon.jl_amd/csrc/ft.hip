@@ -544,6 +544,191 @@ __global__ __launch_bounds__(256) void ft_bf16x3_kernel(int64_t ndata, int nr, c
         }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Pre-packed split-BF16 form. Both operands are split once into bf16 hi/lo and stored in the order
+// the 16x16x32 MFMA consumes them, so the kernel spends its VALU budget on nothing but addresses:
+//
+//  * operand A (ndata x nr complex, per unit) -> apk, stage-major: for stage s (R = 16 s .. 16 s + 15,
+//    zero past nr) and row d, 64 bf16 at apk[((u*nrs + s)*ndata + d)*64]: re_hi[16] im_hi[16]
+//    re_lo[16] im_lo[16]. A lane's fragment (row l%16, K chunk l/16) is 16 contiguous bytes.
+//  * phases e^{2 pi i R.x_p} of a point table -> bpk: for stage s and point p, 96 bf16 at
+//    bpk[(s*npts + p)*96]: hi = im[16] re[16] -im[16], then lo likewise.
+//
+// With K = (re of 16 R, im of 16 R) the complex product of one stage is two real products,
+//   out_re += [A_re A_im] [P_re; -P_im],   out_im += [A_re A_im] [P_im; P_re],
+// whose B operands are the 32-element windows at offsets 16 and 0 of a point's table block, so A
+// is never duplicated. Each term is hi*hi + hi*lo + lo*hi (three MFMAs, FP32 accumulation).
+// Workgroup: 4 waves as 2 (rows) x 2 (columns); a wave owns RB 16-row blocks x 4 16-column blocks;
+// columns are staged per stage through double-buffered LDS with a 208-byte row pitch (conflict-free
+// 16-byte reads), A fragments go straight from global memory to registers.
+// ---------------------------------------------------------------------------------------------
+constexpr int FTP_TN = 128;   // columns per workgroup
+constexpr int FTP_BLD = 104;  // LDS pitch of one column's 96 bf16 (208 bytes)
+
+template <int RB>
+__global__ __launch_bounds__(256) void ft_bf16p_kernel(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
+                                                       const __bf16 *__restrict__ bpk, int64_t npts, int nk,
+                                                       cplx<float> *__restrict__ out, int64_t ldout, int64_t out_unit,
+                                                       const int *__restrict__ col_index, int64_t col_stride,
+                                                       const int *__restrict__ ncol) {
+    constexpr int TM = 2 * RB * 16;  // complex rows per workgroup
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[2][FTP_TN * FTP_BLD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int c = lane >> 4, il = lane & 15;
+    const int u = blockIdx.z;
+    if (ncol) nk = ncol[u];
+    const int64_t j0 = (int64_t)blockIdx.x * FTP_TN;
+    if (j0 >= nk) return;  // uniform for the block
+    const int64_t d0 = (int64_t)blockIdx.y * TM;
+    apk += (int64_t)u * nrs * ndata * 64;
+    out += u * out_unit;
+    if (col_index) col_index += u * col_stride;
+
+    // B staging: the tile's 128 columns x 12 pieces of 8 bf16 per stage; piece e = tid + 256 t.
+    int64_t bsrc[6];
+    int bdst[6];
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        const int e = tid + 256 * t, jl = e / 12, pc = e % 12;
+        const int64_t j = j0 + jl;
+        bsrc[t] = j < nk ? (int64_t)(col_index ? col_index[j] : j) * 96 + 8 * pc : -1;
+        bdst[t] = jl * FTP_BLD + 8 * pc;
+    }
+    const int64_t bstage = npts * 96;
+    const bf16x8 zero8 = {};
+
+    // A fragment offsets of this lane: row blocks rb, K chunk c (hi at +8c, lo at +32+8c)
+    int64_t aoff[RB];
+    bool aval[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+        const int64_t d = d0 + wm * (RB * 16) + rb * 16 + il;
+        aval[rb] = d < ndata;
+        aoff[rb] = d * 64 + 8 * c;
+    }
+    const int64_t astage = ndata * 64;
+
+    f32x4 accr[RB][4], acci[RB][4];
+#pragma unroll
+    for (int i = 0; i < RB; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) accr[i][j] = acci[i][j] = f32x4{0, 0, 0, 0};
+
+    bf16x8 bq[6];
+#pragma unroll
+    for (int t = 0; t < 6; t++) bq[t] = bsrc[t] >= 0 ? *reinterpret_cast<const bf16x8 *>(bpk + bsrc[t]) : zero8;
+#pragma unroll
+    for (int t = 0; t < 6; t++) *reinterpret_cast<bf16x8 *>(&Bs[0][bdst[t]]) = bq[t];
+    __syncthreads();
+
+    for (int s = 0; s < nrs; s++) {
+        const int cur = s & 1;
+        const bool more = s + 1 < nrs;
+        if (more) {
+#pragma unroll
+            for (int t = 0; t < 6; t++)
+                bq[t] = bsrc[t] >= 0 ? *reinterpret_cast<const bf16x8 *>(bpk + (s + 1) * bstage + bsrc[t]) : zero8;
+        }
+        bf16x8 ah[RB], al[RB];
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+            const __bf16 *a = apk + s * astage + aoff[rb];
+            ah[rb] = aval[rb] ? *reinterpret_cast<const bf16x8 *>(a) : zero8;
+            al[rb] = aval[rb] ? *reinterpret_cast<const bf16x8 *>(a + 32) : zero8;
+        }
+        const __bf16 *bb = Bs[cur];
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++) {
+            const __bf16 *col = bb + (wn * 64 + cb * 16 + il) * FTP_BLD + 8 * c;
+            const bf16x8 b2h = *reinterpret_cast<const bf16x8 *>(col);
+            const bf16x8 b1h = *reinterpret_cast<const bf16x8 *>(col + 16);
+            const bf16x8 b2l = *reinterpret_cast<const bf16x8 *>(col + 48);
+            const bf16x8 b1l = *reinterpret_cast<const bf16x8 *>(col + 64);
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rb], b1h, accr[rb][cb], 0, 0, 0);
+                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b1l, accr[rb][cb], 0, 0, 0);
+                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b1h, accr[rb][cb], 0, 0, 0);
+                acci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rb], b2h, acci[rb][cb], 0, 0, 0);
+                acci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b2l, acci[rb][cb], 0, 0, 0);
+                acci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b2h, acci[rb][cb], 0, 0, 0);
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int t = 0; t < 6; t++) *reinterpret_cast<bf16x8 *>(&Bs[cur ^ 1][bdst[t]]) = bq[t];
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++) {
+            const int64_t j = j0 + wn * 64 + cb * 16 + il;
+            if (j >= nk) continue;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int64_t d = d0 + wm * (RB * 16) + rb * 16 + 4 * c + i;
+                if (d < ndata) out[d + ldout * j] = cplx<float>{accr[rb][cb][i], acci[rb][cb][i]};
+            }
+        }
+}
+
+// One thread per (unit, stage, row): 16 complex values -> 64 bf16 of the packed operand.
+__global__ void ft_pack_operand_kernel(int64_t ndata, int nr, int nrs, const cplx<float> *__restrict__ op,
+                                       int64_t ldop, int64_t op_stride, __bf16 *__restrict__ apk) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int u = blockIdx.y;
+    if (e >= ndata * nrs) return;
+    const int64_t d = e % ndata;
+    const int s = (int)(e / ndata);
+    const cplx<float> *o = op + u * op_stride + d;
+    bf16x8 v[8];
+    __bf16 *w = reinterpret_cast<__bf16 *>(v);
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const int r = 16 * s + k;
+        const cplx<float> x = r < nr ? o[ldop * r] : cplx<float>{0.f, 0.f};
+        split_bf16(x.re, w[k], w[32 + k]);
+        split_bf16(x.im, w[16 + k], w[48 + k]);
+    }
+    bf16x8 *dst = reinterpret_cast<bf16x8 *>(apk + (((int64_t)u * nrs + s) * ndata + d) * 64);
+#pragma unroll
+    for (int k = 0; k < 8; k++) dst[k] = v[k];
+}
+
+// One thread per (stage, point): 16 phases -> 96 bf16 of the packed phase table.
+__global__ void ft_pack_phase_kernel(int nr, const int *__restrict__ irvec, int64_t npts, const double *__restrict__ xk,
+                                     int nrs, __bf16 *__restrict__ bpk) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= npts * nrs) return;
+    const int64_t p = e % npts;
+    const int s = (int)(e / npts);
+    const double x0 = xk[3 * p], x1 = xk[3 * p + 1], x2 = xk[3 * p + 2];
+    bf16x8 v[12];
+    __bf16 *w = reinterpret_cast<__bf16 *>(v);
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const int r = 16 * s + k;
+        float cr = 0.f, si = 0.f;
+        if (r < nr) cis2pi(irvec[3 * r] * x0 + irvec[3 * r + 1] * x1 + irvec[3 * r + 2] * x2, cr, si);
+        __bf16 h, l;
+        split_bf16(si, h, l);
+        w[k] = h;
+        w[48 + k] = l;
+        w[32 + k] = -h;
+        w[80 + k] = -l;
+        split_bf16(cr, h, l);
+        w[16 + k] = h;
+        w[64 + k] = l;
+    }
+    bf16x8 *dst = reinterpret_cast<bf16x8 *>(bpk + ((int64_t)s * npts + p) * 96);
+#pragma unroll
+    for (int k = 0; k < 12; k++) dst[k] = v[k];
+}
+
 template <class T>
 __global__ void phase_kernel(int nr, const int *__restrict__ irvec, int nk, const double *__restrict__ xk,
                              cplx<T> *__restrict__ phase) {
@@ -687,3 +872,61 @@ extern "C" int ephip_ft_set_variant(int32_t kind, int32_t variant) {
 }
 
 extern "C" int ephip_ft_get_variant(int32_t kind) { return (kind >= 0 && kind < 3) ? g_ft_variant[kind] : -1; }
+
+extern "C" int ephip_ft_pack_operand(int64_t ndata, int32_t nr, const void *op, int64_t ldop, int32_t nunits,
+                                     int64_t op_stride, void *apk, void *stream) {
+    EPHIP_REQUIRE(ndata >= 0 && nr >= 0 && nunits >= 0 && nunits <= 65535 && ldop >= ndata && op_stride >= 0,
+                  "ft_pack_operand: bad sizes (ndata=%lld nr=%d nunits=%d)", (long long)ndata, nr, nunits);
+    const int nrs = (nr + 15) / 16;
+    if (ndata == 0 || nrs == 0 || nunits == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(op && apk, "ft_pack_operand: null device pointer");
+    dim3 g((unsigned)((ndata * nrs + 255) / 256), (unsigned)nunits);
+    ft_pack_operand_kernel<<<g, 256, 0, as_stream(stream)>>>(ndata, nr, nrs, static_cast<const cplx<float> *>(op),
+                                                             ldop, op_stride, static_cast<__bf16 *>(apk));
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_ft_pack_phases(int32_t nr, const int32_t *irvec, int64_t npts, const double *xk, void *bpk,
+                                    void *stream) {
+    EPHIP_REQUIRE(nr >= 0 && npts >= 0, "ft_pack_phases: negative size");
+    const int nrs = (nr + 15) / 16;
+    if (nrs == 0 || npts == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(irvec && xk && bpk, "ft_pack_phases: null device pointer");
+    ft_pack_phase_kernel<<<(unsigned)((npts * nrs + 255) / 256), 256, 0, as_stream(stream)>>>(
+        nr, irvec, npts, xk, nrs, static_cast<__bf16 *>(bpk));
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_fourier_packed_bf16x3(int64_t ndata, int32_t nr, const void *apk, int32_t nunits, const void *bpk,
+                                           int64_t npts, int32_t nk_max, const int32_t *ncol, const int32_t *col_index,
+                                           int64_t col_stride, void *out, int64_t ldout, int64_t out_stride,
+                                           void *stream) {
+    EPHIP_REQUIRE(ndata >= 0 && nr >= 0 && nunits >= 0 && nunits <= 65535 && nk_max >= 0 && npts >= 0,
+                  "fourier_packed: bad sizes (ndata=%lld nr=%d nunits=%d nk_max=%d)", (long long)ndata, nr, nunits,
+                  nk_max);
+    EPHIP_REQUIRE(ldout >= ndata && out_stride >= 0, "fourier_packed: leading dimension smaller than ndata");
+    EPHIP_REQUIRE(col_index || nk_max <= npts, "fourier_packed: nk_max=%d exceeds the %lld table points", nk_max,
+                  (long long)npts);
+    EPHIP_REQUIRE(!col_index || col_stride >= nk_max, "fourier_packed: col_stride < nk_max");
+    if (ndata == 0 || nk_max == 0 || nunits == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(nr > 0, "fourier_packed: nr must be positive");
+    EPHIP_REQUIRE(apk && bpk && out, "fourier_packed: null device pointer");
+    const int nrs = (nr + 15) / 16;
+    const bool narrow = ndata <= 64 || (ndata % 128 != 0 && ndata % 64 == 0);
+    const int tm = narrow ? 64 : 128;
+    EPHIP_REQUIRE((ndata + tm - 1) / tm <= 65535, "fourier_packed: ndata=%lld too large for one call", (long long)ndata);
+    dim3 g((unsigned)((nk_max + FTP_TN - 1) / FTP_TN), (unsigned)((ndata + tm - 1) / tm), (unsigned)nunits);
+    auto a = static_cast<const __bf16 *>(apk);
+    auto b = static_cast<const __bf16 *>(bpk);
+    auto o = static_cast<cplx<float> *>(out);
+    if (narrow)
+        ft_bf16p_kernel<2><<<g, 256, 0, as_stream(stream)>>>(ndata, nrs, a, b, npts, nk_max, o, ldout, out_stride,
+                                                             col_index, col_stride, ncol);
+    else
+        ft_bf16p_kernel<4><<<g, 256, 0, as_stream(stream)>>>(ndata, nrs, a, b, npts, nk_max, o, ldout, out_stride,
+                                                             col_index, col_stride, ncol);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}

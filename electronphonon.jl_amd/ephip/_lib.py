@@ -54,6 +54,9 @@ SIGNATURES = {
     "ephip_bte_spmv": [_i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ephip_mlp_bf16_prune": [_i64, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, C.c_float, C.c_float,
                              C.c_float, _vp, _vp, _vp],
+    "ephip_ft_pack_operand": [_i64, _i32, _vp, _i64, _i32, _i64, _vp, _vp],
+    "ephip_ft_pack_phases": [_i32, _vp, _i64, _vp, _vp, _vp],
+    "ephip_fourier_packed_bf16x3": [_i64, _i32, _vp, _i32, _vp, _i64, _i32, _vp, _vp, _i64, _vp, _i64, _i64, _vp],
     "ephip_ft_set_variant": [_i32, _i32],
     "ephip_ft_get_variant": [_i32],
     "ephip_eph_rotate_kq_g2": [_i32, _i32, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp,

@@ -11,6 +11,7 @@ tensor, a stack of Hamiltonians ``(nw, nw, nk)`` is ``(nk, nw, nw)`` holding the
 from __future__ import annotations
 
 import dataclasses
+import math
 
 import numpy as np
 
@@ -178,6 +179,47 @@ def fourier_batched(parent: WannierObject, xk_dev, out=None, precision: int = L.
                nk, L.ptr(xk_dev), L.ptr(out), ldout, L.stream_handle())
     else:
         raise L.EphipError(f"unsupported op_r dtype {op.dtype}")
+    return out
+
+
+def ft_stages(nr: int) -> int:
+    """LDS/MFMA stages of 16 R vectors in the packed split-BF16 layout."""
+    return (int(nr) + 15) // 16
+
+
+def ft_pack_operand(op, ndata: int, nr: int, ldop: int, nunits: int = 1, op_stride: int = 0, out=None):
+    """Split complex64 operand(s) into the packed bf16 layout of ephip_fourier_packed_bf16x3:
+    (nunits, nrs, ndata, 64) bfloat16 (see include/ephip.h)."""
+    torch = _torch()
+    shape = (nunits, ft_stages(nr), ndata, 64)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.bfloat16, device=op.device)
+    elif out.dtype != torch.bfloat16 or out.numel() < math.prod(shape) or not out.is_contiguous():
+        raise L.EphipError("ft_pack_operand: out must be a contiguous bfloat16 buffer of (nunits, nrs, ndata, 64)")
+    L.call("ephip_ft_pack_operand", ndata, nr, L.ptr(op), ldop, nunits, op_stride, L.ptr(out), L.stream_handle())
+    return out
+
+
+def ft_pack_phases(irvec_dev, xk_dev, out=None):
+    """Packed split-BF16 phase table of the points xk_dev: (nrs, npts, 96) bfloat16."""
+    torch = _torch()
+    nr, npts = irvec_dev.shape[0], xk_dev.shape[0]
+    shape = (ft_stages(nr), npts, 96)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.bfloat16, device=xk_dev.device)
+    elif out.dtype != torch.bfloat16 or out.numel() < math.prod(shape) or not out.is_contiguous():
+        raise L.EphipError("ft_pack_phases: out must be a contiguous bfloat16 buffer of (nrs, npts, 96)")
+    if xk_dev.dtype != torch.float64 or not xk_dev.is_contiguous():
+        raise L.EphipError("k-points must be a contiguous float64 (nk, 3) device tensor")
+    L.call("ephip_ft_pack_phases", nr, L.ptr(irvec_dev), npts, L.ptr(xk_dev), L.ptr(out), L.stream_handle())
+    return out
+
+
+def fourier_packed(apk, ndata: int, nr: int, bpk, npts: int, nk_max: int, out, ldout: int, nunits: int = 1,
+                   ncol=None, col_index=None, col_stride: int = 0, out_stride: int = 0):
+    """Split-BF16 Fourier contraction of packed operands against a packed phase table."""
+    L.call("ephip_fourier_packed_bf16x3", ndata, nr, L.ptr(apk), nunits, L.ptr(bpk), npts, nk_max, L.ptr(ncol),
+           L.ptr(col_index), col_stride, L.ptr(out), ldout, out_stride, L.stream_handle())
     return out
 
 

@@ -390,6 +390,17 @@ class OuterKLoop:
                L.stream_handle())
         self.el_obj = _RqObject(self.epmat_el, st.irvec_el_dev, self.nd_el)
         self.irvec_ep_dev = st.epmat.irvec_dev()
+        # split-BF16: both contractions run on pre-packed operands. The "el" tensor is split once
+        # (same bytes as complex64) and its complex64 copy released; the q-grid phase table is built
+        # once; k phases and unit operands are packed per batch.
+        self.packed = precision == L.PREC_BF16X3
+        if self.packed:
+            self.apk_el = core.ft_pack_operand(self.epmat_el, self.nd_el, self.nre, self.nd_el)
+            self.epmat_el = None
+            self.el_obj = None
+            self.bpk_q = core.ft_pack_phases(self.irvec_ep_dev, st.ph.xq)
+            self.bpk_k = torch.empty(core.ft_stages(self.nre) * self.nk_batch * 96, dtype=torch.bfloat16,
+                                     device=dev)
         # units of every surviving k, in k order
         bmin = st.el.filt.band_min.cpu().numpy()
         bmax = st.el.filt.band_max.cpu().numpy()
@@ -414,6 +425,9 @@ class OuterKLoop:
         self.count = torch.empty(max_units, dtype=torch.int32, device=dev)
         self.Gq = torch.empty((max_units, self.cs, nd2), dtype=torch.complex64, device=dev)
         self.g2 = torch.empty((max_units, self.cs, st.nbw * nm), dtype=torch.float32, device=dev)
+        if self.packed:
+            self.apk_units = torch.empty(max_units * core.ft_stages(self.nrp) * nd2 * 64, dtype=torch.bfloat16,
+                                         device=dev)
         self.uph = st.ph.u.contiguous()
         self.omega = st.ph.omega.contiguous()
         self.ft_timer = None   # optional list of (event, event, label, flops) around the two Fourier launches
@@ -441,8 +455,14 @@ class OuterKLoop:
         stream = L.stream_handle()
         # 1. RR->kR Fourier over Re for the batch
         xk_b = self.xk_surv[i0:i0 + n]
-        self._timed("RR->kR", 8.0 * self.nd_el * self.nre * n,
-                    lambda: core.fourier_batched(self.el_obj, xk_b, out=self.G[:n], precision=self.precision))
+        if self.packed:
+            def rrkr():
+                bpk = core.ft_pack_phases(st.irvec_el_dev, xk_b, out=self.bpk_k)
+                core.fourier_packed(self.apk_el, self.nd_el, self.nre, bpk, n, n, self.G, self.G.stride(0))
+            self._timed("RR->kR", 8.0 * self.nd_el * self.nre * n, rrkr)
+        else:
+            self._timed("RR->kR", 8.0 * self.nd_el * self.nre * n,
+                        lambda: core.fourier_batched(self.el_obj, xk_b, out=self.G[:n], precision=self.precision))
         # projection; the unit's column in G is its k position minus i0 (base pointer shifted by -i0 columns)
         g_base = L.ptr(self.G) - i0 * self.G.stride(0) * 8
         L.call("ephip_eph_project_kR", nw, nm, self.nrp, nu, g_base, self.G.stride(0), L.ptr(self.unit_kpos[u0:u1]),
@@ -454,10 +474,18 @@ class OuterKLoop:
                L.ptr(self.count), stream)
         # 3. kR->kq batched Fourier over Rp at the kept q
         nd2 = nw * nm
-        self._timed("kR->kq", None, lambda: L.call(
-            "ephip_fourier_batched_c32", nd2, self.nrp, L.ptr(self.A), nd2, nd2 * self.nrp, L.ptr(self.irvec_ep_dev),
-            L.ptr(st.ph.xq), nu, self.nq, L.ptr(self.count), L.ptr(self.col_index), self.cs, L.ptr(self.Gq), nd2,
-            self.cs * nd2, self.precision, stream))
+        if self.packed:
+            def krkq():
+                core.ft_pack_operand(self.A, nd2, self.nrp, nd2, nu, nd2 * self.nrp, out=self.apk_units)
+                core.fourier_packed(self.apk_units, nd2, self.nrp, self.bpk_q, self.nq, self.cs, self.Gq, nd2,
+                                    nunits=nu, ncol=self.count, col_index=self.col_index, col_stride=self.cs,
+                                    out_stride=self.cs * nd2)
+            self._timed("kR->kq", None, krkq)
+        else:
+            self._timed("kR->kq", None, lambda: L.call(
+                "ephip_fourier_batched_c32", nd2, self.nrp, L.ptr(self.A), nd2, nd2 * self.nrp,
+                L.ptr(self.irvec_ep_dev), L.ptr(st.ph.xq), nu, self.nq, L.ptr(self.count), L.ptr(self.col_index),
+                self.cs, L.ptr(self.Gq), nd2, self.cs * nd2, self.precision, stream))
         # 4. rotations + g2
         L.call("ephip_eph_rotate_kq_g2", nw, nm, st.nbw, nu, self.cs, L.ptr(self.count), L.ptr(self.col_index),
                L.ptr(self.ikq), L.ptr(self.Gq), nd2, self.cs * nd2, L.ptr(st.el.U), L.ptr(st.el.filt.band_min),

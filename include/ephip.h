@@ -179,6 +179,24 @@ int ephip_eph_rotate_kq_g2(int32_t nw, int32_t nm, int32_t nbw, int32_t nunits, 
                            int64_t gstride, const void *U, const int64_t *bmin, const int64_t *bmax, const void *uph,
                            const double *omega, float *g2, void *ep, void *stream);
 
+/* Pre-packed split-BF16 Fourier (no counterpart in the reference; the split-BF16 form of the ZGEMM of
+ * src/wannier/batched_fourier_core.jl:117-139 with both operands split once):
+ *  - ephip_ft_pack_operand: nunits complex64 operands (ndata x nr, leading dimension ldop, unit u at
+ *    op + u*op_stride) -> apk, bf16, nunits*nrs*ndata*64 values, nrs = ceil(nr/16): for stage s and row
+ *    d, 64 values at ((u*nrs + s)*ndata + d)*64 = re_hi[16] im_hi[16] re_lo[16] im_lo[16] of R 16s..16s+15
+ *    (zero past nr).
+ *  - ephip_ft_pack_phases: exp(2 pi i R.x_p) for npts points -> bpk, bf16, nrs*npts*96 values: for stage s
+ *    and point p, 96 values at (s*npts + p)*96 = hi{im[16], re[16], -im[16]} lo{...}.
+ *  - ephip_fourier_packed_bf16x3: out[d + ldout*j] (+ u*out_stride) = sum_r op_u[d, r] e^{2 pi i R_r.x_p},
+ *    p = col_index[u*col_stride + j] (or j, then nk_max <= npts), j < ncol[u] (or nk_max); FP32
+ *    accumulation of hi*hi + hi*lo + lo*hi on 16x16x32 bf16 MFMAs. */
+int ephip_ft_pack_operand(int64_t ndata, int32_t nr, const void *op, int64_t ldop, int32_t nunits, int64_t op_stride,
+                          void *apk, void *stream);
+int ephip_ft_pack_phases(int32_t nr, const int32_t *irvec, int64_t npts, const double *xk, void *bpk, void *stream);
+int ephip_fourier_packed_bf16x3(int64_t ndata, int32_t nr, const void *apk, int32_t nunits, const void *bpk,
+                                int64_t npts, int32_t nk_max, const int32_t *ncol, const int32_t *col_index,
+                                int64_t col_stride, void *out, int64_t ldout, int64_t out_stride, void *stream);
+
 /* Kernel variant of the Fourier contractions, chosen by the tuner (ephip/tune.py) from timings on the
  * live shapes. kind: 0 = FP32 with nk > 64, 1 = FP32 with nk <= 64, 2 = FP64. variant: 0 = single-
  * buffered LDS stages of 16 R, 1 = double-buffered stages of 16 R, 2 = double-buffered stages of 8 R.

@@ -278,3 +278,63 @@ def test_fourier_batched_units(backend, prec_name):
             ref = O.fourier(op[u], irvec, xk[cols[u, :c]])
             assert _rel(o[u, :c], ref) < tol
         assert np.all(o[u, c:] == 3.0)
+
+
+@pytest.mark.parametrize("ndata,box,nk,shift", [(1, 3, 1, (0, 0, 0)), (77, 5, 130, (0, 0, 0)),
+                                                (192, 7, 257, (1, -2, 0)), (1000, 7, 64, (0, 0, 0)),
+                                                (261, 5, 129, (-3, 0, 4))])
+def test_fourier_packed_parity(backend, ndata, box, nk, shift):
+    """Pre-packed split-BF16 contraction (both row-tile widths, ragged rows/columns, R count not a
+    multiple of 16, R ranges off centre) against the FP64 oracle."""
+    from ephip import core
+    rng = np.random.default_rng(ndata + nk + box)
+    irvec = M.rbox(box) + np.array(shift, dtype=np.int32)
+    nr = len(irvec)
+    decay = np.exp(-np.linalg.norm(irvec, axis=1) / 1.5)[:, None]
+    op = (rng.standard_normal((nr, ndata)) + 1j * rng.standard_normal((nr, ndata))) * decay
+    xk = rng.random((nk, 3)) * 4 - 2
+    ref = O.fourier(op, irvec, xk)
+    apk = core.ft_pack_operand(_dev(op.astype(np.complex64)), ndata, nr, ndata)
+    bpk = core.ft_pack_phases(_dev(irvec), _dev(xk))
+    out = torch.full((nk, ndata + 3), 5.0 + 0j, dtype=torch.complex64, device="cuda")
+    core.fourier_packed(apk, ndata, nr, bpk, nk, nk, out, ndata + 3)
+    o = out.cpu().numpy()
+    assert _rel(o[:, :ndata], ref) < 2e-5
+    assert np.all(o[:, ndata:] == 5.0)
+
+
+def test_fourier_packed_units(backend):
+    """Packed batched entry: per-unit operands, columns gathered from a point table, per-unit counts;
+    columns past a unit's count untouched."""
+    from ephip import core
+    rng = np.random.default_rng(61)
+    irvec = M.rbox(5)
+    nr, nd, nu, npts, cs = len(irvec), 192, 5, 300, 260
+    op = rng.standard_normal((nu, nr, nd)) + 1j * rng.standard_normal((nu, nr, nd))
+    xk = rng.random((npts, 3))
+    counts = np.array([260, 0, 1, 129, 77], np.int32)
+    cols = np.zeros((nu, cs), np.int32)
+    for u in range(nu):
+        cols[u, :counts[u]] = np.sort(rng.choice(npts, counts[u], replace=False))
+    out = torch.full((nu, cs, nd), 3.0 + 0j, dtype=torch.complex64, device="cuda")
+    apk = core.ft_pack_operand(_dev(op.astype(np.complex64)), nd, nr, nd, nunits=nu, op_stride=nr * nd)
+    bpk = core.ft_pack_phases(_dev(irvec), _dev(xk))
+    core.fourier_packed(apk, nd, nr, bpk, npts, cs, out, nd, nunits=nu, ncol=_dev(counts), col_index=_dev(cols),
+                        col_stride=cs, out_stride=cs * nd)
+    o = out.cpu().numpy()
+    for u in range(nu):
+        c = counts[u]
+        if c:
+            assert _rel(o[u, :c], O.fourier(op[u], irvec, xk[cols[u, :c]])) < 2e-5
+        assert np.all(o[u, c:] == 3.0)
+
+
+def test_fourier_packed_argument_checks(backend):
+    from ephip import _lib as L
+    x = torch.zeros(16, device="cuda")
+    with pytest.raises(L.EphipError):   # more columns than table points without a column list
+        L.call("ephip_fourier_packed_bf16x3", 4, 3, L.ptr(x), 1, L.ptr(x), 2, 3, None, None, 0, L.ptr(x), 4, 0,
+               L.stream_handle())
+    with pytest.raises(L.EphipError):   # ldout < ndata
+        L.call("ephip_fourier_packed_bf16x3", 4, 3, L.ptr(x), 1, L.ptr(x), 8, 3, None, None, 0, L.ptr(x), 3, 0,
+               L.stream_handle())
