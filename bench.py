@@ -4,13 +4,19 @@ Workload (the single-GPU configuration the metric is quoted on): seeded syntheti
 nw = 16 Wannier functions, 4 atoms (nmodes = 12), H(R)/D(R) on a 9^3 R-box, g(Re, Rp) on 7^3 x 7^3
 boxes stored complex64 (2.89 GB), fine k-grid 48^3, q-grid 24^3, window E_F +- 0.3 eV.
 
-A step is one tile of ``--qtile`` q-points through the outer-q hot path: RR->Rq for the tile, then for
-every q and every k-batch of the window-surviving k: Fourier over Re on the matrix cores, k+q state
-lookup, windowed rotations and |g|^2. Setup (model upload, fine-grid states, phonons) is outside the
-timed region; inputs are resident in HBM when timing starts.
+Two paths are timed (``--path both``, the default):
+* plain outer-q, FP32 (the reference's shape: separate Fourier and rotation kernels): a step is one
+  tile of ``--qtile`` q-points over all window-surviving k;
+* fast outer-k (the value): k-side window projection, kept-q compaction, batched Fourier over Rp and
+  the k+q/phonon rotations; a step is one batch of ``--kbatch-k`` surviving k against every q, in
+  split-BF16 (default) or FP32 Fourier precision; the Fourier kernel variant is tuned on the live batch
+  first (``--no-tune`` to skip).
+Setup (model upload, fine-grid states, phonons, operand packing) is outside the timed region; inputs
+are resident in HBM when timing starts.
 
-value = |K_surv| * q-points done / time, where K_surv are the k with at least one band in the window
-by the exact filter (the same numerator for every path).
+value = pairs done / time with pairs = |K_surv| x |Q| per q (plain) or per k (fast), K_surv the k with
+at least one band in the window by the exact filter (the same numerator for every path).
+``--transport`` runs the transport measurement instead (one JSON line of its own).
 """
 from __future__ import annotations
 
@@ -86,6 +92,50 @@ def cpu_baseline(O, m, st, epmat_c64, iq, nk_sample, kgrid):
             "seconds": t2 - t0}
 
 
+def transport_bench(args):
+    """Electron transport at the workload's grids on the time-reversal-symmetric variant of the model
+    (real H(R), Hermitian real g): one full pass of the outer-k e-ph loop feeding the Boltzmann
+    calculator, velocities, SERTA and iterative BTE, in FP32 and split-BF16 Fourier precision.
+    Reports the conductivities, their relative difference and the timings (one JSON line)."""
+    import torch
+
+    from ephip import _lib as L
+    from ephip import bte, core
+    from ephip import model as M
+    t0 = time.perf_counter()
+    backend = core.gpu_backend(0)
+    m, grids = M.make_config_model(args.config, time_reversal=True)
+    log(f"[transport] time-reversal model {args.config} built in {time.perf_counter() - t0:.1f} s")
+    w = grids["window"]
+    epmat_dev = core.WannierObject(m.irvec_ep, torch.from_numpy(m.epmat_ph).to(backend.torch_device), check=False)
+    from ephip import eph
+    st = eph.setup_outer_q(m, grids["kgrid"], grids["qgrid"], (-w, w), backend, epmat_dev=epmat_dev)
+    T = 0.025852  # 300 K
+    out = {}
+    for name, prec in (("fp32", L.PREC_FP32), ("bf16x3", L.PREC_BF16X3)):
+        params = bte.ElectronTransportParams(Tlist=[T], nlist=[1.0], volume=1.0, mulist=[0.0])
+        run = bte.run_transport(m, grids["kgrid"], grids["qgrid"], (-w, w), params, backend, precision=prec,
+                                smearing=("Gaussian", args.transport_eta), occupation_method=5,
+                                nk_batch=args.kbatch_k, max_iter=args.transport_iter, rtol=1e-8, setup=st)
+        r = run.result
+        out[name] = {"sigma_serta": r.sigma_serta[0].tolist(), "sigma": r.sigma[0].tolist(),
+                     "niter": r.niter[0], "converged": r.converged[0], "seconds": run.seconds,
+                     "n_states": run.calc.bs.n,
+                     "s_in_rows": int(sum(b.row_kq.shape[0] for b in run.calc.blocks))}
+        log(f"[transport] {name}: sigma_serta diag {np.diag(r.sigma_serta[0])}, sigma diag {np.diag(r.sigma[0])}, "
+            f"{r.niter[0]} iterations, times {run.seconds}")
+        del run
+        torch.cuda.empty_cache()
+    rel = lambda a, b: float(np.linalg.norm(np.array(a) - np.array(b)) / np.linalg.norm(np.array(b)))
+    res = {"metric": "transport: conductivity at 300 K, 10 meV Gaussian, method 5 (split-BF16 vs FP32 Fourier)",
+           "config": {"workload": f"{args.config} time-reversal variant, k {grids['kgrid']}, q {grids['qgrid']}, "
+                                  f"window +-{w} eV", "eta_eV": args.transport_eta, "T_eV": T},
+           "rel_diff_sigma_serta": rel(out["bf16x3"]["sigma_serta"], out["fp32"]["sigma_serta"]),
+           "rel_diff_sigma": rel(out["bf16x3"]["sigma"], out["fp32"]["sigma"]),
+           "detail": out}
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,17 +145,28 @@ def main():
     ap.add_argument("--kbatch", type=int, default=16384, help="k batch of the plain outer-q path")
     ap.add_argument("--kbatch-k", type=int, default=128, help="k batch (one step) of the fast outer-k path")
     ap.add_argument("--path", choices=["fast", "plain", "both"], default="both")
-    ap.add_argument("--precision", choices=["fp32", "bf16x3"], default="fp32")
+    ap.add_argument("--precision", choices=["fp32", "bf16x3"], default="bf16x3",
+                    help="Fourier precision of the fast path (split-BF16 meets the 1e-4 |g|^2 bound)")
     ap.add_argument("--cpu-sample", type=int, default=3000, help="k-points in the CPU baseline slice (0 = skip)")
     ap.add_argument("--config", default="cfg2")
-    ap.add_argument("--tune", action="store_true", help="autotune the Fourier kernel variant on the live batch first")
+    ap.add_argument("--tune", dest="tune", action="store_true", default=True,
+                    help="autotune the Fourier kernel variant on the live batch first (default)")
+    ap.add_argument("--no-tune", dest="tune", action="store_false")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 256 MiB device-to-device copy kernel (counter calibration)")
+    ap.add_argument("--transport", action="store_true",
+                    help="run the transport measurement (conductivity, FP32 vs split-BF16) instead of the e-ph bench")
+    ap.add_argument("--transport-eta", type=float, default=0.01, help="Gaussian smearing of the transport run (eV)")
+    ap.add_argument("--transport-iter", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                                    "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
         sys.exit(2)
+
+    if args.transport:
+        transport_bench(args)
+        return
 
     import torch
 

@@ -397,7 +397,7 @@ __global__ __launch_bounds__(256) void ft_mfma_db_kernel(int64_t ndata, int nr, 
 // Kernel variant per contraction kind, set by the tuner (ephip_ft_set_variant). Kinds:
 // 0 = FP32 wide (nk > 64), 1 = FP32 narrow (nk <= 64), 2 = FP64. Variants: 0 = single-buffered,
 // 1 = double-buffered KC 16, 2 = double-buffered KC 8.
-static int g_ft_variant[3] = {0, 0, 0};
+static int g_ft_variant[4] = {0, 0, 0, 0};
 
 template <class T, int TN>
 static void launch_ft(int variant, dim3 g, hipStream_t s, int64_t ndata, int nr, const cplx<T> *op, int64_t ldop,
@@ -562,118 +562,327 @@ __global__ __launch_bounds__(256) void ft_bf16x3_kernel(int64_t ndata, int nr, c
 // columns are staged per stage through double-buffered LDS with a 208-byte row pitch (conflict-free
 // 16-byte reads), A fragments go straight from global memory to registers.
 // ---------------------------------------------------------------------------------------------
-constexpr int FTP_TN = 128;   // columns per workgroup
 constexpr int FTP_BLD = 104;  // LDS pitch of one column's 96 bf16 (208 bytes)
 
-template <int RB>
-__global__ __launch_bounds__(256) void ft_bf16p_kernel(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
-                                                       const __bf16 *__restrict__ bpk, int64_t npts, int nk,
-                                                       cplx<float> *__restrict__ out, int64_t ldout, int64_t out_unit,
-                                                       const int *__restrict__ col_index, int64_t col_stride,
-                                                       const int *__restrict__ ncol) {
-    constexpr int TM = 2 * RB * 16;  // complex rows per workgroup
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[2][FTP_TN * FTP_BLD];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
-    const int c = lane >> 4, il = lane & 15;
-    const int u = blockIdx.z;
-    if (ncol) nk = ncol[u];
-    const int64_t j0 = (int64_t)blockIdx.x * FTP_TN;
-    if (j0 >= nk) return;  // uniform for the block
-    const int64_t d0 = (int64_t)blockIdx.y * TM;
-    apk += (int64_t)u * nrs * ndata * 64;
-    out += u * out_unit;
-    if (col_index) col_index += u * col_stride;
+// Workgroup of 4 waves as WR (rows) x 4/WR (columns); a wave owns RB 16-row blocks x CB 16-column
+// blocks. Rows past ndata and columns past the unit's count read a valid row/column (clamped) and
+// are not stored, so no load is predicated. Offsets are 32-bit against per-stage uniform bases.
+// PF: the next stage's A fragments are loaded into a second register set before the current
+// stage's MFMAs (costs 8 RB registers; without it the loads are issued at the top of each stage).
+template <int RB, int CB, int WR>
+struct FtpShape {
+    static constexpr int WC = 4 / WR;
+    static constexpr int TM = WR * RB * 16;       // complex rows per workgroup
+    static constexpr int TN = WC * CB * 16;       // columns per workgroup
+    static constexpr int PPT = TN * 12 / 256;     // 16-byte phase pieces per thread and stage
+    static_assert(TN * 12 % 256 == 0, "phase pieces must split evenly over the workgroup");
+};
 
-    // B staging: the tile's 128 columns x 12 pieces of 8 bf16 per stage; piece e = tid + 256 t.
-    int64_t bsrc[6];
-    int bdst[6];
+// Main loop: accumulates the workgroup tile (rows d0.., columns j0..) of one unit into accr/acci.
+// apk and col_index point at the unit's operand and column list; Bs is 2 x TN x FTP_BLD bf16 of LDS.
+template <int RB, int CB, int WR, bool PF>
+__device__ __forceinline__ void ft_bf16p_mainloop(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
+                                                  const __bf16 *__restrict__ bpk, int64_t npts, int nk,
+                                                  const int *__restrict__ col_index, int64_t j0, int64_t d0,
+                                                  __bf16 *Bs, f32x4 (&accr)[RB][CB], f32x4 (&acci)[RB][CB]) {
+    using S = FtpShape<RB, CB, WR>;
+    constexpr int TN = S::TN, PPT = S::PPT;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave % WR, wc = wave / WR;
+    const int c = lane >> 4, il = lane & 15;
+
+    uint32_t bsrc[PPT], bdst[PPT];
 #pragma unroll
-    for (int t = 0; t < 6; t++) {
+    for (int t = 0; t < PPT; t++) {
         const int e = tid + 256 * t, jl = e / 12, pc = e % 12;
-        const int64_t j = j0 + jl;
-        bsrc[t] = j < nk ? (int64_t)(col_index ? col_index[j] : j) * 96 + 8 * pc : -1;
+        const int64_t j = min(j0 + jl, (int64_t)nk - 1);
+        bsrc[t] = (uint32_t)(col_index ? col_index[j] : j) * 96u + 8u * pc;
         bdst[t] = jl * FTP_BLD + 8 * pc;
     }
     const int64_t bstage = npts * 96;
-    const bf16x8 zero8 = {};
-
-    // A fragment offsets of this lane: row blocks rb, K chunk c (hi at +8c, lo at +32+8c)
-    int64_t aoff[RB];
-    bool aval[RB];
+    uint32_t aoff[RB];
 #pragma unroll
     for (int rb = 0; rb < RB; rb++) {
-        const int64_t d = d0 + wm * (RB * 16) + rb * 16 + il;
-        aval[rb] = d < ndata;
-        aoff[rb] = d * 64 + 8 * c;
+        const int64_t d = min(d0 + wr * (RB * 16) + rb * 16 + il, ndata - 1);
+        aoff[rb] = (uint32_t)d * 64u + 8u * c;
     }
     const int64_t astage = ndata * 64;
 
-    f32x4 accr[RB][4], acci[RB][4];
 #pragma unroll
     for (int i = 0; i < RB; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) accr[i][j] = acci[i][j] = f32x4{0, 0, 0, 0};
+        for (int j = 0; j < CB; j++) accr[i][j] = acci[i][j] = f32x4{0, 0, 0, 0};
 
-    bf16x8 bq[6];
+    bf16x8 bq[PPT], ah[RB], al[RB];
 #pragma unroll
-    for (int t = 0; t < 6; t++) bq[t] = bsrc[t] >= 0 ? *reinterpret_cast<const bf16x8 *>(bpk + bsrc[t]) : zero8;
+    for (int t = 0; t < PPT; t++) bq[t] = *reinterpret_cast<const bf16x8 *>(bpk + bsrc[t]);
 #pragma unroll
-    for (int t = 0; t < 6; t++) *reinterpret_cast<bf16x8 *>(&Bs[0][bdst[t]]) = bq[t];
+    for (int t = 0; t < PPT; t++) *reinterpret_cast<bf16x8 *>(&Bs[bdst[t]]) = bq[t];
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+        ah[rb] = *reinterpret_cast<const bf16x8 *>(apk + aoff[rb]);
+        al[rb] = *reinterpret_cast<const bf16x8 *>(apk + aoff[rb] + 32);
+    }
     __syncthreads();
 
     for (int s = 0; s < nrs; s++) {
         const int cur = s & 1;
         const bool more = s + 1 < nrs;
+        bf16x8 nh[RB], nl[RB];
         if (more) {
+            const __bf16 *bg = bpk + (s + 1) * bstage;
 #pragma unroll
-            for (int t = 0; t < 6; t++)
-                bq[t] = bsrc[t] >= 0 ? *reinterpret_cast<const bf16x8 *>(bpk + (s + 1) * bstage + bsrc[t]) : zero8;
+            for (int t = 0; t < PPT; t++) bq[t] = *reinterpret_cast<const bf16x8 *>(bg + bsrc[t]);
+            if constexpr (PF) {
+                const __bf16 *ag = apk + (s + 1) * astage;
+#pragma unroll
+                for (int rb = 0; rb < RB; rb++) {
+                    nh[rb] = *reinterpret_cast<const bf16x8 *>(ag + aoff[rb]);
+                    nl[rb] = *reinterpret_cast<const bf16x8 *>(ag + aoff[rb] + 32);
+                }
+            }
         }
-        bf16x8 ah[RB], al[RB];
+        if constexpr (!PF) {
+            if (s > 0) {
+                const __bf16 *ag = apk + s * astage;
 #pragma unroll
-        for (int rb = 0; rb < RB; rb++) {
-            const __bf16 *a = apk + s * astage + aoff[rb];
-            ah[rb] = aval[rb] ? *reinterpret_cast<const bf16x8 *>(a) : zero8;
-            al[rb] = aval[rb] ? *reinterpret_cast<const bf16x8 *>(a + 32) : zero8;
+                for (int rb = 0; rb < RB; rb++) {
+                    ah[rb] = *reinterpret_cast<const bf16x8 *>(ag + aoff[rb]);
+                    al[rb] = *reinterpret_cast<const bf16x8 *>(ag + aoff[rb] + 32);
+                }
+            }
         }
-        const __bf16 *bb = Bs[cur];
+        const __bf16 *bb = Bs + cur * (TN * FTP_BLD);
 #pragma unroll
-        for (int cb = 0; cb < 4; cb++) {
-            const __bf16 *col = bb + (wn * 64 + cb * 16 + il) * FTP_BLD + 8 * c;
+        for (int cb = 0; cb < CB; cb++) {
+            const __bf16 *col = bb + (wc * (CB * 16) + cb * 16 + il) * FTP_BLD + 8 * c;
             const bf16x8 b2h = *reinterpret_cast<const bf16x8 *>(col);
             const bf16x8 b1h = *reinterpret_cast<const bf16x8 *>(col + 16);
             const bf16x8 b2l = *reinterpret_cast<const bf16x8 *>(col + 48);
             const bf16x8 b1l = *reinterpret_cast<const bf16x8 *>(col + 64);
+            // small terms first; consecutive MFMAs update different accumulators
 #pragma unroll
             for (int rb = 0; rb < RB; rb++) {
                 accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rb], b1h, accr[rb][cb], 0, 0, 0);
-                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b1l, accr[rb][cb], 0, 0, 0);
-                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b1h, accr[rb][cb], 0, 0, 0);
                 acci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rb], b2h, acci[rb][cb], 0, 0, 0);
+            }
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b1l, accr[rb][cb], 0, 0, 0);
                 acci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b2l, acci[rb][cb], 0, 0, 0);
+            }
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b1h, accr[rb][cb], 0, 0, 0);
                 acci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b2h, acci[rb][cb], 0, 0, 0);
             }
         }
         if (more) {
 #pragma unroll
-            for (int t = 0; t < 6; t++) *reinterpret_cast<bf16x8 *>(&Bs[cur ^ 1][bdst[t]]) = bq[t];
+            for (int t = 0; t < PPT; t++) *reinterpret_cast<bf16x8 *>(&Bs[(cur ^ 1) * (TN * FTP_BLD) + bdst[t]]) = bq[t];
+            if constexpr (PF) {
+#pragma unroll
+                for (int rb = 0; rb < RB; rb++) {
+                    ah[rb] = nh[rb];
+                    al[rb] = nl[rb];
+                }
+            }
         }
         __syncthreads();
     }
+}
 
+// XCD grouping (XG): workgroups are dispatched round-robin over the 8 XCDs in linear order (x
+// fastest). Linear id 8 gx a + 8 b + r is given row tile 8 a + r and column tile b, so the gx column
+// tiles of one row tile run on the same XCD and share its L2 copy of the operand rows; the rows past
+// the last full group of 8 keep the plain order.
+__device__ __forceinline__ void xcd_grouped_tile(int64_t &bx, int64_t &by) {
+    const int64_t gx = gridDim.x, gy = gridDim.y;
+    const int64_t l = blockIdx.x + gx * blockIdx.y;
+    const int64_t full = 8 * gx * (gy / 8);
+    if (l < full) {
+        const int64_t a = l / (8 * gx), rem = l % (8 * gx);
+        by = 8 * a + rem % 8;
+        bx = rem / 8;
+    } else {
+        bx = l % gx;
+        by = l / gx;
+    }
+}
+
+template <int RB, int CB, int WR, bool PF, bool XG = false>
+__device__ __forceinline__ void ft_bf16p_body(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
+                                              const __bf16 *__restrict__ bpk, int64_t npts, int nk,
+                                              cplx<float> *__restrict__ out, int64_t ldout, int64_t out_unit,
+                                              const int *__restrict__ col_index, int64_t col_stride,
+                                              const int *__restrict__ ncol) {
+    using S = FtpShape<RB, CB, WR>;
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[2 * S::TN * FTP_BLD];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave % WR, wc = wave / WR;
+    const int c = lane >> 4, il = lane & 15;
+    const int u = blockIdx.z;
+    if (ncol) nk = ncol[u];
+    int64_t bx = blockIdx.x, by = blockIdx.y;
+    if constexpr (XG) xcd_grouped_tile(bx, by);
+    const int64_t j0 = bx * S::TN;
+    if (j0 >= nk) return;  // uniform for the block
+    const int64_t d0 = by * S::TM;
+    f32x4 accr[RB][CB], acci[RB][CB];
+    ft_bf16p_mainloop<RB, CB, WR, PF>(ndata, nrs, apk + (int64_t)u * nrs * ndata * 64, bpk, npts, nk,
+                                      col_index ? col_index + u * col_stride : nullptr, j0, d0, Bs, accr, acci);
+    out += u * out_unit;
 #pragma unroll
     for (int rb = 0; rb < RB; rb++)
 #pragma unroll
-        for (int cb = 0; cb < 4; cb++) {
-            const int64_t j = j0 + wn * 64 + cb * 16 + il;
+        for (int cb = 0; cb < CB; cb++) {
+            const int64_t j = j0 + wc * (CB * 16) + cb * 16 + il;
             if (j >= nk) continue;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const int64_t d = d0 + wm * (RB * 16) + rb * 16 + 4 * c + i;
+                const int64_t d = d0 + wr * (RB * 16) + rb * 16 + 4 * c + i;
                 if (d < ndata) out[d + ldout * j] = cplx<float>{accr[rb][cb][i], acci[rb][cb][i]};
             }
         }
+}
+
+// Two resident waves per SIMD (at most 256 registers) for the shapes that fit without spilling; the
+// 128 x 128 prefetching shape needs more registers and runs one wave per SIMD.
+template <int RB, int CB, int WR, bool PF, bool XG = false>
+__global__ __launch_bounds__(256, 2) void ft_bf16p_kernel(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
+                                                          const __bf16 *__restrict__ bpk, int64_t npts, int nk,
+                                                          cplx<float> *__restrict__ out, int64_t ldout,
+                                                          int64_t out_unit, const int *__restrict__ col_index,
+                                                          int64_t col_stride, const int *__restrict__ ncol) {
+    ft_bf16p_body<RB, CB, WR, PF, XG>(ndata, nrs, apk, bpk, npts, nk, out, ldout, out_unit, col_index, col_stride,
+                                      ncol);
+}
+
+__global__ __launch_bounds__(256) void ft_bf16p_wide_pf_kernel(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
+                                                               const __bf16 *__restrict__ bpk, int64_t npts, int nk,
+                                                               cplx<float> *__restrict__ out, int64_t ldout,
+                                                               int64_t out_unit, const int *__restrict__ col_index,
+                                                               int64_t col_stride, const int *__restrict__ ncol) {
+    ft_bf16p_body<4, 4, 2, true>(ndata, nrs, apk, bpk, npts, nk, out, ldout, out_unit, col_index, col_stride, ncol);
+}
+
+// ---------------------------------------------------------------------------------------------
+// kR->kq Fourier over Rp fused with the k+q-window and phonon rotations and g2, for the e-ph unit
+// shape nw = 16, nm = 12 (operand rows d = i + 16 nu): the 192 x 64 tile puts modes 3w..3w+2 of all
+// 16 Wannier rows of 64 kept columns in wave w. Per column, t[m][nu] = sum_i conj(U_kq[i, mlo+m])
+// g[i + 16 nu] is a 4-lane shuffle reduction of the accumulators, each wave's share of
+// ep[m][nu'] = sum_nu t[m][nu] u_ph(q)[nu + 12 nu'] is added into LDS in wave order (deterministic),
+// and g2[m][nu'] = |ep|^2 / (2 omega_q,nu') is the only output (no g(k,q) round trip through memory).
+// ---------------------------------------------------------------------------------------------
+constexpr int FTR_NBW_MAX = 4;
+
+template <int NBW>
+__global__ __launch_bounds__(256, 2) void ft_bf16p_rot_kernel(int nrs, const __bf16 *__restrict__ apk,
+                                                              const __bf16 *__restrict__ bpk, int64_t npts,
+                                                              const int *__restrict__ ncol,
+                                                              const int *__restrict__ col_index,
+                                                              const int *__restrict__ ikq, int64_t cs,
+                                                              const cplx<float> *__restrict__ U,
+                                                              const int64_t *__restrict__ bmin,
+                                                              const int64_t *__restrict__ bmax,
+                                                              const cplx<float> *__restrict__ uph,
+                                                              const double *__restrict__ omega, int nbw,
+                                                              float *__restrict__ g2) {
+    constexpr int RB = 3, CB = 4, WR = 4, NW = 16, NM = 12, TN = 64;
+    constexpr int64_t NDATA = NW * NM;
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[2 * TN * FTP_BLD];
+    static_assert(sizeof(float) * 2 * TN * NBW * NM <= sizeof(Bs), "ep tile must fit in the staging buffer");
+    const int lane = threadIdx.x & 63, wr = threadIdx.x >> 6;
+    const int c = lane >> 4, il = lane & 15;
+    const int u = blockIdx.z;
+    const int nk = ncol[u];
+    const int64_t j0 = (int64_t)blockIdx.x * TN;
+    if (j0 >= nk) return;  // uniform for the block
+    f32x4 accr[RB][CB], acci[RB][CB];
+    ft_bf16p_mainloop<RB, CB, WR, true>(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk,
+                                        col_index + u * cs, j0, 0, Bs, accr, acci);
+    // after the main loop's final barrier the staging buffer is free: ep tile [jj][m][nu'] (re, im)
+    float *EP = reinterpret_cast<float *>(Bs);
+
+    // this wave's share of ep, one column block at a time: lane (c, il) owns nu' = c, c+4, c+8 of
+    // column cb*16 + il; the four waves' shares are added into LDS in wave order
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++) {
+        const int64_t j = j0 + cb * 16 + il;
+        const bool jv = j < nk;
+        const int64_t row = (int64_t)u * cs + (jv ? j : j0);
+        const int64_t kq = ikq[row];
+        const int iq = col_index[row];
+        int nbkq = bmin[kq] > 0 ? (int)(bmax[kq] - bmin[kq] + 1) : 0;
+        nbkq = jv ? min(nbkq, nbw) : 0;
+        const int mlo = bmin[kq] > 0 ? (int)bmin[kq] - 1 : 0;
+        const cplx<float> *uk = U + (int64_t)NW * NW * kq;
+        const cplx<float> *up = uph + (int64_t)NM * NM * iq;
+        float pr[NBW][3], pi[NBW][3];
+#pragma unroll
+        for (int m = 0; m < NBW; m++) {
+            // w = U_kq[4c + r, mlo + m]
+            cplx<float> w[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                w[r] = m < nbkq ? uk[4 * c + r + NW * (mlo + m)] : cplx<float>{0.f, 0.f};
+            float tr[RB], ti[RB];
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                float sr = 0.f, si = 0.f;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const float ar = accr[rb][cb][r], ai = acci[rb][cb][r];
+                    sr += w[r].re * ar + w[r].im * ai;
+                    si += w[r].re * ai - w[r].im * ar;
+                }
+                sr += __shfl_xor(sr, 16, 64);
+                si += __shfl_xor(si, 16, 64);
+                sr += __shfl_xor(sr, 32, 64);
+                si += __shfl_xor(si, 32, 64);
+                tr[rb] = sr;
+                ti[rb] = si;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int nu2 = c + 4 * k;
+                float er = 0.f, ei = 0.f;
+#pragma unroll
+                for (int rb = 0; rb < RB; rb++) {
+                    const cplx<float> p = up[(3 * wr + rb) + NM * nu2];
+                    er += tr[rb] * p.re - ti[rb] * p.im;
+                    ei += tr[rb] * p.im + ti[rb] * p.re;
+                }
+                pr[m][k] = er;
+                pi[m][k] = ei;
+            }
+        }
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            if (wr == w) {
+#pragma unroll
+                for (int m = 0; m < NBW; m++)
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        const int o = 2 * (((cb * 16 + il) * NBW + m) * NM + c + 4 * k);
+                        EP[o] = (w == 0 ? 0.f : EP[o]) + pr[m][k];
+                        EP[o + 1] = (w == 0 ? 0.f : EP[o + 1]) + pi[m][k];
+                    }
+            }
+            __syncthreads();
+        }
+    }
+    // g2 for the tile's valid columns, layout [m][nu'] per kept (unit, column), slots m < nbw
+    for (int e = threadIdx.x; e < TN * nbw * NM; e += 256) {
+        const int jj = e / (nbw * NM), rr = e % (nbw * NM), m = rr / NM, nu2 = rr % NM;
+        const int64_t j = j0 + jj;
+        if (j >= nk) continue;
+        const int64_t row = (int64_t)u * cs + j;
+        const int o = 2 * ((jj * NBW + m) * NM + nu2);
+        const float er = EP[o], ei = EP[o + 1];
+        const float w2 = 2.f * static_cast<float>(omega[(int64_t)NM * col_index[row] + nu2]);
+        g2[row * nbw * NM + rr] = (er * er + ei * ei) / w2;
+    }
 }
 
 // One thread per (unit, stage, row): 16 complex values -> 64 bf16 of the packed operand.
@@ -865,13 +1074,14 @@ extern "C" int ephip_fourier_batched_c32(int64_t ndata, int32_t nr, const void *
 }
 
 extern "C" int ephip_ft_set_variant(int32_t kind, int32_t variant) {
-    EPHIP_REQUIRE(kind >= 0 && kind < 3, "ft_set_variant: kind %d outside 0..2", kind);
-    EPHIP_REQUIRE(variant >= 0 && variant < 3, "ft_set_variant: variant %d outside 0..2", variant);
+    EPHIP_REQUIRE(kind >= 0 && kind < 4, "ft_set_variant: kind %d outside 0..3", kind);
+    EPHIP_REQUIRE(variant >= 0 && variant < (kind == 3 ? 4 : 3), "ft_set_variant: variant %d outside the range of kind %d",
+                  variant, kind);
     g_ft_variant[kind] = variant;
     return EPHIP_OK;
 }
 
-extern "C" int ephip_ft_get_variant(int32_t kind) { return (kind >= 0 && kind < 3) ? g_ft_variant[kind] : -1; }
+extern "C" int ephip_ft_get_variant(int32_t kind) { return (kind >= 0 && kind < 4) ? g_ft_variant[kind] : -1; }
 
 extern "C" int ephip_ft_pack_operand(int64_t ndata, int32_t nr, const void *op, int64_t ldop, int32_t nunits,
                                      int64_t op_stride, void *apk, void *stream) {
@@ -914,19 +1124,72 @@ extern "C" int ephip_fourier_packed_bf16x3(int64_t ndata, int32_t nr, const void
     EPHIP_REQUIRE(nr > 0, "fourier_packed: nr must be positive");
     EPHIP_REQUIRE(apk && bpk && out, "fourier_packed: null device pointer");
     const int nrs = (nr + 15) / 16;
-    const bool narrow = ndata <= 64 || (ndata % 128 != 0 && ndata % 64 == 0);
-    const int tm = narrow ? 64 : 128;
+    EPHIP_REQUIRE(ndata * 64 < (1LL << 32) && npts * 96 < (1LL << 32),
+                  "fourier_packed: operand or table too large for 32-bit offsets");
+    // Tile shape: 192 x 64 for 192-row operands (one e-ph unit); for operands of 128 rows or more the
+    // tuner's variant of kind 3 (0: 128 x 128, 1: 128 x 128 with A prefetch, 2: 192 x 64 when
+    // ndata % 192 == 0, 3: 192 x 64 in XCD-grouped order, one unit only); otherwise 64 x 128.
+    int shape;
+    const int var = g_ft_variant[3];
+    if (ndata % 192 == 0 && (nk_max <= 64 || ndata % 128 != 0))
+        shape = 2;
+    else if (ndata >= 128)
+        shape = ((var == 2 || var == 3) && ndata % 192 == 0) ? 2 : (var == 1 ? 3 : 1);
+    else
+        shape = 0;
+    const bool grouped = shape == 2 && var == 3 && nunits == 1;
+    const int tm = shape == 2 ? 192 : (shape == 0 ? 64 : 128), tn = shape == 2 ? 64 : 128;
     EPHIP_REQUIRE((ndata + tm - 1) / tm <= 65535, "fourier_packed: ndata=%lld too large for one call", (long long)ndata);
-    dim3 g((unsigned)((nk_max + FTP_TN - 1) / FTP_TN), (unsigned)((ndata + tm - 1) / tm), (unsigned)nunits);
+    dim3 g((unsigned)((nk_max + tn - 1) / tn), (unsigned)((ndata + tm - 1) / tm), (unsigned)nunits);
     auto a = static_cast<const __bf16 *>(apk);
     auto b = static_cast<const __bf16 *>(bpk);
     auto o = static_cast<cplx<float> *>(out);
-    if (narrow)
-        ft_bf16p_kernel<2><<<g, 256, 0, as_stream(stream)>>>(ndata, nrs, a, b, npts, nk_max, o, ldout, out_stride,
-                                                             col_index, col_stride, ncol);
+    hipStream_t st = as_stream(stream);
+    if (grouped)
+        ft_bf16p_kernel<3, 4, 4, true, true><<<g, 256, 0, st>>>(ndata, nrs, a, b, npts, nk_max, o, ldout, out_stride,
+                                                                 col_index, col_stride, ncol);
+    else if (shape == 2)
+        ft_bf16p_kernel<3, 4, 4, true><<<g, 256, 0, st>>>(ndata, nrs, a, b, npts, nk_max, o, ldout, out_stride,
+                                                           col_index, col_stride, ncol);
+    else if (shape == 3)
+        ft_bf16p_wide_pf_kernel<<<g, 256, 0, st>>>(ndata, nrs, a, b, npts, nk_max, o, ldout, out_stride, col_index,
+                                                    col_stride, ncol);
+    else if (shape == 1)
+        ft_bf16p_kernel<4, 4, 2, false><<<g, 256, 0, st>>>(ndata, nrs, a, b, npts, nk_max, o, ldout, out_stride,
+                                                            col_index, col_stride, ncol);
     else
-        ft_bf16p_kernel<4><<<g, 256, 0, as_stream(stream)>>>(ndata, nrs, a, b, npts, nk_max, o, ldout, out_stride,
-                                                             col_index, col_stride, ncol);
+        ft_bf16p_kernel<2, 4, 2, true><<<g, 256, 0, st>>>(ndata, nrs, a, b, npts, nk_max, o, ldout, out_stride,
+                                                           col_index, col_stride, ncol);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t nbw, int32_t nr, const void *apk,
+                                                 int32_t nunits, const void *bpk, int64_t npts, int64_t cs,
+                                                 const int32_t *count, const int32_t *col_index, const int32_t *ikq,
+                                                 const void *U, const int64_t *bmin, const int64_t *bmax,
+                                                 const void *uph, const double *omega, float *g2, void *stream) {
+    EPHIP_REQUIRE(nw == 16 && nm == 12, "eph_kq_rotate_packed: fused form needs nw = 16 and nm = 12 (got %d, %d)", nw,
+                  nm);
+    EPHIP_REQUIRE(nbw >= 1 && nbw <= FTR_NBW_MAX, "eph_kq_rotate_packed: nbw=%d outside 1..%d", nbw, FTR_NBW_MAX);
+    EPHIP_REQUIRE(nr > 0 && nunits >= 0 && nunits <= 65535 && cs >= 0 && npts > 0, "eph_kq_rotate_packed: bad sizes");
+    EPHIP_REQUIRE(npts * 96 < (1LL << 32), "eph_kq_rotate_packed: phase table too large for 32-bit offsets");
+    if (nunits == 0 || cs == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(apk && bpk && count && col_index && ikq && U && bmin && bmax && uph && omega && g2,
+                  "eph_kq_rotate_packed: null device pointer");
+    const int nrs = (nr + 15) / 16;
+    dim3 g((unsigned)((cs + 63) / 64), 1, (unsigned)nunits);
+    auto a = static_cast<const __bf16 *>(apk);
+    auto b = static_cast<const __bf16 *>(bpk);
+    auto uu = static_cast<const cplx<float> *>(U);
+    auto up = static_cast<const cplx<float> *>(uph);
+    hipStream_t st = as_stream(stream);
+    if (nbw <= 2)
+        ft_bf16p_rot_kernel<2><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax, up, omega,
+                                                  nbw, g2);
+    else
+        ft_bf16p_rot_kernel<4><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax, up, omega,
+                                                  nbw, g2);
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

@@ -152,10 +152,13 @@ __global__ __launch_bounds__(256) void rotate_kq_g2_kernel(int nw, int nm, int n
 // Same contraction, 16 kept columns of one unit per 256-thread workgroup with every operand staged in
 // LDS: the columns' Fourier outputs (contiguous in memory, one coalesced sweep), their k+q window
 // eigenvector columns, phonon eigenvectors and frequencies. Then t = U_kq^H g and ep = t u_ph from LDS.
+// The per-mode blocks of g and u_ph are stored with one complex of padding (pitch nw + 1, nm + 1) so
+// that lanes reading different modes hit different LDS banks.
 constexpr int RKQ_COLS = 16;
 
 __host__ __device__ inline size_t rkq_lds_bytes(int nw, int nm, int nbw) {
-    return sizeof(c32) * (size_t)RKQ_COLS * ((size_t)nw * nm + (size_t)nm * nm + (size_t)nbw * nw + (size_t)nbw * nm) +
+    return sizeof(c32) * (size_t)RKQ_COLS *
+               ((size_t)(nw + 1) * nm + (size_t)nm * (nm + 1) + (size_t)nbw * nw + (size_t)nbw * nm) +
            sizeof(float) * RKQ_COLS * nm + sizeof(int) * RKQ_COLS * 2;
 }
 
@@ -178,9 +181,10 @@ __global__ __launch_bounds__(256) void rotate_kq_g2_lds_kernel(int nw, int nm, i
     const int ncol = (int)min<int64_t>(RKQ_COLS, cnt - c0);
     const int tid = threadIdx.x;
     const int nd = nw * nm;
-    c32 *Gs = reinterpret_cast<c32 *>(rkq_lds);            // [c][nd]
-    c32 *Ps = Gs + RKQ_COLS * nd;                           // [c][nm*nm]
-    c32 *Us = Ps + RKQ_COLS * nm * nm;                      // [c][m][i]
+    const int ndp = (nw + 1) * nm, npp = (nm + 1) * nm;     // padded per-column sizes
+    c32 *Gs = reinterpret_cast<c32 *>(rkq_lds);            // [c][nu][i], pitch nw + 1
+    c32 *Ps = Gs + RKQ_COLS * ndp;                          // [c][nu'][nu], pitch nm + 1
+    c32 *Us = Ps + RKQ_COLS * npp;                          // [c][m][i]
     c32 *Ts = Us + RKQ_COLS * nbw * nw;                     // [c][m][nu]
     float *Ws = reinterpret_cast<float *>(Ts + RKQ_COLS * nbw * nm);  // [c][nu] omega
     int *Ms = reinterpret_cast<int *>(Ws + RKQ_COLS * nm);             // [c] mlo, [c] nbkq
@@ -202,11 +206,11 @@ __global__ __launch_bounds__(256) void rotate_kq_g2_lds_kernel(int nw, int nm, i
     const c32 *gsrc = Gq + gstride * u + ldg * c0;
     for (int e = tid; e < ncol * nd; e += 256) {
         const int c = e / nd, d = e % nd;
-        Gs[e] = gsrc[ldg * c + d];
+        Gs[c * ndp + (nw + 1) * (d / nw) + d % nw] = gsrc[ldg * c + d];
     }
     for (int e = tid; e < ncol * nm * nm; e += 256) {
         const int c = e / (nm * nm), r = e % (nm * nm);
-        Ps[e] = uph[(int64_t)nm * nm * col_index[row0 + c] + r];
+        Ps[c * npp + (nm + 1) * (r / nm) + r % nm] = uph[(int64_t)nm * nm * col_index[row0 + c] + r];
     }
     for (int e = tid; e < ncol * nm; e += 256) {
         const int c = e / nm, r = e % nm;
@@ -225,7 +229,7 @@ __global__ __launch_bounds__(256) void rotate_kq_g2_lds_kernel(int nw, int nm, i
         const int c = e / (nbw * nm), r = e % (nbw * nm), m = r / nm, nu = r % nm;
         float sr = 0.f, si = 0.f;
         const c32 *uu = Us + (c * nbw + m) * nw;
-        const c32 *gg = Gs + c * nd + nw * nu;
+        const c32 *gg = Gs + c * ndp + (nw + 1) * nu;
         for (int i = 0; i < nw; i++) {
             const c32 w = uu[i], a = gg[i];
             sr += w.re * a.re + w.im * a.im;
@@ -240,7 +244,7 @@ __global__ __launch_bounds__(256) void rotate_kq_g2_lds_kernel(int nw, int nm, i
         float er = 0.f, ei = 0.f;
         if (m < Ms[RKQ_COLS + c]) {
             const c32 *tt = Ts + (c * nbw + m) * nm;
-            const c32 *pp = Ps + c * nm * nm + nm * nu2;
+            const c32 *pp = Ps + c * npp + (nm + 1) * nu2;
             for (int nu = 0; nu < nm; nu++) {
                 const c32 t = tt[nu], w = pp[nu];
                 er += t.re * w.re - t.im * w.im;

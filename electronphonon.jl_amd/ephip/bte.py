@@ -376,3 +376,50 @@ def transport_mobility(sigma: np.ndarray, params: ElectronTransportParams) -> np
     carrier density n / V per cell volume)."""
     dens = np.abs(params.nlist) / params.volume
     return sigma / dens[:, None, None]
+
+
+@dataclasses.dataclass
+class TransportRun:
+    result: BTEResult
+    mobility_serta: np.ndarray
+    mobility: np.ndarray
+    calc: BoltzmannCalculator
+    setup: object
+    seconds: dict
+
+
+def run_transport(model, kgrid, qgrid, window, params: ElectronTransportParams, backend,
+                  precision: int = L.PREC_FP32, smearing=("Gaussian", 0.01), occupation_method: int = 5,
+                  nk_batch: int | None = 128, max_iter: int = 100, rtol: float = 1e-10, epmat_dev=None,
+                  setup=None) -> TransportRun:
+    """Transport on one GPU, uniform grids, no symmetry: fine-grid states and window filter, phonons,
+    the outer-k e-ph loop feeding a BoltzmannCalculator, band velocities, then SERTA and the
+    iterative BTE (the flow of run_eph_over_k_and_kq with a BoltzmannCalculator followed by
+    solve_electron_bte, src/calculator/run_eph_over_k_and_kq.jl:584-895,
+    src/boltzmann/electron_lbte.jl:153-190). ``setup`` reuses an OuterQSetup of the same grids."""
+    import time
+    from . import eph
+    torch = _torch()
+    t = {}
+    t0 = time.perf_counter()
+    st = setup if setup is not None else eph.setup_outer_q(model, kgrid, qgrid, window, backend,
+                                                           epmat_dev=epmat_dev)
+    core.synchronize(backend)
+    t["setup"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loop = eph.OuterKLoop(st, nk_batch=nk_batch, precision=precision)
+    calc = BoltzmannCalculator(params, smearing=smearing, occupation_method=occupation_method)
+    run_boltzmann(loop, calc)
+    core.synchronize(backend)
+    t["eph_and_scattering"] = time.perf_counter() - t0
+    del loop
+    t0 = time.perf_counter()
+    vel = velocity_operator(model.irvec_h, model.ham_r).to_device(backend, dtype=torch.complex128)
+    compute_velocities(vel, st.el, calc.bs)
+    core.synchronize(backend)
+    t["velocities"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res = solve_electron_bte(calc, max_iter=max_iter, rtol=rtol)
+    t["solve"] = time.perf_counter() - t0
+    return TransportRun(res, transport_mobility(res.sigma_serta, params), transport_mobility(res.sigma, params),
+                        calc, st, t)

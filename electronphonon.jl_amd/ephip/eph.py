@@ -369,7 +369,7 @@ class OuterKLoop:
     Pairs (k, q) with no in-window band at k+q are exactly zero and are not computed."""
 
     def __init__(self, st: OuterQSetup, nk_batch: int | None = 128, precision: int = L.PREC_FP32,
-                 nk_batch_cap: int = 512):
+                 nk_batch_cap: int = 512, fuse_rotation: bool = True):
         torch = _torch()
         self.st = st
         if nk_batch is None:  # memory-adaptive width (plan_batch), the e-ph tensor already resident
@@ -394,6 +394,9 @@ class OuterKLoop:
         # (same bytes as complex64) and its complex64 copy released; the q-grid phase table is built
         # once; k phases and unit operands are packed per batch.
         self.packed = precision == L.PREC_BF16X3
+        # split-BF16 with nw = 16, nm = 12, nbw <= 4: kR->kq and the rotations run as one kernel that
+        # writes g2 only (no g(k, q) staging buffer)
+        self.fused = bool(fuse_rotation) and self.packed and nw == 16 and nm == 12 and st.nbw <= 4
         if self.packed:
             self.apk_el = core.ft_pack_operand(self.epmat_el, self.nd_el, self.nre, self.nd_el)
             self.epmat_el = None
@@ -423,7 +426,7 @@ class OuterKLoop:
         self.col_index = torch.empty((max_units, self.cs), dtype=torch.int32, device=dev)
         self.ikq = torch.empty((max_units, self.cs), dtype=torch.int32, device=dev)
         self.count = torch.empty(max_units, dtype=torch.int32, device=dev)
-        self.Gq = torch.empty((max_units, self.cs, nd2), dtype=torch.complex64, device=dev)
+        self.Gq = None if self.fused else torch.empty((max_units, self.cs, nd2), dtype=torch.complex64, device=dev)
         self.g2 = torch.empty((max_units, self.cs, st.nbw * nm), dtype=torch.float32, device=dev)
         if self.packed:
             self.apk_units = torch.empty(max_units * core.ft_stages(self.nrp) * nd2 * 64, dtype=torch.bfloat16,
@@ -474,7 +477,15 @@ class OuterKLoop:
                L.ptr(self.count), stream)
         # 3. kR->kq batched Fourier over Rp at the kept q
         nd2 = nw * nm
-        if self.packed:
+        if self.fused:
+            def krkq_rot():
+                core.ft_pack_operand(self.A, nd2, self.nrp, nd2, nu, nd2 * self.nrp, out=self.apk_units)
+                L.call("ephip_eph_kq_rotate_packed_bf16x3", nw, nm, st.nbw, self.nrp, L.ptr(self.apk_units), nu,
+                       L.ptr(self.bpk_q), self.nq, self.cs, L.ptr(self.count), L.ptr(self.col_index), L.ptr(self.ikq),
+                       L.ptr(st.el.U), L.ptr(st.el.filt.band_min), L.ptr(st.el.filt.band_max), L.ptr(self.uph),
+                       L.ptr(self.omega), L.ptr(self.g2), stream)
+            self._timed("kR->kq", None, krkq_rot)
+        elif self.packed:
             def krkq():
                 core.ft_pack_operand(self.A, nd2, self.nrp, nd2, nu, nd2 * self.nrp, out=self.apk_units)
                 core.fourier_packed(self.apk_units, nd2, self.nrp, self.bpk_q, self.nq, self.cs, self.Gq, nd2,
@@ -486,10 +497,12 @@ class OuterKLoop:
                 "ephip_fourier_batched_c32", nd2, self.nrp, L.ptr(self.A), nd2, nd2 * self.nrp,
                 L.ptr(self.irvec_ep_dev), L.ptr(st.ph.xq), nu, self.nq, L.ptr(self.count), L.ptr(self.col_index),
                 self.cs, L.ptr(self.Gq), nd2, self.cs * nd2, self.precision, stream))
-        # 4. rotations + g2
-        L.call("ephip_eph_rotate_kq_g2", nw, nm, st.nbw, nu, self.cs, L.ptr(self.count), L.ptr(self.col_index),
-               L.ptr(self.ikq), L.ptr(self.Gq), nd2, self.cs * nd2, L.ptr(st.el.U), L.ptr(st.el.filt.band_min),
-               L.ptr(st.el.filt.band_max), L.ptr(self.uph), L.ptr(self.omega), L.ptr(self.g2), None, stream)
+        # 4. rotations + g2 (already done by the fused kernel)
+        if not self.fused:
+            L.call("ephip_eph_rotate_kq_g2", nw, nm, st.nbw, nu, self.cs, L.ptr(self.count),
+                   L.ptr(self.col_index), L.ptr(self.ikq), L.ptr(self.Gq), nd2, self.cs * nd2, L.ptr(st.el.U),
+                   L.ptr(st.el.filt.band_min), L.ptr(st.el.filt.band_max), L.ptr(self.uph), L.ptr(self.omega),
+                   L.ptr(self.g2), None, stream)
         payload = EPDataUnits((i0, i0 + n), self.unit_state[u0:u1], self.unit_band[u0:u1], self.count[:nu],
                               self.col_index[:nu], self.ikq[:nu], self.g2[:nu], st.nbw)
         for c in calculators:
@@ -497,11 +510,13 @@ class OuterKLoop:
         return payload
 
     def tune(self, reps: int = 3) -> dict:
-        """Pick the FP32 Fourier kernel variant with the shortest time for one full k batch."""
+        """Pick the Fourier kernel variant (FP32: LDS staging; split-BF16: packed tile shape) with the
+        shortest time for one full k batch."""
         from . import tune as T
-        if self.precision != L.PREC_FP32:
-            return {}
         n = min(self.nk_batch, self.st.ksurv.shape[0])
+        if self.packed:
+            return T.tune_fourier(T.KIND_PACKED_BF16, lambda: self.run_kbatch(0, n), reps=reps,
+                                  variants=T.PACKED_VARIANTS)
         return T.tune_fourier(T.KIND_FP32_WIDE, lambda: self.run_kbatch(0, n), reps=reps)
 
     def run(self, k_start: int = 0, k_stop: int | None = None, calculators=()):

@@ -1,7 +1,8 @@
 """Autotuner for the Fourier-contraction kernels.
 
 The reference hard-codes its crossovers ("tuned on one GPU", src/wannier_to_bloch_batched.jl:397-406).
-Here each contraction kind has several kernel variants (LDS staging depth, single/double buffering);
+Here each contraction kind has several kernel variants (LDS staging depth, single/double buffering,
+and for the pre-packed split-BF16 kernel the workgroup tile shape and operand prefetch);
 ``tune_fourier`` times every variant on the caller's live shape with HIP events on the launch stream
 and installs the fastest through ``ephip_ft_set_variant``. Results are plain dicts, so a caller can
 log or cache them.
@@ -10,9 +11,11 @@ from __future__ import annotations
 
 from . import _lib as L
 
-KIND_FP32_WIDE, KIND_FP32_NARROW, KIND_FP64 = 0, 1, 2
+KIND_FP32_WIDE, KIND_FP32_NARROW, KIND_FP64, KIND_PACKED_BF16 = 0, 1, 2, 3
 VARIANTS = (0, 1, 2)
 VARIANT_NAMES = {0: "single-buffered KC16", 1: "double-buffered KC16", 2: "double-buffered KC8"}
+PACKED_VARIANTS = (0, 1, 2, 3)
+PACKED_VARIANT_NAMES = {0: "128x128", 1: "128x128 + operand prefetch", 2: "192x64", 3: "192x64 XCD-grouped"}
 
 
 def get_variant(kind: int) -> int:

@@ -197,9 +197,25 @@ int ephip_fourier_packed_bf16x3(int64_t ndata, int32_t nr, const void *apk, int3
                                 int64_t npts, int32_t nk_max, const int32_t *ncol, const int32_t *col_index,
                                 int64_t col_stride, void *out, int64_t ldout, int64_t out_stride, void *stream);
 
+/* kR->kq Fourier over Rp fused with the k+q-window and phonon rotations and g2 (the
+ * get_eph_kR_to_kq_batched! + eph_apply_rotations! + g2 sequence of src/wannier_to_bloch_batched.jl:
+ * 245-273 and :442-455 in one kernel) for nw = 16, nm = 12, nbw <= 4, split-BF16: apk holds the
+ * nunits packed unit operands (rows i + 16 nu, nr = nr_ep), bpk the packed q-grid phase table of npts
+ * points; count/col_index/ikq (row stride cs) are the kept-q lists of ephip_kq_compact; U, bmin, bmax,
+ * uph, omega as in ephip_eph_rotate_kq_g2. Writes g2 (nbw x nm per kept (unit, q), [m, nu]) only. */
+int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t nbw, int32_t nr, const void *apk,
+                                      int32_t nunits, const void *bpk, int64_t npts, int64_t cs, const int32_t *count,
+                                      const int32_t *col_index, const int32_t *ikq, const void *U, const int64_t *bmin,
+                                      const int64_t *bmax, const void *uph, const double *omega, float *g2,
+                                      void *stream);
+
 /* Kernel variant of the Fourier contractions, chosen by the tuner (ephip/tune.py) from timings on the
- * live shapes. kind: 0 = FP32 with nk > 64, 1 = FP32 with nk <= 64, 2 = FP64. variant: 0 = single-
+ * live shapes. kind: 0 = FP32 with nk > 64, 1 = FP32 with nk <= 64, 2 = FP64; variant: 0 = single-
  * buffered LDS stages of 16 R, 1 = double-buffered stages of 16 R, 2 = double-buffered stages of 8 R.
+ * kind 3 = pre-packed split-BF16 with ndata >= 128 (except 192-row operands, always 192 x 64);
+ * variant: 0 = 128 x 128 tiles, 1 = 128 x 128 with the next stage's operand prefetched into registers,
+ * 2 = 192 x 64 tiles (ndata a multiple of 192), 3 = 192 x 64 tiles in XCD-grouped workgroup order (the
+ * column tiles of a row tile on one XCD, sharing its L2; single-unit calls).
  * Process-wide setting (no counterpart in the reference, whose tuning constants are hard-coded:
  * src/wannier_to_bloch_batched.jl:397-406). Host-side, synchronous. */
 int ephip_ft_set_variant(int32_t kind, int32_t variant);

@@ -338,3 +338,25 @@ def test_fourier_packed_argument_checks(backend):
     with pytest.raises(L.EphipError):   # ldout < ndata
         L.call("ephip_fourier_packed_bf16x3", 4, 3, L.ptr(x), 1, L.ptr(x), 8, 3, None, None, 0, L.ptr(x), 3, 0,
                L.stream_handle())
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_fourier_packed_variants(backend, variant):
+    """Every tile shape of the packed split-BF16 kernel (kind 3 of the tuner) on an operand whose row
+    count suits all three (384 = 3 x 128 = 2 x 192)."""
+    from ephip import core, tune
+    old = tune.get_variant(tune.KIND_PACKED_BF16)
+    rng = np.random.default_rng(70 + variant)
+    irvec = M.rbox(7)
+    nr, ndata, nk = len(irvec), 384 * 9, 200   # 18 row tiles of 192: two XCD groups of 8 and a tail
+    op = rng.standard_normal((nr, ndata)) + 1j * rng.standard_normal((nr, ndata))
+    xk = rng.random((nk, 3))
+    try:
+        tune.set_variant(tune.KIND_PACKED_BF16, variant)
+        apk = core.ft_pack_operand(_dev(op.astype(np.complex64)), ndata, nr, ndata)
+        bpk = core.ft_pack_phases(_dev(irvec), _dev(xk))
+        out = torch.empty((nk, ndata), dtype=torch.complex64, device="cuda")
+        core.fourier_packed(apk, ndata, nr, bpk, nk, nk, out, ndata)
+        assert _rel(out.cpu().numpy(), O.fourier(op, irvec, xk)) < 2e-5
+    finally:
+        tune.set_variant(tune.KIND_PACKED_BF16, old)

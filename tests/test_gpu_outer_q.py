@@ -79,22 +79,10 @@ def test_outer_q_loop_g2_parity(backend, cfg1, prec_name):
     assert rel < 1e-4, rel
 
 
-@pytest.mark.parametrize("prec_name", ["fp32", "bf16x3"])
-def test_outer_k_loop_g2_parity(backend, cfg1, prec_name):
-    """Outer-k loop with the k-side window projection (RR->kR->kq) against the oracle's RR->Rq->kq
-    on the same pairs: the kept-q lists are exact (q kept iff k+q has an in-window band), |g|^2 within
-    1e-4 norm-wise after degenerate averaging."""
-    from ephip import _lib as L
-    from ephip import eph
-    m, g = cfg1
-    prec = L.PREC_FP32 if prec_name == "fp32" else L.PREC_BF16X3
-    w = g["window"]
-    st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
+def _outer_k_vs_oracle(m, g, st, rec, qstep=23):
+    """Compare recorded outer-k payloads with the oracle's RR->Rq->kq on the same pairs: kept-q lists
+    exact, |g|^2 within 1e-4 norm-wise after degenerate averaging. Returns the relative error."""
     xk, E, U, keep, bmin, bmax, xq, ph = _oracle_states(m, g)
-    loop = eph.OuterKLoop(st, nk_batch=37, precision=prec)
-    rec = eph.UnitRecorder()
-    loop.run_kbatch(0, 37, [rec])
-    loop.run_kbatch(200, 37, [rec])
     nw, nm = m.nw, m.nmodes
     per_kq = {}
     for b in rec.batches:
@@ -105,7 +93,7 @@ def test_outer_k_loop_g2_parity(backend, cfg1, prec_name):
             assert np.array_equal(b["ikq"][u], O.hash_xk(xk[k] + xq[b["cols"][u]], g["kgrid"]))
             for c, q in enumerate(b["cols"][u]):
                 per_kq.setdefault((k, int(q)), {})[n] = b["g2"][u][c].reshape(st.nbw, nm)  # [m, nu]
-    qsel = set(range(0, len(xq), 23)) | {0, 1}
+    qsel = set(range(0, len(xq), qstep)) | {0, 1}
     cutoff, errs = 1e-5, []
     eRpq_cache = {}
     for (k, q), bands in per_kq.items():
@@ -118,7 +106,7 @@ def test_outer_k_loop_g2_parity(backend, cfg1, prec_name):
         Uk = U[k].reshape(nw, nw, order="F")[:, bmin[k] - 1:bmax[k]]
         Ukq = U[kq].reshape(nw, nw, order="F")[:, bmin[kq] - 1:bmax[kq]]
         ep = O.eph_Rq_to_kq(nw, nm, eRpq_cache[q], m.irvec_el, xk[k], Uk, Ukq)  # [nu, n, m]
-        nbk, nbkq = Uk.shape[1], Ukq.shape[1]
+        nbkq = Ukq.shape[1]
         assert sorted(bands) == list(range(bmin[k] - 1, bmax[k]))
         got = np.stack([bands[n][:nbkq, :].T for n in range(bmin[k] - 1, bmax[k])], axis=1)  # [nu, n, m]
         assert not np.any([bands[n][nbkq:, :] for n in bands])
@@ -128,7 +116,62 @@ def test_outer_k_loop_g2_parity(backend, cfg1, prec_name):
         r = O.g2_degenerate_average(ref, ekq, ek, cutoff)[ok]
         gg = O.g2_degenerate_average(got.astype(np.float64), ekq, ek, cutoff)[ok]
         errs.append((np.abs(gg - r).max(), np.abs(r).max()))
-    assert len(errs) > 50
+    assert len(errs) > 20
     e = np.array(errs)
-    rel = e[:, 0].max() / e[:, 1].max()
+    return e[:, 0].max() / e[:, 1].max()
+
+
+@pytest.mark.parametrize("prec_name", ["fp32", "bf16x3"])
+def test_outer_k_loop_g2_parity(backend, cfg1, prec_name):
+    """Outer-k loop with the k-side window projection (RR->kR->kq) against the oracle's RR->Rq->kq
+    on the same pairs: the kept-q lists are exact (q kept iff k+q has an in-window band), |g|^2 within
+    1e-4 norm-wise after degenerate averaging."""
+    from ephip import _lib as L
+    from ephip import eph
+    m, g = cfg1
+    prec = L.PREC_FP32 if prec_name == "fp32" else L.PREC_BF16X3
+    w = g["window"]
+    st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
+    loop = eph.OuterKLoop(st, nk_batch=37, precision=prec)
+    rec = eph.UnitRecorder()
+    loop.run_kbatch(0, 37, [rec])
+    loop.run_kbatch(200, 37, [rec])
+    rel = _outer_k_vs_oracle(m, g, st, rec)
+    assert rel < 1e-4, rel
+
+
+@pytest.fixture(scope="module")
+def model16():
+    """nw 16, 4 atoms (nm 12): the e-ph unit shape of the larger workloads, on small boxes and grids."""
+    m = M.make_model(nw=16, natoms=4, box_hd=5, box_g=3, seed=2024, g_dtype=np.complex128)
+    return m, {"kgrid": (8, 8, 8), "qgrid": (4, 4, 4), "window": 0.5}
+
+
+def test_outer_k_fused_rotation(backend, model16):
+    """Split-BF16 outer-k with the kR->kq Fourier fused to the rotations (nw 16, nm 12): same kept-q
+    lists as the unfused kernels, g2 within 1e-5 of them, and within 1e-4 of the oracle."""
+    from ephip import _lib as L
+    from ephip import eph
+    m, g = model16
+    w = g["window"]
+    st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
+    assert 1 <= st.nbw <= 4
+    fused = eph.OuterKLoop(st, nk_batch=40, precision=L.PREC_BF16X3)
+    plain = eph.OuterKLoop(st, nk_batch=40, precision=L.PREC_BF16X3, fuse_rotation=False)
+    assert fused.fused and not plain.fused and fused.Gq is None
+    rf, rp = eph.UnitRecorder(), eph.UnitRecorder()
+    nks = st.ksurv.shape[0]
+    for i0 in (0, max(0, nks - 40)):
+        fused.run_kbatch(i0, 40, [rf])
+        plain.run_kbatch(i0, 40, [rp])
+    num = den = 0.0
+    for bf, bp in zip(rf.batches, rp.batches):
+        assert np.array_equal(bf["unit_state"], bp["unit_state"])
+        for u in range(len(bf["cols"])):
+            assert np.array_equal(bf["cols"][u], bp["cols"][u])
+            assert np.array_equal(bf["ikq"][u], bp["ikq"][u])
+            num = max(num, float(np.abs(bf["g2"][u] - bp["g2"][u]).max(initial=0.0)))
+            den = max(den, float(np.abs(bp["g2"][u]).max(initial=0.0)))
+    assert den > 0 and num / den < 1e-5, num / den
+    rel = _outer_k_vs_oracle(m, g, st, rf, qstep=5)
     assert rel < 1e-4, rel
