@@ -121,7 +121,7 @@ def transport_bench(args):
         out[name] = {"sigma_serta": r.sigma_serta[0].tolist(), "sigma": r.sigma[0].tolist(),
                      "niter": r.niter[0], "converged": r.converged[0], "seconds": run.seconds,
                      "n_states": run.calc.bs.n,
-                     "s_in_rows": int(sum(b.row_kq.shape[0] for b in run.calc.blocks))}
+                     "s_in_rows": int(sum(b.nrows for b in run.calc.blocks))}
         log(f"[transport] {name}: sigma_serta diag {np.diag(r.sigma_serta[0])}, sigma diag {np.diag(r.sigma[0])}, "
             f"{r.niter[0]} iterations, times {run.seconds}")
         del run
@@ -252,7 +252,8 @@ def main():
         if args.tune:
             tuned = loop.tune()
             log(f"[bench] tuner (ms per k batch by variant): {tuned}")
-            results["tuner"] = {str(k): v for k, v in tuned.items()}
+            results["tuner"] = {str(k): ({str(a): b for a, b in v.items()} if isinstance(v, dict) else v)
+                                for k, v in tuned.items()}
         ev = []
         loop.ft_timer = ev
         nks_steps = args.kbatch_k

@@ -78,7 +78,7 @@ __global__ void bte_increments_kernel(int64_t n, int method, int smearing, const
 // f = state_offset[ikq] + m (final state); energies e_state[i], e_state[f]; modes below w_cut skipped.
 // S_out[i + nstates*iT] += sum_nu s_out (atomic). S_in is compacted: the unit's kept columns are
 // rows rowptr[u] .. rowptr[u] + count[u] - 1, S_in[(row*nbw + m)*nT + iT] = sum_nu s_in, and
-// row_kq[row] keeps the k+q state of the row for the later products.
+// row_code[row] = first final state | (number of k+q window slots << 27) of the row.
 __global__ void bte_accumulate_kernel(int nunits, int64_t cs, int nbw, int nm, const int *__restrict__ count,
                                       const int *__restrict__ col_index, const int *__restrict__ ikq,
                                       const float *__restrict__ g2, const int64_t *__restrict__ unit_state_index,
@@ -88,7 +88,7 @@ __global__ void bte_accumulate_kernel(int nunits, int64_t cs, int nbw, int nm, c
                                       const double *__restrict__ mus, const double *__restrict__ Ts,
                                       const double *__restrict__ etas, int method, int smearing, double w_cut,
                                       int64_t nstates, const int64_t *__restrict__ rowptr, double *__restrict__ S_out,
-                                      double *__restrict__ S_in, int *__restrict__ row_kq) {
+                                      double *__restrict__ S_in, uint32_t *__restrict__ row_code) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int u = blockIdx.y;
     if (u >= nunits || e >= cs * nbw) return;
@@ -98,8 +98,8 @@ __global__ void bte_accumulate_kernel(int nunits, int64_t cs, int nbw, int nm, c
     const int64_t row = (int64_t)u * cs + c;      // payload row
     const int64_t orow = rowptr[u] + c;           // compacted output row
     const int64_t kq = ikq[row];
-    if (m == 0) row_kq[orow] = (int)kq;
-    const int nbkq = bmin[kq] > 0 ? (int)(bmax[kq] - bmin[kq] + 1) : 0;
+    const int nbkq = bmin[kq] > 0 ? min((int)(bmax[kq] - bmin[kq] + 1), nbw) : 0;
+    if (m == 0) row_code[orow] = (uint32_t)state_offset[kq] | ((uint32_t)nbkq << 27);
     double *sin_row = S_in + (orow * nbw + m) * nT;
     if (m >= nbkq) {
         for (int t = 0; t < nT; t++) sin_row[t] = 0.0;
@@ -154,14 +154,12 @@ __global__ void band_velocity_kernel(int nw, int64_t nstates, const int64_t *__r
 }
 
 // y[i, :] = sum over the unit's rows and k+q slots S_in[row, m, iT] x[f, :] (3 components), one
-// wavefront per unit (i = unit_state_index[u]); lanes stride over the (row, slot) entries.
+// wavefront per unit (i = unit_state_index[u]); lanes take consecutive rows, f = first state of the
+// row's code + m for m below the row's slot count.
 __global__ void __launch_bounds__(256) bte_spmv_kernel(int nunits, int nbw, int nT, int iT,
                                                       const int64_t *__restrict__ rowptr,
-                                                      const int *__restrict__ row_kq,
+                                                      const uint32_t *__restrict__ row_code,
                                                       const int64_t *__restrict__ unit_state_index,
-                                                      const int64_t *__restrict__ state_offset,
-                                                      const int64_t *__restrict__ bmin,
-                                                      const int64_t *__restrict__ bmax,
                                                       const double *__restrict__ S_in,
                                                       const double *__restrict__ x, double *__restrict__ y) {
     const int lane = threadIdx.x & 63;
@@ -169,16 +167,17 @@ __global__ void __launch_bounds__(256) bte_spmv_kernel(int nunits, int nbw, int 
     if (u >= nunits) return;  // whole wavefront exits together
     const int64_t r0 = rowptr[u], r1 = rowptr[u + 1];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int64_t e = r0 * nbw + lane; e < r1 * nbw; e += 64) {
-        const int64_t row = e / nbw;
-        const int m = (int)(e % nbw);
-        const int64_t kq = row_kq[row];
-        if (bmin[kq] <= 0 || m > (int)(bmax[kq] - bmin[kq])) continue;
-        const double s = S_in[e * nT + iT];
-        const double *xf = x + 3 * (state_offset[kq] + m);
-        a0 += s * xf[0];
-        a1 += s * xf[1];
-        a2 += s * xf[2];
+    for (int64_t row = r0 + lane; row < r1; row += 64) {
+        const uint32_t code = row_code[row];
+        const int f0 = (int)(code & 0x7ffffffu), nb = (int)(code >> 27);
+        const double *sr = S_in + row * nbw * nT + iT;
+        for (int m = 0; m < nb; m++) {
+            const double sv = sr[(int64_t)m * nT];
+            const double *xf = x + 3 * (int64_t)(f0 + m);
+            a0 += sv * xf[0];
+            a1 += sv * xf[1];
+            a2 += sv * xf[2];
+        }
     }
     a0 = wave_sum(a0);
     a1 = wave_sum(a1);
@@ -215,20 +214,22 @@ extern "C" int ephip_bte_accumulate_units(int32_t nunits, int64_t cs, int32_t nb
                                           const double *omega, double wtq, int32_t nT, const double *mus,
                                           const double *Ts, const double *etas, int32_t method, int32_t smearing,
                                           double w_cut, int64_t nstates, const int64_t *rowptr, double *S_out,
-                                          double *S_in, int32_t *row_kq, void *stream) {
-    EPHIP_REQUIRE(nunits >= 0 && nunits <= 65535 && cs >= 0 && nbw >= 1 && nm >= 1 && nT >= 1,
+                                          double *S_in, uint32_t *row_code, void *stream) {
+    EPHIP_REQUIRE(nunits >= 0 && nunits <= 65535 && cs >= 0 && nbw >= 1 && nbw <= 31 && nm >= 1 && nT >= 1,
                   "bte_accumulate_units: bad sizes");
+    EPHIP_REQUIRE(nstates < (1LL << 27), "bte_accumulate_units: %lld states exceed the row code range",
+                  (long long)nstates);
     EPHIP_REQUIRE(method >= 1 && method <= 6 && (smearing == 1 || smearing == 2),
                   "bte_accumulate_units: bad method/smearing");
     if (nunits == 0 || cs == 0) return EPHIP_OK;
     EPHIP_REQUIRE(count && col_index && ikq && g2 && unit_state_index && state_offset && bmin && bmax && e_state &&
-                      omega && mus && Ts && etas && rowptr && S_out && S_in && row_kq,
+                      omega && mus && Ts && etas && rowptr && S_out && S_in && row_code,
                   "bte_accumulate_units: null device pointer");
     dim3 g(nbk(cs * nbw), (unsigned)nunits);
     bte_accumulate_kernel<<<g, 256, 0, as_stream(stream)>>>(nunits, cs, nbw, nm, count, col_index, ikq, g2,
                                                             unit_state_index, state_offset, bmin, bmax, e_state, omega,
                                                             wtq, nT, mus, Ts, etas, method, smearing, w_cut, nstates,
-                                                            rowptr, S_out, S_in, row_kq);
+                                                            rowptr, S_out, S_in, row_code);
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }
@@ -247,15 +248,13 @@ extern "C" int ephip_band_velocity(int32_t nw, int64_t nstates, const int64_t *s
 }
 
 extern "C" int ephip_bte_spmv(int32_t nunits, int32_t nbw, int32_t nT, int32_t iT, const int64_t *rowptr,
-                              const int32_t *row_kq, const int64_t *unit_state_index, const int64_t *state_offset,
-                              const int64_t *bmin, const int64_t *bmax, const double *S_in, const double *x, double *y,
-                              void *stream) {
+                              const uint32_t *row_code, const int64_t *unit_state_index, const double *S_in,
+                              const double *x, double *y, void *stream) {
     EPHIP_REQUIRE(nunits >= 0 && nbw >= 1 && nT >= 1 && iT >= 0 && iT < nT, "bte_spmv: bad sizes");
     if (nunits == 0) return EPHIP_OK;
-    EPHIP_REQUIRE(rowptr && row_kq && unit_state_index && state_offset && bmin && bmax && S_in && x && y,
-                  "bte_spmv: null device pointer");
-    bte_spmv_kernel<<<(unsigned)((nunits + 3) / 4), 256, 0, as_stream(stream)>>>(
-        nunits, nbw, nT, iT, rowptr, row_kq, unit_state_index, state_offset, bmin, bmax, S_in, x, y);
+    EPHIP_REQUIRE(rowptr && row_code && unit_state_index && S_in && x && y, "bte_spmv: null device pointer");
+    bte_spmv_kernel<<<(unsigned)((nunits + 3) / 4), 256, 0, as_stream(stream)>>>(nunits, nbw, nT, iT, rowptr, row_code,
+                                                                                unit_state_index, S_in, x, y);
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

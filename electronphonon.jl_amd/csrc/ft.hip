@@ -397,7 +397,7 @@ __global__ __launch_bounds__(256) void ft_mfma_db_kernel(int64_t ndata, int nr, 
 // Kernel variant per contraction kind, set by the tuner (ephip_ft_set_variant). Kinds:
 // 0 = FP32 wide (nk > 64), 1 = FP32 narrow (nk <= 64), 2 = FP64. Variants: 0 = single-buffered,
 // 1 = double-buffered KC 16, 2 = double-buffered KC 8.
-static int g_ft_variant[4] = {0, 0, 0, 0};
+static int g_ft_variant[5] = {0, 0, 0, 0, 0};
 
 template <class T, int TN>
 static void launch_ft(int variant, dim3 g, hipStream_t s, int64_t ndata, int nr, const cplx<T> *op, int64_t ldop,
@@ -580,7 +580,9 @@ struct FtpShape {
 
 // Main loop: accumulates the workgroup tile (rows d0.., columns j0..) of one unit into accr/acci.
 // apk and col_index point at the unit's operand and column list; Bs is 2 x TN x FTP_BLD bf16 of LDS.
-template <int RB, int CB, int WR, bool PF>
+// MID: the next stage's phase pieces are written to LDS halfway through the MFMA sequence instead of
+// after it, so the LDS write latency overlaps the second half.
+template <int RB, int CB, int WR, bool PF, bool MID = false>
 __device__ __forceinline__ void ft_bf16p_mainloop(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
                                                   const __bf16 *__restrict__ bpk, int64_t npts, int nk,
                                                   const int *__restrict__ col_index, int64_t j0, int64_t d0,
@@ -655,6 +657,13 @@ __device__ __forceinline__ void ft_bf16p_mainloop(int64_t ndata, int nrs, const 
         const __bf16 *bb = Bs + cur * (TN * FTP_BLD);
 #pragma unroll
         for (int cb = 0; cb < CB; cb++) {
+            if constexpr (MID) {
+                if (cb == CB / 2 && more) {
+#pragma unroll
+                    for (int t = 0; t < PPT; t++)
+                        *reinterpret_cast<bf16x8 *>(&Bs[(cur ^ 1) * (TN * FTP_BLD) + bdst[t]]) = bq[t];
+                }
+            }
             const __bf16 *col = bb + (wc * (CB * 16) + cb * 16 + il) * FTP_BLD + 8 * c;
             const bf16x8 b2h = *reinterpret_cast<const bf16x8 *>(col);
             const bf16x8 b1h = *reinterpret_cast<const bf16x8 *>(col + 16);
@@ -678,8 +687,11 @@ __device__ __forceinline__ void ft_bf16p_mainloop(int64_t ndata, int nrs, const 
             }
         }
         if (more) {
+            if constexpr (!MID) {
 #pragma unroll
-            for (int t = 0; t < PPT; t++) *reinterpret_cast<bf16x8 *>(&Bs[(cur ^ 1) * (TN * FTP_BLD) + bdst[t]]) = bq[t];
+                for (int t = 0; t < PPT; t++)
+                    *reinterpret_cast<bf16x8 *>(&Bs[(cur ^ 1) * (TN * FTP_BLD) + bdst[t]]) = bq[t];
+            }
             if constexpr (PF) {
 #pragma unroll
                 for (int rb = 0; rb < RB; rb++) {
@@ -710,7 +722,7 @@ __device__ __forceinline__ void xcd_grouped_tile(int64_t &bx, int64_t &by) {
     }
 }
 
-template <int RB, int CB, int WR, bool PF, bool XG = false>
+template <int RB, int CB, int WR, bool PF, bool XG = false, bool MID = false>
 __device__ __forceinline__ void ft_bf16p_body(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
                                               const __bf16 *__restrict__ bpk, int64_t npts, int nk,
                                               cplx<float> *__restrict__ out, int64_t ldout, int64_t out_unit,
@@ -729,8 +741,9 @@ __device__ __forceinline__ void ft_bf16p_body(int64_t ndata, int nrs, const __bf
     if (j0 >= nk) return;  // uniform for the block
     const int64_t d0 = by * S::TM;
     f32x4 accr[RB][CB], acci[RB][CB];
-    ft_bf16p_mainloop<RB, CB, WR, PF>(ndata, nrs, apk + (int64_t)u * nrs * ndata * 64, bpk, npts, nk,
-                                      col_index ? col_index + u * col_stride : nullptr, j0, d0, Bs, accr, acci);
+    ft_bf16p_mainloop<RB, CB, WR, PF, MID>(ndata, nrs, apk + (int64_t)u * nrs * ndata * 64, bpk, npts, nk,
+                                           col_index ? col_index + u * col_stride : nullptr, j0, d0, Bs, accr,
+                                           acci);
     out += u * out_unit;
 #pragma unroll
     for (int rb = 0; rb < RB; rb++)
@@ -748,14 +761,14 @@ __device__ __forceinline__ void ft_bf16p_body(int64_t ndata, int nrs, const __bf
 
 // Two resident waves per SIMD (at most 256 registers) for the shapes that fit without spilling; the
 // 128 x 128 prefetching shape needs more registers and runs one wave per SIMD.
-template <int RB, int CB, int WR, bool PF, bool XG = false>
+template <int RB, int CB, int WR, bool PF, bool XG = false, bool MID = false>
 __global__ __launch_bounds__(256, 2) void ft_bf16p_kernel(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
                                                           const __bf16 *__restrict__ bpk, int64_t npts, int nk,
                                                           cplx<float> *__restrict__ out, int64_t ldout,
                                                           int64_t out_unit, const int *__restrict__ col_index,
                                                           int64_t col_stride, const int *__restrict__ ncol) {
-    ft_bf16p_body<RB, CB, WR, PF, XG>(ndata, nrs, apk, bpk, npts, nk, out, ldout, out_unit, col_index, col_stride,
-                                      ncol);
+    ft_bf16p_body<RB, CB, WR, PF, XG, MID>(ndata, nrs, apk, bpk, npts, nk, out, ldout, out_unit, col_index,
+                                           col_stride, ncol);
 }
 
 __global__ __launch_bounds__(256) void ft_bf16p_wide_pf_kernel(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
@@ -776,7 +789,7 @@ __global__ __launch_bounds__(256) void ft_bf16p_wide_pf_kernel(int64_t ndata, in
 // ---------------------------------------------------------------------------------------------
 constexpr int FTR_NBW_MAX = 4;
 
-template <int NBW>
+template <int NBW, bool MID = false>
 __global__ __launch_bounds__(256, 2) void ft_bf16p_rot_kernel(int nrs, const __bf16 *__restrict__ apk,
                                                               const __bf16 *__restrict__ bpk, int64_t npts,
                                                               const int *__restrict__ ncol,
@@ -799,8 +812,8 @@ __global__ __launch_bounds__(256, 2) void ft_bf16p_rot_kernel(int nrs, const __b
     const int64_t j0 = (int64_t)blockIdx.x * TN;
     if (j0 >= nk) return;  // uniform for the block
     f32x4 accr[RB][CB], acci[RB][CB];
-    ft_bf16p_mainloop<RB, CB, WR, true>(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk,
-                                        col_index + u * cs, j0, 0, Bs, accr, acci);
+    ft_bf16p_mainloop<RB, CB, WR, true, MID>(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk,
+                                             col_index + u * cs, j0, 0, Bs, accr, acci);
     // after the main loop's final barrier the staging buffer is free: ep tile [jj][m][nu'] (re, im)
     float *EP = reinterpret_cast<float *>(Bs);
 
@@ -1074,14 +1087,15 @@ extern "C" int ephip_fourier_batched_c32(int64_t ndata, int32_t nr, const void *
 }
 
 extern "C" int ephip_ft_set_variant(int32_t kind, int32_t variant) {
-    EPHIP_REQUIRE(kind >= 0 && kind < 4, "ft_set_variant: kind %d outside 0..3", kind);
-    EPHIP_REQUIRE(variant >= 0 && variant < (kind == 3 ? 4 : 3), "ft_set_variant: variant %d outside the range of kind %d",
+    static const int nvar[5] = {3, 3, 3, 5, 2};
+    EPHIP_REQUIRE(kind >= 0 && kind < 5, "ft_set_variant: kind %d outside 0..4", kind);
+    EPHIP_REQUIRE(variant >= 0 && variant < nvar[kind], "ft_set_variant: variant %d outside the range of kind %d",
                   variant, kind);
     g_ft_variant[kind] = variant;
     return EPHIP_OK;
 }
 
-extern "C" int ephip_ft_get_variant(int32_t kind) { return (kind >= 0 && kind < 4) ? g_ft_variant[kind] : -1; }
+extern "C" int ephip_ft_get_variant(int32_t kind) { return (kind >= 0 && kind < 5) ? g_ft_variant[kind] : -1; }
 
 extern "C" int ephip_ft_pack_operand(int64_t ndata, int32_t nr, const void *op, int64_t ldop, int32_t nunits,
                                      int64_t op_stride, void *apk, void *stream) {
@@ -1134,10 +1148,11 @@ extern "C" int ephip_fourier_packed_bf16x3(int64_t ndata, int32_t nr, const void
     if (ndata % 192 == 0 && (nk_max <= 64 || ndata % 128 != 0))
         shape = 2;
     else if (ndata >= 128)
-        shape = ((var == 2 || var == 3) && ndata % 192 == 0) ? 2 : (var == 1 ? 3 : 1);
+        shape = ((var >= 2) && ndata % 192 == 0) ? 2 : (var == 1 ? 3 : 1);
     else
         shape = 0;
-    const bool grouped = shape == 2 && var == 3 && nunits == 1;
+    const bool grouped = shape == 2 && (var == 3 || var == 4) && nunits == 1;
+    const bool mid = grouped && var == 4;
     const int tm = shape == 2 ? 192 : (shape == 0 ? 64 : 128), tn = shape == 2 ? 64 : 128;
     EPHIP_REQUIRE((ndata + tm - 1) / tm <= 65535, "fourier_packed: ndata=%lld too large for one call", (long long)ndata);
     dim3 g((unsigned)((nk_max + tn - 1) / tn), (unsigned)((ndata + tm - 1) / tm), (unsigned)nunits);
@@ -1145,7 +1160,10 @@ extern "C" int ephip_fourier_packed_bf16x3(int64_t ndata, int32_t nr, const void
     auto b = static_cast<const __bf16 *>(bpk);
     auto o = static_cast<cplx<float> *>(out);
     hipStream_t st = as_stream(stream);
-    if (grouped)
+    if (mid)
+        ft_bf16p_kernel<3, 4, 4, true, true, true><<<g, 256, 0, st>>>(ndata, nrs, a, b, npts, nk_max, o, ldout,
+                                                                       out_stride, col_index, col_stride, ncol);
+    else if (grouped)
         ft_bf16p_kernel<3, 4, 4, true, true><<<g, 256, 0, st>>>(ndata, nrs, a, b, npts, nk_max, o, ldout, out_stride,
                                                                  col_index, col_stride, ncol);
     else if (shape == 2)
@@ -1184,9 +1202,16 @@ extern "C" int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t
     auto uu = static_cast<const cplx<float> *>(U);
     auto up = static_cast<const cplx<float> *>(uph);
     hipStream_t st = as_stream(stream);
-    if (nbw <= 2)
+    const bool mid = g_ft_variant[4] == 1;
+    if (nbw <= 2 && mid)
+        ft_bf16p_rot_kernel<2, true><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax, up,
+                                                        omega, nbw, g2);
+    else if (nbw <= 2)
         ft_bf16p_rot_kernel<2><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax, up, omega,
                                                   nbw, g2);
+    else if (mid)
+        ft_bf16p_rot_kernel<4, true><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax, up,
+                                                        omega, nbw, g2);
     else
         ft_bf16p_rot_kernel<4><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax, up, omega,
                                                   nbw, g2);

@@ -185,12 +185,14 @@ def bte_compute_mu(params: ElectronTransportParams, bs: BandStates) -> np.ndarra
 # ------------------------------------------------------------------------------------------------
 @dataclasses.dataclass
 class SinBlock:
-    """Compressed scattering-in rows of one outer-k batch: unit u (initial state unit_index[u]) owns
-    rows rowptr[u]..rowptr[u+1]-1; row r holds nbw x nT values for the k+q state row_kq[r]."""
+    """Compressed scattering-in rows: unit u (initial state unit_index[u]) owns rows
+    rowptr[u]..rowptr[u+1]-1; row r holds nbw x nT values for the final states f0 .. f0 + nb - 1 (the
+    k+q window slots) with row_code[r] = f0 | (nb << 27)."""
     unit_index: object
     rowptr: object
-    row_kq: object
+    row_code: object
     S_in: object
+    nrows: int
 
 
 class BoltzmannCalculator:
@@ -251,17 +253,30 @@ class BoltzmannCalculator:
         rowptr[1:] = torch.cumsum(p.count, 0)
         nrows = int(rowptr[-1])
         S_in = torch.empty((max(nrows, 1), p.nbw, self.nT), dtype=torch.float64, device=usi.device)
-        row_kq = torch.empty(max(nrows, 1), dtype=torch.int32, device=usi.device)
+        row_code = torch.empty(max(nrows, 1), dtype=torch.int32, device=usi.device)
         cs = p.col_index.stride(0)
         L.call("ephip_bte_accumulate_units", nu, cs, p.nbw, st.nm, L.ptr(p.count), L.ptr(p.col_index),
                L.ptr(p.ikq), L.ptr(p.g2), L.ptr(usi), L.ptr(bs.state_offset), L.ptr(f.band_min),
                L.ptr(f.band_max), L.ptr(bs.es), L.ptr(self.loop.omega), self.wtq, self.nT, L.ptr(self.mus),
                L.ptr(self.Ts), L.ptr(self.etas_d), self.method, self.kind, self.omega_cutoff, bs.n,
-               L.ptr(rowptr), L.ptr(self.S_out), L.ptr(S_in), L.ptr(row_kq), L.stream_handle())
-        self.blocks.append(SinBlock(usi, rowptr, row_kq, S_in))
+               L.ptr(rowptr), L.ptr(self.S_out), L.ptr(S_in), L.ptr(row_code), L.stream_handle())
+        self.blocks.append(SinBlock(usi, rowptr, row_code, S_in[:nrows], nrows))
 
     def postprocess(self):
+        """Merge the per-batch compressed rows into one block (one product launch over every state)."""
+        torch = _torch()
         self.done = True
+        if len(self.blocks) > 1:
+            off, parts = 0, []
+            for b in self.blocks:
+                parts.append(b.rowptr[:-1] + off)
+                off += b.nrows
+            rowptr = torch.cat(parts + [torch.tensor([off], dtype=torch.int64, device=self.S_out.device)])
+            merged = SinBlock(torch.cat([b.unit_index for b in self.blocks]), rowptr,
+                              torch.cat([b.row_code[:b.nrows] for b in self.blocks]),
+                              torch.cat([b.S_in for b in self.blocks]), off)
+            self.blocks = [merged]
+            torch.cuda.empty_cache()
 
     def begin_q(self, iq):
         pass
@@ -274,29 +289,24 @@ class BoltzmannCalculator:
         """Host dense S_in (n x n) for one temperature (tests and small grids)."""
         n = self.bs.n
         out = np.zeros((n, n))
-        off = self.bs.state_offset.cpu().numpy()
         for b in self.blocks:
             ui = b.unit_index.cpu().numpy()
             rp = b.rowptr.cpu().numpy()
-            rk = b.row_kq.cpu().numpy()
+            code = b.row_code.cpu().numpy().view(np.uint32)
+            rf = (code & 0x7FFFFFF).astype(np.int64)
+            rn = (code >> 27).astype(np.int64)
             S = b.S_in.cpu().numpy()
-            nbw = S.shape[1]
             for u in range(len(ui)):
                 for r in range(rp[u], rp[u + 1]):
-                    kq = rk[r]
-                    nbkq = off[kq + 1] - off[kq]
-                    out[ui[u], off[kq]:off[kq] + nbkq] += S[r, :nbkq, iT]
+                    out[ui[u], rf[r]:rf[r] + rn[r]] += S[r, :rn[r], iT]
         return out
 
     def spmv(self, iT: int, x, y):
         """y = S_in(iT) x for (n, 3) float64 device vectors over the in-window states."""
-        st = self.loop.st
-        f = st.el.filt
         for b in self.blocks:
             nu = b.unit_index.shape[0]
-            L.call("ephip_bte_spmv", nu, b.S_in.shape[1], self.nT, iT, L.ptr(b.rowptr), L.ptr(b.row_kq),
-                   L.ptr(b.unit_index), L.ptr(self.bs.state_offset), L.ptr(f.band_min), L.ptr(f.band_max),
-                   L.ptr(b.S_in), L.ptr(x), L.ptr(y), L.stream_handle())
+            L.call("ephip_bte_spmv", nu, b.S_in.shape[1], self.nT, iT, L.ptr(b.rowptr), L.ptr(b.row_code),
+                   L.ptr(b.unit_index), L.ptr(b.S_in), L.ptr(x), L.ptr(y), L.stream_handle())
         return y
 
 

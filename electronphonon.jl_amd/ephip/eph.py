@@ -515,8 +515,12 @@ class OuterKLoop:
         from . import tune as T
         n = min(self.nk_batch, self.st.ksurv.shape[0])
         if self.packed:
-            return T.tune_fourier(T.KIND_PACKED_BF16, lambda: self.run_kbatch(0, n), reps=reps,
-                                  variants=T.PACKED_VARIANTS)
+            res = {"packed": T.tune_fourier(T.KIND_PACKED_BF16, lambda: self.run_kbatch(0, n), reps=reps,
+                                            variants=T.PACKED_VARIANTS)}
+            if self.fused:
+                res["fused"] = T.tune_fourier(T.KIND_FUSED_ROT, lambda: self.run_kbatch(0, n), reps=reps,
+                                              variants=T.FUSED_VARIANTS)
+            return res
         return T.tune_fourier(T.KIND_FP32_WIDE, lambda: self.run_kbatch(0, n), reps=reps)
 
     def run(self, k_start: int = 0, k_stop: int | None = None, calculators=()):

@@ -215,7 +215,9 @@ int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t nbw, int32
  * kind 3 = pre-packed split-BF16 with ndata >= 128 (except 192-row operands, always 192 x 64);
  * variant: 0 = 128 x 128 tiles, 1 = 128 x 128 with the next stage's operand prefetched into registers,
  * 2 = 192 x 64 tiles (ndata a multiple of 192), 3 = 192 x 64 tiles in XCD-grouped workgroup order (the
- * column tiles of a row tile on one XCD, sharing its L2; single-unit calls).
+ * column tiles of a row tile on one XCD, sharing its L2; single-unit calls), 4 = as 3 with the next
+ * stage's phase pieces written to LDS halfway through the MFMA sequence. kind 4 = the fused kR->kq
+ * rotation kernel; variant 0 = phase pieces written after the MFMA sequence, 1 = halfway through it.
  * Process-wide setting (no counterpart in the reference, whose tuning constants are hard-coded:
  * src/wannier_to_bloch_batched.jl:397-406). Host-side, synchronous. */
 int ephip_ft_set_variant(int32_t kind, int32_t variant);
@@ -278,7 +280,8 @@ int ephip_bte_increments(int64_t n, int32_t method, int32_t smearing, const doub
  * unit_state_index[u]) has count[u] kept q (payload arrays col_index, ikq, g2 of ephip_eph_rotate_kq_g2,
  * row stride cs, g2 rows of nbw x nm); final state f = state_offset[ikq] + m for k+q window slot m.
  * For each temperature t: S_out[i + nstates*t] += sum over kept q, m, modes with omega >= w_cut
- * (atomic), S_in[((rowptr[u] + c)*nbw + m)*nT + t] = sum over modes, row_kq[rowptr[u] + c] = ikq.
+ * (atomic), S_in[((rowptr[u] + c)*nbw + m)*nT + t] = sum over modes; row r = rowptr[u] + c records
+ * row_code[r] = state_offset[ikq] | (number of k+q window slots << 27) (nstates < 2^27, nbw <= 31).
  * e_state: per-state energies; omega (nm x nq); wtq: q weight; mus/Ts/etas: nT per-temperature values. */
 int ephip_bte_accumulate_units(int32_t nunits, int64_t cs, int32_t nbw, int32_t nm, const int32_t *count,
                                const int32_t *col_index, const int32_t *ikq, const float *g2,
@@ -286,7 +289,7 @@ int ephip_bte_accumulate_units(int32_t nunits, int64_t cs, int32_t nbw, int32_t 
                                const int64_t *bmax, const double *e_state, const double *omega, double wtq,
                                int32_t nT, const double *mus, const double *Ts, const double *etas, int32_t method,
                                int32_t smearing, double w_cut, int64_t nstates, const int64_t *rowptr, double *S_out,
-                               double *S_in, int32_t *row_kq, void *stream);
+                               double *S_in, uint32_t *row_code, void *stream);
 
 /* Band-diagonal velocities v[3*s + a] = Re(u^H dH_a u) of nstates states (k block state_k[s], band
  * state_band[s] of the nw x nw complex64 eigenvector blocks U); dH: complex128 operator columns of
@@ -297,12 +300,13 @@ int ephip_band_velocity(int32_t nw, int64_t nstates, const int64_t *state_col, c
                         void *stream);
 
 /* Scattering-in product of the iterative BTE for one temperature: y[3*i + a] = sum over the unit's
- * rows rowptr[u]..rowptr[u+1]-1 and k+q slots m of S_in[(row*nbw + m)*nT + iT] x[3*f + a],
- * f = state_offset[row_kq[row]] + m (the scat_mat * map_i_to_f * dF product of _solve_bte,
+ * rows rowptr[u]..rowptr[u+1]-1 and slots m < (row_code[row] >> 27) of S_in[(row*nbw + m)*nT + iT]
+ * x[3*f + a], f = (row_code[row] & (2^27 - 1)) + m, i = unit_state_index[u] (the
+ * scat_mat * map_i_to_f * dF product of _solve_bte,
  * src/boltzmann/electron_lbte.jl:116-150). */
-int ephip_bte_spmv(int32_t nunits, int32_t nbw, int32_t nT, int32_t iT, const int64_t *rowptr, const int32_t *row_kq,
-                   const int64_t *unit_state_index, const int64_t *state_offset, const int64_t *bmin,
-                   const int64_t *bmax, const double *S_in, const double *x, double *y, void *stream);
+int ephip_bte_spmv(int32_t nunits, int32_t nbw, int32_t nT, int32_t iT, const int64_t *rowptr,
+                   const uint32_t *row_code, const int64_t *unit_state_index, const double *S_in, const double *x,
+                   double *y, void *stream);
 
 #ifdef __cplusplus
 }

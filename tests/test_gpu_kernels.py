@@ -340,7 +340,7 @@ def test_fourier_packed_argument_checks(backend):
                L.stream_handle())
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_fourier_packed_variants(backend, variant):
     """Every tile shape of the packed split-BF16 kernel (kind 3 of the tuner) on an operand whose row
     count suits all three (384 = 3 x 128 = 2 x 192)."""
@@ -360,3 +360,25 @@ def test_fourier_packed_variants(backend, variant):
         assert _rel(out.cpu().numpy(), O.fourier(op, irvec, xk)) < 2e-5
     finally:
         tune.set_variant(tune.KIND_PACKED_BF16, old)
+
+
+@pytest.mark.parametrize("nr", [10, 16, 17, 33])
+def test_fourier_packed_few_stages(backend, nr):
+    """Operands with one, one full, and two or three R stages (the loads run up to two stages ahead)."""
+    from ephip import core
+    rng = np.random.default_rng(80 + nr)
+    irvec = M.rbox(5)[:nr]
+    nd, nu, npts, cs = 192, 2, 90, 70
+    op = rng.standard_normal((nu, nr, nd)) + 1j * rng.standard_normal((nu, nr, nd))
+    xk = rng.random((npts, 3))
+    counts = np.array([70, 33], np.int32)
+    cols = np.stack([np.sort(rng.choice(npts, cs, replace=False)) for _ in range(nu)]).astype(np.int32)
+    out = torch.zeros((nu, cs, nd), dtype=torch.complex64, device="cuda")
+    apk = core.ft_pack_operand(_dev(op.astype(np.complex64)), nd, nr, nd, nunits=nu, op_stride=nr * nd)
+    bpk = core.ft_pack_phases(_dev(irvec), _dev(xk))
+    core.fourier_packed(apk, nd, nr, bpk, npts, cs, out, nd, nunits=nu, ncol=_dev(counts), col_index=_dev(cols),
+                        col_stride=cs, out_stride=cs * nd)
+    o = out.cpu().numpy()
+    for u in range(nu):
+        c = counts[u]
+        assert _rel(o[u, :c], O.fourier(op[u], irvec, xk[cols[u, :c]])) < 2e-5

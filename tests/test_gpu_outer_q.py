@@ -147,11 +147,14 @@ def model16():
     return m, {"kgrid": (8, 8, 8), "qgrid": (4, 4, 4), "window": 0.5}
 
 
-def test_outer_k_fused_rotation(backend, model16):
-    """Split-BF16 outer-k with the kR->kq Fourier fused to the rotations (nw 16, nm 12): same kept-q
-    lists as the unfused kernels, g2 within 1e-5 of them, and within 1e-4 of the oracle."""
+@pytest.mark.parametrize("variant", [0, 1])
+def test_outer_k_fused_rotation(backend, model16, variant):
+    """Split-BF16 outer-k with the kR->kq Fourier fused to the rotations (nw 16, nm 12), both kernel
+    variants: same kept-q lists as the unfused kernels, g2 within 1e-5 of them, and within 1e-4 of the
+    oracle."""
     from ephip import _lib as L
-    from ephip import eph
+    from ephip import eph, tune
+    tune.set_variant(tune.KIND_FUSED_ROT, variant)
     m, g = model16
     w = g["window"]
     st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
@@ -174,4 +177,5 @@ def test_outer_k_fused_rotation(backend, model16):
             den = max(den, float(np.abs(bp["g2"][u]).max(initial=0.0)))
     assert den > 0 and num / den < 1e-5, num / den
     rel = _outer_k_vs_oracle(m, g, st, rf, qstep=5)
+    tune.set_variant(tune.KIND_FUSED_ROT, 0)
     assert rel < 1e-4, rel
