@@ -35,10 +35,14 @@ sys.path.insert(0, os.path.join(ROOT, "electronphonon.jl_amd"))
 METRIC = ("g(k,q) k·q pairs/s at |g|²≤1e-4 rel err vs FP64; plain vs "
           "mixed+batched+surrogate")
 PEAK = {"fp32": 157.3, "bf16x3": 2500.0}  # TFLOP/s, MI355X dense MFMA (fp32, bf16)
-# HBM bytes per launch of the RR->kR Fourier at k batch 128 on this workload, from the rocprofv3
-# FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00) passes calibrated on a 256 MiB copy; see
-# profiles/cfg2_hbm_counters.md. Counters cannot be read inside the timed run.
-RRKR_TRAFFIC_BYTES_KB128 = (1412486 * 2 + 1054128) * 1024
+# HBM bytes per launch of the two Fourier kernels of the fast path at k batch 128 on the cfg2 workload,
+# from rocprofv3 FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00) passes calibrated on a 256 MiB copy
+# (profiles/cfg2_fast_bf16_hbm_counters.md). Counters cannot be read inside the timed run.
+TRAFFIC_BYTES_CFG2_KB128 = {
+    ("bf16x3", "kR->kq"): (1592903 * 2 + 68234) * 1024,    # packed Fourier fused with the rotations
+    ("bf16x3", "RR->kR"): (1498651 * 2 + 1053918) * 1024,  # packed Fourier, XCD-grouped tiles
+    ("fp32", "RR->kR"): (1412486 * 2 + 1054128) * 1024,
+}
 
 
 def log(*a):
@@ -249,6 +253,7 @@ def main():
     # ---- fast outer-k path (k-side projection, k+q window skip)
     if args.path in ("fast", "both"):
         loop = eph.OuterKLoop(st, nk_batch=args.kbatch_k, precision=prec)
+        fused = bool(getattr(loop, "fused", False))
         if args.tune:
             tuned = loop.tune()
             log(f"[bench] tuner (ms per k batch by variant): {tuned}")
@@ -277,6 +282,8 @@ def main():
             kin[lab].append(a.elapsed_time(b) * 1e-3)
         f_rrkr = 8.0 * m.nw * m.nw * m.nmodes * nrp * len(m.irvec_el) * nks_steps
         f_krkq_total = 8.0 * nd2 * nrp * cols
+        if fused:  # the fused kernel also applies U_kq^H (nw x nbw) and u_ph (nm x nm) per column
+            f_krkq_total += 8.0 * (m.nw * st.nbw * m.nmodes + st.nbw * m.nmodes * m.nmodes) * cols
         t_rrkr, t_krkq = sum(kin["RR->kR"]), sum(kin["kR->kq"])
         results["fast"] = dict(value=nks_steps * nq_grid * args.steps / el, ms_per_step=1e3 * el / args.steps,
                                steps=args.steps, computed_columns=cols,
@@ -284,7 +291,8 @@ def main():
                                rrkr_avg_ms=1e3 * t_rrkr / len(kin["RR->kR"]),
                                krkq_tflops=f_krkq_total / t_krkq / 1e12,
                                krkq_avg_ms=1e3 * t_krkq / len(kin["kR->kq"]),
-                               rrkr_flop_per_launch=f_rrkr, krkq_flop_per_launch=f_krkq_total / len(kin["kR->kq"]))
+                               rrkr_flop_per_launch=f_rrkr, krkq_flop_per_launch=f_krkq_total / len(kin["kR->kq"]),
+                               fused_rotation=fused)
         log(f"[bench] fast outer-k {args.precision}: {results['fast']['value']:.3e} pairs/s; RR->kR "
             f"{results['fast']['rrkr_tflops']:.1f} TFLOP/s ({results['fast']['rrkr_avg_ms']:.2f} ms), kR->kq "
             f"{results['fast']['krkq_tflops']:.1f} TFLOP/s ({results['fast']['krkq_avg_ms']:.2f} ms)")
@@ -300,19 +308,23 @@ def main():
     if "fast" in results:
         r = results["fast"]
         value, ms, steps = r["value"], r["ms_per_step"], r["steps"]
-        # dominant kernel: the larger of the two Fourier launches per step
+        # dominant kernel: the longer of the two Fourier launches per step
+        bf = args.precision == "bf16x3"
         if r["rrkr_avg_ms"] >= r["krkq_avg_ms"]:
-            dom = ("ft kernel, RR->kR Fourier over Re (ndata = nw^2 nm nr_ep, nr = nr_el, nk = k batch)",
-                   r["rrkr_tflops"], r["rrkr_avg_ms"], r["rrkr_flop_per_launch"])
+            dom = (("ft_bf16p_kernel (packed split-BF16)" if bf else "ft_mfma_kernel<float,128>")
+                   + ", RR->kR Fourier over Re (ndata = nw^2 nm nr_ep, nr = nr_el, nk = k batch)",
+                   r["rrkr_tflops"], r["rrkr_avg_ms"], r["rrkr_flop_per_launch"], "RR->kR")
         else:
-            dom = ("batched ft kernel, kR->kq Fourier over Rp (units x kept q)", r["krkq_tflops"],
-                   r["krkq_avg_ms"], r["krkq_flop_per_launch"])
+            dom = (("ft_bf16p_rot_kernel: packed split-BF16 kR->kq Fourier over Rp fused with the k+q/phonon "
+                    "rotations and g2" if r.get("fused_rotation") else
+                    "batched Fourier kernel, kR->kq over Rp") + " (units x kept q)",
+                   r["krkq_tflops"], r["krkq_avg_ms"], r["krkq_flop_per_launch"], "kR->kq")
         path = "outer-k, k-side window projection, k+q window skip"
         pairs_step = args.kbatch_k * nq_total
     else:
         r = results["plain"]
         value, ms, steps = r["value"], r["ms_per_step"], r["steps"]
-        dom = (r["kernel"], r["ft_tflops"], r["ft_avg_ms"], r["ft_flop_per_launch"])
+        dom = (r["kernel"], r["ft_tflops"], r["ft_avg_ms"], r["ft_flop_per_launch"], "Rq->kq")
         path = "outer-q plain (separate Fourier / rotate kernels)"
         pairs_step = nks * args.qtile
     peak = PEAK[args.precision if "fast" in results else "fp32"]
@@ -336,9 +348,9 @@ def main():
                    "plain_outer_q_fp32_pairs_per_s": results.get("plain", {}).get("value")},
         "roofline": {"bound": "mfma", "achieved": dom[1], "peak": peak, "unit": "TFLOP/s",
                      "frac": dom[1] / peak,
-                     "traffic": (RRKR_TRAFFIC_BYTES_KB128 if ("fast" in results and dom[0].startswith("ft kernel, RR->kR")
-                                                              and args.kbatch_k == 128 and args.config == "cfg2")
-                                 else None),
+                     "traffic": (TRAFFIC_BYTES_CFG2_KB128.get((args.precision, dom[4]))
+                                 if ("fast" in results and args.kbatch_k == 128 and args.config == "cfg2"
+                                     and (dom[4] != "kR->kq" or r.get("fused_rotation"))) else None),
                      "kernel": dom[0], "avg_launch_ms": dom[2], "flop_per_launch": dom[3]},
         "cpu_baseline": cpu,
         "detail": results,

@@ -459,10 +459,9 @@ class OuterKLoop:
         # 1. RR->kR Fourier over Re for the batch
         xk_b = self.xk_surv[i0:i0 + n]
         if self.packed:
-            def rrkr():
-                bpk = core.ft_pack_phases(st.irvec_el_dev, xk_b, out=self.bpk_k)
-                core.fourier_packed(self.apk_el, self.nd_el, self.nre, bpk, n, n, self.G, self.G.stride(0))
-            self._timed("RR->kR", 8.0 * self.nd_el * self.nre * n, rrkr)
+            bpk = core.ft_pack_phases(st.irvec_el_dev, xk_b, out=self.bpk_k)
+            self._timed("RR->kR", 8.0 * self.nd_el * self.nre * n, lambda: core.fourier_packed(
+                self.apk_el, self.nd_el, self.nre, bpk, n, n, self.G, self.G.stride(0)))
         else:
             self._timed("RR->kR", 8.0 * self.nd_el * self.nre * n,
                         lambda: core.fourier_batched(self.el_obj, xk_b, out=self.G[:n], precision=self.precision))
@@ -478,13 +477,12 @@ class OuterKLoop:
         # 3. kR->kq batched Fourier over Rp at the kept q
         nd2 = nw * nm
         if self.fused:
-            def krkq_rot():
-                core.ft_pack_operand(self.A, nd2, self.nrp, nd2, nu, nd2 * self.nrp, out=self.apk_units)
-                L.call("ephip_eph_kq_rotate_packed_bf16x3", nw, nm, st.nbw, self.nrp, L.ptr(self.apk_units), nu,
-                       L.ptr(self.bpk_q), self.nq, self.cs, L.ptr(self.count), L.ptr(self.col_index), L.ptr(self.ikq),
-                       L.ptr(st.el.U), L.ptr(st.el.filt.band_min), L.ptr(st.el.filt.band_max), L.ptr(self.uph),
-                       L.ptr(self.omega), L.ptr(self.g2), stream)
-            self._timed("kR->kq", None, krkq_rot)
+            core.ft_pack_operand(self.A, nd2, self.nrp, nd2, nu, nd2 * self.nrp, out=self.apk_units)
+            self._timed("kR->kq", None, lambda: L.call(
+                "ephip_eph_kq_rotate_packed_bf16x3", nw, nm, st.nbw, self.nrp, L.ptr(self.apk_units), nu,
+                L.ptr(self.bpk_q), self.nq, self.cs, L.ptr(self.count), L.ptr(self.col_index), L.ptr(self.ikq),
+                L.ptr(st.el.U), L.ptr(st.el.filt.band_min), L.ptr(st.el.filt.band_max), L.ptr(self.uph),
+                L.ptr(self.omega), L.ptr(self.g2), stream))
         elif self.packed:
             def krkq():
                 core.ft_pack_operand(self.A, nd2, self.nrp, nd2, nu, nd2 * self.nrp, out=self.apk_units)
