@@ -551,31 +551,39 @@ __global__ __launch_bounds__(256) void ft_bf16x3_kernel(int64_t ndata, int nr, c
 //  * operand A (ndata x nr complex, per unit) -> apk, stage-major: for stage s (R = 16 s .. 16 s + 15,
 //    zero past nr) and row d, 64 bf16 at apk[((u*nrs + s)*ndata + d)*64]: re_hi[16] im_hi[16]
 //    re_lo[16] im_lo[16]. A lane's fragment (row l%16, K chunk l/16) is 16 contiguous bytes.
-//  * phases e^{2 pi i R.x_p} of a point table -> bpk: for stage s and point p, 96 bf16 at
-//    bpk[(s*npts + p)*96]: hi = im[16] re[16] -im[16], then lo likewise.
+//  * phases e^{2 pi i R.x_p} of a point table -> bpk: for stage s and point p, 64 bf16 at
+//    bpk[(s*npts + p)*64]: hi = im[16] re[16], then lo likewise.
 //
 // With K = (re of 16 R, im of 16 R) the complex product of one stage is two real products,
 //   out_re += [A_re A_im] [P_re; -P_im],   out_im += [A_re A_im] [P_im; P_re],
-// whose B operands are the 32-element windows at offsets 16 and 0 of a point's table block, so A
-// is never duplicated. Each term is hi*hi + hi*lo + lo*hi (three MFMAs, FP32 accumulation).
-// Workgroup: 4 waves as 2 (rows) x 2 (columns); a wave owns RB 16-row blocks x 4 16-column blocks;
-// columns are staged per stage through double-buffered LDS with a 208-byte row pitch (conflict-free
+// so A is never duplicated. [P_im; P_re] is a point's hi (or lo) block as stored; [P_re; -P_im] is the
+// same block read two 8-element chunks further on, with the sign bits of lanes 32..63 flipped. Each
+// term is hi*hi + hi*lo + lo*hi (three MFMAs, FP32 accumulation).
+// Columns are staged per stage through double-buffered LDS with a 144-byte pitch (conflict-free
 // 16-byte reads), A fragments go straight from global memory to registers.
 // ---------------------------------------------------------------------------------------------
-constexpr int FTP_BLD = 104;  // LDS pitch of one column's 96 bf16 (208 bytes)
+constexpr int FTP_BLD = 72;  // LDS pitch of one column's 64 bf16 (144 bytes)
 
 // Workgroup of 4 waves as WR (rows) x 4/WR (columns); a wave owns RB 16-row blocks x CB 16-column
 // blocks. Rows past ndata and columns past the unit's count read a valid row/column (clamped) and
 // are not stored, so no load is predicated. Offsets are 32-bit against per-stage uniform bases.
 // PF: the next stage's A fragments are loaded into a second register set before the current
 // stage's MFMAs (costs 8 RB registers; without it the loads are issued at the top of each stage).
+// Flip the sign bits selected by mask in each 32-bit pair of bf16 values.
+__device__ __forceinline__ bf16x8 bf16_flip(bf16x8 v, uint32_t mask) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 w = __builtin_bit_cast(u32x4, v);
+    w ^= mask;
+    return __builtin_bit_cast(bf16x8, w);
+}
+
 template <int RB, int CB, int WR>
 struct FtpShape {
     static constexpr int WC = 4 / WR;
     static constexpr int TM = WR * RB * 16;       // complex rows per workgroup
     static constexpr int TN = WC * CB * 16;       // columns per workgroup
-    static constexpr int PPT = TN * 12 / 256;     // 16-byte phase pieces per thread and stage
-    static_assert(TN * 12 % 256 == 0, "phase pieces must split evenly over the workgroup");
+    static constexpr int PPT = TN * 8 / 256;      // 16-byte phase pieces per thread and stage
+    static_assert(TN * 8 % 256 == 0, "phase pieces must split evenly over the workgroup");
 };
 
 // Main loop: accumulates the workgroup tile (rows d0.., columns j0..) of one unit into accr/acci.
@@ -596,12 +604,15 @@ __device__ __forceinline__ void ft_bf16p_mainloop(int64_t ndata, int nrs, const 
     uint32_t bsrc[PPT], bdst[PPT];
 #pragma unroll
     for (int t = 0; t < PPT; t++) {
-        const int e = tid + 256 * t, jl = e / 12, pc = e % 12;
+        const int e = tid + 256 * t, jl = e / 8, pc = e % 8;
         const int64_t j = min(j0 + jl, (int64_t)nk - 1);
-        bsrc[t] = (uint32_t)(col_index ? col_index[j] : j) * 96u + 8u * pc;
+        bsrc[t] = (uint32_t)(col_index ? col_index[j] : j) * 64u + 8u * pc;
         bdst[t] = jl * FTP_BLD + 8 * pc;
     }
-    const int64_t bstage = npts * 96;
+    const int64_t bstage = npts * 64;
+    // [P_re; -P_im] fragment: chunk (c + 2) % 4 of the stored [P_im; P_re], negated on lanes 32..63
+    const int c1 = (c + 2) & 3;
+    const uint32_t nmask = c >= 2 ? 0x80008000u : 0u;
     uint32_t aoff[RB];
 #pragma unroll
     for (int rb = 0; rb < RB; rb++) {
@@ -664,11 +675,11 @@ __device__ __forceinline__ void ft_bf16p_mainloop(int64_t ndata, int nrs, const 
                         *reinterpret_cast<bf16x8 *>(&Bs[(cur ^ 1) * (TN * FTP_BLD) + bdst[t]]) = bq[t];
                 }
             }
-            const __bf16 *col = bb + (wc * (CB * 16) + cb * 16 + il) * FTP_BLD + 8 * c;
-            const bf16x8 b2h = *reinterpret_cast<const bf16x8 *>(col);
-            const bf16x8 b1h = *reinterpret_cast<const bf16x8 *>(col + 16);
-            const bf16x8 b2l = *reinterpret_cast<const bf16x8 *>(col + 48);
-            const bf16x8 b1l = *reinterpret_cast<const bf16x8 *>(col + 64);
+            const __bf16 *col = bb + (wc * (CB * 16) + cb * 16 + il) * FTP_BLD;
+            const bf16x8 b2h = *reinterpret_cast<const bf16x8 *>(col + 8 * c);
+            const bf16x8 b2l = *reinterpret_cast<const bf16x8 *>(col + 32 + 8 * c);
+            const bf16x8 b1h = bf16_flip(*reinterpret_cast<const bf16x8 *>(col + 8 * c1), nmask);
+            const bf16x8 b1l = bf16_flip(*reinterpret_cast<const bf16x8 *>(col + 32 + 8 * c1), nmask);
             // small terms first; consecutive MFMAs update different accumulators
 #pragma unroll
             for (int rb = 0; rb < RB; rb++) {
@@ -704,6 +715,115 @@ __device__ __forceinline__ void ft_bf16p_mainloop(int64_t ndata, int nrs, const 
     }
 }
 
+// Pipelined form of the main loop (unrolled by two, parity register sets, no copies): the phase
+// pieces of stage s + 2 and the operand fragments of stage s + 1 are loaded at the top of stage s, so
+// the only wait in a stage is for loads issued one stage earlier.
+template <int RB, int CB, int WR, bool MID>
+__device__ __forceinline__ void ft_bf16p_mainloop_pipe(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
+                                                       const __bf16 *__restrict__ bpk, int64_t npts, int nk,
+                                                       const int *__restrict__ col_index, int64_t j0, int64_t d0,
+                                                       __bf16 *Bs, f32x4 (&accr)[RB][CB], f32x4 (&acci)[RB][CB]) {
+    using S = FtpShape<RB, CB, WR>;
+    constexpr int TN = S::TN, PPT = S::PPT;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave % WR, wc = wave / WR;
+    const int c = lane >> 4, il = lane & 15;
+
+    uint32_t bsrc[PPT], bdst[PPT];
+#pragma unroll
+    for (int t = 0; t < PPT; t++) {
+        const int e = tid + 256 * t, jl = e / 8, pc = e % 8;
+        const int64_t j = min(j0 + jl, (int64_t)nk - 1);
+        bsrc[t] = (uint32_t)(col_index ? col_index[j] : j) * 64u + 8u * pc;
+        bdst[t] = jl * FTP_BLD + 8 * pc;
+    }
+    const int64_t bstage = npts * 64;
+    const int c1 = (c + 2) & 3;
+    const uint32_t nmask = c >= 2 ? 0x80008000u : 0u;
+    uint32_t aoff[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+        const int64_t d = min(d0 + wr * (RB * 16) + rb * 16 + il, ndata - 1);
+        aoff[rb] = (uint32_t)d * 64u + 8u * c;
+    }
+    const int64_t astage = ndata * 64;
+#pragma unroll
+    for (int i = 0; i < RB; i++)
+#pragma unroll
+        for (int j = 0; j < CB; j++) accr[i][j] = acci[i][j] = f32x4{0, 0, 0, 0};
+
+    auto load_b = [&](int s, bf16x8 (&q)[PPT]) {
+        const __bf16 *bg = bpk + s * bstage;
+#pragma unroll
+        for (int t = 0; t < PPT; t++) q[t] = *reinterpret_cast<const bf16x8 *>(bg + bsrc[t]);
+    };
+    auto store_b = [&](int buf, const bf16x8 (&q)[PPT]) {
+#pragma unroll
+        for (int t = 0; t < PPT; t++) *reinterpret_cast<bf16x8 *>(&Bs[buf * (TN * FTP_BLD) + bdst[t]]) = q[t];
+    };
+    auto load_a = [&](int s, bf16x8 (&h)[RB], bf16x8 (&l)[RB]) {
+        const __bf16 *ag = apk + s * astage;
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+            h[rb] = *reinterpret_cast<const bf16x8 *>(ag + aoff[rb]);
+            l[rb] = *reinterpret_cast<const bf16x8 *>(ag + aoff[rb] + 32);
+        }
+    };
+    // one stage: qn holds the phase pieces of s + 1 (written to the other buffer), qf receives those of
+    // s + 2; (ch, cl) are this stage's operand fragments, (nh, nl) receive those of s + 1
+    auto stage = [&](int s, bf16x8 (&qn)[PPT], bf16x8 (&qf)[PPT], bf16x8 (&ch)[RB], bf16x8 (&cl)[RB],
+                     bf16x8 (&nh)[RB], bf16x8 (&nl)[RB]) {
+        const int cur = s & 1;
+        const bool more = s + 1 < nrs;
+        if (s + 2 < nrs) load_b(s + 2, qf);
+        if (more) load_a(s + 1, nh, nl);
+        const __bf16 *bb = Bs + cur * (TN * FTP_BLD);
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++) {
+            if constexpr (MID) {
+                if (cb == CB / 2 && more) store_b(cur ^ 1, qn);
+            }
+            const __bf16 *col = bb + (wc * (CB * 16) + cb * 16 + il) * FTP_BLD;
+            const bf16x8 b2h = *reinterpret_cast<const bf16x8 *>(col + 8 * c);
+            const bf16x8 b2l = *reinterpret_cast<const bf16x8 *>(col + 32 + 8 * c);
+            const bf16x8 b1h = bf16_flip(*reinterpret_cast<const bf16x8 *>(col + 8 * c1), nmask);
+            const bf16x8 b1l = bf16_flip(*reinterpret_cast<const bf16x8 *>(col + 32 + 8 * c1), nmask);
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cl[rb], b1h, accr[rb][cb], 0, 0, 0);
+                acci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cl[rb], b2h, acci[rb][cb], 0, 0, 0);
+            }
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ch[rb], b1l, accr[rb][cb], 0, 0, 0);
+                acci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ch[rb], b2l, acci[rb][cb], 0, 0, 0);
+            }
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ch[rb], b1h, accr[rb][cb], 0, 0, 0);
+                acci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ch[rb], b2h, acci[rb][cb], 0, 0, 0);
+            }
+        }
+        if constexpr (!MID) {
+            if (more) store_b(cur ^ 1, qn);
+        }
+        __syncthreads();
+    };
+
+    bf16x8 qa[PPT], qb[PPT], xah[RB], xal[RB], xbh[RB], xbl[RB];
+    load_b(0, qa);
+    store_b(0, qa);
+    load_a(0, xah, xal);
+    if (nrs > 1) load_b(1, qb);
+    __syncthreads();
+    int s = 0;
+    for (; s + 1 < nrs; s += 2) {
+        stage(s, qb, qa, xah, xal, xbh, xbl);
+        stage(s + 1, qa, qb, xbh, xbl, xah, xal);
+    }
+    if (s < nrs) stage(s, qb, qa, xah, xal, xbh, xbl);
+}
+
 // XCD grouping (XG): workgroups are dispatched round-robin over the 8 XCDs in linear order (x
 // fastest). Linear id 8 gx a + 8 b + r is given row tile 8 a + r and column tile b, so the gx column
 // tiles of one row tile run on the same XCD and share its L2 copy of the operand rows; the rows past
@@ -722,7 +842,7 @@ __device__ __forceinline__ void xcd_grouped_tile(int64_t &bx, int64_t &by) {
     }
 }
 
-template <int RB, int CB, int WR, bool PF, bool XG = false, bool MID = false>
+template <int RB, int CB, int WR, bool PF, bool XG = false, bool MID = false, bool PIPE = false>
 __device__ __forceinline__ void ft_bf16p_body(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
                                               const __bf16 *__restrict__ bpk, int64_t npts, int nk,
                                               cplx<float> *__restrict__ out, int64_t ldout, int64_t out_unit,
@@ -741,9 +861,14 @@ __device__ __forceinline__ void ft_bf16p_body(int64_t ndata, int nrs, const __bf
     if (j0 >= nk) return;  // uniform for the block
     const int64_t d0 = by * S::TM;
     f32x4 accr[RB][CB], acci[RB][CB];
-    ft_bf16p_mainloop<RB, CB, WR, PF, MID>(ndata, nrs, apk + (int64_t)u * nrs * ndata * 64, bpk, npts, nk,
-                                           col_index ? col_index + u * col_stride : nullptr, j0, d0, Bs, accr,
-                                           acci);
+    if constexpr (PIPE)
+        ft_bf16p_mainloop_pipe<RB, CB, WR, MID>(ndata, nrs, apk + (int64_t)u * nrs * ndata * 64, bpk, npts, nk,
+                                                col_index ? col_index + u * col_stride : nullptr, j0, d0, Bs, accr,
+                                                acci);
+    else
+        ft_bf16p_mainloop<RB, CB, WR, PF, MID>(ndata, nrs, apk + (int64_t)u * nrs * ndata * 64, bpk, npts, nk,
+                                               col_index ? col_index + u * col_stride : nullptr, j0, d0, Bs, accr,
+                                               acci);
     out += u * out_unit;
 #pragma unroll
     for (int rb = 0; rb < RB; rb++)
@@ -761,14 +886,14 @@ __device__ __forceinline__ void ft_bf16p_body(int64_t ndata, int nrs, const __bf
 
 // Two resident waves per SIMD (at most 256 registers) for the shapes that fit without spilling; the
 // 128 x 128 prefetching shape needs more registers and runs one wave per SIMD.
-template <int RB, int CB, int WR, bool PF, bool XG = false, bool MID = false>
+template <int RB, int CB, int WR, bool PF, bool XG = false, bool MID = false, bool PIPE = false>
 __global__ __launch_bounds__(256, 2) void ft_bf16p_kernel(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
                                                           const __bf16 *__restrict__ bpk, int64_t npts, int nk,
                                                           cplx<float> *__restrict__ out, int64_t ldout,
                                                           int64_t out_unit, const int *__restrict__ col_index,
                                                           int64_t col_stride, const int *__restrict__ ncol) {
-    ft_bf16p_body<RB, CB, WR, PF, XG, MID>(ndata, nrs, apk, bpk, npts, nk, out, ldout, out_unit, col_index,
-                                           col_stride, ncol);
+    ft_bf16p_body<RB, CB, WR, PF, XG, MID, PIPE>(ndata, nrs, apk, bpk, npts, nk, out, ldout, out_unit, col_index,
+                                                 col_stride, ncol);
 }
 
 __global__ __launch_bounds__(256) void ft_bf16p_wide_pf_kernel(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
@@ -789,7 +914,7 @@ __global__ __launch_bounds__(256) void ft_bf16p_wide_pf_kernel(int64_t ndata, in
 // ---------------------------------------------------------------------------------------------
 constexpr int FTR_NBW_MAX = 4;
 
-template <int NBW, bool MID = false>
+template <int NBW, bool MID = false, bool PIPE = false>
 __global__ __launch_bounds__(256, 2) void ft_bf16p_rot_kernel(int nrs, const __bf16 *__restrict__ apk,
                                                               const __bf16 *__restrict__ bpk, int64_t npts,
                                                               const int *__restrict__ ncol,
@@ -803,8 +928,9 @@ __global__ __launch_bounds__(256, 2) void ft_bf16p_rot_kernel(int nrs, const __b
                                                               float *__restrict__ g2) {
     constexpr int RB = 3, CB = 4, WR = 4, NW = 16, NM = 12, TN = 64;
     constexpr int64_t NDATA = NW * NM;
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[2 * TN * FTP_BLD];
-    static_assert(sizeof(float) * 2 * TN * NBW * NM <= sizeof(Bs), "ep tile must fit in the staging buffer");
+    // phase staging during the main loop, then the ep tile [jj][m][nu'] (re, im) of the epilogue
+    constexpr int BS_ELEMS = 2 * TN * FTP_BLD > 4 * TN * NBW * NM ? 2 * TN * FTP_BLD : 4 * TN * NBW * NM;
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[BS_ELEMS];
     const int lane = threadIdx.x & 63, wr = threadIdx.x >> 6;
     const int c = lane >> 4, il = lane & 15;
     const int u = blockIdx.z;
@@ -812,8 +938,12 @@ __global__ __launch_bounds__(256, 2) void ft_bf16p_rot_kernel(int nrs, const __b
     const int64_t j0 = (int64_t)blockIdx.x * TN;
     if (j0 >= nk) return;  // uniform for the block
     f32x4 accr[RB][CB], acci[RB][CB];
-    ft_bf16p_mainloop<RB, CB, WR, true, MID>(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk,
-                                             col_index + u * cs, j0, 0, Bs, accr, acci);
+    if constexpr (PIPE)
+        ft_bf16p_mainloop_pipe<RB, CB, WR, MID>(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk,
+                                                col_index + u * cs, j0, 0, Bs, accr, acci);
+    else
+        ft_bf16p_mainloop<RB, CB, WR, true, MID>(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk,
+                                                 col_index + u * cs, j0, 0, Bs, accr, acci);
     // after the main loop's final barrier the staging buffer is free: ep tile [jj][m][nu'] (re, im)
     float *EP = reinterpret_cast<float *>(Bs);
 
@@ -921,7 +1051,7 @@ __global__ void ft_pack_operand_kernel(int64_t ndata, int nr, int nrs, const cpl
     for (int k = 0; k < 8; k++) dst[k] = v[k];
 }
 
-// One thread per (stage, point): 16 phases -> 96 bf16 of the packed phase table.
+// One thread per (stage, point): 16 phases -> 64 bf16 of the packed phase table.
 __global__ void ft_pack_phase_kernel(int nr, const int *__restrict__ irvec, int64_t npts, const double *__restrict__ xk,
                                      int nrs, __bf16 *__restrict__ bpk) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -929,26 +1059,19 @@ __global__ void ft_pack_phase_kernel(int nr, const int *__restrict__ irvec, int6
     const int64_t p = e % npts;
     const int s = (int)(e / npts);
     const double x0 = xk[3 * p], x1 = xk[3 * p + 1], x2 = xk[3 * p + 2];
-    bf16x8 v[12];
+    bf16x8 v[8];
     __bf16 *w = reinterpret_cast<__bf16 *>(v);
 #pragma unroll
     for (int k = 0; k < 16; k++) {
         const int r = 16 * s + k;
         float cr = 0.f, si = 0.f;
         if (r < nr) cis2pi(irvec[3 * r] * x0 + irvec[3 * r + 1] * x1 + irvec[3 * r + 2] * x2, cr, si);
-        __bf16 h, l;
-        split_bf16(si, h, l);
-        w[k] = h;
-        w[48 + k] = l;
-        w[32 + k] = -h;
-        w[80 + k] = -l;
-        split_bf16(cr, h, l);
-        w[16 + k] = h;
-        w[64 + k] = l;
+        split_bf16(si, w[k], w[32 + k]);
+        split_bf16(cr, w[16 + k], w[48 + k]);
     }
-    bf16x8 *dst = reinterpret_cast<bf16x8 *>(bpk + ((int64_t)s * npts + p) * 96);
+    bf16x8 *dst = reinterpret_cast<bf16x8 *>(bpk + ((int64_t)s * npts + p) * 64);
 #pragma unroll
-    for (int k = 0; k < 12; k++) dst[k] = v[k];
+    for (int k = 0; k < 8; k++) dst[k] = v[k];
 }
 
 template <class T>
@@ -1087,7 +1210,7 @@ extern "C" int ephip_fourier_batched_c32(int64_t ndata, int32_t nr, const void *
 }
 
 extern "C" int ephip_ft_set_variant(int32_t kind, int32_t variant) {
-    static const int nvar[5] = {3, 3, 3, 5, 2};
+    static const int nvar[5] = {3, 3, 3, 7, 4};
     EPHIP_REQUIRE(kind >= 0 && kind < 5, "ft_set_variant: kind %d outside 0..4", kind);
     EPHIP_REQUIRE(variant >= 0 && variant < nvar[kind], "ft_set_variant: variant %d outside the range of kind %d",
                   variant, kind);
@@ -1138,7 +1261,7 @@ extern "C" int ephip_fourier_packed_bf16x3(int64_t ndata, int32_t nr, const void
     EPHIP_REQUIRE(nr > 0, "fourier_packed: nr must be positive");
     EPHIP_REQUIRE(apk && bpk && out, "fourier_packed: null device pointer");
     const int nrs = (nr + 15) / 16;
-    EPHIP_REQUIRE(ndata * 64 < (1LL << 32) && npts * 96 < (1LL << 32),
+    EPHIP_REQUIRE(ndata * 64 < (1LL << 32) && npts * 64 < (1LL << 32),
                   "fourier_packed: operand or table too large for 32-bit offsets");
     // Tile shape: 192 x 64 for 192-row operands (one e-ph unit); for operands of 128 rows or more the
     // tuner's variant of kind 3 (0: 128 x 128, 1: 128 x 128 with A prefetch, 2: 192 x 64 when
@@ -1151,8 +1274,9 @@ extern "C" int ephip_fourier_packed_bf16x3(int64_t ndata, int32_t nr, const void
         shape = ((var >= 2) && ndata % 192 == 0) ? 2 : (var == 1 ? 3 : 1);
     else
         shape = 0;
-    const bool grouped = shape == 2 && (var == 3 || var == 4) && nunits == 1;
-    const bool mid = grouped && var == 4;
+    const bool grouped = shape == 2 && var >= 3 && nunits == 1;
+    const bool mid = grouped && (var == 4 || var == 5);
+    const bool pipe = grouped && (var == 5 || var == 6);
     const int tm = shape == 2 ? 192 : (shape == 0 ? 64 : 128), tn = shape == 2 ? 64 : 128;
     EPHIP_REQUIRE((ndata + tm - 1) / tm <= 65535, "fourier_packed: ndata=%lld too large for one call", (long long)ndata);
     dim3 g((unsigned)((nk_max + tn - 1) / tn), (unsigned)((ndata + tm - 1) / tm), (unsigned)nunits);
@@ -1160,7 +1284,13 @@ extern "C" int ephip_fourier_packed_bf16x3(int64_t ndata, int32_t nr, const void
     auto b = static_cast<const __bf16 *>(bpk);
     auto o = static_cast<cplx<float> *>(out);
     hipStream_t st = as_stream(stream);
-    if (mid)
+    if (pipe && mid)
+        ft_bf16p_kernel<3, 4, 4, true, true, true, true><<<g, 256, 0, st>>>(ndata, nrs, a, b, npts, nk_max, o, ldout,
+                                                                             out_stride, col_index, col_stride, ncol);
+    else if (pipe)
+        ft_bf16p_kernel<3, 4, 4, true, true, false, true><<<g, 256, 0, st>>>(ndata, nrs, a, b, npts, nk_max, o, ldout,
+                                                                              out_stride, col_index, col_stride, ncol);
+    else if (mid)
         ft_bf16p_kernel<3, 4, 4, true, true, true><<<g, 256, 0, st>>>(ndata, nrs, a, b, npts, nk_max, o, ldout,
                                                                        out_stride, col_index, col_stride, ncol);
     else if (grouped)
@@ -1191,7 +1321,7 @@ extern "C" int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t
                   nm);
     EPHIP_REQUIRE(nbw >= 1 && nbw <= FTR_NBW_MAX, "eph_kq_rotate_packed: nbw=%d outside 1..%d", nbw, FTR_NBW_MAX);
     EPHIP_REQUIRE(nr > 0 && nunits >= 0 && nunits <= 65535 && cs >= 0 && npts > 0, "eph_kq_rotate_packed: bad sizes");
-    EPHIP_REQUIRE(npts * 96 < (1LL << 32), "eph_kq_rotate_packed: phase table too large for 32-bit offsets");
+    EPHIP_REQUIRE(npts * 64 < (1LL << 32), "eph_kq_rotate_packed: phase table too large for 32-bit offsets");
     if (nunits == 0 || cs == 0) return EPHIP_OK;
     EPHIP_REQUIRE(apk && bpk && count && col_index && ikq && U && bmin && bmax && uph && omega && g2,
                   "eph_kq_rotate_packed: null device pointer");
@@ -1202,8 +1332,18 @@ extern "C" int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t
     auto uu = static_cast<const cplx<float> *>(U);
     auto up = static_cast<const cplx<float> *>(uph);
     hipStream_t st = as_stream(stream);
-    const bool mid = g_ft_variant[4] == 1;
-    if (nbw <= 2 && mid)
+    const int fv = g_ft_variant[4];
+    const bool mid = fv == 1 || fv == 3, pipe = fv >= 2;
+    if (nbw <= 2 && pipe && mid)
+        ft_bf16p_rot_kernel<2, true, true><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax,
+                                                              up, omega, nbw, g2);
+    else if (nbw <= 2 && pipe)
+        ft_bf16p_rot_kernel<2, false, true><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin,
+                                                               bmax, up, omega, nbw, g2);
+    else if (nbw > 2 && pipe)
+        ft_bf16p_rot_kernel<4, false, true><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin,
+                                                               bmax, up, omega, nbw, g2);
+    else if (nbw <= 2 && mid)
         ft_bf16p_rot_kernel<2, true><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax, up,
                                                         omega, nbw, g2);
     else if (nbw <= 2)

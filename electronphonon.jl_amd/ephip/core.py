@@ -201,14 +201,14 @@ def ft_pack_operand(op, ndata: int, nr: int, ldop: int, nunits: int = 1, op_stri
 
 
 def ft_pack_phases(irvec_dev, xk_dev, out=None):
-    """Packed split-BF16 phase table of the points xk_dev: (nrs, npts, 96) bfloat16."""
+    """Packed split-BF16 phase table of the points xk_dev: (nrs, npts, 64) bfloat16."""
     torch = _torch()
     nr, npts = irvec_dev.shape[0], xk_dev.shape[0]
-    shape = (ft_stages(nr), npts, 96)
+    shape = (ft_stages(nr), npts, 64)
     if out is None:
         out = torch.empty(shape, dtype=torch.bfloat16, device=xk_dev.device)
     elif out.dtype != torch.bfloat16 or out.numel() < math.prod(shape) or not out.is_contiguous():
-        raise L.EphipError("ft_pack_phases: out must be a contiguous bfloat16 buffer of (nrs, npts, 96)")
+        raise L.EphipError("ft_pack_phases: out must be a contiguous bfloat16 buffer of (nrs, npts, 64)")
     if xk_dev.dtype != torch.float64 or not xk_dev.is_contiguous():
         raise L.EphipError("k-points must be a contiguous float64 (nk, 3) device tensor")
     L.call("ephip_ft_pack_phases", nr, L.ptr(irvec_dev), npts, L.ptr(xk_dev), L.ptr(out), L.stream_handle())

@@ -402,7 +402,7 @@ class OuterKLoop:
             self.epmat_el = None
             self.el_obj = None
             self.bpk_q = core.ft_pack_phases(self.irvec_ep_dev, st.ph.xq)
-            self.bpk_k = torch.empty(core.ft_stages(self.nre) * self.nk_batch * 96, dtype=torch.bfloat16,
+            self.bpk_k = torch.empty(core.ft_stages(self.nre) * self.nk_batch * 64, dtype=torch.bfloat16,
                                      device=dev)
         # units of every surviving k, in k order
         bmin = st.el.filt.band_min.cpu().numpy()

@@ -151,6 +151,29 @@ class Model:
                           self.g_decay, dtype or np.complex64)
 
 
+def plant_mirror_degeneracy(ham: np.ndarray, irvec: np.ndarray, nw: int) -> np.ndarray:
+    """Symmetrise H(R) (nr, nw*nw column-major blocks) under the antiunitary A = J K combined with the
+    mirror k3 -> -k3, J = i sigma_y (x) 1 on the two halves of the orbitals (J^2 = -1):
+    H(R1, R2, -R3) = J H(R)^T J^{-1}. The map is an involution that preserves H(-R) = H(R)^dagger; on the
+    planes k3 = 0 and k3 = 1/2 the operator A commutes with H(k) and squares to -1, so every band there
+    is exactly two-fold degenerate (Kramers pairs)."""
+    if nw % 2:
+        raise ValueError("planted degeneracies need an even number of Wannier functions")
+    n = nw // 2
+    J = np.zeros((nw, nw))
+    J[:n, n:] = np.eye(n)
+    J[n:, :n] = -np.eye(n)          # i sigma_y (x) 1 = [[0, 1], [-1, 0]] (x) 1
+    Jinv = -J
+    index = {tuple(r): i for i, r in enumerate(irvec.tolist())}
+    out = np.empty_like(ham)
+    for ir, r in enumerate(irvec.tolist()):
+        jm = index[(r[0], r[1], -r[2])]
+        h = ham[ir].reshape(nw, nw, order="F")
+        hm = ham[jm].reshape(nw, nw, order="F")
+        out[ir] = (0.5 * (h + J @ hm.T @ Jinv)).ravel(order="F")
+    return out
+
+
 def hermitian_eph(ep: np.ndarray, irvec: np.ndarray, nw: int, nm: int) -> np.ndarray:
     """Symmetrise a "ph"-layout e-ph tensor (Re and Rp on the same box ``irvec``) so that the vertex is
     the matrix element of a Hermitian perturbation: g_ij,nu(Re, Rp) = conj(g_ji,nu(-Re, Rp - Re)),
@@ -175,17 +198,20 @@ def hermitian_eph(ep: np.ndarray, irvec: np.ndarray, nw: int, nm: int) -> np.nda
 def make_model(nw: int, natoms: int, box_hd: int, box_g: int, seed: int,
                onsite=(-2.0, 2.0), g_dtype=np.complex128, with_g: bool = True,
                h_decay: float = 1.0, g_decay: float = 0.8, hermitian_g: bool = False,
-               time_reversal: bool = False) -> Model:
+               time_reversal: bool = False, planted_degeneracy: bool = False) -> Model:
     """Build a model. ``box_hd`` is the R-box edge of H and D, ``box_g`` that of both g boxes.
     ``hermitian_g`` symmetrises the e-ph tensor (``hermitian_eph``), which balances scattering in and
     out in the linearised BTE. ``time_reversal`` keeps the real parts of H(R) and of the symmetrised
     g (so E(k) = E(-k), v(-k) = -v(k)); the BTE right-hand side is then odd in k and the fixed-point
-    iteration converges."""
+    iteration converges. ``planted_degeneracy`` makes every band exactly two-fold degenerate on the
+    planes k3 = 0 and k3 = 1/2 (``plant_mirror_degeneracy``)."""
     hermitian_g = hermitian_g or time_reversal
     irvec_h = rbox(box_hd)
     ham = hermitian_operator(seed, nw, irvec_h, h_decay, onsite, key=1)
     if time_reversal:
         ham = ham.real.astype(ham.dtype)
+    if planted_degeneracy:
+        ham = plant_mirror_degeneracy(ham, irvec_h, nw)
     dyn = dynamical_matrix(seed, natoms, irvec_h).astype(np.complex128)
     nm = 3 * natoms
     irvec_g = rbox(box_g)
@@ -209,6 +235,9 @@ CONFIGS = {
     "cfg1": dict(nw=8, natoms=2, box_hd=5, box_g=5, seed=1001, onsite=(-2.0, 2.0),
                  g_dtype=np.complex128, kgrid=(8, 8, 8), qgrid=(8, 8, 8), window=1.0),
     "cfg2": dict(nw=16, natoms=4, box_hd=9, box_g=7, seed=1002, onsite=(-2.0, 2.0),
+                 g_dtype=np.complex64, kgrid=(48, 48, 48), qgrid=(24, 24, 24), window=0.3),
+    # cfg2's model with exact two-fold degeneracies planted on the planes k3 = 0, 1/2 (precision form)
+    "cfg3": dict(nw=16, natoms=4, box_hd=9, box_g=7, seed=1002, onsite=(-2.0, 2.0), planted_degeneracy=True,
                  g_dtype=np.complex64, kgrid=(48, 48, 48), qgrid=(24, 24, 24), window=0.3),
     "cfg4": dict(nw=16, natoms=4, box_hd=9, box_g=7, seed=1004, onsite=(-3.0, 3.0),
                  g_dtype=np.complex64, kgrid=(64, 64, 64), qgrid=(32, 32, 32), window=0.3),

@@ -185,8 +185,8 @@ int ephip_eph_rotate_kq_g2(int32_t nw, int32_t nm, int32_t nbw, int32_t nunits, 
  *    op + u*op_stride) -> apk, bf16, nunits*nrs*ndata*64 values, nrs = ceil(nr/16): for stage s and row
  *    d, 64 values at ((u*nrs + s)*ndata + d)*64 = re_hi[16] im_hi[16] re_lo[16] im_lo[16] of R 16s..16s+15
  *    (zero past nr).
- *  - ephip_ft_pack_phases: exp(2 pi i R.x_p) for npts points -> bpk, bf16, nrs*npts*96 values: for stage s
- *    and point p, 96 values at (s*npts + p)*96 = hi{im[16], re[16], -im[16]} lo{...}.
+ *  - ephip_ft_pack_phases: exp(2 pi i R.x_p) for npts points -> bpk, bf16, nrs*npts*64 values: for stage s
+ *    and point p, 64 values at (s*npts + p)*64 = hi{im[16], re[16]} lo{im[16], re[16]}.
  *  - ephip_fourier_packed_bf16x3: out[d + ldout*j] (+ u*out_stride) = sum_r op_u[d, r] e^{2 pi i R_r.x_p},
  *    p = col_index[u*col_stride + j] (or j, then nk_max <= npts), j < ncol[u] (or nk_max); FP32
  *    accumulation of hi*hi + hi*lo + lo*hi on 16x16x32 bf16 MFMAs. */
@@ -216,8 +216,10 @@ int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t nbw, int32
  * variant: 0 = 128 x 128 tiles, 1 = 128 x 128 with the next stage's operand prefetched into registers,
  * 2 = 192 x 64 tiles (ndata a multiple of 192), 3 = 192 x 64 tiles in XCD-grouped workgroup order (the
  * column tiles of a row tile on one XCD, sharing its L2; single-unit calls), 4 = as 3 with the next
- * stage's phase pieces written to LDS halfway through the MFMA sequence. kind 4 = the fused kR->kq
- * rotation kernel; variant 0 = phase pieces written after the MFMA sequence, 1 = halfway through it.
+ * stage's phase pieces written to LDS halfway through the MFMA sequence, 5 = as 4 with the pipelined
+ * main loop (phase pieces two stages ahead, operand fragments one stage ahead, no register copies),
+ * 6 = as 3 pipelined. kind 4 = the fused kR->kq rotation kernel; variant 0 = phase pieces written
+ * after the MFMA sequence, 1 = halfway through it, 2 = pipelined, 3 = pipelined and halfway.
  * Process-wide setting (no counterpart in the reference, whose tuning constants are hard-coded:
  * src/wannier_to_bloch_batched.jl:397-406). Host-side, synchronous. */
 int ephip_ft_set_variant(int32_t kind, int32_t variant);

@@ -340,7 +340,7 @@ def test_fourier_packed_argument_checks(backend):
                L.stream_handle())
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
 def test_fourier_packed_variants(backend, variant):
     """Every tile shape of the packed split-BF16 kernel (kind 3 of the tuner) on an operand whose row
     count suits all three (384 = 3 x 128 = 2 x 192)."""
