@@ -904,6 +904,138 @@ __global__ __launch_bounds__(256) void ft_bf16p_wide_pf_kernel(int64_t ndata, in
     ft_bf16p_body<4, 4, 2, true>(ndata, nrs, apk, bpk, npts, nk, out, ldout, out_unit, col_index, col_stride, ncol);
 }
 
+// Main loop of the 192 x 64 unit tile with the phases of a uniform point grid generated on the fly
+// instead of gathered from a packed table: point p = (i1, i2, i3) of an n1 x n2 x n3 grid (third index
+// fastest, no shift) has e^{2 pi i R.x_p} = prod_a t_a[(R_a i_a) mod n_a], t_a[m] = e^{2 pi i m / n_a},
+// an exact integer reduction and two FP32 complex products. Lane l of wave w generates column l's
+// phases for R slots 2w, 2w+1, 8+2w, 9+2w of each stage and writes them split into the LDS block.
+constexpr int FTG_NMAX = 64;  // largest grid dimension of the on-the-fly phases
+
+__device__ __forceinline__ void ftg_phase(const float2 *tab, const int *__restrict__ R, const int i[3],
+                                          const int n[3], float &c, float &s) {
+    float2 p[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int m = (R[a] * i[a]) % n[a];
+        m += m < 0 ? n[a] : 0;
+        p[a] = tab[a * FTG_NMAX + m];
+    }
+    const float r01 = p[0].x * p[1].x - p[0].y * p[1].y, i01 = p[0].x * p[1].y + p[0].y * p[1].x;
+    c = r01 * p[2].x - i01 * p[2].y;
+    s = r01 * p[2].y + i01 * p[2].x;
+}
+
+template <int RB, int CB, int WR>
+__device__ __forceinline__ void ft_bf16p_mainloop_gp(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
+                                                     int nr, const int *__restrict__ irvec, const int n[3],
+                                                     const float2 *tab, int nk, const int *__restrict__ col_index,
+                                                     int64_t j0, int64_t d0, __bf16 *Bs, f32x4 (&accr)[RB][CB],
+                                                     f32x4 (&acci)[RB][CB]) {
+    using S = FtpShape<RB, CB, WR>;
+    constexpr int TN = S::TN;
+    static_assert(TN == 64 && WR == 4, "on-the-fly phases assume 64 columns and 4 waves");
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave % WR, wc = wave / WR;
+    const int c = lane >> 4, il = lane & 15;
+    const int c1 = (c + 2) & 3;
+    const uint32_t nmask = c >= 2 ? 0x80008000u : 0u;
+    // grid coordinates of this lane's column (columns past the count reuse the last one)
+    int gi[3];
+    {
+        const int p = col_index ? col_index[min(j0 + lane, (int64_t)nk - 1)] : (int)min(j0 + lane, (int64_t)nk - 1);
+        gi[0] = p / (n[1] * n[2]);
+        gi[1] = (p / n[2]) % n[1];
+        gi[2] = p % n[2];
+    }
+    const int rs[2] = {2 * wave, 8 + 2 * wave};
+    auto gen = [&](int s, int buf) {
+        __bf16 *col = Bs + buf * (TN * FTP_BLD) + lane * FTP_BLD;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int r0 = 16 * s + rs[h];
+            float c0 = 0.f, s0 = 0.f, cc1 = 0.f, ss1 = 0.f;
+            if (r0 < nr) ftg_phase(tab, irvec + 3 * r0, gi, n, c0, s0);
+            if (r0 + 1 < nr) ftg_phase(tab, irvec + 3 * (r0 + 1), gi, n, cc1, ss1);
+            __bf16 sh0, sl0, sh1, sl1, ch0, cl0, ch1, cl1;
+            split_bf16(s0, sh0, sl0);
+            split_bf16(ss1, sh1, sl1);
+            split_bf16(c0, ch0, cl0);
+            split_bf16(cc1, ch1, cl1);
+            const int r = rs[h];
+            *reinterpret_cast<bf16x2 *>(col + r) = bf16x2{sh0, sh1};
+            *reinterpret_cast<bf16x2 *>(col + 16 + r) = bf16x2{ch0, ch1};
+            *reinterpret_cast<bf16x2 *>(col + 32 + r) = bf16x2{sl0, sl1};
+            *reinterpret_cast<bf16x2 *>(col + 48 + r) = bf16x2{cl0, cl1};
+        }
+    };
+    uint32_t aoff[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+        const int64_t d = min(d0 + wr * (RB * 16) + rb * 16 + il, ndata - 1);
+        aoff[rb] = (uint32_t)d * 64u + 8u * c;
+    }
+    const int64_t astage = ndata * 64;
+#pragma unroll
+    for (int i = 0; i < RB; i++)
+#pragma unroll
+        for (int j = 0; j < CB; j++) accr[i][j] = acci[i][j] = f32x4{0, 0, 0, 0};
+
+    bf16x8 ah[RB], al[RB];
+    gen(0, 0);
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+        ah[rb] = *reinterpret_cast<const bf16x8 *>(apk + aoff[rb]);
+        al[rb] = *reinterpret_cast<const bf16x8 *>(apk + aoff[rb] + 32);
+    }
+    __syncthreads();
+    for (int s = 0; s < nrs; s++) {
+        const int cur = s & 1;
+        const bool more = s + 1 < nrs;
+        bf16x8 nh[RB], nl[RB];
+        if (more) {
+            const __bf16 *ag = apk + (s + 1) * astage;
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                nh[rb] = *reinterpret_cast<const bf16x8 *>(ag + aoff[rb]);
+                nl[rb] = *reinterpret_cast<const bf16x8 *>(ag + aoff[rb] + 32);
+            }
+            gen(s + 1, cur ^ 1);
+        }
+        const __bf16 *bb = Bs + cur * (TN * FTP_BLD);
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++) {
+            const __bf16 *col = bb + (wc * (CB * 16) + cb * 16 + il) * FTP_BLD;
+            const bf16x8 b2h = *reinterpret_cast<const bf16x8 *>(col + 8 * c);
+            const bf16x8 b2l = *reinterpret_cast<const bf16x8 *>(col + 32 + 8 * c);
+            const bf16x8 b1h = bf16_flip(*reinterpret_cast<const bf16x8 *>(col + 8 * c1), nmask);
+            const bf16x8 b1l = bf16_flip(*reinterpret_cast<const bf16x8 *>(col + 32 + 8 * c1), nmask);
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rb], b1h, accr[rb][cb], 0, 0, 0);
+                acci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rb], b2h, acci[rb][cb], 0, 0, 0);
+            }
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b1l, accr[rb][cb], 0, 0, 0);
+                acci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b2l, acci[rb][cb], 0, 0, 0);
+            }
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b1h, accr[rb][cb], 0, 0, 0);
+                acci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b2h, acci[rb][cb], 0, 0, 0);
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                ah[rb] = nh[rb];
+                al[rb] = nl[rb];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // kR->kq Fourier over Rp fused with the k+q-window and phonon rotations and g2, for the e-ph unit
 // shape nw = 16, nm = 12 (operand rows d = i + 16 nu): the 192 x 64 tile puts modes 3w..3w+2 of all
@@ -914,7 +1046,7 @@ __global__ __launch_bounds__(256) void ft_bf16p_wide_pf_kernel(int64_t ndata, in
 // ---------------------------------------------------------------------------------------------
 constexpr int FTR_NBW_MAX = 4;
 
-template <int NBW, bool MID = false, bool PIPE = false>
+template <int NBW, bool MID = false, bool PIPE = false, bool GP = false>
 __global__ __launch_bounds__(256, 2) void ft_bf16p_rot_kernel(int nrs, const __bf16 *__restrict__ apk,
                                                               const __bf16 *__restrict__ bpk, int64_t npts,
                                                               const int *__restrict__ ncol,
@@ -925,7 +1057,8 @@ __global__ __launch_bounds__(256, 2) void ft_bf16p_rot_kernel(int nrs, const __b
                                                               const int64_t *__restrict__ bmax,
                                                               const cplx<float> *__restrict__ uph,
                                                               const double *__restrict__ omega, int nbw,
-                                                              float *__restrict__ g2) {
+                                                              float *__restrict__ g2, int nr,
+                                                              const int *__restrict__ irvec, int3 qgrid) {
     constexpr int RB = 3, CB = 4, WR = 4, NW = 16, NM = 12, TN = 64;
     constexpr int64_t NDATA = NW * NM;
     // phase staging during the main loop, then the ep tile [jj][m][nu'] (re, im) of the epilogue
@@ -938,7 +1071,19 @@ __global__ __launch_bounds__(256, 2) void ft_bf16p_rot_kernel(int nrs, const __b
     const int64_t j0 = (int64_t)blockIdx.x * TN;
     if (j0 >= nk) return;  // uniform for the block
     f32x4 accr[RB][CB], acci[RB][CB];
-    if constexpr (PIPE)
+    if constexpr (GP) {
+        __shared__ float2 tab[3 * FTG_NMAX];
+        const int n[3] = {qgrid.x, qgrid.y, qgrid.z};
+        for (int e = threadIdx.x; e < 3 * FTG_NMAX; e += blockDim.x) {
+            const int a = e / FTG_NMAX, m = e % FTG_NMAX;
+            float cr = 0.f, si = 0.f;
+            if (m < n[a]) cis2pi((double)m / n[a], cr, si);
+            tab[e] = make_float2(cr, si);
+        }
+        __syncthreads();
+        ft_bf16p_mainloop_gp<RB, CB, WR>(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, nr, irvec, n, tab, nk,
+                                         col_index + u * cs, j0, 0, Bs, accr, acci);
+    } else if constexpr (PIPE)
         ft_bf16p_mainloop_pipe<RB, CB, WR, MID>(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk,
                                                 col_index + u * cs, j0, 0, Bs, accr, acci);
     else
@@ -1210,7 +1355,7 @@ extern "C" int ephip_fourier_batched_c32(int64_t ndata, int32_t nr, const void *
 }
 
 extern "C" int ephip_ft_set_variant(int32_t kind, int32_t variant) {
-    static const int nvar[5] = {3, 3, 3, 7, 4};
+    static const int nvar[5] = {3, 3, 3, 7, 5};
     EPHIP_REQUIRE(kind >= 0 && kind < 5, "ft_set_variant: kind %d outside 0..4", kind);
     EPHIP_REQUIRE(variant >= 0 && variant < nvar[kind], "ft_set_variant: variant %d outside the range of kind %d",
                   variant, kind);
@@ -1316,7 +1461,8 @@ extern "C" int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t
                                                  int32_t nunits, const void *bpk, int64_t npts, int64_t cs,
                                                  const int32_t *count, const int32_t *col_index, const int32_t *ikq,
                                                  const void *U, const int64_t *bmin, const int64_t *bmax,
-                                                 const void *uph, const double *omega, float *g2, void *stream) {
+                                                 const void *uph, const double *omega, float *g2,
+                                                 const int32_t *irvec, const int32_t *qgrid, void *stream) {
     EPHIP_REQUIRE(nw == 16 && nm == 12, "eph_kq_rotate_packed: fused form needs nw = 16 and nm = 12 (got %d, %d)", nw,
                   nm);
     EPHIP_REQUIRE(nbw >= 1 && nbw <= FTR_NBW_MAX, "eph_kq_rotate_packed: nbw=%d outside 1..%d", nbw, FTR_NBW_MAX);
@@ -1333,28 +1479,50 @@ extern "C" int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t
     auto up = static_cast<const cplx<float> *>(uph);
     hipStream_t st = as_stream(stream);
     const int fv = g_ft_variant[4];
-    const bool mid = fv == 1 || fv == 3, pipe = fv >= 2;
+    const bool gp = fv == 4;
+    const bool mid = fv == 1 || fv == 3, pipe = fv == 2 || fv == 3;
+    int3 qg = make_int3(0, 0, 0);
+    if (gp) {
+        EPHIP_REQUIRE(irvec && qgrid, "eph_kq_rotate_packed: on-the-fly phases need irvec and the q-grid");
+        EPHIP_REQUIRE(qgrid[0] >= 1 && qgrid[1] >= 1 && qgrid[2] >= 1 && qgrid[0] <= FTG_NMAX &&
+                          qgrid[1] <= FTG_NMAX && qgrid[2] <= FTG_NMAX &&
+                          (int64_t)qgrid[0] * qgrid[1] * qgrid[2] == npts,
+                      "eph_kq_rotate_packed: q-grid dimensions must be 1..%d with n1 n2 n3 = npts", FTG_NMAX);
+        qg = make_int3(qgrid[0], qgrid[1], qgrid[2]);
+    }
+    if (gp && nbw <= 2) {
+        ft_bf16p_rot_kernel<2, false, false, true><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu,
+                                                                      bmin, bmax, up, omega, nbw, g2, nr, irvec, qg);
+        EPHIP_LAUNCHED();
+        return EPHIP_OK;
+    }
+    if (gp) {
+        ft_bf16p_rot_kernel<4, false, false, true><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu,
+                                                                      bmin, bmax, up, omega, nbw, g2, nr, irvec, qg);
+        EPHIP_LAUNCHED();
+        return EPHIP_OK;
+    }
     if (nbw <= 2 && pipe && mid)
         ft_bf16p_rot_kernel<2, true, true><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax,
-                                                              up, omega, nbw, g2);
+                                                              up, omega, nbw, g2, nr, irvec, qg);
     else if (nbw <= 2 && pipe)
         ft_bf16p_rot_kernel<2, false, true><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin,
-                                                               bmax, up, omega, nbw, g2);
+                                                               bmax, up, omega, nbw, g2, nr, irvec, qg);
     else if (nbw > 2 && pipe)
         ft_bf16p_rot_kernel<4, false, true><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin,
-                                                               bmax, up, omega, nbw, g2);
+                                                               bmax, up, omega, nbw, g2, nr, irvec, qg);
     else if (nbw <= 2 && mid)
         ft_bf16p_rot_kernel<2, true><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax, up,
-                                                        omega, nbw, g2);
+                                                        omega, nbw, g2, nr, irvec, qg);
     else if (nbw <= 2)
         ft_bf16p_rot_kernel<2><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax, up, omega,
-                                                  nbw, g2);
+                                                  nbw, g2, nr, irvec, qg);
     else if (mid)
         ft_bf16p_rot_kernel<4, true><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax, up,
-                                                        omega, nbw, g2);
+                                                        omega, nbw, g2, nr, irvec, qg);
     else
         ft_bf16p_rot_kernel<4><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax, up, omega,
-                                                  nbw, g2);
+                                                  nbw, g2, nr, irvec, qg);
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

@@ -397,6 +397,8 @@ class OuterKLoop:
         # split-BF16 with nw = 16, nm = 12, nbw <= 4: kR->kq and the rotations run as one kernel that
         # writes g2 only (no g(k, q) staging buffer)
         self.fused = bool(fuse_rotation) and self.packed and nw == 16 and nm == 12 and st.nbw <= 4
+        import ctypes
+        self.qgrid_c = (ctypes.c_int32 * 3)(*map(int, st.qgrid))  # for the on-the-fly q phases
         if self.packed:
             self.apk_el = core.ft_pack_operand(self.epmat_el, self.nd_el, self.nre, self.nd_el)
             self.epmat_el = None
@@ -482,7 +484,7 @@ class OuterKLoop:
                 "ephip_eph_kq_rotate_packed_bf16x3", nw, nm, st.nbw, self.nrp, L.ptr(self.apk_units), nu,
                 L.ptr(self.bpk_q), self.nq, self.cs, L.ptr(self.count), L.ptr(self.col_index), L.ptr(self.ikq),
                 L.ptr(st.el.U), L.ptr(st.el.filt.band_min), L.ptr(st.el.filt.band_max), L.ptr(self.uph),
-                L.ptr(self.omega), L.ptr(self.g2), stream))
+                L.ptr(self.omega), L.ptr(self.g2), L.ptr(self.irvec_ep_dev), C.addressof(self.qgrid_c), stream))
         elif self.packed:
             def krkq():
                 core.ft_pack_operand(self.A, nd2, self.nrp, nd2, nu, nd2 * self.nrp, out=self.apk_units)

@@ -17,7 +17,7 @@ VARIANT_NAMES = {0: "single-buffered KC16", 1: "double-buffered KC16", 2: "doubl
 PACKED_VARIANTS = (0, 1, 2, 3, 4, 5, 6)
 PACKED_VARIANT_NAMES = {0: "128x128", 1: "128x128 + operand prefetch", 2: "192x64", 3: "192x64 XCD-grouped",
                         4: "192x64 XCD-grouped, mid-sequence LDS writes", 5: "as 4, pipelined", 6: "as 3, pipelined"}
-FUSED_VARIANTS = (0, 1, 2, 3)
+FUSED_VARIANTS = (0, 1, 2, 3, 4)
 
 
 def get_variant(kind: int) -> int:

@@ -202,12 +202,15 @@ int ephip_fourier_packed_bf16x3(int64_t ndata, int32_t nr, const void *apk, int3
  * 245-273 and :442-455 in one kernel) for nw = 16, nm = 12, nbw <= 4, split-BF16: apk holds the
  * nunits packed unit operands (rows i + 16 nu, nr = nr_ep), bpk the packed q-grid phase table of npts
  * points; count/col_index/ikq (row stride cs) are the kept-q lists of ephip_kq_compact; U, bmin, bmax,
- * uph, omega as in ephip_eph_rotate_kq_g2. Writes g2 (nbw x nm per kept (unit, q), [m, nu]) only. */
+ * uph, omega as in ephip_eph_rotate_kq_g2. Writes g2 (nbw x nm per kept (unit, q), [m, nu]) only.
+ * irvec (int32 nr x 3, the Rp list) and qgrid (3 ints, n1 n2 n3 = npts, third index fastest, no shift)
+ * are read by the tuner variant that generates the phases on the fly (kind 4, variant 4); they may be
+ * NULL otherwise. */
 int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t nbw, int32_t nr, const void *apk,
                                       int32_t nunits, const void *bpk, int64_t npts, int64_t cs, const int32_t *count,
                                       const int32_t *col_index, const int32_t *ikq, const void *U, const int64_t *bmin,
                                       const int64_t *bmax, const void *uph, const double *omega, float *g2,
-                                      void *stream);
+                                      const int32_t *irvec, const int32_t *qgrid, void *stream);
 
 /* Kernel variant of the Fourier contractions, chosen by the tuner (ephip/tune.py) from timings on the
  * live shapes. kind: 0 = FP32 with nk > 64, 1 = FP32 with nk <= 64, 2 = FP64; variant: 0 = single-
@@ -219,7 +222,8 @@ int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t nbw, int32
  * stage's phase pieces written to LDS halfway through the MFMA sequence, 5 = as 4 with the pipelined
  * main loop (phase pieces two stages ahead, operand fragments one stage ahead, no register copies),
  * 6 = as 3 pipelined. kind 4 = the fused kR->kq rotation kernel; variant 0 = phase pieces written
- * after the MFMA sequence, 1 = halfway through it, 2 = pipelined, 3 = pipelined and halfway.
+ * after the MFMA sequence, 1 = halfway through it, 2 = pipelined, 3 = pipelined and halfway, 4 = phases
+ * of the uniform q-grid generated on the fly from per-axis tables (no phase-table reads).
  * Process-wide setting (no counterpart in the reference, whose tuning constants are hard-coded:
  * src/wannier_to_bloch_batched.jl:397-406). Host-side, synchronous. */
 int ephip_ft_set_variant(int32_t kind, int32_t variant);

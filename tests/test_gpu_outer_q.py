@@ -147,7 +147,7 @@ def model16():
     return m, {"kgrid": (8, 8, 8), "qgrid": (4, 4, 4), "window": 0.5}
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_outer_k_fused_rotation(backend, model16, variant):
     """Split-BF16 outer-k with the kR->kq Fourier fused to the rotations (nw 16, nm 12), both kernel
     variants: same kept-q lists as the unfused kernels, g2 within 1e-5 of them, and within 1e-4 of the
