@@ -842,6 +842,138 @@ __device__ __forceinline__ void xcd_grouped_tile(int64_t &bx, int64_t &by) {
     }
 }
 
+// Eight-wave main loop: 512 threads as WR = 4 (rows) x WC = 2 (columns), tile 192 x 128. The
+// operand tile of a stage (192 rows x 64 bf16, contiguous in the stage-major layout) is staged once
+// into LDS with coalesced 16-byte loads and read by both column halves, so operand traffic into the
+// CU per column is half that of the 4-wave 192 x 64 tile. Operand rows and phase columns use the
+// conflict-free 144-byte pitch; both are double-buffered, one barrier per stage.
+constexpr int FT8_ROWS = 192, FT8_COLS = 128;
+
+__device__ __forceinline__ void ft8_mainloop(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
+                                             const __bf16 *__restrict__ bpk, int64_t npts, int nk,
+                                             const int *__restrict__ col_index, int64_t j0, int64_t d0, __bf16 *As,
+                                             __bf16 *Bs, f32x4 (&accr)[3][4], f32x4 (&acci)[3][4]) {
+    constexpr int RB = 3, CB = 4, TN = FT8_COLS, TM = FT8_ROWS;
+    constexpr int APT = TM * 8 / 512, BPT = TN * 8 / 512;  // 16-byte pieces per thread and stage: 3, 2
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave & 3, wc = wave >> 2;
+    const int c = lane >> 4, il = lane & 15;
+    const int c1 = (c + 2) & 3;
+    const uint32_t nmask = c >= 2 ? 0x80008000u : 0u;
+    uint32_t asrc[APT], adst[APT], bsrc[BPT], bdst[BPT];
+#pragma unroll
+    for (int t = 0; t < APT; t++) {
+        const int e = tid + 512 * t, row = e / 8, pc = e % 8;
+        asrc[t] = (uint32_t)min(d0 + row, ndata - 1) * 64u + 8u * pc;
+        adst[t] = row * FTP_BLD + 8 * pc;
+    }
+#pragma unroll
+    for (int t = 0; t < BPT; t++) {
+        const int e = tid + 512 * t, jl = e / 8, pc = e % 8;
+        const int64_t j = min(j0 + jl, (int64_t)nk - 1);
+        bsrc[t] = (uint32_t)(col_index ? col_index[j] : j) * 64u + 8u * pc;
+        bdst[t] = jl * FTP_BLD + 8 * pc;
+    }
+    const int64_t astage = ndata * 64, bstage = npts * 64;
+#pragma unroll
+    for (int i = 0; i < RB; i++)
+#pragma unroll
+        for (int j = 0; j < CB; j++) accr[i][j] = acci[i][j] = f32x4{0, 0, 0, 0};
+    bf16x8 qa[APT], qb[BPT];
+#pragma unroll
+    for (int t = 0; t < APT; t++) qa[t] = *reinterpret_cast<const bf16x8 *>(apk + asrc[t]);
+#pragma unroll
+    for (int t = 0; t < BPT; t++) qb[t] = *reinterpret_cast<const bf16x8 *>(bpk + bsrc[t]);
+#pragma unroll
+    for (int t = 0; t < APT; t++) *reinterpret_cast<bf16x8 *>(&As[adst[t]]) = qa[t];
+#pragma unroll
+    for (int t = 0; t < BPT; t++) *reinterpret_cast<bf16x8 *>(&Bs[bdst[t]]) = qb[t];
+    __syncthreads();
+    for (int s = 0; s < nrs; s++) {
+        const int cur = s & 1;
+        const bool more = s + 1 < nrs;
+        if (more) {
+            const __bf16 *ag = apk + (s + 1) * astage;
+            const __bf16 *bg = bpk + (s + 1) * bstage;
+#pragma unroll
+            for (int t = 0; t < APT; t++) qa[t] = *reinterpret_cast<const bf16x8 *>(ag + asrc[t]);
+#pragma unroll
+            for (int t = 0; t < BPT; t++) qb[t] = *reinterpret_cast<const bf16x8 *>(bg + bsrc[t]);
+        }
+        const __bf16 *aa = As + cur * (TM * FTP_BLD);
+        bf16x8 ah[RB], al[RB];
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+            const __bf16 *row = aa + (wr * (RB * 16) + rb * 16 + il) * FTP_BLD + 8 * c;
+            ah[rb] = *reinterpret_cast<const bf16x8 *>(row);
+            al[rb] = *reinterpret_cast<const bf16x8 *>(row + 32);
+        }
+        const __bf16 *bb = Bs + cur * (TN * FTP_BLD);
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++) {
+            if (cb == CB / 2 && more) {
+#pragma unroll
+                for (int t = 0; t < APT; t++)
+                    *reinterpret_cast<bf16x8 *>(&As[(cur ^ 1) * (TM * FTP_BLD) + adst[t]]) = qa[t];
+#pragma unroll
+                for (int t = 0; t < BPT; t++)
+                    *reinterpret_cast<bf16x8 *>(&Bs[(cur ^ 1) * (TN * FTP_BLD) + bdst[t]]) = qb[t];
+            }
+            const __bf16 *col = bb + (wc * (CB * 16) + cb * 16 + il) * FTP_BLD;
+            const bf16x8 b2h = *reinterpret_cast<const bf16x8 *>(col + 8 * c);
+            const bf16x8 b2l = *reinterpret_cast<const bf16x8 *>(col + 32 + 8 * c);
+            const bf16x8 b1h = bf16_flip(*reinterpret_cast<const bf16x8 *>(col + 8 * c1), nmask);
+            const bf16x8 b1l = bf16_flip(*reinterpret_cast<const bf16x8 *>(col + 32 + 8 * c1), nmask);
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rb], b1h, accr[rb][cb], 0, 0, 0);
+                acci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rb], b2h, acci[rb][cb], 0, 0, 0);
+            }
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b1l, accr[rb][cb], 0, 0, 0);
+                acci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b2l, acci[rb][cb], 0, 0, 0);
+            }
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b1h, accr[rb][cb], 0, 0, 0);
+                acci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b2h, acci[rb][cb], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// RR->kR-type contraction with the eight-wave 192 x 128 tile (single unit, XCD-grouped order).
+__global__ __launch_bounds__(512, 2) void ft_bf16p8_kernel(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
+                                                           const __bf16 *__restrict__ bpk, int64_t npts, int nk,
+                                                           cplx<float> *__restrict__ out, int64_t ldout) {
+    __shared__ __attribute__((aligned(16))) __bf16 As[2 * FT8_ROWS * FTP_BLD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[2 * FT8_COLS * FTP_BLD];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave & 3, wc = wave >> 2;
+    const int c = lane >> 4, il = lane & 15;
+    int64_t bx, by;
+    xcd_grouped_tile(bx, by);
+    const int64_t j0 = bx * FT8_COLS;
+    if (j0 >= nk) return;  // uniform for the block
+    const int64_t d0 = by * FT8_ROWS;
+    f32x4 accr[3][4], acci[3][4];
+    ft8_mainloop(ndata, nrs, apk, bpk, npts, nk, nullptr, j0, d0, As, Bs, accr, acci);
+#pragma unroll
+    for (int rb = 0; rb < 3; rb++)
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++) {
+            const int64_t j = j0 + wc * 64 + cb * 16 + il;
+            if (j >= nk) continue;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int64_t d = d0 + wr * 48 + rb * 16 + 4 * c + i;
+                if (d < ndata) out[d + ldout * j] = cplx<float>{accr[rb][cb][i], acci[rb][cb][i]};
+            }
+        }
+}
+
 template <int RB, int CB, int WR, bool PF, bool XG = false, bool MID = false, bool PIPE = false>
 __device__ __forceinline__ void ft_bf16p_body(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
                                               const __bf16 *__restrict__ bpk, int64_t npts, int nk,
@@ -1046,57 +1178,27 @@ __device__ __forceinline__ void ft_bf16p_mainloop_gp(int64_t ndata, int nrs, con
 // ---------------------------------------------------------------------------------------------
 constexpr int FTR_NBW_MAX = 4;
 
-template <int NBW, bool MID = false, bool PIPE = false, bool GP = false>
-__global__ __launch_bounds__(256, 2) void ft_bf16p_rot_kernel(int nrs, const __bf16 *__restrict__ apk,
-                                                              const __bf16 *__restrict__ bpk, int64_t npts,
-                                                              const int *__restrict__ ncol,
-                                                              const int *__restrict__ col_index,
-                                                              const int *__restrict__ ikq, int64_t cs,
-                                                              const cplx<float> *__restrict__ U,
-                                                              const int64_t *__restrict__ bmin,
-                                                              const int64_t *__restrict__ bmax,
-                                                              const cplx<float> *__restrict__ uph,
-                                                              const double *__restrict__ omega, int nbw,
-                                                              float *__restrict__ g2, int nr,
-                                                              const int *__restrict__ irvec, int3 qgrid) {
-    constexpr int RB = 3, CB = 4, WR = 4, NW = 16, NM = 12, TN = 64;
-    constexpr int64_t NDATA = NW * NM;
-    // phase staging during the main loop, then the ep tile [jj][m][nu'] (re, im) of the epilogue
-    constexpr int BS_ELEMS = 2 * TN * FTP_BLD > 4 * TN * NBW * NM ? 2 * TN * FTP_BLD : 4 * TN * NBW * NM;
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[BS_ELEMS];
-    const int lane = threadIdx.x & 63, wr = threadIdx.x >> 6;
+// Epilogue of the fused kR->kq kernels. The wave with row group wr holds modes 3wr..3wr+2 of all 16
+// Wannier rows for the columns jw + cb*16 + il of the tile (jw = the wave's column offset). Lane (c, il)
+// owns nu' = c, c+4, c+8 of its column; the four row groups' shares of ep are added into EP in row-group
+// order (deterministic; waves of different column offsets write disjoint columns), then g2 is written
+// for the tile's valid columns, layout [m][nu'] per kept (unit, column), slots m < nbw. Called by every
+// thread of the block (it holds barriers); EP holds TN * NBW * 12 complex values.
+template <int NBW, int TN>
+__device__ __forceinline__ void ftr_epilogue(const f32x4 (&accr)[3][4], const f32x4 (&acci)[3][4], int wr, int jw,
+                                             int u, int nk, int64_t j0, const int *__restrict__ col_index,
+                                             const int *__restrict__ ikq, int64_t cs,
+                                             const cplx<float> *__restrict__ U, const int64_t *__restrict__ bmin,
+                                             const int64_t *__restrict__ bmax, const cplx<float> *__restrict__ uph,
+                                             const double *__restrict__ omega, int nbw, float *__restrict__ g2,
+                                             float *EP) {
+    constexpr int RB = 3, CB = 4, NW = 16, NM = 12;
+    const int lane = threadIdx.x & 63;
     const int c = lane >> 4, il = lane & 15;
-    const int u = blockIdx.z;
-    const int nk = ncol[u];
-    const int64_t j0 = (int64_t)blockIdx.x * TN;
-    if (j0 >= nk) return;  // uniform for the block
-    f32x4 accr[RB][CB], acci[RB][CB];
-    if constexpr (GP) {
-        __shared__ float2 tab[3 * FTG_NMAX];
-        const int n[3] = {qgrid.x, qgrid.y, qgrid.z};
-        for (int e = threadIdx.x; e < 3 * FTG_NMAX; e += blockDim.x) {
-            const int a = e / FTG_NMAX, m = e % FTG_NMAX;
-            float cr = 0.f, si = 0.f;
-            if (m < n[a]) cis2pi((double)m / n[a], cr, si);
-            tab[e] = make_float2(cr, si);
-        }
-        __syncthreads();
-        ft_bf16p_mainloop_gp<RB, CB, WR>(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, nr, irvec, n, tab, nk,
-                                         col_index + u * cs, j0, 0, Bs, accr, acci);
-    } else if constexpr (PIPE)
-        ft_bf16p_mainloop_pipe<RB, CB, WR, MID>(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk,
-                                                col_index + u * cs, j0, 0, Bs, accr, acci);
-    else
-        ft_bf16p_mainloop<RB, CB, WR, true, MID>(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk,
-                                                 col_index + u * cs, j0, 0, Bs, accr, acci);
-    // after the main loop's final barrier the staging buffer is free: ep tile [jj][m][nu'] (re, im)
-    float *EP = reinterpret_cast<float *>(Bs);
-
-    // this wave's share of ep, one column block at a time: lane (c, il) owns nu' = c, c+4, c+8 of
-    // column cb*16 + il; the four waves' shares are added into LDS in wave order
 #pragma unroll
     for (int cb = 0; cb < CB; cb++) {
-        const int64_t j = j0 + cb * 16 + il;
+        const int jj = jw + cb * 16 + il;
+        const int64_t j = j0 + jj;
         const bool jv = j < nk;
         const int64_t row = (int64_t)u * cs + (jv ? j : j0);
         const int64_t kq = ikq[row];
@@ -1152,7 +1254,7 @@ __global__ __launch_bounds__(256, 2) void ft_bf16p_rot_kernel(int nrs, const __b
                 for (int m = 0; m < NBW; m++)
 #pragma unroll
                     for (int k = 0; k < 3; k++) {
-                        const int o = 2 * (((cb * 16 + il) * NBW + m) * NM + c + 4 * k);
+                        const int o = 2 * ((jj * NBW + m) * NM + c + 4 * k);
                         EP[o] = (w == 0 ? 0.f : EP[o]) + pr[m][k];
                         EP[o + 1] = (w == 0 ? 0.f : EP[o + 1]) + pi[m][k];
                     }
@@ -1160,8 +1262,7 @@ __global__ __launch_bounds__(256, 2) void ft_bf16p_rot_kernel(int nrs, const __b
             __syncthreads();
         }
     }
-    // g2 for the tile's valid columns, layout [m][nu'] per kept (unit, column), slots m < nbw
-    for (int e = threadIdx.x; e < TN * nbw * NM; e += 256) {
+    for (int e = threadIdx.x; e < TN * nbw * NM; e += blockDim.x) {
         const int jj = e / (nbw * NM), rr = e % (nbw * NM), m = rr / NM, nu2 = rr % NM;
         const int64_t j = j0 + jj;
         if (j >= nk) continue;
@@ -1171,6 +1272,88 @@ __global__ __launch_bounds__(256, 2) void ft_bf16p_rot_kernel(int nrs, const __b
         const float w2 = 2.f * static_cast<float>(omega[(int64_t)NM * col_index[row] + nu2]);
         g2[row * nbw * NM + rr] = (er * er + ei * ei) / w2;
     }
+}
+
+template <int NBW, bool MID = false, bool PIPE = false, bool GP = false>
+__global__ __launch_bounds__(256, 2) void ft_bf16p_rot_kernel(int nrs, const __bf16 *__restrict__ apk,
+                                                              const __bf16 *__restrict__ bpk, int64_t npts,
+                                                              const int *__restrict__ ncol,
+                                                              const int *__restrict__ col_index,
+                                                              const int *__restrict__ ikq, int64_t cs,
+                                                              const cplx<float> *__restrict__ U,
+                                                              const int64_t *__restrict__ bmin,
+                                                              const int64_t *__restrict__ bmax,
+                                                              const cplx<float> *__restrict__ uph,
+                                                              const double *__restrict__ omega, int nbw,
+                                                              float *__restrict__ g2, int nr,
+                                                              const int *__restrict__ irvec, int3 qgrid) {
+    constexpr int RB = 3, CB = 4, WR = 4, NW = 16, NM = 12, TN = 64;
+    constexpr int64_t NDATA = NW * NM;
+    // phase staging during the main loop, then the ep tile [jj][m][nu'] (re, im) of the epilogue
+    constexpr int BS_ELEMS = 2 * TN * FTP_BLD > 4 * TN * NBW * NM ? 2 * TN * FTP_BLD : 4 * TN * NBW * NM;
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[BS_ELEMS];
+    const int lane = threadIdx.x & 63, wr = threadIdx.x >> 6;
+    const int c = lane >> 4, il = lane & 15;
+    const int u = blockIdx.z;
+    const int nk = ncol[u];
+    const int64_t j0 = (int64_t)blockIdx.x * TN;
+    if (j0 >= nk) return;  // uniform for the block
+    f32x4 accr[RB][CB], acci[RB][CB];
+    if constexpr (GP) {
+        __shared__ float2 tab[3 * FTG_NMAX];
+        const int n[3] = {qgrid.x, qgrid.y, qgrid.z};
+        for (int e = threadIdx.x; e < 3 * FTG_NMAX; e += blockDim.x) {
+            const int a = e / FTG_NMAX, m = e % FTG_NMAX;
+            float cr = 0.f, si = 0.f;
+            if (m < n[a]) cis2pi((double)m / n[a], cr, si);
+            tab[e] = make_float2(cr, si);
+        }
+        __syncthreads();
+        ft_bf16p_mainloop_gp<RB, CB, WR>(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, nr, irvec, n, tab, nk,
+                                         col_index + u * cs, j0, 0, Bs, accr, acci);
+    } else if constexpr (PIPE)
+        ft_bf16p_mainloop_pipe<RB, CB, WR, MID>(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk,
+                                                col_index + u * cs, j0, 0, Bs, accr, acci);
+    else
+        ft_bf16p_mainloop<RB, CB, WR, true, MID>(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk,
+                                                 col_index + u * cs, j0, 0, Bs, accr, acci);
+    // after the main loop's final barrier the staging buffer is free: ep tile [jj][m][nu'] (re, im)
+    ftr_epilogue<NBW, TN>(accr, acci, wr, 0, u, nk, j0, col_index, ikq, cs, U, bmin, bmax, uph, omega, nbw, g2,
+                          reinterpret_cast<float *>(Bs));
+}
+
+// Eight-wave form of the fused kR->kq kernel: 128 kept columns per workgroup as two column halves of
+// 64 sharing the unit's operand tile, staged once per stage through LDS (the operand is read into the
+// CU once per 128 columns instead of once per 64). The epilogue reuses the operand buffer for ep.
+template <int NBW>
+__global__ __launch_bounds__(512, 2) void ft_bf16p8_rot_kernel(int nrs, const __bf16 *__restrict__ apk,
+                                                               const __bf16 *__restrict__ bpk, int64_t npts,
+                                                               const int *__restrict__ ncol,
+                                                               const int *__restrict__ col_index,
+                                                               const int *__restrict__ ikq, int64_t cs,
+                                                               const cplx<float> *__restrict__ U,
+                                                               const int64_t *__restrict__ bmin,
+                                                               const int64_t *__restrict__ bmax,
+                                                               const cplx<float> *__restrict__ uph,
+                                                               const double *__restrict__ omega, int nbw,
+                                                               float *__restrict__ g2) {
+    constexpr int NW = 16, NM = 12, TN = FT8_COLS;
+    constexpr int64_t NDATA = NW * NM;
+    static_assert(NDATA == FT8_ROWS, "one e-ph unit per row tile");
+    static_assert(2 * FT8_ROWS * FTP_BLD * 2 >= TN * NBW * NM * 2 * 4, "ep tile must fit the operand buffer");
+    __shared__ __attribute__((aligned(16))) __bf16 As[2 * FT8_ROWS * FTP_BLD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[2 * FT8_COLS * FTP_BLD];
+    const int wave = threadIdx.x >> 6;
+    const int wr = wave & 3, wc = wave >> 2;
+    const int u = blockIdx.z;
+    const int nk = ncol[u];
+    const int64_t j0 = (int64_t)blockIdx.x * TN;
+    if (j0 >= nk) return;  // uniform for the block
+    f32x4 accr[3][4], acci[3][4];
+    ft8_mainloop(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk, col_index + u * cs, j0, 0, As, Bs,
+                 accr, acci);
+    ftr_epilogue<NBW, TN>(accr, acci, wr, 64 * wc, u, nk, j0, col_index, ikq, cs, U, bmin, bmax, uph, omega, nbw, g2,
+                          reinterpret_cast<float *>(As));
 }
 
 // One thread per (unit, stage, row): 16 complex values -> 64 bf16 of the packed operand.
@@ -1355,7 +1538,7 @@ extern "C" int ephip_fourier_batched_c32(int64_t ndata, int32_t nr, const void *
 }
 
 extern "C" int ephip_ft_set_variant(int32_t kind, int32_t variant) {
-    static const int nvar[5] = {3, 3, 3, 7, 5};
+    static const int nvar[5] = {3, 3, 3, 8, 6};
     EPHIP_REQUIRE(kind >= 0 && kind < 5, "ft_set_variant: kind %d outside 0..4", kind);
     EPHIP_REQUIRE(variant >= 0 && variant < nvar[kind], "ft_set_variant: variant %d outside the range of kind %d",
                   variant, kind);
@@ -1410,7 +1593,9 @@ extern "C" int ephip_fourier_packed_bf16x3(int64_t ndata, int32_t nr, const void
                   "fourier_packed: operand or table too large for 32-bit offsets");
     // Tile shape: 192 x 64 for 192-row operands (one e-ph unit); for operands of 128 rows or more the
     // tuner's variant of kind 3 (0: 128 x 128, 1: 128 x 128 with A prefetch, 2: 192 x 64 when
-    // ndata % 192 == 0, 3: 192 x 64 in XCD-grouped order, one unit only); otherwise 64 x 128.
+    // ndata % 192 == 0, 3: 192 x 64 in XCD-grouped order, one unit only, 4-6: variants of 3 with
+    // mid-sequence LDS writes and/or a pipelined main loop, 7: eight waves on 192 x 128 with the
+    // operand staged through LDS, one unit without column lists); otherwise 64 x 128.
     int shape;
     const int var = g_ft_variant[3];
     if (ndata % 192 == 0 && (nk_max <= 64 || ndata % 128 != 0))
@@ -1424,6 +1609,14 @@ extern "C" int ephip_fourier_packed_bf16x3(int64_t ndata, int32_t nr, const void
     const bool pipe = grouped && (var == 5 || var == 6);
     const int tm = shape == 2 ? 192 : (shape == 0 ? 64 : 128), tn = shape == 2 ? 64 : 128;
     EPHIP_REQUIRE((ndata + tm - 1) / tm <= 65535, "fourier_packed: ndata=%lld too large for one call", (long long)ndata);
+    if (shape == 2 && var == 7 && nunits == 1 && !col_index && !ncol) {
+        dim3 g8((unsigned)((nk_max + FT8_COLS - 1) / FT8_COLS), (unsigned)((ndata + FT8_ROWS - 1) / FT8_ROWS), 1);
+        ft_bf16p8_kernel<<<g8, 512, 0, as_stream(stream)>>>(ndata, nrs, static_cast<const __bf16 *>(apk),
+                                                            static_cast<const __bf16 *>(bpk), npts, nk_max,
+                                                            static_cast<cplx<float> *>(out), ldout);
+        EPHIP_LAUNCHED();
+        return EPHIP_OK;
+    }
     dim3 g((unsigned)((nk_max + tn - 1) / tn), (unsigned)((ndata + tm - 1) / tm), (unsigned)nunits);
     auto a = static_cast<const __bf16 *>(apk);
     auto b = static_cast<const __bf16 *>(bpk);
@@ -1489,6 +1682,17 @@ extern "C" int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t
                           (int64_t)qgrid[0] * qgrid[1] * qgrid[2] == npts,
                       "eph_kq_rotate_packed: q-grid dimensions must be 1..%d with n1 n2 n3 = npts", FTG_NMAX);
         qg = make_int3(qgrid[0], qgrid[1], qgrid[2]);
+    }
+    if (fv == 5) {
+        dim3 g8((unsigned)((cs + FT8_COLS - 1) / FT8_COLS), 1, (unsigned)nunits);
+        if (nbw <= 2)
+            ft_bf16p8_rot_kernel<2><<<g8, 512, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax, up,
+                                                        omega, nbw, g2);
+        else
+            ft_bf16p8_rot_kernel<4><<<g8, 512, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax, up,
+                                                        omega, nbw, g2);
+        EPHIP_LAUNCHED();
+        return EPHIP_OK;
     }
     if (gp && nbw <= 2) {
         ft_bf16p_rot_kernel<2, false, false, true><<<g, 256, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu,

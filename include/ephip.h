@@ -221,9 +221,11 @@ int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t nbw, int32
  * column tiles of a row tile on one XCD, sharing its L2; single-unit calls), 4 = as 3 with the next
  * stage's phase pieces written to LDS halfway through the MFMA sequence, 5 = as 4 with the pipelined
  * main loop (phase pieces two stages ahead, operand fragments one stage ahead, no register copies),
- * 6 = as 3 pipelined. kind 4 = the fused kR->kq rotation kernel; variant 0 = phase pieces written
+ * 6 = as 3 pipelined, 7 = eight-wave workgroups on 192 x 128 tiles with the operand tile staged
+ * through LDS once for both column halves (single-unit calls without column lists). kind 4 = the fused kR->kq rotation kernel; variant 0 = phase pieces written
  * after the MFMA sequence, 1 = halfway through it, 2 = pipelined, 3 = pipelined and halfway, 4 = phases
- * of the uniform q-grid generated on the fly from per-axis tables (no phase-table reads).
+ * of the uniform q-grid generated on the fly from per-axis tables (no phase-table reads), 5 = eight-
+ * wave workgroups on 128 kept columns with the unit's operand staged through LDS once for both halves.
  * Process-wide setting (no counterpart in the reference, whose tuning constants are hard-coded:
  * src/wannier_to_bloch_batched.jl:397-406). Host-side, synchronous. */
 int ephip_ft_set_variant(int32_t kind, int32_t variant);

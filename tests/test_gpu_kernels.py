@@ -340,7 +340,7 @@ def test_fourier_packed_argument_checks(backend):
                L.stream_handle())
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7])
 def test_fourier_packed_variants(backend, variant):
     """Every tile shape of the packed split-BF16 kernel (kind 3 of the tuner) on an operand whose row
     count suits all three (384 = 3 x 128 = 2 x 192)."""
@@ -356,6 +356,28 @@ def test_fourier_packed_variants(backend, variant):
         apk = core.ft_pack_operand(_dev(op.astype(np.complex64)), ndata, nr, ndata)
         bpk = core.ft_pack_phases(_dev(irvec), _dev(xk))
         out = torch.empty((nk, ndata), dtype=torch.complex64, device="cuda")
+        core.fourier_packed(apk, ndata, nr, bpk, nk, nk, out, ndata)
+        assert _rel(out.cpu().numpy(), O.fourier(op, irvec, xk)) < 2e-5
+    finally:
+        tune.set_variant(tune.KIND_PACKED_BF16, old)
+
+
+@pytest.mark.parametrize("nr", [10, 16, 17, 33])
+def test_fourier_packed_eight_wave_stages(backend, nr):
+    """The eight-wave 192 x 128 tile (kind 3, variant 7) with one, one full, two and three R stages
+    (its operand and phase tiles are double-buffered one stage ahead), with a partial last column tile."""
+    from ephip import core, tune
+    old = tune.get_variant(tune.KIND_PACKED_BF16)
+    rng = np.random.default_rng(90 + nr)
+    irvec = M.rbox(5)[:nr]
+    ndata, nk = 192 * 3, 131
+    op = rng.standard_normal((nr, ndata)) + 1j * rng.standard_normal((nr, ndata))
+    xk = rng.random((nk, 3))
+    try:
+        tune.set_variant(tune.KIND_PACKED_BF16, 7)
+        apk = core.ft_pack_operand(_dev(op.astype(np.complex64)), ndata, nr, ndata)
+        bpk = core.ft_pack_phases(_dev(irvec), _dev(xk))
+        out = torch.full((nk, ndata), 7.0, dtype=torch.complex64, device="cuda")
         core.fourier_packed(apk, ndata, nr, bpk, nk, nk, out, ndata)
         assert _rel(out.cpu().numpy(), O.fourier(op, irvec, xk)) < 2e-5
     finally:

@@ -147,15 +147,16 @@ def model16():
     return m, {"kgrid": (8, 8, 8), "qgrid": (4, 4, 4), "window": 0.5}
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
-def test_outer_k_fused_rotation(backend, model16, variant):
-    """Split-BF16 outer-k with the kR->kq Fourier fused to the rotations (nw 16, nm 12), both kernel
-    variants: same kept-q lists as the unfused kernels, g2 within 1e-5 of them, and within 1e-4 of the
-    oracle."""
+@pytest.mark.parametrize("variant,nq", [(0, 4), (1, 4), (2, 4), (3, 4), (4, 4), (5, 4), (0, 8), (5, 8)])
+def test_outer_k_fused_rotation(backend, model16, variant, nq):
+    """Split-BF16 outer-k with the kR->kq Fourier fused to the rotations (nw 16, nm 12), every kernel
+    variant: same kept-q lists as the unfused kernels, g2 within 1e-5 of them, and within 1e-4 of the
+    oracle. The 8^3 q-grid gives units several column tiles of both widths (64 and 128)."""
     from ephip import _lib as L
     from ephip import eph, tune
     tune.set_variant(tune.KIND_FUSED_ROT, variant)
     m, g = model16
+    g = dict(g, qgrid=(nq, nq, nq))
     w = g["window"]
     st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
     assert 1 <= st.nbw <= 4
@@ -176,6 +177,6 @@ def test_outer_k_fused_rotation(backend, model16, variant):
             num = max(num, float(np.abs(bf["g2"][u] - bp["g2"][u]).max(initial=0.0)))
             den = max(den, float(np.abs(bp["g2"][u]).max(initial=0.0)))
     assert den > 0 and num / den < 1e-5, num / den
-    rel = _outer_k_vs_oracle(m, g, st, rf, qstep=5)
+    rel = _outer_k_vs_oracle(m, g, st, rf, qstep=5 if nq == 4 else 37)
     tune.set_variant(tune.KIND_FUSED_ROT, 0)
     assert rel < 1e-4, rel
