@@ -35,13 +35,21 @@ sys.path.insert(0, os.path.join(ROOT, "electronphonon.jl_amd"))
 METRIC = ("g(k,q) k·q pairs/s at |g|²≤1e-4 rel err vs FP64; plain vs "
           "mixed+batched+surrogate")
 PEAK = {"fp32": 157.3, "bf16x3": 2500.0}  # TFLOP/s, MI355X dense MFMA (fp32, bf16)
-# HBM bytes per launch of the two Fourier kernels of the fast path at k batch 128 on the cfg2 workload,
-# from rocprofv3 FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00) passes calibrated on a 256 MiB copy
-# (profiles/cfg2_fast_bf16_hbm_counters.md). Counters cannot be read inside the timed run.
-TRAFFIC_BYTES_CFG2_KB128 = {
-    ("bf16x3", "kR->kq"): (1592903 * 2 + 68234) * 1024,    # packed Fourier fused with the rotations
-    ("bf16x3", "RR->kR"): (1498651 * 2 + 1053918) * 1024,  # packed Fourier, XCD-grouped tiles
-    ("fp32", "RR->kR"): (1412486 * 2 + 1054128) * 1024,
+# HBM bytes per launch of the two Fourier kernels of the fast path on the cfg2 workload, keyed by
+# (precision, stage, k batch, kernel variant), from rocprofv3 FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00)
+# passes calibrated on a 256 MiB copy (profiles/cfg2_fast_bf16_hbm_counters.md). Counters cannot be read
+# inside the timed run; a variant or batch without a measurement reports traffic null.
+TRAFFIC_BYTES_CFG2 = {
+    ("bf16x3", "kR->kq", 128, 0): (1592903 * 2 + 68234) * 1024,    # packed Fourier fused with the rotations
+    ("bf16x3", "RR->kR", 128, 4): (1498651 * 2 + 1053918) * 1024,  # packed Fourier, XCD-grouped tiles
+}
+# kernel names by (stage, variant) for the split-BF16 fast path (tuner kinds 3 and 4)
+KERNEL_NAMES = {
+    ("RR->kR", "proj"): "ft_bf16p8_proj_kernel: eight-wave 192x128 packed split-BF16 RR->kR Fourier over Re "
+                        "fused with the k-side window projection",
+    ("RR->kR", 7): "ft_bf16p8_kernel: eight-wave 192x128 packed split-BF16 RR->kR Fourier over Re",
+    ("kR->kq", 5): "ft_bf16p8_rot_kernel: eight-wave packed split-BF16 kR->kq Fourier over Rp fused with the "
+                   "k+q/phonon rotations and g2",
 }
 
 
@@ -147,7 +155,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--qtile", type=int, default=8)
     ap.add_argument("--kbatch", type=int, default=16384, help="k batch of the plain outer-q path")
-    ap.add_argument("--kbatch-k", type=int, default=128, help="k batch (one step) of the fast outer-k path")
+    ap.add_argument("--kbatch-k", type=int, default=384, help="k batch (one step) of the fast outer-k path")
     ap.add_argument("--path", choices=["fast", "plain", "both"], default="both")
     ap.add_argument("--precision", choices=["fp32", "bf16x3"], default="bf16x3",
                     help="Fourier precision of the fast path (split-BF16 meets the 1e-4 |g|^2 bound)")
@@ -177,6 +185,7 @@ def main():
     from ephip import _lib as L
     from ephip import core, eph
     from ephip import model as M
+    from ephip import tune as T
 
     t_setup = time.perf_counter()
     backend = core.gpu_backend(0)
@@ -292,7 +301,10 @@ def main():
                                krkq_tflops=f_krkq_total / t_krkq / 1e12,
                                krkq_avg_ms=1e3 * t_krkq / len(kin["kR->kq"]),
                                rrkr_flop_per_launch=f_rrkr, krkq_flop_per_launch=f_krkq_total / len(kin["kR->kq"]),
-                               fused_rotation=fused)
+                               fused_rotation=fused, fused_projection=bool(getattr(loop, "fused_proj", False)),
+                               variant_rrkr=("proj" if getattr(loop, "fused_proj", False) else
+                                             T.get_variant(T.KIND_PACKED_BF16)),
+                               variant_krkq=T.get_variant(T.KIND_FUSED_ROT))
         log(f"[bench] fast outer-k {args.precision}: {results['fast']['value']:.3e} pairs/s; RR->kR "
             f"{results['fast']['rrkr_tflops']:.1f} TFLOP/s ({results['fast']['rrkr_avg_ms']:.2f} ms), kR->kq "
             f"{results['fast']['krkq_tflops']:.1f} TFLOP/s ({results['fast']['krkq_avg_ms']:.2f} ms)")
@@ -310,15 +322,20 @@ def main():
         value, ms, steps = r["value"], r["ms_per_step"], r["steps"]
         # dominant kernel: the longer of the two Fourier launches per step
         bf = args.precision == "bf16x3"
+        var = {"RR->kR": r.get("variant_rrkr"), "kR->kq": r.get("variant_krkq")}
         if r["rrkr_avg_ms"] >= r["krkq_avg_ms"]:
-            dom = (("ft_bf16p_kernel (packed split-BF16)" if bf else "ft_mfma_kernel<float,128>")
-                   + ", RR->kR Fourier over Re (ndata = nw^2 nm nr_ep, nr = nr_el, nk = k batch)",
+            name = KERNEL_NAMES.get(("RR->kR", var["RR->kR"]) if bf else None) or (
+                (f"ft_bf16p_kernel (packed split-BF16, variant {var['RR->kR']})" if bf else
+                 "ft_mfma_kernel<float,128>") + ", RR->kR Fourier over Re")
+            dom = (name + " (ndata = nw^2 nm nr_ep, nr = nr_el, nk = k batch)",
                    r["rrkr_tflops"], r["rrkr_avg_ms"], r["rrkr_flop_per_launch"], "RR->kR")
         else:
-            dom = (("ft_bf16p_rot_kernel: packed split-BF16 kR->kq Fourier over Rp fused with the k+q/phonon "
-                    "rotations and g2" if r.get("fused_rotation") else
-                    "batched Fourier kernel, kR->kq over Rp") + " (units x kept q)",
-                   r["krkq_tflops"], r["krkq_avg_ms"], r["krkq_flop_per_launch"], "kR->kq")
+            name = KERNEL_NAMES.get(("kR->kq", var["kR->kq"]) if r.get("fused_rotation") else None) or (
+                f"ft_bf16p_rot_kernel (variant {var['kR->kq']}): packed split-BF16 kR->kq Fourier over Rp fused "
+                "with the k+q/phonon rotations and g2" if r.get("fused_rotation") else
+                "batched Fourier kernel, kR->kq over Rp")
+            dom = (name + " (units x kept q)", r["krkq_tflops"], r["krkq_avg_ms"], r["krkq_flop_per_launch"],
+                   "kR->kq")
         path = "outer-k, k-side window projection, k+q window skip"
         pairs_step = args.kbatch_k * nq_total
     else:
@@ -348,8 +365,8 @@ def main():
                    "plain_outer_q_fp32_pairs_per_s": results.get("plain", {}).get("value")},
         "roofline": {"bound": "mfma", "achieved": dom[1], "peak": peak, "unit": "TFLOP/s",
                      "frac": dom[1] / peak,
-                     "traffic": (TRAFFIC_BYTES_CFG2_KB128.get((args.precision, dom[4]))
-                                 if ("fast" in results and args.kbatch_k == 128 and args.config == "cfg2"
+                     "traffic": (TRAFFIC_BYTES_CFG2.get((args.precision, dom[4], args.kbatch_k, var[dom[4]]))
+                                 if ("fast" in results and args.config == "cfg2"
                                      and (dom[4] != "kR->kq" or r.get("fused_rotation"))) else None),
                      "kernel": dom[0], "avg_launch_ms": dom[2], "flop_per_launch": dom[3]},
         "cpu_baseline": cpu,

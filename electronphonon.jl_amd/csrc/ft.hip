@@ -974,6 +974,66 @@ __global__ __launch_bounds__(512, 2) void ft_bf16p8_kernel(int64_t ndata, int nr
         }
 }
 
+// RR->kR Fourier over Re fused with the k-side window projection (nw = 16). The operand rows are packed
+// j-fastest, d = j + 16 (i + 16 (nu + nm Rp)) (ephip_ft_pack_operand_swapped with nb = 16), so each 16-row
+// MFMA block is one complete j group of the row group G = i + 16 (nu + nm Rp). For batch column kk (the
+// k-point) and each of its units u (one per in-window band n, in band order, kus[kk] <= u < kus[kk+1]):
+// A[u][G] = sum_j g(k)[i, j, nu, Rp] U_k[j, n], a 4-lane shuffle reduction of the accumulators. The
+// g(k, Rp) tensor itself is never written.
+__global__ __launch_bounds__(512, 2) void ft_bf16p8_proj_kernel(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
+                                                                const __bf16 *__restrict__ bpk, int64_t npts, int nk,
+                                                                const int *__restrict__ kus,
+                                                                const int64_t *__restrict__ ustate,
+                                                                const int *__restrict__ uband,
+                                                                const cplx<float> *__restrict__ U, int nbw,
+                                                                cplx<float> *__restrict__ A, int64_t lda) {
+    __shared__ __attribute__((aligned(16))) __bf16 As[2 * FT8_ROWS * FTP_BLD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[2 * FT8_COLS * FTP_BLD];
+    constexpr int NW = 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave & 3, wc = wave >> 2;
+    const int c = lane >> 4, il = lane & 15;
+    int64_t bx, by;
+    xcd_grouped_tile(bx, by);
+    const int64_t j0 = bx * FT8_COLS;
+    if (j0 >= nk) return;  // uniform for the block
+    const int64_t d0 = by * FT8_ROWS;
+    f32x4 accr[3][4], acci[3][4];
+    ft8_mainloop(ndata, nrs, apk, bpk, npts, nk, nullptr, j0, d0, As, Bs, accr, acci);
+    const int ku0 = kus[0];
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++) {
+        const int64_t kk = j0 + wc * 64 + cb * 16 + il;
+        const bool kv = kk < nk;
+        const int ub = kv ? kus[kk] - ku0 : 0;
+        const int nb = kv ? kus[kk + 1] - kus[kk] : 0;
+        const cplx<float> *uk = U + (int64_t)NW * NW * (nb > 0 ? ustate[ub] : 0);
+        for (int m = 0; m < nbw; m++) {
+            const bool act = m < nb;
+            const int band = act ? uband[ub + m] : 0;
+            cplx<float> w[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) w[r] = act ? uk[4 * c + r + NW * band] : cplx<float>{0.f, 0.f};
+#pragma unroll
+            for (int rb = 0; rb < 3; rb++) {
+                float sr = 0.f, si = 0.f;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const float ar = accr[rb][cb][r], ai = acci[rb][cb][r];
+                    sr += ar * w[r].re - ai * w[r].im;
+                    si += ar * w[r].im + ai * w[r].re;
+                }
+                sr += __shfl_xor(sr, 16, 64);
+                si += __shfl_xor(si, 16, 64);
+                sr += __shfl_xor(sr, 32, 64);
+                si += __shfl_xor(si, 32, 64);
+                const int64_t G = (d0 + wr * 48 + rb * 16) / 16;
+                if (act && c == rb) A[(int64_t)(ub + m) * lda + G] = cplx<float>{sr, si};
+            }
+        }
+    }
+}
+
 template <int RB, int CB, int WR, bool PF, bool XG = false, bool MID = false, bool PIPE = false>
 __device__ __forceinline__ void ft_bf16p_body(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
                                               const __bf16 *__restrict__ bpk, int64_t npts, int nk,
@@ -1356,15 +1416,22 @@ __global__ __launch_bounds__(512, 2) void ft_bf16p8_rot_kernel(int nrs, const __
                           reinterpret_cast<float *>(As));
 }
 
-// One thread per (unit, stage, row): 16 complex values -> 64 bf16 of the packed operand.
+// One thread per (unit, stage, row): 16 complex values -> 64 bf16 of the packed operand. With swap = nb > 0
+// the two fastest row indices (each of size nb) trade places: packed row j + nb (i + nb rest) holds operand
+// row i + nb (j + nb rest).
 __global__ void ft_pack_operand_kernel(int64_t ndata, int nr, int nrs, const cplx<float> *__restrict__ op,
-                                       int64_t ldop, int64_t op_stride, __bf16 *__restrict__ apk) {
+                                       int64_t ldop, int64_t op_stride, __bf16 *__restrict__ apk, int swap = 0) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int u = blockIdx.y;
     if (e >= ndata * nrs) return;
     const int64_t d = e % ndata;
     const int s = (int)(e / ndata);
-    const cplx<float> *o = op + u * op_stride + d;
+    int64_t src = d;
+    if (swap > 0) {
+        const int64_t j = d % swap, i = (d / swap) % swap, rest = d / ((int64_t)swap * swap);
+        src = i + swap * (j + swap * rest);
+    }
+    const cplx<float> *o = op + u * op_stride + src;
     bf16x8 v[8];
     __bf16 *w = reinterpret_cast<__bf16 *>(v);
 #pragma unroll
@@ -1562,6 +1629,22 @@ extern "C" int ephip_ft_pack_operand(int64_t ndata, int32_t nr, const void *op, 
     return EPHIP_OK;
 }
 
+extern "C" int ephip_ft_pack_operand_swapped(int64_t ndata, int32_t nr, int32_t nb, const void *op, int64_t ldop,
+                                             void *apk, void *stream) {
+    EPHIP_REQUIRE(ndata >= 0 && nr >= 0 && ldop >= ndata && nb >= 1,
+                  "ft_pack_operand_swapped: bad sizes (ndata=%lld nr=%d nb=%d)", (long long)ndata, nr, nb);
+    EPHIP_REQUIRE(ndata % ((int64_t)nb * nb) == 0, "ft_pack_operand_swapped: ndata=%lld not a multiple of nb^2=%d",
+                  (long long)ndata, nb * nb);
+    const int nrs = (nr + 15) / 16;
+    if (ndata == 0 || nrs == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(op && apk, "ft_pack_operand_swapped: null device pointer");
+    dim3 g((unsigned)((ndata * nrs + 255) / 256), 1);
+    ft_pack_operand_kernel<<<g, 256, 0, as_stream(stream)>>>(ndata, nr, nrs, static_cast<const cplx<float> *>(op),
+                                                             ldop, 0, static_cast<__bf16 *>(apk), nb);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
 extern "C" int ephip_ft_pack_phases(int32_t nr, const int32_t *irvec, int64_t npts, const double *xk, void *bpk,
                                     void *stream) {
     EPHIP_REQUIRE(nr >= 0 && npts >= 0, "ft_pack_phases: negative size");
@@ -1646,6 +1729,34 @@ extern "C" int ephip_fourier_packed_bf16x3(int64_t ndata, int32_t nr, const void
     else
         ft_bf16p_kernel<2, 4, 2, true><<<g, 256, 0, st>>>(ndata, nrs, a, b, npts, nk_max, o, ldout, out_stride,
                                                            col_index, col_stride, ncol);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_eph_RR_to_kR_project_packed_bf16x3(int32_t nw, int32_t nm, int32_t nrp, int32_t nre,
+                                                       const void *apk, const void *bpk, int64_t npts, int32_t nk,
+                                                       const int32_t *kus, const int64_t *ustate,
+                                                       const int32_t *uband, const void *U, int32_t nbw, void *A,
+                                                       int64_t lda, void *stream) {
+    EPHIP_REQUIRE(nw == 16, "eph_RR_to_kR_project: fused projection needs nw = 16 (got %d)", nw);
+    EPHIP_REQUIRE(nm >= 1 && nrp >= 1 && nre >= 1 && nk >= 0 && npts >= nk && nbw >= 1,
+                  "eph_RR_to_kR_project: bad sizes (nm=%d nrp=%d nre=%d nk=%d npts=%lld nbw=%d)", nm, nrp, nre, nk,
+                  (long long)npts, nbw);
+    const int64_t ndata = (int64_t)nw * nw * nm * nrp;
+    EPHIP_REQUIRE(ndata % FT8_ROWS == 0, "eph_RR_to_kR_project: nw^2 nm nrp = %lld not a multiple of %d",
+                  (long long)ndata, FT8_ROWS);
+    EPHIP_REQUIRE(lda >= (int64_t)nw * nm * nrp, "eph_RR_to_kR_project: lda smaller than nw nm nrp");
+    EPHIP_REQUIRE(ndata * 64 < (1LL << 32) && npts * 64 < (1LL << 32),
+                  "eph_RR_to_kR_project: operand or table too large for 32-bit offsets");
+    EPHIP_REQUIRE(ndata / FT8_ROWS <= 65535, "eph_RR_to_kR_project: ndata=%lld too large for one call",
+                  (long long)ndata);
+    if (nk == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(apk && bpk && kus && ustate && uband && U && A, "eph_RR_to_kR_project: null device pointer");
+    const int nrs = (nre + 15) / 16;
+    dim3 g((unsigned)((nk + FT8_COLS - 1) / FT8_COLS), (unsigned)(ndata / FT8_ROWS), 1);
+    ft_bf16p8_proj_kernel<<<g, 512, 0, as_stream(stream)>>>(
+        ndata, nrs, static_cast<const __bf16 *>(apk), static_cast<const __bf16 *>(bpk), npts, nk, kus, ustate, uband,
+        static_cast<const cplx<float> *>(U), nbw, static_cast<cplx<float> *>(A), lda);
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

@@ -55,7 +55,10 @@ SIGNATURES = {
     "ephip_mlp_bf16_prune": [_i64, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, C.c_float, C.c_float,
                              C.c_float, _vp, _vp, _vp],
     "ephip_ft_pack_operand": [_i64, _i32, _vp, _i64, _i32, _i64, _vp, _vp],
+    "ephip_ft_pack_operand_swapped": [_i64, _i32, _i32, _vp, _i64, _vp, _vp],
     "ephip_ft_pack_phases": [_i32, _vp, _i64, _vp, _vp, _vp],
+    "ephip_eph_RR_to_kR_project_packed_bf16x3": [_i32, _i32, _i32, _i32, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp,
+                                                 _i32, _vp, _i64, _vp],
     "ephip_fourier_packed_bf16x3": [_i64, _i32, _vp, _i32, _vp, _i64, _i32, _vp, _vp, _i64, _vp, _i64, _i64, _vp],
     "ephip_eph_kq_rotate_packed_bf16x3": [_i32, _i32, _i32, _i32, _vp, _i32, _vp, _i64, _i64, _vp, _vp, _vp, _vp,
                                           _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

@@ -200,6 +200,19 @@ def ft_pack_operand(op, ndata: int, nr: int, ldop: int, nunits: int = 1, op_stri
     return out
 
 
+def ft_pack_operand_swapped(op, ndata: int, nr: int, nb: int, ldop: int, out=None):
+    """As ft_pack_operand for one operand, with the two fastest row indices (each of size nb) swapped:
+    packed row j + nb (i + nb rest) holds operand row i + nb (j + nb rest)."""
+    torch = _torch()
+    shape = (ft_stages(nr), ndata, 64)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.bfloat16, device=op.device)
+    elif out.dtype != torch.bfloat16 or out.numel() < math.prod(shape) or not out.is_contiguous():
+        raise L.EphipError("ft_pack_operand_swapped: out must be a contiguous bfloat16 buffer of (nrs, ndata, 64)")
+    L.call("ephip_ft_pack_operand_swapped", ndata, nr, nb, L.ptr(op), ldop, L.ptr(out), L.stream_handle())
+    return out
+
+
 def ft_pack_phases(irvec_dev, xk_dev, out=None):
     """Packed split-BF16 phase table of the points xk_dev: (nrs, npts, 64) bfloat16."""
     torch = _torch()

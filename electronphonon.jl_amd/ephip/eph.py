@@ -369,7 +369,7 @@ class OuterKLoop:
     Pairs (k, q) with no in-window band at k+q are exactly zero and are not computed."""
 
     def __init__(self, st: OuterQSetup, nk_batch: int | None = 128, precision: int = L.PREC_FP32,
-                 nk_batch_cap: int = 512, fuse_rotation: bool = True):
+                 nk_batch_cap: int = 512, fuse_rotation: bool = True, fuse_projection: bool = True):
         torch = _torch()
         self.st = st
         if nk_batch is None:  # memory-adaptive width (plan_batch), the e-ph tensor already resident
@@ -397,10 +397,16 @@ class OuterKLoop:
         # split-BF16 with nw = 16, nm = 12, nbw <= 4: kR->kq and the rotations run as one kernel that
         # writes g2 only (no g(k, q) staging buffer)
         self.fused = bool(fuse_rotation) and self.packed and nw == 16 and nm == 12 and st.nbw <= 4
+        # split-BF16 with nw = 16: RR->kR and the k-side window projection run as one kernel on the "el"
+        # operand packed with j fastest; g(k, Rp) is never stored
+        self.fused_proj = bool(fuse_projection) and self.packed and nw == 16 and self.nd_el % 192 == 0
         import ctypes
         self.qgrid_c = (ctypes.c_int32 * 3)(*map(int, st.qgrid))  # for the on-the-fly q phases
-        if self.packed:
+        if self.fused_proj:
+            self.apk_el = core.ft_pack_operand_swapped(self.epmat_el, self.nd_el, self.nre, nw, self.nd_el)
+        elif self.packed:
             self.apk_el = core.ft_pack_operand(self.epmat_el, self.nd_el, self.nre, self.nd_el)
+        if self.packed:
             self.epmat_el = None
             self.el_obj = None
             self.bpk_q = core.ft_pack_phases(self.irvec_ep_dev, st.ph.xq)
@@ -412,6 +418,8 @@ class OuterKLoop:
         ks = st.ksurv.cpu().numpy()
         nb = (bmax[ks] - bmin[ks] + 1).astype(np.int64)
         self.k_unit_start = np.concatenate([[0], np.cumsum(nb)])
+        self.k_unit_start_dev = torch.from_numpy(self.k_unit_start.astype(np.int32)).to(dev)
+        self.max_units_per_k = int(nb.max()) if len(nb) else 1
         self.unit_kpos = torch.from_numpy(np.repeat(np.arange(len(ks)), nb).astype(np.int32)).to(dev)
         self.unit_state = torch.from_numpy(np.repeat(ks, nb).astype(np.int64)).to(dev)
         band = np.concatenate([np.arange(bmin[k] - 1, bmax[k]) for k in ks]) if len(ks) else np.zeros(0)
@@ -423,7 +431,8 @@ class OuterKLoop:
             if len(ks) else 1
         self.max_units = max_units
         nd2 = nw * nm
-        self.G = torch.empty((self.nk_batch, self.nd_el), dtype=torch.complex64, device=dev)
+        self.G = None if self.fused_proj else torch.empty((self.nk_batch, self.nd_el), dtype=torch.complex64,
+                                                           device=dev)
         self.A = torch.empty((max_units, nd2 * self.nrp), dtype=torch.complex64, device=dev)
         self.col_index = torch.empty((max_units, self.cs), dtype=torch.int32, device=dev)
         self.ikq = torch.empty((max_units, self.cs), dtype=torch.int32, device=dev)
@@ -460,17 +469,25 @@ class OuterKLoop:
         stream = L.stream_handle()
         # 1. RR->kR Fourier over Re for the batch
         xk_b = self.xk_surv[i0:i0 + n]
-        if self.packed:
+        if self.fused_proj:  # RR->kR with the projection in its epilogue: writes the unit operators A
+            bpk = core.ft_pack_phases(st.irvec_el_dev, xk_b, out=self.bpk_k)
+            self._timed("RR->kR", 8.0 * self.nd_el * self.nre * n, lambda: L.call(
+                "ephip_eph_RR_to_kR_project_packed_bf16x3", nw, nm, self.nrp, self.nre, L.ptr(self.apk_el), L.ptr(bpk),
+                n, n, L.ptr(self.k_unit_start_dev[i0:]), L.ptr(self.unit_state[u0:u1]), L.ptr(self.unit_band[u0:u1]),
+                L.ptr(st.el.U), self.max_units_per_k, L.ptr(self.A), self.A.stride(0), stream))
+        elif self.packed:
             bpk = core.ft_pack_phases(st.irvec_el_dev, xk_b, out=self.bpk_k)
             self._timed("RR->kR", 8.0 * self.nd_el * self.nre * n, lambda: core.fourier_packed(
                 self.apk_el, self.nd_el, self.nre, bpk, n, n, self.G, self.G.stride(0)))
         else:
             self._timed("RR->kR", 8.0 * self.nd_el * self.nre * n,
                         lambda: core.fourier_batched(self.el_obj, xk_b, out=self.G[:n], precision=self.precision))
-        # projection; the unit's column in G is its k position minus i0 (base pointer shifted by -i0 columns)
-        g_base = L.ptr(self.G) - i0 * self.G.stride(0) * 8
-        L.call("ephip_eph_project_kR", nw, nm, self.nrp, nu, g_base, self.G.stride(0), L.ptr(self.unit_kpos[u0:u1]),
-               L.ptr(self.unit_state[u0:u1]), L.ptr(self.unit_band[u0:u1]), L.ptr(st.el.U), L.ptr(self.A), stream)
+        if not self.fused_proj:
+            # projection; the unit's column in G is its k position minus i0 (base pointer shifted by -i0 columns)
+            g_base = L.ptr(self.G) - i0 * self.G.stride(0) * 8
+            L.call("ephip_eph_project_kR", nw, nm, self.nrp, nu, g_base, self.G.stride(0),
+                   L.ptr(self.unit_kpos[u0:u1]), L.ptr(self.unit_state[u0:u1]), L.ptr(self.unit_band[u0:u1]),
+                   L.ptr(st.el.U), L.ptr(self.A), stream)
         # 2. kept q per unit
         g = (C.c_int32 * 3)(*map(int, st.kgrid))
         L.call("ephip_kq_compact", nu, L.ptr(self.unit_state[u0:u1]), L.ptr(st.el.xk), self.nq, L.ptr(st.ph.xq),
@@ -515,8 +532,10 @@ class OuterKLoop:
         from . import tune as T
         n = min(self.nk_batch, self.st.ksurv.shape[0])
         if self.packed:
-            res = {"packed": T.tune_fourier(T.KIND_PACKED_BF16, lambda: self.run_kbatch(0, n), reps=reps,
-                                            variants=T.PACKED_VARIANTS)}
+            # the fused RR->kR + projection kernel has one shape; tile variants apply to the unfused one
+            res = {} if self.fused_proj else {
+                "packed": T.tune_fourier(T.KIND_PACKED_BF16, lambda: self.run_kbatch(0, n), reps=reps,
+                                         variants=T.PACKED_VARIANTS)}
             if self.fused:
                 res["fused"] = T.tune_fourier(T.KIND_FUSED_ROT, lambda: self.run_kbatch(0, n), reps=reps,
                                               variants=T.FUSED_VARIANTS)

@@ -193,9 +193,28 @@ int ephip_eph_rotate_kq_g2(int32_t nw, int32_t nm, int32_t nbw, int32_t nunits, 
 int ephip_ft_pack_operand(int64_t ndata, int32_t nr, const void *op, int64_t ldop, int32_t nunits, int64_t op_stride,
                           void *apk, void *stream);
 int ephip_ft_pack_phases(int32_t nr, const int32_t *irvec, int64_t npts, const double *xk, void *bpk, void *stream);
+/* As ephip_ft_pack_operand for one operand, with the two fastest row indices (each of size nb) swapped:
+ * packed row j + nb*(i + nb*rest) holds operand row i + nb*(j + nb*rest); ndata a multiple of nb^2. */
+int ephip_ft_pack_operand_swapped(int64_t ndata, int32_t nr, int32_t nb, const void *op, int64_t ldop, void *apk,
+                                  void *stream);
 int ephip_fourier_packed_bf16x3(int64_t ndata, int32_t nr, const void *apk, int32_t nunits, const void *bpk,
                                 int64_t npts, int32_t nk_max, const int32_t *ncol, const int32_t *col_index,
                                 int64_t col_stride, void *out, int64_t ldout, int64_t out_stride, void *stream);
+
+/* RR->kR Fourier over Re fused with the k-side window projection (get_eph_RR_to_kR! of
+ * src/wannier_to_bloch_batched.jl:178-197 followed by the outer-k projection of
+ * src/calculator/run_eph_over_k_and_kq.jl:657-659, replacing ephip_fourier_packed_bf16x3 +
+ * ephip_eph_project_kR) for nw = 16, split-BF16: apk = the "el" tensor (rows i + nw*(j + nw*(nu + nm*Rp)),
+ * nr = nre) packed by ephip_ft_pack_operand_swapped with nb = 16; bpk = packed phases of the batch's nk
+ * k-points (npts >= nk). kus: int32, nk+1 entries, units of batch column t are kus[t]-kus[0] ..
+ * kus[t+1]-kus[0]-1 (one per in-window band, at most nbw per column), with ustate (int64) and 0-based
+ * uband (int32) indexed from the batch's first unit. Writes A[u*lda + (i + nw*nu) + nw*nm*Rp] =
+ * sum_j g(k_t)[i, j, nu, Rp] U[ustate[u]][j, uband[u]] as ephip_eph_project_kR does; g(k, Rp) is not
+ * stored. nw^2*nm*nrp must be a multiple of 192. */
+int ephip_eph_RR_to_kR_project_packed_bf16x3(int32_t nw, int32_t nm, int32_t nrp, int32_t nre, const void *apk,
+                                             const void *bpk, int64_t npts, int32_t nk, const int32_t *kus,
+                                             const int64_t *ustate, const int32_t *uband, const void *U, int32_t nbw,
+                                             void *A, int64_t lda, void *stream);
 
 /* kR->kq Fourier over Rp fused with the k+q-window and phonon rotations and g2 (the
  * get_eph_kR_to_kq_batched! + eph_apply_rotations! + g2 sequence of src/wannier_to_bloch_batched.jl:

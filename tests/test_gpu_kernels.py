@@ -384,6 +384,24 @@ def test_fourier_packed_eight_wave_stages(backend, nr):
         tune.set_variant(tune.KIND_PACKED_BF16, old)
 
 
+def test_pack_operand_swapped(backend):
+    """The j-fastest packing is ft_pack_operand of the row-permuted operand, bit for bit."""
+    from ephip import _lib as L
+    from ephip import core
+    rng = np.random.default_rng(95)
+    nb, rest, nr = 4, 6, 21
+    ndata = nb * nb * rest
+    op = (rng.standard_normal((nr, ndata)) + 1j * rng.standard_normal((nr, ndata))).astype(np.complex64)
+    d = np.arange(ndata)
+    j, i, r = d % nb, (d // nb) % nb, d // (nb * nb)
+    perm = op[:, i + nb * (j + nb * r)]
+    a = core.ft_pack_operand_swapped(_dev(op), ndata, nr, nb, ndata)
+    b = core.ft_pack_operand(_dev(np.ascontiguousarray(perm)), ndata, nr, ndata)
+    assert torch.equal(a.view(torch.int16).reshape(-1), b.view(torch.int16).reshape(-1))
+    with pytest.raises(L.EphipError):   # ndata not a multiple of nb^2
+        core.ft_pack_operand_swapped(_dev(op), ndata - 1, nr, nb, ndata)
+
+
 @pytest.mark.parametrize("nr", [10, 16, 17, 33])
 def test_fourier_packed_few_stages(backend, nr):
     """Operands with one, one full, and two or three R stages (the loads run up to two stages ahead)."""

@@ -180,3 +180,36 @@ def test_outer_k_fused_rotation(backend, model16, variant, nq):
     rel = _outer_k_vs_oracle(m, g, st, rf, qstep=5 if nq == 4 else 37)
     tune.set_variant(tune.KIND_FUSED_ROT, 0)
     assert rel < 1e-4, rel
+
+
+@pytest.mark.parametrize("nq", [4, 8])
+def test_outer_k_fused_projection(backend, model16, nq):
+    """Split-BF16 outer-k with RR->kR fused to the k-side window projection (nw 16): the unit operators
+    A and the g2 payloads agree with the separate Fourier + projection kernels to 1e-5, and g2 with the
+    oracle to 1e-4. Batches of 40 and 150 k give one and two column tiles of 128 k, with a partial tile."""
+    from ephip import _lib as L
+    from ephip import eph
+    m, g = model16
+    g = dict(g, qgrid=(nq, nq, nq))
+    w = g["window"]
+    st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
+    nks = st.ksurv.shape[0]
+    for nb in (40, 150):
+        fused = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_BF16X3)
+        plain = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_BF16X3, fuse_projection=False)
+        assert fused.fused_proj and not plain.fused_proj and fused.G is None
+        rf, rp = eph.UnitRecorder(), eph.UnitRecorder()
+        for i0 in (0, max(0, nks - nb)):
+            fused.run_kbatch(i0, nb, [rf])
+            plain.run_kbatch(i0, nb, [rp])
+            nu = int(fused.k_unit_start[min(i0 + nb, nks)] - fused.k_unit_start[i0])
+            a, b = fused.A[:nu].cpu().numpy(), plain.A[:nu].cpu().numpy()
+            assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)
+        for bf, bp in zip(rf.batches, rp.batches):
+            assert np.array_equal(bf["unit_state"], bp["unit_state"])
+            for u in range(len(bf["cols"])):
+                assert np.array_equal(bf["cols"][u], bp["cols"][u])
+                d = np.abs(bf["g2"][u] - bp["g2"][u]).max(initial=0.0)
+                assert d <= 1e-5 * max(np.abs(bp["g2"][u]).max(initial=0.0), 1e-30)
+    rel = _outer_k_vs_oracle(m, g, st, rf, qstep=5 if nq == 4 else 37)
+    assert rel < 1e-4, rel
