@@ -397,7 +397,7 @@ __global__ __launch_bounds__(256) void ft_mfma_db_kernel(int64_t ndata, int nr, 
 // Kernel variant per contraction kind, set by the tuner (ephip_ft_set_variant). Kinds:
 // 0 = FP32 wide (nk > 64), 1 = FP32 narrow (nk <= 64), 2 = FP64. Variants: 0 = single-buffered,
 // 1 = double-buffered KC 16, 2 = double-buffered KC 8.
-static int g_ft_variant[5] = {0, 0, 0, 0, 0};
+static int g_ft_variant[6] = {0, 0, 0, 0, 0, 0};
 
 template <class T, int TN>
 static void launch_ft(int variant, dim3 g, hipStream_t s, int64_t ndata, int nr, const cplx<T> *op, int64_t ldop,
@@ -944,6 +944,104 @@ __device__ __forceinline__ void ft8_mainloop(int64_t ndata, int nrs, const __bf1
     }
 }
 
+// ft8_mainloop with the global loads two stages ahead: the operand and phase pieces of stage s+2 are
+// issued at the start of stage s into the register set that held stage s, and written to LDS halfway
+// through stage s+1 (about 1.5 stages after issue instead of half a stage). Two register sets; the stage
+// loop is unrolled by two so both sets stay in fixed registers. Same products in the same order.
+__device__ __forceinline__ void ft8_mainloop_deep(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
+                                                  const __bf16 *__restrict__ bpk, int64_t npts, int nk,
+                                                  const int *__restrict__ col_index, int64_t j0, int64_t d0,
+                                                  __bf16 *As, __bf16 *Bs, f32x4 (&accr)[3][4], f32x4 (&acci)[3][4]) {
+    constexpr int RB = 3, CB = 4, TN = FT8_COLS, TM = FT8_ROWS;
+    constexpr int APT = TM * 8 / 512, BPT = TN * 8 / 512;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave & 3, wc = wave >> 2;
+    const int c = lane >> 4, il = lane & 15;
+    const int c1 = (c + 2) & 3;
+    const uint32_t nmask = c >= 2 ? 0x80008000u : 0u;
+    uint32_t asrc[APT], adst[APT], bsrc[BPT], bdst[BPT];
+#pragma unroll
+    for (int t = 0; t < APT; t++) {
+        const int e = tid + 512 * t, row = e / 8, pc = e % 8;
+        asrc[t] = (uint32_t)min(d0 + row, ndata - 1) * 64u + 8u * pc;
+        adst[t] = row * FTP_BLD + 8 * pc;
+    }
+#pragma unroll
+    for (int t = 0; t < BPT; t++) {
+        const int e = tid + 512 * t, jl = e / 8, pc = e % 8;
+        const int64_t j = min(j0 + jl, (int64_t)nk - 1);
+        bsrc[t] = (uint32_t)(col_index ? col_index[j] : j) * 64u + 8u * pc;
+        bdst[t] = jl * FTP_BLD + 8 * pc;
+    }
+    const int64_t astage = ndata * 64, bstage = npts * 64;
+#pragma unroll
+    for (int i = 0; i < RB; i++)
+#pragma unroll
+        for (int j = 0; j < CB; j++) accr[i][j] = acci[i][j] = f32x4{0, 0, 0, 0};
+    bf16x8 qa[2][APT], qb[2][BPT];
+    auto load = [&](int st, bf16x8(&ra)[APT], bf16x8(&rb)[BPT]) {
+        const __bf16 *ag = apk + st * astage;
+        const __bf16 *bg = bpk + st * bstage;
+#pragma unroll
+        for (int t = 0; t < APT; t++) ra[t] = *reinterpret_cast<const bf16x8 *>(ag + asrc[t]);
+#pragma unroll
+        for (int t = 0; t < BPT; t++) rb[t] = *reinterpret_cast<const bf16x8 *>(bg + bsrc[t]);
+    };
+    auto store = [&](int buf, const bf16x8(&ra)[APT], const bf16x8(&rb)[BPT]) {
+#pragma unroll
+        for (int t = 0; t < APT; t++) *reinterpret_cast<bf16x8 *>(&As[buf * (TM * FTP_BLD) + adst[t]]) = ra[t];
+#pragma unroll
+        for (int t = 0; t < BPT; t++) *reinterpret_cast<bf16x8 *>(&Bs[buf * (TN * FTP_BLD) + bdst[t]]) = rb[t];
+    };
+    load(0, qa[0], qb[0]);
+    store(0, qa[0], qb[0]);
+    if (nrs > 1) load(1, qa[1], qb[1]);
+    __syncthreads();
+    // stage s computes from LDS buffer s & 1; register set s & 1 is free at its start
+    auto stage = [&](int s, bf16x8(&fa)[APT], bf16x8(&fb)[BPT], const bf16x8(&na)[APT], const bf16x8(&nb)[BPT]) {
+        const int cur = s & 1;
+        if (s + 2 < nrs) load(s + 2, fa, fb);
+        const __bf16 *aa = As + cur * (TM * FTP_BLD);
+        bf16x8 ah[RB], al[RB];
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+            const __bf16 *row = aa + (wr * (RB * 16) + rb * 16 + il) * FTP_BLD + 8 * c;
+            ah[rb] = *reinterpret_cast<const bf16x8 *>(row);
+            al[rb] = *reinterpret_cast<const bf16x8 *>(row + 32);
+        }
+        const __bf16 *bb = Bs + cur * (TN * FTP_BLD);
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++) {
+            if (cb == CB / 2 && s + 1 < nrs) store(cur ^ 1, na, nb);
+            const __bf16 *col = bb + (wc * (CB * 16) + cb * 16 + il) * FTP_BLD;
+            const bf16x8 b2h = *reinterpret_cast<const bf16x8 *>(col + 8 * c);
+            const bf16x8 b2l = *reinterpret_cast<const bf16x8 *>(col + 32 + 8 * c);
+            const bf16x8 b1h = bf16_flip(*reinterpret_cast<const bf16x8 *>(col + 8 * c1), nmask);
+            const bf16x8 b1l = bf16_flip(*reinterpret_cast<const bf16x8 *>(col + 32 + 8 * c1), nmask);
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rb], b1h, accr[rb][cb], 0, 0, 0);
+                acci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rb], b2h, acci[rb][cb], 0, 0, 0);
+            }
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b1l, accr[rb][cb], 0, 0, 0);
+                acci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b2l, acci[rb][cb], 0, 0, 0);
+            }
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                accr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b1h, accr[rb][cb], 0, 0, 0);
+                acci[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], b2h, acci[rb][cb], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    };
+    for (int s = 0; s < nrs; s += 2) {
+        stage(s, qa[0], qb[0], qa[1], qb[1]);
+        if (s + 1 < nrs) stage(s + 1, qa[1], qb[1], qa[0], qb[0]);
+    }
+}
+
 // RR->kR-type contraction with the eight-wave 192 x 128 tile (single unit, XCD-grouped order).
 __global__ __launch_bounds__(512, 2) void ft_bf16p8_kernel(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
                                                            const __bf16 *__restrict__ bpk, int64_t npts, int nk,
@@ -980,6 +1078,7 @@ __global__ __launch_bounds__(512, 2) void ft_bf16p8_kernel(int64_t ndata, int nr
 // k-point) and each of its units u (one per in-window band n, in band order, kus[kk] <= u < kus[kk+1]):
 // A[u][G] = sum_j g(k)[i, j, nu, Rp] U_k[j, n], a 4-lane shuffle reduction of the accumulators. The
 // g(k, Rp) tensor itself is never written.
+template <bool DEEP = false>
 __global__ __launch_bounds__(512, 2) void ft_bf16p8_proj_kernel(int64_t ndata, int nrs, const __bf16 *__restrict__ apk,
                                                                 const __bf16 *__restrict__ bpk, int64_t npts, int nk,
                                                                 const int *__restrict__ kus,
@@ -999,7 +1098,10 @@ __global__ __launch_bounds__(512, 2) void ft_bf16p8_proj_kernel(int64_t ndata, i
     if (j0 >= nk) return;  // uniform for the block
     const int64_t d0 = by * FT8_ROWS;
     f32x4 accr[3][4], acci[3][4];
-    ft8_mainloop(ndata, nrs, apk, bpk, npts, nk, nullptr, j0, d0, As, Bs, accr, acci);
+    if constexpr (DEEP)
+        ft8_mainloop_deep(ndata, nrs, apk, bpk, npts, nk, nullptr, j0, d0, As, Bs, accr, acci);
+    else
+        ft8_mainloop(ndata, nrs, apk, bpk, npts, nk, nullptr, j0, d0, As, Bs, accr, acci);
     const int ku0 = kus[0];
 #pragma unroll
     for (int cb = 0; cb < 4; cb++) {
@@ -1334,6 +1436,126 @@ __device__ __forceinline__ void ftr_epilogue(const f32x4 (&accr)[3][4], const f3
     }
 }
 
+// Column metadata of the fused epilogue for the lane's four column blocks (k+q state, q, number of k+q
+// window bands clipped to nbw, first band), loaded before the main loop so that the epilogue's only
+// global reads are the U_kq and u_ph values, which it fetches one column block ahead.
+struct FtrCols {
+    int kq[4], iq[4], nbkq[4], mlo[4];
+};
+
+__device__ __forceinline__ void ftr_columns(FtrCols &m, int jw, int u, int nk, int64_t j0,
+                                            const int *__restrict__ col_index, const int *__restrict__ ikq,
+                                            int64_t cs, const int64_t *__restrict__ bmin,
+                                            const int64_t *__restrict__ bmax, int nbw) {
+    const int il = threadIdx.x & 15;
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++) {
+        const int64_t j = j0 + jw + cb * 16 + il;
+        const bool jv = j < nk;
+        const int64_t row = (int64_t)u * cs + (jv ? j : j0);
+        const int kq = ikq[row];
+        m.kq[cb] = kq;
+        m.iq[cb] = col_index[row];
+        const int64_t b0 = bmin[kq], b1 = bmax[kq];
+        const int nb = b0 > 0 ? (int)(b1 - b0 + 1) : 0;
+        m.nbkq[cb] = jv ? min(nb, nbw) : 0;
+        m.mlo[cb] = b0 > 0 ? (int)b0 - 1 : 0;
+    }
+}
+
+// ftr_epilogue with the column metadata of ftr_columns and the U_kq / u_ph reads of the next column block
+// issued before the current block's arithmetic and barriers. Same arithmetic and summation order.
+template <int NBW, int TN>
+__device__ __forceinline__ void ftr_epilogue_cols(const f32x4 (&accr)[3][4], const f32x4 (&acci)[3][4], int wr,
+                                                  int jw, const FtrCols &mc, int u, int nk, int64_t j0,
+                                                  const int *__restrict__ col_index, int64_t cs,
+                                                  const cplx<float> *__restrict__ U,
+                                                  const cplx<float> *__restrict__ uph,
+                                                  const double *__restrict__ omega, int nbw, float *__restrict__ g2,
+                                                  float *EP) {
+    constexpr int RB = 3, CB = 4, NW = 16, NM = 12;
+    const int lane = threadIdx.x & 63;
+    const int c = lane >> 4, il = lane & 15;
+    cplx<float> w[2][NBW][4], p[2][RB][3];
+    auto fetch = [&](int cb, cplx<float> (&wd)[NBW][4], cplx<float> (&pd)[RB][3]) {
+        const cplx<float> *uk = U + (int64_t)NW * NW * mc.kq[cb];
+        const cplx<float> *up = uph + (int64_t)NM * NM * mc.iq[cb];
+#pragma unroll
+        for (int m = 0; m < NBW; m++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                wd[m][r] = m < mc.nbkq[cb] ? uk[4 * c + r + NW * (mc.mlo[cb] + m)] : cplx<float>{0.f, 0.f};
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) pd[rb][k] = up[(3 * wr + rb) + NM * (c + 4 * k)];
+    };
+    fetch(0, w[0], p[0]);
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++) {
+        const int cur = cb & 1;
+        if (cb + 1 < CB) fetch(cb + 1, w[cur ^ 1], p[cur ^ 1]);
+        const int jj = jw + cb * 16 + il;
+        float pr[NBW][3], pi[NBW][3];
+#pragma unroll
+        for (int m = 0; m < NBW; m++) {
+            float tr[RB], ti[RB];
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                float sr = 0.f, si = 0.f;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const float ar = accr[rb][cb][r], ai = acci[rb][cb][r];
+                    sr += w[cur][m][r].re * ar + w[cur][m][r].im * ai;
+                    si += w[cur][m][r].re * ai - w[cur][m][r].im * ar;
+                }
+                sr += __shfl_xor(sr, 16, 64);
+                si += __shfl_xor(si, 16, 64);
+                sr += __shfl_xor(sr, 32, 64);
+                si += __shfl_xor(si, 32, 64);
+                tr[rb] = sr;
+                ti[rb] = si;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                float er = 0.f, ei = 0.f;
+#pragma unroll
+                for (int rb = 0; rb < RB; rb++) {
+                    const cplx<float> q = p[cur][rb][k];
+                    er += tr[rb] * q.re - ti[rb] * q.im;
+                    ei += tr[rb] * q.im + ti[rb] * q.re;
+                }
+                pr[m][k] = er;
+                pi[m][k] = ei;
+            }
+        }
+#pragma unroll
+        for (int wv = 0; wv < 4; wv++) {
+            if (wr == wv) {
+#pragma unroll
+                for (int m = 0; m < NBW; m++)
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        const int o = 2 * ((jj * NBW + m) * NM + c + 4 * k);
+                        EP[o] = (wv == 0 ? 0.f : EP[o]) + pr[m][k];
+                        EP[o + 1] = (wv == 0 ? 0.f : EP[o + 1]) + pi[m][k];
+                    }
+            }
+            __syncthreads();
+        }
+    }
+    for (int e = threadIdx.x; e < TN * nbw * NM; e += blockDim.x) {
+        const int jj = e / (nbw * NM), rr = e % (nbw * NM), m = rr / NM, nu2 = rr % NM;
+        const int64_t j = j0 + jj;
+        if (j >= nk) continue;
+        const int64_t row = (int64_t)u * cs + j;
+        const int o = 2 * ((jj * NBW + m) * NM + nu2);
+        const float er = EP[o], ei = EP[o + 1];
+        const float w2 = 2.f * static_cast<float>(omega[(int64_t)NM * col_index[row] + nu2]);
+        g2[row * nbw * NM + rr] = (er * er + ei * ei) / w2;
+    }
+}
+
 template <int NBW, bool MID = false, bool PIPE = false, bool GP = false>
 __global__ __launch_bounds__(256, 2) void ft_bf16p_rot_kernel(int nrs, const __bf16 *__restrict__ apk,
                                                               const __bf16 *__restrict__ bpk, int64_t npts,
@@ -1385,7 +1607,13 @@ __global__ __launch_bounds__(256, 2) void ft_bf16p_rot_kernel(int nrs, const __b
 // Eight-wave form of the fused kR->kq kernel: 128 kept columns per workgroup as two column halves of
 // 64 sharing the unit's operand tile, staged once per stage through LDS (the operand is read into the
 // CU once per 128 columns instead of once per 64). The epilogue reuses the operand buffer for ep.
-template <int NBW>
+//
+// Workgroup order (ug > 0): the units are dealt to the 8 XCDs (unit x + 8v on XCD x, workgroups being
+// dispatched round-robin over the XCDs), and each XCD walks its units in groups of ug, column tile by
+// column tile, so the tiles running together on one XCD share its L2 for the group's operands and for the
+// q range of the phase table they gather. The grid is then (ntiles, 1, 8 ceil(nunits / 8)). ug = 0 keeps
+// the plain order (all tiles of a unit in sequence).
+template <int NBW, bool PFE = false, bool DEEP = false>
 __global__ __launch_bounds__(512, 2) void ft_bf16p8_rot_kernel(int nrs, const __bf16 *__restrict__ apk,
                                                                const __bf16 *__restrict__ bpk, int64_t npts,
                                                                const int *__restrict__ ncol,
@@ -1396,7 +1624,7 @@ __global__ __launch_bounds__(512, 2) void ft_bf16p8_rot_kernel(int nrs, const __
                                                                const int64_t *__restrict__ bmax,
                                                                const cplx<float> *__restrict__ uph,
                                                                const double *__restrict__ omega, int nbw,
-                                                               float *__restrict__ g2) {
+                                                               float *__restrict__ g2, int nunits, int ug) {
     constexpr int NW = 16, NM = 12, TN = FT8_COLS;
     constexpr int64_t NDATA = NW * NM;
     static_assert(NDATA == FT8_ROWS, "one e-ph unit per row tile");
@@ -1405,15 +1633,41 @@ __global__ __launch_bounds__(512, 2) void ft_bf16p8_rot_kernel(int nrs, const __
     __shared__ __attribute__((aligned(16))) __bf16 Bs[2 * FT8_COLS * FTP_BLD];
     const int wave = threadIdx.x >> 6;
     const int wr = wave & 3, wc = wave >> 2;
-    const int u = blockIdx.z;
+    int u, tile;
+    if (ug > 0) {
+        const int64_t T = gridDim.x, V = gridDim.z / 8;
+        const int64_t l = blockIdx.x + T * (int64_t)blockIdx.z;
+        const int64_t x = l % 8, n = l / 8;  // n-th workgroup of XCD x, n < V T
+        const int64_t gi = n / ((int64_t)ug * T), w = n % ((int64_t)ug * T);
+        const int64_t g = min((int64_t)ug, V - gi * ug);  // units in this group (the last may be short)
+        tile = (int)(w / g);
+        u = (int)(x + 8 * (gi * ug + w % g));
+        if (u >= nunits) return;  // uniform for the block
+    } else {
+        u = blockIdx.z;
+        tile = blockIdx.x;
+    }
     const int nk = ncol[u];
-    const int64_t j0 = (int64_t)blockIdx.x * TN;
+    const int64_t j0 = (int64_t)tile * TN;
     if (j0 >= nk) return;  // uniform for the block
     f32x4 accr[3][4], acci[3][4];
-    ft8_mainloop(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk, col_index + u * cs, j0, 0, As, Bs,
-                 accr, acci);
-    ftr_epilogue<NBW, TN>(accr, acci, wr, 64 * wc, u, nk, j0, col_index, ikq, cs, U, bmin, bmax, uph, omega, nbw, g2,
-                          reinterpret_cast<float *>(As));
+    if constexpr (PFE) {
+        FtrCols mc;
+        ftr_columns(mc, 64 * wc, u, nk, j0, col_index, ikq, cs, bmin, bmax, nbw);
+        if constexpr (DEEP)
+            ft8_mainloop_deep(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk, col_index + u * cs, j0,
+                              0, As, Bs, accr, acci);
+        else
+            ft8_mainloop(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk, col_index + u * cs, j0, 0,
+                         As, Bs, accr, acci);
+        ftr_epilogue_cols<NBW, TN>(accr, acci, wr, 64 * wc, mc, u, nk, j0, col_index, cs, U, uph, omega, nbw, g2,
+                                   reinterpret_cast<float *>(As));
+    } else {
+        ft8_mainloop(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk, col_index + u * cs, j0, 0, As,
+                     Bs, accr, acci);
+        ftr_epilogue<NBW, TN>(accr, acci, wr, 64 * wc, u, nk, j0, col_index, ikq, cs, U, bmin, bmax, uph, omega, nbw,
+                              g2, reinterpret_cast<float *>(As));
+    }
 }
 
 // One thread per (unit, stage, row): 16 complex values -> 64 bf16 of the packed operand. With swap = nb > 0
@@ -1605,15 +1859,15 @@ extern "C" int ephip_fourier_batched_c32(int64_t ndata, int32_t nr, const void *
 }
 
 extern "C" int ephip_ft_set_variant(int32_t kind, int32_t variant) {
-    static const int nvar[5] = {3, 3, 3, 8, 6};
-    EPHIP_REQUIRE(kind >= 0 && kind < 5, "ft_set_variant: kind %d outside 0..4", kind);
+    static const int nvar[6] = {3, 3, 3, 8, 13, 2};
+    EPHIP_REQUIRE(kind >= 0 && kind < 6, "ft_set_variant: kind %d outside 0..5", kind);
     EPHIP_REQUIRE(variant >= 0 && variant < nvar[kind], "ft_set_variant: variant %d outside the range of kind %d",
                   variant, kind);
     g_ft_variant[kind] = variant;
     return EPHIP_OK;
 }
 
-extern "C" int ephip_ft_get_variant(int32_t kind) { return (kind >= 0 && kind < 5) ? g_ft_variant[kind] : -1; }
+extern "C" int ephip_ft_get_variant(int32_t kind) { return (kind >= 0 && kind < 6) ? g_ft_variant[kind] : -1; }
 
 extern "C" int ephip_ft_pack_operand(int64_t ndata, int32_t nr, const void *op, int64_t ldop, int32_t nunits,
                                      int64_t op_stride, void *apk, void *stream) {
@@ -1754,9 +2008,12 @@ extern "C" int ephip_eph_RR_to_kR_project_packed_bf16x3(int32_t nw, int32_t nm, 
     EPHIP_REQUIRE(apk && bpk && kus && ustate && uband && U && A, "eph_RR_to_kR_project: null device pointer");
     const int nrs = (nre + 15) / 16;
     dim3 g((unsigned)((nk + FT8_COLS - 1) / FT8_COLS), (unsigned)(ndata / FT8_ROWS), 1);
-    ft_bf16p8_proj_kernel<<<g, 512, 0, as_stream(stream)>>>(
-        ndata, nrs, static_cast<const __bf16 *>(apk), static_cast<const __bf16 *>(bpk), npts, nk, kus, ustate, uband,
-        static_cast<const cplx<float> *>(U), nbw, static_cast<cplx<float> *>(A), lda);
+    // tuner kind 5: variant 0 = loads one stage ahead, 1 = two stages ahead
+    auto kern = g_ft_variant[5] == 1 ? ft_bf16p8_proj_kernel<true> : ft_bf16p8_proj_kernel<false>;
+    kern<<<g, 512, 0, as_stream(stream)>>>(ndata, nrs, static_cast<const __bf16 *>(apk),
+                                           static_cast<const __bf16 *>(bpk), npts, nk, kus, ustate, uband,
+                                           static_cast<const cplx<float> *>(U), nbw, static_cast<cplx<float> *>(A),
+                                           lda);
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }
@@ -1794,14 +2051,32 @@ extern "C" int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t
                       "eph_kq_rotate_packed: q-grid dimensions must be 1..%d with n1 n2 n3 = npts", FTG_NMAX);
         qg = make_int3(qgrid[0], qgrid[1], qgrid[2]);
     }
-    if (fv == 5) {
-        dim3 g8((unsigned)((cs + FT8_COLS - 1) / FT8_COLS), 1, (unsigned)nunits);
-        if (nbw <= 2)
+    if (fv >= 5) {
+        // variant 5: plain order; 6, 7, 8: XCD-dealt units walked in groups of 2, 4, 8; 9, 10: as 7, 8 with
+        // the epilogue's column metadata loaded before the main loop and U_kq / u_ph fetched ahead
+        // 11, 12: as 9, 10 with the main loop's loads two stages ahead
+        const int ug = fv == 5 ? 0 : (fv == 6 ? 2 : ((fv == 7 || fv == 9 || fv == 11) ? 4 : 8));
+        const bool pfe = fv >= 9, deep = fv >= 11;
+        const unsigned gz = ug > 0 ? (unsigned)(8 * ((nunits + 7) / 8)) : (unsigned)nunits;
+        dim3 g8((unsigned)((cs + FT8_COLS - 1) / FT8_COLS), 1, gz);
+        if (nbw <= 2 && deep)
+            ft_bf16p8_rot_kernel<2, true, true><<<g8, 512, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu,
+                                                                    bmin, bmax, up, omega, nbw, g2, nunits, ug);
+        else if (deep)
+            ft_bf16p8_rot_kernel<4, true, true><<<g8, 512, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu,
+                                                                    bmin, bmax, up, omega, nbw, g2, nunits, ug);
+        else if (nbw <= 2 && pfe)
+            ft_bf16p8_rot_kernel<2, true><<<g8, 512, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin,
+                                                              bmax, up, omega, nbw, g2, nunits, ug);
+        else if (pfe)
+            ft_bf16p8_rot_kernel<4, true><<<g8, 512, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin,
+                                                              bmax, up, omega, nbw, g2, nunits, ug);
+        else if (nbw <= 2)
             ft_bf16p8_rot_kernel<2><<<g8, 512, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax, up,
-                                                        omega, nbw, g2);
+                                                        omega, nbw, g2, nunits, ug);
         else
             ft_bf16p8_rot_kernel<4><<<g8, 512, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax, up,
-                                                        omega, nbw, g2);
+                                                        omega, nbw, g2, nunits, ug);
         EPHIP_LAUNCHED();
         return EPHIP_OK;
     }

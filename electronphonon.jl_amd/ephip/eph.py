@@ -532,8 +532,9 @@ class OuterKLoop:
         from . import tune as T
         n = min(self.nk_batch, self.st.ksurv.shape[0])
         if self.packed:
-            # the fused RR->kR + projection kernel has one shape; tile variants apply to the unfused one
-            res = {} if self.fused_proj else {
+            # the fused RR->kR + projection kernel has its own variants; tile shapes apply to the unfused one
+            res = {"proj": T.tune_fourier(T.KIND_FUSED_PROJ, lambda: self.run_kbatch(0, n), reps=reps,
+                                          variants=T.PROJ_VARIANTS)} if self.fused_proj else {
                 "packed": T.tune_fourier(T.KIND_PACKED_BF16, lambda: self.run_kbatch(0, n), reps=reps,
                                          variants=T.PACKED_VARIANTS)}
             if self.fused:

@@ -11,14 +11,15 @@ from __future__ import annotations
 
 from . import _lib as L
 
-KIND_FP32_WIDE, KIND_FP32_NARROW, KIND_FP64, KIND_PACKED_BF16, KIND_FUSED_ROT = 0, 1, 2, 3, 4
+KIND_FP32_WIDE, KIND_FP32_NARROW, KIND_FP64, KIND_PACKED_BF16, KIND_FUSED_ROT, KIND_FUSED_PROJ = 0, 1, 2, 3, 4, 5
 VARIANTS = (0, 1, 2)
 VARIANT_NAMES = {0: "single-buffered KC16", 1: "double-buffered KC16", 2: "double-buffered KC8"}
 PACKED_VARIANTS = (0, 1, 2, 3, 4, 5, 6, 7)
 PACKED_VARIANT_NAMES = {0: "128x128", 1: "128x128 + operand prefetch", 2: "192x64", 3: "192x64 XCD-grouped",
                         4: "192x64 XCD-grouped, mid-sequence LDS writes", 5: "as 4, pipelined", 6: "as 3, pipelined",
                         7: "192x128 eight waves, operand through LDS"}
-FUSED_VARIANTS = (0, 1, 2, 3, 4, 5)
+FUSED_VARIANTS = (0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12)
+PROJ_VARIANTS = (0, 1)
 
 
 def get_variant(kind: int) -> int:

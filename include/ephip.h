@@ -244,7 +244,12 @@ int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t nbw, int32
  * through LDS once for both column halves (single-unit calls without column lists). kind 4 = the fused kR->kq rotation kernel; variant 0 = phase pieces written
  * after the MFMA sequence, 1 = halfway through it, 2 = pipelined, 3 = pipelined and halfway, 4 = phases
  * of the uniform q-grid generated on the fly from per-axis tables (no phase-table reads), 5 = eight-
- * wave workgroups on 128 kept columns with the unit's operand staged through LDS once for both halves.
+ * wave workgroups on 128 kept columns with the unit's operand staged through LDS once for both halves,
+ * 6, 7, 8 = as 5 with the units dealt to the XCDs and walked in groups of 2, 4, 8, column tile by tile,
+ * 9, 10 = as 7, 8 with the epilogue's column metadata read before the main loop and the U_kq / u_ph
+ * values of the next column block fetched ahead, 11, 12 = as 9, 10 with the main loop's global loads
+ * issued two stages ahead. kind 5 = the fused RR->kR + projection kernel; variant 0 = loads one stage
+ * ahead, 1 = two stages ahead.
  * Process-wide setting (no counterpart in the reference, whose tuning constants are hard-coded:
  * src/wannier_to_bloch_batched.jl:397-406). Host-side, synchronous. */
 int ephip_ft_set_variant(int32_t kind, int32_t variant);

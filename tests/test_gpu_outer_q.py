@@ -147,7 +147,9 @@ def model16():
     return m, {"kgrid": (8, 8, 8), "qgrid": (4, 4, 4), "window": 0.5}
 
 
-@pytest.mark.parametrize("variant,nq", [(0, 4), (1, 4), (2, 4), (3, 4), (4, 4), (5, 4), (0, 8), (5, 8)])
+@pytest.mark.parametrize("variant,nq", [(0, 4), (1, 4), (2, 4), (3, 4), (4, 4), (5, 4), (6, 4), (7, 4), (8, 4),
+                                        (9, 4), (10, 4), (11, 4), (12, 4), (0, 8), (5, 8), (6, 8), (8, 8),
+                                        (10, 8), (12, 8)])
 def test_outer_k_fused_rotation(backend, model16, variant, nq):
     """Split-BF16 outer-k with the kR->kq Fourier fused to the rotations (nw 16, nm 12), every kernel
     variant: same kept-q lists as the unfused kernels, g2 within 1e-5 of them, and within 1e-4 of the
@@ -182,18 +184,19 @@ def test_outer_k_fused_rotation(backend, model16, variant, nq):
     assert rel < 1e-4, rel
 
 
-@pytest.mark.parametrize("nq", [4, 8])
-def test_outer_k_fused_projection(backend, model16, nq):
+@pytest.mark.parametrize("nq,variant", [(4, 0), (8, 0), (4, 1), (8, 1)])
+def test_outer_k_fused_projection(backend, model16, nq, variant):
     """Split-BF16 outer-k with RR->kR fused to the k-side window projection (nw 16): the unit operators
     A and the g2 payloads agree with the separate Fourier + projection kernels to 1e-5, and g2 with the
     oracle to 1e-4. Batches of 40 and 150 k give one and two column tiles of 128 k, with a partial tile."""
     from ephip import _lib as L
-    from ephip import eph
+    from ephip import eph, tune
     m, g = model16
     g = dict(g, qgrid=(nq, nq, nq))
     w = g["window"]
     st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
     nks = st.ksurv.shape[0]
+    tune.set_variant(tune.KIND_FUSED_PROJ, variant)
     for nb in (40, 150):
         fused = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_BF16X3)
         plain = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_BF16X3, fuse_projection=False)
@@ -211,5 +214,6 @@ def test_outer_k_fused_projection(backend, model16, nq):
                 assert np.array_equal(bf["cols"][u], bp["cols"][u])
                 d = np.abs(bf["g2"][u] - bp["g2"][u]).max(initial=0.0)
                 assert d <= 1e-5 * max(np.abs(bp["g2"][u]).max(initial=0.0), 1e-30)
+    tune.set_variant(tune.KIND_FUSED_PROJ, 0)
     rel = _outer_k_vs_oracle(m, g, st, rf, qstep=5 if nq == 4 else 37)
     assert rel < 1e-4, rel
