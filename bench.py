@@ -40,16 +40,24 @@ PEAK = {"fp32": 157.3, "bf16x3": 2500.0}  # TFLOP/s, MI355X dense MFMA (fp32, bf
 # passes calibrated on a 256 MiB copy (profiles/cfg2_fast_bf16_hbm_counters.md). Counters cannot be read
 # inside the timed run; a variant or batch without a measurement reports traffic null.
 TRAFFIC_BYTES_CFG2 = {
-    ("bf16x3", "kR->kq", 128, 0): (1592903 * 2 + 68234) * 1024,    # packed Fourier fused with the rotations
-    ("bf16x3", "RR->kR", 128, 4): (1498651 * 2 + 1053918) * 1024,  # packed Fourier, XCD-grouped tiles
+    ("bf16x3", "kR->kq", 384, 10): (2998082 * 2 + 204444) * 1024,   # fused kR->kq + rotations, 8 waves
+    ("bf16x3", "RR->kR", 384, "proj0"): (1827474 * 2 + 200604) * 1024,  # RR->kR fused with the projection
 }
 # kernel names by (stage, variant) for the split-BF16 fast path (tuner kinds 3 and 4)
 KERNEL_NAMES = {
-    ("RR->kR", "proj"): "ft_bf16p8_proj_kernel: eight-wave 192x128 packed split-BF16 RR->kR Fourier over Re "
-                        "fused with the k-side window projection",
+    ("RR->kR", "proj0"): "ft_bf16p8_proj_kernel: eight-wave 192x128 packed split-BF16 RR->kR Fourier over Re "
+                         "fused with the k-side window projection",
+    ("RR->kR", "proj1"): "ft_bf16p8_proj_kernel<deep>: eight-wave 192x128 packed split-BF16 RR->kR Fourier over "
+                         "Re fused with the k-side window projection, loads two stages ahead",
     ("RR->kR", 7): "ft_bf16p8_kernel: eight-wave 192x128 packed split-BF16 RR->kR Fourier over Re",
     ("kR->kq", 5): "ft_bf16p8_rot_kernel: eight-wave packed split-BF16 kR->kq Fourier over Rp fused with the "
                    "k+q/phonon rotations and g2",
+    ("kR->kq", 8): "ft_bf16p8_rot_kernel: eight-wave packed split-BF16 kR->kq Fourier over Rp fused with the "
+                   "k+q/phonon rotations and g2, units dealt to XCDs in groups of 8",
+    ("kR->kq", 10): "ft_bf16p8_rot_kernel<prefetched epilogue>: eight-wave packed split-BF16 kR->kq Fourier over "
+                    "Rp fused with the k+q/phonon rotations and g2, units dealt to XCDs in groups of 8",
+    ("kR->kq", 12): "ft_bf16p8_rot_kernel<prefetched epilogue, deep>: eight-wave packed split-BF16 kR->kq Fourier "
+                    "over Rp fused with the k+q/phonon rotations and g2, loads two stages ahead",
 }
 
 
@@ -72,36 +80,50 @@ def make_epmat(model_mod, m, workers=16):
     return ep
 
 
-def cpu_baseline(O, m, st, epmat_c64, iq, nk_sample, kgrid):
-    """Time the FP64 oracle (C restatement of the reference's CPU outer-q path, OpenMP) on one q:
-    RR->Rq for the q and the Rq->kq pairs of the first nk_sample surviving k. Returns pairs/s
-    extrapolated to one full q (RR->Rq once + |K_surv| pairs)."""
-    import torch
+def cpu_baseline(O, m, st, epmat_c64, nk_sample, kgrid, min_seconds):
+    """Time the FP64 oracle (C restatement of the reference's CPU outer-q path, OpenMP) on whole q
+    slices of the workload: for q = 0, then q spread over the grid, RR->Rq for the q and the Rq->kq
+    pairs of the first nk_sample surviving k (all of them when nk_sample < 0), until min_seconds of
+    CPU work are done. Returns the measured pairs/s (no extrapolation when nk_sample < 0)."""
     ksurv = st.ksurv.cpu().numpy()
     nks = len(ksurv)
-    ik = ksurv[:nk_sample]
+    ik = ksurv if nk_sample < 0 else ksurv[:nk_sample]
     xk = st.el.xk.cpu().numpy()
-    xq = st.ph.xq[iq].cpu().numpy()
-    ikq = O.hash_xk(xk[ik] + xq, kgrid)
-    need = np.unique(np.concatenate([ik, ikq]))
-    U = np.zeros((xk.shape[0], m.nw * m.nw), np.complex128)
-    U[need] = st.el.U[torch.from_numpy(need).to(st.el.U.device)].cpu().numpy().reshape(len(need), -1)
+    nq = st.ph.xq.shape[0]
     bmin = st.el.filt.band_min.cpu().numpy()
     bmax = st.el.filt.band_max.cpu().numpy()
-    uph = st.ph.u[iq].cpu().numpy().astype(np.complex128).ravel()
+    U = st.el.U.cpu().numpy().reshape(xk.shape[0], -1).astype(np.complex128)
     ep128 = epmat_c64.astype(np.complex128)
-    t0 = time.perf_counter()
-    eRpq = O.eph_RR_to_Rq(m.nw, m.nmodes, ep128, m.irvec_el, m.irvec_ep, xq, uph)
-    t1 = time.perf_counter()
-    O.outer_q_block(m.nw, m.nmodes, eRpq, m.irvec_el, xk, U, bmin, bmax, ik, ikq, st.nbw)
-    t2 = time.perf_counter()
-    per_k = (t2 - t1) / len(ik)
-    t_q = (t1 - t0) + per_k * nks
-    return {"value": nks / t_q, "unit": "pairs/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"FP64 C oracle (restatement of the reference CPU outer-q path, OpenMP): 1 q "
-                      f"(RR->Rq {t1 - t0:.2f} s) + Rq->kq for {len(ik)} of {nks} surviving k "
-                      f"({t2 - t1:.2f} s), extrapolated to one full q",
-            "seconds": t2 - t0}
+    t_rr = t_kq = 0.0
+    pairs = 0
+    qs = []
+    while not qs or t_rr + t_kq < min_seconds:
+        iq = (len(qs) * 4999) % nq
+        if iq in qs:
+            break
+        qs.append(iq)
+        xq = st.ph.xq[iq].cpu().numpy()
+        ikq = O.hash_xk(xk[ik] + xq, kgrid)
+        uph = st.ph.u[iq].cpu().numpy().astype(np.complex128).ravel()
+        t0 = time.perf_counter()
+        eRpq = O.eph_RR_to_Rq(m.nw, m.nmodes, ep128, m.irvec_el, m.irvec_ep, xq, uph)
+        t1 = time.perf_counter()
+        O.outer_q_block(m.nw, m.nmodes, eRpq, m.irvec_el, xk, U, bmin, bmax, ik, ikq, st.nbw)
+        t2 = time.perf_counter()
+        t_rr += t1 - t0
+        t_kq += t2 - t1
+        pairs += len(ik)
+    if len(ik) == nks:
+        value = pairs / (t_rr + t_kq)
+        how = "measured"
+    else:   # extrapolate each q slice to all surviving k
+        value = len(qs) * nks / (t_rr + t_kq * nks / len(ik))
+        how = f"Rq->kq extrapolated from {len(ik)} to {nks} k per q"
+    return {"value": value, "unit": "pairs/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"FP64 C oracle (restatement of the reference CPU outer-q path, OpenMP): {len(qs)} q "
+                      f"slice(s) (q = {qs[:4]}{'...' if len(qs) > 4 else ''}), each RR->Rq + Rq->kq for "
+                      f"{len(ik)} of {nks} surviving k; RR->Rq {t_rr:.2f} s, Rq->kq {t_kq:.2f} s; {how}",
+            "seconds": t_rr + t_kq}
 
 
 def transport_bench(args):
@@ -159,11 +181,16 @@ def main():
     ap.add_argument("--path", choices=["fast", "plain", "both"], default="both")
     ap.add_argument("--precision", choices=["fp32", "bf16x3"], default="bf16x3",
                     help="Fourier precision of the fast path (split-BF16 meets the 1e-4 |g|^2 bound)")
-    ap.add_argument("--cpu-sample", type=int, default=3000, help="k-points in the CPU baseline slice (0 = skip)")
+    ap.add_argument("--cpu-sample", type=int, default=-1,
+                    help="surviving k per q slice of the CPU baseline (-1 = all, 0 = skip the baseline)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="q slices are added to the CPU baseline until this much CPU time is spent")
     ap.add_argument("--config", default="cfg2")
     ap.add_argument("--tune", dest="tune", action="store_true", default=True,
                     help="autotune the Fourier kernel variant on the live batch first (default)")
     ap.add_argument("--no-tune", dest="tune", action="store_false")
+    ap.add_argument("--variant", action="append", default=[], metavar="KIND=V",
+                    help="pin a Fourier kernel variant (tuner kind=variant, repeatable); skips the tuner")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 256 MiB device-to-device copy kernel (counter calibration)")
     ap.add_argument("--transport", action="store_true",
@@ -263,7 +290,13 @@ def main():
     if args.path in ("fast", "both"):
         loop = eph.OuterKLoop(st, nk_batch=args.kbatch_k, precision=prec)
         fused = bool(getattr(loop, "fused", False))
-        if args.tune:
+        if args.variant:
+            for kv in args.variant:
+                kind, v = (int(x) for x in kv.split("="))
+                T.set_variant(kind, v)
+            results["pinned_variants"] = list(args.variant)
+            log(f"[bench] pinned variants {args.variant}")
+        elif args.tune:
             tuned = loop.tune()
             log(f"[bench] tuner (ms per k batch by variant): {tuned}")
             results["tuner"] = {str(k): ({str(a): b for a, b in v.items()} if isinstance(v, dict) else v)
@@ -302,7 +335,8 @@ def main():
                                krkq_avg_ms=1e3 * t_krkq / len(kin["kR->kq"]),
                                rrkr_flop_per_launch=f_rrkr, krkq_flop_per_launch=f_krkq_total / len(kin["kR->kq"]),
                                fused_rotation=fused, fused_projection=bool(getattr(loop, "fused_proj", False)),
-                               variant_rrkr=("proj" if getattr(loop, "fused_proj", False) else
+                               variant_rrkr=(f"proj{T.get_variant(T.KIND_FUSED_PROJ)}"
+                                             if getattr(loop, "fused_proj", False) else
                                              T.get_variant(T.KIND_PACKED_BF16)),
                                variant_krkq=T.get_variant(T.KIND_FUSED_ROT))
         log(f"[bench] fast outer-k {args.precision}: {results['fast']['value']:.3e} pairs/s; RR->kR "
@@ -310,11 +344,11 @@ def main():
             f"{results['fast']['krkq_tflops']:.1f} TFLOP/s ({results['fast']['krkq_avg_ms']:.2f} ms)")
 
     cpu = None
-    if args.cpu_sample > 0:
+    if args.cpu_sample != 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
         log("[bench] timing the CPU oracle slice")
-        cpu = cpu_baseline(O, m, st, epmat, 0, args.cpu_sample, grids["kgrid"])
+        cpu = cpu_baseline(O, m, st, epmat, args.cpu_sample, grids["kgrid"], args.cpu_seconds)
         log(f"[bench] cpu baseline {cpu['value']:.3e} pairs/s ({cpu['seconds']:.1f} s)")
 
     if "fast" in results:

@@ -533,13 +533,18 @@ class OuterKLoop:
         n = min(self.nk_batch, self.st.ksurv.shape[0])
         if self.packed:
             # the fused RR->kR + projection kernel has its own variants; tile shapes apply to the unfused one
-            res = {"proj": T.tune_fourier(T.KIND_FUSED_PROJ, lambda: self.run_kbatch(0, n), reps=reps,
-                                          variants=T.PROJ_VARIANTS)} if self.fused_proj else {
-                "packed": T.tune_fourier(T.KIND_PACKED_BF16, lambda: self.run_kbatch(0, n), reps=reps,
-                                         variants=T.PACKED_VARIANTS)}
+            if self.fused_proj and len(T.PROJ_TUNE) == 1:
+                T.set_variant(T.KIND_FUSED_PROJ, T.PROJ_TUNE[0])
+                res = {}
+            elif self.fused_proj:
+                res = {"proj": T.tune_fourier(T.KIND_FUSED_PROJ, lambda: self.run_kbatch(0, n), reps=reps,
+                                              variants=T.PROJ_TUNE)}
+            else:
+                res = {"packed": T.tune_fourier(T.KIND_PACKED_BF16, lambda: self.run_kbatch(0, n), reps=reps,
+                                                variants=T.PACKED_VARIANTS)}
             if self.fused:
                 res["fused"] = T.tune_fourier(T.KIND_FUSED_ROT, lambda: self.run_kbatch(0, n), reps=reps,
-                                              variants=T.FUSED_VARIANTS)
+                                              variants=T.FUSED_TUNE)
             return res
         return T.tune_fourier(T.KIND_FP32_WIDE, lambda: self.run_kbatch(0, n), reps=reps)
 

@@ -20,6 +20,10 @@ PACKED_VARIANT_NAMES = {0: "128x128", 1: "128x128 + operand prefetch", 2: "192x6
                         7: "192x128 eight waves, operand through LDS"}
 FUSED_VARIANTS = (0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12)
 PROJ_VARIANTS = (0, 1)
+# candidates the outer-k loop times by default: one representative per distinct design (variants that
+# measured within noise of, or clearly behind, these stay selectable by set_variant)
+FUSED_TUNE = (0, 5, 8, 10, 12)
+PROJ_TUNE = (0,)
 
 
 def get_variant(kind: int) -> int:
