@@ -1476,7 +1476,10 @@ __device__ __forceinline__ void ftr_epilogue_cols(const f32x4 (&accr)[3][4], con
     constexpr int RB = 3, CB = 4, NW = 16, NM = 12;
     const int lane = threadIdx.x & 63;
     const int c = lane >> 4, il = lane & 15;
-    cplx<float> w[2][NBW][4], p[2][RB][3];
+    // two register sets (the next block fetched during this one) for NBW <= 2; one set for wider
+    // windows, whose doubled U_kq values would not fit beside the accumulators
+    constexpr int NSET = NBW <= 2 ? 2 : 1;
+    cplx<float> w[NSET][NBW][4], p[NSET][RB][3];
     auto fetch = [&](int cb, cplx<float> (&wd)[NBW][4], cplx<float> (&pd)[RB][3]) {
         const cplx<float> *uk = U + (int64_t)NW * NW * mc.kq[cb];
         const cplx<float> *up = uph + (int64_t)NM * NM * mc.iq[cb];
@@ -1490,11 +1493,15 @@ __device__ __forceinline__ void ftr_epilogue_cols(const f32x4 (&accr)[3][4], con
 #pragma unroll
             for (int k = 0; k < 3; k++) pd[rb][k] = up[(3 * wr + rb) + NM * (c + 4 * k)];
     };
-    fetch(0, w[0], p[0]);
+    if constexpr (NSET == 2) fetch(0, w[0], p[0]);
 #pragma unroll
     for (int cb = 0; cb < CB; cb++) {
-        const int cur = cb & 1;
-        if (cb + 1 < CB) fetch(cb + 1, w[cur ^ 1], p[cur ^ 1]);
+        const int cur = NSET == 2 ? (cb & 1) : 0;
+        if constexpr (NSET == 2) {
+            if (cb + 1 < CB) fetch(cb + 1, w[cur ^ 1], p[cur ^ 1]);
+        } else {
+            fetch(cb, w[0], p[0]);
+        }
         const int jj = jw + cb * 16 + il;
         float pr[NBW][3], pi[NBW][3];
 #pragma unroll
@@ -1648,25 +1655,28 @@ __global__ __launch_bounds__(512, 2) void ft_bf16p8_rot_kernel(int nrs, const __
         tile = blockIdx.x;
     }
     const int nk = ncol[u];
-    const int64_t j0 = (int64_t)tile * TN;
-    if (j0 >= nk) return;  // uniform for the block
-    f32x4 accr[3][4], acci[3][4];
-    if constexpr (PFE) {
-        FtrCols mc;
-        ftr_columns(mc, 64 * wc, u, nk, j0, col_index, ikq, cs, bmin, bmax, nbw);
-        if constexpr (DEEP)
-            ft8_mainloop_deep(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk, col_index + u * cs, j0,
-                              0, As, Bs, accr, acci);
-        else
+    // column tiles tile, tile + gridDim.x, ... of the unit: the grid may be sized below the longest kept
+    // list (the host's bound on count), and a list longer than the bound is still covered
+    for (int64_t j0 = (int64_t)tile * TN; j0 < nk; j0 += (int64_t)gridDim.x * TN) {  // uniform for the block
+        f32x4 accr[3][4], acci[3][4];
+        if constexpr (PFE) {
+            FtrCols mc;
+            ftr_columns(mc, 64 * wc, u, nk, j0, col_index, ikq, cs, bmin, bmax, nbw);
+            if constexpr (DEEP)
+                ft8_mainloop_deep(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk, col_index + u * cs,
+                                  j0, 0, As, Bs, accr, acci);
+            else
+                ft8_mainloop(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk, col_index + u * cs, j0,
+                             0, As, Bs, accr, acci);
+            ftr_epilogue_cols<NBW, TN>(accr, acci, wr, 64 * wc, mc, u, nk, j0, col_index, cs, U, uph, omega, nbw,
+                                       g2, reinterpret_cast<float *>(As));
+        } else {
             ft8_mainloop(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk, col_index + u * cs, j0, 0,
                          As, Bs, accr, acci);
-        ftr_epilogue_cols<NBW, TN>(accr, acci, wr, 64 * wc, mc, u, nk, j0, col_index, cs, U, uph, omega, nbw, g2,
-                                   reinterpret_cast<float *>(As));
-    } else {
-        ft8_mainloop(NDATA, nrs, apk + (int64_t)u * nrs * NDATA * 64, bpk, npts, nk, col_index + u * cs, j0, 0, As,
-                     Bs, accr, acci);
-        ftr_epilogue<NBW, TN>(accr, acci, wr, 64 * wc, u, nk, j0, col_index, ikq, cs, U, bmin, bmax, uph, omega, nbw,
-                              g2, reinterpret_cast<float *>(As));
+            ftr_epilogue<NBW, TN>(accr, acci, wr, 64 * wc, u, nk, j0, col_index, ikq, cs, U, bmin, bmax, uph, omega,
+                                  nbw, g2, reinterpret_cast<float *>(As));
+        }
+        __syncthreads();  // the epilogue's reads of the ep tile finish before the next tile's staging writes
     }
 }
 
@@ -2020,7 +2030,8 @@ extern "C" int ephip_eph_RR_to_kR_project_packed_bf16x3(int32_t nw, int32_t nm, 
 
 extern "C" int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t nbw, int32_t nr, const void *apk,
                                                  int32_t nunits, const void *bpk, int64_t npts, int64_t cs,
-                                                 const int32_t *count, const int32_t *col_index, const int32_t *ikq,
+                                                 int64_t max_count, const int32_t *count, const int32_t *col_index,
+                                                 const int32_t *ikq,
                                                  const void *U, const int64_t *bmin, const int64_t *bmax,
                                                  const void *uph, const double *omega, float *g2,
                                                  const int32_t *irvec, const int32_t *qgrid, void *stream) {
@@ -2058,25 +2069,19 @@ extern "C" int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t
         const int ug = fv == 5 ? 0 : (fv == 6 ? 2 : ((fv == 7 || fv == 9 || fv == 11) ? 4 : 8));
         const bool pfe = fv >= 9, deep = fv >= 11;
         const unsigned gz = ug > 0 ? (unsigned)(8 * ((nunits + 7) / 8)) : (unsigned)nunits;
-        dim3 g8((unsigned)((cs + FT8_COLS - 1) / FT8_COLS), 1, gz);
-        if (nbw <= 2 && deep)
-            ft_bf16p8_rot_kernel<2, true, true><<<g8, 512, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu,
-                                                                    bmin, bmax, up, omega, nbw, g2, nunits, ug);
-        else if (deep)
-            ft_bf16p8_rot_kernel<4, true, true><<<g8, 512, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu,
-                                                                    bmin, bmax, up, omega, nbw, g2, nunits, ug);
-        else if (nbw <= 2 && pfe)
-            ft_bf16p8_rot_kernel<2, true><<<g8, 512, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin,
-                                                              bmax, up, omega, nbw, g2, nunits, ug);
+        const int64_t ncols = max_count > 0 ? min(max_count, cs) : cs;  // grid width; longer lists loop
+        dim3 g8((unsigned)((ncols + FT8_COLS - 1) / FT8_COLS), 1, gz);
+        // instantiations by window width: 2 (nbw <= 2), 3, 4; the two-stage-ahead variants by 2 and 4
+        auto kern = ft_bf16p8_rot_kernel<2>;
+        if (deep)
+            kern = nbw <= 2 ? ft_bf16p8_rot_kernel<2, true, true> : ft_bf16p8_rot_kernel<4, true, true>;
         else if (pfe)
-            ft_bf16p8_rot_kernel<4, true><<<g8, 512, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin,
-                                                              bmax, up, omega, nbw, g2, nunits, ug);
-        else if (nbw <= 2)
-            ft_bf16p8_rot_kernel<2><<<g8, 512, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax, up,
-                                                        omega, nbw, g2, nunits, ug);
+            kern = nbw <= 2 ? ft_bf16p8_rot_kernel<2, true>
+                            : (nbw == 3 ? ft_bf16p8_rot_kernel<3, true> : ft_bf16p8_rot_kernel<4, true>);
         else
-            ft_bf16p8_rot_kernel<4><<<g8, 512, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax, up,
-                                                        omega, nbw, g2, nunits, ug);
+            kern = nbw <= 2 ? ft_bf16p8_rot_kernel<2> : (nbw == 3 ? ft_bf16p8_rot_kernel<3> : ft_bf16p8_rot_kernel<4>);
+        kern<<<g8, 512, 0, st>>>(nrs, a, b, npts, count, col_index, ikq, cs, uu, bmin, bmax, up, omega, nbw, g2, nunits,
+                                 ug);
         EPHIP_LAUNCHED();
         return EPHIP_OK;
     }

@@ -60,7 +60,7 @@ SIGNATURES = {
     "ephip_eph_RR_to_kR_project_packed_bf16x3": [_i32, _i32, _i32, _i32, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp,
                                                  _i32, _vp, _i64, _vp],
     "ephip_fourier_packed_bf16x3": [_i64, _i32, _vp, _i32, _vp, _i64, _i32, _vp, _vp, _i64, _vp, _i64, _i64, _vp],
-    "ephip_eph_kq_rotate_packed_bf16x3": [_i32, _i32, _i32, _i32, _vp, _i32, _vp, _i64, _i64, _vp, _vp, _vp, _vp,
+    "ephip_eph_kq_rotate_packed_bf16x3": [_i32, _i32, _i32, _i32, _vp, _i32, _vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp,
                                           _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ephip_ft_set_variant": [_i32, _i32],
     "ephip_ft_get_variant": [_i32],

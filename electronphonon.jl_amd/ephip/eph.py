@@ -426,6 +426,7 @@ class OuterKLoop:
         self.unit_band = torch.from_numpy(band.astype(np.int32)).to(dev)
         self.xk_surv = st.el.xk.index_select(0, st.ksurv).contiguous()
         self.nq = st.ph.xq.shape[0]
+        self.max_kept = kept_q_bound(ks, st.kgrid, st.qgrid)
         self.cs = self.nq
         max_units = int(max(nb[i:i + self.nk_batch].sum() for i in range(0, max(len(ks), 1), self.nk_batch))) \
             if len(ks) else 1
@@ -499,7 +500,7 @@ class OuterKLoop:
             core.ft_pack_operand(self.A, nd2, self.nrp, nd2, nu, nd2 * self.nrp, out=self.apk_units)
             self._timed("kR->kq", None, lambda: L.call(
                 "ephip_eph_kq_rotate_packed_bf16x3", nw, nm, st.nbw, self.nrp, L.ptr(self.apk_units), nu,
-                L.ptr(self.bpk_q), self.nq, self.cs, L.ptr(self.count), L.ptr(self.col_index), L.ptr(self.ikq),
+                L.ptr(self.bpk_q), self.nq, self.cs, self.max_kept, L.ptr(self.count), L.ptr(self.col_index), L.ptr(self.ikq),
                 L.ptr(st.el.U), L.ptr(st.el.filt.band_min), L.ptr(st.el.filt.band_max), L.ptr(self.uph),
                 L.ptr(self.omega), L.ptr(self.g2), L.ptr(self.irvec_ep_dev), C.addressof(self.qgrid_c), stream))
         elif self.packed:
@@ -553,6 +554,24 @@ class OuterKLoop:
         k_stop = nks if k_stop is None else min(nks, k_stop)
         for i0 in range(k_start, k_stop, self.nk_batch):
             self.run_kbatch(i0, min(self.nk_batch, k_stop - i0), calculators)
+
+
+def kept_q_bound(ksurv, kgrid, qgrid) -> int:
+    """Largest number of kept q of any surviving k: k + q runs over the coset of k in the k grid modulo
+    the q grid (the q grid divides the k grid, both unshifted, third index fastest), so k keeps exactly
+    the surviving k of its coset. Sizes the fused kR->kq grid (a hint: its workgroups loop over tiles)."""
+    n = np.asarray(kgrid, dtype=np.int64)
+    m = np.asarray(qgrid, dtype=np.int64)
+    ks = np.asarray(ksurv, dtype=np.int64)
+    if len(ks) == 0:
+        return 1
+    if np.any(n % m):
+        return int(np.prod(m))
+    r = n // m
+    i1, rest = ks // (n[1] * n[2]), ks % (n[1] * n[2])
+    i2, i3 = rest // n[2], rest % n[2]
+    coset = ((i1 % r[0]) * r[1] + i2 % r[1]) * r[2] + i3 % r[2]
+    return int(np.bincount(coset, minlength=int(np.prod(r))).max())
 
 
 class UnitRecorder(AbstractCalculator):

@@ -22,7 +22,7 @@ FUSED_VARIANTS = (0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12)
 PROJ_VARIANTS = (0, 1)
 # candidates the outer-k loop times by default: one representative per distinct design (variants that
 # measured within noise of, or clearly behind, these stay selectable by set_variant)
-FUSED_TUNE = (0, 5, 8, 10, 12)
+FUSED_TUNE = (0, 5, 8, 10)
 PROJ_TUNE = (0,)
 
 

@@ -224,12 +224,15 @@ int ephip_eph_RR_to_kR_project_packed_bf16x3(int32_t nw, int32_t nm, int32_t nrp
  * uph, omega as in ephip_eph_rotate_kq_g2. Writes g2 (nbw x nm per kept (unit, q), [m, nu]) only.
  * irvec (int32 nr x 3, the Rp list) and qgrid (3 ints, n1 n2 n3 = npts, third index fastest, no shift)
  * are read by the tuner variant that generates the phases on the fly (kind 4, variant 4); they may be
- * NULL otherwise. */
+ * NULL otherwise. max_count (<= 0: cs) sizes the grid of the eight-wave variants (kind 4, 5..12), whose
+ * workgroups loop over the column tiles of their unit, so it is a hint (an upper bound on count[u] gives
+ * no idle workgroups) and a longer list is still processed in full. */
 int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t nbw, int32_t nr, const void *apk,
-                                      int32_t nunits, const void *bpk, int64_t npts, int64_t cs, const int32_t *count,
-                                      const int32_t *col_index, const int32_t *ikq, const void *U, const int64_t *bmin,
-                                      const int64_t *bmax, const void *uph, const double *omega, float *g2,
-                                      const int32_t *irvec, const int32_t *qgrid, void *stream);
+                                      int32_t nunits, const void *bpk, int64_t npts, int64_t cs, int64_t max_count,
+                                      const int32_t *count, const int32_t *col_index, const int32_t *ikq,
+                                      const void *U, const int64_t *bmin, const int64_t *bmax, const void *uph,
+                                      const double *omega, float *g2, const int32_t *irvec, const int32_t *qgrid,
+                                      void *stream);
 
 /* Kernel variant of the Fourier contractions, chosen by the tuner (ephip/tune.py) from timings on the
  * live shapes. kind: 0 = FP32 with nk > 64, 1 = FP32 with nk <= 64, 2 = FP64; variant: 0 = single-

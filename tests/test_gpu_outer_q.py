@@ -147,21 +147,25 @@ def model16():
     return m, {"kgrid": (8, 8, 8), "qgrid": (4, 4, 4), "window": 0.5}
 
 
-@pytest.mark.parametrize("variant,nq", [(0, 4), (1, 4), (2, 4), (3, 4), (4, 4), (5, 4), (6, 4), (7, 4), (8, 4),
-                                        (9, 4), (10, 4), (11, 4), (12, 4), (0, 8), (5, 8), (6, 8), (8, 8),
-                                        (10, 8), (12, 8)])
-def test_outer_k_fused_rotation(backend, model16, variant, nq):
+@pytest.mark.parametrize("variant,nq,window", [
+    (0, 4, 0.5), (1, 4, 0.5), (2, 4, 0.5), (3, 4, 0.5), (4, 4, 0.5), (5, 4, 0.5), (6, 4, 0.5), (7, 4, 0.5),
+    (8, 4, 0.5), (9, 4, 0.5), (10, 4, 0.5), (11, 4, 0.5), (12, 4, 0.5), (0, 8, 0.5), (5, 8, 0.5), (6, 8, 0.5),
+    (8, 8, 0.5), (10, 8, 0.5), (12, 8, 0.5),
+    (0, 4, 0.8), (5, 4, 0.8), (10, 4, 0.8), (10, 8, 0.8), (12, 4, 0.8),
+    (0, 4, 1.5), (4, 4, 1.5), (5, 4, 1.5), (10, 4, 1.5), (12, 4, 1.5)])
+def test_outer_k_fused_rotation(backend, model16, variant, nq, window):
     """Split-BF16 outer-k with the kR->kq Fourier fused to the rotations (nw 16, nm 12), every kernel
     variant: same kept-q lists as the unfused kernels, g2 within 1e-5 of them, and within 1e-4 of the
-    oracle. The 8^3 q-grid gives units several column tiles of both widths (64 and 128)."""
+    oracle. The 8^3 q-grid gives units several column tiles of both widths (64 and 128); windows of
+    0.5, 0.8 and 1.5 eV give nbw = 2, 3 and 4 (the kernels' window-width instantiations)."""
     from ephip import _lib as L
     from ephip import eph, tune
     tune.set_variant(tune.KIND_FUSED_ROT, variant)
     m, g = model16
-    g = dict(g, qgrid=(nq, nq, nq))
+    g = dict(g, qgrid=(nq, nq, nq), window=window)
     w = g["window"]
     st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
-    assert 1 <= st.nbw <= 4
+    assert st.nbw == {0.5: 2, 0.8: 3, 1.5: 4}[window]
     fused = eph.OuterKLoop(st, nk_batch=40, precision=L.PREC_BF16X3)
     plain = eph.OuterKLoop(st, nk_batch=40, precision=L.PREC_BF16X3, fuse_rotation=False)
     assert fused.fused and not plain.fused and fused.Gq is None
@@ -179,6 +183,18 @@ def test_outer_k_fused_rotation(backend, model16, variant, nq):
             num = max(num, float(np.abs(bf["g2"][u] - bp["g2"][u]).max(initial=0.0)))
             den = max(den, float(np.abs(bp["g2"][u]).max(initial=0.0)))
     assert den > 0 and num / den < 1e-5, num / den
+    if variant >= 5:
+        # eight-wave variants: a grid of one column tile (workgroups loop over the unit's tiles) gives the
+        # same g2 bit for bit as the grid sized by the kept-q bound
+        assert fused.max_kept >= max(len(c) for b in rf.batches for c in b["cols"])
+        narrow = eph.OuterKLoop(st, nk_batch=40, precision=L.PREC_BF16X3)
+        narrow.max_kept = 1
+        rn = eph.UnitRecorder()
+        for i0 in (0, max(0, nks - 40)):
+            narrow.run_kbatch(i0, 40, [rn])
+        for bf, bn in zip(rf.batches, rn.batches):
+            for u in range(len(bf["cols"])):
+                assert np.array_equal(bf["g2"][u], bn["g2"][u])
     rel = _outer_k_vs_oracle(m, g, st, rf, qstep=5 if nq == 4 else 37)
     tune.set_variant(tune.KIND_FUSED_ROT, 0)
     assert rel < 1e-4, rel

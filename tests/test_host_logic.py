@@ -33,3 +33,28 @@ def test_outer_k_bytes_hand_formula():
     assert committed == 2 * 8 * nw * nw * nm * nre * nrp + (8 * nw + 8 * nw * nw + 49) * nk + (
         8 * nm + 8 * nm * nm + 24) * nq
     assert 40e6 < per_k < 60e6 and 5.5e9 < committed < 6.5e9
+
+
+@pytest.mark.parametrize("kgrid,qgrid", [((8, 8, 8), (4, 4, 4)), ((6, 4, 9), (3, 2, 3)), ((4, 4, 4), (4, 4, 4)),
+                                         ((6, 6, 6), (2, 3, 1))])
+def test_kept_q_bound_brute_force(kgrid, qgrid):
+    """The coset bound equals the largest kept-q count found by adding every q to every surviving k
+    on the grids (unshifted, third index fastest)."""
+    import numpy as np
+
+    from ephip import eph
+    rng = np.random.default_rng(sum(kgrid) + sum(qgrid))
+    nk = int(np.prod(kgrid))
+    surv = np.sort(rng.choice(nk, size=max(1, nk // 3), replace=False))
+    keep = np.zeros(nk, bool)
+    keep[surv] = True
+    n = np.array(kgrid)
+    r = n // np.array(qgrid)
+    qs = np.array([(a, b, c) for a in range(qgrid[0]) for b in range(qgrid[1]) for c in range(qgrid[2])])
+    best = 0
+    for k in surv:
+        i = np.array([k // (n[1] * n[2]), (k // n[2]) % n[1], k % n[2]])
+        j = (i + qs * r) % n
+        h = (j[:, 0] * n[1] + j[:, 1]) * n[2] + j[:, 2]
+        best = max(best, int(keep[h].sum()))
+    assert eph.kept_q_bound(surv, kgrid, qgrid) == best
