@@ -200,7 +200,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
+                                   "(multi-GPU sharding is a design note only, not implemented)"}))
         sys.exit(2)
 
     if args.transport:

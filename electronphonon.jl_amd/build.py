@@ -17,8 +17,8 @@ OUT = os.path.join(HERE, "libephip.so")
 BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("EPHIP_OFFLOAD_ARCH", "gfx950")
 
-# Packed FP32 VALU ops are switched off for device code; hipcc's host pass does not
-# know the feature and warns, which is harmless.
+# Packed FP32 VALU ops are switched off for device code (some of their forms have given wrong results on
+# this hardware); hipcc's host pass does not know the feature and warns, which is harmless.
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wall", "-Wno-unused-result", "-Wno-unused-function"]

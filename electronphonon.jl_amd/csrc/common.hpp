@@ -47,6 +47,12 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 
 namespace ephip {
 
+// Complex value with interleaved (re, im), the layout of complex64/complex128 device arrays.
+template <class T>
+struct cplx {
+    T re, im;
+};
+
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));

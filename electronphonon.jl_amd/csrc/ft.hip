@@ -22,11 +22,6 @@
 namespace ephip {
 
 template <class T>
-struct cplx {
-    T re, im;
-};
-
-template <class T>
 __device__ __forceinline__ typename std::conditional<sizeof(T) == 4, f32x4, f64x4>::type
 mfma16x16x4(T a, T b, typename std::conditional<sizeof(T) == 4, f32x4, f64x4>::type c) {
     if constexpr (sizeof(T) == 4)

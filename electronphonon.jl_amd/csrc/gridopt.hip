@@ -1,0 +1,243 @@
+// Staged ("GridOpt") Fourier transform of the e-ph tensor over Re for k-points that share coordinates,
+// the device form of src/wannier/GridOpt.jl (gridopt_set23! :73-107, gridopt_set3! :110-131,
+// gridopt_get3! :133-139) and of the batched k3 step of src/wannier/batched_gridopt_interpolator.jl,
+// with the outer-k window projection (src/calculator/run_eph_over_k_and_kq.jl:657-659) fused in.
+//
+// The Re set is embedded in its bounding box r = (r1min + a, r2min + b, r3min + c), a < n1, b < n2,
+// c < n3; rmap[a + n1 (b + n2 c)] is the column of that R in the operand, or -1 (absent: zero); the
+// (b, c) face is carried padded to np x np, np = max(n2, n3, 3). Then
+//   e^{2 pi i k.R} = e^{2 pi i k1 r1} e^{2 pi i k2 r2} e^{2 pi i k3 r3}
+// and g(k) = sum_c e3 sum_b e2 sum_a e1 op[., (a, b, c)]:
+//   stage 1 (once per k1 plane): op1[., (b, c)] = sum_a e^{2 pi i k1 r1} op[., (a, b, c)];
+//   stage 2 (once per (k1, k2) line): op2[., c] = sum_b e^{2 pi i k2 r2} op1[., (b, c)];
+//   stage 3 (per k): g(k) = sum_c e^{2 pi i k3 r3} op2[., c], projected on the k-side window bands.
+// All in FP32 with phases from FP64 arguments reduced mod 1. Flops per k fall from 8 nd nr to about
+// 8 nd (n3 + n2 n3 / (k per line)) plus stage 1 per plane, which streams the operand once.
+#include "common.hpp"
+
+namespace ephip {
+
+constexpr int GO_N1_MAX = 32;
+
+// Stage 1. One thread per operand row d; op1 rows are written in the j-fastest order
+// d' = j + nb (i + nb rest) of d = i + nb (j + nb rest) when nb > 1 (the order the projection reduces
+// over), in order d when nb <= 1. The (b, c) face is written padded to np x np (zeros outside n2 x n3),
+// so the next stage runs on a fixed square face.
+__global__ __launch_bounds__(256) void gridopt_stage1_kernel(int64_t nd, int n1, int n2, int n3, int np, int r1min,
+                                                             const int *__restrict__ rmap,
+                                                             const cplx<float> *__restrict__ op, int64_t ldop,
+                                                             double x1, int nb, cplx<float> *__restrict__ op1) {
+    __shared__ float2 ph[GO_N1_MAX];
+    if ((int)threadIdx.x < n1) {
+        float c, s;
+        cis2pi(x1 * (double)(r1min + (int)threadIdx.x), c, s);
+        ph[threadIdx.x] = make_float2(c, s);
+    }
+    __syncthreads();
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= nd) return;
+    int64_t dd = d;
+    if (nb > 1) {
+        const int64_t i = d % nb, j = (d / nb) % nb, rest = d / ((int64_t)nb * nb);
+        dd = j + nb * (i + nb * rest);
+    }
+    const cplx<float> *o = op + d;
+    for (int c = 0; c < np; c++)
+        for (int b = 0; b < np; b++) {
+            float sr = 0.f, si = 0.f;
+            if (b < n2 && c < n3) {
+                const int bc = b + n2 * c;
+#pragma unroll 8
+                for (int a = 0; a < n1; a++) {
+                    const int ir = rmap[a + n1 * bc];  // uniform across the block
+                    if (ir < 0) continue;
+                    const cplx<float> v = o[ldop * ir];
+                    const float2 p = ph[a];
+                    sr += p.x * v.re - p.y * v.im;
+                    si += p.x * v.im + p.y * v.re;
+                }
+            }
+            op1[dd + nd * (b + np * c)] = cplx<float>{sr, si};
+        }
+}
+
+// Stages 2 and 3 with the window projection, for the k of one k1 plane. One thread per row d'
+// (j-fastest, NW consecutive lanes = one j group of G = d' / NW); the thread holds op1[d', (b, c)] in
+// registers for the whole launch. For each line l in [l0, l1) (surviving k line_start[l] ..
+// line_start[l+1]-1, clipped to the batch [k0, k1)), op2 = sum_b e2 op1, then per k of the line
+// g = sum_c e3 op2 and, per window band n of k (units kus[k] .. kus[k+1]-1, relative to kus[k0]):
+//   A[u][G] = sum_j g[j + NW G] U_k[j, n]   (an NW-lane shuffle reduction).
+// xk holds the surviving k (crystal coordinates, 3 per k, FP64).
+constexpr int GO_KLINE_MAX = 64, GO_NBW_MAX = 4;
+
+// Sum over groups of NW = 8 or 16 consecutive lanes with DPP moves inside a 16-lane row (no LDS):
+// partner lane i^1 and i^2 (quad permutations), then i <-> 7-i within each 8-lane half (pairs quads
+// 0/1 and 2/3), then for NW = 16 i <-> 15-i within the row (pairs the halves). Every lane of the group
+// ends with the group's sum.
+template <int NW>
+__device__ __forceinline__ float dpp_group_sum(float v) {
+    auto step = [](float x, int ctrl_sel) {
+        int xi = __builtin_bit_cast(int, x), y;
+        switch (ctrl_sel) {
+            case 0: y = __builtin_amdgcn_update_dpp(0, xi, 0xB1, 0xF, 0xF, false); break;   // quad_perm [1,0,3,2]
+            case 1: y = __builtin_amdgcn_update_dpp(0, xi, 0x4E, 0xF, 0xF, false); break;   // quad_perm [2,3,0,1]
+            case 2: y = __builtin_amdgcn_update_dpp(0, xi, 0x141, 0xF, 0xF, false); break;  // row_half_mirror
+            default: y = __builtin_amdgcn_update_dpp(0, xi, 0x140, 0xF, 0xF, false); break; // row_mirror
+        }
+        return x + __builtin_bit_cast(float, y);
+    };
+    v = step(v, 0);
+    v = step(v, 1);
+    v = step(v, 2);
+    if constexpr (NW == 16) v = step(v, 3);
+    return v;
+}
+
+template <int N, int NW>
+__global__ __launch_bounds__(256) void gridopt_kR_project_kernel(
+    int64_t nd, int r2min, int r3min, const cplx<float> *__restrict__ op1,
+    const double *__restrict__ xk, int64_t k0, int64_t k1, int l0, int l1, const int *__restrict__ line_start,
+    const int *__restrict__ kus, const int64_t *__restrict__ ustate, const int *__restrict__ uband,
+    const cplx<float> *__restrict__ U, int nbw, cplx<float> *__restrict__ A, int64_t lda) {
+    // per line: stage-2 phases, stage-3 phases of its k, and each k's first unit, unit count and U_k
+    // columns of its window bands (read once per line instead of by every thread per k)
+    constexpr int N2 = N, N3 = N;
+    __shared__ float2 ph2[N2];
+    __shared__ float2 ph3[GO_KLINE_MAX * N3];
+    __shared__ int kub[GO_KLINE_MAX], knb[GO_KLINE_MAX];
+    __shared__ float2 uc[GO_KLINE_MAX * GO_NBW_MAX * NW];
+    const int t = threadIdx.x;
+    const int64_t dp = (int64_t)blockIdx.x * blockDim.x + t;
+    const bool dv = dp < nd;
+    const int j = t % NW;
+    const int64_t G = dp / NW;
+    float vr[N3][N2], vi[N3][N2];
+#pragma unroll
+    for (int c = 0; c < N3; c++)
+#pragma unroll
+        for (int b = 0; b < N2; b++) {
+            const cplx<float> v = dv ? op1[dp + nd * (b + N2 * c)] : cplx<float>{0.f, 0.f};
+            vr[c][b] = v.re;
+            vi[c][b] = v.im;
+        }
+    const int ku0 = kus[k0];
+    for (int l = l0; l < l1; l++) {
+        const int64_t kb = max((int64_t)line_start[l], k0), ke = min((int64_t)line_start[l + 1], k1);
+        const int nkl = (int)(ke - kb);  // uniform; at most GO_KLINE_MAX (host check)
+        if (nkl <= 0) continue;
+        __syncthreads();  // the previous line's tables are no longer read
+        if (t < N2) {
+            float c, s;
+            cis2pi(xk[3 * kb + 1] * (double)(r2min + t), c, s);
+            ph2[t] = make_float2(c, s);
+        }
+        for (int e = t; e < nkl * N3; e += blockDim.x) {
+            const int kk = e / N3, c3 = e % N3;
+            float c, s;
+            cis2pi(xk[3 * (kb + kk) + 2] * (double)(r3min + c3), c, s);
+            ph3[kk * N3 + c3] = make_float2(c, s);
+        }
+        for (int kk = t; kk < nkl; kk += blockDim.x) {
+            kub[kk] = kus[kb + kk] - ku0;
+            knb[kk] = min(kus[kb + kk + 1] - kus[kb + kk], nbw);
+        }
+        __syncthreads();
+        for (int e = t; e < nkl * nbw * NW; e += blockDim.x) {
+            const int kk = e / (nbw * NW), m = (e / NW) % nbw, jj = e % NW;
+            float2 w = make_float2(0.f, 0.f);
+            if (m < knb[kk]) {
+                const int ub = kub[kk];
+                const cplx<float> v = U[(int64_t)NW * NW * ustate[ub] + jj + NW * uband[ub + m]];
+                w = make_float2(v.re, v.im);
+            }
+            uc[(kk * GO_NBW_MAX + m) * NW + jj] = w;
+        }
+        float o2r[N3], o2i[N3];
+#pragma unroll
+        for (int c = 0; c < N3; c++) {
+            float sr = 0.f, si = 0.f;
+#pragma unroll
+            for (int b = 0; b < N2; b++) {
+                const float2 p = ph2[b];
+                sr += p.x * vr[c][b] - p.y * vi[c][b];
+                si += p.x * vi[c][b] + p.y * vr[c][b];
+            }
+            o2r[c] = sr;
+            o2i[c] = si;
+        }
+        __syncthreads();  // U columns staged
+        for (int kk = 0; kk < nkl; kk++) {
+            float gr = 0.f, gi = 0.f;
+#pragma unroll
+            for (int c = 0; c < N3; c++) {
+                const float2 p = ph3[kk * N3 + c];
+                gr += p.x * o2r[c] - p.y * o2i[c];
+                gi += p.x * o2i[c] + p.y * o2r[c];
+            }
+            const int ub = kub[kk], nbk = knb[kk];
+            for (int m = 0; m < nbk; m++) {  // uniform across the block
+                const float2 w = uc[(kk * GO_NBW_MAX + m) * NW + j];
+                const float sr = dpp_group_sum<NW>(gr * w.x - gi * w.y);
+                const float si = dpp_group_sum<NW>(gr * w.y + gi * w.x);
+                if (dv && j == 0) A[(int64_t)(ub + m) * lda + G] = cplx<float>{sr, si};
+            }
+        }
+    }
+}
+
+}  // namespace ephip
+
+using namespace ephip;
+
+extern "C" int ephip_gridopt_stage1(int64_t nd, int32_t n1, int32_t n2, int32_t n3, int32_t np, int32_t r1min,
+                                    const int32_t *rmap, const void *op, int64_t ldop, double x1, int32_t nb,
+                                    void *op1, void *stream) {
+    EPHIP_REQUIRE(nd >= 0 && n1 >= 1 && n2 >= 1 && n3 >= 1 && n1 <= GO_N1_MAX && ldop >= nd && np >= n2 && np >= n3,
+                  "gridopt_stage1: bad sizes (nd=%lld n1=%d n2=%d n3=%d np=%d, n1 <= %d)", (long long)nd, n1, n2, n3,
+                  np, GO_N1_MAX);
+    EPHIP_REQUIRE(nb <= 1 || nd % ((int64_t)nb * nb) == 0, "gridopt_stage1: nd not a multiple of nb^2");
+    if (nd == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(rmap && op && op1, "gridopt_stage1: null device pointer");
+    gridopt_stage1_kernel<<<(unsigned)((nd + 255) / 256), 256, 0, as_stream(stream)>>>(
+        nd, n1, n2, n3, np, r1min, rmap, static_cast<const cplx<float> *>(op), ldop, x1, nb,
+        static_cast<cplx<float> *>(op1));
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_gridopt_kR_project(int32_t nw, int32_t nm, int32_t nrp, int32_t np, int32_t r2min,
+                                        int32_t r3min, const void *op1, const double *xk, int64_t k0, int64_t k1,
+                                        int32_t l0, int32_t l1, const int32_t *line_start, int32_t max_line_k,
+                                        const int32_t *kus, const int64_t *ustate, const int32_t *uband,
+                                        const void *U, int32_t nbw, void *A, int64_t lda, void *stream) {
+    EPHIP_REQUIRE(nw == 8 || nw == 16, "gridopt_kR_project: nw must be 8 or 16 (got %d)", nw);
+    EPHIP_REQUIRE(nm >= 1 && nrp >= 1 && nbw >= 1 && nbw <= GO_NBW_MAX && k0 >= 0 && k1 >= k0 && l1 >= l0 &&
+                      l0 >= 0,
+                  "gridopt_kR_project: bad sizes (nbw=%d, at most %d window bands)", nbw, GO_NBW_MAX);
+    EPHIP_REQUIRE(np >= 3 && np <= 8, "gridopt_kR_project: padded box face np x np with 3 <= np <= 8 (got %d)", np);
+    EPHIP_REQUIRE(max_line_k >= 0 && max_line_k <= GO_KLINE_MAX, "gridopt_kR_project: %d k on a line > %d",
+                  max_line_k, GO_KLINE_MAX);
+    EPHIP_REQUIRE(lda >= (int64_t)nw * nm * nrp, "gridopt_kR_project: lda smaller than nw nm nrp");
+    const int64_t nd = (int64_t)nw * nw * nm * nrp;
+    if (k1 == k0 || l1 == l0) return EPHIP_OK;
+    EPHIP_REQUIRE(op1 && xk && line_start && kus && ustate && uband && U && A,
+                  "gridopt_kR_project: null device pointer");
+    const unsigned g = (unsigned)((nd + 255) / 256);
+    auto o1 = static_cast<const cplx<float> *>(op1);
+    auto uu = static_cast<const cplx<float> *>(U);
+    auto aa = static_cast<cplx<float> *>(A);
+    hipStream_t st = as_stream(stream);
+    // one instantiation per padded face size (registers hold the thread's np x np stage-1 values)
+    using K = decltype(&gridopt_kR_project_kernel<3, 16>);
+    static const K k16[6] = {gridopt_kR_project_kernel<3, 16>, gridopt_kR_project_kernel<4, 16>,
+                             gridopt_kR_project_kernel<5, 16>, gridopt_kR_project_kernel<6, 16>,
+                             gridopt_kR_project_kernel<7, 16>, gridopt_kR_project_kernel<8, 16>};
+    static const K k8[6] = {gridopt_kR_project_kernel<3, 8>, gridopt_kR_project_kernel<4, 8>,
+                            gridopt_kR_project_kernel<5, 8>, gridopt_kR_project_kernel<6, 8>,
+                            gridopt_kR_project_kernel<7, 8>, gridopt_kR_project_kernel<8, 8>};
+    const K kern = (nw == 16 ? k16 : k8)[np - 3];
+    kern<<<g, 256, 0, st>>>(nd, r2min, r3min, o1, xk, k0, k1, l0, l1, line_start, kus, ustate, uband, uu, nbw, aa,
+                            lda);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}

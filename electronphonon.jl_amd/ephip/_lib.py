@@ -56,6 +56,9 @@ SIGNATURES = {
                              C.c_float, _vp, _vp, _vp],
     "ephip_ft_pack_operand": [_i64, _i32, _vp, _i64, _i32, _i64, _vp, _vp],
     "ephip_ft_pack_operand_swapped": [_i64, _i32, _i32, _vp, _i64, _vp, _vp],
+    "ephip_gridopt_stage1": [_i64, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i64, C.c_double, _i32, _vp, _vp],
+    "ephip_gridopt_kR_project": [_i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i64, _i64, _i32, _i32, _vp,
+                                 _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i64, _vp],
     "ephip_ft_pack_phases": [_i32, _vp, _i64, _vp, _vp, _vp],
     "ephip_eph_RR_to_kR_project_packed_bf16x3": [_i32, _i32, _i32, _i32, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp,
                                                  _i32, _vp, _i64, _vp],

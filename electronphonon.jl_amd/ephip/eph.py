@@ -369,7 +369,8 @@ class OuterKLoop:
     Pairs (k, q) with no in-window band at k+q are exactly zero and are not computed."""
 
     def __init__(self, st: OuterQSetup, nk_batch: int | None = 128, precision: int = L.PREC_FP32,
-                 nk_batch_cap: int = 512, fuse_rotation: bool = True, fuse_projection: bool = True):
+                 nk_batch_cap: int = 512, fuse_rotation: bool = True, fuse_projection: bool = True,
+                 gridopt: bool = True):
         torch = _torch()
         self.st = st
         if nk_batch is None:  # memory-adaptive width (plan_batch), the e-ph tensor already resident
@@ -400,15 +401,24 @@ class OuterKLoop:
         # split-BF16 with nw = 16: RR->kR and the k-side window projection run as one kernel on the "el"
         # operand packed with j fastest; g(k, Rp) is never stored
         self.fused_proj = bool(fuse_projection) and self.packed and nw == 16 and self.nd_el % 192 == 0
+        # RR->kR as the staged (GridOpt) transform with the projection fused, in FP32 for either precision:
+        # k1 planes and (k1, k2) lines of the grid share their partial sums
+        ks_host = st.ksurv.cpu().numpy()
+        self.gridopt = _GridOptKR.build(st, ks_host, self.nd_el) if (gridopt and nw in (8, 16)) else None
+        if self.gridopt is not None:
+            self.fused_proj = False
         import ctypes
         self.qgrid_c = (ctypes.c_int32 * 3)(*map(int, st.qgrid))  # for the on-the-fly q phases
-        if self.fused_proj:
+        if self.gridopt is not None:
+            self.el_obj = None   # the complex64 "el" tensor stays: stage 1 streams it
+        elif self.fused_proj:
             self.apk_el = core.ft_pack_operand_swapped(self.epmat_el, self.nd_el, self.nre, nw, self.nd_el)
         elif self.packed:
             self.apk_el = core.ft_pack_operand(self.epmat_el, self.nd_el, self.nre, self.nd_el)
         if self.packed:
-            self.epmat_el = None
-            self.el_obj = None
+            if self.gridopt is None:
+                self.epmat_el = None
+                self.el_obj = None
             self.bpk_q = core.ft_pack_phases(self.irvec_ep_dev, st.ph.xq)
             self.bpk_k = torch.empty(core.ft_stages(self.nre) * self.nk_batch * 64, dtype=torch.bfloat16,
                                      device=dev)
@@ -432,8 +442,8 @@ class OuterKLoop:
             if len(ks) else 1
         self.max_units = max_units
         nd2 = nw * nm
-        self.G = None if self.fused_proj else torch.empty((self.nk_batch, self.nd_el), dtype=torch.complex64,
-                                                           device=dev)
+        self.G = None if (self.fused_proj or self.gridopt is not None) else torch.empty(
+            (self.nk_batch, self.nd_el), dtype=torch.complex64, device=dev)
         self.A = torch.empty((max_units, nd2 * self.nrp), dtype=torch.complex64, device=dev)
         self.col_index = torch.empty((max_units, self.cs), dtype=torch.int32, device=dev)
         self.ikq = torch.empty((max_units, self.cs), dtype=torch.int32, device=dev)
@@ -470,7 +480,10 @@ class OuterKLoop:
         stream = L.stream_handle()
         # 1. RR->kR Fourier over Re for the batch
         xk_b = self.xk_surv[i0:i0 + n]
-        if self.fused_proj:  # RR->kR with the projection in its epilogue: writes the unit operators A
+        if self.gridopt is not None:  # staged RR->kR with the projection fused: writes the unit operators A
+            self._timed("RR->kR", 8.0 * self.nd_el * self.nre * n,
+                        lambda: self.gridopt.run(self, i0, n, u0, u1, stream))
+        elif self.fused_proj:  # RR->kR with the projection in its epilogue: writes the unit operators A
             bpk = core.ft_pack_phases(st.irvec_el_dev, xk_b, out=self.bpk_k)
             self._timed("RR->kR", 8.0 * self.nd_el * self.nre * n, lambda: L.call(
                 "ephip_eph_RR_to_kR_project_packed_bf16x3", nw, nm, self.nrp, self.nre, L.ptr(self.apk_el), L.ptr(bpk),
@@ -483,7 +496,7 @@ class OuterKLoop:
         else:
             self._timed("RR->kR", 8.0 * self.nd_el * self.nre * n,
                         lambda: core.fourier_batched(self.el_obj, xk_b, out=self.G[:n], precision=self.precision))
-        if not self.fused_proj:
+        if not self.fused_proj and self.gridopt is None:
             # projection; the unit's column in G is its k position minus i0 (base pointer shifted by -i0 columns)
             g_base = L.ptr(self.G) - i0 * self.G.stride(0) * 8
             L.call("ephip_eph_project_kR", nw, nm, self.nrp, nu, g_base, self.G.stride(0),
@@ -534,7 +547,9 @@ class OuterKLoop:
         n = min(self.nk_batch, self.st.ksurv.shape[0])
         if self.packed:
             # the fused RR->kR + projection kernel has its own variants; tile shapes apply to the unfused one
-            if self.fused_proj and len(T.PROJ_TUNE) == 1:
+            if self.gridopt is not None:
+                res = {}
+            elif self.fused_proj and len(T.PROJ_TUNE) == 1:
                 T.set_variant(T.KIND_FUSED_PROJ, T.PROJ_TUNE[0])
                 res = {}
             elif self.fused_proj:
@@ -554,6 +569,82 @@ class OuterKLoop:
         k_stop = nks if k_stop is None else min(nks, k_stop)
         for i0 in range(k_start, k_stop, self.nk_batch):
             self.run_kbatch(i0, min(self.nk_batch, k_stop - i0), calculators)
+
+
+class _GridOptKR:
+    """Staged RR->kR over Re (ephip_gridopt_stage1 + ephip_gridopt_kR_project), the device GridOpt of
+    src/wannier/GridOpt.jl: Re embedded in its bounding box, the surviving k (grid order, third index
+    fastest) grouped into k1 planes and (k1, k2) lines. Stage 1 results are kept for the two most
+    recent planes, so a pass over the k batches in order computes each plane once."""
+
+    N1_MAX, N23_MAX, KLINE_MAX, NBW_MAX = 32, 8, 64, 4
+
+    @classmethod
+    def build(cls, st, ks, nd):
+        r = np.asarray(st.irvec_el_dev.cpu().numpy(), dtype=np.int64)
+        lo, hi = r.min(0), r.max(0)
+        n = hi - lo + 1
+        kg = np.asarray(st.kgrid, dtype=np.int64)
+        if (len(ks) == 0 or n[0] > cls.N1_MAX or n[1] > cls.N23_MAX or n[2] > cls.N23_MAX or kg[2] > cls.KLINE_MAX
+                or st.nbw > cls.NBW_MAX):
+            return None
+        return cls(st, ks, nd, r, lo, n, kg)
+
+    def __init__(self, st, ks, nd, r, lo, n, kg):
+        torch = _torch()
+        dev = st.el.xk.device
+        self.nd = nd
+        self._nb = st.nw  # op1 rows j-fastest, the order of the projection's lane reduction
+        self.n1, self.n2, self.n3 = (int(x) for x in n)
+        self.np = max(self.n2, self.n3, 3)
+        self.rmin = [int(x) for x in lo]
+        rmap = -np.ones((self.n3, self.n2, self.n1), np.int32)
+        rmap[r[:, 2] - lo[2], r[:, 1] - lo[1], r[:, 0] - lo[0]] = np.arange(len(r), dtype=np.int32)
+        self.rmap = torch.from_numpy(rmap.ravel()).to(dev)
+        i1, i2 = ks // (kg[1] * kg[2]), (ks // kg[2]) % kg[1]
+        change = np.flatnonzero((np.diff(i1) != 0) | (np.diff(i2) != 0)) + 1
+        starts = np.concatenate([[0], change, [len(ks)]]).astype(np.int64)
+        self.line_start_host = starts
+        self.line_start = torch.from_numpy(starts.astype(np.int32)).to(dev)
+        self.line_plane = i1[starts[:-1]]
+        self.line_of_k = np.repeat(np.arange(len(starts) - 1), np.diff(starts))
+        self.max_line_k = int(np.diff(starts).max())
+        self.x1 = st.el.xk.index_select(0, st.ksurv).cpu().numpy()[:, 0]
+        self.cache = []  # [plane, buffer] pairs, most recent last
+
+    def _plane_buffer(self, plane, k_first, epmat_el, stream):
+        torch = _torch()
+        for entry in self.cache:
+            if entry[0] == plane:
+                return entry[1]
+        if len(self.cache) < 2:
+            buf = torch.empty((self.np * self.np, self.nd), dtype=torch.complex64, device=epmat_el.device)
+            self.cache.append([plane, buf])
+        else:
+            entry = self.cache.pop(0)
+            entry[0] = plane
+            buf = entry[1]
+            self.cache.append(entry)
+        L.call("ephip_gridopt_stage1", self.nd, self.n1, self.n2, self.n3, self.np, self.rmin[0], L.ptr(self.rmap),
+               L.ptr(epmat_el), epmat_el.stride(0), float(self.x1[k_first]), self._nb, L.ptr(buf), stream)
+        return buf
+
+    def run(self, loop, i0, n, u0, u1, stream):
+        st = loop.st
+        l_first, l_last = int(self.line_of_k[i0]), int(self.line_of_k[i0 + n - 1]) + 1
+        l = l_first
+        while l < l_last:
+            plane = self.line_plane[l]
+            l_end = l
+            while l_end < l_last and self.line_plane[l_end] == plane:
+                l_end += 1
+            k_first = max(int(self.line_start_host[l]), i0)
+            buf = self._plane_buffer(plane, k_first, loop.epmat_el, stream)
+            L.call("ephip_gridopt_kR_project", st.nw, st.nm, loop.nrp, self.np, self.rmin[1], self.rmin[2],
+                   L.ptr(buf), L.ptr(loop.xk_surv), i0, i0 + n, l, l_end, L.ptr(self.line_start), self.max_line_k,
+                   L.ptr(loop.k_unit_start_dev), L.ptr(loop.unit_state[u0:u1]), L.ptr(loop.unit_band[u0:u1]),
+                   L.ptr(st.el.U), loop.max_units_per_k, L.ptr(loop.A), loop.A.stride(0), stream)
+            l = l_end
 
 
 def kept_q_bound(ksurv, kgrid, qgrid) -> int:

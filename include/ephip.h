@@ -216,6 +216,31 @@ int ephip_eph_RR_to_kR_project_packed_bf16x3(int32_t nw, int32_t nm, int32_t nrp
                                              const int64_t *ustate, const int32_t *uband, const void *U, int32_t nbw,
                                              void *A, int64_t lda, void *stream);
 
+/* Staged (GridOpt) RR->kR over Re with the k-side window projection fused: the device form of
+ * src/wannier/GridOpt.jl (gridopt_set23! :73-107, gridopt_set3! :110-131, gridopt_get3! :133-139) and
+ * of the batched k3 step of src/wannier/batched_gridopt_interpolator.jl, feeding the outer-k projection
+ * of src/calculator/run_eph_over_k_and_kq.jl:657-659. FP32 throughout, phases from FP64 arguments.
+ * The Re list is embedded in its bounding box (r1min + a, r2min + b, r3min + c), a < n1 <= 32,
+ * b < n2 <= 8, c < n3 <= 8; rmap (int32, n1*n2*n3) gives the operand column of each box point, or -1.
+ *  - ephip_gridopt_stage1: op1[(b + np*c)*nd + d'] = sum_a e^{2 pi i x1 (r1min + a)} op[d + ldop*rmap(a,b,c)]
+ *    for the nd rows of op (c64, the "el" layout rows (i, j, nu, Rp)), written j-fastest
+ *    (d' = j + nb*(i + nb*rest) for d = i + nb*(j + nb*rest)) when nb > 1; the face is padded to
+ *    np x np (np >= n2, n3; zeros outside the box).
+ *  - ephip_gridopt_kR_project: for the surviving k k0 <= k < k1 (xk: their crystal coordinates, FP64,
+ *    3 per k; op1 padded to np x np, 3 <= np <= 8) on the lines l0 <= l < l1 (line_start: first surviving k of each (k1, k2) line, all in
+ *    the k1 plane whose op1 is given; at most max_line_k <= 64 k per line): op2 = sum_b e^{2 pi i k2 r2}
+ *    op1, g(k) = sum_c e^{2 pi i k3 r3} op2, and for each unit u of k (kus, ustate, uband as in
+ *    ephip_eph_RR_to_kR_project_packed_bf16x3) A[u*lda + (i + nw*nu) + nw*nm*Rp] =
+ *    sum_j g(k)[i, j, nu, Rp] U[ustate[u]][j, uband[u]]. nw = 8 or 16, nbw <= 4, nb = nw in stage 1. */
+int ephip_gridopt_stage1(int64_t nd, int32_t n1, int32_t n2, int32_t n3, int32_t np, int32_t r1min,
+                         const int32_t *rmap, const void *op, int64_t ldop, double x1, int32_t nb, void *op1,
+                         void *stream);
+int ephip_gridopt_kR_project(int32_t nw, int32_t nm, int32_t nrp, int32_t np, int32_t r2min, int32_t r3min,
+                             const void *op1, const double *xk, int64_t k0, int64_t k1, int32_t l0, int32_t l1,
+                             const int32_t *line_start, int32_t max_line_k, const int32_t *kus,
+                             const int64_t *ustate, const int32_t *uband, const void *U, int32_t nbw, void *A,
+                             int64_t lda, void *stream);
+
 /* kR->kq Fourier over Rp fused with the k+q-window and phonon rotations and g2 (the
  * get_eph_kR_to_kq_batched! + eph_apply_rotations! + g2 sequence of src/wannier_to_bloch_batched.jl:
  * 245-273 and :442-455 in one kernel) for nw = 16, nm = 12, nbw <= 4, split-BF16: apk holds the

@@ -233,3 +233,30 @@ def test_outer_k_fused_projection(backend, model16, nq, variant):
     tune.set_variant(tune.KIND_FUSED_PROJ, 0)
     rel = _outer_k_vs_oracle(m, g, st, rf, qstep=5 if nq == 4 else 37)
     assert rel < 1e-4, rel
+
+
+@pytest.mark.parametrize("which,nb", [("model16", 40), ("model16", 150), ("cfg1", 64), ("cfg1", 7)])
+def test_outer_k_gridopt(backend, model16, cfg1, which, nb):
+    """RR->kR as the staged (GridOpt) transform with the projection fused (FP32): the unit operators A
+    agree with the direct FP32 Fourier + projection kernels to 1e-5 and g2 with the oracle to 1e-4.
+    Batches of 7 to 150 k cut lines and k1 planes; the batches are run out of order and one twice, so
+    cached plane sums are reused and evicted."""
+    from ephip import _lib as L
+    from ephip import eph
+    m, g = model16 if which == "model16" else cfg1
+    w = g["window"]
+    st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
+    nks = st.ksurv.shape[0]
+    staged = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32)
+    direct = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32, gridopt=False)
+    assert staged.gridopt is not None and direct.gridopt is None
+    rs = eph.UnitRecorder()
+    starts = [0, max(0, nks - nb), nb % nks, 0, min(2 * nb, max(0, nks - 1))]
+    for i0 in starts:
+        staged.run_kbatch(i0, nb, [rs])
+        direct.run_kbatch(i0, nb)
+        nu = int(staged.k_unit_start[min(i0 + nb, nks)] - staged.k_unit_start[i0])
+        a, b = staged.A[:nu].cpu().numpy(), direct.A[:nu].cpu().numpy()
+        assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b), (i0, np.linalg.norm(a - b) / np.linalg.norm(b))
+    rel = _outer_k_vs_oracle(m, g, st, rs, qstep=5)
+    assert rel < 1e-4, rel
