@@ -241,3 +241,230 @@ extern "C" int ephip_gridopt_kR_project(int32_t nw, int32_t nm, int32_t nrp, int
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }
+
+// ---------------------------------------------------------------------------------------------
+// Staged kR->kq over Rp for the q of a uniform grid (m1, m2, m3), fused with the k+q-window and phonon
+// rotations and g2: the GridOpt factorisation applied to the unit operators A_u[G, Rp]
+// (G = i + nw nu) of the outer-k loop, replacing the per-(unit, q) Fourier sum of
+// src/wannier_to_bloch_batched.jl:245-273 and the rotations :442-455.
+// ---------------------------------------------------------------------------------------------
+namespace ephip {
+
+// Per unit, the first kept-list position of every q line (q1, q2): line_off[u (nlines + 1) + l] =
+// lower_bound(col_index[u, 0:count[u]], l m3), and count[u] at l = nlines. One workgroup per unit.
+__global__ __launch_bounds__(256) void kq_line_offsets_kernel(int nunits, const int *__restrict__ count,
+                                                              const int *__restrict__ col_index, int64_t cs,
+                                                              int m3, int nlines, int *__restrict__ line_off) {
+    const int u = blockIdx.x;
+    if (u >= nunits) return;
+    const int n = count[u];
+    const int *col = col_index + (int64_t)u * cs;
+    for (int l = threadIdx.x; l <= nlines; l += blockDim.x) {
+        const int v = l * m3;
+        int lo = 0, hi = n;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (col[mid] < v)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        line_off[(int64_t)u * (nlines + 1) + l] = lo;
+    }
+}
+
+constexpr int GQ_M3_MAX = 64, GQ_SLAB_MAX = 1024;
+
+// One workgroup per (q1, unit), one thread per row G < nw nm (threads past it only help stage the
+// tables). Rp box: (p1min + a, p2min + b, p3min + c), a < n1, b, c < N; rmap[a + n1 (b + N c)] = Rp index or
+// -1. Stage 1 (registers): B1[b][c] = sum_a e^{2 pi i (q1/m1)(p1min + a)} A[u][G + ldr Rp]. For each line
+// q2 with kept q: B2[c] = sum_b e2 B1[b][c]; per kept q3: g = sum_c e3 B2[c]; t[m, nu] = sum_i
+// conj(U_kq[i, mlo + m]) g[i + 16 nu] (DPP row reduction over i); ep[m, nu'] = sum_nu t[m, nu]
+// uph(q)[nu, nu'], g2 = |ep|^2 / (2 omega(q)[nu']) at row u cs + (kept position), slots m < nbw (zero
+// past the k+q window), the layout of ephip_eph_kq_rotate_packed_bf16x3.
+template <int N, int NBW>
+__global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
+    int nm, int n1, int p1min, int p2min, int p3min, const int *__restrict__ rmap, int m1, int m2, int m3,
+    const cplx<float> *__restrict__ A, int64_t lda, int64_t ldr, const int *__restrict__ line_off,
+    const int *__restrict__ col_index, const int *__restrict__ ikq, int64_t cs, const cplx<float> *__restrict__ U,
+    const int64_t *__restrict__ bmin, const int64_t *__restrict__ bmax, const cplx<float> *__restrict__ uph,
+    const double *__restrict__ omega, int nbw, float *__restrict__ g2) {
+    constexpr int NW = 16;
+    __shared__ float2 ph1[32], ph2[N], ph3[GQ_M3_MAX * N];
+    // metadata of every kept q of the (u, q1) slab: k+q state, q, and (window bands | first band << 8)
+    __shared__ int sk_kq[GQ_SLAB_MAX], sk_iq[GQ_SLAB_MAX], sk_nb[GQ_SLAB_MAX];
+    __shared__ float2 ucol[GQ_M3_MAX * NBW * NW];
+    __shared__ float2 tq[GQ_M3_MAX * NBW * NW];  // t[kk][m][nu], nu < nm <= 16
+    const int q1 = blockIdx.x, u = blockIdx.y;
+    const int t = threadIdx.x;
+    const int nd2 = NW * nm;
+    const bool rv = t < nd2;
+    const int i = t % NW, nu = t / NW;
+    const int nlines = m1 * m2;
+    const int *lo_u = line_off + (int64_t)u * (nlines + 1);
+    const int64_t row0 = (int64_t)u * cs;
+    const int slab0 = lo_u[q1 * m2], nslab = lo_u[(q1 + 1) * m2] - slab0;  // <= m2 m3 <= GQ_SLAB_MAX
+    if (t < n1) {
+        float c, s;
+        cis2pi((double)q1 / m1 * (double)(p1min + t), c, s);
+        ph1[t] = make_float2(c, s);
+    }
+    for (int kk = t; kk < nslab; kk += blockDim.x) {
+        const int64_t row = row0 + slab0 + kk;
+        const int kq = ikq[row];
+        sk_kq[kk] = kq;
+        sk_iq[kk] = col_index[row];
+        const int64_t b0 = bmin[kq], b1 = bmax[kq];
+        const int nb = b0 > 0 ? (int)(b1 - b0 + 1) : 0;
+        sk_nb[kk] = min(nb, nbw) | ((b0 > 0 ? (int)b0 - 1 : 0) << 8);
+    }
+    __syncthreads();
+    // stage 1: the np x np face of every R1 plane is loaded at once (independent loads), then weighted
+    float br[N][N], bi[N][N];
+#pragma unroll
+    for (int c = 0; c < N; c++)
+#pragma unroll
+        for (int b = 0; b < N; b++) br[c][b] = bi[c][b] = 0.f;
+    const cplx<float> *a_u = A + (int64_t)u * lda + (rv ? t : 0);
+#pragma unroll
+    for (int c = 0; c < N; c++)
+        for (int a = 0; a < n1; a++) {
+            const float2 p = ph1[a];
+            cplx<float> v[N];
+#pragma unroll
+            for (int b = 0; b < N; b++) {  // one row of the face per step: N loads in flight
+                const int ir = rmap[a + n1 * (b + N * c)];  // uniform
+                v[b] = ir >= 0 ? a_u[ldr * ir] : cplx<float>{0.f, 0.f};
+            }
+#pragma unroll
+            for (int b = 0; b < N; b++) {
+                br[c][b] += p.x * v[b].re - p.y * v[b].im;
+                bi[c][b] += p.x * v[b].im + p.y * v[b].re;
+            }
+        }
+    for (int q2 = 0; q2 < m2; q2++) {
+        const int line = q1 * m2 + q2;
+        const int lo = lo_u[line], nk = lo_u[line + 1] - lo;  // uniform; nk <= m3
+        if (nk <= 0) continue;
+        const int ks = lo - slab0;  // the line's first entry in the slab tables
+        __syncthreads();  // the previous line's tables are no longer read
+        if (t < N) {
+            float c, s;
+            cis2pi((double)q2 / m2 * (double)(p2min + t), c, s);
+            ph2[t] = make_float2(c, s);
+        }
+        for (int e = t; e < nk * N; e += blockDim.x) {
+            const int kk = e / N, c3 = e % N;
+            const int q3 = sk_iq[ks + kk] % m3;
+            float c, s;
+            cis2pi((double)q3 / m3 * (double)(p3min + c3), c, s);
+            ph3[kk * N + c3] = make_float2(c, s);
+        }
+        for (int e = t; e < nk * NBW * NW; e += blockDim.x) {
+            const int kk = e / (NBW * NW), m = (e / NW) % NBW, ii = e % NW;
+            const int nbm = sk_nb[ks + kk], nbk = nbm & 0xff, mlo = nbm >> 8;
+            float2 w = make_float2(0.f, 0.f);
+            if (m < nbk) {
+                const cplx<float> v = U[(int64_t)NW * NW * sk_kq[ks + kk] + ii + NW * (mlo + m)];
+                w = make_float2(v.re, v.im);
+            }
+            ucol[e] = w;
+        }
+        __syncthreads();  // phase tables and U columns staged
+        float b2r[N], b2i[N];
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+            float sr = 0.f, si = 0.f;
+#pragma unroll
+            for (int b = 0; b < N; b++) {
+                const float2 p = ph2[b];
+                sr += p.x * br[c][b] - p.y * bi[c][b];
+                si += p.x * bi[c][b] + p.y * br[c][b];
+            }
+            b2r[c] = sr;
+            b2i[c] = si;
+        }
+        for (int kk = 0; kk < nk; kk++) {
+            float gr = 0.f, gi = 0.f;
+#pragma unroll
+            for (int c = 0; c < N; c++) {
+                const float2 p = ph3[kk * N + c];
+                gr += p.x * b2r[c] - p.y * b2i[c];
+                gi += p.x * b2i[c] + p.y * b2r[c];
+            }
+#pragma unroll
+            for (int m = 0; m < NBW; m++) {
+                const float2 w = ucol[(kk * NBW + m) * NW + i];  // conj(w) * g
+                const float tr = dpp_group_sum<NW>(w.x * gr + w.y * gi);
+                const float ti = dpp_group_sum<NW>(w.x * gi - w.y * gr);
+                if (rv && i == 0) tq[(kk * NBW + m) * NW + nu] = make_float2(tr, ti);
+            }
+        }
+        __syncthreads();  // t of every kept q of the line
+        for (int e = t; e < nk * nbw * nm; e += blockDim.x) {
+            const int kk = e / (nbw * nm), rr = e % (nbw * nm), m = rr / nm, nu2 = rr % nm;
+            const int iq = sk_iq[ks + kk];
+            const cplx<float> *up = uph + (int64_t)nm * nm * iq + nm * nu2;
+            float er = 0.f, ei = 0.f;
+#pragma unroll
+            for (int n = 0; n < NW; n++) {  // nm <= NW: fixed trip count, loads issued together
+                const int nn = min(n, nm - 1);
+                const float on = n < nm ? 1.f : 0.f;
+                const float2 tv = tq[(kk * NBW + m) * NW + nn];
+                const cplx<float> p = up[nn];
+                er += on * (tv.x * p.re - tv.y * p.im);
+                ei += on * (tv.x * p.im + tv.y * p.re);
+            }
+            const float w2 = 2.f * static_cast<float>(omega[(int64_t)nm * iq + nu2]);
+            g2[(row0 + lo + kk) * nbw * nm + rr] = (er * er + ei * ei) / w2;
+        }
+    }
+}
+
+}  // namespace ephip
+
+extern "C" int ephip_kq_line_offsets(int32_t nunits, const int32_t *count, const int32_t *col_index, int64_t cs,
+                                     int32_t m3, int32_t nlines, int32_t *line_off, void *stream) {
+    EPHIP_REQUIRE(nunits >= 0 && cs >= 0 && m3 >= 1 && nlines >= 1, "kq_line_offsets: bad sizes");
+    if (nunits == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(count && col_index && line_off, "kq_line_offsets: null device pointer");
+    kq_line_offsets_kernel<<<nunits, 256, 0, as_stream(stream)>>>(nunits, count, col_index, cs, m3, nlines, line_off);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int32_t nunits, int32_t n1, int32_t np,
+                                       int32_t p1min, int32_t p2min, int32_t p3min, const int32_t *rmap,
+                                       const int32_t *qgrid, const void *A, int64_t lda, int64_t ldr,
+                                       const int32_t *line_off, const int32_t *col_index, const int32_t *ikq,
+                                       int64_t cs, const void *U, const int64_t *bmin, const int64_t *bmax,
+                                       const void *uph, const double *omega, float *g2, void *stream) {
+    EPHIP_REQUIRE(nw == 16 && nm >= 1 && nm <= 16, "gridopt_kq_rotate: nw must be 16 and nm <= 16 (got %d, %d)", nw,
+                  nm);
+    EPHIP_REQUIRE(nbw >= 1 && nbw <= 4, "gridopt_kq_rotate: nbw=%d outside 1..4", nbw);
+    EPHIP_REQUIRE(n1 >= 1 && n1 <= 32 && np >= 3 && np <= 8, "gridopt_kq_rotate: Rp box n1=%d (<= 32), np=%d (3..8)",
+                  n1, np);
+    EPHIP_REQUIRE(qgrid && qgrid[0] >= 1 && qgrid[1] >= 1 && qgrid[2] >= 1 && qgrid[2] <= GQ_M3_MAX &&
+                      qgrid[0] <= 65535 && qgrid[1] * qgrid[2] <= GQ_SLAB_MAX,
+                  "gridopt_kq_rotate: q grid must be n1 n2 n3 with n3 <= %d and n2 n3 <= %d", GQ_M3_MAX, GQ_SLAB_MAX);
+    EPHIP_REQUIRE(nunits >= 0 && nunits <= 65535 && ldr >= (int64_t)nw * nm && lda >= ldr && cs >= 0,
+                  "gridopt_kq_rotate: bad sizes");
+    if (nunits == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(rmap && A && line_off && col_index && ikq && U && bmin && bmax && uph && omega && g2,
+                  "gridopt_kq_rotate: null device pointer");
+    const int nd2 = nw * nm;
+    const unsigned threads = (unsigned)(((nd2 + 63) / 64) * 64);
+    dim3 g((unsigned)qgrid[0], (unsigned)nunits);
+    using K = decltype(&gridopt_kq_rotate_kernel<3, 2>);
+#define GQ_ROW(nb) {gridopt_kq_rotate_kernel<3, nb>, gridopt_kq_rotate_kernel<4, nb>, gridopt_kq_rotate_kernel<5, nb>, \
+                    gridopt_kq_rotate_kernel<6, nb>, gridopt_kq_rotate_kernel<7, nb>, gridopt_kq_rotate_kernel<8, nb>}
+    static const K table[3][6] = {GQ_ROW(2), GQ_ROW(3), GQ_ROW(4)};
+#undef GQ_ROW
+    const K kern = table[nbw <= 2 ? 0 : nbw - 2][np - 3];
+    kern<<<g, threads, 0, as_stream(stream)>>>(nm, n1, p1min, p2min, p3min, rmap, qgrid[0], qgrid[1], qgrid[2],
+                                                static_cast<const cplx<float> *>(A), lda, ldr, line_off, col_index,
+                                                ikq, cs, static_cast<const cplx<float> *>(U), bmin, bmax,
+                                                static_cast<const cplx<float> *>(uph), omega, nbw, g2);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}

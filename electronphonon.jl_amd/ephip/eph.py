@@ -407,6 +407,10 @@ class OuterKLoop:
         self.gridopt = _GridOptKR.build(st, ks_host, self.nd_el) if (gridopt and nw in (8, 16)) else None
         if self.gridopt is not None:
             self.fused_proj = False
+        # kR->kq as the staged transform over the uniform q grid with the rotations and g2 fused (FP32)
+        self.gridopt_q = _GridOptKQ.build(st) if (gridopt and nw == 16 and nm <= 16 and st.nbw <= 4) else None
+        if self.gridopt_q is not None:
+            self.fused = False
         import ctypes
         self.qgrid_c = (ctypes.c_int32 * 3)(*map(int, st.qgrid))  # for the on-the-fly q phases
         if self.gridopt is not None:
@@ -448,9 +452,12 @@ class OuterKLoop:
         self.col_index = torch.empty((max_units, self.cs), dtype=torch.int32, device=dev)
         self.ikq = torch.empty((max_units, self.cs), dtype=torch.int32, device=dev)
         self.count = torch.empty(max_units, dtype=torch.int32, device=dev)
-        self.Gq = None if self.fused else torch.empty((max_units, self.cs, nd2), dtype=torch.complex64, device=dev)
+        self.Gq = None if (self.fused or self.gridopt_q is not None) else torch.empty(
+            (max_units, self.cs, nd2), dtype=torch.complex64, device=dev)
+        if self.gridopt_q is not None:
+            self.line_off = torch.empty((max_units, self.gridopt_q.nlines + 1), dtype=torch.int32, device=dev)
         self.g2 = torch.empty((max_units, self.cs, st.nbw * nm), dtype=torch.float32, device=dev)
-        if self.packed:
+        if self.packed and self.gridopt_q is None:
             self.apk_units = torch.empty(max_units * core.ft_stages(self.nrp) * nd2 * 64, dtype=torch.bfloat16,
                                          device=dev)
         self.uph = st.ph.u.contiguous()
@@ -509,7 +516,9 @@ class OuterKLoop:
                L.ptr(self.count), stream)
         # 3. kR->kq batched Fourier over Rp at the kept q
         nd2 = nw * nm
-        if self.fused:
+        if self.gridopt_q is not None:
+            self._timed("kR->kq", None, lambda: self.gridopt_q.run(self, nu, stream))
+        elif self.fused:
             core.ft_pack_operand(self.A, nd2, self.nrp, nd2, nu, nd2 * self.nrp, out=self.apk_units)
             self._timed("kR->kq", None, lambda: L.call(
                 "ephip_eph_kq_rotate_packed_bf16x3", nw, nm, st.nbw, self.nrp, L.ptr(self.apk_units), nu,
@@ -528,8 +537,8 @@ class OuterKLoop:
                 "ephip_fourier_batched_c32", nd2, self.nrp, L.ptr(self.A), nd2, nd2 * self.nrp,
                 L.ptr(self.irvec_ep_dev), L.ptr(st.ph.xq), nu, self.nq, L.ptr(self.count), L.ptr(self.col_index),
                 self.cs, L.ptr(self.Gq), nd2, self.cs * nd2, self.precision, stream))
-        # 4. rotations + g2 (already done by the fused kernel)
-        if not self.fused:
+        # 4. rotations + g2 (already done by the fused kernels)
+        if not self.fused and self.gridopt_q is None:
             L.call("ephip_eph_rotate_kq_g2", nw, nm, st.nbw, nu, self.cs, L.ptr(self.count),
                    L.ptr(self.col_index), L.ptr(self.ikq), L.ptr(self.Gq), nd2, self.cs * nd2, L.ptr(st.el.U),
                    L.ptr(st.el.filt.band_min), L.ptr(st.el.filt.band_max), L.ptr(self.uph), L.ptr(self.omega),
@@ -558,7 +567,7 @@ class OuterKLoop:
             else:
                 res = {"packed": T.tune_fourier(T.KIND_PACKED_BF16, lambda: self.run_kbatch(0, n), reps=reps,
                                                 variants=T.PACKED_VARIANTS)}
-            if self.fused:
+            if self.fused and self.gridopt_q is None:
                 res["fused"] = T.tune_fourier(T.KIND_FUSED_ROT, lambda: self.run_kbatch(0, n), reps=reps,
                                               variants=T.FUSED_TUNE)
             return res
@@ -645,6 +654,47 @@ class _GridOptKR:
                    L.ptr(loop.k_unit_start_dev), L.ptr(loop.unit_state[u0:u1]), L.ptr(loop.unit_band[u0:u1]),
                    L.ptr(st.el.U), loop.max_units_per_k, L.ptr(loop.A), loop.A.stride(0), stream)
             l = l_end
+
+
+class _GridOptKQ:
+    """Staged kR->kq over Rp for the uniform q grid (ephip_kq_line_offsets + ephip_gridopt_kq_rotate):
+    the unit operators' Rp set embedded in its bounding box, the q of each unit's kept list grouped
+    into (q1, q2) lines; writes g2 in the layout of the fused Fourier/rotation kernel."""
+
+    @classmethod
+    def build(cls, st):
+        r = np.asarray(st.epmat.irvec_dev().cpu().numpy(), dtype=np.int64)
+        lo, hi = r.min(0), r.max(0)
+        n = hi - lo + 1
+        m = np.asarray(st.qgrid, dtype=np.int64)
+        if n[0] > 32 or n[1] > 8 or n[2] > 8 or m[2] > 64 or m[1] * m[2] > 1024 or m[0] > 65535:
+            return None
+        return cls(st, r, lo, n, m)
+
+    def __init__(self, st, r, lo, n, m):
+        torch = _torch()
+        dev = st.el.xk.device
+        self.n1 = int(n[0])
+        self.np = max(int(n[1]), int(n[2]), 3)
+        self.pmin = [int(x) for x in lo]
+        rmap = -np.ones((self.np, self.np, self.n1), np.int32)
+        rmap[r[:, 2] - lo[2], r[:, 1] - lo[1], r[:, 0] - lo[0]] = np.arange(len(r), dtype=np.int32)
+        self.rmap = torch.from_numpy(rmap.ravel()).to(dev)
+        self.m = [int(x) for x in m]
+        self.nlines = self.m[0] * self.m[1]
+        import ctypes
+        self.qgrid_c = (ctypes.c_int32 * 3)(*self.m)
+
+    def run(self, loop, nu, stream):
+        import ctypes as C
+        st = loop.st
+        L.call("ephip_kq_line_offsets", nu, L.ptr(loop.count), L.ptr(loop.col_index), loop.cs, self.m[2],
+               self.nlines, L.ptr(loop.line_off), stream)
+        L.call("ephip_gridopt_kq_rotate", st.nw, st.nm, st.nbw, nu, self.n1, self.np, self.pmin[0], self.pmin[1],
+               self.pmin[2], L.ptr(self.rmap), C.addressof(self.qgrid_c), L.ptr(loop.A), loop.A.stride(0),
+               st.nw * st.nm, L.ptr(loop.line_off), L.ptr(loop.col_index), L.ptr(loop.ikq), loop.cs,
+               L.ptr(st.el.U), L.ptr(st.el.filt.band_min), L.ptr(st.el.filt.band_max), L.ptr(loop.uph),
+               L.ptr(loop.omega), L.ptr(loop.g2), stream)
 
 
 def kept_q_bound(ksurv, kgrid, qgrid) -> int:

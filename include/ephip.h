@@ -241,6 +241,25 @@ int ephip_gridopt_kR_project(int32_t nw, int32_t nm, int32_t nrp, int32_t np, in
                              const int64_t *ustate, const int32_t *uband, const void *U, int32_t nbw, void *A,
                              int64_t lda, void *stream);
 
+/* Staged (GridOpt) kR->kq over Rp for the q of a uniform grid qgrid = (m1, m2, m3) (q index
+ * (q1*m2 + q2)*m3 + q3 at q = (q1/m1, q2/m2, q3/m3)), fused with the k+q-window and phonon rotations and
+ * g2: the same output as ephip_eph_kq_rotate_packed_bf16x3 (the get_eph_kR_to_kq_batched! +
+ * eph_apply_rotations! + g2 sequence of src/wannier_to_bloch_batched.jl:245-273 and :442-455), computed
+ * in FP32 by per-axis partial sums (src/wannier/GridOpt.jl) over the unit operators A (c64, unit u at
+ * A + u*lda, element (G, Rp) at G + ldr*Rp, G = i + nw*nu). The Rp list is embedded in its bounding box
+ * (p1min + a, p2min + b, p3min + c), a < n1 <= 32, b, c < np (3..8); rmap (int32, n1*np*np, index
+ * a + n1*(b + np*c)) gives the Rp index or -1. line_off (int32, nunits x (m1*m2 + 1)) holds each unit's
+ * first kept-list position per q line, from ephip_kq_line_offsets (lower bounds in the sorted kept
+ * lists of ephip_kq_compact). nw = 16, nm <= 16, nbw <= 4, m3 <= 64, m2*m3 <= 1024. */
+int ephip_kq_line_offsets(int32_t nunits, const int32_t *count, const int32_t *col_index, int64_t cs, int32_t m3,
+                          int32_t nlines, int32_t *line_off, void *stream);
+int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int32_t nunits, int32_t n1, int32_t np,
+                            int32_t p1min, int32_t p2min, int32_t p3min, const int32_t *rmap, const int32_t *qgrid,
+                            const void *A, int64_t lda, int64_t ldr, const int32_t *line_off,
+                            const int32_t *col_index, const int32_t *ikq, int64_t cs, const void *U,
+                            const int64_t *bmin, const int64_t *bmax, const void *uph, const double *omega, float *g2,
+                            void *stream);
+
 /* kR->kq Fourier over Rp fused with the k+q-window and phonon rotations and g2 (the
  * get_eph_kR_to_kq_batched! + eph_apply_rotations! + g2 sequence of src/wannier_to_bloch_batched.jl:
  * 245-273 and :442-455 in one kernel) for nw = 16, nm = 12, nbw <= 4, split-BF16: apk holds the

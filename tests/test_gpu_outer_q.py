@@ -166,8 +166,8 @@ def test_outer_k_fused_rotation(backend, model16, variant, nq, window):
     w = g["window"]
     st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
     assert st.nbw == {0.5: 2, 0.8: 3, 1.5: 4}[window]
-    fused = eph.OuterKLoop(st, nk_batch=40, precision=L.PREC_BF16X3)
-    plain = eph.OuterKLoop(st, nk_batch=40, precision=L.PREC_BF16X3, fuse_rotation=False)
+    fused = eph.OuterKLoop(st, nk_batch=40, precision=L.PREC_BF16X3, gridopt=False)
+    plain = eph.OuterKLoop(st, nk_batch=40, precision=L.PREC_BF16X3, fuse_rotation=False, gridopt=False)
     assert fused.fused and not plain.fused and fused.Gq is None
     rf, rp = eph.UnitRecorder(), eph.UnitRecorder()
     nks = st.ksurv.shape[0]
@@ -187,7 +187,7 @@ def test_outer_k_fused_rotation(backend, model16, variant, nq, window):
         # eight-wave variants: a grid of one column tile (workgroups loop over the unit's tiles) gives the
         # same g2 bit for bit as the grid sized by the kept-q bound
         assert fused.max_kept >= max(len(c) for b in rf.batches for c in b["cols"])
-        narrow = eph.OuterKLoop(st, nk_batch=40, precision=L.PREC_BF16X3)
+        narrow = eph.OuterKLoop(st, nk_batch=40, precision=L.PREC_BF16X3, gridopt=False)
         narrow.max_kept = 1
         rn = eph.UnitRecorder()
         for i0 in (0, max(0, nks - 40)):
@@ -214,8 +214,8 @@ def test_outer_k_fused_projection(backend, model16, nq, variant):
     nks = st.ksurv.shape[0]
     tune.set_variant(tune.KIND_FUSED_PROJ, variant)
     for nb in (40, 150):
-        fused = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_BF16X3)
-        plain = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_BF16X3, fuse_projection=False)
+        fused = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_BF16X3, gridopt=False)
+        plain = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_BF16X3, fuse_projection=False, gridopt=False)
         assert fused.fused_proj and not plain.fused_proj and fused.G is None
         rf, rp = eph.UnitRecorder(), eph.UnitRecorder()
         for i0 in (0, max(0, nks - nb)):
@@ -259,4 +259,35 @@ def test_outer_k_gridopt(backend, model16, cfg1, which, nb):
         a, b = staged.A[:nu].cpu().numpy(), direct.A[:nu].cpu().numpy()
         assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b), (i0, np.linalg.norm(a - b) / np.linalg.norm(b))
     rel = _outer_k_vs_oracle(m, g, st, rs, qstep=5)
+    assert rel < 1e-4, rel
+
+
+@pytest.mark.parametrize("nq,window,nb", [(4, 0.5, 40), (8, 0.5, 150), (4, 0.8, 40), (8, 1.5, 64)])
+def test_outer_k_gridopt_kq(backend, model16, nq, window, nb):
+    """kR->kq as the staged (GridOpt) transform over the uniform q grid with the k+q/phonon rotations and
+    g2 fused (FP32): kept-q lists and g2 agree with the direct FP32 Fourier + rotation kernels (g2 to
+    1e-5) and with the oracle (1e-4), for nbw = 2, 3 and 4 and q grids of 4^3 and 8^3."""
+    from ephip import _lib as L
+    from ephip import eph
+    m, g = model16
+    g = dict(g, qgrid=(nq, nq, nq), window=window)
+    w = g["window"]
+    st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
+    nks = st.ksurv.shape[0]
+    staged = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32)
+    direct = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32, gridopt=False)
+    assert staged.gridopt_q is not None and staged.gridopt is not None and direct.gridopt_q is None
+    rs, rd = eph.UnitRecorder(), eph.UnitRecorder()
+    for i0 in (0, max(0, nks - nb)):
+        staged.run_kbatch(i0, nb, [rs])
+        direct.run_kbatch(i0, nb, [rd])
+    num = den = 0.0
+    for bs, bd in zip(rs.batches, rd.batches):
+        assert np.array_equal(bs["unit_state"], bd["unit_state"])
+        for u in range(len(bs["cols"])):
+            assert np.array_equal(bs["cols"][u], bd["cols"][u])
+            num = max(num, float(np.abs(bs["g2"][u] - bd["g2"][u]).max(initial=0.0)))
+            den = max(den, float(np.abs(bd["g2"][u]).max(initial=0.0)))
+    assert den > 0 and num / den < 1e-5, num / den
+    rel = _outer_k_vs_oracle(m, g, st, rs, qstep=5 if nq == 4 else 37)
     assert rel < 1e-4, rel
