@@ -45,6 +45,10 @@ TRAFFIC_BYTES_CFG2 = {
 }
 # kernel names by (stage, variant) for the split-BF16 fast path (tuner kinds 3 and 4)
 KERNEL_NAMES = {
+    ("RR->kR", "gridopt"): "gridopt_stage1_kernel + gridopt_kR_project_kernel: staged (GridOpt) RR->kR over Re with "
+                           "the k-side window projection, FP32 VALU",
+    ("kR->kq", "gridopt"): "gridopt_kq_rotate_kernel: staged (GridOpt) kR->kq over the q grid with the k+q/phonon "
+                           "rotations and g2, FP32 VALU",
     ("RR->kR", "proj0"): "ft_bf16p8_proj_kernel: eight-wave 192x128 packed split-BF16 RR->kR Fourier over Re "
                          "fused with the k-side window projection",
     ("RR->kR", "proj1"): "ft_bf16p8_proj_kernel<deep>: eight-wave 192x128 packed split-BF16 RR->kR Fourier over "
@@ -314,31 +318,36 @@ def main():
             loop.run_kbatch((k_base + s * nks_steps) % max(1, nks - nks_steps), nks_steps)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-        ks_done = [((k_base + s * nks_steps) % max(1, nks - nks_steps)) + i for s in range(args.steps)
-                   for i in range(nks_steps)]
-        cols = kept_pairs(ks_done)
-        nd2 = m.nw * m.nmodes
-        nrp = len(m.irvec_ep)
+        # algorithmic flops of the launches actually made (direct or staged kernels), step by step
+        cols, f_rrkr_total, f_krkq_total = 0, 0.0, 0.0
+        for s in range(args.steps):
+            i0 = (k_base + s * nks_steps) % max(1, nks - nks_steps)
+            kept = kept_pairs(range(i0, min(i0 + nks_steps, nks)))
+            fr, fk = loop.kernel_flops(i0, nks_steps, kept)
+            cols += kept
+            f_rrkr_total += fr
+            f_krkq_total += fk
         kin = {"RR->kR": [], "kR->kq": []}
         for a, b, lab, fl in ev:
             kin[lab].append(a.elapsed_time(b) * 1e-3)
-        f_rrkr = 8.0 * m.nw * m.nw * m.nmodes * nrp * len(m.irvec_el) * nks_steps
-        f_krkq_total = 8.0 * nd2 * nrp * cols
-        if fused:  # the fused kernel also applies U_kq^H (nw x nbw) and u_ph (nm x nm) per column
-            f_krkq_total += 8.0 * (m.nw * st.nbw * m.nmodes + st.nbw * m.nmodes * m.nmodes) * cols
         t_rrkr, t_krkq = sum(kin["RR->kR"]), sum(kin["kR->kq"])
+        staged_rr = getattr(loop, "gridopt", None) is not None
+        staged_kq = getattr(loop, "gridopt_q", None) is not None
         results["fast"] = dict(value=nks_steps * nq_grid * args.steps / el, ms_per_step=1e3 * el / args.steps,
                                steps=args.steps, computed_columns=cols,
-                               rrkr_tflops=f_rrkr * len(kin["RR->kR"]) / t_rrkr / 1e12,
+                               rrkr_tflops=f_rrkr_total / t_rrkr / 1e12,
                                rrkr_avg_ms=1e3 * t_rrkr / len(kin["RR->kR"]),
                                krkq_tflops=f_krkq_total / t_krkq / 1e12,
                                krkq_avg_ms=1e3 * t_krkq / len(kin["kR->kq"]),
-                               rrkr_flop_per_launch=f_rrkr, krkq_flop_per_launch=f_krkq_total / len(kin["kR->kq"]),
-                               fused_rotation=fused, fused_projection=bool(getattr(loop, "fused_proj", False)),
-                               variant_rrkr=(f"proj{T.get_variant(T.KIND_FUSED_PROJ)}"
+                               rrkr_flop_per_launch=f_rrkr_total / len(kin["RR->kR"]),
+                               krkq_flop_per_launch=f_krkq_total / len(kin["kR->kq"]),
+                               fused_rotation=fused or staged_kq,
+                               fused_projection=bool(getattr(loop, "fused_proj", False)) or staged_rr,
+                               variant_rrkr=("gridopt" if staged_rr else
+                                             f"proj{T.get_variant(T.KIND_FUSED_PROJ)}"
                                              if getattr(loop, "fused_proj", False) else
                                              T.get_variant(T.KIND_PACKED_BF16)),
-                               variant_krkq=T.get_variant(T.KIND_FUSED_ROT))
+                               variant_krkq="gridopt" if staged_kq else T.get_variant(T.KIND_FUSED_ROT))
         log(f"[bench] fast outer-k {args.precision}: {results['fast']['value']:.3e} pairs/s; RR->kR "
             f"{results['fast']['rrkr_tflops']:.1f} TFLOP/s ({results['fast']['rrkr_avg_ms']:.2f} ms), kR->kq "
             f"{results['fast']['krkq_tflops']:.1f} TFLOP/s ({results['fast']['krkq_avg_ms']:.2f} ms)")
@@ -357,19 +366,28 @@ def main():
         # dominant kernel: the longer of the two Fourier launches per step
         bf = args.precision == "bf16x3"
         var = {"RR->kR": r.get("variant_rrkr"), "kR->kq": r.get("variant_krkq")}
-        if r["rrkr_avg_ms"] >= r["krkq_avg_ms"]:
-            name = KERNEL_NAMES.get(("RR->kR", var["RR->kR"]) if bf else None) or (
-                (f"ft_bf16p_kernel (packed split-BF16, variant {var['RR->kR']})" if bf else
-                 "ft_mfma_kernel<float,128>") + ", RR->kR Fourier over Re")
-            dom = (name + " (ndata = nw^2 nm nr_ep, nr = nr_el, nk = k batch)",
-                   r["rrkr_tflops"], r["rrkr_avg_ms"], r["rrkr_flop_per_launch"], "RR->kR")
+        stage = "RR->kR" if r["rrkr_avg_ms"] >= r["krkq_avg_ms"] else "kR->kq"
+        if stage == "RR->kR":
+            if var[stage] == "gridopt" or bf:
+                name = KERNEL_NAMES.get((stage, var[stage])) or (
+                    f"ft_bf16p_kernel (packed split-BF16, variant {var[stage]}), RR->kR Fourier over Re")
+            else:
+                name = "ft_mfma_kernel<float,128>, RR->kR Fourier over Re"
+            dom = (name + " (k batch)", r["rrkr_tflops"], r["rrkr_avg_ms"], r["rrkr_flop_per_launch"], stage)
         else:
-            name = KERNEL_NAMES.get(("kR->kq", var["kR->kq"]) if r.get("fused_rotation") else None) or (
-                f"ft_bf16p_rot_kernel (variant {var['kR->kq']}): packed split-BF16 kR->kq Fourier over Rp fused "
-                "with the k+q/phonon rotations and g2" if r.get("fused_rotation") else
-                "batched Fourier kernel, kR->kq over Rp")
+            if var[stage] == "gridopt" or r.get("fused_rotation"):
+                name = KERNEL_NAMES.get((stage, var[stage])) or (
+                    f"ft_bf16p_rot_kernel (variant {var[stage]}): packed split-BF16 kR->kq Fourier over Rp fused "
+                    "with the k+q/phonon rotations and g2")
+            else:
+                name = "batched Fourier kernel, kR->kq over Rp"
             dom = (name + " (units x kept q)", r["krkq_tflops"], r["krkq_avg_ms"], r["krkq_flop_per_launch"],
-                   "kR->kq")
+                   stage)
+        # arithmetic of the two contractions as run: the staged (GridOpt) kernels are FP32 whatever the
+        # precision argument
+        kinds = {"fp32" if var[k] == "gridopt" else args.precision for k in var}
+        path_dtype = "+".join(sorted(kinds))
+        dom_fp32 = var[dom[4]] == "gridopt" or args.precision == "fp32"
         path = "outer-k, k-side window projection, k+q window skip"
         pairs_step = args.kbatch_k * nq_total
     else:
@@ -378,7 +396,8 @@ def main():
         dom = (r["kernel"], r["ft_tflops"], r["ft_avg_ms"], r["ft_flop_per_launch"], "Rq->kq")
         path = "outer-q plain (separate Fourier / rotate kernels)"
         pairs_step = nks * args.qtile
-    peak = PEAK[args.precision if "fast" in results else "fp32"]
+        dom_fp32 = True
+    peak = PEAK["fp32" if ("fast" not in results or dom_fp32) else args.precision]
     out = {
         "metric": METRIC,
         "value": value,
@@ -390,7 +409,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": args.precision if "fast" in results else "fp32",
+        "dtype": path_dtype if "fast" in results else "fp32",
         "data": "synthetic (seeded model generator, no checkpoint)",
         "config": {"workload": f"{args.config}: nw {m.nw}, nmodes {m.nmodes}, H/D 9^3, g 7^3 x 7^3 c64, "
                                f"k {grids['kgrid']}, q {grids['qgrid']}, window +-{w} eV",
@@ -399,6 +418,10 @@ def main():
                    "plain_outer_q_fp32_pairs_per_s": results.get("plain", {}).get("value")},
         "roofline": {"bound": "mfma", "achieved": dom[1], "peak": peak, "unit": "TFLOP/s",
                      "frac": dom[1] / peak,
+                     "compute": ("FP32 VALU (no MFMA; MI355X's FP32 peak is the same 157.3 TFLOP/s for vector and "
+                                 "matrix)" if "fast" in results and var[dom[4]] == "gridopt" else
+                                 "MFMA (split-BF16 flops counted once)" if "fast" in results and not dom_fp32
+                                 else "MFMA FP32"),
                      "traffic": (TRAFFIC_BYTES_CFG2.get((args.precision, dom[4], args.kbatch_k, var[dom[4]]))
                                  if ("fast" in results and args.config == "cfg2"
                                      and (dom[4] != "kR->kq" or r.get("fused_rotation"))) else None),

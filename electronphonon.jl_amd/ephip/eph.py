@@ -549,6 +549,34 @@ class OuterKLoop:
             c.run(payload)
         return payload
 
+    def kernel_flops(self, i0: int, n: int, kept: int) -> tuple[float, float]:
+        """Algorithmic flops (8 per complex multiply-add) of the batch's RR->kR and kR->kq launches for the
+        kernels in use; kept = the batch's kept (unit, q) pairs. Direct Fourier kernels: the full phase
+        sums. Staged (GridOpt) kernels: their own stages, the per-plane stage 1 of RR->kR counted once
+        per surviving k of its plane, the stage 2 of kR->kq for every q line of every unit."""
+        st = self.st
+        nw, nm = st.nw, st.nm
+        n = min(n, st.ksurv.shape[0] - i0)
+        nu = int(self.k_unit_start[i0 + n] - self.k_unit_start[i0])
+        nd2 = nw * nm
+        rot = 8.0 * (nw * st.nbw * nm + st.nbw * nm * nm) * kept
+        if self.gridopt is not None:
+            go = self.gridopt
+            planes = go.line_plane[go.line_of_k[i0:i0 + n]]
+            per_plane = np.bincount(go.line_plane[go.line_of_k], minlength=int(planes.max()) + 1)
+            f_rr = 8.0 * self.nd_el * (self.nre * float(np.sum(1.0 / per_plane[planes]))
+                                       + go.n2 * go.n3 * (int(go.line_of_k[i0 + n - 1]) - int(go.line_of_k[i0]) + 1)
+                                       + go.n3 * n + nu)
+        else:
+            f_rr = 8.0 * self.nd_el * self.nre * n
+        if self.gridopt_q is not None:
+            gq = self.gridopt_q
+            n2n3 = int((gq.rmap.view(gq.np, gq.np, gq.n1) >= 0).any(2).sum().item())
+            f_kq = 8.0 * nd2 * (self.nrp * gq.m[0] * nu + n2n3 * gq.nlines * nu + gq.np * kept) + rot
+        else:
+            f_kq = 8.0 * nd2 * self.nrp * kept + (rot if self.fused else 0.0)
+        return f_rr, f_kq
+
     def tune(self, reps: int = 3) -> dict:
         """Pick the Fourier kernel variant (FP32: LDS staging; split-BF16: packed tile shape) with the
         shortest time for one full k batch."""
