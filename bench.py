@@ -40,6 +40,8 @@ PEAK = {"fp32": 157.3, "bf16x3": 2500.0}  # TFLOP/s, MI355X dense MFMA (fp32, bf
 # passes calibrated on a 256 MiB copy (profiles/cfg2_fast_bf16_hbm_counters.md). Counters cannot be read
 # inside the timed run; a variant or batch without a measurement reports traffic null.
 TRAFFIC_BYTES_CFG2 = {
+    ("fp32", "kR->kq", 384, "gridopt"): (1081120 * 2 + 205708) * 1024,   # staged kR->kq + rotations + g2
+    ("fp32", "RR->kR", 384, "gridopt"): (203556 * 2 + 202199) * 1024,    # staged RR->kR projection launch
     ("bf16x3", "kR->kq", 384, 10): (2998082 * 2 + 204444) * 1024,   # fused kR->kq + rotations, 8 waves
     ("bf16x3", "RR->kR", 384, "proj0"): (1827474 * 2 + 200604) * 1024,  # RR->kR fused with the projection
 }
@@ -193,6 +195,8 @@ def main():
     ap.add_argument("--tune", dest="tune", action="store_true", default=True,
                     help="autotune the Fourier kernel variant on the live batch first (default)")
     ap.add_argument("--no-tune", dest="tune", action="store_false")
+    ap.add_argument("--no-gridopt", dest="gridopt", action="store_false", default=True,
+                    help="direct (MFMA) Fourier contractions instead of the staged GridOpt kernels")
     ap.add_argument("--variant", action="append", default=[], metavar="KIND=V",
                     help="pin a Fourier kernel variant (tuner kind=variant, repeatable); skips the tuner")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -292,7 +296,7 @@ def main():
         del loop
     # ---- fast outer-k path (k-side projection, k+q window skip)
     if args.path in ("fast", "both"):
-        loop = eph.OuterKLoop(st, nk_batch=args.kbatch_k, precision=prec)
+        loop = eph.OuterKLoop(st, nk_batch=args.kbatch_k, precision=prec, gridopt=args.gridopt)
         fused = bool(getattr(loop, "fused", False))
         if args.variant:
             for kv in args.variant:
@@ -422,7 +426,8 @@ def main():
                                  "matrix)" if "fast" in results and var[dom[4]] == "gridopt" else
                                  "MFMA (split-BF16 flops counted once)" if "fast" in results and not dom_fp32
                                  else "MFMA FP32"),
-                     "traffic": (TRAFFIC_BYTES_CFG2.get((args.precision, dom[4], args.kbatch_k, var[dom[4]]))
+                     "traffic": (TRAFFIC_BYTES_CFG2.get(("fp32" if var[dom[4]] == "gridopt" else args.precision,
+                                                         dom[4], args.kbatch_k, var[dom[4]]))
                                  if ("fast" in results and args.config == "cfg2"
                                      and (dom[4] != "kR->kq" or r.get("fused_rotation"))) else None),
                      "kernel": dom[0], "avg_launch_ms": dom[2], "flop_per_launch": dom[3]},
