@@ -277,11 +277,14 @@ constexpr int GQ_M3_MAX = 64, GQ_SLAB_MAX = 1024;
 
 // One workgroup per (q1, unit), one thread per row G < nw nm (threads past it only help stage the
 // tables). Rp box: (p1min + a, p2min + b, p3min + c), a < n1, b, c < N; rmap[a + n1 (b + N c)] = Rp index or
-// -1. Stage 1 (registers): B1[b][c] = sum_a e^{2 pi i (q1/m1)(p1min + a)} A[u][G + ldr Rp]. For each line
-// q2 with kept q: B2[c] = sum_b e2 B1[b][c]; per kept q3: g = sum_c e3 B2[c]; t[m, nu] = sum_i
-// conj(U_kq[i, mlo + m]) g[i + 16 nu] (DPP row reduction over i); ep[m, nu'] = sum_nu t[m, nu]
-// uph(q)[nu, nu'], g2 = |ep|^2 / (2 omega(q)[nu']) at row u cs + (kept position), slots m < nbw (zero
-// past the k+q window), the layout of ephip_eph_kq_rotate_packed_bf16x3.
+// -1. Stage 1 (registers): B1[b][c] = sum_a e^{2 pi i (q1/m1)(p1min + a)} A[u][G + ldr Rp]. The kept q of
+// the (u, q1) slab are taken in chunks of whole q lines with at most GQ_CHUNK entries; per chunk, per
+// kept q (in line order; B2[c] = sum_b e2 B1[b][c] is recomputed when the line changes): g = sum_c e3 B2[c];
+// t[m, nu] = sum_i conj(U_kq[i, mlo + m]) g[i + 16 nu] (DPP row reduction over i); ep[m, nu'] = sum_nu
+// t[m, nu] uph(q)[nu, nu'], g2 = |ep|^2 / (2 omega(q)[nu']) at row u cs + (kept position), slots m < nbw
+// (zero past the k+q window), the layout of ephip_eph_kq_rotate_packed_bf16x3.
+constexpr int GQ_CHUNK = 32, GQ_M2_MAX = 64;
+
 template <int N, int NBW>
 __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
     int nm, int n1, int p1min, int p2min, int p3min, const int *__restrict__ rmap, int m1, int m2, int m3,
@@ -290,11 +293,13 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
     const int64_t *__restrict__ bmin, const int64_t *__restrict__ bmax, const cplx<float> *__restrict__ uph,
     const double *__restrict__ omega, int nbw, float *__restrict__ g2) {
     constexpr int NW = 16;
-    __shared__ float2 ph1[32], ph2[N], ph3[GQ_M3_MAX * N];
-    // metadata of every kept q of the (u, q1) slab: k+q state, q, and (window bands | first band << 8)
+    __shared__ float2 ph1[32], ph2[GQ_M2_MAX * N], ph3[GQ_M3_MAX * N];  // q1, all q2, all q3 phases
+    __shared__ int slo[GQ_M2_MAX + 1];  // kept-list offsets of the slab's lines
+    // metadata of every kept q of the (u, q1) slab: k+q state, q, and the packed word
+    // (window bands | first band << 8 | q3 << 16 | q2 << 22), q2, q3 < 64
     __shared__ int sk_kq[GQ_SLAB_MAX], sk_iq[GQ_SLAB_MAX], sk_nb[GQ_SLAB_MAX];
-    __shared__ float2 ucol[GQ_M3_MAX * NBW * NW];
-    __shared__ float2 tq[GQ_M3_MAX * NBW * NW];  // t[kk][m][nu], nu < nm <= 16
+    __shared__ float2 ucol[GQ_CHUNK * NBW * NW];
+    __shared__ float2 tq[GQ_CHUNK * NBW * NW];  // t[kk][m][nu], nu < nm <= 16
     const int q1 = blockIdx.x, u = blockIdx.y;
     const int t = threadIdx.x;
     const int nd2 = NW * nm;
@@ -303,23 +308,35 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
     const int nlines = m1 * m2;
     const int *lo_u = line_off + (int64_t)u * (nlines + 1);
     const int64_t row0 = (int64_t)u * cs;
-    const int slab0 = lo_u[q1 * m2], nslab = lo_u[(q1 + 1) * m2] - slab0;  // <= m2 m3 <= GQ_SLAB_MAX
+    if (t <= m2) slo[t] = lo_u[q1 * m2 + t];
     if (t < n1) {
         float c, s;
         cis2pi((double)q1 / m1 * (double)(p1min + t), c, s);
         ph1[t] = make_float2(c, s);
     }
+    for (int e = t; e < m2 * N; e += blockDim.x) {
+        float c, s;
+        cis2pi((double)(e / N) / m2 * (double)(p2min + e % N), c, s);
+        ph2[e] = make_float2(c, s);
+    }
+    for (int e = t; e < m3 * N; e += blockDim.x) {
+        float c, s;
+        cis2pi((double)(e / N) / m3 * (double)(p3min + e % N), c, s);
+        ph3[e] = make_float2(c, s);
+    }
+    __syncthreads();
+    const int slab0 = slo[0], nslab = slo[m2] - slab0;  // <= m2 m3 <= GQ_SLAB_MAX
     for (int kk = t; kk < nslab; kk += blockDim.x) {
         const int64_t row = row0 + slab0 + kk;
         const int kq = ikq[row];
+        const int iq = col_index[row];
         sk_kq[kk] = kq;
-        sk_iq[kk] = col_index[row];
+        sk_iq[kk] = iq;
         const int64_t b0 = bmin[kq], b1 = bmax[kq];
         const int nb = b0 > 0 ? (int)(b1 - b0 + 1) : 0;
-        sk_nb[kk] = min(nb, nbw) | ((b0 > 0 ? (int)b0 - 1 : 0) << 8);
+        sk_nb[kk] = min(nb, nbw) | ((b0 > 0 ? (int)b0 - 1 : 0) << 8) | ((iq % m3) << 16) | (((iq / m3) % m2) << 22);
     }
-    __syncthreads();
-    // stage 1: the np x np face of every R1 plane is loaded at once (independent loads), then weighted
+    // stage 1: one row of the face at a time (N independent loads in flight)
     float br[N][N], bi[N][N];
 #pragma unroll
     for (int c = 0; c < N; c++)
@@ -332,7 +349,7 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
             const float2 p = ph1[a];
             cplx<float> v[N];
 #pragma unroll
-            for (int b = 0; b < N; b++) {  // one row of the face per step: N loads in flight
+            for (int b = 0; b < N; b++) {
                 const int ir = rmap[a + n1 * (b + N * c)];  // uniform
                 v[b] = ir >= 0 ? a_u[ldr * ir] : cplx<float>{0.f, 0.f};
             }
@@ -342,27 +359,17 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
                 bi[c][b] += p.x * v[b].im + p.y * v[b].re;
             }
         }
-    for (int q2 = 0; q2 < m2; q2++) {
-        const int line = q1 * m2 + q2;
-        const int lo = lo_u[line], nk = lo_u[line + 1] - lo;  // uniform; nk <= m3
+    int l0 = 0;
+    while (l0 < m2) {  // uniform
+        int l1 = l0 + 1;
+        while (l1 < m2 && slo[l1 + 1] - slo[l0] <= GQ_CHUNK) l1++;
+        const int ks = slo[l0] - slab0, nk = slo[l1] - slo[l0];  // chunk entries in the slab tables, <= GQ_CHUNK
+        l0 = l1;
         if (nk <= 0) continue;
-        const int ks = lo - slab0;  // the line's first entry in the slab tables
-        __syncthreads();  // the previous line's tables are no longer read
-        if (t < N) {
-            float c, s;
-            cis2pi((double)q2 / m2 * (double)(p2min + t), c, s);
-            ph2[t] = make_float2(c, s);
-        }
-        for (int e = t; e < nk * N; e += blockDim.x) {
-            const int kk = e / N, c3 = e % N;
-            const int q3 = sk_iq[ks + kk] % m3;
-            float c, s;
-            cis2pi((double)q3 / m3 * (double)(p3min + c3), c, s);
-            ph3[kk * N + c3] = make_float2(c, s);
-        }
+        __syncthreads();  // slab tables written; the previous chunk's tables no longer read
         for (int e = t; e < nk * NBW * NW; e += blockDim.x) {
             const int kk = e / (NBW * NW), m = (e / NW) % NBW, ii = e % NW;
-            const int nbm = sk_nb[ks + kk], nbk = nbm & 0xff, mlo = nbm >> 8;
+            const int nbm = sk_nb[ks + kk], nbk = nbm & 0xff, mlo = (nbm >> 8) & 0xff;
             float2 w = make_float2(0.f, 0.f);
             if (m < nbk) {
                 const cplx<float> v = U[(int64_t)NW * NW * sk_kq[ks + kk] + ii + NW * (mlo + m)];
@@ -370,25 +377,31 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
             }
             ucol[e] = w;
         }
-        __syncthreads();  // phase tables and U columns staged
+        __syncthreads();  // U columns staged
         float b2r[N], b2i[N];
-#pragma unroll
-        for (int c = 0; c < N; c++) {
-            float sr = 0.f, si = 0.f;
-#pragma unroll
-            for (int b = 0; b < N; b++) {
-                const float2 p = ph2[b];
-                sr += p.x * br[c][b] - p.y * bi[c][b];
-                si += p.x * bi[c][b] + p.y * br[c][b];
-            }
-            b2r[c] = sr;
-            b2i[c] = si;
-        }
+        int cur = -1;
         for (int kk = 0; kk < nk; kk++) {
+            const int w23 = sk_nb[ks + kk];
+            const int q2 = (w23 >> 22) & 63, q3 = (w23 >> 16) & 63;
+            if (q2 != cur) {  // uniform
+                cur = q2;
+#pragma unroll
+                for (int c = 0; c < N; c++) {
+                    float sr = 0.f, si = 0.f;
+#pragma unroll
+                    for (int b = 0; b < N; b++) {
+                        const float2 p = ph2[q2 * N + b];
+                        sr += p.x * br[c][b] - p.y * bi[c][b];
+                        si += p.x * bi[c][b] + p.y * br[c][b];
+                    }
+                    b2r[c] = sr;
+                    b2i[c] = si;
+                }
+            }
             float gr = 0.f, gi = 0.f;
 #pragma unroll
             for (int c = 0; c < N; c++) {
-                const float2 p = ph3[kk * N + c];
+                const float2 p = ph3[q3 * N + c];
                 gr += p.x * b2r[c] - p.y * b2i[c];
                 gi += p.x * b2i[c] + p.y * b2r[c];
             }
@@ -400,7 +413,7 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
                 if (rv && i == 0) tq[(kk * NBW + m) * NW + nu] = make_float2(tr, ti);
             }
         }
-        __syncthreads();  // t of every kept q of the line
+        __syncthreads();  // t of every kept q of the chunk
         for (int e = t; e < nk * nbw * nm; e += blockDim.x) {
             const int kk = e / (nbw * nm), rr = e % (nbw * nm), m = rr / nm, nu2 = rr % nm;
             const int iq = sk_iq[ks + kk];
@@ -416,7 +429,7 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
                 ei += on * (tv.x * p.im + tv.y * p.re);
             }
             const float w2 = 2.f * static_cast<float>(omega[(int64_t)nm * iq + nu2]);
-            g2[(row0 + lo + kk) * nbw * nm + rr] = (er * er + ei * ei) / w2;
+            g2[(row0 + slab0 + ks + kk) * nbw * nm + rr] = (er * er + ei * ei) / w2;
         }
     }
 }
@@ -445,8 +458,9 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
     EPHIP_REQUIRE(n1 >= 1 && n1 <= 32 && np >= 3 && np <= 8, "gridopt_kq_rotate: Rp box n1=%d (<= 32), np=%d (3..8)",
                   n1, np);
     EPHIP_REQUIRE(qgrid && qgrid[0] >= 1 && qgrid[1] >= 1 && qgrid[2] >= 1 && qgrid[2] <= GQ_M3_MAX &&
-                      qgrid[0] <= 65535 && qgrid[1] * qgrid[2] <= GQ_SLAB_MAX,
-                  "gridopt_kq_rotate: q grid must be n1 n2 n3 with n3 <= %d and n2 n3 <= %d", GQ_M3_MAX, GQ_SLAB_MAX);
+                      qgrid[1] <= GQ_M2_MAX && qgrid[0] <= 65535 && qgrid[1] * qgrid[2] <= GQ_SLAB_MAX,
+                  "gridopt_kq_rotate: q grid must be n1 n2 n3 with n2, n3 <= %d and n2 n3 <= %d", GQ_M3_MAX,
+                  GQ_SLAB_MAX);
     EPHIP_REQUIRE(nunits >= 0 && nunits <= 65535 && ldr >= (int64_t)nw * nm && lda >= ldr && cs >= 0,
                   "gridopt_kq_rotate: bad sizes");
     if (nunits == 0) return EPHIP_OK;

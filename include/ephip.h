@@ -250,7 +250,7 @@ int ephip_gridopt_kR_project(int32_t nw, int32_t nm, int32_t nrp, int32_t np, in
  * (p1min + a, p2min + b, p3min + c), a < n1 <= 32, b, c < np (3..8); rmap (int32, n1*np*np, index
  * a + n1*(b + np*c)) gives the Rp index or -1. line_off (int32, nunits x (m1*m2 + 1)) holds each unit's
  * first kept-list position per q line, from ephip_kq_line_offsets (lower bounds in the sorted kept
- * lists of ephip_kq_compact). nw = 16, nm <= 16, nbw <= 4, m3 <= 64, m2*m3 <= 1024. */
+ * lists of ephip_kq_compact). nw = 16, nm <= 16, nbw <= 4, m2, m3 <= 64, m2*m3 <= 1024. */
 int ephip_kq_line_offsets(int32_t nunits, const int32_t *count, const int32_t *col_index, int64_t cs, int32_t m3,
                           int32_t nlines, int32_t *line_off, void *stream);
 int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int32_t nunits, int32_t n1, int32_t np,
