@@ -195,6 +195,8 @@ def main():
     ap.add_argument("--tune", dest="tune", action="store_true", default=True,
                     help="autotune the Fourier kernel variant on the live batch first (default)")
     ap.add_argument("--no-tune", dest="tune", action="store_false")
+    ap.add_argument("--pipeline", action="store_true",
+                    help="run the k batches on two overlapping streams (RR->kR of the next batch beside kR->kq)")
     ap.add_argument("--no-gridopt", dest="gridopt", action="store_false", default=True,
                     help="direct (MFMA) Fourier contractions instead of the staged GridOpt kernels")
     ap.add_argument("--variant", action="append", default=[], metavar="KIND=V",
@@ -312,20 +314,27 @@ def main():
         ev = []
         loop.ft_timer = ev
         nks_steps = args.kbatch_k
-        for s in range(args.warmup):
-            loop.run_kbatch(s * nks_steps, nks_steps)
+        warm = [s * nks_steps for s in range(args.warmup)]
+        k_base = args.warmup * nks_steps
+        timed = [(k_base + s * nks_steps) % max(1, nks - nks_steps) for s in range(args.steps)]
+        if args.pipeline:  # RR->kR of batch b+1 on a second stream beside kR->kq of batch b
+            loop.run_pipelined(warm, nks_steps)
+        else:
+            for i0 in warm:
+                loop.run_kbatch(i0, nks_steps)
         torch.cuda.synchronize()
         ev.clear()
-        k_base = args.warmup * nks_steps
         t0 = time.perf_counter()
-        for s in range(args.steps):
-            loop.run_kbatch((k_base + s * nks_steps) % max(1, nks - nks_steps), nks_steps)
+        if args.pipeline:
+            loop.run_pipelined(timed, nks_steps)
+        else:
+            for i0 in timed:
+                loop.run_kbatch(i0, nks_steps)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         # algorithmic flops of the launches actually made (direct or staged kernels), step by step
         cols, f_rrkr_total, f_krkq_total = 0, 0.0, 0.0
-        for s in range(args.steps):
-            i0 = (k_base + s * nks_steps) % max(1, nks - nks_steps)
+        for i0 in timed:
             kept = kept_pairs(range(i0, min(i0 + nks_steps, nks)))
             fr, fk = loop.kernel_flops(i0, nks_steps, kept)
             cols += kept

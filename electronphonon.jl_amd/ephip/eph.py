@@ -476,26 +476,66 @@ class OuterKLoop:
         return r
 
     def run_kbatch(self, i0: int, n: int, calculators=()):
-        import ctypes as C
-        st = self.st
-        nw, nm = st.nw, st.nm
-        n = min(n, st.ksurv.shape[0] - i0)
+        n = min(n, self.st.ksurv.shape[0] - i0)
         if n <= 0:
             return None
         u0, u1 = int(self.k_unit_start[i0]), int(self.k_unit_start[i0 + n])
-        nu = u1 - u0
         stream = L.stream_handle()
+        self._stage_rrkr(i0, n, u0, u1, self.A, stream)
+        return self._stage_krkq(i0, n, u0, u1, self.A, calculators, stream)
+
+    def run_pipelined(self, starts, n: int, calculators=()):
+        """Run the k batches starting at the surviving-k positions ``starts`` (n k each) on two streams of
+        default priority: RR->kR of batch b+1 overlaps kR->kq (and the consumers) of batch b, with two
+        unit-operator buffers and events between the streams. Consumers run on the kR->kq stream in batch
+        order; the caller's stream waits for both at the end."""
+        torch = _torch()
+        if getattr(self, "A2", None) is None:
+            self.A2 = torch.empty_like(self.A)
+        cur = torch.cuda.current_stream()
+        s_rr, s_kq = torch.cuda.Stream(), torch.cuda.Stream()
+        s_rr.wait_stream(cur)
+        s_kq.wait_stream(cur)
+        bufs, free = (self.A, self.A2), [None, None]
+        nks = self.st.ksurv.shape[0]
+        out = []
+        for b, i0 in enumerate(starts):
+            nb = min(n, nks - i0)
+            if nb <= 0:
+                continue
+            u0, u1 = int(self.k_unit_start[i0]), int(self.k_unit_start[i0 + nb])
+            A = bufs[b % 2]
+            with torch.cuda.stream(s_rr):
+                if free[b % 2] is not None:
+                    s_rr.wait_event(free[b % 2])
+                self._stage_rrkr(i0, nb, u0, u1, A, L.stream_handle())
+                ready = torch.cuda.Event()
+                ready.record(s_rr)
+            with torch.cuda.stream(s_kq):
+                s_kq.wait_event(ready)
+                out.append(self._stage_krkq(i0, nb, u0, u1, A, calculators, L.stream_handle()))
+                free[b % 2] = torch.cuda.Event()
+                free[b % 2].record(s_kq)
+        cur.wait_stream(s_rr)
+        cur.wait_stream(s_kq)
+        return out
+
+    def _stage_rrkr(self, i0, n, u0, u1, A, stream):
+        """RR->kR over Re for the batch and the k-side projection: writes the unit operators into A."""
+        st = self.st
+        nw, nm = st.nw, st.nm
+        nu = u1 - u0
         # 1. RR->kR Fourier over Re for the batch
         xk_b = self.xk_surv[i0:i0 + n]
         if self.gridopt is not None:  # staged RR->kR with the projection fused: writes the unit operators A
             self._timed("RR->kR", 8.0 * self.nd_el * self.nre * n,
-                        lambda: self.gridopt.run(self, i0, n, u0, u1, stream))
+                        lambda: self.gridopt.run(self, i0, n, u0, u1, A, stream))
         elif self.fused_proj:  # RR->kR with the projection in its epilogue: writes the unit operators A
             bpk = core.ft_pack_phases(st.irvec_el_dev, xk_b, out=self.bpk_k)
             self._timed("RR->kR", 8.0 * self.nd_el * self.nre * n, lambda: L.call(
                 "ephip_eph_RR_to_kR_project_packed_bf16x3", nw, nm, self.nrp, self.nre, L.ptr(self.apk_el), L.ptr(bpk),
                 n, n, L.ptr(self.k_unit_start_dev[i0:]), L.ptr(self.unit_state[u0:u1]), L.ptr(self.unit_band[u0:u1]),
-                L.ptr(st.el.U), self.max_units_per_k, L.ptr(self.A), self.A.stride(0), stream))
+                L.ptr(st.el.U), self.max_units_per_k, L.ptr(A), A.stride(0), stream))
         elif self.packed:
             bpk = core.ft_pack_phases(st.irvec_el_dev, xk_b, out=self.bpk_k)
             self._timed("RR->kR", 8.0 * self.nd_el * self.nre * n, lambda: core.fourier_packed(
@@ -508,7 +548,15 @@ class OuterKLoop:
             g_base = L.ptr(self.G) - i0 * self.G.stride(0) * 8
             L.call("ephip_eph_project_kR", nw, nm, self.nrp, nu, g_base, self.G.stride(0),
                    L.ptr(self.unit_kpos[u0:u1]), L.ptr(self.unit_state[u0:u1]), L.ptr(self.unit_band[u0:u1]),
-                   L.ptr(st.el.U), L.ptr(self.A), stream)
+                   L.ptr(st.el.U), L.ptr(A), stream)
+
+    def _stage_krkq(self, i0, n, u0, u1, A, calculators, stream):
+        """Kept q per unit, kR->kq over Rp with the rotations and g2 from the unit operators in A, and the
+        consumers of the batch payload."""
+        import ctypes as C
+        st = self.st
+        nw, nm = st.nw, st.nm
+        nu = u1 - u0
         # 2. kept q per unit
         g = (C.c_int32 * 3)(*map(int, st.kgrid))
         L.call("ephip_kq_compact", nu, L.ptr(self.unit_state[u0:u1]), L.ptr(st.el.xk), self.nq, L.ptr(st.ph.xq),
@@ -517,9 +565,9 @@ class OuterKLoop:
         # 3. kR->kq batched Fourier over Rp at the kept q
         nd2 = nw * nm
         if self.gridopt_q is not None:
-            self._timed("kR->kq", None, lambda: self.gridopt_q.run(self, nu, stream))
+            self._timed("kR->kq", None, lambda: self.gridopt_q.run(self, nu, A, stream))
         elif self.fused:
-            core.ft_pack_operand(self.A, nd2, self.nrp, nd2, nu, nd2 * self.nrp, out=self.apk_units)
+            core.ft_pack_operand(A, nd2, self.nrp, nd2, nu, nd2 * self.nrp, out=self.apk_units)
             self._timed("kR->kq", None, lambda: L.call(
                 "ephip_eph_kq_rotate_packed_bf16x3", nw, nm, st.nbw, self.nrp, L.ptr(self.apk_units), nu,
                 L.ptr(self.bpk_q), self.nq, self.cs, self.max_kept, L.ptr(self.count), L.ptr(self.col_index), L.ptr(self.ikq),
@@ -527,14 +575,14 @@ class OuterKLoop:
                 L.ptr(self.omega), L.ptr(self.g2), L.ptr(self.irvec_ep_dev), C.addressof(self.qgrid_c), stream))
         elif self.packed:
             def krkq():
-                core.ft_pack_operand(self.A, nd2, self.nrp, nd2, nu, nd2 * self.nrp, out=self.apk_units)
+                core.ft_pack_operand(A, nd2, self.nrp, nd2, nu, nd2 * self.nrp, out=self.apk_units)
                 core.fourier_packed(self.apk_units, nd2, self.nrp, self.bpk_q, self.nq, self.cs, self.Gq, nd2,
                                     nunits=nu, ncol=self.count, col_index=self.col_index, col_stride=self.cs,
                                     out_stride=self.cs * nd2)
             self._timed("kR->kq", None, krkq)
         else:
             self._timed("kR->kq", None, lambda: L.call(
-                "ephip_fourier_batched_c32", nd2, self.nrp, L.ptr(self.A), nd2, nd2 * self.nrp,
+                "ephip_fourier_batched_c32", nd2, self.nrp, L.ptr(A), nd2, nd2 * self.nrp,
                 L.ptr(self.irvec_ep_dev), L.ptr(st.ph.xq), nu, self.nq, L.ptr(self.count), L.ptr(self.col_index),
                 self.cs, L.ptr(self.Gq), nd2, self.cs * nd2, self.precision, stream))
         # 4. rotations + g2 (already done by the fused kernels)
@@ -666,7 +714,7 @@ class _GridOptKR:
                L.ptr(epmat_el), epmat_el.stride(0), float(self.x1[k_first]), self._nb, L.ptr(buf), stream)
         return buf
 
-    def run(self, loop, i0, n, u0, u1, stream):
+    def run(self, loop, i0, n, u0, u1, A, stream):
         st = loop.st
         l_first, l_last = int(self.line_of_k[i0]), int(self.line_of_k[i0 + n - 1]) + 1
         l = l_first
@@ -680,7 +728,7 @@ class _GridOptKR:
             L.call("ephip_gridopt_kR_project", st.nw, st.nm, loop.nrp, self.np, self.rmin[1], self.rmin[2],
                    L.ptr(buf), L.ptr(loop.xk_surv), i0, i0 + n, l, l_end, L.ptr(self.line_start), self.max_line_k,
                    L.ptr(loop.k_unit_start_dev), L.ptr(loop.unit_state[u0:u1]), L.ptr(loop.unit_band[u0:u1]),
-                   L.ptr(st.el.U), loop.max_units_per_k, L.ptr(loop.A), loop.A.stride(0), stream)
+                   L.ptr(st.el.U), loop.max_units_per_k, L.ptr(A), A.stride(0), stream)
             l = l_end
 
 
@@ -713,13 +761,13 @@ class _GridOptKQ:
         import ctypes
         self.qgrid_c = (ctypes.c_int32 * 3)(*self.m)
 
-    def run(self, loop, nu, stream):
+    def run(self, loop, nu, A, stream):
         import ctypes as C
         st = loop.st
         L.call("ephip_kq_line_offsets", nu, L.ptr(loop.count), L.ptr(loop.col_index), loop.cs, self.m[2],
                self.nlines, L.ptr(loop.line_off), stream)
         L.call("ephip_gridopt_kq_rotate", st.nw, st.nm, st.nbw, nu, self.n1, self.np, self.pmin[0], self.pmin[1],
-               self.pmin[2], L.ptr(self.rmap), C.addressof(self.qgrid_c), L.ptr(loop.A), loop.A.stride(0),
+               self.pmin[2], L.ptr(self.rmap), C.addressof(self.qgrid_c), L.ptr(A), A.stride(0),
                st.nw * st.nm, L.ptr(loop.line_off), L.ptr(loop.col_index), L.ptr(loop.ikq), loop.cs,
                L.ptr(st.el.U), L.ptr(st.el.filt.band_min), L.ptr(st.el.filt.band_max), L.ptr(loop.uph),
                L.ptr(loop.omega), L.ptr(loop.g2), stream)

@@ -291,3 +291,31 @@ def test_outer_k_gridopt_kq(backend, model16, nq, window, nb):
     assert den > 0 and num / den < 1e-5, num / den
     rel = _outer_k_vs_oracle(m, g, st, rs, qstep=5 if nq == 4 else 37)
     assert rel < 1e-4, rel
+
+
+@pytest.mark.parametrize("gridopt", [True, False])
+def test_outer_k_pipelined(backend, model16, gridopt):
+    """Batches run on two streams (RR->kR of the next batch beside kR->kq of the current one, double-
+    buffered unit operators) give the same payloads, bit for bit, as the batches run one by one."""
+    from ephip import _lib as L
+    from ephip import eph
+    m, g = model16
+    g = dict(g, qgrid=(8, 8, 8))
+    w = g["window"]
+    st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
+    nks = st.ksurv.shape[0]
+    nb = 37
+    starts = list(range(0, nks, nb)) + [0, nb]  # every batch, then two again (buffers reused)
+    prec = L.PREC_FP32 if gridopt else L.PREC_BF16X3
+    seq = eph.OuterKLoop(st, nk_batch=nb, precision=prec, gridopt=gridopt)
+    pip = eph.OuterKLoop(st, nk_batch=nb, precision=prec, gridopt=gridopt)
+    rs, rp = eph.UnitRecorder(), eph.UnitRecorder()
+    for i0 in starts:
+        seq.run_kbatch(i0, nb, [rs])
+    pip.run_pipelined(starts, nb, [rp])
+    assert len(rs.batches) == len(rp.batches) == len(starts)
+    for bs, bp in zip(rs.batches, rp.batches):
+        assert np.array_equal(bs["unit_state"], bp["unit_state"])
+        for u in range(len(bs["cols"])):
+            assert np.array_equal(bs["cols"][u], bp["cols"][u])
+            assert np.array_equal(bs["g2"][u], bp["g2"][u])
