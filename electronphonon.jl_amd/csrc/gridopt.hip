@@ -17,6 +17,15 @@
 
 namespace ephip {
 
+// (sr, si) += (ar + i ai)(br + i bi) as four FMAs (the form `sr += ar*br - ai*bi` compiles to six
+// instructions: a product, an FMA and an add per component).
+__device__ __forceinline__ void cmac(float &sr, float &si, float ar, float ai, float br, float bi) {
+    sr = fmaf(ar, br, sr);
+    sr = fmaf(-ai, bi, sr);
+    si = fmaf(ar, bi, si);
+    si = fmaf(ai, br, si);
+}
+
 constexpr int GO_N1_MAX = 32;
 
 // Stage 1. One thread per operand row d; op1 rows are written in the j-fastest order
@@ -53,8 +62,7 @@ __global__ __launch_bounds__(256) void gridopt_stage1_kernel(int64_t nd, int n1,
                     if (ir < 0) continue;
                     const cplx<float> v = o[ldop * ir];
                     const float2 p = ph[a];
-                    sr += p.x * v.re - p.y * v.im;
-                    si += p.x * v.im + p.y * v.re;
+                    cmac(sr, si, p.x, p.y, v.re, v.im);
                 }
             }
             op1[dd + nd * (b + np * c)] = cplx<float>{sr, si};
@@ -159,8 +167,7 @@ __global__ __launch_bounds__(256) void gridopt_kR_project_kernel(
 #pragma unroll
             for (int b = 0; b < N2; b++) {
                 const float2 p = ph2[b];
-                sr += p.x * vr[c][b] - p.y * vi[c][b];
-                si += p.x * vi[c][b] + p.y * vr[c][b];
+                cmac(sr, si, p.x, p.y, vr[c][b], vi[c][b]);
             }
             o2r[c] = sr;
             o2i[c] = si;
@@ -171,8 +178,7 @@ __global__ __launch_bounds__(256) void gridopt_kR_project_kernel(
 #pragma unroll
             for (int c = 0; c < N3; c++) {
                 const float2 p = ph3[kk * N3 + c];
-                gr += p.x * o2r[c] - p.y * o2i[c];
-                gi += p.x * o2i[c] + p.y * o2r[c];
+                cmac(gr, gi, p.x, p.y, o2r[c], o2i[c]);
             }
             const int ub = kub[kk], nbk = knb[kk];
             for (int m = 0; m < nbk; m++) {  // uniform across the block
@@ -355,8 +361,7 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
             }
 #pragma unroll
             for (int b = 0; b < N; b++) {
-                br[c][b] += p.x * v[b].re - p.y * v[b].im;
-                bi[c][b] += p.x * v[b].im + p.y * v[b].re;
+                cmac(br[c][b], bi[c][b], p.x, p.y, v[b].re, v[b].im);
             }
         }
     int l0 = 0;
@@ -391,8 +396,7 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
 #pragma unroll
                     for (int b = 0; b < N; b++) {
                         const float2 p = ph2[q2 * N + b];
-                        sr += p.x * br[c][b] - p.y * bi[c][b];
-                        si += p.x * bi[c][b] + p.y * br[c][b];
+                        cmac(sr, si, p.x, p.y, br[c][b], bi[c][b]);
                     }
                     b2r[c] = sr;
                     b2i[c] = si;
@@ -402,8 +406,7 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
 #pragma unroll
             for (int c = 0; c < N; c++) {
                 const float2 p = ph3[q3 * N + c];
-                gr += p.x * b2r[c] - p.y * b2i[c];
-                gi += p.x * b2i[c] + p.y * b2r[c];
+                cmac(gr, gi, p.x, p.y, b2r[c], b2i[c]);
             }
 #pragma unroll
             for (int m = 0; m < NBW; m++) {
@@ -425,8 +428,7 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
                 const float on = n < nm ? 1.f : 0.f;
                 const float2 tv = tq[(kk * NBW + m) * NW + nn];
                 const cplx<float> p = up[nn];
-                er += on * (tv.x * p.re - tv.y * p.im);
-                ei += on * (tv.x * p.im + tv.y * p.re);
+                cmac(er, ei, on * tv.x, on * tv.y, p.re, p.im);
             }
             const float w2 = 2.f * static_cast<float>(omega[(int64_t)nm * iq + nu2]);
             g2[(row0 + slab0 + ks + kk) * nbw * nm + rr] = (er * er + ei * ei) / w2;
