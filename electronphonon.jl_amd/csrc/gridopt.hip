@@ -291,7 +291,7 @@ constexpr int GQ_M3_MAX = 64, GQ_SLAB_MAX = 1024;
 // (zero past the k+q window), the layout of ephip_eph_kq_rotate_packed_bf16x3.
 constexpr int GQ_CHUNK = 32, GQ_M2_MAX = 64;
 
-template <int N, int NBW>
+template <int N, int NBW, int NMC = 0>  // NMC: compile-time mode count (12), or 0 for nm <= 16 at run time
 __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
     int nm, int n1, int p1min, int p2min, int p3min, const int *__restrict__ rmap, int m1, int m2, int m3,
     const cplx<float> *__restrict__ A, int64_t lda, int64_t ldr, const int *__restrict__ line_off,
@@ -422,13 +422,22 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
             const int iq = sk_iq[ks + kk];
             const cplx<float> *up = uph + (int64_t)nm * nm * iq + nm * nu2;
             float er = 0.f, ei = 0.f;
+            if constexpr (NMC > 0) {
 #pragma unroll
-            for (int n = 0; n < NW; n++) {  // nm <= NW: fixed trip count, loads issued together
-                const int nn = min(n, nm - 1);
-                const float on = n < nm ? 1.f : 0.f;
-                const float2 tv = tq[(kk * NBW + m) * NW + nn];
-                const cplx<float> p = up[nn];
-                cmac(er, ei, on * tv.x, on * tv.y, p.re, p.im);
+                for (int n = 0; n < NMC; n++) {  // constant offsets from one base address
+                    const float2 tv = tq[(kk * NBW + m) * NW + n];
+                    const cplx<float> p = up[n];
+                    cmac(er, ei, tv.x, tv.y, p.re, p.im);
+                }
+            } else {
+#pragma unroll
+                for (int n = 0; n < NW; n++) {  // nm <= NW: fixed trip count, loads issued together
+                    const int nn = min(n, nm - 1);
+                    const float on = n < nm ? 1.f : 0.f;
+                    const float2 tv = tq[(kk * NBW + m) * NW + nn];
+                    const cplx<float> p = up[nn];
+                    cmac(er, ei, on * tv.x, on * tv.y, p.re, p.im);
+                }
             }
             const float w2 = 2.f * static_cast<float>(omega[(int64_t)nm * iq + nu2]);
             g2[(row0 + slab0 + ks + kk) * nbw * nm + rr] = (er * er + ei * ei) / w2;
@@ -472,11 +481,13 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
     const unsigned threads = (unsigned)(((nd2 + 63) / 64) * 64);
     dim3 g((unsigned)qgrid[0], (unsigned)nunits);
     using K = decltype(&gridopt_kq_rotate_kernel<3, 2>);
-#define GQ_ROW(nb) {gridopt_kq_rotate_kernel<3, nb>, gridopt_kq_rotate_kernel<4, nb>, gridopt_kq_rotate_kernel<5, nb>, \
-                    gridopt_kq_rotate_kernel<6, nb>, gridopt_kq_rotate_kernel<7, nb>, gridopt_kq_rotate_kernel<8, nb>}
-    static const K table[3][6] = {GQ_ROW(2), GQ_ROW(3), GQ_ROW(4)};
+#define GQ_ROW(nb, c) {gridopt_kq_rotate_kernel<3, nb, c>, gridopt_kq_rotate_kernel<4, nb, c>, \
+                       gridopt_kq_rotate_kernel<5, nb, c>, gridopt_kq_rotate_kernel<6, nb, c>, \
+                       gridopt_kq_rotate_kernel<7, nb, c>, gridopt_kq_rotate_kernel<8, nb, c>}
+    static const K table[2][3][6] = {{GQ_ROW(2, 0), GQ_ROW(3, 0), GQ_ROW(4, 0)},
+                                     {GQ_ROW(2, 12), GQ_ROW(3, 12), GQ_ROW(4, 12)}};
 #undef GQ_ROW
-    const K kern = table[nbw <= 2 ? 0 : nbw - 2][np - 3];
+    const K kern = table[nm == 12 ? 1 : 0][nbw <= 2 ? 0 : nbw - 2][np - 3];
     kern<<<g, threads, 0, as_stream(stream)>>>(nm, n1, p1min, p2min, p3min, rmap, qgrid[0], qgrid[1], qgrid[2],
                                                 static_cast<const cplx<float> *>(A), lda, ldr, line_off, col_index,
                                                 ikq, cs, static_cast<const cplx<float> *>(U), bmin, bmax,
