@@ -36,10 +36,10 @@ __global__ __launch_bounds__(256) void gridopt_stage1_kernel(int64_t nd, int n1,
                                                              const int *__restrict__ rmap,
                                                              const cplx<float> *__restrict__ op, int64_t ldop,
                                                              double x1, int nb, cplx<float> *__restrict__ op1) {
-    __shared__ float2 ph[GO_N1_MAX];
-    if ((int)threadIdx.x < n1) {
-        float c, s;
-        cis2pi(x1 * (double)(r1min + (int)threadIdx.x), c, s);
+    __shared__ float2 ph[GO_N1_MAX];  // zero past n1 (the tail of the last block of 8 reads it)
+    if ((int)threadIdx.x < GO_N1_MAX) {
+        float c = 0.f, s = 0.f;
+        if ((int)threadIdx.x < n1) cis2pi(x1 * (double)(r1min + (int)threadIdx.x), c, s);
         ph[threadIdx.x] = make_float2(c, s);
     }
     __syncthreads();
@@ -56,13 +56,18 @@ __global__ __launch_bounds__(256) void gridopt_stage1_kernel(int64_t nd, int n1,
             float sr = 0.f, si = 0.f;
             if (b < n2 && c < n3) {
                 const int bc = b + n2 * c;
-#pragma unroll 8
-                for (int a = 0; a < n1; a++) {
-                    const int ir = rmap[a + n1 * bc];  // uniform across the block
-                    if (ir < 0) continue;
-                    const cplx<float> v = o[ldop * ir];
-                    const float2 p = ph[a];
-                    cmac(sr, si, p.x, p.y, v.re, v.im);
+                // r1 in blocks of 8, branch-free, so a block's index and operand loads issue together: an
+                // absent R (rmap -1) and the block's tail past n1 load column 0 and contribute zero
+                for (int a0 = 0; a0 < n1; a0 += 8) {
+#pragma unroll
+                    for (int da = 0; da < 8; da++) {
+                        const int a = a0 + da;
+                        const int ir = a < n1 ? rmap[a + n1 * bc] : -1;  // uniform across the block
+                        cplx<float> v = o[ldop * max(ir, 0)];
+                        if (ir < 0) v = cplx<float>{0.f, 0.f};
+                        const float2 p = ph[min(a, GO_N1_MAX - 1)];
+                        cmac(sr, si, p.x, p.y, v.re, v.im);
+                    }
                 }
             }
             op1[dd + nd * (b + np * c)] = cplx<float>{sr, si};
@@ -173,19 +178,167 @@ __global__ __launch_bounds__(256) void gridopt_kR_project_kernel(
             o2i[c] = si;
         }
         __syncthreads();  // U columns staged
+        // each lane keeps one k's first unit and band count (nkl <= 64 = wave size); the loop reads them
+        // with readlane, so no LDS round trip sits in front of a k's loads
+        const int lane = t & 63;
+        const int kubv = lane < nkl ? kub[lane] : 0, knbv = lane < nkl ? knb[lane] : 0;
         for (int kk = 0; kk < nkl; kk++) {
+            float2 w[GO_NBW_MAX];  // all slots read up front (in bounds; slots past the k's bands unused)
+#pragma unroll
+            for (int m = 0; m < GO_NBW_MAX; m++) w[m] = uc[(kk * GO_NBW_MAX + m) * NW + j];
             float gr = 0.f, gi = 0.f;
 #pragma unroll
             for (int c = 0; c < N3; c++) {
                 const float2 p = ph3[kk * N3 + c];
                 cmac(gr, gi, p.x, p.y, o2r[c], o2i[c]);
             }
-            const int ub = kub[kk], nbk = knb[kk];
-            for (int m = 0; m < nbk; m++) {  // uniform across the block
-                const float2 w = uc[(kk * GO_NBW_MAX + m) * NW + j];
-                const float sr = dpp_group_sum<NW>(gr * w.x - gi * w.y);
-                const float si = dpp_group_sum<NW>(gr * w.y + gi * w.x);
-                if (dv && j == 0) A[(int64_t)(ub + m) * lda + G] = cplx<float>{sr, si};
+            const int ub = __builtin_amdgcn_readlane(kubv, kk), nbk = __builtin_amdgcn_readlane(knbv, kk);
+#pragma unroll
+            for (int m = 0; m < GO_NBW_MAX; m++) {
+                if (m < nbk) {  // uniform across the block
+                    const float sr = dpp_group_sum<NW>(gr * w[m].x - gi * w[m].y);
+                    const float si = dpp_group_sum<NW>(gr * w[m].y + gi * w[m].x);
+                    if (dv && j == 0) A[(int64_t)(ub + m) * lda + G] = cplx<float>{sr, si};
+                }
+            }
+        }
+    }
+}
+
+// Stages 2 and 3 with the window projection on the matrix cores (nw = 16). Along a line the projected
+// operators are one complex product with a k-independent right factor:
+//   A[(k, n)][G] = sum_{j, c} W[(k, n)][(j, c)] op2[(j, c)][G],   W = U_k[j, n] e^{2 pi i k3 (r3min + c)}.
+// A workgroup holds one block of 16 G (i = lane % 16) and all 16 j (four per wave, j = thread / 16): op1
+// in its natural row order d = i + 16 (j + 16 rest), so each lane's op2[c] is its own B operand of the
+// 16x16x4 FP32 MFMA (B[j][i]) and W is built in the A operand (A[pair][j]). The pairs (k, n) of a line
+// are taken 16 at a time; per c each wave runs four MFMAs (a complex product, four independent
+// accumulators) over its four j, and the four waves' partial tiles are summed through LDS.
+template <int N>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 : 2))) void
+gridopt_kR_project_mfma_kernel(
+    int64_t nd, int r2min, int r3min, const cplx<float> *__restrict__ op1,
+    const double *__restrict__ xk, int64_t k0, int64_t k1, int l0, int l1, const int *__restrict__ line_start,
+    const int *__restrict__ kus, const int64_t *__restrict__ ustate, const int *__restrict__ uband,
+    const cplx<float> *__restrict__ U, int nbw, cplx<float> *__restrict__ A, int64_t lda) {
+    constexpr int NW = 16, NP = GO_KLINE_MAX * GO_NBW_MAX;
+    __shared__ float2 ph2[N];
+    __shared__ float2 ph3[GO_KLINE_MAX * N];
+    __shared__ int kub[GO_KLINE_MAX], knb[GO_KLINE_MAX];
+    extern __shared__ float2 uc[];                      // [kk][m][j], kk < max_line_k, m < nbw (dynamic)
+    __shared__ int prow[NP], punit[NP];                 // pair -> kk * nbw + m, unit
+    __shared__ int npair;
+    __shared__ f32x4 red[2][2][4][64];  // the waves' partial tiles (real, imaginary), two buffers
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int j = t >> 4;  // = 4 wave + lane / 16
+    const int64_t dp = (int64_t)blockIdx.x * blockDim.x + t;
+    const bool dv = dp < nd;
+    const int64_t gcol = (int64_t)blockIdx.x * NW + (t & 15);  // G = i + 16 rest
+    float vr[N][N], vi[N][N];
+#pragma unroll
+    for (int c = 0; c < N; c++)
+#pragma unroll
+        for (int b = 0; b < N; b++) {
+            const cplx<float> v = dv ? op1[dp + nd * (b + N * c)] : cplx<float>{0.f, 0.f};
+            vr[c][b] = v.re;
+            vi[c][b] = v.im;
+        }
+    const int ku0 = kus[k0];
+    for (int l = l0; l < l1; l++) {
+        const int64_t kb = max((int64_t)line_start[l], k0), ke = min((int64_t)line_start[l + 1], k1);
+        const int nkl = (int)(ke - kb);  // uniform; at most GO_KLINE_MAX (host check)
+        if (nkl <= 0) continue;
+        __syncthreads();  // the previous line's tables are no longer read
+        if (t < N) {
+            float c, s;
+            cis2pi(xk[3 * kb + 1] * (double)(r2min + t), c, s);
+            ph2[t] = make_float2(c, s);
+        }
+        for (int e = t; e < nkl * N; e += blockDim.x) {
+            const int kk = e / N, c3 = e % N;
+            float c, s;
+            cis2pi(xk[3 * (kb + kk) + 2] * (double)(r3min + c3), c, s);
+            ph3[kk * N + c3] = make_float2(c, s);
+        }
+        if (t < nkl) {
+            kub[t] = kus[kb + t] - ku0;
+            knb[t] = min(kus[kb + t + 1] - kus[kb + t], nbw);
+        }
+        __syncthreads();
+        for (int e = t; e < nkl * nbw * NW; e += blockDim.x) {
+            const int kk = e / (nbw * NW), m = (e / NW) % nbw, jj = e % NW;
+            float2 w = make_float2(0.f, 0.f);
+            if (m < knb[kk]) {
+                const int ub = kub[kk];
+                const cplx<float> v = U[(int64_t)NW * NW * ustate[ub] + jj + NW * uband[ub + m]];
+                w = make_float2(v.re, v.im);
+            }
+            uc[(kk * nbw + m) * NW + jj] = w;
+        }
+        if (t < 64) {  // the line's (k, n) pairs in k order: exclusive prefix of the band counts (<= 4)
+            const int nbk = t < nkl ? knb[t] : 0, ub = t < nkl ? kub[t] : 0;
+            const unsigned long long below = (1ull << t) - 1ull;
+            int start = 0;
+#pragma unroll
+            for (int bit = 0; bit < 3; bit++) start += __popcll(__ballot((nbk >> bit) & 1) & below) << bit;
+            for (int m = 0; m < nbk; m++) {
+                prow[start + m] = t * nbw + m;
+                punit[start + m] = ub + m;
+            }
+            if (t == 63) npair = start + nbk;
+        }
+        float o2r[N], o2i[N];
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+            float sr = 0.f, si = 0.f;
+#pragma unroll
+            for (int b = 0; b < N; b++) {
+                const float2 p = ph2[b];
+                cmac(sr, si, p.x, p.y, vr[c][b], vi[c][b]);
+            }
+            o2r[c] = sr;
+            o2i[c] = si;
+        }
+        __syncthreads();  // U columns, pairs staged
+        const int np = npair;
+        for (int p0 = 0; p0 < np; p0 += 16) {
+            // A operand: row pair p0 + lane % 16 (zero past the line's pairs), column j
+            const int p = p0 + (lane & 15);
+            int kk = 0;
+            float ur = 0.f, ui = 0.f;
+            if (p < np) {
+                const int rw = prow[p];
+                kk = rw / nbw;
+                const float2 w = uc[rw * NW + j];
+                ur = w.x;
+                ui = w.y;
+            }
+            f32x4 arr = {0.f, 0.f, 0.f, 0.f}, aii = arr, ari = arr, air = arr;
+#pragma unroll
+            for (int c = 0; c < N; c++) {
+                const float2 e = ph3[kk * N + c];
+                const float wr = ur * e.x - ui * e.y, wi = ur * e.y + ui * e.x;
+                arr = __builtin_amdgcn_mfma_f32_16x16x4f32(wr, o2r[c], arr, 0, 0, 0);
+                aii = __builtin_amdgcn_mfma_f32_16x16x4f32(wi, o2i[c], aii, 0, 0, 0);
+                ari = __builtin_amdgcn_mfma_f32_16x16x4f32(wr, o2i[c], ari, 0, 0, 0);
+                air = __builtin_amdgcn_mfma_f32_16x16x4f32(wi, o2r[c], air, 0, 0, 0);
+            }
+            // this wave's four j: lane holds pair rows 4 (lane / 16) + r, column i = lane % 16. Tiles
+            // alternate between two buffers, so one barrier per tile separates writes from reads (the
+            // buffer written next was last read before this tile's barrier)
+            const int buf = (p0 >> 4) & 1;
+            red[buf][0][wave][lane] = arr - aii;
+            red[buf][1][wave][lane] = ari + air;
+            __syncthreads();
+            {
+                const int row = t >> 4, src = (t & 15) + 16 * (row >> 2), r = row & 3;
+                const float *rf = reinterpret_cast<const float *>(&red[buf][0][0][0]);
+                float sr = 0.f, si = 0.f;
+#pragma unroll
+                for (int w = 0; w < 4; w++) {
+                    sr += rf[(w * 64 + src) * 4 + r];
+                    si += rf[((4 + w) * 64 + src) * 4 + r];
+                }
+                if (dv && p0 + row < np) A[(int64_t)punit[p0 + row] * lda + gcol] = cplx<float>{sr, si};
             }
         }
     }
@@ -248,6 +401,37 @@ extern "C" int ephip_gridopt_kR_project(int32_t nw, int32_t nm, int32_t nrp, int
     return EPHIP_OK;
 }
 
+extern "C" int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp, int32_t np, int32_t r2min,
+                                             int32_t r3min, const void *op1, const double *xk, int64_t k0,
+                                             int64_t k1, int32_t l0, int32_t l1, const int32_t *line_start,
+                                             int32_t max_line_k, const int32_t *kus, const int64_t *ustate,
+                                             const int32_t *uband, const void *U, int32_t nbw, void *A, int64_t lda,
+                                             void *stream) {
+    EPHIP_REQUIRE(nw == 16, "gridopt_kR_project_mfma: nw must be 16 (got %d)", nw);
+    EPHIP_REQUIRE(nm >= 1 && nrp >= 1 && nbw >= 1 && nbw <= GO_NBW_MAX && k0 >= 0 && k1 >= k0 && l1 >= l0 &&
+                      l0 >= 0,
+                  "gridopt_kR_project_mfma: bad sizes (nbw=%d, at most %d window bands)", nbw, GO_NBW_MAX);
+    EPHIP_REQUIRE(np >= 3 && np <= 8, "gridopt_kR_project_mfma: padded box face np x np with 3 <= np <= 8 (got %d)",
+                  np);
+    EPHIP_REQUIRE(max_line_k >= 0 && max_line_k <= GO_KLINE_MAX, "gridopt_kR_project_mfma: %d k on a line > %d",
+                  max_line_k, GO_KLINE_MAX);
+    EPHIP_REQUIRE(lda >= (int64_t)nw * nm * nrp, "gridopt_kR_project_mfma: lda smaller than nw nm nrp");
+    const int64_t nd = (int64_t)nw * nw * nm * nrp;
+    if (k1 == k0 || l1 == l0) return EPHIP_OK;
+    EPHIP_REQUIRE(op1 && xk && line_start && kus && ustate && uband && U && A,
+                  "gridopt_kR_project_mfma: null device pointer");
+    using K = decltype(&gridopt_kR_project_mfma_kernel<3>);
+    static const K kern[6] = {gridopt_kR_project_mfma_kernel<3>, gridopt_kR_project_mfma_kernel<4>,
+                              gridopt_kR_project_mfma_kernel<5>, gridopt_kR_project_mfma_kernel<6>,
+                              gridopt_kR_project_mfma_kernel<7>, gridopt_kR_project_mfma_kernel<8>};
+    const size_t lds = (size_t)max(max_line_k, 1) * nbw * nw * sizeof(float2);  // U columns of a line
+    kern[np - 3]<<<(unsigned)(nd / 256), 256, lds, as_stream(stream)>>>(
+        nd, r2min, r3min, static_cast<const cplx<float> *>(op1), xk, k0, k1, l0, l1, line_start, kus, ustate,
+        uband, static_cast<const cplx<float> *>(U), nbw, static_cast<cplx<float> *>(A), lda);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Staged kR->kq over Rp for the q of a uniform grid (m1, m2, m3), fused with the k+q-window and phonon
 // rotations and g2: the GridOpt factorisation applied to the unit operators A_u[G, Rp]
@@ -282,9 +466,10 @@ __global__ __launch_bounds__(256) void kq_line_offsets_kernel(int nunits, const 
 constexpr int GQ_M3_MAX = 64, GQ_SLAB_MAX = 1024;
 
 // One workgroup per (q1, unit), one thread per row G < nw nm (threads past it only help stage the
-// tables). Rp box: (p1min + a, p2min + b, p3min + c), a < n1, b, c < N; rmap[a + n1 (b + N c)] = Rp index or
-// -1. Stage 1 (registers): B1[b][c] = sum_a e^{2 pi i (q1/m1)(p1min + a)} A[u][G + ldr Rp]. The kept q of
-// the (u, q1) slab are taken in chunks of whole q lines with at most GQ_CHUNK entries; per chunk, per
+// tables). Rp box: (p1min + a, p2min + b, p3min + c), a < n1, b, c < N; rmap[b + 8 (a + n1 c)] = Rp index or
+// -1 (b fastest, rows padded to 8, so the N indices of one (a, c) come in one scalar load). Stage 1 (registers): B1[b][c] = sum_a e^{2 pi i (q1/m1)(p1min + a)} A[u][G + ldr Rp]. The kept q of
+// the (u, q1) slab are taken in chunks of whole q lines with at most GQ_CHUNK entries (a line with more
+// kept q is split into pieces of GQ_CHUNK); per chunk, per
 // kept q (in line order; B2[c] = sum_b e2 B1[b][c] is recomputed when the line changes): g = sum_c e3 B2[c];
 // t[m, nu] = sum_i conj(U_kq[i, mlo + m]) g[i + 16 nu] (DPP row reduction over i); ep[m, nu'] = sum_nu
 // t[m, nu] uph(q)[nu, nu'], g2 = |ep|^2 / (2 omega(q)[nu']) at row u cs + (kept position), slots m < nbw
@@ -355,22 +540,26 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
             const float2 p = ph1[a];
             cplx<float> v[N];
 #pragma unroll
-            for (int b = 0; b < N; b++) {
-                const int ir = rmap[a + n1 * (b + N * c)];  // uniform
-                v[b] = ir >= 0 ? a_u[ldr * ir] : cplx<float>{0.f, 0.f};
+            for (int b = 0; b < N; b++) {  // branch-free (absent R: column 0 loaded, zero used)
+                const int ir = rmap[b + 8 * (a + n1 * c)];  // uniform; the N entries are contiguous
+                v[b] = a_u[ldr * max(ir, 0)];
+                if (ir < 0) v[b] = cplx<float>{0.f, 0.f};
             }
 #pragma unroll
             for (int b = 0; b < N; b++) {
                 cmac(br[c][b], bi[c][b], p.x, p.y, v[b].re, v[b].im);
             }
         }
-    int l0 = 0;
-    while (l0 < m2) {  // uniform
-        int l1 = l0 + 1;
-        while (l1 < m2 && slo[l1 + 1] - slo[l0] <= GQ_CHUNK) l1++;
-        const int ks = slo[l0] - slab0, nk = slo[l1] - slo[l0];  // chunk entries in the slab tables, <= GQ_CHUNK
-        l0 = l1;
-        if (nk <= 0) continue;
+    const int lane = t & 63;
+    int e0 = 0, lc = 0;  // chunk start in the slab tables and the line that holds it
+    while (e0 < nslab) {  // uniform
+        while (slo[lc + 1] - slab0 <= e0) lc++;
+        // the chunk: whole lines up to GQ_CHUNK entries, or the next GQ_CHUNK entries of a longer line
+        const int e1 = min(e0 + GQ_CHUNK, nslab);
+        int cut = e1;
+        for (int l = lc + 1; l <= m2 && slo[l] - slab0 <= e1; l++) cut = slo[l] - slab0;
+        const int ks = e0, nk = cut - e0;  // 0 < nk <= GQ_CHUNK
+        e0 = cut;
         __syncthreads();  // slab tables written; the previous chunk's tables no longer read
         for (int e = t; e < nk * NBW * NW; e += blockDim.x) {
             const int kk = e / (NBW * NW), m = (e / NW) % NBW, ii = e % NW;
@@ -383,10 +572,15 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
             ucol[e] = w;
         }
         __syncthreads();  // U columns staged
+        // lane kk keeps entry kk's metadata word (nk <= GQ_CHUNK < 64); the loop takes it with readlane
+        const int meta = lane < nk ? sk_nb[ks + lane] : 0;
         float b2r[N], b2i[N];
         int cur = -1;
         for (int kk = 0; kk < nk; kk++) {
-            const int w23 = sk_nb[ks + kk];
+            float2 w[NBW];  // U_kq columns of this kept q, read before the stage-2/3 sums that hide them
+#pragma unroll
+            for (int m = 0; m < NBW; m++) w[m] = ucol[(kk * NBW + m) * NW + i];
+            const int w23 = __builtin_amdgcn_readlane(meta, kk);
             const int q2 = (w23 >> 22) & 63, q3 = (w23 >> 16) & 63;
             if (q2 != cur) {  // uniform
                 cur = q2;
@@ -409,10 +603,9 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
                 cmac(gr, gi, p.x, p.y, b2r[c], b2i[c]);
             }
 #pragma unroll
-            for (int m = 0; m < NBW; m++) {
-                const float2 w = ucol[(kk * NBW + m) * NW + i];  // conj(w) * g
-                const float tr = dpp_group_sum<NW>(w.x * gr + w.y * gi);
-                const float ti = dpp_group_sum<NW>(w.x * gi - w.y * gr);
+            for (int m = 0; m < NBW; m++) {  // conj(w) * g
+                const float tr = dpp_group_sum<NW>(w[m].x * gr + w[m].y * gi);
+                const float ti = dpp_group_sum<NW>(w[m].x * gi - w[m].y * gr);
                 if (rv && i == 0) tq[(kk * NBW + m) * NW + nu] = make_float2(tr, ti);
             }
         }

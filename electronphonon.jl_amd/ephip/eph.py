@@ -370,7 +370,7 @@ class OuterKLoop:
 
     def __init__(self, st: OuterQSetup, nk_batch: int | None = 128, precision: int = L.PREC_FP32,
                  nk_batch_cap: int = 512, fuse_rotation: bool = True, fuse_projection: bool = True,
-                 gridopt: bool = True):
+                 gridopt: bool = True, gridopt_mfma: bool = True):
         torch = _torch()
         self.st = st
         if nk_batch is None:  # memory-adaptive width (plan_batch), the e-ph tensor already resident
@@ -404,7 +404,8 @@ class OuterKLoop:
         # RR->kR as the staged (GridOpt) transform with the projection fused, in FP32 for either precision:
         # k1 planes and (k1, k2) lines of the grid share their partial sums
         ks_host = st.ksurv.cpu().numpy()
-        self.gridopt = _GridOptKR.build(st, ks_host, self.nd_el) if (gridopt and nw in (8, 16)) else None
+        self.gridopt = _GridOptKR.build(st, ks_host, self.nd_el, gridopt_mfma) if (gridopt and nw in (8, 16)) \
+            else None
         if self.gridopt is not None:
             self.fused_proj = False
         # kR->kq as the staged transform over the uniform q grid with the rotations and g2 fused (FP32)
@@ -442,8 +443,9 @@ class OuterKLoop:
         self.nq = st.ph.xq.shape[0]
         self.max_kept = kept_q_bound(ks, st.kgrid, st.qgrid)
         self.cs = self.nq
-        max_units = int(max(nb[i:i + self.nk_batch].sum() for i in range(0, max(len(ks), 1), self.nk_batch))) \
-            if len(ks) else 1
+        # units of the largest run of nk_batch consecutive surviving k: a batch may start at any position
+        w = min(self.nk_batch, len(ks))
+        max_units = int((self.k_unit_start[w:] - self.k_unit_start[:len(ks) + 1 - w]).max()) if len(ks) else 1
         self.max_units = max_units
         nd2 = nw * nm
         self.G = None if (self.fused_proj or self.gridopt is not None) else torch.empty(
@@ -475,11 +477,22 @@ class OuterKLoop:
         self.ft_timer.append((a, b, label, flops))
         return r
 
-    def run_kbatch(self, i0: int, n: int, calculators=()):
+    def _batch_units(self, i0: int, n: int):
+        """(n, u0, u1) of the batch of surviving k i0 .. i0 + n - 1 (n clipped to the list); the batch
+        buffers hold nk_batch k and max_units units, so a longer batch is refused."""
+        if n > self.nk_batch or i0 < 0:
+            raise ValueError(f"k batch of {n} at {i0}: the loop's buffers hold batches of at most {self.nk_batch}")
         n = min(n, self.st.ksurv.shape[0] - i0)
         if n <= 0:
-            return None
+            return 0, 0, 0
         u0, u1 = int(self.k_unit_start[i0]), int(self.k_unit_start[i0 + n])
+        assert u1 - u0 <= self.max_units
+        return n, u0, u1
+
+    def run_kbatch(self, i0: int, n: int, calculators=()):
+        n, u0, u1 = self._batch_units(i0, n)
+        if n <= 0:
+            return None
         stream = L.stream_handle()
         self._stage_rrkr(i0, n, u0, u1, self.A, stream)
         return self._stage_krkq(i0, n, u0, u1, self.A, calculators, stream)
@@ -497,13 +510,11 @@ class OuterKLoop:
         s_rr.wait_stream(cur)
         s_kq.wait_stream(cur)
         bufs, free = (self.A, self.A2), [None, None]
-        nks = self.st.ksurv.shape[0]
         out = []
         for b, i0 in enumerate(starts):
-            nb = min(n, nks - i0)
+            nb, u0, u1 = self._batch_units(i0, n)
             if nb <= 0:
                 continue
-            u0, u1 = int(self.k_unit_start[i0]), int(self.k_unit_start[i0 + nb])
             A = bufs[b % 2]
             with torch.cuda.stream(s_rr):
                 if free[b % 2] is not None:
@@ -619,7 +630,7 @@ class OuterKLoop:
             f_rr = 8.0 * self.nd_el * self.nre * n
         if self.gridopt_q is not None:
             gq = self.gridopt_q
-            n2n3 = int((gq.rmap.view(gq.np, gq.np, gq.n1) >= 0).any(2).sum().item())
+            n2n3 = int((gq.rmap.view(gq.np, gq.n1, 8) >= 0).any(1).sum().item())  # (c, b) columns present
             f_kq = 8.0 * nd2 * (self.nrp * gq.m[0] * nu + n2n3 * gq.nlines * nu + gq.np * kept) + rot
         else:
             f_kq = 8.0 * nd2 * self.nrp * kept + (rot if self.fused else 0.0)
@@ -665,7 +676,7 @@ class _GridOptKR:
     N1_MAX, N23_MAX, KLINE_MAX, NBW_MAX = 32, 8, 64, 4
 
     @classmethod
-    def build(cls, st, ks, nd):
+    def build(cls, st, ks, nd, mfma=True):
         r = np.asarray(st.irvec_el_dev.cpu().numpy(), dtype=np.int64)
         lo, hi = r.min(0), r.max(0)
         n = hi - lo + 1
@@ -673,13 +684,16 @@ class _GridOptKR:
         if (len(ks) == 0 or n[0] > cls.N1_MAX or n[1] > cls.N23_MAX or n[2] > cls.N23_MAX or kg[2] > cls.KLINE_MAX
                 or st.nbw > cls.NBW_MAX):
             return None
-        return cls(st, ks, nd, r, lo, n, kg)
+        return cls(st, ks, nd, r, lo, n, kg, mfma)
 
-    def __init__(self, st, ks, nd, r, lo, n, kg):
+    def __init__(self, st, ks, nd, r, lo, n, kg, mfma):
         torch = _torch()
         dev = st.el.xk.device
         self.nd = nd
-        self._nb = st.nw  # op1 rows j-fastest, the order of the projection's lane reduction
+        # stages 2/3 on the matrix cores for nw = 16 (op1 rows in natural order); otherwise the VALU form,
+        # whose lane reduction wants op1 rows j-fastest
+        self.mfma = bool(mfma) and st.nw == 16
+        self._nb = 1 if self.mfma else st.nw
         self.n1, self.n2, self.n3 = (int(x) for x in n)
         self.np = max(self.n2, self.n3, 3)
         self.rmin = [int(x) for x in lo]
@@ -725,7 +739,8 @@ class _GridOptKR:
                 l_end += 1
             k_first = max(int(self.line_start_host[l]), i0)
             buf = self._plane_buffer(plane, k_first, loop.epmat_el, stream)
-            L.call("ephip_gridopt_kR_project", st.nw, st.nm, loop.nrp, self.np, self.rmin[1], self.rmin[2],
+            entry = "ephip_gridopt_kR_project_mfma" if self.mfma else "ephip_gridopt_kR_project"
+            L.call(entry, st.nw, st.nm, loop.nrp, self.np, self.rmin[1], self.rmin[2],
                    L.ptr(buf), L.ptr(loop.xk_surv), i0, i0 + n, l, l_end, L.ptr(self.line_start), self.max_line_k,
                    L.ptr(loop.k_unit_start_dev), L.ptr(loop.unit_state[u0:u1]), L.ptr(loop.unit_band[u0:u1]),
                    L.ptr(st.el.U), loop.max_units_per_k, L.ptr(A), A.stride(0), stream)
@@ -753,8 +768,8 @@ class _GridOptKQ:
         self.n1 = int(n[0])
         self.np = max(int(n[1]), int(n[2]), 3)
         self.pmin = [int(x) for x in lo]
-        rmap = -np.ones((self.np, self.np, self.n1), np.int32)
-        rmap[r[:, 2] - lo[2], r[:, 1] - lo[1], r[:, 0] - lo[0]] = np.arange(len(r), dtype=np.int32)
+        rmap = -np.ones((self.np, self.n1, 8), np.int32)  # index b + 8 (a + n1 c)
+        rmap[r[:, 2] - lo[2], r[:, 0] - lo[0], r[:, 1] - lo[1]] = np.arange(len(r), dtype=np.int32)
         self.rmap = torch.from_numpy(rmap.ravel()).to(dev)
         self.m = [int(x) for x in m]
         self.nlines = self.m[0] * self.m[1]

@@ -240,6 +240,15 @@ int ephip_gridopt_kR_project(int32_t nw, int32_t nm, int32_t nrp, int32_t np, in
                              const int32_t *line_start, int32_t max_line_k, const int32_t *kus,
                              const int64_t *ustate, const int32_t *uband, const void *U, int32_t nbw, void *A,
                              int64_t lda, void *stream);
+/* ephip_gridopt_kR_project with stages 2/3 and the projection on the matrix cores (16x16x4 FP32 MFMAs:
+ * along a line, A[(k, n)][G] = sum_{j, c} U_k[j, n] e^{2 pi i k3 r3(c)} op2[(j, c)][G]), for nw = 16; op1 in
+ * the natural row order d = i + nw*(j + nw*rest) (ephip_gridopt_stage1 with nb = 1). Same arguments and
+ * output as ephip_gridopt_kR_project. */
+int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp, int32_t np, int32_t r2min, int32_t r3min,
+                                  const void *op1, const double *xk, int64_t k0, int64_t k1, int32_t l0, int32_t l1,
+                                  const int32_t *line_start, int32_t max_line_k, const int32_t *kus,
+                                  const int64_t *ustate, const int32_t *uband, const void *U, int32_t nbw, void *A,
+                                  int64_t lda, void *stream);
 
 /* Staged (GridOpt) kR->kq over Rp for the q of a uniform grid qgrid = (m1, m2, m3) (q index
  * (q1*m2 + q2)*m3 + q3 at q = (q1/m1, q2/m2, q3/m3)), fused with the k+q-window and phonon rotations and
@@ -247,8 +256,8 @@ int ephip_gridopt_kR_project(int32_t nw, int32_t nm, int32_t nrp, int32_t np, in
  * eph_apply_rotations! + g2 sequence of src/wannier_to_bloch_batched.jl:245-273 and :442-455), computed
  * in FP32 by per-axis partial sums (src/wannier/GridOpt.jl) over the unit operators A (c64, unit u at
  * A + u*lda, element (G, Rp) at G + ldr*Rp, G = i + nw*nu). The Rp list is embedded in its bounding box
- * (p1min + a, p2min + b, p3min + c), a < n1 <= 32, b, c < np (3..8); rmap (int32, n1*np*np, index
- * a + n1*(b + np*c)) gives the Rp index or -1. line_off (int32, nunits x (m1*m2 + 1)) holds each unit's
+ * (p1min + a, p2min + b, p3min + c), a < n1 <= 32, b, c < np (3..8); rmap (int32, 8*n1*np, index
+ * b + 8*(a + n1*c), rows padded to 8) gives the Rp index or -1. line_off (int32, nunits x (m1*m2 + 1)) holds each unit's
  * first kept-list position per q line, from ephip_kq_line_offsets (lower bounds in the sorted kept
  * lists of ephip_kq_compact). nw = 16, nm <= 16, nbw <= 4, m2, m3 <= 64, m2*m3 <= 1024. */
 int ephip_kq_line_offsets(int32_t nunits, const int32_t *count, const int32_t *col_index, int64_t cs, int32_t m3,

@@ -235,21 +235,23 @@ def test_outer_k_fused_projection(backend, model16, nq, variant):
     assert rel < 1e-4, rel
 
 
-@pytest.mark.parametrize("which,nb", [("model16", 40), ("model16", 150), ("cfg1", 64), ("cfg1", 7)])
-def test_outer_k_gridopt(backend, model16, cfg1, which, nb):
-    """RR->kR as the staged (GridOpt) transform with the projection fused (FP32): the unit operators A
-    agree with the direct FP32 Fourier + projection kernels to 1e-5 and g2 with the oracle to 1e-4.
-    Batches of 7 to 150 k cut lines and k1 planes; the batches are run out of order and one twice, so
-    cached plane sums are reused and evicted."""
+@pytest.mark.parametrize("which,nb,mfma", [("model16", 40, True), ("model16", 150, True), ("model16", 40, False),
+                                          ("model16", 150, False), ("cfg1", 64, True), ("cfg1", 7, True)])
+def test_outer_k_gridopt(backend, model16, cfg1, which, nb, mfma):
+    """RR->kR as the staged (GridOpt) transform with the projection fused (FP32; for nw 16 stages 2/3 on
+    the matrix cores or on the vector ALUs): the unit operators A agree with the direct FP32 Fourier +
+    projection kernels to 1e-5 and g2 with the oracle to 1e-4. Batches of 7 to 150 k cut lines and k1
+    planes; the batches are run out of order and one twice, so cached plane sums are reused and evicted."""
     from ephip import _lib as L
     from ephip import eph
     m, g = model16 if which == "model16" else cfg1
     w = g["window"]
     st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
     nks = st.ksurv.shape[0]
-    staged = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32)
+    staged = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32, gridopt_mfma=mfma)
     direct = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32, gridopt=False)
     assert staged.gridopt is not None and direct.gridopt is None
+    assert staged.gridopt.mfma == (mfma and m.nw == 16)
     rs = eph.UnitRecorder()
     starts = [0, max(0, nks - nb), nb % nks, 0, min(2 * nb, max(0, nks - 1))]
     for i0 in starts:
@@ -290,6 +292,40 @@ def test_outer_k_gridopt_kq(backend, model16, nq, window, nb):
             den = max(den, float(np.abs(bd["g2"][u]).max(initial=0.0)))
     assert den > 0 and num / den < 1e-5, num / den
     rel = _outer_k_vs_oracle(m, g, st, rs, qstep=5 if nq == 4 else 37)
+    assert rel < 1e-4, rel
+
+
+@pytest.mark.parametrize("window", [0.5, 0.8])
+def test_outer_k_gridopt_kq_long_lines(backend, model16, window):
+    """Staged kR->kq on a q grid whose lines hold more kept q than one chunk of the kernel (32): k and q
+    grids (2, 4, 40), so the kernel splits lines; g2 agrees with the direct FP32 kernels and the oracle."""
+    from ephip import _lib as L
+    from ephip import eph
+    m, g = model16
+    g = dict(g, kgrid=(2, 4, 40), qgrid=(2, 4, 40), window=window)
+    w = g["window"]
+    st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
+    nks = st.ksurv.shape[0]
+    nb = 40
+    staged = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32)
+    direct = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32, gridopt=False)
+    assert staged.gridopt_q is not None and staged.gridopt is not None
+    rs, rd = eph.UnitRecorder(), eph.UnitRecorder()
+    for i0 in (0, max(0, nks - nb)):
+        staged.run_kbatch(i0, nb, [rs])
+        direct.run_kbatch(i0, nb, [rd])
+    longest = 0
+    num = den = 0.0
+    for bs, bd in zip(rs.batches, rd.batches):
+        for u in range(len(bs["cols"])):
+            cols = np.asarray(bs["cols"][u])
+            assert np.array_equal(cols, bd["cols"][u])
+            longest = max(longest, int(np.bincount(cols // 40).max(initial=0)))
+            num = max(num, float(np.abs(bs["g2"][u] - bd["g2"][u]).max(initial=0.0)))
+            den = max(den, float(np.abs(bd["g2"][u]).max(initial=0.0)))
+    assert longest > 32, longest  # some (unit, q1, q2) line is split across chunks
+    assert den > 0 and num / den < 1e-5, num / den
+    rel = _outer_k_vs_oracle(m, g, st, rs, qstep=37)
     assert rel < 1e-4, rel
 
 
