@@ -106,6 +106,36 @@ __device__ __forceinline__ float dpp_group_sum(float v) {
     return v;
 }
 
+// Row sums of four values at once, scattered over the 16-lane row: on return lanes 4s .. 4s + 3 of every
+// row hold the row sum of the s-th argument. A reduce-scatter: lanes split by bit 3 (partner l ^ 8, row
+// rotation by 8) and by bit 2 (partner l ^ 7, half-row mirror) keep one half of their values each, then
+// two quad permutations (partners l ^ 2, l ^ 1) finish the sums; {8, 7, 2, 1} span all 16 lanes. Eleven
+// VALU operations where four separate row sums take sixteen DPP additions and their moves.
+__device__ __forceinline__ float dpp_row_sum4(float a, float b, float c, float d) {
+    auto dpp = [](float x, int ctrl) {
+        const int xi = __builtin_bit_cast(int, x);
+        int y;
+        switch (ctrl) {
+            case 0: y = __builtin_amdgcn_update_dpp(0, xi, 0x128, 0xF, 0xF, false); break;  // row_ror:8
+            case 1: y = __builtin_amdgcn_update_dpp(0, xi, 0x141, 0xF, 0xF, false); break;  // row_half_mirror
+            case 2: y = __builtin_amdgcn_update_dpp(0, xi, 0x4E, 0xF, 0xF, false); break;   // quad_perm [2,3,0,1]
+            default: y = __builtin_amdgcn_update_dpp(0, xi, 0xB1, 0xF, 0xF, false); break;  // quad_perm [1,0,3,2]
+        }
+        return __builtin_bit_cast(float, y);
+    };
+    const int l = threadIdx.x & 15;
+    const bool h3 = (l & 8) != 0, h2 = (l & 4) != 0;
+    float x0 = h3 ? c : a, y0 = h3 ? a : c, x1 = h3 ? d : b, y1 = h3 ? b : d;
+    x0 += dpp(y0, 0);
+    x1 += dpp(y1, 0);
+    float x = h2 ? x1 : x0;
+    const float y = h2 ? x0 : x1;
+    x += dpp(y, 1);
+    x += dpp(x, 2);
+    x += dpp(x, 3);
+    return x;
+}
+
 template <int N, int NW>
 __global__ __launch_bounds__(256) void gridopt_kR_project_kernel(
     int64_t nd, int r2min, int r3min, const cplx<float> *__restrict__ op1,
@@ -602,11 +632,21 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
                 const float2 p = ph3[q3 * N + c];
                 cmac(gr, gi, p.x, p.y, b2r[c], b2i[c]);
             }
+            // conj(w) g for the NBW columns: 2 NBW values summed over the 16 lanes of the row, four at a
+            // time; lanes 4s of the row write value s of each group
+            float v[2 * NBW];
 #pragma unroll
-            for (int m = 0; m < NBW; m++) {  // conj(w) * g
-                const float tr = dpp_group_sum<NW>(w[m].x * gr + w[m].y * gi);
-                const float ti = dpp_group_sum<NW>(w[m].x * gi - w[m].y * gr);
-                if (rv && i == 0) tq[(kk * NBW + m) * NW + nu] = make_float2(tr, ti);
+            for (int m = 0; m < NBW; m++) {
+                v[2 * m] = w[m].x * gr + w[m].y * gi;
+                v[2 * m + 1] = w[m].x * gi - w[m].y * gr;
+            }
+            float *tqf = reinterpret_cast<float *>(tq);
+#pragma unroll
+            for (int g0 = 0; g0 < 2 * NBW; g0 += 4) {
+                const float x = dpp_row_sum4(v[g0], v[g0 + 1], g0 + 2 < 2 * NBW ? v[g0 + 2] : 0.f,
+                                             g0 + 3 < 2 * NBW ? v[g0 + 3] : 0.f);
+                const int vi = g0 + (i >> 2);  // value index: column vi / 2, real (0) or imaginary (1)
+                if (rv && (i & 3) == 0 && vi < 2 * NBW) tqf[((kk * NBW + (vi >> 1)) * NW + nu) * 2 + (vi & 1)] = x;
             }
         }
         __syncthreads();  // t of every kept q of the chunk
