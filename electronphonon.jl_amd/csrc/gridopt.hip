@@ -245,17 +245,17 @@ __global__ __launch_bounds__(256) void gridopt_kR_project_kernel(
 // accumulators) over its four j, and the four waves' partial tiles are summed through LDS.
 template <int N>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 : 2))) void
-gridopt_kR_project_mfma_kernel(
-    int64_t nd, int r2min, int r3min, const cplx<float> *__restrict__ op1,
-    const double *__restrict__ xk, int64_t k0, int64_t k1, int l0, int l1, const int *__restrict__ line_start,
-    const int *__restrict__ kus, const int64_t *__restrict__ ustate, const int *__restrict__ uband,
-    const cplx<float> *__restrict__ U, int nbw, cplx<float> *__restrict__ A, int64_t lda) {
+gridopt_kR_project_mfma_kernel(int64_t nd, int r2min, int r3min, const cplx<float> *__restrict__ op1,
+                               const double *__restrict__ xk, int64_t k0, int64_t k1, int l0, int l1,
+                               const int *__restrict__ line_start, const int *__restrict__ kus,
+                               const int64_t *__restrict__ ustate, const int *__restrict__ uband,
+                               const cplx<float> *__restrict__ U, int nbw, cplx<float> *__restrict__ A,
+                               int64_t lda) {
     constexpr int NW = 16, NP = GO_KLINE_MAX * GO_NBW_MAX;
     __shared__ float2 ph2[N];
     __shared__ float2 ph3[GO_KLINE_MAX * N];
-    __shared__ int kub[GO_KLINE_MAX], knb[GO_KLINE_MAX];
-    extern __shared__ float2 uc[];                      // [kk][m][j], kk < max_line_k, m < nbw (dynamic)
-    __shared__ int prow[NP], punit[NP];                 // pair -> kk * nbw + m, unit
+    extern __shared__ float2 uc[];       // [pair][j]: U_k[j, n] of the line's pairs (dynamic, max_line_k nbw)
+    __shared__ int pk[NP], punit[NP];    // pair -> its k on the line, its unit (relative to kus[k0])
     __shared__ int npair;
     __shared__ f32x4 red[2][2][4][64];  // the waves' partial tiles (real, imaginary), two buffers
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -289,32 +289,29 @@ gridopt_kR_project_mfma_kernel(
             cis2pi(xk[3 * (kb + kk) + 2] * (double)(r3min + c3), c, s);
             ph3[kk * N + c3] = make_float2(c, s);
         }
-        if (t < nkl) {
-            kub[t] = kus[kb + t] - ku0;
-            knb[t] = min(kus[kb + t + 1] - kus[kb + t], nbw);
-        }
-        __syncthreads();
-        for (int e = t; e < nkl * nbw * NW; e += blockDim.x) {
-            const int kk = e / (nbw * NW), m = (e / NW) % nbw, jj = e % NW;
-            float2 w = make_float2(0.f, 0.f);
-            if (m < knb[kk]) {
-                const int ub = kub[kk];
-                const cplx<float> v = U[(int64_t)NW * NW * ustate[ub] + jj + NW * uband[ub + m]];
-                w = make_float2(v.re, v.im);
-            }
-            uc[(kk * nbw + m) * NW + jj] = w;
-        }
         if (t < 64) {  // the line's (k, n) pairs in k order: exclusive prefix of the band counts (<= 4)
-            const int nbk = t < nkl ? knb[t] : 0, ub = t < nkl ? kub[t] : 0;
+            int nbk = 0, ub = 0;
+            if (t < nkl) {
+                ub = kus[kb + t];
+                nbk = min(kus[kb + t + 1] - ub, nbw);
+                ub -= ku0;
+            }
             const unsigned long long below = (1ull << t) - 1ull;
             int start = 0;
 #pragma unroll
             for (int bit = 0; bit < 3; bit++) start += __popcll(__ballot((nbk >> bit) & 1) & below) << bit;
             for (int m = 0; m < nbk; m++) {
-                prow[start + m] = t * nbw + m;
+                pk[start + m] = t;
                 punit[start + m] = ub + m;
             }
             if (t == 63) npair = start + nbk;
+        }
+        __syncthreads();  // phase tables and pairs
+        const int np = npair;
+        for (int e = t; e < np * NW; e += blockDim.x) {  // U columns of the pairs
+            const int u = punit[e >> 4];
+            const cplx<float> v = U[(int64_t)NW * NW * ustate[u] + (e & 15) + NW * uband[u]];
+            uc[e] = make_float2(v.re, v.im);
         }
         float o2r[N], o2i[N];
 #pragma unroll
@@ -328,17 +325,15 @@ gridopt_kR_project_mfma_kernel(
             o2r[c] = sr;
             o2i[c] = si;
         }
-        __syncthreads();  // U columns, pairs staged
-        const int np = npair;
+        __syncthreads();  // U columns staged
         for (int p0 = 0; p0 < np; p0 += 16) {
             // A operand: row pair p0 + lane % 16 (zero past the line's pairs), column j
             const int p = p0 + (lane & 15);
             int kk = 0;
             float ur = 0.f, ui = 0.f;
             if (p < np) {
-                const int rw = prow[p];
-                kk = rw / nbw;
-                const float2 w = uc[rw * NW + j];
+                kk = pk[p];
+                const float2 w = uc[p * NW + j];
                 ur = w.x;
                 ui = w.y;
             }
@@ -454,7 +449,7 @@ extern "C" int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp
     static const K kern[6] = {gridopt_kR_project_mfma_kernel<3>, gridopt_kR_project_mfma_kernel<4>,
                               gridopt_kR_project_mfma_kernel<5>, gridopt_kR_project_mfma_kernel<6>,
                               gridopt_kR_project_mfma_kernel<7>, gridopt_kR_project_mfma_kernel<8>};
-    const size_t lds = (size_t)max(max_line_k, 1) * nbw * nw * sizeof(float2);  // U columns of a line
+    const size_t lds = (size_t)max(max_line_k, 1) * nbw * nw * sizeof(float2);  // U columns of a line's pairs
     kern[np - 3]<<<(unsigned)(nd / 256), 256, lds, as_stream(stream)>>>(
         nd, r2min, r3min, static_cast<const cplx<float> *>(op1), xk, k0, k1, l0, l1, line_start, kus, ustate,
         uband, static_cast<const cplx<float> *>(U), nbw, static_cast<cplx<float> *>(A), lda);
@@ -650,19 +645,29 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
             }
         }
         __syncthreads();  // t of every kept q of the chunk
-        for (int e = t; e < nk * nbw * nm; e += blockDim.x) {
-            const int kk = e / (nbw * nm), rr = e % (nbw * nm), m = rr / nm, nu2 = rr % nm;
-            const int iq = sk_iq[ks + kk];
-            const cplx<float> *up = uph + (int64_t)nm * nm * iq + nm * nu2;
-            float er = 0.f, ei = 0.f;
-            if constexpr (NMC > 0) {
+        if constexpr (NMC > 0) {
+            // entries (kk, slot m < NBW, mode): compile-time index arithmetic; slots m >= nbw skipped
+            for (int e = t; e < nk * NBW * NMC; e += blockDim.x) {
+                const int kk = e / (NBW * NMC), m = (e / NMC) % NBW, nu2 = e % NMC;
+                if (m >= nbw) continue;
+                const int iq = sk_iq[ks + kk];
+                const cplx<float> *up = uph + (int64_t)NMC * NMC * iq + NMC * nu2;
+                float er = 0.f, ei = 0.f;
 #pragma unroll
                 for (int n = 0; n < NMC; n++) {  // constant offsets from one base address
                     const float2 tv = tq[(kk * NBW + m) * NW + n];
                     const cplx<float> p = up[n];
                     cmac(er, ei, tv.x, tv.y, p.re, p.im);
                 }
-            } else {
+                const float w2 = 2.f * static_cast<float>(omega[(int64_t)NMC * iq + nu2]);
+                g2[(row0 + slab0 + ks + kk) * nbw * NMC + m * NMC + nu2] = (er * er + ei * ei) / w2;
+            }
+        } else {
+            for (int e = t; e < nk * nbw * nm; e += blockDim.x) {
+                const int kk = e / (nbw * nm), rr = e % (nbw * nm), m = rr / nm, nu2 = rr % nm;
+                const int iq = sk_iq[ks + kk];
+                const cplx<float> *up = uph + (int64_t)nm * nm * iq + nm * nu2;
+                float er = 0.f, ei = 0.f;
 #pragma unroll
                 for (int n = 0; n < NW; n++) {  // nm <= NW: fixed trip count, loads issued together
                     const int nn = min(n, nm - 1);
@@ -671,9 +676,9 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
                     const cplx<float> p = up[nn];
                     cmac(er, ei, on * tv.x, on * tv.y, p.re, p.im);
                 }
+                const float w2 = 2.f * static_cast<float>(omega[(int64_t)nm * iq + nu2]);
+                g2[(row0 + slab0 + ks + kk) * nbw * nm + rr] = (er * er + ei * ei) / w2;
             }
-            const float w2 = 2.f * static_cast<float>(omega[(int64_t)nm * iq + nu2]);
-            g2[(row0 + slab0 + ks + kk) * nbw * nm + rr] = (er * er + ei * ei) / w2;
         }
     }
 }
