@@ -37,20 +37,19 @@ METRIC = ("g(k,q) k·q pairs/s at |g|²≤1e-4 rel err vs FP64; plain vs "
 PEAK = {"fp32": 157.3, "bf16x3": 2500.0}  # TFLOP/s, MI355X dense MFMA (fp32, bf16)
 # HBM bytes per launch of the two Fourier kernels of the fast path on the cfg2 workload, keyed by
 # (precision, stage, k batch, kernel variant), from rocprofv3 FETCH_SIZE (x2.00) and WRITE_SIZE (x1.00)
-# passes calibrated on a 256 MiB copy (profiles/cfg2_fast_bf16_hbm_counters.md). Counters cannot be read
+# passes calibrated on a 256 MiB copy (profiles/cfg2_fast_gridopt_hbm_counters.md,
+# profiles/cfg2_fast_bf16_mfma_hbm_counters.md). Counters cannot be read
 # inside the timed run; a variant or batch without a measurement reports traffic null.
 TRAFFIC_BYTES_CFG2 = {
-    ("fp32", "kR->kq", 768, "gridopt"): (2180012 * 2 + 414596) * 1024,   # staged kR->kq + rotations + g2
-    ("fp32", "RR->kR", 768, "gridopt"): (203747 * 2 + 237134) * 1024,    # staged RR->kR projection launch
-    ("fp32", "kR->kq", 384, "gridopt"): (1081120 * 2 + 205708) * 1024,
-    ("fp32", "RR->kR", 384, "gridopt"): (203556 * 2 + 202199) * 1024,
+    ("fp32", "kR->kq", 768, "gridopt"): (2177998 * 2 + 414549) * 1024,   # staged kR->kq + rotations + g2
+    ("fp32", "RR->kR", 768, "gridopt"): (204803 * 2 + 238507) * 1024,    # staged RR->kR projection launch (MFMA)
     ("bf16x3", "kR->kq", 384, 10): (2998082 * 2 + 204444) * 1024,   # fused kR->kq + rotations, 8 waves
     ("bf16x3", "RR->kR", 384, "proj0"): (1827474 * 2 + 200604) * 1024,  # RR->kR fused with the projection
 }
 # kernel names by (stage, variant) for the split-BF16 fast path (tuner kinds 3 and 4)
 KERNEL_NAMES = {
-    ("RR->kR", "gridopt"): "gridopt_stage1_kernel + gridopt_kR_project_kernel: staged (GridOpt) RR->kR over Re with "
-                           "the k-side window projection, FP32 VALU",
+    ("RR->kR", "gridopt"): "gridopt_stage1_kernel + gridopt_kR_project_mfma_kernel: staged (GridOpt) RR->kR over "
+                           "Re, stages 2/3 with the k-side window projection on FP32 MFMAs",
     ("kR->kq", "gridopt"): "gridopt_kq_rotate_kernel: staged (GridOpt) kR->kq over the q grid with the k+q/phonon "
                            "rotations and g2, FP32 VALU",
     ("RR->kR", "proj0"): "ft_bf16p8_proj_kernel: eight-wave 192x128 packed split-BF16 RR->kR Fourier over Re "
@@ -433,8 +432,9 @@ def main():
                    "plain_outer_q_fp32_pairs_per_s": results.get("plain", {}).get("value")},
         "roofline": {"bound": "mfma", "achieved": dom[1], "peak": peak, "unit": "TFLOP/s",
                      "frac": dom[1] / peak,
-                     "compute": ("FP32 VALU (no MFMA; MI355X's FP32 peak is the same 157.3 TFLOP/s for vector and "
-                                 "matrix)" if "fast" in results and var[dom[4]] == "gridopt" else
+                     "compute": ("FP32 VALU, no MFMA: 157.3 TFLOP/s is MI355X's published FP32 peak, which counts "
+                                 "packed FP32 (not used here); the unpacked vector peak is 78.6 TFLOP/s"
+                                 if "fast" in results and var[dom[4]] == "gridopt" else
                                  "MFMA (split-BF16 flops counted once)" if "fast" in results and not dom_fp32
                                  else "MFMA FP32"),
                      "traffic": (TRAFFIC_BYTES_CFG2.get(("fp32" if var[dom[4]] == "gridopt" else args.precision,
