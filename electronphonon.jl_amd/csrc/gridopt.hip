@@ -29,18 +29,27 @@ __device__ __forceinline__ void cmac(float &sr, float &si, float ar, float ai, f
 constexpr int GO_N1_MAX = 32;
 
 // Stage 1. One thread per operand row d; op1 rows are written in the j-fastest order
-// d' = j + nb (i + nb rest) of d = i + nb (j + nb rest) when nb > 1 (the order the projection reduces
-// over), in order d when nb <= 1. The (b, c) face is written padded to np x np (zeros outside n2 x n3),
-// so the next stage runs on a fixed square face.
+// d' = j + nb (i + nb rest) of d = i + nb (j + nb rest) when nb > 1 (the order the VALU projection
+// reduces over), in order d when nb <= 1. The (b, c) face is written padded to np x np (zeros outside
+// n2 x n3), so the next stage runs on a fixed square face. P planes (k1 values x1[p], outputs op1[p])
+// come from one pass over the operand: each operand element loaded feeds P sums.
+constexpr int GO_PLANES_MAX = 8;
+struct GoPlanes {
+    double x1[GO_PLANES_MAX];
+    cplx<float> *op1[GO_PLANES_MAX];
+};
+
+template <int P>
 __global__ __launch_bounds__(256) void gridopt_stage1_kernel(int64_t nd, int n1, int n2, int n3, int np, int r1min,
                                                              const int *__restrict__ rmap,
                                                              const cplx<float> *__restrict__ op, int64_t ldop,
-                                                             double x1, int nb, cplx<float> *__restrict__ op1) {
-    __shared__ float2 ph[GO_N1_MAX];  // zero past n1 (the tail of the last block of 8 reads it)
-    if ((int)threadIdx.x < GO_N1_MAX) {
+                                                             GoPlanes pl, int nb) {
+    __shared__ float2 ph[P][GO_N1_MAX];  // zero past n1 (the tail of the last block of 8 reads it)
+    for (int e = threadIdx.x; e < P * GO_N1_MAX; e += blockDim.x) {
+        const int p = e / GO_N1_MAX, a = e % GO_N1_MAX;
         float c = 0.f, s = 0.f;
-        if ((int)threadIdx.x < n1) cis2pi(x1 * (double)(r1min + (int)threadIdx.x), c, s);
-        ph[threadIdx.x] = make_float2(c, s);
+        if (a < n1) cis2pi(pl.x1[p] * (double)(r1min + a), c, s);
+        ph[p][a] = make_float2(c, s);
     }
     __syncthreads();
     const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -53,7 +62,9 @@ __global__ __launch_bounds__(256) void gridopt_stage1_kernel(int64_t nd, int n1,
     const cplx<float> *o = op + d;
     for (int c = 0; c < np; c++)
         for (int b = 0; b < np; b++) {
-            float sr = 0.f, si = 0.f;
+            float sr[P], si[P];
+#pragma unroll
+            for (int p = 0; p < P; p++) sr[p] = si[p] = 0.f;
             if (b < n2 && c < n3) {
                 const int bc = b + n2 * c;
                 // r1 in blocks of 8, branch-free, so a block's index and operand loads issue together: an
@@ -65,12 +76,16 @@ __global__ __launch_bounds__(256) void gridopt_stage1_kernel(int64_t nd, int n1,
                         const int ir = a < n1 ? rmap[a + n1 * bc] : -1;  // uniform across the block
                         cplx<float> v = o[ldop * max(ir, 0)];
                         if (ir < 0) v = cplx<float>{0.f, 0.f};
-                        const float2 p = ph[min(a, GO_N1_MAX - 1)];
-                        cmac(sr, si, p.x, p.y, v.re, v.im);
+#pragma unroll
+                        for (int p = 0; p < P; p++) {
+                            const float2 e = ph[p][min(a, GO_N1_MAX - 1)];
+                            cmac(sr[p], si[p], e.x, e.y, v.re, v.im);
+                        }
                     }
                 }
             }
-            op1[dd + nd * (b + np * c)] = cplx<float>{sr, si};
+#pragma unroll
+            for (int p = 0; p < P; p++) pl.op1[p][dd + nd * (b + np * c)] = cplx<float>{sr[p], si[p]};
         }
 }
 
@@ -373,20 +388,38 @@ gridopt_kR_project_mfma_kernel(int64_t nd, int r2min, int r3min, const cplx<floa
 
 using namespace ephip;
 
-extern "C" int ephip_gridopt_stage1(int64_t nd, int32_t n1, int32_t n2, int32_t n3, int32_t np, int32_t r1min,
-                                    const int32_t *rmap, const void *op, int64_t ldop, double x1, int32_t nb,
-                                    void *op1, void *stream) {
+extern "C" int ephip_gridopt_stage1_planes(int64_t nd, int32_t n1, int32_t n2, int32_t n3, int32_t np,
+                                           int32_t r1min, const int32_t *rmap, const void *op, int64_t ldop,
+                                           int32_t nplanes, const double *x1, int32_t nb, void *const *op1,
+                                           void *stream) {
     EPHIP_REQUIRE(nd >= 0 && n1 >= 1 && n2 >= 1 && n3 >= 1 && n1 <= GO_N1_MAX && ldop >= nd && np >= n2 && np >= n3,
                   "gridopt_stage1: bad sizes (nd=%lld n1=%d n2=%d n3=%d np=%d, n1 <= %d)", (long long)nd, n1, n2, n3,
                   np, GO_N1_MAX);
+    EPHIP_REQUIRE(nplanes >= 1 && nplanes <= GO_PLANES_MAX, "gridopt_stage1: %d planes (1..%d)", nplanes,
+                  GO_PLANES_MAX);
     EPHIP_REQUIRE(nb <= 1 || nd % ((int64_t)nb * nb) == 0, "gridopt_stage1: nd not a multiple of nb^2");
     if (nd == 0) return EPHIP_OK;
-    EPHIP_REQUIRE(rmap && op && op1, "gridopt_stage1: null device pointer");
-    gridopt_stage1_kernel<<<(unsigned)((nd + 255) / 256), 256, 0, as_stream(stream)>>>(
-        nd, n1, n2, n3, np, r1min, rmap, static_cast<const cplx<float> *>(op), ldop, x1, nb,
-        static_cast<cplx<float> *>(op1));
+    EPHIP_REQUIRE(rmap && op && x1 && op1, "gridopt_stage1: null pointer");
+    GoPlanes pl{};
+    for (int p = 0; p < nplanes; p++) {
+        EPHIP_REQUIRE(op1[p], "gridopt_stage1: null output for plane %d", p);
+        pl.x1[p] = x1[p];
+        pl.op1[p] = static_cast<cplx<float> *>(op1[p]);
+    }
+    using K = decltype(&gridopt_stage1_kernel<1>);
+    static const K kern[GO_PLANES_MAX] = {gridopt_stage1_kernel<1>, gridopt_stage1_kernel<2>, gridopt_stage1_kernel<3>,
+                                          gridopt_stage1_kernel<4>, gridopt_stage1_kernel<5>, gridopt_stage1_kernel<6>,
+                                          gridopt_stage1_kernel<7>, gridopt_stage1_kernel<8>};
+    kern[nplanes - 1]<<<(unsigned)((nd + 255) / 256), 256, 0, as_stream(stream)>>>(
+        nd, n1, n2, n3, np, r1min, rmap, static_cast<const cplx<float> *>(op), ldop, pl, nb);
     EPHIP_LAUNCHED();
     return EPHIP_OK;
+}
+
+extern "C" int ephip_gridopt_stage1(int64_t nd, int32_t n1, int32_t n2, int32_t n3, int32_t np, int32_t r1min,
+                                    const int32_t *rmap, const void *op, int64_t ldop, double x1, int32_t nb,
+                                    void *op1, void *stream) {
+    return ephip_gridopt_stage1_planes(nd, n1, n2, n3, np, r1min, rmap, op, ldop, 1, &x1, nb, &op1, stream);
 }
 
 extern "C" int ephip_gridopt_kR_project(int32_t nw, int32_t nm, int32_t nrp, int32_t np, int32_t r2min,

@@ -57,6 +57,7 @@ SIGNATURES = {
     "ephip_ft_pack_operand": [_i64, _i32, _vp, _i64, _i32, _i64, _vp, _vp],
     "ephip_ft_pack_operand_swapped": [_i64, _i32, _i32, _vp, _i64, _vp, _vp],
     "ephip_gridopt_stage1": [_i64, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i64, C.c_double, _i32, _vp, _vp],
+    "ephip_gridopt_stage1_planes": [_i64, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i64, _i32, _vp, _i32, _vp, _vp],
     "ephip_gridopt_kR_project": [_i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i64, _i64, _i32, _i32, _vp,
                                  _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i64, _vp],
     "ephip_gridopt_kR_project_mfma": [_i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i64, _i64, _i32, _i32, _vp,

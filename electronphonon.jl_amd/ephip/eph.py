@@ -668,12 +668,14 @@ class OuterKLoop:
 
 
 class _GridOptKR:
-    """Staged RR->kR over Re (ephip_gridopt_stage1 + ephip_gridopt_kR_project), the device GridOpt of
-    src/wannier/GridOpt.jl: Re embedded in its bounding box, the surviving k (grid order, third index
-    fastest) grouped into k1 planes and (k1, k2) lines. Stage 1 results are kept for the two most
-    recent planes, so a pass over the k batches in order computes each plane once."""
+    """Staged RR->kR over Re (ephip_gridopt_stage1_planes + ephip_gridopt_kR_project[_mfma]), the device
+    GridOpt of src/wannier/GridOpt.jl: Re embedded in its bounding box, the surviving k (grid order,
+    third index fastest) grouped into k1 planes and (k1, k2) lines. Stage 1 runs for a group of PLANES
+    consecutive planes at once (one pass over the operand for all of them); the two most recent groups
+    are kept, so a pass over the k batches in order computes each plane once."""
 
     N1_MAX, N23_MAX, KLINE_MAX, NBW_MAX = 32, 8, 64, 4
+    PLANES = 8
 
     @classmethod
     def build(cls, st, ks, nd, mfma=True):
@@ -708,25 +710,35 @@ class _GridOptKR:
         self.line_plane = i1[starts[:-1]]
         self.line_of_k = np.repeat(np.arange(len(starts) - 1), np.diff(starts))
         self.max_line_k = int(np.diff(starts).max())
-        self.x1 = st.el.xk.index_select(0, st.ksurv).cpu().numpy()[:, 0]
-        self.cache = []  # [plane, buffer] pairs, most recent last
+        x1 = st.el.xk.index_select(0, st.ksurv).cpu().numpy()[:, 0]
+        # planes with surviving k in order, their k1 coordinates, and the stage-1 group of each
+        first_line = np.flatnonzero(np.concatenate([[True], np.diff(self.line_plane) != 0]))
+        self.planes = [int(p) for p in self.line_plane[first_line]]
+        self.plane_x1 = {p: float(x1[starts[l]]) for p, l in zip(self.planes, first_line)}
+        self.plane_pos = {p: i for i, p in enumerate(self.planes)}
+        self.cache = []  # [group, {plane: buffer}, buffers], most recent last, at most two
 
-    def _plane_buffer(self, plane, k_first, epmat_el, stream):
+    def _plane_buffer(self, plane, epmat_el, stream):
+        import ctypes as C
         torch = _torch()
         for entry in self.cache:
-            if entry[0] == plane:
-                return entry[1]
+            if plane in entry[1]:
+                return entry[1][plane]
+        g = self.plane_pos[plane] // self.PLANES
+        members = self.planes[g * self.PLANES:(g + 1) * self.PLANES]
         if len(self.cache) < 2:
-            buf = torch.empty((self.np * self.np, self.nd), dtype=torch.complex64, device=epmat_el.device)
-            self.cache.append([plane, buf])
-        else:
-            entry = self.cache.pop(0)
-            entry[0] = plane
-            buf = entry[1]
-            self.cache.append(entry)
-        L.call("ephip_gridopt_stage1", self.nd, self.n1, self.n2, self.n3, self.np, self.rmin[0], L.ptr(self.rmap),
-               L.ptr(epmat_el), epmat_el.stride(0), float(self.x1[k_first]), self._nb, L.ptr(buf), stream)
-        return buf
+            bufs = [torch.empty((self.np * self.np, self.nd), dtype=torch.complex64, device=epmat_el.device)
+                    for _ in range(self.PLANES)]
+        else:  # reuse the older group's buffers (stream order: its projections are already queued)
+            bufs = self.cache.pop(0)[2]
+        self.cache.append([g, dict(zip(members, bufs)), bufs])
+        n = len(members)
+        x1 = (C.c_double * n)(*[self.plane_x1[p] for p in members])
+        out = (C.c_void_p * n)(*[L.ptr(b) for b in bufs[:n]])
+        L.call("ephip_gridopt_stage1_planes", self.nd, self.n1, self.n2, self.n3, self.np, self.rmin[0],
+               L.ptr(self.rmap), L.ptr(epmat_el), epmat_el.stride(0), n, C.addressof(x1), self._nb,
+               C.addressof(out), stream)
+        return self.cache[-1][1][plane]
 
     def run(self, loop, i0, n, u0, u1, A, stream):
         st = loop.st
@@ -737,8 +749,7 @@ class _GridOptKR:
             l_end = l
             while l_end < l_last and self.line_plane[l_end] == plane:
                 l_end += 1
-            k_first = max(int(self.line_start_host[l]), i0)
-            buf = self._plane_buffer(plane, k_first, loop.epmat_el, stream)
+            buf = self._plane_buffer(int(plane), loop.epmat_el, stream)
             entry = "ephip_gridopt_kR_project_mfma" if self.mfma else "ephip_gridopt_kR_project"
             L.call(entry, st.nw, st.nm, loop.nrp, self.np, self.rmin[1], self.rmin[2],
                    L.ptr(buf), L.ptr(loop.xk_surv), i0, i0 + n, l, l_end, L.ptr(self.line_start), self.max_line_k,

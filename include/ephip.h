@@ -235,6 +235,12 @@ int ephip_eph_RR_to_kR_project_packed_bf16x3(int32_t nw, int32_t nm, int32_t nrp
 int ephip_gridopt_stage1(int64_t nd, int32_t n1, int32_t n2, int32_t n3, int32_t np, int32_t r1min,
                          const int32_t *rmap, const void *op, int64_t ldop, double x1, int32_t nb, void *op1,
                          void *stream);
+/* ephip_gridopt_stage1 for nplanes <= 8 k1 planes in one pass over op (each operand element feeds every
+ * plane): x1 (host array, nplanes) holds their k1 coordinates and op1 (host array of nplanes device
+ * pointers) their outputs, each laid out as for ephip_gridopt_stage1. */
+int ephip_gridopt_stage1_planes(int64_t nd, int32_t n1, int32_t n2, int32_t n3, int32_t np, int32_t r1min,
+                                const int32_t *rmap, const void *op, int64_t ldop, int32_t nplanes, const double *x1,
+                                int32_t nb, void *const *op1, void *stream);
 int ephip_gridopt_kR_project(int32_t nw, int32_t nm, int32_t nrp, int32_t np, int32_t r2min, int32_t r3min,
                              const void *op1, const double *xk, int64_t k0, int64_t k1, int32_t l0, int32_t l1,
                              const int32_t *line_start, int32_t max_line_k, const int32_t *kus,
