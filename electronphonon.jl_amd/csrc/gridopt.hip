@@ -60,7 +60,9 @@ __global__ __launch_bounds__(256) void gridopt_stage1_kernel(int64_t nd, int n1,
         dd = j + nb * (i + nb * rest);
     }
     const cplx<float> *o = op + d;
+#pragma unroll 1
     for (int c = 0; c < np; c++)
+#pragma unroll 1
         for (int b = 0; b < np; b++) {
             float sr[P], si[P];
 #pragma unroll
@@ -69,6 +71,7 @@ __global__ __launch_bounds__(256) void gridopt_stage1_kernel(int64_t nd, int n1,
                 const int bc = b + n2 * c;
                 // r1 in blocks of 8, branch-free, so a block's index and operand loads issue together: an
                 // absent R (rmap -1) and the block's tail past n1 load column 0 and contribute zero
+#pragma unroll 1
                 for (int a0 = 0; a0 < n1; a0 += 8) {
 #pragma unroll
                     for (int da = 0; da < 8; da++) {
@@ -250,28 +253,63 @@ __global__ __launch_bounds__(256) void gridopt_kR_project_kernel(
     }
 }
 
+// Per-batch tables of the MFMA form: the U column of every unit of the batch (ucol[u nw + j] =
+// U[ustate[u]][j, uband[u]]), the stage-3 phases of every k of the batch (ph3[(k - k0) np + c] =
+// e^{2 pi i k3 (r3min + c)}) and the stage-2 phases of every line (ph2[(l - l0) np + b] =
+// e^{2 pi i k2 (r2min + b)}, k2 from the line's first k in the batch). One thread per entry.
+__global__ __launch_bounds__(256) void gridopt_kR_tables_kernel(int nw, int np, int r2min, int r3min,
+                                                                const double *__restrict__ xk, int64_t k0, int64_t k1,
+                                                                int l0, int l1, const int *__restrict__ line_start,
+                                                                int nunits, const int64_t *__restrict__ ustate,
+                                                                const int *__restrict__ uband,
+                                                                const cplx<float> *__restrict__ U,
+                                                                cplx<float> *__restrict__ ucol,
+                                                                cplx<float> *__restrict__ ph2,
+                                                                cplx<float> *__restrict__ ph3) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n_u = (int64_t)nunits * nw, n_3 = (k1 - k0) * np, n_2 = (int64_t)(l1 - l0) * np;
+    if (e < n_u) {
+        const int64_t u = e / nw, j = e % nw;
+        ucol[e] = U[(int64_t)nw * nw * ustate[u] + j + (int64_t)nw * uband[u]];
+    } else if (e < n_u + n_3) {
+        const int64_t f = e - n_u, k = k0 + f / np;
+        const int c = (int)(f % np);
+        float cr, ci;
+        cis2pi(xk[3 * k + 2] * (double)(r3min + c), cr, ci);
+        ph3[f] = cplx<float>{cr, ci};
+    } else if (e < n_u + n_3 + n_2) {
+        const int64_t f = e - n_u - n_3;
+        const int l = l0 + (int)(f / np), b = (int)(f % np);
+        const int64_t k = max((int64_t)line_start[l], k0);
+        float cr, ci;
+        cis2pi(xk[3 * k + 1] * (double)(r2min + b), cr, ci);
+        ph2[f] = cplx<float>{cr, ci};
+    }
+}
+
 // Stages 2 and 3 with the window projection on the matrix cores (nw = 16). Along a line the projected
 // operators are one complex product with a k-independent right factor:
 //   A[(k, n)][G] = sum_{j, c} W[(k, n)][(j, c)] op2[(j, c)][G],   W = U_k[j, n] e^{2 pi i k3 (r3min + c)}.
 // A workgroup holds one block of 16 G (i = lane % 16) and all 16 j (four per wave, j = thread / 16): op1
 // in its natural row order d = i + 16 (j + 16 rest), so each lane's op2[c] is its own B operand of the
 // 16x16x4 FP32 MFMA (B[j][i]) and W is built in the A operand (A[pair][j]). The pairs (k, n) of a line
-// are taken 16 at a time; per c each wave runs four MFMAs (a complex product, four independent
-// accumulators) over its four j, and the four waves' partial tiles are summed through LDS.
+// (its units, consecutive) are taken 16 at a time; per c each wave runs four MFMAs (a complex product,
+// four independent accumulators) over its four j, and the four waves' partial tiles are summed through
+// LDS. The line's phases and U columns come from the batch tables (gridopt_kR_tables_kernel) into LDS
+// buffers that alternate between lines, so one barrier per line separates a line's staging from the
+// previous line's reads; partial tiles alternate likewise, one barrier per tile.
 template <int N>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 : 2))) void
-gridopt_kR_project_mfma_kernel(int64_t nd, int r2min, int r3min, const cplx<float> *__restrict__ op1,
-                               const double *__restrict__ xk, int64_t k0, int64_t k1, int l0, int l1,
-                               const int *__restrict__ line_start, const int *__restrict__ kus,
-                               const int64_t *__restrict__ ustate, const int *__restrict__ uband,
-                               const cplx<float> *__restrict__ U, int nbw, cplx<float> *__restrict__ A,
-                               int64_t lda) {
+gridopt_kR_project_mfma_kernel(int64_t nd, const cplx<float> *__restrict__ op1, int64_t k0, int64_t k1, int l0,
+                               int l1, int lb0, const int *__restrict__ line_start, int max_pairs,
+                               const int *__restrict__ kus, const cplx<float> *__restrict__ ucol,
+                               const cplx<float> *__restrict__ ph2g, const cplx<float> *__restrict__ ph3g, int nbw,
+                               cplx<float> *__restrict__ A, int64_t lda) {
     constexpr int NW = 16, NP = GO_KLINE_MAX * GO_NBW_MAX;
-    __shared__ float2 ph2[N];
-    __shared__ float2 ph3[GO_KLINE_MAX * N];
-    extern __shared__ float2 uc[];       // [pair][j]: U_k[j, n] of the line's pairs (dynamic, max_line_k nbw)
-    __shared__ int pk[NP], punit[NP];    // pair -> its k on the line, its unit (relative to kus[k0])
-    __shared__ int npair;
+    __shared__ float2 ph2[2][N];
+    __shared__ float2 ph3[2][GO_KLINE_MAX * N];
+    extern __shared__ float2 uc[];    // two buffers of max_pairs x 16: U_k[j, n] of the line's pairs
+    __shared__ int pk[2][NP];         // pair -> its k on the line
     __shared__ f32x4 red[2][2][4][64];  // the waves' partial tiles (real, imaginary), two buffers
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int j = t >> 4;  // = 4 wave + lane / 16
@@ -288,83 +326,69 @@ gridopt_kR_project_mfma_kernel(int64_t nd, int r2min, int r3min, const cplx<floa
             vi[c][b] = v.im;
         }
     const int ku0 = kus[k0];
+    int lb = 0;  // line buffer
     for (int l = l0; l < l1; l++) {
         const int64_t kb = max((int64_t)line_start[l], k0), ke = min((int64_t)line_start[l + 1], k1);
         const int nkl = (int)(ke - kb);  // uniform; at most GO_KLINE_MAX (host check)
         if (nkl <= 0) continue;
-        __syncthreads();  // the previous line's tables are no longer read
+        // the line's units, consecutive (at most nbw per k, so at most max_pairs)
+        const int u_first = kus[kb] - ku0, np_l = min(kus[ke] - kus[kb], max_pairs);
+        float2 *ucl = uc + (size_t)lb * max_pairs * NW;
         if (t < N) {
-            float c, s;
-            cis2pi(xk[3 * kb + 1] * (double)(r2min + t), c, s);
-            ph2[t] = make_float2(c, s);
+            const cplx<float> v = ph2g[(l - lb0) * N + t];
+            ph2[lb][t] = make_float2(v.re, v.im);
         }
         for (int e = t; e < nkl * N; e += blockDim.x) {
-            const int kk = e / N, c3 = e % N;
-            float c, s;
-            cis2pi(xk[3 * (kb + kk) + 2] * (double)(r3min + c3), c, s);
-            ph3[kk * N + c3] = make_float2(c, s);
+            const cplx<float> v = ph3g[(kb - k0) * N + e];
+            ph3[lb][e] = make_float2(v.re, v.im);
         }
-        if (t < 64) {  // the line's (k, n) pairs in k order: exclusive prefix of the band counts (<= 4)
-            int nbk = 0, ub = 0;
-            if (t < nkl) {
-                ub = kus[kb + t];
-                nbk = min(kus[kb + t + 1] - ub, nbw);
-                ub -= ku0;
-            }
+        for (int e = t; e < np_l * NW; e += blockDim.x) {
+            const cplx<float> v = ucol[(int64_t)u_first * NW + e];
+            ucl[e] = make_float2(v.re, v.im);
+        }
+        if (t < 64) {  // pair -> k: exclusive prefix of the band counts (<= 4) over the line's k
+            const int nbk = t < nkl ? min(kus[kb + t + 1] - kus[kb + t], nbw) : 0;
             const unsigned long long below = (1ull << t) - 1ull;
             int start = 0;
 #pragma unroll
             for (int bit = 0; bit < 3; bit++) start += __popcll(__ballot((nbk >> bit) & 1) & below) << bit;
-            for (int m = 0; m < nbk; m++) {
-                pk[start + m] = t;
-                punit[start + m] = ub + m;
-            }
-            if (t == 63) npair = start + nbk;
+            for (int m = 0; m < nbk; m++) pk[lb][start + m] = t;
         }
-        __syncthreads();  // phase tables and pairs
-        const int np = npair;
-        for (int e = t; e < np * NW; e += blockDim.x) {  // U columns of the pairs
-            const int u = punit[e >> 4];
-            const cplx<float> v = U[(int64_t)NW * NW * ustate[u] + (e & 15) + NW * uband[u]];
-            uc[e] = make_float2(v.re, v.im);
-        }
+        __syncthreads();  // this line's tables staged (the other buffers were last read before it)
         float o2r[N], o2i[N];
 #pragma unroll
         for (int c = 0; c < N; c++) {
             float sr = 0.f, si = 0.f;
 #pragma unroll
             for (int b = 0; b < N; b++) {
-                const float2 p = ph2[b];
+                const float2 p = ph2[lb][b];
                 cmac(sr, si, p.x, p.y, vr[c][b], vi[c][b]);
             }
             o2r[c] = sr;
             o2i[c] = si;
         }
-        __syncthreads();  // U columns staged
-        for (int p0 = 0; p0 < np; p0 += 16) {
+        for (int p0 = 0; p0 < np_l; p0 += 16) {
             // A operand: row pair p0 + lane % 16 (zero past the line's pairs), column j
             const int p = p0 + (lane & 15);
             int kk = 0;
             float ur = 0.f, ui = 0.f;
-            if (p < np) {
-                kk = pk[p];
-                const float2 w = uc[p * NW + j];
+            if (p < np_l) {
+                kk = pk[lb][p];
+                const float2 w = ucl[p * NW + j];
                 ur = w.x;
                 ui = w.y;
             }
             f32x4 arr = {0.f, 0.f, 0.f, 0.f}, aii = arr, ari = arr, air = arr;
 #pragma unroll
             for (int c = 0; c < N; c++) {
-                const float2 e = ph3[kk * N + c];
+                const float2 e = ph3[lb][kk * N + c];
                 const float wr = ur * e.x - ui * e.y, wi = ur * e.y + ui * e.x;
                 arr = __builtin_amdgcn_mfma_f32_16x16x4f32(wr, o2r[c], arr, 0, 0, 0);
                 aii = __builtin_amdgcn_mfma_f32_16x16x4f32(wi, o2i[c], aii, 0, 0, 0);
                 ari = __builtin_amdgcn_mfma_f32_16x16x4f32(wr, o2i[c], ari, 0, 0, 0);
                 air = __builtin_amdgcn_mfma_f32_16x16x4f32(wi, o2r[c], air, 0, 0, 0);
             }
-            // this wave's four j: lane holds pair rows 4 (lane / 16) + r, column i = lane % 16. Tiles
-            // alternate between two buffers, so one barrier per tile separates writes from reads (the
-            // buffer written next was last read before this tile's barrier)
+            // this wave's four j: lane holds pair rows 4 (lane / 16) + r, column i = lane % 16
             const int buf = (p0 >> 4) & 1;
             red[buf][0][wave][lane] = arr - aii;
             red[buf][1][wave][lane] = ari + air;
@@ -378,9 +402,10 @@ gridopt_kR_project_mfma_kernel(int64_t nd, int r2min, int r3min, const cplx<floa
                     sr += rf[(w * 64 + src) * 4 + r];
                     si += rf[((4 + w) * 64 + src) * 4 + r];
                 }
-                if (dv && p0 + row < np) A[(int64_t)punit[p0 + row] * lda + gcol] = cplx<float>{sr, si};
+                if (dv && p0 + row < np_l) A[(int64_t)(u_first + p0 + row) * lda + gcol] = cplx<float>{sr, si};
             }
         }
+        lb ^= 1;
     }
 }
 
@@ -459,15 +484,31 @@ extern "C" int ephip_gridopt_kR_project(int32_t nw, int32_t nm, int32_t nrp, int
     return EPHIP_OK;
 }
 
-extern "C" int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp, int32_t np, int32_t r2min,
-                                             int32_t r3min, const void *op1, const double *xk, int64_t k0,
-                                             int64_t k1, int32_t l0, int32_t l1, const int32_t *line_start,
-                                             int32_t max_line_k, const int32_t *kus, const int64_t *ustate,
-                                             const int32_t *uband, const void *U, int32_t nbw, void *A, int64_t lda,
-                                             void *stream) {
+extern "C" int ephip_gridopt_kR_tables(int32_t nw, int32_t np, int32_t r2min, int32_t r3min, const double *xk,
+                                      int64_t k0, int64_t k1, int32_t l0, int32_t l1, const int32_t *line_start,
+                                      int32_t nunits, const int64_t *ustate, const int32_t *uband, const void *U,
+                                      void *ucol, void *ph2, void *ph3, void *stream) {
+    EPHIP_REQUIRE(nw >= 1 && np >= 1 && k1 >= k0 && k0 >= 0 && l1 >= l0 && l0 >= 0 && nunits >= 0,
+                  "gridopt_kR_tables: bad sizes");
+    const int64_t n = (int64_t)nunits * nw + (k1 - k0) * np + (int64_t)(l1 - l0) * np;
+    if (n == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(xk && line_start && ustate && uband && U && ucol && ph2 && ph3, "gridopt_kR_tables: null pointer");
+    gridopt_kR_tables_kernel<<<(unsigned)((n + 255) / 256), 256, 0, as_stream(stream)>>>(
+        nw, np, r2min, r3min, xk, k0, k1, l0, l1, line_start, nunits, ustate, uband,
+        static_cast<const cplx<float> *>(U), static_cast<cplx<float> *>(ucol), static_cast<cplx<float> *>(ph2),
+        static_cast<cplx<float> *>(ph3));
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp, int32_t np, const void *op1,
+                                             int64_t k0, int64_t k1, int32_t l0, int32_t l1, int32_t lb0,
+                                             const int32_t *line_start, int32_t max_line_k, const int32_t *kus,
+                                             const void *ucol, const void *ph2, const void *ph3, int32_t nbw,
+                                             void *A, int64_t lda, void *stream) {
     EPHIP_REQUIRE(nw == 16, "gridopt_kR_project_mfma: nw must be 16 (got %d)", nw);
     EPHIP_REQUIRE(nm >= 1 && nrp >= 1 && nbw >= 1 && nbw <= GO_NBW_MAX && k0 >= 0 && k1 >= k0 && l1 >= l0 &&
-                      l0 >= 0,
+                      l0 >= lb0 && lb0 >= 0,
                   "gridopt_kR_project_mfma: bad sizes (nbw=%d, at most %d window bands)", nbw, GO_NBW_MAX);
     EPHIP_REQUIRE(np >= 3 && np <= 8, "gridopt_kR_project_mfma: padded box face np x np with 3 <= np <= 8 (got %d)",
                   np);
@@ -476,16 +517,17 @@ extern "C" int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp
     EPHIP_REQUIRE(lda >= (int64_t)nw * nm * nrp, "gridopt_kR_project_mfma: lda smaller than nw nm nrp");
     const int64_t nd = (int64_t)nw * nw * nm * nrp;
     if (k1 == k0 || l1 == l0) return EPHIP_OK;
-    EPHIP_REQUIRE(op1 && xk && line_start && kus && ustate && uband && U && A,
-                  "gridopt_kR_project_mfma: null device pointer");
+    EPHIP_REQUIRE(op1 && line_start && kus && ucol && ph2 && ph3 && A, "gridopt_kR_project_mfma: null device pointer");
     using K = decltype(&gridopt_kR_project_mfma_kernel<3>);
     static const K kern[6] = {gridopt_kR_project_mfma_kernel<3>, gridopt_kR_project_mfma_kernel<4>,
                               gridopt_kR_project_mfma_kernel<5>, gridopt_kR_project_mfma_kernel<6>,
                               gridopt_kR_project_mfma_kernel<7>, gridopt_kR_project_mfma_kernel<8>};
-    const size_t lds = (size_t)max(max_line_k, 1) * nbw * nw * sizeof(float2);  // U columns of a line's pairs
+    const int max_pairs = max(max_line_k, 1) * nbw;
+    const size_t lds = 2 * (size_t)max_pairs * nw * sizeof(float2);  // U columns of a line's pairs, two buffers
     kern[np - 3]<<<(unsigned)(nd / 256), 256, lds, as_stream(stream)>>>(
-        nd, r2min, r3min, static_cast<const cplx<float> *>(op1), xk, k0, k1, l0, l1, line_start, kus, ustate,
-        uband, static_cast<const cplx<float> *>(U), nbw, static_cast<cplx<float> *>(A), lda);
+        nd, static_cast<const cplx<float> *>(op1), k0, k1, l0, l1, lb0, line_start, max_pairs, kus,
+        static_cast<const cplx<float> *>(ucol), static_cast<const cplx<float> *>(ph2),
+        static_cast<const cplx<float> *>(ph3), nbw, static_cast<cplx<float> *>(A), lda);
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

@@ -740,9 +740,23 @@ class _GridOptKR:
                C.addressof(out), stream)
         return self.cache[-1][1][plane]
 
+    def _tables(self, loop, dev):
+        """Per-batch tables of the MFMA form (U column of each unit, stage-2/3 phases), allocated once."""
+        torch = _torch()
+        if getattr(self, "ucol", None) is None:
+            self.ucol = torch.empty((loop.max_units, loop.st.nw), dtype=torch.complex64, device=dev)
+            self.ph3 = torch.empty((loop.nk_batch, self.np), dtype=torch.complex64, device=dev)
+            self.ph2 = torch.empty((loop.nk_batch, self.np), dtype=torch.complex64, device=dev)  # lines <= k
+        return self.ucol, self.ph2, self.ph3
+
     def run(self, loop, i0, n, u0, u1, A, stream):
         st = loop.st
         l_first, l_last = int(self.line_of_k[i0]), int(self.line_of_k[i0 + n - 1]) + 1
+        if self.mfma:
+            ucol, ph2, ph3 = self._tables(loop, A.device)
+            L.call("ephip_gridopt_kR_tables", st.nw, self.np, self.rmin[1], self.rmin[2], L.ptr(loop.xk_surv), i0,
+                   i0 + n, l_first, l_last, L.ptr(self.line_start), u1 - u0, L.ptr(loop.unit_state[u0:u1]),
+                   L.ptr(loop.unit_band[u0:u1]), L.ptr(st.el.U), L.ptr(ucol), L.ptr(ph2), L.ptr(ph3), stream)
         l = l_first
         while l < l_last:
             plane = self.line_plane[l]
@@ -750,11 +764,16 @@ class _GridOptKR:
             while l_end < l_last and self.line_plane[l_end] == plane:
                 l_end += 1
             buf = self._plane_buffer(int(plane), loop.epmat_el, stream)
-            entry = "ephip_gridopt_kR_project_mfma" if self.mfma else "ephip_gridopt_kR_project"
-            L.call(entry, st.nw, st.nm, loop.nrp, self.np, self.rmin[1], self.rmin[2],
-                   L.ptr(buf), L.ptr(loop.xk_surv), i0, i0 + n, l, l_end, L.ptr(self.line_start), self.max_line_k,
-                   L.ptr(loop.k_unit_start_dev), L.ptr(loop.unit_state[u0:u1]), L.ptr(loop.unit_band[u0:u1]),
-                   L.ptr(st.el.U), loop.max_units_per_k, L.ptr(A), A.stride(0), stream)
+            if self.mfma:
+                L.call("ephip_gridopt_kR_project_mfma", st.nw, st.nm, loop.nrp, self.np, L.ptr(buf), i0, i0 + n, l,
+                       l_end, l_first, L.ptr(self.line_start), self.max_line_k, L.ptr(loop.k_unit_start_dev),
+                       L.ptr(ucol), L.ptr(ph2), L.ptr(ph3), loop.max_units_per_k, L.ptr(A), A.stride(0), stream)
+            else:
+                L.call("ephip_gridopt_kR_project", st.nw, st.nm, loop.nrp, self.np, self.rmin[1], self.rmin[2],
+                       L.ptr(buf), L.ptr(loop.xk_surv), i0, i0 + n, l, l_end, L.ptr(self.line_start),
+                       self.max_line_k, L.ptr(loop.k_unit_start_dev), L.ptr(loop.unit_state[u0:u1]),
+                       L.ptr(loop.unit_band[u0:u1]), L.ptr(st.el.U), loop.max_units_per_k, L.ptr(A), A.stride(0),
+                       stream)
             l = l_end
 
 

@@ -246,15 +246,23 @@ int ephip_gridopt_kR_project(int32_t nw, int32_t nm, int32_t nrp, int32_t np, in
                              const int32_t *line_start, int32_t max_line_k, const int32_t *kus,
                              const int64_t *ustate, const int32_t *uband, const void *U, int32_t nbw, void *A,
                              int64_t lda, void *stream);
-/* ephip_gridopt_kR_project with stages 2/3 and the projection on the matrix cores (16x16x4 FP32 MFMAs:
- * along a line, A[(k, n)][G] = sum_{j, c} U_k[j, n] e^{2 pi i k3 r3(c)} op2[(j, c)][G]), for nw = 16; op1 in
- * the natural row order d = i + nw*(j + nw*rest) (ephip_gridopt_stage1 with nb = 1). Same arguments and
- * output as ephip_gridopt_kR_project. */
-int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp, int32_t np, int32_t r2min, int32_t r3min,
-                                  const void *op1, const double *xk, int64_t k0, int64_t k1, int32_t l0, int32_t l1,
-                                  const int32_t *line_start, int32_t max_line_k, const int32_t *kus,
-                                  const int64_t *ustate, const int32_t *uband, const void *U, int32_t nbw, void *A,
-                                  int64_t lda, void *stream);
+/* Stages 2/3 and the projection on the matrix cores (16x16x4 FP32 MFMAs), nw = 16: along a line,
+ * A[(k, n)][G] = sum_{j, c} U_k[j, n] e^{2 pi i k3 (r3min + c)} op2[(j, c)][G]. op1 in the natural row order
+ * d = i + nw*(j + nw*rest) (ephip_gridopt_stage1 with nb = 1). The per-batch tables come from
+ * ephip_gridopt_kR_tables for the batch's k k0 <= k < k1, its lines lb0 <= l < lb1 and its units:
+ * ucol (complex64, units x nw): the U column of each unit (unit u relative to kus[k0]: U[ustate[u]][:, uband[u]]);
+ * ph3 (complex64, (k1 - k0) x np): e^{2 pi i k3 (r3min + c)}; ph2 (complex64, (lb1 - lb0) x np):
+ * e^{2 pi i k2 (r2min + b)} of each line. ephip_gridopt_kR_project_mfma then covers the lines
+ * lb0 <= l0 <= l < l1 of one k1 plane (as ephip_gridopt_kR_project; its units are consecutive, at most
+ * nbw per k). Output as ephip_gridopt_kR_project. */
+int ephip_gridopt_kR_tables(int32_t nw, int32_t np, int32_t r2min, int32_t r3min, const double *xk, int64_t k0,
+                            int64_t k1, int32_t lb0, int32_t lb1, const int32_t *line_start, int32_t nunits,
+                            const int64_t *ustate, const int32_t *uband, const void *U, void *ucol, void *ph2,
+                            void *ph3, void *stream);
+int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp, int32_t np, const void *op1, int64_t k0,
+                                  int64_t k1, int32_t l0, int32_t l1, int32_t lb0, const int32_t *line_start,
+                                  int32_t max_line_k, const int32_t *kus, const void *ucol, const void *ph2,
+                                  const void *ph3, int32_t nbw, void *A, int64_t lda, void *stream);
 
 /* Staged (GridOpt) kR->kq over Rp for the q of a uniform grid qgrid = (m1, m2, m3) (q index
  * (q1*m2 + q2)*m3 + q3 at q = (q1/m1, q2/m2, q3/m3)), fused with the k+q-window and phonon rotations and
