@@ -584,7 +584,10 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
     const int64_t *__restrict__ bmin, const int64_t *__restrict__ bmax, const cplx<float> *__restrict__ uph,
     const double *__restrict__ omega, int nbw, float *__restrict__ g2) {
     constexpr int NW = 16;
-    __shared__ float2 ph1[32], ph2[GQ_M2_MAX * N], ph3[GQ_M3_MAX * N];  // q1, all q2, all q3 phases
+    // q1, all q2, all q3 phases; the q2/q3 rows padded to an even count and read as 16-byte pairs
+    constexpr int NE = (N + 1) & ~1;
+    __shared__ float2 ph1[32];
+    __shared__ float4 ph2[GQ_M2_MAX * NE / 2], ph3[GQ_M3_MAX * NE / 2];
     __shared__ int slo[GQ_M2_MAX + 1];  // kept-list offsets of the slab's lines
     // metadata of every kept q of the (u, q1) slab: k+q state, q, and the packed word
     // (window bands | first band << 8 | q3 << 16 | q2 << 22), q2, q3 < 64
@@ -605,15 +608,15 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
         cis2pi((double)q1 / m1 * (double)(p1min + t), c, s);
         ph1[t] = make_float2(c, s);
     }
-    for (int e = t; e < m2 * N; e += blockDim.x) {
-        float c, s;
-        cis2pi((double)(e / N) / m2 * (double)(p2min + e % N), c, s);
-        ph2[e] = make_float2(c, s);
+    for (int e = t; e < m2 * NE; e += blockDim.x) {
+        float c = 0.f, s = 0.f;
+        if (e % NE < N) cis2pi((double)(e / NE) / m2 * (double)(p2min + e % NE), c, s);
+        reinterpret_cast<float2 *>(ph2)[e] = make_float2(c, s);
     }
-    for (int e = t; e < m3 * N; e += blockDim.x) {
-        float c, s;
-        cis2pi((double)(e / N) / m3 * (double)(p3min + e % N), c, s);
-        ph3[e] = make_float2(c, s);
+    for (int e = t; e < m3 * NE; e += blockDim.x) {
+        float c = 0.f, s = 0.f;
+        if (e % NE < N) cis2pi((double)(e / NE) / m3 * (double)(p3min + e % NE), c, s);
+        reinterpret_cast<float2 *>(ph3)[e] = make_float2(c, s);
     }
     __syncthreads();
     const int slab0 = slo[0], nslab = slo[m2] - slab0;  // <= m2 m3 <= GQ_SLAB_MAX
@@ -684,24 +687,32 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
             const int q2 = (w23 >> 22) & 63, q3 = (w23 >> 16) & 63;
             if (q2 != cur) {  // uniform
                 cur = q2;
+                float2 e2[NE];
+#pragma unroll
+                for (int h = 0; h < NE / 2; h++) {
+                    const float4 v = ph2[q2 * (NE / 2) + h];
+                    e2[2 * h] = make_float2(v.x, v.y);
+                    e2[2 * h + 1] = make_float2(v.z, v.w);
+                }
 #pragma unroll
                 for (int c = 0; c < N; c++) {
                     float sr = 0.f, si = 0.f;
 #pragma unroll
-                    for (int b = 0; b < N; b++) {
-                        const float2 p = ph2[q2 * N + b];
-                        cmac(sr, si, p.x, p.y, br[c][b], bi[c][b]);
-                    }
+                    for (int b = 0; b < N; b++) cmac(sr, si, e2[b].x, e2[b].y, br[c][b], bi[c][b]);
                     b2r[c] = sr;
                     b2i[c] = si;
                 }
             }
+            float2 e3[NE];
+#pragma unroll
+            for (int h = 0; h < NE / 2; h++) {
+                const float4 v = ph3[q3 * (NE / 2) + h];
+                e3[2 * h] = make_float2(v.x, v.y);
+                e3[2 * h + 1] = make_float2(v.z, v.w);
+            }
             float gr = 0.f, gi = 0.f;
 #pragma unroll
-            for (int c = 0; c < N; c++) {
-                const float2 p = ph3[q3 * N + c];
-                cmac(gr, gi, p.x, p.y, b2r[c], b2i[c]);
-            }
+            for (int c = 0; c < N; c++) cmac(gr, gi, e3[c].x, e3[c].y, b2r[c], b2i[c]);
             // conj(w) g for the NBW columns: 2 NBW values summed over the 16 lanes of the row, four at a
             // time; lanes 4s of the row write value s of each group
             float v[2 * NBW];
