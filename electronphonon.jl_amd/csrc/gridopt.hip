@@ -293,9 +293,9 @@ __global__ __launch_bounds__(256) void gridopt_kR_tables_kernel(int nw, int np, 
 // A workgroup holds one block of 16 G (i = lane % 16) and all 16 j (four per wave, j = thread / 16): op1
 // in its natural row order d = i + 16 (j + 16 rest), so each lane's op2[c] is its own B operand of the
 // 16x16x4 FP32 MFMA (B[j][i]) and W is built in the A operand (A[pair][j]). The pairs (k, n) of a line
-// (its units, consecutive) are taken 16 at a time; per c each wave runs four MFMAs (a complex product,
-// four independent accumulators) over its four j, and the four waves' partial tiles are summed through
-// LDS. The line's phases and U columns come from the batch tables (gridopt_kR_tables_kernel) into LDS
+// (its units, consecutive) are taken 16 at a time; per c each wave runs three MFMAs (a complex product
+// from three real products, three independent accumulators) over its four j, and the four waves'
+// partial tiles are summed through LDS. The line's phases and U columns come from the batch tables (gridopt_kR_tables_kernel) into LDS
 // buffers that alternate between lines, so one barrier per line separates a line's staging from the
 // previous line's reads; partial tiles alternate likewise, one barrier per tile.
 template <int N>
@@ -355,7 +355,7 @@ gridopt_kR_project_mfma_kernel(int64_t nd, const cplx<float> *__restrict__ op1, 
             for (int m = 0; m < nbk; m++) pk[lb][start + m] = t;
         }
         __syncthreads();  // this line's tables staged (the other buffers were last read before it)
-        float o2r[N], o2i[N];
+        float o2r[N], o2i[N], o2s[N];  // op2 and its real + imaginary part (three-product complex MFMA)
 #pragma unroll
         for (int c = 0; c < N; c++) {
             float sr = 0.f, si = 0.f;
@@ -366,6 +366,7 @@ gridopt_kR_project_mfma_kernel(int64_t nd, const cplx<float> *__restrict__ op1, 
             }
             o2r[c] = sr;
             o2i[c] = si;
+            o2s[c] = sr + si;
         }
         for (int p0 = 0; p0 < np_l; p0 += 16) {
             // A operand: row pair p0 + lane % 16 (zero past the line's pairs), column j
@@ -378,20 +379,20 @@ gridopt_kR_project_mfma_kernel(int64_t nd, const cplx<float> *__restrict__ op1, 
                 ur = w.x;
                 ui = w.y;
             }
-            f32x4 arr = {0.f, 0.f, 0.f, 0.f}, aii = arr, ari = arr, air = arr;
+            // W op2 with three real products (Gauss): re = wr o2r - wi o2i, im = (wr + wi)(o2r + o2i) - wr o2r - wi o2i
+            f32x4 prr = {0.f, 0.f, 0.f, 0.f}, pii = prr, pss = prr;
 #pragma unroll
             for (int c = 0; c < N; c++) {
                 const float2 e = ph3[lb][kk * N + c];
                 const float wr = ur * e.x - ui * e.y, wi = ur * e.y + ui * e.x;
-                arr = __builtin_amdgcn_mfma_f32_16x16x4f32(wr, o2r[c], arr, 0, 0, 0);
-                aii = __builtin_amdgcn_mfma_f32_16x16x4f32(wi, o2i[c], aii, 0, 0, 0);
-                ari = __builtin_amdgcn_mfma_f32_16x16x4f32(wr, o2i[c], ari, 0, 0, 0);
-                air = __builtin_amdgcn_mfma_f32_16x16x4f32(wi, o2r[c], air, 0, 0, 0);
+                prr = __builtin_amdgcn_mfma_f32_16x16x4f32(wr, o2r[c], prr, 0, 0, 0);
+                pii = __builtin_amdgcn_mfma_f32_16x16x4f32(wi, o2i[c], pii, 0, 0, 0);
+                pss = __builtin_amdgcn_mfma_f32_16x16x4f32(wr + wi, o2s[c], pss, 0, 0, 0);
             }
             // this wave's four j: lane holds pair rows 4 (lane / 16) + r, column i = lane % 16
             const int buf = (p0 >> 4) & 1;
-            red[buf][0][wave][lane] = arr - aii;
-            red[buf][1][wave][lane] = ari + air;
+            red[buf][0][wave][lane] = prr - pii;
+            red[buf][1][wave][lane] = pss - prr - pii;
             __syncthreads();
             {
                 const int row = t >> 4, src = (t & 15) + 16 * (row >> 2), r = row & 3;
