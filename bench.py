@@ -441,7 +441,10 @@ def main():
                                                          dom[4], args.kbatch_k, var[dom[4]]))
                                  if ("fast" in results and args.config == "cfg2"
                                      and (dom[4] != "kR->kq" or r.get("fused_rotation"))) else None),
-                     "kernel": dom[0], "avg_launch_ms": dom[2], "flop_per_launch": dom[3]},
+                     "kernel": dom[0], "avg_launch_ms": dom[2], "flop_per_launch": dom[3],
+                     # FP32 VALU kernels: the same achieved rate against the unpacked vector peak
+                     "frac_unpacked_vector_peak": (dom[1] / 78.6 if "fast" in results and var[dom[4]] == "gridopt"
+                                                   else None)},
         "cpu_baseline": cpu,
         "detail": results,
     }

@@ -154,6 +154,34 @@ __device__ __forceinline__ float dpp_row_sum4(float a, float b, float c, float d
     return x;
 }
 
+// dpp_row_sum4 fed so that it needs no lane selects: the row sums of conj(w) g for two columns a, b of
+// w, scattered as in dpp_row_sum4 (lanes 4s .. 4s + 3 hold re a, im a, re b, im b for s = 0 .. 3). Each
+// lane passes its own column wo (b if lane bit 3 is set, else a) and the other one wx, and the parts of
+// g arranged by lane bit 2: (p1, p2) = (gr, gi) or (gi, -gr), the part the lane keeps (real or
+// imaginary), and (q1, q2) the other; then w.x p1 + w.y p2 is that part of conj(w) g.
+__device__ __forceinline__ float dpp_row_sum4_pairs(float2 wo, float2 wx, float p1, float p2, float q1, float q2) {
+    auto dpp = [](float x, int ctrl) {
+        const int xi = __builtin_bit_cast(int, x);
+        int y;
+        switch (ctrl) {
+            case 0: y = __builtin_amdgcn_update_dpp(0, xi, 0x128, 0xF, 0xF, false); break;  // row_ror:8
+            case 1: y = __builtin_amdgcn_update_dpp(0, xi, 0x141, 0xF, 0xF, false); break;  // row_half_mirror
+            case 2: y = __builtin_amdgcn_update_dpp(0, xi, 0x4E, 0xF, 0xF, false); break;   // quad_perm [2,3,0,1]
+            default: y = __builtin_amdgcn_update_dpp(0, xi, 0xB1, 0xF, 0xF, false); break;  // quad_perm [1,0,3,2]
+        }
+        return __builtin_bit_cast(float, y);
+    };
+    float x0 = wo.x * p1 + wo.y * p2;  // own column, kept part
+    float x1 = wo.x * q1 + wo.y * q2;  // own column, other part
+    const float y0 = wx.x * p1 + wx.y * p2, y1 = wx.x * q1 + wx.y * q2;
+    x0 += dpp(y0, 0);  // partner l ^ 8 holds the other column as its own, same parts
+    x1 += dpp(y1, 0);
+    float x = x0 + dpp(x1, 1);  // partner l ^ 7 keeps the other part
+    x += dpp(x, 2);
+    x += dpp(x, 3);
+    return x;
+}
+
 template <int N, int NW>
 __global__ __launch_bounds__(256) void gridopt_kR_project_kernel(
     int64_t nd, int r2min, int r3min, const cplx<float> *__restrict__ op1,
@@ -593,7 +621,8 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
     // metadata of every kept q of the (u, q1) slab: k+q state, q, and the packed word
     // (window bands | first band << 8 | q3 << 16 | q2 << 22), q2, q3 < 64
     __shared__ int sk_kq[GQ_SLAB_MAX], sk_iq[GQ_SLAB_MAX], sk_nb[GQ_SLAB_MAX];
-    __shared__ float2 ucol[GQ_CHUNK * NBW * NW];
+    constexpr int NBWE = (NBW + 1) & ~1;  // U_kq columns staged in pairs (a zero column pads an odd count)
+    __shared__ float2 ucol[GQ_CHUNK * NBWE * NW];
     __shared__ float2 tq[GQ_CHUNK * NBW * NW];  // t[kk][m][nu], nu < nm <= 16
     const int q1 = blockIdx.x, u = blockIdx.y;
     const int t = threadIdx.x;
@@ -665,8 +694,8 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
         const int ks = e0, nk = cut - e0;  // 0 < nk <= GQ_CHUNK
         e0 = cut;
         __syncthreads();  // slab tables written; the previous chunk's tables no longer read
-        for (int e = t; e < nk * NBW * NW; e += blockDim.x) {
-            const int kk = e / (NBW * NW), m = (e / NW) % NBW, ii = e % NW;
+        for (int e = t; e < nk * NBWE * NW; e += blockDim.x) {
+            const int kk = e / (NBWE * NW), m = (e / NW) % NBWE, ii = e % NW;
             const int nbm = sk_nb[ks + kk], nbk = nbm & 0xff, mlo = (nbm >> 8) & 0xff;
             float2 w = make_float2(0.f, 0.f);
             if (m < nbk) {
@@ -680,10 +709,16 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
         const int meta = lane < nk ? sk_nb[ks + lane] : 0;
         float b2r[N], b2i[N];
         int cur = -1;
+        // the columns of each pair in lane order: lanes with bit 3 set take the pair's second column as
+        // their own (see dpp_row_sum4_pairs); read before the stage-2/3 sums that hide the latency
+        const int h3 = (i >> 3) & 1;
         for (int kk = 0; kk < nk; kk++) {
-            float2 w[NBW];  // U_kq columns of this kept q, read before the stage-2/3 sums that hide them
+            float2 wo[NBWE / 2], wx[NBWE / 2];
 #pragma unroll
-            for (int m = 0; m < NBW; m++) w[m] = ucol[(kk * NBW + m) * NW + i];
+            for (int h = 0; h < NBWE / 2; h++) {
+                wo[h] = ucol[(kk * NBWE + 2 * h + h3) * NW + i];
+                wx[h] = ucol[(kk * NBWE + 2 * h + 1 - h3) * NW + i];
+            }
             const int w23 = __builtin_amdgcn_readlane(meta, kk);
             const int q2 = (w23 >> 22) & 63, q3 = (w23 >> 16) & 63;
             if (q2 != cur) {  // uniform
@@ -714,20 +749,15 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
             float gr = 0.f, gi = 0.f;
 #pragma unroll
             for (int c = 0; c < N; c++) cmac(gr, gi, e3[c].x, e3[c].y, b2r[c], b2i[c]);
-            // conj(w) g for the NBW columns: 2 NBW values summed over the 16 lanes of the row, four at a
-            // time; lanes 4s of the row write value s of each group
-            float v[2 * NBW];
-#pragma unroll
-            for (int m = 0; m < NBW; m++) {
-                v[2 * m] = w[m].x * gr + w[m].y * gi;
-                v[2 * m + 1] = w[m].x * gi - w[m].y * gr;
-            }
+            // conj(w) g for the column pairs, summed over the 16 lanes of the row; lanes 4s of the row
+            // write value s of each pair (re, im of its first column, re, im of its second)
+            const bool h2 = (i & 4) != 0;
+            const float gp1 = h2 ? gi : gr, gp2 = h2 ? -gr : gi, gq1 = h2 ? gr : gi, gq2 = h2 ? gi : -gr;
             float *tqf = reinterpret_cast<float *>(tq);
 #pragma unroll
-            for (int g0 = 0; g0 < 2 * NBW; g0 += 4) {
-                const float x = dpp_row_sum4(v[g0], v[g0 + 1], g0 + 2 < 2 * NBW ? v[g0 + 2] : 0.f,
-                                             g0 + 3 < 2 * NBW ? v[g0 + 3] : 0.f);
-                const int vi = g0 + (i >> 2);  // value index: column vi / 2, real (0) or imaginary (1)
+            for (int h = 0; h < NBWE / 2; h++) {
+                const float x = dpp_row_sum4_pairs(wo[h], wx[h], gp1, gp2, gq1, gq2);
+                const int vi = 4 * h + (i >> 2);  // value index: column vi / 2, real (0) or imaginary (1)
                 if (rv && (i & 3) == 0 && vi < 2 * NBW) tqf[((kk * NBW + (vi >> 1)) * NW + nu) * 2 + (vi & 1)] = x;
             }
         }
