@@ -422,3 +422,50 @@ def test_fourier_packed_few_stages(backend, nr):
     for u in range(nu):
         c = counts[u]
         assert _rel(o[u, :c], O.fourier(op[u], irvec, xk[cols[u, :c]])) < 2e-5
+
+
+@pytest.mark.parametrize("nb", [1, 16])
+def test_gridopt_stage1_planes(backend, nb):
+    """Stage 1 of the staged RR->kR for several k1 planes in one pass equals one pass per plane bit for
+    bit, and the FP64 sum over r1 of the box to 1e-5 (absent box points contribute zero, faces padded
+    with zeros, rows j-fastest for nb > 1); more than 8 planes is refused."""
+    import ctypes as C
+    from ephip import _lib as L
+    rng = np.random.default_rng(41)
+    n1, n2, n3, np_, r1min, nd = 5, 3, 4, 4, -2, 16 * 16 * 3
+    nr, ldop = 80, 16 * 16 * 3 + 7
+    rmap = rng.permutation(nr)[:n1 * n2 * n3].astype(np.int32)
+    assert rmap.size == n1 * n2 * n3
+    rmap[rng.random(rmap.size) < 0.2] = -1
+    op = (rng.standard_normal((nr, ldop)) + 1j * rng.standard_normal((nr, ldop))).astype(np.complex64)
+    x1 = [0.0, 0.125, 0.7]
+    dmap, dop = _dev(rmap), _dev(op)
+    outs = [torch.zeros(np_ * np_ * nd, dtype=torch.complex64, device="cuda") for _ in x1]
+    singles = [torch.zeros_like(o) for o in outs]
+    xa = (C.c_double * len(x1))(*x1)
+    pa = (C.c_void_p * len(x1))(*[L.ptr(o) for o in outs])
+    L.call("ephip_gridopt_stage1_planes", nd, n1, n2, n3, np_, r1min, L.ptr(dmap), L.ptr(dop), ldop, len(x1),
+           C.addressof(xa), nb, C.addressof(pa), L.stream_handle())
+    for x, o in zip(x1, singles):
+        L.call("ephip_gridopt_stage1", nd, n1, n2, n3, np_, r1min, L.ptr(dmap), L.ptr(dop), ldop, x, nb, L.ptr(o),
+               L.stream_handle())
+    torch.cuda.synchronize()
+    d = np.arange(nd)
+    dd = d if nb <= 1 else (d // nb) % nb + nb * (d % nb + nb * (d // (nb * nb)))
+    opd = op.astype(np.complex128)
+    for p, x in enumerate(x1):
+        assert torch.equal(outs[p].view(torch.float32), singles[p].view(torch.float32))
+        ref = np.zeros((np_ * np_, nd), np.complex128)
+        for c in range(n3):
+            for b in range(n2):
+                for a in range(n1):
+                    ir = rmap[a + n1 * (b + n2 * c)]
+                    if ir >= 0:
+                        ref[b + np_ * c, dd] += np.exp(2j * np.pi * x * (r1min + a)) * opd[ir, :nd]
+        got = outs[p].cpu().numpy().reshape(np_ * np_, nd)
+        assert _rel(got, ref) < 1e-5
+    pa9 = (C.c_void_p * 9)(*([L.ptr(outs[0])] * 9))
+    xa9 = (C.c_double * 9)(*([0.0] * 9))
+    with pytest.raises(L.EphipError):
+        L.call("ephip_gridopt_stage1_planes", nd, n1, n2, n3, np_, r1min, L.ptr(dmap), L.ptr(dop), ldop, 9,
+               C.addressof(xa9), nb, C.addressof(pa9), L.stream_handle())
