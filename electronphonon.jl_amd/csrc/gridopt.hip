@@ -621,8 +621,10 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
     // metadata of every kept q of the (u, q1) slab: k+q state, q, and the packed word
     // (window bands | first band << 8 | q3 << 16 | q2 << 22), q2, q3 < 64
     __shared__ int sk_kq[GQ_SLAB_MAX], sk_iq[GQ_SLAB_MAX], sk_nb[GQ_SLAB_MAX];
-    constexpr int NBWE = (NBW + 1) & ~1;  // U_kq columns staged in pairs (a zero column pads an odd count)
-    __shared__ float2 ucol[GQ_CHUNK * NBWE * NW];
+    // an even column count is summed in pairs fed in lane order (dpp_row_sum4_pairs); an odd one with
+    // dpp_row_sum4 (padding it to pairs costs more in staging than the selects it saves)
+    constexpr bool PAIRS = NBW % 2 == 0;
+    __shared__ float2 ucol[GQ_CHUNK * NBW * NW];
     __shared__ float2 tq[GQ_CHUNK * NBW * NW];  // t[kk][m][nu], nu < nm <= 16
     const int q1 = blockIdx.x, u = blockIdx.y;
     const int t = threadIdx.x;
@@ -694,8 +696,8 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
         const int ks = e0, nk = cut - e0;  // 0 < nk <= GQ_CHUNK
         e0 = cut;
         __syncthreads();  // slab tables written; the previous chunk's tables no longer read
-        for (int e = t; e < nk * NBWE * NW; e += blockDim.x) {
-            const int kk = e / (NBWE * NW), m = (e / NW) % NBWE, ii = e % NW;
+        for (int e = t; e < nk * NBW * NW; e += blockDim.x) {
+            const int kk = e / (NBW * NW), m = (e / NW) % NBW, ii = e % NW;
             const int nbm = sk_nb[ks + kk], nbk = nbm & 0xff, mlo = (nbm >> 8) & 0xff;
             float2 w = make_float2(0.f, 0.f);
             if (m < nbk) {
@@ -713,11 +715,16 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
         // their own (see dpp_row_sum4_pairs); read before the stage-2/3 sums that hide the latency
         const int h3 = (i >> 3) & 1;
         for (int kk = 0; kk < nk; kk++) {
-            float2 wo[NBWE / 2], wx[NBWE / 2];
+            float2 wo[(NBW + 1) / 2], wx[(NBW + 1) / 2], w[NBW];
+            if constexpr (PAIRS) {
 #pragma unroll
-            for (int h = 0; h < NBWE / 2; h++) {
-                wo[h] = ucol[(kk * NBWE + 2 * h + h3) * NW + i];
-                wx[h] = ucol[(kk * NBWE + 2 * h + 1 - h3) * NW + i];
+                for (int h = 0; h < NBW / 2; h++) {
+                    wo[h] = ucol[(kk * NBW + 2 * h + h3) * NW + i];
+                    wx[h] = ucol[(kk * NBW + 2 * h + 1 - h3) * NW + i];
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < NBW; m++) w[m] = ucol[(kk * NBW + m) * NW + i];
             }
             const int w23 = __builtin_amdgcn_readlane(meta, kk);
             const int q2 = (w23 >> 22) & 63, q3 = (w23 >> 16) & 63;
@@ -751,14 +758,31 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
             for (int c = 0; c < N; c++) cmac(gr, gi, e3[c].x, e3[c].y, b2r[c], b2i[c]);
             // conj(w) g for the column pairs, summed over the 16 lanes of the row; lanes 4s of the row
             // write value s of each pair (re, im of its first column, re, im of its second)
-            const bool h2 = (i & 4) != 0;
-            const float gp1 = h2 ? gi : gr, gp2 = h2 ? -gr : gi, gq1 = h2 ? gr : gi, gq2 = h2 ? gi : -gr;
             float *tqf = reinterpret_cast<float *>(tq);
+            if constexpr (PAIRS) {
+                const bool h2 = (i & 4) != 0;
+                const float gp1 = h2 ? gi : gr, gp2 = h2 ? -gr : gi, gq1 = h2 ? gr : gi, gq2 = h2 ? gi : -gr;
 #pragma unroll
-            for (int h = 0; h < NBWE / 2; h++) {
-                const float x = dpp_row_sum4_pairs(wo[h], wx[h], gp1, gp2, gq1, gq2);
-                const int vi = 4 * h + (i >> 2);  // value index: column vi / 2, real (0) or imaginary (1)
-                if (rv && (i & 3) == 0 && vi < 2 * NBW) tqf[((kk * NBW + (vi >> 1)) * NW + nu) * 2 + (vi & 1)] = x;
+                for (int h = 0; h < NBW / 2; h++) {
+                    const float x = dpp_row_sum4_pairs(wo[h], wx[h], gp1, gp2, gq1, gq2);
+                    const int vi = 4 * h + (i >> 2);  // value index: column vi / 2, real (0) or imaginary (1)
+                    if (rv && (i & 3) == 0) tqf[((kk * NBW + (vi >> 1)) * NW + nu) * 2 + (vi & 1)] = x;
+                }
+            } else {
+                float v[2 * NBW];
+#pragma unroll
+                for (int m = 0; m < NBW; m++) {
+                    v[2 * m] = w[m].x * gr + w[m].y * gi;
+                    v[2 * m + 1] = w[m].x * gi - w[m].y * gr;
+                }
+#pragma unroll
+                for (int g0 = 0; g0 < 2 * NBW; g0 += 4) {
+                    const float x = dpp_row_sum4(v[g0], v[g0 + 1], g0 + 2 < 2 * NBW ? v[g0 + 2] : 0.f,
+                                                 g0 + 3 < 2 * NBW ? v[g0 + 3] : 0.f);
+                    const int vi = g0 + (i >> 2);
+                    if (rv && (i & 3) == 0 && vi < 2 * NBW)
+                        tqf[((kk * NBW + (vi >> 1)) * NW + nu) * 2 + (vi & 1)] = x;
+                }
             }
         }
         __syncthreads();  // t of every kept q of the chunk
