@@ -41,8 +41,8 @@ PEAK = {"fp32": 157.3, "bf16x3": 2500.0}  # TFLOP/s, MI355X dense MFMA (fp32, bf
 # profiles/cfg2_fast_bf16_mfma_hbm_counters.md). Counters cannot be read
 # inside the timed run; a variant or batch without a measurement reports traffic null.
 TRAFFIC_BYTES_CFG2 = {
-    ("fp32", "kR->kq", 768, "gridopt"): (2185301 * 2 + 414730) * 1024,   # staged kR->kq + rotations + g2
-    ("fp32", "RR->kR", 768, "gridopt"): (204197 * 2 + 230276) * 1024,    # staged RR->kR projection launch (MFMA)
+    ("fp32", "kR->kq", 768, "gridopt"): (2186567 * 2 + 414734) * 1024,   # staged kR->kq + rotations + g2
+    ("fp32", "RR->kR", 768, "gridopt"): (204184 * 2 + 230276) * 1024,    # staged RR->kR projection launch (MFMA)
     ("bf16x3", "kR->kq", 384, 10): (2998082 * 2 + 204444) * 1024,   # fused kR->kq + rotations, 8 waves
     ("bf16x3", "RR->kR", 384, "proj0"): (1827474 * 2 + 200604) * 1024,  # RR->kR fused with the projection
 }
