@@ -41,8 +41,10 @@ PEAK = {"fp32": 157.3, "bf16x3": 2500.0}  # TFLOP/s, MI355X dense MFMA (fp32, bf
 # profiles/cfg2_fast_bf16_mfma_hbm_counters.md). Counters cannot be read
 # inside the timed run; a variant or batch without a measurement reports traffic null.
 TRAFFIC_BYTES_CFG2 = {
-    ("fp32", "kR->kq", 768, "gridopt"): (2186567 * 2 + 414734) * 1024,   # staged kR->kq + rotations + g2
-    ("fp32", "RR->kR", 768, "gridopt"): (204184 * 2 + 230276) * 1024,    # staged RR->kR projection launch (MFMA)
+    ("fp32", "kR->kq", 1152, "gridopt"): (3326982 * 2 + 622497) * 1024,  # staged kR->kq + rotations + g2
+    ("fp32", "RR->kR", 1152, "gridopt"): (204754 * 2 + 268683) * 1024,   # staged RR->kR projection launch (MFMA)
+    ("fp32", "kR->kq", 768, "gridopt"): (2186567 * 2 + 414734) * 1024,
+    ("fp32", "RR->kR", 768, "gridopt"): (204184 * 2 + 230276) * 1024,
     ("bf16x3", "kR->kq", 384, 10): (2998082 * 2 + 204444) * 1024,   # fused kR->kq + rotations, 8 waves
     ("bf16x3", "RR->kR", 384, "proj0"): (1827474 * 2 + 200604) * 1024,  # RR->kR fused with the projection
 }
@@ -184,7 +186,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--qtile", type=int, default=8)
     ap.add_argument("--kbatch", type=int, default=16384, help="k batch of the plain outer-q path")
-    ap.add_argument("--kbatch-k", type=int, default=768, help="k batch (one step) of the fast outer-k path")
+    ap.add_argument("--kbatch-k", type=int, default=1152, help="k batch (one step) of the fast outer-k path")
     ap.add_argument("--path", choices=["fast", "plain", "both"], default="both")
     ap.add_argument("--precision", choices=["fp32", "bf16x3"], default="bf16x3",
                     help="Fourier precision of the fast path (split-BF16 meets the 1e-4 |g|^2 bound)")
