@@ -323,9 +323,10 @@ __global__ __launch_bounds__(256) void gridopt_kR_tables_kernel(int nw, int np, 
 // 16x16x4 FP32 MFMA (B[j][i]) and W is built in the A operand (A[pair][j]). The pairs (k, n) of a line
 // (its units, consecutive) are taken 16 at a time; per c each wave runs three MFMAs (a complex product
 // from three real products, three independent accumulators) over its four j, and the four waves'
-// partial tiles are summed through LDS. The line's phases and U columns come from the batch tables (gridopt_kR_tables_kernel) into LDS
-// buffers that alternate between lines, so one barrier per line separates a line's staging from the
-// previous line's reads; partial tiles alternate likewise, one barrier per tile.
+// partial tiles are summed through LDS. The line's phases and U columns come from the batch tables
+// (gridopt_kR_tables_kernel) into LDS buffers that alternate between lines, so one barrier per line
+// separates a line's staging from the previous line's reads; partial tiles alternate likewise, one
+// barrier per tile.
 template <int N>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 : 2))) void
 gridopt_kR_project_mfma_kernel(int64_t nd, const cplx<float> *__restrict__ op1, int64_t k0, int64_t k1, int l0,
@@ -407,7 +408,8 @@ gridopt_kR_project_mfma_kernel(int64_t nd, const cplx<float> *__restrict__ op1, 
                 ur = w.x;
                 ui = w.y;
             }
-            // W op2 with three real products (Gauss): re = wr o2r - wi o2i, im = (wr + wi)(o2r + o2i) - wr o2r - wi o2i
+            // W op2 from three real products (Gauss):
+            //   re = wr o2r - wi o2i,  im = (wr + wi)(o2r + o2i) - wr o2r - wi o2i
             f32x4 prr = {0.f, 0.f, 0.f, 0.f}, pii = prr, pss = prr;
 #pragma unroll
             for (int c = 0; c < N; c++) {
@@ -595,8 +597,9 @@ __global__ __launch_bounds__(256) void kq_line_offsets_kernel(int nunits, const 
 constexpr int GQ_M3_MAX = 64, GQ_SLAB_MAX = 1024;
 
 // One workgroup per (q1, unit), one thread per row G < nw nm (threads past it only help stage the
-// tables). Rp box: (p1min + a, p2min + b, p3min + c), a < n1, b, c < N; rmap[b + 8 (a + n1 c)] = Rp index or
-// -1 (b fastest, rows padded to 8, so the N indices of one (a, c) come in one scalar load). Stage 1 (registers): B1[b][c] = sum_a e^{2 pi i (q1/m1)(p1min + a)} A[u][G + ldr Rp]. The kept q of
+// tables). Rp box: (p1min + a, p2min + b, p3min + c), a < n1, b, c < N; rmap[b + 8 (a + n1 c)] = Rp
+// index or -1 (b fastest, rows padded to 8, so the N indices of one (a, c) come in one scalar load).
+// Stage 1 (registers): B1[b][c] = sum_a e^{2 pi i (q1/m1)(p1min + a)} A[u][G + ldr Rp]. The kept q of
 // the (u, q1) slab are taken in chunks of whole q lines with at most GQ_CHUNK entries (a line with more
 // kept q is split into pieces of GQ_CHUNK); per chunk, per
 // kept q (in line order; B2[c] = sum_b e2 B1[b][c] is recomputed when the line changes): g = sum_c e3 B2[c];
