@@ -295,6 +295,41 @@ def test_outer_k_gridopt_kq(backend, model16, nq, window, nb):
     assert rel < 1e-4, rel
 
 
+@pytest.mark.parametrize("box_g", [5])
+def test_outer_k_gridopt_box_faces(backend, box_g):
+    """The staged kernels on 5^3 e-ph R boxes (other instantiations of the stage-2/3 and kR->kq kernels
+    than the 3^3 and 7^3 boxes elsewhere): unit operators and g2 agree with the direct FP32 kernels to
+    1e-5 and g2 with the oracle to 1e-4."""
+    from ephip import _lib as L
+    from ephip import eph
+    m = M.make_model(nw=16, natoms=4, box_hd=5, box_g=box_g, seed=2025 + box_g, g_dtype=np.complex128)
+    g = {"kgrid": (6, 6, 6), "qgrid": (3, 3, 3), "window": 0.5}
+    w = g["window"]
+    st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
+    nks = st.ksurv.shape[0]
+    nb = 50
+    staged = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32)
+    direct = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32, gridopt=False)
+    assert staged.gridopt is not None and staged.gridopt.mfma and staged.gridopt_q is not None
+    assert staged.gridopt.np == max(box_g, 3)
+    rs, rd = eph.UnitRecorder(), eph.UnitRecorder()
+    for i0 in (0, max(0, nks - nb)):
+        staged.run_kbatch(i0, nb, [rs])
+        direct.run_kbatch(i0, nb, [rd])
+        nu = int(staged.k_unit_start[min(i0 + nb, nks)] - staged.k_unit_start[i0])
+        a, b = staged.A[:nu].cpu().numpy(), direct.A[:nu].cpu().numpy()
+        assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)
+    num = den = 0.0
+    for bs, bd in zip(rs.batches, rd.batches):
+        for u in range(len(bs["cols"])):
+            assert np.array_equal(bs["cols"][u], bd["cols"][u])
+            num = max(num, float(np.abs(bs["g2"][u] - bd["g2"][u]).max(initial=0.0)))
+            den = max(den, float(np.abs(bd["g2"][u]).max(initial=0.0)))
+    assert den > 0 and num / den < 1e-5, num / den
+    rel = _outer_k_vs_oracle(m, g, st, rs, qstep=7)
+    assert rel < 1e-4, rel
+
+
 @pytest.mark.parametrize("window", [0.5, 0.8])
 def test_outer_k_gridopt_kq_long_lines(backend, model16, window):
     """Staged kR->kq on a q grid whose lines hold more kept q than one chunk of the kernel (32): k and q
