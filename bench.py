@@ -364,7 +364,9 @@ def main():
                                              if getattr(loop, "fused_proj", False) else
                                              T.get_variant(T.KIND_PACKED_BF16)),
                                variant_krkq="gridopt" if staged_kq else T.get_variant(T.KIND_FUSED_ROT))
-        log(f"[bench] fast outer-k {args.precision}: {results['fast']['value']:.3e} pairs/s; RR->kR "
+        run_as = ("fp32 staged" if staged_rr and staged_kq else
+                  "+".join(sorted({"fp32 staged" if st_ else args.precision for st_ in (staged_rr, staged_kq)})))
+        log(f"[bench] fast outer-k ({run_as}): {results['fast']['value']:.3e} pairs/s; RR->kR "
             f"{results['fast']['rrkr_tflops']:.1f} TFLOP/s ({results['fast']['rrkr_avg_ms']:.2f} ms), kR->kq "
             f"{results['fast']['krkq_tflops']:.1f} TFLOP/s ({results['fast']['krkq_avg_ms']:.2f} ms)")
 
