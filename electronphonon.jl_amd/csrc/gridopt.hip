@@ -627,8 +627,11 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
     // an even column count is summed in pairs fed in lane order (dpp_row_sum4_pairs); an odd one with
     // dpp_row_sum4 (padding it to pairs costs more in staging than the selects it saves)
     constexpr bool PAIRS = NBW % 2 == 0;
-    __shared__ float2 ucol[GQ_CHUNK * NBW * NW];
-    __shared__ float2 tq[GQ_CHUNK * NBW * NW];  // t[kk][m][nu], nu < nm <= 16
+    // chunk size: GQ_CHUNK kept q, fewer for wide windows so that four workgroups still fit a CU's LDS
+    // (cfg4, window width 3: 12.6 instead of 15.7 ms per 1152-k batch)
+    constexpr int CHUNK = NBW >= 4 ? GQ_CHUNK / 2 : NBW == 3 ? GQ_CHUNK * 3 / 4 : GQ_CHUNK;
+    __shared__ float2 ucol[CHUNK * NBW * NW];
+    __shared__ float2 tq[CHUNK * NBW * NW];  // t[kk][m][nu], nu < nm <= 16
     const int q1 = blockIdx.x, u = blockIdx.y;
     const int t = threadIdx.x;
     const int nd2 = NW * nm;
@@ -692,11 +695,11 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
     int e0 = 0, lc = 0;  // chunk start in the slab tables and the line that holds it
     while (e0 < nslab) {  // uniform
         while (slo[lc + 1] - slab0 <= e0) lc++;
-        // the chunk: whole lines up to GQ_CHUNK entries, or the next GQ_CHUNK entries of a longer line
-        const int e1 = min(e0 + GQ_CHUNK, nslab);
+        // the chunk: whole lines up to CHUNK entries, or the next CHUNK entries of a longer line
+        const int e1 = min(e0 + CHUNK, nslab);
         int cut = e1;
         for (int l = lc + 1; l <= m2 && slo[l] - slab0 <= e1; l++) cut = slo[l] - slab0;
-        const int ks = e0, nk = cut - e0;  // 0 < nk <= GQ_CHUNK
+        const int ks = e0, nk = cut - e0;  // 0 < nk <= CHUNK
         e0 = cut;
         __syncthreads();  // slab tables written; the previous chunk's tables no longer read
         for (int e = t; e < nk * NBW * NW; e += blockDim.x) {
@@ -710,7 +713,7 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
             ucol[e] = w;
         }
         __syncthreads();  // U columns staged
-        // lane kk keeps entry kk's metadata word (nk <= GQ_CHUNK < 64); the loop takes it with readlane
+        // lane kk keeps entry kk's metadata word (nk <= CHUNK < 64); the loop takes it with readlane
         const int meta = lane < nk ? sk_nb[ks + lane] : 0;
         float b2r[N], b2i[N];
         int cur = -1;
