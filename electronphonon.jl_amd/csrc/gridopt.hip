@@ -534,9 +534,10 @@ extern "C" int ephip_gridopt_kR_tables(int32_t nw, int32_t np, int32_t r2min, in
 
 extern "C" int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp, int32_t np, const void *op1,
                                              int64_t k0, int64_t k1, int32_t l0, int32_t l1, int32_t lb0,
-                                             const int32_t *line_start, int32_t max_line_k, const int32_t *kus,
-                                             const void *ucol, const void *ph2, const void *ph3, int32_t nbw,
-                                             void *A, int64_t lda, void *stream) {
+                                             const int32_t *line_start, int32_t max_line_k,
+                                             int32_t max_line_units, const int32_t *kus, const void *ucol,
+                                             const void *ph2, const void *ph3, int32_t nbw, void *A, int64_t lda,
+                                             void *stream) {
     EPHIP_REQUIRE(nw == 16, "gridopt_kR_project_mfma: nw must be 16 (got %d)", nw);
     EPHIP_REQUIRE(nm >= 1 && nrp >= 1 && nbw >= 1 && nbw <= GO_NBW_MAX && k0 >= 0 && k1 >= k0 && l1 >= l0 &&
                       l0 >= lb0 && lb0 >= 0,
@@ -553,7 +554,9 @@ extern "C" int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp
     static const K kern[6] = {gridopt_kR_project_mfma_kernel<3>, gridopt_kR_project_mfma_kernel<4>,
                               gridopt_kR_project_mfma_kernel<5>, gridopt_kR_project_mfma_kernel<6>,
                               gridopt_kR_project_mfma_kernel<7>, gridopt_kR_project_mfma_kernel<8>};
-    const int max_pairs = max(max_line_k, 1) * nbw;
+    EPHIP_REQUIRE(max_line_units >= 0 && max_line_units <= max_line_k * nbw,
+                  "gridopt_kR_project_mfma: max_line_units %d > max_line_k * nbw", max_line_units);
+    const int max_pairs = max_line_units > 0 ? max_line_units : max(max_line_k, 1) * nbw;
     const size_t lds = 2 * (size_t)max_pairs * nw * sizeof(float2);  // U columns of a line's pairs, two buffers
     kern[np - 3]<<<(unsigned)(nd / 256), 256, lds, as_stream(stream)>>>(
         nd, static_cast<const cplx<float> *>(op1), k0, k1, l0, l1, lb0, line_start, max_pairs, kus,

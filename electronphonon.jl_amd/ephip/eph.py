@@ -710,6 +710,9 @@ class _GridOptKR:
         self.line_plane = i1[starts[:-1]]
         self.line_of_k = np.repeat(np.arange(len(starts) - 1), np.diff(starts))
         self.max_line_k = int(np.diff(starts).max())
+        nb_k = (st.el.filt.band_max.cpu().numpy()[ks] - st.el.filt.band_min.cpu().numpy()[ks] + 1).astype(np.int64)
+        cum = np.concatenate([[0], np.cumsum(nb_k)])
+        self.max_line_units = int((cum[starts[1:]] - cum[starts[:-1]]).max())  # units of the fullest line
         x1 = st.el.xk.index_select(0, st.ksurv).cpu().numpy()[:, 0]
         # planes with surviving k in order, their k1 coordinates, and the stage-1 group of each
         first_line = np.flatnonzero(np.concatenate([[True], np.diff(self.line_plane) != 0]))
@@ -766,7 +769,8 @@ class _GridOptKR:
             buf = self._plane_buffer(int(plane), loop.epmat_el, stream)
             if self.mfma:
                 L.call("ephip_gridopt_kR_project_mfma", st.nw, st.nm, loop.nrp, self.np, L.ptr(buf), i0, i0 + n, l,
-                       l_end, l_first, L.ptr(self.line_start), self.max_line_k, L.ptr(loop.k_unit_start_dev),
+                       l_end, l_first, L.ptr(self.line_start), self.max_line_k, self.max_line_units,
+                       L.ptr(loop.k_unit_start_dev),
                        L.ptr(ucol), L.ptr(ph2), L.ptr(ph3), loop.max_units_per_k, L.ptr(A), A.stride(0), stream)
             else:
                 L.call("ephip_gridopt_kR_project", st.nw, st.nm, loop.nrp, self.np, self.rmin[1], self.rmin[2],

@@ -254,15 +254,17 @@ int ephip_gridopt_kR_project(int32_t nw, int32_t nm, int32_t nrp, int32_t np, in
  * ph3 (complex64, (k1 - k0) x np): e^{2 pi i k3 (r3min + c)}; ph2 (complex64, (lb1 - lb0) x np):
  * e^{2 pi i k2 (r2min + b)} of each line. ephip_gridopt_kR_project_mfma then covers the lines
  * lb0 <= l0 <= l < l1 of one k1 plane (as ephip_gridopt_kR_project; its units are consecutive, at most
- * nbw per k). Output as ephip_gridopt_kR_project. */
+ * nbw per k). max_line_units (<= max_line_k*nbw; 0 means max_line_k*nbw) bounds the units of any line and
+ * sizes the kernel's LDS. Output as ephip_gridopt_kR_project. */
 int ephip_gridopt_kR_tables(int32_t nw, int32_t np, int32_t r2min, int32_t r3min, const double *xk, int64_t k0,
                             int64_t k1, int32_t lb0, int32_t lb1, const int32_t *line_start, int32_t nunits,
                             const int64_t *ustate, const int32_t *uband, const void *U, void *ucol, void *ph2,
                             void *ph3, void *stream);
 int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp, int32_t np, const void *op1, int64_t k0,
                                   int64_t k1, int32_t l0, int32_t l1, int32_t lb0, const int32_t *line_start,
-                                  int32_t max_line_k, const int32_t *kus, const void *ucol, const void *ph2,
-                                  const void *ph3, int32_t nbw, void *A, int64_t lda, void *stream);
+                                  int32_t max_line_k, int32_t max_line_units, const int32_t *kus, const void *ucol,
+                                  const void *ph2, const void *ph3, int32_t nbw, void *A, int64_t lda,
+                                  void *stream);
 
 /* Staged (GridOpt) kR->kq over Rp for the q of a uniform grid qgrid = (m1, m2, m3) (q index
  * (q1*m2 + q2)*m3 + q3 at q = (q1/m1, q2/m2, q3/m3)), fused with the k+q-window and phonon rotations and
