@@ -124,33 +124,31 @@ __device__ __forceinline__ float dpp_group_sum(float v) {
     return v;
 }
 
+// The value of x in lane l ^ 8 (row rotation by 8), 7 - l within each half row (half-row mirror),
+// l ^ 2 and l ^ 1 (quad permutations), inside each 16-lane row.
+constexpr int DPP_ROR8 = 0x128, DPP_HALF_MIRROR = 0x141, DPP_QUAD_2301 = 0x4E, DPP_QUAD_1032 = 0xB1;
+template <int CTRL>
+__device__ __forceinline__ float dpp_take(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF, 0xF,
+                                                                 false));
+}
+
 // Row sums of four values at once, scattered over the 16-lane row: on return lanes 4s .. 4s + 3 of every
 // row hold the row sum of the s-th argument. A reduce-scatter: lanes split by bit 3 (partner l ^ 8, row
 // rotation by 8) and by bit 2 (partner l ^ 7, half-row mirror) keep one half of their values each, then
 // two quad permutations (partners l ^ 2, l ^ 1) finish the sums; {8, 7, 2, 1} span all 16 lanes. Eleven
 // VALU operations where four separate row sums take sixteen DPP additions and their moves.
 __device__ __forceinline__ float dpp_row_sum4(float a, float b, float c, float d) {
-    auto dpp = [](float x, int ctrl) {
-        const int xi = __builtin_bit_cast(int, x);
-        int y;
-        switch (ctrl) {
-            case 0: y = __builtin_amdgcn_update_dpp(0, xi, 0x128, 0xF, 0xF, false); break;  // row_ror:8
-            case 1: y = __builtin_amdgcn_update_dpp(0, xi, 0x141, 0xF, 0xF, false); break;  // row_half_mirror
-            case 2: y = __builtin_amdgcn_update_dpp(0, xi, 0x4E, 0xF, 0xF, false); break;   // quad_perm [2,3,0,1]
-            default: y = __builtin_amdgcn_update_dpp(0, xi, 0xB1, 0xF, 0xF, false); break;  // quad_perm [1,0,3,2]
-        }
-        return __builtin_bit_cast(float, y);
-    };
     const int l = threadIdx.x & 15;
     const bool h3 = (l & 8) != 0, h2 = (l & 4) != 0;
     float x0 = h3 ? c : a, y0 = h3 ? a : c, x1 = h3 ? d : b, y1 = h3 ? b : d;
-    x0 += dpp(y0, 0);
-    x1 += dpp(y1, 0);
+    x0 += dpp_take<DPP_ROR8>(y0);
+    x1 += dpp_take<DPP_ROR8>(y1);
     float x = h2 ? x1 : x0;
     const float y = h2 ? x0 : x1;
-    x += dpp(y, 1);
-    x += dpp(x, 2);
-    x += dpp(x, 3);
+    x += dpp_take<DPP_HALF_MIRROR>(y);
+    x += dpp_take<DPP_QUAD_2301>(x);
+    x += dpp_take<DPP_QUAD_1032>(x);
     return x;
 }
 
@@ -160,25 +158,26 @@ __device__ __forceinline__ float dpp_row_sum4(float a, float b, float c, float d
 // g arranged by lane bit 2: (p1, p2) = (gr, gi) or (gi, -gr), the part the lane keeps (real or
 // imaginary), and (q1, q2) the other; then w.x p1 + w.y p2 is that part of conj(w) g.
 __device__ __forceinline__ float dpp_row_sum4_pairs(float2 wo, float2 wx, float p1, float p2, float q1, float q2) {
-    auto dpp = [](float x, int ctrl) {
-        const int xi = __builtin_bit_cast(int, x);
-        int y;
-        switch (ctrl) {
-            case 0: y = __builtin_amdgcn_update_dpp(0, xi, 0x128, 0xF, 0xF, false); break;  // row_ror:8
-            case 1: y = __builtin_amdgcn_update_dpp(0, xi, 0x141, 0xF, 0xF, false); break;  // row_half_mirror
-            case 2: y = __builtin_amdgcn_update_dpp(0, xi, 0x4E, 0xF, 0xF, false); break;   // quad_perm [2,3,0,1]
-            default: y = __builtin_amdgcn_update_dpp(0, xi, 0xB1, 0xF, 0xF, false); break;  // quad_perm [1,0,3,2]
-        }
-        return __builtin_bit_cast(float, y);
-    };
     float x0 = wo.x * p1 + wo.y * p2;  // own column, kept part
     float x1 = wo.x * q1 + wo.y * q2;  // own column, other part
     const float y0 = wx.x * p1 + wx.y * p2, y1 = wx.x * q1 + wx.y * q2;
-    x0 += dpp(y0, 0);  // partner l ^ 8 holds the other column as its own, same parts
-    x1 += dpp(y1, 0);
-    float x = x0 + dpp(x1, 1);  // partner l ^ 7 keeps the other part
-    x += dpp(x, 2);
-    x += dpp(x, 3);
+    x0 += dpp_take<DPP_ROR8>(y0);  // partner l ^ 8 holds the other column as its own, same parts
+    x1 += dpp_take<DPP_ROR8>(y1);
+    float x = x0 + dpp_take<DPP_HALF_MIRROR>(x1);  // partner l ^ 7 keeps the other part
+    x += dpp_take<DPP_QUAD_2301>(x);
+    x += dpp_take<DPP_QUAD_1032>(x);
+    return x;
+}
+
+// The same for one column: lanes with bit 3 clear end with the row sum of re(conj(w) g), lanes with
+// bit 3 set with im; (p1, p2) is the lane's part of g ((gr, gi) or (gi, -gr) by bit 3), (q1, q2) the other.
+__device__ __forceinline__ float dpp_row_sum2(float2 w, float p1, float p2, float q1, float q2) {
+    float x = w.x * p1 + w.y * p2;
+    const float y = w.x * q1 + w.y * q2;
+    x += dpp_take<DPP_ROR8>(y);  // partner l ^ 8 keeps the other part
+    x += dpp_take<DPP_HALF_MIRROR>(x);
+    x += dpp_take<DPP_QUAD_2301>(x);
+    x += dpp_take<DPP_QUAD_1032>(x);
     return x;
 }
 
@@ -765,9 +764,20 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
             float gr = 0.f, gi = 0.f;
 #pragma unroll
             for (int c = 0; c < N; c++) cmac(gr, gi, e3[c].x, e3[c].y, b2r[c], b2i[c]);
-            // conj(w) g for the column pairs, summed over the 16 lanes of the row; lanes 4s of the row
-            // write value s of each pair (re, im of its first column, re, im of its second)
+            // conj(w) g summed over the 16 lanes of the row. Even widths: the column pairs, lanes 4s of
+            // the row write value s of each pair (re, im of its first column, re, im of its second). Odd
+            // widths: one column when the k+q window holds one band (uniform; most kept q), lanes 0 and 8
+            // of the row writing its re and im, else dpp_row_sum4 over the columns (a one-column path
+            // for even widths measured slower than the pair sums it skips)
             float *tqf = reinterpret_cast<float *>(tq);
+            if constexpr (!PAIRS) {
+                if ((w23 & 0xff) <= 1) {
+                    const float p1 = h3 ? gi : gr, p2 = h3 ? -gr : gi, q1 = h3 ? gr : gi, q2 = h3 ? gi : -gr;
+                    const float x = dpp_row_sum2(w[0], p1, p2, q1, q2);
+                    if (rv && (i & 7) == 0) tqf[(kk * NBW * NW + nu) * 2 + h3] = x;
+                    continue;
+                }
+            }
             if constexpr (PAIRS) {
                 const bool h2 = (i & 4) != 0;
                 const float gp1 = h2 ? gi : gr, gp2 = h2 ? -gr : gi, gq1 = h2 ? gr : gi, gq2 = h2 ? gi : -gr;
@@ -800,6 +810,12 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
             for (int e = t; e < nk * NBW * NMC; e += blockDim.x) {
                 const int kk = e / (NBW * NMC), m = (e / NMC) % NBW, nu2 = e % NMC;
                 if (m >= nbw) continue;
+                if constexpr (!PAIRS) {  // past the k+q window: zero (its t may not have been formed)
+                    if (m >= (sk_nb[ks + kk] & 0xff)) {
+                        g2[(row0 + slab0 + ks + kk) * nbw * NMC + m * NMC + nu2] = 0.f;
+                        continue;
+                    }
+                }
                 const int iq = sk_iq[ks + kk];
                 const cplx<float> *up = uph + (int64_t)NMC * NMC * iq + NMC * nu2;
                 float er = 0.f, ei = 0.f;
@@ -815,6 +831,12 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
         } else {
             for (int e = t; e < nk * nbw * nm; e += blockDim.x) {
                 const int kk = e / (nbw * nm), rr = e % (nbw * nm), m = rr / nm, nu2 = rr % nm;
+                if constexpr (!PAIRS) {  // past the k+q window: zero (its t may not have been formed)
+                    if (m >= (sk_nb[ks + kk] & 0xff)) {
+                        g2[(row0 + slab0 + ks + kk) * nbw * nm + rr] = 0.f;
+                        continue;
+                    }
+                }
                 const int iq = sk_iq[ks + kk];
                 const cplx<float> *up = uph + (int64_t)nm * nm * iq + nm * nu2;
                 float er = 0.f, ei = 0.f;
