@@ -47,44 +47,47 @@ __global__ void project_kR_kernel(int nw, int nm, int nrp, int nunits, const c32
 
 // Per unit: the q of the list whose k+q (on the k grid) has an in-window band, in q order.
 // col_index[u*cs + c] = q, ikq[u*cs + c] = state index of k+q, count[u] = number kept.
-// One workgroup of 256 threads per unit; positions from wave ballots + a 4-wave prefix.
+// One workgroup of 256 threads per unit; positions from wave ballots + a 4-wave prefix. The wave counts
+// alternate between two LDS slots, so one barrier per 256 q separates their writes from their reads
+// (a slot is rewritten two iterations later, after the next barrier). Grid coordinates in int32
+// (|rint(x n)| < 2^31 for the reduced coordinates of a k-grid the size check admits).
 __global__ __launch_bounds__(256) void kq_compact_kernel(int nunits, const int64_t *__restrict__ ustate,
                                                         const double *__restrict__ xk, int nq,
                                                         const double *__restrict__ xq, int n1, int n2, int n3,
                                                         const int64_t *__restrict__ bmin, int64_t cs,
                                                         int *__restrict__ col_index, int *__restrict__ ikq,
                                                         int *__restrict__ count) {
-    __shared__ int wave_cnt[4];
+    __shared__ int wave_cnt[2][4];
     const int u = blockIdx.x;
     if (u >= nunits) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t s = ustate[u];
     const double k0 = xk[3 * s], k1 = xk[3 * s + 1], k2 = xk[3 * s + 2];
     auto red = [](double x, int n) {
-        int64_t c = (int64_t)rint(x * n) % n;
+        const int c = (int)rint(x * n) % n;
         return c < 0 ? c + n : c;
     };
-    int base = 0;
+    int base = 0, slot = 0;
     for (int c0 = 0; c0 < nq; c0 += 256) {
         const int c = c0 + tid;
         bool keep = false;
         int h = 0;
         if (c < nq) {
-            h = (int)((red(k0 + xq[3 * c], n1) * n2 + red(k1 + xq[3 * c + 1], n2)) * n3 + red(k2 + xq[3 * c + 2], n3));
+            h = (red(k0 + xq[3 * c], n1) * n2 + red(k1 + xq[3 * c + 1], n2)) * n3 + red(k2 + xq[3 * c + 2], n3);
             keep = bmin[h] > 0;
         }
         const unsigned long long ball = __ballot(keep);
         const int before = __popcll(ball & ((1ull << lane) - 1ull));
-        if (lane == 0) wave_cnt[wave] = __popcll(ball);
+        if (lane == 0) wave_cnt[slot][wave] = __popcll(ball);
         __syncthreads();
         int off = base;
-        for (int w = 0; w < wave; w++) off += wave_cnt[w];
+        for (int w = 0; w < wave; w++) off += wave_cnt[slot][w];
         if (keep) {
             col_index[u * cs + off + before] = c;
             ikq[u * cs + off + before] = h;
         }
-        base += wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
-        __syncthreads();
+        base += wave_cnt[slot][0] + wave_cnt[slot][1] + wave_cnt[slot][2] + wave_cnt[slot][3];
+        slot ^= 1;
     }
     if (tid == 0) count[u] = base;
 }
