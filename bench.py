@@ -255,18 +255,22 @@ def main():
 
     def kept_pairs(ks):
         """Number of (unit, q) columns the outer-k path computes for the surviving-k positions ks
-        (host arithmetic on the filter: q kept iff k+q has an in-window band)."""
+        (host arithmetic on the filter: q kept iff k+q has an in-window band), and the number of
+        (unit, q, k+q window band) slots among them (the rotations' algorithmic extent)."""
         bmin = st.el.filt.band_min.cpu().numpy()
         bmax = st.el.filt.band_max.cpu().numpy()
         ksurv = st.ksurv.cpu().numpy()
         xk = M.kpoints_grid(kgrid)
         xq = M.kpoints_grid(grids["qgrid"])
-        tot = 0
+        tot = slots = 0
         for p in ks:
             k = ksurv[p]
             h = O_hash(xk[k] + xq, kgrid)
-            tot += int((bmin[h] > 0).sum()) * int(bmax[k] - bmin[k] + 1)
-        return tot
+            kept_h = h[bmin[h] > 0]
+            nbk = int(bmax[k] - bmin[k] + 1)
+            tot += len(kept_h) * nbk
+            slots += int(np.minimum(bmax[kept_h] - bmin[kept_h] + 1, st.nbw).sum()) * nbk
+        return tot, slots
 
     def O_hash(x, ng):
         c = np.mod(np.rint(x * np.asarray(ng)).astype(np.int64), np.asarray(ng))
@@ -338,8 +342,8 @@ def main():
         # algorithmic flops of the launches actually made (direct or staged kernels), step by step
         cols, f_rrkr_total, f_krkq_total = 0, 0.0, 0.0
         for i0 in timed:
-            kept = kept_pairs(range(i0, min(i0 + nks_steps, nks)))
-            fr, fk = loop.kernel_flops(i0, nks_steps, kept)
+            kept, slots = kept_pairs(range(i0, min(i0 + nks_steps, nks)))
+            fr, fk = loop.kernel_flops(i0, nks_steps, kept, slots)
             cols += kept
             f_rrkr_total += fr
             f_krkq_total += fk

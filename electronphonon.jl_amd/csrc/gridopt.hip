@@ -806,27 +806,28 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
         }
         __syncthreads();  // t of every kept q of the chunk
         if constexpr (NMC > 0) {
-            // entries (kk, slot m < NBW, mode): compile-time index arithmetic; slots m >= nbw skipped
-            for (int e = t; e < nk * NBW * NMC; e += blockDim.x) {
-                const int kk = e / (NBW * NMC), m = (e / NMC) % NBW, nu2 = e % NMC;
-                if (m >= nbw) continue;
-                if constexpr (!PAIRS) {  // past the k+q window: zero (its t may not have been formed)
-                    if (m >= (sk_nb[ks + kk] & 0xff)) {
-                        g2[(row0 + slab0 + ks + kk) * nbw * NMC + m * NMC + nu2] = 0.f;
+            // entries (slot m, kk, mode) with the slot outermost: past the k+q window (most entries of the
+            // slots m >= 1) a whole wave skips the rotation and writes zeros; compile-time index arithmetic
+            for (int m = 0; m < nbw; m++) {  // uniform
+                for (int e = t; e < nk * NMC; e += blockDim.x) {
+                    const int kk = e / NMC, nu2 = e % NMC;
+                    float* out = g2 + (row0 + slab0 + ks + kk) * nbw * NMC + m * NMC + nu2;
+                    if (m >= (sk_nb[ks + kk] & 0xff)) {  // past the k+q window: zero
+                        *out = 0.f;
                         continue;
                     }
-                }
-                const int iq = sk_iq[ks + kk];
-                const cplx<float> *up = uph + (int64_t)NMC * NMC * iq + NMC * nu2;
-                float er = 0.f, ei = 0.f;
+                    const int iq = sk_iq[ks + kk];
+                    const cplx<float> *up = uph + (int64_t)NMC * NMC * iq + NMC * nu2;
+                    float er = 0.f, ei = 0.f;
 #pragma unroll
-                for (int n = 0; n < NMC; n++) {  // constant offsets from one base address
-                    const float2 tv = tq[(kk * NBW + m) * NW + n];
-                    const cplx<float> p = up[n];
-                    cmac(er, ei, tv.x, tv.y, p.re, p.im);
+                    for (int n = 0; n < NMC; n++) {  // constant offsets from one base address
+                        const float2 tv = tq[(kk * NBW + m) * NW + n];
+                        const cplx<float> p = up[n];
+                        cmac(er, ei, tv.x, tv.y, p.re, p.im);
+                    }
+                    const float w2 = 2.f * static_cast<float>(omega[(int64_t)NMC * iq + nu2]);
+                    *out = (er * er + ei * ei) / w2;
                 }
-                const float w2 = 2.f * static_cast<float>(omega[(int64_t)NMC * iq + nu2]);
-                g2[(row0 + slab0 + ks + kk) * nbw * NMC + m * NMC + nu2] = (er * er + ei * ei) / w2;
             }
         } else {
             for (int e = t; e < nk * nbw * nm; e += blockDim.x) {

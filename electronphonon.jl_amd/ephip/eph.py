@@ -608,17 +608,19 @@ class OuterKLoop:
             c.run(payload)
         return payload
 
-    def kernel_flops(self, i0: int, n: int, kept: int) -> tuple[float, float]:
+    def kernel_flops(self, i0: int, n: int, kept: int, kept_slots: int | None = None) -> tuple[float, float]:
         """Algorithmic flops (8 per complex multiply-add) of the batch's RR->kR and kR->kq launches for the
-        kernels in use; kept = the batch's kept (unit, q) pairs. Direct Fourier kernels: the full phase
-        sums. Staged (GridOpt) kernels: their own stages, the per-plane stage 1 of RR->kR counted once
-        per surviving k of its plane, the stage 2 of kR->kq for every q line of every unit."""
+        kernels in use; kept = the batch's kept (unit, q) pairs, kept_slots = their (unit, q, k+q window
+        band) slots (default: nbw per pair). Direct Fourier kernels: the full phase sums. Staged (GridOpt)
+        kernels: their own stages, the per-plane stage 1 of RR->kR counted once per surviving k of its
+        plane, the stage 2 of kR->kq for every q line of every unit. Rotations: per k+q window band."""
         st = self.st
         nw, nm = st.nw, st.nm
         n = min(n, st.ksurv.shape[0] - i0)
         nu = int(self.k_unit_start[i0 + n] - self.k_unit_start[i0])
         nd2 = nw * nm
-        rot = 8.0 * (nw * st.nbw * nm + st.nbw * nm * nm) * kept
+        slots = st.nbw * kept if kept_slots is None else kept_slots
+        rot = 8.0 * (nw * nm + nm * nm) * slots
         if self.gridopt is not None:
             go = self.gridopt
             planes = go.line_plane[go.line_of_k[i0:i0 + n]]
