@@ -41,8 +41,8 @@ PEAK = {"fp32": 157.3, "bf16x3": 2500.0}  # TFLOP/s, MI355X dense MFMA (fp32, bf
 # profiles/cfg2_fast_bf16_mfma_hbm_counters.md). Counters cannot be read
 # inside the timed run; a variant or batch without a measurement reports traffic null.
 TRAFFIC_BYTES_CFG2 = {
-    ("fp32", "kR->kq", 1152, "gridopt"): (3326982 * 2 + 622497) * 1024,  # staged kR->kq + rotations + g2
-    ("fp32", "RR->kR", 1152, "gridopt"): (204754 * 2 + 268683) * 1024,   # staged RR->kR projection launch (MFMA)
+    ("fp32", "kR->kq", 1152, "gridopt"): (3324624 * 2 + 624898) * 1024,  # staged kR->kq + rotations + g2
+    ("fp32", "RR->kR", 1152, "gridopt"): (204774 * 2 + 268683) * 1024,   # staged RR->kR projection launch (MFMA)
     ("fp32", "kR->kq", 768, "gridopt"): (2186567 * 2 + 414734) * 1024,
     ("fp32", "RR->kR", 768, "gridopt"): (204184 * 2 + 230276) * 1024,
     ("bf16x3", "kR->kq", 384, 10): (2998082 * 2 + 204444) * 1024,   # fused kR->kq + rotations, 8 waves
