@@ -330,6 +330,44 @@ def test_outer_k_gridopt_box_faces(backend, box_g):
     assert rel < 1e-4, rel
 
 
+@pytest.mark.parametrize("natoms,window", [(3, 0.5), (3, 0.8), (5, 0.5), (5, 0.8)])
+def test_outer_k_gridopt_mode_counts(backend, natoms, window):
+    """The staged kernels at mode counts other than 12 (nm 9 and 15: the kR->kq kernel's run-time mode
+    loop, the MFMA projection over nm blocks of 16 rows; nm 15 fills 240 of 256 threads), window widths
+    2 (column pairs) and 3 (odd path): unit operators and g2 agree with the direct FP32 kernels to 1e-5
+    and g2 with the oracle to 1e-4."""
+    from ephip import _lib as L
+    from ephip import eph
+    m = M.make_model(nw=16, natoms=natoms, box_hd=5, box_g=3, seed=3031, g_dtype=np.complex128)
+    assert m.nmodes == 3 * natoms
+    g = {"kgrid": (6, 6, 6), "qgrid": (4, 4, 4), "window": window}
+    w = g["window"]
+    st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
+    assert st.nbw == {0.5: 2, 0.8: 3}[window]
+    nks = st.ksurv.shape[0]
+    nb = 60
+    staged = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32)
+    direct = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32, gridopt=False)
+    assert staged.gridopt is not None and staged.gridopt.mfma and staged.gridopt_q is not None
+    rs, rd = eph.UnitRecorder(), eph.UnitRecorder()
+    for i0 in (0, max(0, nks - nb)):
+        staged.run_kbatch(i0, nb, [rs])
+        direct.run_kbatch(i0, nb, [rd])
+        nu = int(staged.k_unit_start[min(i0 + nb, nks)] - staged.k_unit_start[i0])
+        a, b = staged.A[:nu].cpu().numpy(), direct.A[:nu].cpu().numpy()
+        assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)
+    num = den = 0.0
+    for bs, bd in zip(rs.batches, rd.batches):
+        assert np.array_equal(bs["unit_state"], bd["unit_state"])
+        for u in range(len(bs["cols"])):
+            assert np.array_equal(bs["cols"][u], bd["cols"][u])
+            num = max(num, float(np.abs(bs["g2"][u] - bd["g2"][u]).max(initial=0.0)))
+            den = max(den, float(np.abs(bd["g2"][u]).max(initial=0.0)))
+    assert den > 0 and num / den < 1e-5, num / den
+    rel = _outer_k_vs_oracle(m, g, st, rs, qstep=7)
+    assert rel < 1e-4, rel
+
+
 @pytest.mark.parametrize("window", [0.5, 0.8])
 def test_outer_k_gridopt_kq_long_lines(backend, model16, window):
     """Staged kR->kq on a q grid whose lines hold more kept q than one chunk of the kernel (32): k and q
