@@ -264,11 +264,12 @@ def test_outer_k_gridopt(backend, model16, cfg1, which, nb, mfma):
     assert rel < 1e-4, rel
 
 
-@pytest.mark.parametrize("nq,window,nb", [(4, 0.5, 40), (8, 0.5, 150), (4, 0.8, 40), (8, 1.5, 64)])
+@pytest.mark.parametrize("nq,window,nb", [(4, 0.5, 40), (8, 0.5, 150), (4, 0.8, 40), (8, 1.5, 64), (8, 0.15, 40)])
 def test_outer_k_gridopt_kq(backend, model16, nq, window, nb):
     """kR->kq as the staged (GridOpt) transform over the uniform q grid with the k+q/phonon rotations and
     g2 fused (FP32): kept-q lists and g2 agree with the direct FP32 Fourier + rotation kernels (g2 to
-    1e-5) and with the oracle (1e-4), for nbw = 2, 3 and 4 and q grids of 4^3 and 8^3."""
+    1e-5) and with the oracle (1e-4), for nbw = 1 (one column through the two-column kernel), 2, 3 and 4
+    and q grids of 4^3 and 8^3."""
     from ephip import _lib as L
     from ephip import eph
     m, g = model16
@@ -279,6 +280,7 @@ def test_outer_k_gridopt_kq(backend, model16, nq, window, nb):
     staged = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32)
     direct = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32, gridopt=False)
     assert staged.gridopt_q is not None and staged.gridopt is not None and direct.gridopt_q is None
+    assert st.nbw == {0.15: 1, 0.5: 2, 0.8: 3, 1.5: 4}[window]
     rs, rd = eph.UnitRecorder(), eph.UnitRecorder()
     for i0 in (0, max(0, nks - nb)):
         staged.run_kbatch(i0, nb, [rs])
