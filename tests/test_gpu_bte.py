@@ -5,7 +5,9 @@
   Gaussian smearing 100 meV, occupation convention 5, two temperatures): in-window state set
   (exact), band velocities, S_out, S_in, and the SERTA and iterative-BTE conductivities, all within
   1e-5 relative (Frobenius norm) of the oracle; mobility = sigma / n inherits the bound;
-* the compressed scattering-in product against the dense matrix, and independence of the k batching.
+* the compressed scattering-in product against the dense matrix, and independence of the k batching;
+* the same quantities on the default nw-16 e-ph path (matrix-core RR->kR projection, staged kR->kq)
+  for a 4-atom time-reversal-symmetric model on 6^3 grids, window widths 2 and 3.
 """
 import json
 import os
@@ -164,3 +166,38 @@ def test_mu_from_density(transport, backend):
     bte.bte_compute_mu(p, bs)
     ref = B.find_mu(0.5, 0.025852, s["es"], s["weight"])
     assert abs(p.mulist[0] - ref) <= 1e-9
+
+
+@pytest.mark.parametrize("window", [0.3, 0.5])
+def test_staged_nw16_transport(backend, window):
+    """Transport on the default nw-16 e-ph path (RR->kR projection on the matrix cores, staged kR->kq
+    with the rotations fused; window widths 2 and 3) against the oracle: S_out, S_in, the SERTA and BTE
+    conductivities and the mobility within 1e-5 relative (nw 16, 4 atoms, 6^3 grids)."""
+    from ephip import _lib as L
+    from ephip import bte, core, eph
+    m = M.make_model(nw=16, natoms=4, box_hd=5, box_g=3, seed=2024, g_dtype=np.complex128, time_reversal=True)
+    kgrid = qgrid = (6, 6, 6)
+    st = eph.setup_outer_q(m, kgrid, qgrid, (-window, window), backend)
+    assert st.nbw == {0.3: 2, 0.5: 3}[window]
+    loop = eph.OuterKLoop(st, nk_batch=64, precision=L.PREC_FP32)
+    assert loop.gridopt is not None and loop.gridopt.mfma and loop.gridopt_q is not None
+    params = bte.ElectronTransportParams(Tlist=T_LIST, nlist=[0.1, 0.1], volume=1.0, mulist=MU_LIST)
+    calc = bte.BoltzmannCalculator(params, smearing=("Gaussian", ETA), occupation_method=METHOD)
+    calc = bte.run_boltzmann(loop, calc)
+    vel = core.WannierObject(m.irvec_h, bte.velocity_operator(m.irvec_h, m.ham_r).op_r).to_device(
+        backend, dtype=torch.complex128)
+    bte.compute_velocities(vel, st.el, calc.bs)
+    res = bte.solve_electron_bte(calc, max_iter=600, rtol=1e-10)
+    s = B.states(m, kgrid, (-window, window))
+    v = B.velocities(m, s)
+    So, Si = B.scattering(m, s, qgrid, kgrid, T_LIST, MU_LIST, np.full(2, ETA), method=METHOD)
+    ss, sb, its = B.solve_bte(So, Si, s["es"], v, s["weight"], T_LIST, MU_LIST, max_iter=600, rtol=1e-10)
+    assert np.array_equal(calc.bs.ks.cpu().numpy(), s["ks"])
+    So_d = calc.S_out.cpu().numpy()
+    for it in range(len(T_LIST)):
+        e_out, e_in = _rel(So_d[it], So[it]), _rel(calc.scattering_in_dense(it), Si[it])
+        es, eb = _rel(res.sigma_serta[it], ss[it]), _rel(res.sigma[it], sb[it])
+        print(f"T={T_LIST[it]}: S_out {e_out:.2e} S_in {e_in:.2e} SERTA {es:.2e} BTE {eb:.2e}")
+        assert res.converged[it]
+        assert e_out <= 1e-5 and e_in <= 1e-5 and es <= 1e-5 and eb <= 1e-5
+    assert _rel(bte.transport_mobility(res.sigma, params), bte.transport_mobility(sb, params)) <= 1e-5
