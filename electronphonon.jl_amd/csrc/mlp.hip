@@ -1,7 +1,7 @@
 // Band-energy surrogate: a small MLP (periodic Fourier features of k -> 2 hidden SiLU layers -> band
 // energies) trained on the GPU on exact eigenvalues, then run in BF16 on the matrix cores to prune
 // fine-grid k-points whose predicted bands all lie farther than a safety margin outside the energy
-// window. The exact filter (src/filter.jl:40-100) stays the judge: survivors are re-solved exactly.
+// window. The exact filter (src/filter.jl:40-100) stays the arbiter: survivors are re-solved exactly.
 // No counterpart exists in the reference; this is the engine's own pre-filter.
 //
 // Training kernels (FP32): features, a tiled row-major GEMM with transpose flags (forward, data and

@@ -151,7 +151,7 @@ def test_electron_states_and_filter_bit_exact(backend):
     ham = core.WannierObject(m.irvec_h, m.ham_r).to_device(backend)
     st = eph.compute_electron_states(ham, _dev(xk), (-w, w), chunk=200)
     E = st.E.cpu().numpy()
-    assert np.abs(E - E_ref).max() < 1e-6  # eV; FP64 refinement gives far less
+    assert np.abs(E - E_ref).max() < 1e-6  # eV (the eigenvalue bound); FP64 refinement gives far less
     assert np.abs(E - E_ref).max() < 1e-13 * np.abs(E_ref).max()
     f = st.filt
     assert np.array_equal(f.keep.cpu().numpy(), keep_r)

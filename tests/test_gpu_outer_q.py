@@ -4,7 +4,7 @@
 Gauge-invariant comparison: g2 summed over degenerate subspaces (epstate_g2_degenerate_average!,
 src/EPState.jl:205-238) with the 0.01 meV cutoff, modes below the acoustic cutoff excluded (the
 reference's consumers skip them, src/common/constants.jl:5). Tolerance: norm-wise relative error
-of |g|^2 <= 1e-4 for FP32 and split-BF16."""
+of |g|^2 <= 1e-4 (the e-ph bound) for FP32 and split-BF16."""
 import numpy as np
 import pytest
 
