@@ -153,6 +153,58 @@ __global__ void band_velocity_kernel(int nw, int64_t nstates, const int64_t *__r
     v[3 * s + a] = acc;
 }
 
+// Full rotated operator of one (direction a, k) per workgroup, FP64 (U promoted from complex64):
+// out[m + nw n] = (U_k^H M_a(k) U_k)[m, n], plus the Berry-connection term i (e_m - e_n) rbar[m, n] when the
+// energies and rbar are given. U_k, M_a(k) and T = M_a U_k sit in LDS (nw <= 32: 3 x 16 KB); each thread
+// forms whole entries of T, then of out, in the same order for every k.
+constexpr int VEL_NW_MAX = 32;
+__global__ void __launch_bounds__(256) el_velocity_full_kernel(int nw, const double2 *__restrict__ M, int64_t ld_m,
+                                                               const float2 *__restrict__ U,
+                                                               const double *__restrict__ E,
+                                                               const double2 *__restrict__ rbar, int64_t ld_r,
+                                                               double2 *__restrict__ out, int64_t ld_out) {
+    __shared__ double2 us[VEL_NW_MAX * VEL_NW_MAX], ms[VEL_NW_MAX * VEL_NW_MAX], ts[VEL_NW_MAX * VEL_NW_MAX];
+    const int a = (int)(blockIdx.x % 3);
+    const int64_t k = blockIdx.x / 3;
+    const int n2 = nw * nw;
+    const double2 *mk = M + ld_m * k + (int64_t)a * n2;
+    const float2 *uk = U + (int64_t)n2 * k;
+    for (int e = threadIdx.x; e < n2; e += blockDim.x) {
+        const float2 v = uk[e];
+        us[e] = make_double2(v.x, v.y);
+        ms[e] = mk[e];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n2; e += blockDim.x) {  // T[i, n] = sum_j M[i, j] U[j, n]
+        const int i = e % nw, n = e / nw;
+        double re = 0.0, im = 0.0;
+        for (int j = 0; j < nw; j++) {
+            const double2 x = ms[i + nw * j], y = us[j + nw * n];
+            re += x.x * y.x - x.y * y.y;
+            im += x.x * y.y + x.y * y.x;
+        }
+        ts[e] = make_double2(re, im);
+    }
+    __syncthreads();
+    double2 *ok = out + ld_out * k + (int64_t)a * n2;
+    for (int e = threadIdx.x; e < n2; e += blockDim.x) {  // out[m, n] = sum_i conj(U[i, m]) T[i, n]
+        const int m = e % nw, n = e / nw;
+        double re = 0.0, im = 0.0;
+        for (int i = 0; i < nw; i++) {
+            const double2 x = us[i + nw * m], y = ts[i + nw * n];
+            re += x.x * y.x + x.y * y.y;
+            im += x.x * y.y - x.y * y.x;
+        }
+        if (rbar) {  // + i (e_m - e_n) rbar[m, n]
+            const double de = E[(int64_t)nw * k + m] - E[(int64_t)nw * k + n];
+            const double2 r = rbar[ld_r * k + (int64_t)a * n2 + e];
+            re -= de * r.y;
+            im += de * r.x;
+        }
+        ok[e] = make_double2(re, im);
+    }
+}
+
 // y[i, :] = sum over the unit's rows and k+q slots S_in[row, m, iT] x[f, :] (3 components), one
 // wavefront per unit (i = unit_state_index[u]); lanes take consecutive rows, f = first state of the
 // row's code + m for m below the row's slot count.
@@ -243,6 +295,23 @@ extern "C" int ephip_band_velocity(int32_t nw, int64_t nstates, const int64_t *s
     band_velocity_kernel<<<nbk(nstates * 3), 256, 0, as_stream(stream)>>>(
         nw, nstates, state_col, state_band, state_k, static_cast<const double2 *>(dH), ld_dh,
         static_cast<const float2 *>(U), v);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_el_velocity_direct_batched(int32_t nw, int64_t nk, const void *M, int64_t ld_m, const void *U,
+                                                const double *E, const void *rbar, int64_t ld_r, void *out,
+                                                int64_t ld_out, void *stream) {
+    EPHIP_REQUIRE(nw >= 1 && nw <= VEL_NW_MAX && nk >= 0 && nk <= 0x7fffffffLL / 3,
+                  "el_velocity_direct_batched: nw=%d (1..%d), nk=%lld", nw, VEL_NW_MAX, (long long)nk);
+    EPHIP_REQUIRE(ld_m >= 3LL * nw * nw && ld_out >= 3LL * nw * nw, "el_velocity_direct_batched: ld below 3 nw^2");
+    EPHIP_REQUIRE((E == nullptr) == (rbar == nullptr), "el_velocity_direct_batched: E and rbar come together");
+    EPHIP_REQUIRE(rbar == nullptr || ld_r >= 3LL * nw * nw, "el_velocity_direct_batched: ld_r below 3 nw^2");
+    if (nk == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(M && U && out, "el_velocity_direct_batched: null device pointer");
+    el_velocity_full_kernel<<<(unsigned)(3 * nk), 256, 0, as_stream(stream)>>>(
+        nw, static_cast<const double2 *>(M), ld_m, static_cast<const float2 *>(U), E,
+        static_cast<const double2 *>(rbar), ld_r, static_cast<double2 *>(out), ld_out);
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

@@ -142,6 +142,36 @@ def compute_velocities(vel_dev: WannierObject, el, bs: BandStates, chunk: int = 
     return v
 
 
+def el_velocity_direct_batched(op_dev: WannierObject, el, kidx, rbar=None):
+    """U_k^H M_a(k) U_k for the k of ``kidx`` (indices into ``el``): complex128 (nk, 3, nw, nw) indexed
+    [k, a, m, n], with M the FP64 Fourier of a 3 nw^2-data operator (``velocity_operator`` for the
+    velocity, the position operator for rbar). With ``rbar`` (this function's output for the position
+    operator, same k), the Berry-connection term i (e_m - e_n) rbar[m, n] is added
+    (get_el_velocity_direct_batched, src/wannier_to_bloch_batched.jl:121-148; the Berry-connection
+    update of _compute_electron_states_device!, src/compute_states.jl:149-168)."""
+    torch = _torch()
+    nw = el.nw
+    if op_dev.ndata != 3 * nw * nw:
+        raise L.EphipError(f"operator has ndata={op_dev.ndata}, expected 3*nw^2={3 * nw * nw}")
+    dev = el.E.device
+    kidx = torch.as_tensor(kidx, dtype=torch.int64, device=dev).reshape(-1)
+    nk = kidx.shape[0]
+    out = torch.empty((nk, 3 * nw * nw), dtype=torch.complex128, device=dev)
+    if nk == 0:
+        return out.view(0, 3, nw, nw)
+    M = core.fourier_batched(op_dev, el.xk.index_select(0, kidx).contiguous())
+    U = el.U.index_select(0, kidx).contiguous()
+    E = r = None
+    if rbar is not None:
+        if tuple(rbar.shape) != (nk, 3, nw, nw) or rbar.dtype != torch.complex128:
+            raise L.EphipError(f"rbar must be complex128 ({nk}, 3, {nw}, {nw}), got {tuple(rbar.shape)} {rbar.dtype}")
+        E = el.E.index_select(0, kidx).contiguous()
+        r = rbar.transpose(-1, -2).contiguous().view(nk, 3 * nw * nw)  # back to column-major blocks
+    L.call("ephip_el_velocity_direct_batched", nw, nk, L.ptr(M), M.stride(0), L.ptr(U), L.ptr(E), L.ptr(r),
+           3 * nw * nw, L.ptr(out), out.stride(0), L.stream_handle())
+    return out.view(nk, 3, nw, nw).transpose(-1, -2)
+
+
 # ------------------------------------------------------------------------------------------------
 # Chemical potential (host; a handful of bisection steps over the state energies)
 # ------------------------------------------------------------------------------------------------

@@ -403,6 +403,17 @@ int ephip_band_velocity(int32_t nw, int64_t nstates, const int64_t *state_col, c
                         const int64_t *state_k, const void *dH, int64_t ld_dh, const void *U, double *v,
                         void *stream);
 
+/* Full rotated operator of nk k-points: out[k ld_out + a nw^2 + m + nw n] = (U_k^H M_a(k) U_k)[m, n] in FP64
+ * for the three directions a, with M (complex128, column k of ld_m >= 3 nw^2 entries, direction slowest)
+ * the Fourier of a 3 nw^2-data operator (i R H(R) for the velocity, the position operator for rbar) and U
+ * the nw x nw complex64 eigenvector blocks of the same k; nw <= 32. With E (nw per k) and rbar (same layout
+ * as out, ld_r) both given, the Berry-connection term i (e_m - e_n) rbar[m, n] is added; both NULL gives
+ * the plain rotation. Replaces get_el_velocity_direct_batched (src/wannier_to_bloch_batched.jl:121-148)
+ * and the Berry-connection update of _compute_electron_states_device! (src/compute_states.jl:149-168). */
+int ephip_el_velocity_direct_batched(int32_t nw, int64_t nk, const void *M, int64_t ld_m, const void *U,
+                                     const double *E, const void *rbar, int64_t ld_r, void *out, int64_t ld_out,
+                                     void *stream);
+
 /* Scattering-in product of the iterative BTE for one temperature: y[3*i + a] = sum over the unit's
  * rows rowptr[u]..rowptr[u+1]-1 and slots m < (row_code[row] >> 27) of S_in[(row*nbw + m)*nT + iT]
  * x[3*f + a], f = (row_code[row] & (2^27 - 1)) + m, i = unit_state_index[u] (the

@@ -201,3 +201,65 @@ def test_staged_nw16_transport(backend, window):
         assert res.converged[it]
         assert e_out <= 1e-5 and e_in <= 1e-5 and es <= 1e-5 and eb <= 1e-5
     assert _rel(bte.transport_mobility(res.sigma, params), bte.transport_mobility(sb, params)) <= 1e-5
+
+
+def test_velocity_matrices_berry_connection(backend):
+    """Full velocity matrices (U^H dH U, and with the Berry-connection term i (e_m - e_n) rbar) against
+    mathematics rather than a restated formula: with Wannier centres tau (position operator diag(tau)
+    at R = 0), the Berry-connection velocity must equal U'^H dH'(k) U' in the tau gauge, where
+    H'(k)_ij = sum_R e^{2 pi i k.(R + tau_j - tau_i)} H_ij(R) and U' = diag(e^{-2 pi i k.tau}) U, built
+    here in FP64 numpy (nw 8 and 16, 40 random k, 1e-6 relative). Also: the diagonal of the plain
+    rotation equals ephip_band_velocity, both outputs are Hermitian and tr(rbar) = sum tau."""
+    from ephip import _lib as L
+    from ephip import bte, core, eph
+    dev = backend.torch_device
+    rng = np.random.default_rng(7)
+    for nw in (8, 16):
+        m = M.make_model(nw=nw, natoms=2, box_hd=5, box_g=3, seed=1001 + nw, with_g=False)
+        irvec, ham_r = m.irvec_h, m.ham_r
+        nr = irvec.shape[0]
+        tau = rng.uniform(-0.5, 0.5, (nw, 3))
+        pos = np.zeros((nr, 3 * nw * nw), dtype=np.complex128)
+        r0 = int(np.flatnonzero((irvec == 0).all(1))[0])
+        for a in range(3):
+            pos[r0, a * nw * nw + np.arange(nw) * (nw + 1)] = tau[:, a]
+        xk = rng.random((40, 3))
+        xk_dev = torch.from_numpy(xk).to(dev)
+        ham = core.WannierObject(irvec, ham_r).to_device(backend, dtype=torch.complex128)
+        el = eph.compute_electron_states(ham, xk_dev)
+        vel = core.WannierObject(irvec, bte.velocity_operator(irvec, ham_r).op_r).to_device(backend)
+        pos_dev = core.WannierObject(irvec, pos).to_device(backend)
+        kidx = torch.arange(40, device=dev)
+        rbar = bte.el_velocity_direct_batched(pos_dev, el, kidx)
+        v0 = bte.el_velocity_direct_batched(vel, el, kidx)
+        vb = bte.el_velocity_direct_batched(vel, el, kidx, rbar=rbar)
+        rbar_h, v0_h, vb_h = rbar.cpu().numpy(), v0.cpu().numpy(), vb.cpu().numpy()
+        # diagonal of the plain rotation = the band-diagonal velocity kernel
+        dH = core.fourier_batched(vel, xk_dev)
+        vd = torch.empty((40 * nw, 3), dtype=torch.float64, device=dev)
+        col = torch.arange(40, device=dev).repeat_interleave(nw)
+        band = torch.arange(nw, device=dev, dtype=torch.int32).repeat(40)
+        L.call("ephip_band_velocity", nw, 40 * nw, L.ptr(col), L.ptr(band), L.ptr(col), L.ptr(dH), dH.stride(0),
+               L.ptr(el.U), L.ptr(vd), L.stream_handle())
+        diag = np.stack([np.diagonal(v0_h[:, a], axis1=1, axis2=2).real for a in range(3)], axis=-1)  # (k, n, a)
+        assert np.abs(diag.reshape(-1, 3) - vd.cpu().numpy()).max() <= 1e-12 * np.abs(diag).max()
+        for x in (rbar_h, v0_h, vb_h):
+            assert np.abs(x - np.conj(np.swapaxes(x, -1, -2))).max() <= 1e-12 * np.abs(x).max()
+        assert np.abs(np.trace(rbar_h, axis1=-2, axis2=-1) - tau.sum(0)).max() <= 1e-6
+        # tau-gauge derivative, FP64, from H(R) directly
+        U = el.U.cpu().numpy().astype(np.complex128).transpose(0, 2, 1)  # [k, i, n]
+        Hr = ham_r.reshape(nr, nw, nw).transpose(0, 2, 1)                 # [R, i, j]
+        shift = tau[None, :, :] - tau[:, None, :]                          # [i, j, a] = tau_j - tau_i
+        num = den = 0.0
+        for k in range(40):
+            arg = irvec[:, None, None, :] + shift[None]                      # [R, i, j, a]
+            ph = np.exp(2j * np.pi * (arg @ xk[k]))                          # [R, i, j]
+            D = np.exp(2j * np.pi * (tau @ xk[k]))
+            Ut = np.conj(D)[:, None] * U[k]
+            for a in range(3):
+                dHt = np.sum(1j * arg[..., a] * ph * Hr, axis=0)
+                ref = np.conj(Ut).T @ dHt @ Ut
+                num = max(num, float(np.abs(vb_h[k, a] - ref).max()))
+                den = max(den, float(np.abs(ref).max()))
+        print(f"nw {nw}: Berry-connection velocity vs tau-gauge derivative, max rel {num / den:.2e}")
+        assert num / den <= 1e-6, num / den
