@@ -1,5 +1,7 @@
 // Energy-window filter, phonon post-processing, k-point index maps and the e-ph rotations.
 // These are bandwidth-bound: one thread (or one wave) per point, coalesced along the fastest index.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace ephip {
@@ -251,6 +253,122 @@ extern "C" int ephip_eph_rotate_rqkq_g2(int32_t nw, int32_t nm, int64_t ncol, co
     rotate_rqkq_g2_kernel<<<blocks_for(ncol, 4), 256, lds, as_stream(stream)>>>(
         nw, nm, ncol, static_cast<const c32 *>(g), ldg, ik, ikq, static_cast<const c32 *>(U), bmin, bmax, omega,
         nbw, g2, static_cast<c32 *>(ep));
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Generic device primitives of the backend method table: strided-batched complex GEMM (alpha 1,
+// beta 0) and the window scatter of device-resident calculators.
+// ---------------------------------------------------------------------------------------------
+namespace ephip {
+
+// C_b[i, j] = sum_p op(A_b)[i, p] op(B_b)[p, j], one thread per (b, i, j); op is the matrix ('N'), its
+// transpose ('T') or its adjoint ('C'). Column-major operands; accumulation in the element type.
+template <typename V, typename S>
+__global__ void batched_gemm_kernel(int ta, int tb, int m, int n, int k, int64_t nbatch, const V *__restrict__ A,
+                                    int64_t lda, int64_t sa, const V *__restrict__ B, int64_t ldb, int64_t sb,
+                                    V *__restrict__ C, int64_t ldc, int64_t sc) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t mn = (int64_t)m * n;
+    if (e >= mn * nbatch) return;
+    const int64_t b = e / mn;
+    const int r = (int)(e % mn), i = r % m, j = r / m;
+    const V *a = A + sa * b;
+    const V *bb = B + sb * b;
+    S re = 0, im = 0;
+    for (int p = 0; p < k; p++) {
+        V x = ta == 'N' ? a[i + lda * p] : a[p + lda * i];
+        V y = tb == 'N' ? bb[p + ldb * j] : bb[j + ldb * p];
+        if (ta == 'C') x.y = -x.y;
+        if (tb == 'C') y.y = -y.y;
+        re += x.x * y.x - x.y * y.y;
+        im += x.x * y.y + x.y * y.x;
+    }
+    V out;
+    out.x = re;
+    out.y = im;
+    C[sc * b + i + ldc * j] = out;
+}
+
+// For every (m, n, nu, iq) of g2vals (nbkq, nbk, nm, nq) column-major: i = imap_i[n], f = imap_f[m + ld_f
+// (ikqs[iq] - 1)] (1-based states, 0 = outside the window; imap_f has nk_f columns); when both are positive,
+// slot nu + nm (i - i0 - 1) + nm ni_stride (f - 1) of g2_out / w_out takes g2vals and w[nu + nm iq]. k+q
+// indices outside 1..nk_f and slots outside [0, n_out) are skipped.
+template <typename T>
+__global__ void window_scatter_kernel(T *__restrict__ g2_out, T *__restrict__ w_out, const T *__restrict__ g2vals,
+                                      const int64_t *__restrict__ imap_i, const int64_t *__restrict__ imap_f,
+                                      int64_t ld_f, int64_t nk_f, const int64_t *__restrict__ ikqs, const T *__restrict__ w,
+                                      int nbkq, int nbk, int nm, int64_t nq, int64_t ni_stride, int64_t i0,
+                                      int64_t n_out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nbkq * nbk * nm * nq) return;
+    const int m = (int)(e % nbkq);
+    const int n = (int)((e / nbkq) % nbk);
+    const int nu = (int)((e / ((int64_t)nbkq * nbk)) % nm);
+    const int64_t iq = e / ((int64_t)nbkq * nbk * nm);
+    const int64_t i = imap_i[n];
+    const int64_t kq = ikqs[iq];
+    if (kq < 1 || kq > nk_f) return;
+    const int64_t f = imap_f[m + ld_f * (kq - 1)];
+    if (i <= 0 || f <= 0) return;
+    const int64_t lin = nu + (int64_t)nm * (i - i0 - 1) + (int64_t)nm * ni_stride * (f - 1);
+    if (lin < 0 || lin >= n_out) return;
+    g2_out[lin] = g2vals[e];
+    w_out[lin] = w[nu + (int64_t)nm * iq];
+}
+
+}  // namespace ephip
+
+static bool valid_trans(int32_t t) { return t == 'N' || t == 'T' || t == 'C'; }
+
+extern "C" int ephip_batched_gemm(int32_t fp64, int32_t transA, int32_t transB, int32_t m, int32_t n, int32_t k,
+                                  int64_t nbatch, const void *A, int64_t lda, int64_t strideA, const void *B,
+                                  int64_t ldb, int64_t strideB, void *C, int64_t ldc, int64_t strideC,
+                                  void *stream) {
+    EPHIP_REQUIRE(valid_trans(transA) && valid_trans(transB), "batched_gemm: trans must be 'N', 'T' or 'C'");
+    EPHIP_REQUIRE(m >= 0 && n >= 0 && k >= 0 && nbatch >= 0, "batched_gemm: negative size");
+    EPHIP_REQUIRE(lda >= std::max(1, transA == 'N' ? m : k) && ldb >= std::max(1, transB == 'N' ? k : n) &&
+                      ldc >= std::max(1, m),
+                  "batched_gemm: leading dimension too small");
+    const int64_t mn = (int64_t)m * n;
+    if (mn == 0 || nbatch == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(strideC >= ldc * n || nbatch == 1, "batched_gemm: batches of C overlap");
+    EPHIP_REQUIRE(C && (k == 0 || (A && B)), "batched_gemm: null device pointer");  // k = 0: C = 0
+    EPHIP_REQUIRE(mn * nbatch <= 0x7fffffffLL * 256, "batched_gemm: too many entries");
+    if (fp64) {
+        batched_gemm_kernel<double2, double><<<blocks_for(mn * nbatch), 256, 0, as_stream(stream)>>>(
+            transA, transB, m, n, k, nbatch, static_cast<const double2 *>(A), lda, strideA,
+            static_cast<const double2 *>(B), ldb, strideB, static_cast<double2 *>(C), ldc, strideC);
+    } else {
+        batched_gemm_kernel<float2, float><<<blocks_for(mn * nbatch), 256, 0, as_stream(stream)>>>(
+            transA, transB, m, n, k, nbatch, static_cast<const float2 *>(A), lda, strideA,
+            static_cast<const float2 *>(B), ldb, strideB, static_cast<float2 *>(C), ldc, strideC);
+    }
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_eph_window_scatter(int32_t fp64, void *g2_out, void *w_out, int64_t n_out, const void *g2vals,
+                                        const int64_t *imap_i, const int64_t *imap_f, int64_t ld_f, int64_t nk_f,
+                                        const int64_t *ikqs, const void *w, int32_t nbkq, int32_t nbk, int32_t nm,
+                                        int64_t nq, int64_t ni_stride, int64_t i0, void *stream) {
+    EPHIP_REQUIRE(nbkq >= 0 && nbk >= 0 && nm >= 0 && nq >= 0 && n_out >= 0 && ni_stride >= 0 && i0 >= 0,
+                  "eph_window_scatter: negative size");
+    EPHIP_REQUIRE(nk_f >= 0, "eph_window_scatter: negative nk_f");
+    EPHIP_REQUIRE(ld_f >= std::max(1, nbkq), "eph_window_scatter: ld_f below nbkq");
+    const int64_t ne = (int64_t)nbkq * nbk * nm * nq;
+    if (ne == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(g2_out && w_out && g2vals && imap_i && imap_f && ikqs && w, "eph_window_scatter: null device pointer");
+    EPHIP_REQUIRE(ne <= 0x7fffffffLL * 256, "eph_window_scatter: too many entries");
+    if (fp64)
+        window_scatter_kernel<double><<<blocks_for(ne), 256, 0, as_stream(stream)>>>(
+            static_cast<double *>(g2_out), static_cast<double *>(w_out), static_cast<const double *>(g2vals), imap_i,
+            imap_f, ld_f, nk_f, ikqs, static_cast<const double *>(w), nbkq, nbk, nm, nq, ni_stride, i0, n_out);
+    else
+        window_scatter_kernel<float><<<blocks_for(ne), 256, 0, as_stream(stream)>>>(
+            static_cast<float *>(g2_out), static_cast<float *>(w_out), static_cast<const float *>(g2vals), imap_i,
+            imap_f, ld_f, nk_f, ikqs, static_cast<const float *>(w), nbkq, nbk, nm, nq, ni_stride, i0, n_out);
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

@@ -52,6 +52,10 @@ SIGNATURES = {
                                    _i32, _vp, _vp, _vp, _i32, _i32, _d, _i64, _vp, _vp, _vp, _vp, _vp],
     "ephip_band_velocity": [_i32, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
     "ephip_el_velocity_direct_batched": [_i32, _i64, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _i64, _vp],
+    "ephip_batched_gemm": [_i32, _i32, _i32, _i32, _i32, _i32, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp,
+                           _i64, _i64, _vp],
+    "ephip_eph_window_scatter": [_i32, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _i64, _vp, _vp, _i32, _i32, _i32,
+                                 _i64, _i64, _i64, _vp],
     "ephip_bte_spmv": [_i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ephip_mlp_bf16_prune": [_i64, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, C.c_float, C.c_float,
                              C.c_float, _vp, _vp, _vp],

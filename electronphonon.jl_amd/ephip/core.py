@@ -102,6 +102,58 @@ def to_device(backend: GPUBackend, x):
     return x.to(backend.torch_device)
 
 
+def batched_gemm(transA: str, transB: str, A, B, C):
+    """batched_gemm! (src/common/gpu_utils.jl:60-78): C[:, :, b] = op(A[:, :, b]) op(B[:, :, b]) with
+    alpha 1, beta 0 and op 'N' (the matrix), 'T' (transpose) or 'C' (adjoint), for column-major
+    (rows, cols, batch) arrays held as contiguous tensors of shape (batch, cols, rows) (as ``alloc``
+    makes them); complex64 or complex128."""
+    torch = _torch()
+    if transA not in ("N", "T", "C") or transB not in ("N", "T", "C"):
+        raise L.EphipError(f"batched_gemm: trans must be 'N', 'T' or 'C' (got {transA!r}, {transB!r})")
+    for X in (A, B, C):
+        if X.dim() != 3 or not X.is_contiguous():
+            raise L.EphipError("batched_gemm: operands must be contiguous 3-d tensors (batch, cols, rows)")
+    if not (A.dtype == B.dtype == C.dtype) or A.dtype not in (torch.complex64, torch.complex128):
+        raise L.EphipError(f"batched_gemm: complex64 or complex128 operands of one type (got {A.dtype}, "
+                           f"{B.dtype}, {C.dtype})")
+    nb = C.shape[0]
+    ra, ca, rb, cb = A.shape[2], A.shape[1], B.shape[2], B.shape[1]
+    m, k = (ra, ca) if transA == "N" else (ca, ra)
+    k2, n = (rb, cb) if transB == "N" else (cb, rb)
+    if A.shape[0] != nb or B.shape[0] != nb or k != k2 or C.shape[2] != m or C.shape[1] != n:
+        raise L.EphipError(f"batched_gemm: shapes do not match (op(A) {m}x{k}, op(B) {k2}x{n}, "
+                           f"C {C.shape[2]}x{C.shape[1]}, batches {A.shape[0]}, {B.shape[0]}, {nb})")
+    L.call("ephip_batched_gemm", int(A.dtype == torch.complex128), ord(transA), ord(transB), m, n, k, nb,
+           L.ptr(A), max(ra, 1), ra * ca, L.ptr(B), max(rb, 1), rb * cb, L.ptr(C), max(m, 1), m * n,
+           L.stream_handle())
+    return C
+
+
+def eph_window_scatter(g2_out, w_out, g2vals, imap_i_col, imap_f, ikqs, w, nbandkq: int, nbandk: int, nm: int,
+                       nq_batch: int, ni_stride: int, i0: int):
+    """eph_window_scatter! (src/calculator/calculator_utils.jl:6-52): for every (m, n, nu, iq) entry of
+    g2vals (nbandkq, nbandk, nm, nq_batch) column-major, i = imap_i_col[n] and f = imap_f[m, ikqs[iq]]
+    (1-based states, 0 = outside the window); when both are positive, the mode-fastest slot
+    nu + nm (i - i0 - 1) + nm ni_stride (f - 1) of the flat g2_out / w_out takes the value and
+    w[nu, iq]. imap_f is a column-major (ld, nk) int64 matrix held as a contiguous (nk, ld) tensor;
+    ikqs are 1-based k indices. float32 or float64 values, all of one type."""
+    torch = _torch()
+    vals = (g2_out, w_out, g2vals, w)
+    if not all(x.dtype == g2vals.dtype and x.is_contiguous() for x in vals) or \
+            g2vals.dtype not in (torch.float32, torch.float64):
+        raise L.EphipError("eph_window_scatter: contiguous float32 or float64 values of one type")
+    maps = (imap_i_col, imap_f, ikqs)
+    if not all(x.dtype == torch.int64 and x.is_contiguous() for x in maps) or imap_f.dim() != 2:
+        raise L.EphipError("eph_window_scatter: contiguous int64 maps, imap_f 2-d (nk, ld)")
+    nk_f, ld_f = imap_f.shape
+    if (g2vals.numel() < nbandkq * nbandk * nm * nq_batch or w.numel() < nm * nq_batch or
+            imap_i_col.numel() < nbandk or ikqs.numel() < nq_batch or ld_f < nbandkq):
+        raise L.EphipError("eph_window_scatter: an input is smaller than its declared size")
+    L.call("ephip_eph_window_scatter", int(g2vals.dtype == torch.float64), L.ptr(g2_out), L.ptr(w_out),
+           min(g2_out.numel(), w_out.numel()), L.ptr(g2vals), L.ptr(imap_i_col), L.ptr(imap_f), ld_f, nk_f,
+           L.ptr(ikqs), L.ptr(w), nbandkq, nbandk, nm, nq_batch, ni_stride, i0, L.stream_handle())
+
+
 # ------------------------------------------------------------------------------------------------
 # Wannier objects and the phase-sum Fourier transform
 # ------------------------------------------------------------------------------------------------

@@ -403,6 +403,26 @@ int ephip_band_velocity(int32_t nw, int64_t nstates, const int64_t *state_col, c
                         const int64_t *state_k, const void *dH, int64_t ld_dh, const void *U, double *v,
                         void *stream);
 
+/* C_b = op(A_b) op(B_b) for b < nbatch (alpha 1, beta 0), complex128 (fp64 = 1) or complex64 (fp64 = 0),
+ * column-major m x n result, inner size k; op selected by transA / transB = 'N' (the matrix), 'T'
+ * (transpose) or 'C' (adjoint); batch b of X starts at X + b strideX (elements). Replaces the backend
+ * method batched_gemm! (src/common/gpu_utils.jl:60-78; CUDA method ext/ElectronPhononCUDAExt.jl:116-126). */
+int ephip_batched_gemm(int32_t fp64, int32_t transA, int32_t transB, int32_t m, int32_t n, int32_t k, int64_t nbatch,
+                       const void *A, int64_t lda, int64_t strideA, const void *B, int64_t ldb, int64_t strideB,
+                       void *C, int64_t ldc, int64_t strideC, void *stream);
+
+/* Window scatter of a device-resident calculator: for every (m, n, nu, iq) of g2vals (nbkq, nbk, nm, nq,
+ * column-major; float64 when fp64 = 1, else float32, like w, g2_out and w_out), i = imap_i[n] and
+ * f = imap_f[m + ld_f (ikqs[iq] - 1)] (int64, 1-based, 0 = outside the window; imap_f has nk_f columns);
+ * when both are positive, slot nu + nm (i - i0 - 1) + nm ni_stride (f - 1) of g2_out and w_out (n_out
+ * entries each) takes g2vals and w[nu + nm iq]. Out-of-range k+q indices and slots are skipped.
+ * Replaces eph_window_scatter! (src/calculator/calculator_utils.jl:6-52; CUDA method
+ * ext/ElectronPhononCUDAExt.jl:280-313). */
+int ephip_eph_window_scatter(int32_t fp64, void *g2_out, void *w_out, int64_t n_out, const void *g2vals,
+                             const int64_t *imap_i, const int64_t *imap_f, int64_t ld_f, int64_t nk_f,
+                             const int64_t *ikqs, const void *w, int32_t nbkq, int32_t nbk, int32_t nm, int64_t nq,
+                             int64_t ni_stride, int64_t i0, void *stream);
+
 /* Full rotated operator of nk k-points: out[k ld_out + a nw^2 + m + nw n] = (U_k^H M_a(k) U_k)[m, n] in FP64
  * for the three directions a, with M (complex128, column k of ld_m >= 3 nw^2 entries, direction slowest)
  * the Fourier of a 3 nw^2-data operator (i R H(R) for the velocity, the position operator for rbar) and U

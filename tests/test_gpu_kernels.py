@@ -469,3 +469,86 @@ def test_gridopt_stage1_planes(backend, nb):
     with pytest.raises(L.EphipError):
         L.call("ephip_gridopt_stage1_planes", nd, n1, n2, n3, np_, r1min, L.ptr(dmap), L.ptr(dop), ldop, 9,
                C.addressof(xa9), nb, C.addressof(pa9), L.stream_handle())
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_batched_gemm(backend, dtype):
+    """batched_gemm! (alpha 1, beta 0) for every op pair of 'N', 'T', 'C' on ragged shapes against
+    numpy's matrix product; inner size 0 gives zeros, empty batches are a no-op."""
+    from ephip import core
+    dev = backend.torch_device
+    tdt = getattr(torch, dtype)
+    rng = np.random.default_rng(11)
+    m, n, k, nb = 5, 7, 3, 4
+    tol = 1e-12 if dtype == "complex128" else 2e-6
+
+    def op(x, t):  # x: (rows, cols) matrix
+        return x if t == "N" else (x.T if t == "T" else x.conj().T)
+
+    for ta in "NTC":
+        for tb in "NTC":
+            ra, ca = (m, k) if ta == "N" else (k, m)
+            rb, cb = (k, n) if tb == "N" else (n, k)
+            a = rng.standard_normal((nb, ra, ca)) + 1j * rng.standard_normal((nb, ra, ca))
+            b = rng.standard_normal((nb, rb, cb)) + 1j * rng.standard_normal((nb, rb, cb))
+            # column-major (rows, cols, batch) arrays as (batch, cols, rows) tensors
+            A = torch.from_numpy(np.ascontiguousarray(a.transpose(0, 2, 1))).to(dev, tdt)
+            B = torch.from_numpy(np.ascontiguousarray(b.transpose(0, 2, 1))).to(dev, tdt)
+            C = torch.full((nb, n, m), 7 + 7j, dtype=tdt, device=dev)
+            core.batched_gemm(ta, tb, A, B, C)
+            got = C.cpu().numpy().transpose(0, 2, 1)
+            ref = np.stack([op(a[i], ta) @ op(b[i], tb) for i in range(nb)])
+            assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), (ta, tb)
+    A = torch.zeros((2, 0, m), dtype=tdt, device=dev)
+    B = torch.zeros((2, n, 0), dtype=tdt, device=dev)
+    C = torch.full((2, n, m), 1.0, dtype=tdt, device=dev)
+    core.batched_gemm("N", "N", A, B, C)
+    assert not C.cpu().numpy().any()
+    core.batched_gemm("N", "N", torch.zeros((0, 3, m), dtype=tdt, device=dev), torch.zeros((0, n, 3), dtype=tdt,
+                      device=dev), torch.zeros((0, n, m), dtype=tdt, device=dev))
+    with pytest.raises(Exception):
+        core.batched_gemm("X", "N", A, B, C)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_eph_window_scatter(backend, dtype):
+    """eph_window_scatter! on random injective state maps with out-of-window zeros, a per-batch buffer
+    (i0 > 0) and k+q indices repeated across q: every in-window entry lands at the mode-fastest slot
+    (nu, i - i0 - 1, f - 1) of an (nm, ni_stride, nf) array (np.ravel_multi_index), untouched slots keep
+    their sentinel, and no two entries share a slot."""
+    from ephip import core
+    dev = backend.torch_device
+    tdt = getattr(torch, dtype)
+    rng = np.random.default_rng(12)
+    nbkq, nbk, nm, nq, nk, ld_f = 3, 4, 5, 6, 9, 4
+    ni_stride, i0, nf = 6, 10, 40
+    imap_i = np.array([11, 0, 14, 16], dtype=np.int64)               # global outer states, 0 = outside
+    imap_f = np.zeros((nk, ld_f), dtype=np.int64)                     # column-major (ld_f, nk): [ikq, m]
+    fvals = rng.permutation(np.arange(1, nf + 1))[: nk * nbkq].reshape(nk, nbkq)
+    fvals[rng.random((nk, nbkq)) < 0.3] = 0
+    imap_f[:, :nbkq] = fvals
+    ikqs = rng.permutation(np.arange(1, nk + 1))[:nq].astype(np.int64)  # distinct k+q per q
+    g2 = rng.random((nq, nm, nbk, nbkq))                                 # column-major (nbkq, nbk, nm, nq)
+    w = rng.random((nq, nm))
+    nout = nm * ni_stride * nf
+    out = torch.full((nout,), -1.0, dtype=tdt, device=dev)
+    wout = torch.full((nout,), -1.0, dtype=tdt, device=dev)
+    t = lambda x, d=tdt: torch.from_numpy(np.ascontiguousarray(x)).to(dev, d)  # noqa: E731
+    core.eph_window_scatter(out, wout, t(g2), t(imap_i, torch.int64), t(imap_f, torch.int64),
+                            t(ikqs, torch.int64), t(w), nbkq, nbk, nm, nq, ni_stride, i0)
+    exp = np.full(nout, -1.0)
+    expw = np.full(nout, -1.0)
+    slots = []
+    for iq in range(nq):
+        for nu in range(nm):
+            for n in range(nbk):
+                for mm in range(nbkq):
+                    i, f = imap_i[n], imap_f[ikqs[iq] - 1, mm]
+                    if i > 0 and f > 0:
+                        s = np.ravel_multi_index((f - 1, i - i0 - 1, nu), (nf, ni_stride, nm))
+                        slots.append(s)
+                        exp[s], expw[s] = g2[iq, nu, n, mm], w[iq, nu]
+    assert len(slots) > 20 and len(set(slots)) == len(slots)
+    tol = 0.0 if dtype == "float64" else 1e-7
+    assert np.abs(out.cpu().numpy() - exp).max() <= tol
+    assert np.abs(wout.cpu().numpy() - expw).max() <= tol
