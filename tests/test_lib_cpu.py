@@ -64,3 +64,22 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(L, "LIB_PATH", str(tmp_path / "libephip.so"))
     with pytest.raises(L.EphipError, match="no CPU fallback"):
         L.load()
+
+
+def test_generic_backend_methods_validate_arguments():
+    # batched_gemm!: op characters and leading dimensions; empty work is a no-op
+    with pytest.raises(L.EphipError, match="'N', 'T' or 'C'"):
+        L.call("ephip_batched_gemm", 0, ord("X"), ord("N"), 2, 2, 2, 1, 1, 2, 4, 1, 2, 4, 1, 2, 4, None)
+    with pytest.raises(L.EphipError, match="leading dimension"):
+        L.call("ephip_batched_gemm", 1, ord("T"), ord("N"), 4, 2, 3, 1, 1, 2, 12, 1, 3, 6, 1, 4, 8, None)
+    L.call("ephip_batched_gemm", 1, ord("N"), ord("C"), 4, 2, 3, 0, None, 4, 12, None, 2, 6, None, 4, 8, None)
+    # eph_window_scatter!: imap_f's leading dimension must hold the k+q bands
+    with pytest.raises(L.EphipError, match="ld_f"):
+        L.call("ephip_eph_window_scatter", 1, 1, 1, 10, 1, 1, 1, 2, 5, 1, 1, 3, 2, 2, 1, 4, 0, None)
+    L.call("ephip_eph_window_scatter", 0, None, None, 0, None, None, None, 3, 5, None, None, 3, 2, 2, 0, 4, 0, None)
+    # velocity matrices: nw <= 32, energies and rbar together
+    with pytest.raises(L.EphipError, match="nw=33"):
+        L.call("ephip_el_velocity_direct_batched", 33, 1, 1, 3 * 33 * 33, 1, None, None, 0, 1, 3 * 33 * 33, None)
+    with pytest.raises(L.EphipError, match="come together"):
+        L.call("ephip_el_velocity_direct_batched", 8, 1, 1, 192, 1, 1, None, 192, 1, 192, None)
+    L.call("ephip_el_velocity_direct_batched", 8, 0, None, 192, None, None, None, 0, None, 192, None)
