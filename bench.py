@@ -21,7 +21,6 @@ at least one band in the window by the exact filter (the same numerator for ever
 from __future__ import annotations
 
 import argparse
-import concurrent.futures as cf
 import json
 import os
 import sys
@@ -74,21 +73,6 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_epmat(model_mod, m, workers=16):
-    """Build the complex64 "ph" layout e-ph tensor column by column in parallel threads."""
-    nr_ep = len(m.irvec_ep)
-    nd = m.nw * m.nw * m.nmodes * len(m.irvec_el)
-    ep = np.empty((nr_ep, nd), dtype=np.complex64)
-
-    def col(ip):
-        ep[ip] = model_mod.eph_column(m.seed, m.nw, m.nmodes, m.irvec_el, m.irvec_ep[ip], ip, m.g_decay,
-                                      np.complex64)
-
-    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
-        list(ex.map(col, range(nr_ep)))
-    return ep
-
-
 def cpu_baseline(O, m, st, epmat_c64, nk_sample, kgrid, min_seconds):
     """Time the FP64 oracle (C restatement of the reference's CPU outer-q path, OpenMP) on whole q
     slices of the workload: for q = 0, then q spread over the grid, RR->Rq for the q and the Rq->kq
@@ -133,6 +117,38 @@ def cpu_baseline(O, m, st, epmat_c64, nk_sample, kgrid, min_seconds):
                       f"slice(s) (q = {qs[:4]}{'...' if len(qs) > 4 else ''}), each RR->Rq + Rq->kq for "
                       f"{len(ik)} of {nks} surviving k; RR->Rq {t_rr:.2f} s, Rq->kq {t_kq:.2f} s; {how}",
             "seconds": t_rr + t_kq}
+
+
+def oracle_check(m, st, epmat, grids, w, payload, nk_sample=32, nq_sample=2):
+    """Check what was timed against the FP64 oracle (test infrastructure, run after the timed region):
+    the window filter of the whole fine grid bit for bit (hence K_surv, the metric's numerator), and the
+    gauge-invariant |g|^2 of sampled (k, q) pairs of the LAST timed batch (its payload buffers), averaged
+    over degenerate bands, norm-wise relative error (the metric's 1e-4 bound)."""
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import eph_checks as C
+    from ephip import model as M
+    t0 = time.perf_counter()
+    f = st.el.filt
+    E_o = C.full_grid_states(m, M.kpoints_grid(grids["kgrid"]))
+    r = C.filter_compare(E_o, f.keep.cpu().numpy(), f.band_min.cpu().numpy(), f.band_max.cpu().numpy(),
+                         st.el.E.cpu().numpy(), -w, w)
+    out = {"filter": r, "filter_bit_exact": r["keep_equal"] and r["band_min_equal"] and r["band_max_equal"]}
+    if payload is not None:
+        nq = int(np.prod(grids["qgrid"]))
+        qsel = [int(q) for q in np.random.default_rng(7).choice(nq, nq_sample, replace=False)]
+        got = C.sample_units_g2(payload, qsel, nk_sample, torch, seed=8)
+        m.epmat_ph = epmat
+        g = C.g2_parity(m, epmat, grids["kgrid"], grids["qgrid"], got, f.band_min.cpu().numpy(),
+                        f.band_max.cpu().numpy(), st.nbw)
+        m.epmat_ph = None
+        out.update(g2_rel_err=g["rel_err"], g2_pairs=g["pairs"], g2_max_block_rel_err=g["max_block_rel_err"],
+                   g2_sample=f"{nk_sample} k of the last timed batch x q {qsel}")
+    out["within_bound"] = bool(out["filter_bit_exact"] and out.get("g2_rel_err", 0.0) <= 1e-4)
+    out["seconds"] = time.perf_counter() - t0
+    log(f"[bench] oracle check: filter bit-exact {out['filter_bit_exact']} (K_surv {r['K_surv_oracle']}), "
+        f"|g|^2 rel err {out.get('g2_rel_err')} over {out.get('g2_pairs')} pairs ({out['seconds']:.1f} s)")
+    return out
 
 
 def transport_bench(args):
@@ -210,6 +226,8 @@ def main():
                     help="run the transport measurement (conductivity, FP32 vs split-BF16) instead of the e-ph bench")
     ap.add_argument("--transport-eta", type=float, default=0.01, help="Gaussian smearing of the transport run (eV)")
     ap.add_argument("--transport-iter", type=int, default=200)
+    ap.add_argument("--no-check", dest="check", action="store_false", default=True,
+                    help="skip the oracle check (full-grid filter and sampled |g|^2 of the last timed batch)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -238,7 +256,7 @@ def main():
         del src, dst
     m, grids = M.make_config_model(args.config, with_g=False)
     log(f"[bench] generating e-ph tensor ({args.config})")
-    epmat = make_epmat(M, m)
+    epmat = M.build_epmat(m)
     log(f"[bench] e-ph tensor {epmat.nbytes / 1e9:.2f} GB generated in {time.perf_counter() - t_setup:.1f} s")
     epmat_dev = core.WannierObject(m.irvec_ep, torch.from_numpy(epmat).to(backend.torch_device), check=False)
     w = grids["window"]
@@ -332,11 +350,12 @@ def main():
         torch.cuda.synchronize()
         ev.clear()
         t0 = time.perf_counter()
+        last_payload = None
         if args.pipeline:
             loop.run_pipelined(timed, nks_steps)
         else:
             for i0 in timed:
-                loop.run_kbatch(i0, nks_steps)
+                last_payload = loop.run_kbatch(i0, nks_steps)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         # algorithmic flops of the launches actually made (direct or staged kernels), step by step
@@ -381,6 +400,10 @@ def main():
         log("[bench] timing the CPU oracle slice")
         cpu = cpu_baseline(O, m, st, epmat, args.cpu_sample, grids["kgrid"], args.cpu_seconds)
         log(f"[bench] cpu baseline {cpu['value']:.3e} pairs/s ({cpu['seconds']:.1f} s)")
+
+    check = None
+    if args.check:   # the FP64 oracle as the checker of what was timed (after the timed region)
+        check = oracle_check(m, st, epmat, grids, w, last_payload if "fast" in results else None)
 
     if "fast" in results:
         r = results["fast"]
@@ -438,7 +461,8 @@ def main():
                    "path": path, "K_surv": nks, "pairs_per_step": pairs_step,
                    "pre_filter_pairs": nk_total * nq_total,
                    "plain_outer_q_fp32_pairs_per_s": results.get("plain", {}).get("value")},
-        "roofline": {"bound": "mfma", "achieved": dom[1], "peak": peak, "unit": "TFLOP/s",
+        "roofline": {"bound": ("valu" if "fast" in results and var[dom[4]] == "gridopt" else "mfma"),
+                     "achieved": dom[1], "peak": peak, "unit": "TFLOP/s",
                      "frac": dom[1] / peak,
                      "compute": ("FP32 VALU, no MFMA: 157.3 TFLOP/s is MI355X's published FP32 peak, which counts "
                                  "packed FP32 (not used here); the unpacked vector peak is 78.6 TFLOP/s"
@@ -454,6 +478,7 @@ def main():
                      "frac_unpacked_vector_peak": (dom[1] / 78.6 if "fast" in results and var[dom[4]] == "gridopt"
                                                    else None)},
         "cpu_baseline": cpu,
+        "check": check,
         "detail": results,
     }
     print(json.dumps(out))

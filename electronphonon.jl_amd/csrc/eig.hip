@@ -8,9 +8,9 @@
 //   1. Householder reduction of the Hermitian matrix to tridiagonal form, one reflector per column;
 //      the reflector vectors are kept in the zeroed part of the matrix for the back-transform.
 //   2. Diagonal phase scaling turns the complex tridiagonal into a real symmetric one.
-//   3. Implicit QL with Wilkinson-type shifts on (d, e): the scalar recurrence is uniform across the
-//      wave (lane i holds d_i, e_i; reads go through readlane), the rotations are applied to the rows
-//      of Z in parallel (lane = row).
+//   3. Implicit QL with Wilkinson-type shifts on (d, e) (the published EISPACK imtql2 scheme): the
+//      scalar recurrence is uniform across the wave (lane i holds d_i, e_i; reads go through
+//      readlane), the rotations are applied to the rows of Z in parallel (lane = row).
 //   4. Eigenvectors V = H_0 ... H_{n-3} D Z, one column per lane.
 // A second kernel refines in FP64 against the FP64 matrix: a full Rayleigh-Ritz step in the basis of
 // the FP32 eigenvectors (FP64 orthonormalisation, projection, parallel Jacobi), which gives FP64-grade

@@ -294,13 +294,14 @@ __global__ void batched_gemm_kernel(int ta, int tb, int m, int n, int k, int64_t
 // For every (m, n, nu, iq) of g2vals (nbkq, nbk, nm, nq) column-major: i = imap_i[n], f = imap_f[m + ld_f
 // (ikqs[iq] - 1)] (1-based states, 0 = outside the window; imap_f has nk_f columns); when both are positive,
 // slot nu + nm (i - i0 - 1) + nm ni_stride (f - 1) of g2_out / w_out takes g2vals and w[nu + nm iq]. k+q
-// indices outside 1..nk_f and slots outside [0, n_out) are skipped.
+// indices outside 1..nk_f and slots outside [0, n_out) are not written; each such entry adds 1 to
+// *dropped (when given), so a wrong i0 / ni_stride surfaces instead of losing data silently.
 template <typename T>
 __global__ void window_scatter_kernel(T *__restrict__ g2_out, T *__restrict__ w_out, const T *__restrict__ g2vals,
                                       const int64_t *__restrict__ imap_i, const int64_t *__restrict__ imap_f,
                                       int64_t ld_f, int64_t nk_f, const int64_t *__restrict__ ikqs, const T *__restrict__ w,
                                       int nbkq, int nbk, int nm, int64_t nq, int64_t ni_stride, int64_t i0,
-                                      int64_t n_out) {
+                                      int64_t n_out, unsigned int *__restrict__ dropped) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (int64_t)nbkq * nbk * nm * nq) return;
     const int m = (int)(e % nbkq);
@@ -309,11 +310,17 @@ __global__ void window_scatter_kernel(T *__restrict__ g2_out, T *__restrict__ w_
     const int64_t iq = e / ((int64_t)nbkq * nbk * nm);
     const int64_t i = imap_i[n];
     const int64_t kq = ikqs[iq];
-    if (kq < 1 || kq > nk_f) return;
+    if (kq < 1 || kq > nk_f) {
+        if (dropped) atomicAdd(dropped, 1u);
+        return;
+    }
     const int64_t f = imap_f[m + ld_f * (kq - 1)];
     if (i <= 0 || f <= 0) return;
     const int64_t lin = nu + (int64_t)nm * (i - i0 - 1) + (int64_t)nm * ni_stride * (f - 1);
-    if (lin < 0 || lin >= n_out) return;
+    if (lin < 0 || lin >= n_out) {
+        if (dropped) atomicAdd(dropped, 1u);
+        return;
+    }
     g2_out[lin] = g2vals[e];
     w_out[lin] = w[nu + (int64_t)nm * iq];
 }
@@ -352,7 +359,8 @@ extern "C" int ephip_batched_gemm(int32_t fp64, int32_t transA, int32_t transB, 
 extern "C" int ephip_eph_window_scatter(int32_t fp64, void *g2_out, void *w_out, int64_t n_out, const void *g2vals,
                                         const int64_t *imap_i, const int64_t *imap_f, int64_t ld_f, int64_t nk_f,
                                         const int64_t *ikqs, const void *w, int32_t nbkq, int32_t nbk, int32_t nm,
-                                        int64_t nq, int64_t ni_stride, int64_t i0, void *stream) {
+                                        int64_t nq, int64_t ni_stride, int64_t i0, uint32_t *dropped,
+                                        void *stream) {
     EPHIP_REQUIRE(nbkq >= 0 && nbk >= 0 && nm >= 0 && nq >= 0 && n_out >= 0 && ni_stride >= 0 && i0 >= 0,
                   "eph_window_scatter: negative size");
     EPHIP_REQUIRE(nk_f >= 0, "eph_window_scatter: negative nk_f");
@@ -364,11 +372,13 @@ extern "C" int ephip_eph_window_scatter(int32_t fp64, void *g2_out, void *w_out,
     if (fp64)
         window_scatter_kernel<double><<<blocks_for(ne), 256, 0, as_stream(stream)>>>(
             static_cast<double *>(g2_out), static_cast<double *>(w_out), static_cast<const double *>(g2vals), imap_i,
-            imap_f, ld_f, nk_f, ikqs, static_cast<const double *>(w), nbkq, nbk, nm, nq, ni_stride, i0, n_out);
+            imap_f, ld_f, nk_f, ikqs, static_cast<const double *>(w), nbkq, nbk, nm, nq, ni_stride, i0, n_out,
+            dropped);
     else
         window_scatter_kernel<float><<<blocks_for(ne), 256, 0, as_stream(stream)>>>(
             static_cast<float *>(g2_out), static_cast<float *>(w_out), static_cast<const float *>(g2vals), imap_i,
-            imap_f, ld_f, nk_f, ikqs, static_cast<const float *>(w), nbkq, nbk, nm, nq, ni_stride, i0, n_out);
+            imap_f, ld_f, nk_f, ikqs, static_cast<const float *>(w), nbkq, nbk, nm, nq, ni_stride, i0, n_out,
+            dropped);
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

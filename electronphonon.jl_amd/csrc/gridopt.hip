@@ -642,7 +642,7 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
     const int nlines = m1 * m2;
     const int *lo_u = line_off + (int64_t)u * (nlines + 1);
     const int64_t row0 = (int64_t)u * cs;
-    if (t <= m2) slo[t] = lo_u[q1 * m2 + t];
+    for (int l = t; l <= m2; l += blockDim.x) slo[l] = lo_u[q1 * m2 + l];  // m2 + 1 may exceed the block
     if (t < n1) {
         float c, s;
         cis2pi((double)q1 / m1 * (double)(p1min + t), c, s);

@@ -114,12 +114,25 @@ def velocity_operator(irvec: np.ndarray, ham_r: np.ndarray, lattice=None) -> Wan
     return WannierObject(irvec, np.ascontiguousarray(op, dtype=np.complex128))
 
 
+def _check_velocity_dtypes(op_dev: WannierObject, el):
+    """The velocity kernels read the Fourier-transformed operator as FP64 complex (double2) and the
+    eigenvectors as complex64 (float2): a complex64 operator would be read past its end. Checked before
+    any launch."""
+    torch = _torch()
+    if op_dev.op_r.dtype != torch.complex128:
+        raise L.EphipError(f"the velocity/position operator must be complex128 on the device (got {op_dev.op_r.dtype}); "
+                           "use to_device(backend, dtype=torch.complex128)")
+    if el.U.dtype != torch.complex64:
+        raise L.EphipError(f"the electron eigenvectors must be complex64 (got {el.U.dtype})")
+
+
 def compute_velocities(vel_dev: WannierObject, el, bs: BandStates, chunk: int = 16384):
     """Band-diagonal velocities (bs.n, 3) float64 of every in-window state; fills bs.vs."""
     torch = _torch()
     nw = el.nw
     if vel_dev.ndata != 3 * nw * nw:
         raise L.EphipError(f"velocity operator has ndata={vel_dev.ndata}, expected 3*nw^2={3 * nw * nw}")
+    _check_velocity_dtypes(vel_dev, el)
     dev = el.E.device
     v = torch.zeros((bs.n, 3), dtype=torch.float64, device=dev)
     if bs.n == 0:
@@ -153,6 +166,7 @@ def el_velocity_direct_batched(op_dev: WannierObject, el, kidx, rbar=None):
     nw = el.nw
     if op_dev.ndata != 3 * nw * nw:
         raise L.EphipError(f"operator has ndata={op_dev.ndata}, expected 3*nw^2={3 * nw * nw}")
+    _check_velocity_dtypes(op_dev, el)
     dev = el.E.device
     kidx = torch.as_tensor(kidx, dtype=torch.int64, device=dev).reshape(-1)
     nk = kidx.shape[0]

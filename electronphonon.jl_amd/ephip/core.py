@@ -130,13 +130,15 @@ def batched_gemm(transA: str, transB: str, A, B, C):
 
 
 def eph_window_scatter(g2_out, w_out, g2vals, imap_i_col, imap_f, ikqs, w, nbandkq: int, nbandk: int, nm: int,
-                       nq_batch: int, ni_stride: int, i0: int):
+                       nq_batch: int, ni_stride: int, i0: int, check: bool = True):
     """eph_window_scatter! (src/calculator/calculator_utils.jl:6-52): for every (m, n, nu, iq) entry of
     g2vals (nbandkq, nbandk, nm, nq_batch) column-major, i = imap_i_col[n] and f = imap_f[m, ikqs[iq]]
     (1-based states, 0 = outside the window); when both are positive, the mode-fastest slot
     nu + nm (i - i0 - 1) + nm ni_stride (f - 1) of the flat g2_out / w_out takes the value and
     w[nu, iq]. imap_f is a column-major (ld, nk) int64 matrix held as a contiguous (nk, ld) tensor;
-    ikqs are 1-based k indices. float32 or float64 values, all of one type."""
+    ikqs are 1-based k indices. float32 or float64 values, all of one type. Entries whose k+q index or
+    output slot falls outside the buffers are not written; with ``check`` (default) their count is read
+    back after the launch (a stream synchronisation) and a non-zero count raises EphipError."""
     torch = _torch()
     vals = (g2_out, w_out, g2vals, w)
     if not all(x.dtype == g2vals.dtype and x.is_contiguous() for x in vals) or \
@@ -149,9 +151,15 @@ def eph_window_scatter(g2_out, w_out, g2vals, imap_i_col, imap_f, ikqs, w, nband
     if (g2vals.numel() < nbandkq * nbandk * nm * nq_batch or w.numel() < nm * nq_batch or
             imap_i_col.numel() < nbandk or ikqs.numel() < nq_batch or ld_f < nbandkq):
         raise L.EphipError("eph_window_scatter: an input is smaller than its declared size")
+    dropped = torch.zeros(1, dtype=torch.int32, device=g2vals.device) if check else None
     L.call("ephip_eph_window_scatter", int(g2vals.dtype == torch.float64), L.ptr(g2_out), L.ptr(w_out),
            min(g2_out.numel(), w_out.numel()), L.ptr(g2vals), L.ptr(imap_i_col), L.ptr(imap_f), ld_f, nk_f,
-           L.ptr(ikqs), L.ptr(w), nbandkq, nbandk, nm, nq_batch, ni_stride, i0, L.stream_handle())
+           L.ptr(ikqs), L.ptr(w), nbandkq, nbandk, nm, nq_batch, ni_stride, i0, L.ptr(dropped), L.stream_handle())
+    if check:
+        n = int(dropped.item())
+        if n:
+            raise L.EphipError(f"eph_window_scatter: {n} entries fell outside the output (k+q index or slot out of "
+                               f"range: check i0={i0}, ni_stride={ni_stride} and ikqs)")
 
 
 # ------------------------------------------------------------------------------------------------

@@ -391,27 +391,32 @@ class OuterKLoop:
                L.stream_handle())
         self.el_obj = _RqObject(self.epmat_el, st.irvec_el_dev, self.nd_el)
         self.irvec_ep_dev = st.epmat.irvec_dev()
-        # split-BF16: both contractions run on pre-packed operands. The "el" tensor is split once
-        # (same bytes as complex64) and its complex64 copy released; the q-grid phase table is built
-        # once; k phases and unit operands are packed per batch.
-        self.packed = precision == L.PREC_BF16X3
-        # split-BF16 with nw = 16, nm = 12, nbw <= 4: kR->kq and the rotations run as one kernel that
-        # writes g2 only (no g(k, q) staging buffer)
-        self.fused = bool(fuse_rotation) and self.packed and nw == 16 and nm == 12 and st.nbw <= 4
-        # split-BF16 with nw = 16: RR->kR and the k-side window projection run as one kernel on the "el"
-        # operand packed with j fastest; g(k, Rp) is never stored
-        self.fused_proj = bool(fuse_projection) and self.packed and nw == 16 and self.nd_el % 192 == 0
         # RR->kR as the staged (GridOpt) transform with the projection fused, in FP32 for either precision:
         # k1 planes and (k1, k2) lines of the grid share their partial sums
         ks_host = st.ksurv.cpu().numpy()
         self.gridopt = _GridOptKR.build(st, ks_host, self.nd_el, gridopt_mfma) if (gridopt and nw in (8, 16)) \
             else None
-        if self.gridopt is not None:
-            self.fused_proj = False
         # kR->kq as the staged transform over the uniform q grid with the rotations and g2 fused (FP32)
         self.gridopt_q = _GridOptKQ.build(st) if (gridopt and nw == 16 and nm <= 16 and st.nbw <= 4) else None
-        if self.gridopt_q is not None:
-            self.fused = False
+        # split-BF16: the contractions that are not staged run on pre-packed operands. The "el" tensor is
+        # split once (same bytes as complex64) and its complex64 copy released; the q-grid phase table is
+        # built once; k phases and unit operands are packed per batch.
+        self.packed = precision == L.PREC_BF16X3 and (self.gridopt is None or self.gridopt_q is None)
+        # split-BF16 with nw = 16, nm = 12, nbw <= 4: kR->kq and the rotations run as one kernel that
+        # writes g2 only (no g(k, q) staging buffer)
+        self.fused = (bool(fuse_rotation) and self.packed and nw == 16 and nm == 12 and st.nbw <= 4
+                      and self.gridopt_q is None)
+        # split-BF16 with nw = 16: RR->kR and the k-side window projection run as one kernel on the "el"
+        # operand packed with j fastest; g(k, Rp) is never stored
+        self.fused_proj = (bool(fuse_projection) and self.packed and nw == 16 and self.nd_el % 192 == 0
+                           and self.gridopt is None)
+        # arithmetic each contraction runs in: the staged kernels are FP32 whatever ``precision`` asks for
+        self.precision_requested = precision
+        self.stage_precision = {
+            "RR->kR": L.PREC_FP32 if self.gridopt is not None else precision,
+            "kR->kq": L.PREC_FP32 if self.gridopt_q is not None else precision,
+        }
+        self.precision_overridden = any(p != precision for p in self.stage_precision.values())
         import ctypes
         self.qgrid_c = (ctypes.c_int32 * 3)(*map(int, st.qgrid))  # for the on-the-fly q phases
         if self.gridopt is not None:
@@ -424,9 +429,10 @@ class OuterKLoop:
             if self.gridopt is None:
                 self.epmat_el = None
                 self.el_obj = None
-            self.bpk_q = core.ft_pack_phases(self.irvec_ep_dev, st.ph.xq)
-            self.bpk_k = torch.empty(core.ft_stages(self.nre) * self.nk_batch * 64, dtype=torch.bfloat16,
-                                     device=dev)
+                self.bpk_k = torch.empty(core.ft_stages(self.nre) * self.nk_batch * 64, dtype=torch.bfloat16,
+                                         device=dev)
+            if self.gridopt_q is None:
+                self.bpk_q = core.ft_pack_phases(self.irvec_ep_dev, st.ph.xq)
         # units of every surviving k, in k order
         bmin = st.el.filt.band_min.cpu().numpy()
         bmax = st.el.filt.band_max.cpu().numpy()
@@ -643,6 +649,8 @@ class OuterKLoop:
         shortest time for one full k batch."""
         from . import tune as T
         n = min(self.nk_batch, self.st.ksurv.shape[0])
+        if self.gridopt is not None and self.gridopt_q is not None:
+            return {}  # both contractions staged: no direct Fourier kernel to pick
         if self.packed:
             # the fused RR->kR + projection kernel has its own variants; tile shapes apply to the unfused one
             if self.gridopt is not None:

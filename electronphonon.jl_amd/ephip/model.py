@@ -241,7 +241,25 @@ CONFIGS = {
                  g_dtype=np.complex64, kgrid=(48, 48, 48), qgrid=(24, 24, 24), window=0.3),
     "cfg4": dict(nw=16, natoms=4, box_hd=9, box_g=7, seed=1004, onsite=(-3.0, 3.0),
                  g_dtype=np.complex64, kgrid=(64, 64, 64), qgrid=(32, 32, 32), window=0.3),
+    # cfg4's model on a 96^3 fine grid (learned-surrogate form: the band-energy network prunes the grid)
+    "cfg5": dict(nw=16, natoms=4, box_hd=9, box_g=7, seed=1004, onsite=(-3.0, 3.0),
+                 g_dtype=np.complex64, kgrid=(96, 96, 96), qgrid=(32, 32, 32), window=0.3),
 }
+
+
+def build_epmat(m: Model, workers: int = 16, dtype=np.complex64) -> np.ndarray:
+    """The "ph"-layout e-ph tensor of ``m`` (its seed and boxes), generated column by column in parallel
+    threads (numpy releases the GIL); the same values as ``make_model(with_g=True)`` without symmetrisation."""
+    import concurrent.futures as cf
+    nd = m.nw * m.nw * m.nmodes * len(m.irvec_el)
+    ep = np.empty((len(m.irvec_ep), nd), dtype=dtype)
+
+    def col(ip):
+        ep[ip] = eph_column(m.seed, m.nw, m.nmodes, m.irvec_el, m.irvec_ep[ip], ip, m.g_decay, dtype)
+
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        list(ex.map(col, range(len(m.irvec_ep))))
+    return ep
 
 
 def make_config_model(name: str, with_g: bool = True, **overrides) -> tuple[Model, dict]:

@@ -415,13 +415,16 @@ int ephip_batched_gemm(int32_t fp64, int32_t transA, int32_t transB, int32_t m, 
  * column-major; float64 when fp64 = 1, else float32, like w, g2_out and w_out), i = imap_i[n] and
  * f = imap_f[m + ld_f (ikqs[iq] - 1)] (int64, 1-based, 0 = outside the window; imap_f has nk_f columns);
  * when both are positive, slot nu + nm (i - i0 - 1) + nm ni_stride (f - 1) of g2_out and w_out (n_out
- * entries each) takes g2vals and w[nu + nm iq]. Out-of-range k+q indices and slots are skipped.
+ * entries each) takes g2vals and w[nu + nm iq]. Entries with a k+q index outside 1..nk_f or a slot
+ * outside [0, n_out) are not written; each adds 1 to the device counter *dropped (uint32, may be NULL;
+ * the caller zeroes it and reads it back), so a wrong i0 / ni_stride is reported, not lost. The
+ * reference has no such entries (its slots are in range by construction).
  * Replaces eph_window_scatter! (src/calculator/calculator_utils.jl:6-52; CUDA method
  * ext/ElectronPhononCUDAExt.jl:280-313). */
 int ephip_eph_window_scatter(int32_t fp64, void *g2_out, void *w_out, int64_t n_out, const void *g2vals,
                              const int64_t *imap_i, const int64_t *imap_f, int64_t ld_f, int64_t nk_f,
                              const int64_t *ikqs, const void *w, int32_t nbkq, int32_t nbk, int32_t nm, int64_t nq,
-                             int64_t ni_stride, int64_t i0, void *stream);
+                             int64_t ni_stride, int64_t i0, uint32_t *dropped, void *stream);
 
 /* Full rotated operator of nk k-points: out[k ld_out + a nw^2 + m + nw n] = (U_k^H M_a(k) U_k)[m, n] in FP64
  * for the three directions a, with M (complex128, column k of ld_m >= 3 nw^2 entries, direction slowest)

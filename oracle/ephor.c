@@ -13,8 +13,9 @@
  *   - O(k) = sum_R exp(+2 pi i k.R) O(R), k in crystal coordinates, R integer
  *     (src/wannier/WannierInterpolator.jl:127,135).
  *
- * How each routine is pinned (beyond restating the reference) is listed in oracle/README.md and
- * in the tests that exercise it.
+ * How each routine is pinned (beyond restating the reference) is stated in tests/test_oracle.py and
+ * tests/test_bte_oracle.py, which check it against independent references (numpy FFT and LAPACK,
+ * closed forms, the reference's own golden vectors).
  */
 #define _GNU_SOURCE
 #include <complex.h>
@@ -81,7 +82,7 @@ void ephor_fourier(int64_t ndata, int nr, const zc *op, int64_t ldop, const int3
  * src/diagonalize.jl:35-47). Restated from the published method: Householder reduction of the
  * Hermitian matrix to a real symmetric tridiagonal (Golub & Van Loan, Matrix Computations 4th ed.
  * sec. 8.3.1, with the complex phase normalisation of the off-diagonal), implicit QL iterations
- * with Wilkinson-type shifts (the EISPACK tql2 scheme), and back-transformation of the
+ * with Wilkinson-type shifts (the EISPACK imtql2 scheme), and back-transformation of the
  * eigenvectors. Eigenvalues are returned ascending; eigenvectors are orthonormal columns. The
  * upper triangle of A is read ('U'), A is destroyed.
  * ------------------------------------------------------------------------------------------- */

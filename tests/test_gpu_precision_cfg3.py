@@ -67,12 +67,21 @@ def test_cfg3_eigenvalues(cfg3):
 
 @pytest.mark.parametrize("prec_name", ["fp32", "bf16x3"])
 def test_cfg3_g2(cfg3, prec_name):
+    """fp32: the default staged (GridOpt) FP32 kernels. bf16x3: the direct split-BF16 MFMA contractions
+    (gridopt=False: the packed RR->kR fused with the k-side projection and the packed kR->kq fused with the
+    rotations and g2), asserted to be the kernels that ran."""
     from ephip import _lib as L
     from ephip import eph
     m, g, st = cfg3
     prec = L.PREC_FP32 if prec_name == "fp32" else L.PREC_BF16X3
     nw, nm = m.nw, m.nmodes
-    loop = eph.OuterKLoop(st, nk_batch=32, precision=prec)
+    if prec_name == "bf16x3":
+        loop = eph.OuterKLoop(st, nk_batch=32, precision=prec, gridopt=False)
+        assert loop.gridopt is None and loop.gridopt_q is None and loop.fused and loop.fused_proj
+        assert loop.stage_precision == {"RR->kR": L.PREC_BF16X3, "kR->kq": L.PREC_BF16X3}
+    else:
+        loop = eph.OuterKLoop(st, nk_batch=32, precision=prec)
+        assert loop.gridopt is not None and loop.gridopt_q is not None
     rec = eph.UnitRecorder()
     ks = st.ksurv.cpu().numpy()
     n3 = g["kgrid"][2]

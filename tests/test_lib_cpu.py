@@ -75,11 +75,34 @@ def test_generic_backend_methods_validate_arguments():
     L.call("ephip_batched_gemm", 1, ord("N"), ord("C"), 4, 2, 3, 0, None, 4, 12, None, 2, 6, None, 4, 8, None)
     # eph_window_scatter!: imap_f's leading dimension must hold the k+q bands
     with pytest.raises(L.EphipError, match="ld_f"):
-        L.call("ephip_eph_window_scatter", 1, 1, 1, 10, 1, 1, 1, 2, 5, 1, 1, 3, 2, 2, 1, 4, 0, None)
-    L.call("ephip_eph_window_scatter", 0, None, None, 0, None, None, None, 3, 5, None, None, 3, 2, 2, 0, 4, 0, None)
+        L.call("ephip_eph_window_scatter", 1, 1, 1, 10, 1, 1, 1, 2, 5, 1, 1, 3, 2, 2, 1, 4, 0, None, None)
+    L.call("ephip_eph_window_scatter", 0, None, None, 0, None, None, None, 3, 5, None, None, 3, 2, 2, 0, 4, 0, None,
+           None)
     # velocity matrices: nw <= 32, energies and rbar together
     with pytest.raises(L.EphipError, match="nw=33"):
         L.call("ephip_el_velocity_direct_batched", 33, 1, 1, 3 * 33 * 33, 1, None, None, 0, 1, 3 * 33 * 33, None)
     with pytest.raises(L.EphipError, match="come together"):
         L.call("ephip_el_velocity_direct_batched", 8, 1, 1, 192, 1, 1, None, 192, 1, 192, None)
     L.call("ephip_el_velocity_direct_batched", 8, 0, None, 192, None, None, None, 0, None, 192, None)
+
+
+def test_velocity_wrappers_reject_wrong_dtypes_before_launch():
+    """The velocity kernels read the operator as FP64 complex and U as complex64; a complex64 operator
+    (what to_device gives without a dtype) would be read past its end, so the wrappers refuse it first."""
+    torch = pytest.importorskip("torch")
+    import numpy as np
+    from ephip import bte, core, eph
+    nw, nk = 4, 3
+    irvec = np.zeros((1, 3), np.int32)
+    el = eph.ElectronStates(torch.zeros((nk, 3), dtype=torch.float64), torch.zeros((nk, nw), dtype=torch.float64),
+                            torch.zeros((nk, nw, nw), dtype=torch.complex64), None, None)
+    op32 = core.WannierObject(irvec, torch.zeros((1, 3 * nw * nw), dtype=torch.complex64))
+    with pytest.raises(L.EphipError, match="complex128"):
+        bte.el_velocity_direct_batched(op32, el, [0, 1])
+    bs = bte.BandStates.__new__(bte.BandStates)
+    with pytest.raises(L.EphipError, match="complex128"):
+        bte.compute_velocities(op32, el, bs)
+    op64 = core.WannierObject(irvec, torch.zeros((1, 3 * nw * nw), dtype=torch.complex128))
+    el_u64 = eph.ElectronStates(el.xk, el.E, el.U.to(torch.complex128), None, None)
+    with pytest.raises(L.EphipError, match="complex64"):
+        bte.el_velocity_direct_batched(op64, el_u64, [0])
