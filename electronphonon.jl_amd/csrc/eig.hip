@@ -489,3 +489,37 @@ extern "C" int ephip_heev_refine(int32_t n, int64_t nb, const void *H64, void *U
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }
+
+// eigen_batched / eigvals_batched with the FP64 accuracy of the reference's zheev: the FP32 wave solve
+// followed by the FP64 refinement, as one entry each. Per matrix, eigvals_batched keeps the FP32
+// eigenvectors (8 n^2 bytes) and eigenvalues (4 n bytes) in the caller's workspace: the refinement needs
+// the vectors even when only values are returned.
+static int64_t eigvals_work_bytes_per_matrix(int n) { return 8LL * n * n + 4LL * n; }
+
+static int eigen_refined_run(int32_t n, int64_t nb, const void *H64, double *E64, void *U, float *E32, int32_t *info,
+                             void *stream) {
+    int st = ephip_heev_c32(n, nb, H64, 1, U, E32, info, stream);
+    if (st) return st;
+    return ephip_heev_refine(n, nb, H64, U, E32, E64, 0.0, stream);
+}
+
+extern "C" int ephip_eigen_batched(int32_t n, int64_t nb, const void *H64, double *E64, void *U, float *work,
+                                   int32_t *info, void *stream) {
+    int st = heev_launch_check(n, nb);
+    if (st) return st;
+    EPHIP_REQUIRE(n <= 31, "eigen_batched: n=%d > 31 (FP64 refinement limit)", n);
+    if (nb == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(H64 && E64 && U && work, "eigen_batched: null device pointer");
+    return eigen_refined_run(n, nb, H64, E64, U, work, info, stream);
+}
+
+extern "C" int ephip_eigvals_batched(int32_t n, int64_t nb, const void *H64, double *E64, void *work, int32_t *info,
+                                     void *stream) {
+    int st = heev_launch_check(n, nb);
+    if (st) return st;
+    EPHIP_REQUIRE(n <= 31, "eigvals_batched: n=%d > 31 (FP64 refinement limit)", n);
+    if (nb == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(H64 && E64 && work, "eigvals_batched: null device pointer");
+    char *w = static_cast<char *>(work);
+    return eigen_refined_run(n, nb, H64, E64, w, reinterpret_cast<float *>(w + 8LL * n * n * nb), info, stream);
+}

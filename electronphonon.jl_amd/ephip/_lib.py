@@ -26,6 +26,8 @@ SIGNATURES = {
     "ephip_fourier_c64": [_i64, _i32, _vp, _i64, _vp, _i32, _vp, _vp, _i64, _vp],
     "ephip_heev_c32": [_i32, _i64, _vp, _i32, _vp, _vp, _vp, _vp],
     "ephip_heev_refine": [_i32, _i64, _vp, _vp, _vp, _vp, _d, _vp],
+    "ephip_eigen_batched": [_i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
+    "ephip_eigvals_batched": [_i32, _i64, _vp, _vp, _vp, _vp, _vp],
     "ephip_window_filter": [_i32, _i64, _vp, _d, _d, _vp, _vp, _vp, _vp, _vp],
     "ephip_ph_mass_scale": [_i32, _i64, _vp, _vp, _vp],
     "ephip_ph_finalize": [_i32, _i64, _vp, _vp, _vp, _vp, _vp],
@@ -55,7 +57,7 @@ SIGNATURES = {
     "ephip_batched_gemm": [_i32, _i32, _i32, _i32, _i32, _i32, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp,
                            _i64, _i64, _vp],
     "ephip_eph_window_scatter": [_i32, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _i64, _vp, _vp, _i32, _i32, _i32,
-                                 _i64, _i64, _i64, _vp],
+                                 _i64, _i64, _i64, _vp, _vp],
     "ephip_bte_spmv": [_i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ephip_mlp_bf16_prune": [_i64, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, C.c_float, C.c_float,
                              C.c_float, _vp, _vp, _vp],
@@ -119,6 +121,8 @@ def exported_symbols() -> list[str]:
 
 def call(name: str, *args) -> None:
     lib = load()
+    if name in SIGNATURES and len(args) != len(SIGNATURES[name]):
+        raise EphipError(f"{name}: {len(args)} arguments given, the C-ABI takes {len(SIGNATURES[name])}")
     st = getattr(lib, name)(*args)
     if st != EPHIP_OK:
         msg = lib.ephip_last_error().decode(errors="replace")

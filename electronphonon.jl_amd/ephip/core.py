@@ -361,12 +361,42 @@ def eigen_batched_refined(H64, tol: float = 0.0):
     H64 = _as_stack(H64).contiguous()
     if H64.dtype != torch.complex128:
         raise L.EphipError("eigen_batched_refined needs complex128 matrices")
-    E32, U = _heev(H64, vectors=True)
-    nb, n = E32.shape
-    E64 = torch.empty((nb, n), dtype=torch.float64, device=H64.device)
-    L.call("ephip_heev_refine", n, nb, L.ptr(H64), L.ptr(U), L.ptr(E32), L.ptr(E64), float(tol),
-           L.stream_handle())
+    nb, n = H64.shape[0], H64.shape[1]
+    dev = H64.device
+    E64 = torch.empty((nb, n), dtype=torch.float64, device=dev)
+    U = torch.empty((nb, n, n), dtype=torch.complex64, device=dev)
+    if tol == 0.0:   # the reference-shaped entry (ephip_eigen_batched)
+        work = torch.empty((nb, n), dtype=torch.float32, device=dev)
+        info = torch.zeros(nb, dtype=torch.int32, device=dev)
+        L.call("ephip_eigen_batched", n, nb, L.ptr(H64), L.ptr(E64), L.ptr(U), L.ptr(work), L.ptr(info),
+               L.stream_handle())
+    else:
+        E32, U = _heev(H64, vectors=True)
+        info = None
+        L.call("ephip_heev_refine", n, nb, L.ptr(H64), L.ptr(U), L.ptr(E32), L.ptr(E64), float(tol),
+               L.stream_handle())
+    if info is not None and nb and int(info.max()) != 0:
+        raise L.EphipError(f"eigen: {int((info != 0).sum())} matrices did not converge")
     return E64, U
+
+
+def eigvals_batched_refined(H64):
+    """eigvals_batched (src/wannier_to_bloch_batched.jl:30-41) at FP64 accuracy through the C-ABI entry
+    ephip_eigvals_batched: E64 (nk, n) float64 ascending of the complex128 stack ``H64``; the FP32
+    eigenvectors the refinement needs live in a workspace freed on return."""
+    torch = _torch()
+    H64 = _as_stack(H64).contiguous()
+    if H64.dtype != torch.complex128:
+        raise L.EphipError("eigvals_batched_refined needs complex128 matrices")
+    nb, n = H64.shape[0], H64.shape[1]
+    dev = H64.device
+    E64 = torch.empty((nb, n), dtype=torch.float64, device=dev)
+    work = torch.empty(max(1, nb * (8 * n * n + 4 * n)), dtype=torch.uint8, device=dev)
+    info = torch.zeros(nb, dtype=torch.int32, device=dev)
+    L.call("ephip_eigvals_batched", n, nb, L.ptr(H64), L.ptr(E64), L.ptr(work), L.ptr(info), L.stream_handle())
+    if nb and int(info.max()) != 0:
+        raise L.EphipError(f"eigvals: {int((info != 0).sum())} matrices did not converge")
+    return E64
 
 
 # ------------------------------------------------------------------------------------------------

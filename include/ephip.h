@@ -88,6 +88,21 @@ int ephip_heev_c32(int32_t n, int64_t nb, const void *H, int32_t h_is_c64, void 
 int ephip_heev_refine(int32_t n, int64_t nb, const void *H64, void *U, const float *E32, double *E64,
                       double tol, void *stream);
 
+/* eigen_batched(Hk) -- src/wannier_to_bloch_batched.jl:43-65 (CUDA method ext/ElectronPhononCUDAExt.jl:101-115),
+ * at the FP64 accuracy of the reference's zheev path: ephip_heev_c32 then ephip_heev_refine. H64: c64
+ * (n x n x nb), upper triangle read, not modified. E64: double (n x nb) ascending; U: c32 (n x n x nb)
+ * eigenvectors; work: float (n x nb) device workspace (the FP32 eigenvalues); info: int32 per matrix
+ * (0 = the FP32 solve converged) or NULL. n <= 31. */
+int ephip_eigen_batched(int32_t n, int64_t nb, const void *H64, double *E64, void *U, float *work, int32_t *info,
+                        void *stream);
+
+/* eigvals_batched(Hk) -- src/wannier_to_bloch_batched.jl:30-41 (CUDA method ext/ElectronPhononCUDAExt.jl:80-93):
+ * eigenvalues only, E64 double (n x nb) ascending, FP64-refined as in ephip_eigen_batched. work: device
+ * workspace of nb (8 n^2 + 4 n) bytes, 8-byte aligned (the FP32 eigenvectors the refinement needs, then the
+ * FP32 eigenvalues). info as in ephip_eigen_batched. n <= 31. */
+int ephip_eigvals_batched(int32_t n, int64_t nb, const void *H64, double *E64, void *work, int32_t *info,
+                          void *stream);
+
 /* _filter_kpoints / inside_window -- src/filter.jl:19, :40-100 (device branch :46-72).
  * E: double (nw x nk) ascending. keep[k] = any band in [wmin, wmax]; bmin/bmax: 1-based
  * searchsortedfirst(E, wmin) / searchsortedlast(E, wmax), 0 when keep = 0; nbelow[k] = bmin - 1

@@ -106,3 +106,25 @@ def test_velocity_wrappers_reject_wrong_dtypes_before_launch():
     el_u64 = eph.ElectronStates(el.xk, el.E, el.U.to(torch.complex128), None, None)
     with pytest.raises(L.EphipError, match="complex64"):
         bte.el_velocity_direct_batched(op64, el_u64, [0])
+
+
+def test_binding_arity_matches_header():
+    """Every ctypes signature has exactly as many arguments as the C declaration (an extra argument would
+    be converted by ctypes' defaults, truncating a 64-bit stream handle to an int)."""
+    src = open(os.path.join(ROOT, "include", "ephip.h")).read()
+    decls = re.findall(r"^int (ephip_\w+)\(([^;]*)\);", src, flags=re.M)
+    assert decls
+    for name, params in decls:
+        params = " ".join(params.split())
+        n = 0 if params in ("", "void") else params.count(",") + 1
+        if name in L.SIGNATURES:
+            assert len(L.SIGNATURES[name]) == n, (name, len(L.SIGNATURES[name]), n)
+
+
+def test_eigen_entries_validate_arguments():
+    with pytest.raises(L.EphipError, match="n=32 > 31"):
+        L.call("ephip_eigen_batched", 32, 4, 1, 1, 1, 1, None, None)
+    with pytest.raises(L.EphipError, match="null device pointer"):
+        L.call("ephip_eigvals_batched", 16, 4, 1, 1, None, None, None)
+    L.call("ephip_eigvals_batched", 16, 0, None, None, None, None, None)
+    L.call("ephip_eigen_batched", 8, 0, None, None, None, None, None, None)
