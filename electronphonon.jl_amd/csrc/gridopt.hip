@@ -856,6 +856,233 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
     }
 }
 
+// The same staged kR->kq with stage 3 and the k+q-window rotation as one contraction on FP32 MFMAs.
+// Per q line (q1, q2) and per column (kept q, k+q window band m) of the line:
+//   t[nu][(q, m)] = sum_{i, c} B2[i + 16 nu][c] W[(i, c)][(q, m)],  W = conj(U_kq[i, mlo + m]) e3[q3][c],
+// a (16 x 16 N) by (16 N x columns) product: nu on the MFMA rows (nm <= 16), the columns in tiles of 16
+// (only the k+q window's bands, packed per line), K = (i, c) split over the four waves by i (wave w
+// takes i = 4w .. 4w + 3), W built in the B operand from U_kq and the q3 phases, the complex product
+// from three real MFMAs (Gauss). Stages 1 and 2 stay on the vector ALUs with one row G = i + 16 nu per
+// thread (threads t < 16 nm, coalesced loads); the stage-2 sums B2 of a line pass through LDS into the
+// MFMA operand layout (lane: nu = lane % 16, i = 4 wave + lane / 16). The four waves' partial tiles are
+// summed through LDS into t, and the phonon rotation and g2 run on up to GQM_TQ collected columns at a
+// time. Output: the layout of gridopt_kq_rotate_kernel (zeros past the k+q window).
+constexpr int GQM_TQ = 64;
+
+template <int N, int NMC = 0>  // NMC: compile-time mode count (12), or 0 for nm <= 16 at run time
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 : 2))) void
+gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min, const int *__restrict__ rmap,
+                              int m1, int m2, int m3, const cplx<float> *__restrict__ A, int64_t lda, int64_t ldr,
+                              const int *__restrict__ line_off, const int *__restrict__ col_index,
+                              const int *__restrict__ ikq, int64_t cs, const cplx<float> *__restrict__ U,
+                              const int64_t *__restrict__ bmin, const int64_t *__restrict__ bmax,
+                              const cplx<float> *__restrict__ uph, const double *__restrict__ omega, int nbw,
+                              float *__restrict__ g2) {
+    constexpr int NW = 16;
+    constexpr int NE = (N + 1) & ~1;
+    constexpr int ROWS = NMC > 0 ? NW * NMC : 256;
+    const int nm = NMC > 0 ? NMC : nm_rt;
+    // dynamic: q2 and q3 phase rows (m2, m3 rows of NE / 2 float4), the slab's k+q states and metadata words
+    extern __shared__ float4 gqm_dyn[];
+    float4 *ph2 = gqm_dyn;
+    float4 *ph3 = ph2 + m2 * (NE / 2);
+    int *sk_kq = reinterpret_cast<int *>(ph3 + m3 * (NE / 2));
+    int *sk_nb = sk_kq + m2 * m3;
+    __shared__ float2 ph1[GO_N1_MAX];
+    __shared__ int slo[GQ_M2_MAX + 1];
+    __shared__ float2 b2s[N][ROWS];          // stage-2 sums of the current line, [c][G]
+    __shared__ f32x4 red[2][2][4][64];       // the waves' partial tiles (real, imaginary), two buffers
+    __shared__ int colk[2][GQ_M3_MAX * 4];   // the line's columns: slab entry | band slot << 16
+    __shared__ int ncols[2];
+    __shared__ float2 tq[GQM_TQ][NW];        // t of the collected columns, [column][nu]
+    __shared__ int tqid[GQM_TQ];
+    const int q1 = blockIdx.x, u = blockIdx.y;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const bool rv = t < NW * nm;
+    const int nlines = m1 * m2;
+    const int *lo_u = line_off + (int64_t)u * (nlines + 1);
+    const int64_t row0 = (int64_t)u * cs;
+    for (int l = t; l <= m2; l += blockDim.x) slo[l] = lo_u[q1 * m2 + l];
+    if (t < n1) {
+        float c, s;
+        cis2pi((double)q1 / m1 * (double)(p1min + t), c, s);
+        ph1[t] = make_float2(c, s);
+    }
+    for (int e = t; e < m2 * NE; e += blockDim.x) {
+        float c = 0.f, s = 0.f;
+        if (e % NE < N) cis2pi((double)(e / NE) / m2 * (double)(p2min + e % NE), c, s);
+        reinterpret_cast<float2 *>(ph2)[e] = make_float2(c, s);
+    }
+    for (int e = t; e < m3 * NE; e += blockDim.x) {
+        float c = 0.f, s = 0.f;
+        if (e % NE < N) cis2pi((double)(e / NE) / m3 * (double)(p3min + e % NE), c, s);
+        reinterpret_cast<float2 *>(ph3)[e] = make_float2(c, s);
+    }
+    __syncthreads();
+    const int slab0 = slo[0], nslab = slo[m2] - slab0;  // <= m2 m3
+    for (int kk = t; kk < nslab; kk += blockDim.x) {
+        const int64_t row = row0 + slab0 + kk;
+        const int kq = ikq[row];
+        const int iq = col_index[row];
+        sk_kq[kk] = kq;
+        const int64_t b0 = bmin[kq], b1 = bmax[kq];
+        const int nb = b0 > 0 ? min((int)(b1 - b0 + 1), nbw) : 0;
+        sk_nb[kk] = nb | ((b0 > 0 ? (int)b0 - 1 : 0) << 8) | ((iq % m3) << 16) | (((iq / m3) % m2) << 22);
+        float *o = g2 + row * nbw * nm;
+        for (int e = nb * nm; e < nbw * nm; e++) o[e] = 0.f;  // slots past the k+q window
+    }
+    // stage 1 (rows G = t): one row of the face at a time (N independent loads in flight)
+    float br[N][N], bi[N][N];
+#pragma unroll
+    for (int c = 0; c < N; c++)
+#pragma unroll
+        for (int b = 0; b < N; b++) br[c][b] = bi[c][b] = 0.f;
+    const cplx<float> *a_u = A + (int64_t)u * lda + (rv ? t : 0);
+#pragma unroll
+    for (int c = 0; c < N; c++)
+        for (int a = 0; a < n1; a++) {
+            const float2 p = ph1[a];
+            cplx<float> v[N];
+#pragma unroll
+            for (int b = 0; b < N; b++) {
+                const int ir = rmap[b + 8 * (a + n1 * c)];
+                v[b] = a_u[ldr * max(ir, 0)];
+                if (ir < 0) v[b] = cplx<float>{0.f, 0.f};
+            }
+#pragma unroll
+            for (int b = 0; b < N; b++) cmac(br[c][b], bi[c][b], p.x, p.y, v[b].re, v[b].im);
+        }
+    // the phonon rotation and g2 of the collected columns: ep[nu'] = sum_nu t[nu] uph(q)[nu, nu']
+    int tqn = 0;
+    auto flush = [&]() {
+        __syncthreads();  // t of the collected columns written
+        for (int e = t; e < tqn * nm; e += blockDim.x) {
+            const int col = e / nm, nu2 = e - col * nm;
+            const int cid = tqid[col], kk = cid & 0xffff, m = cid >> 16;
+            const int meta = sk_nb[kk];
+            const int iq = (q1 * m2 + ((meta >> 22) & 63)) * m3 + ((meta >> 16) & 63);
+            const cplx<float> *up = uph + (int64_t)nm * nm * iq + nm * nu2;
+            float er = 0.f, ei = 0.f;
+            if constexpr (NMC > 0) {
+#pragma unroll
+                for (int n = 0; n < NMC; n++) {
+                    const float2 tv = tq[col][n];
+                    const cplx<float> p = up[n];
+                    cmac(er, ei, tv.x, tv.y, p.re, p.im);
+                }
+            } else {
+#pragma unroll
+                for (int n = 0; n < NW; n++) {
+                    const int nn = min(n, nm - 1);
+                    const float on = n < nm ? 1.f : 0.f;
+                    const float2 tv = tq[col][nn];
+                    const cplx<float> p = up[nn];
+                    cmac(er, ei, on * tv.x, on * tv.y, p.re, p.im);
+                }
+            }
+            const float w2 = 2.f * static_cast<float>(omega[(int64_t)nm * iq + nu2]);
+            g2[(row0 + slab0 + kk) * nbw * nm + m * nm + nu2] = (er * er + ei * ei) / w2;
+        }
+        __syncthreads();  // the collected columns consumed
+        tqn = 0;
+    };
+    // MFMA operand coordinates of this lane: row nu = lane % 16, K index i = 4 wave + lane / 16
+    const int anu = lane & 15, ai_ = 4 * wave + (lane >> 4);
+    const bool aok = anu < nm;
+    int tb = 0, lb = 0;
+    for (int q2 = 0; q2 < m2; q2++) {  // uniform
+        const int eb = slo[q2] - slab0, ee = slo[q2 + 1] - slab0;
+        if (ee == eb) continue;
+        if (rv) {  // stage 2 of this line for row G = t
+            float2 e2[NE];
+#pragma unroll
+            for (int h = 0; h < NE / 2; h++) {
+                const float4 v = ph2[q2 * (NE / 2) + h];
+                e2[2 * h] = make_float2(v.x, v.y);
+                e2[2 * h + 1] = make_float2(v.z, v.w);
+            }
+#pragma unroll
+            for (int c = 0; c < N; c++) {
+                float sr = 0.f, si = 0.f;
+#pragma unroll
+                for (int b = 0; b < N; b++) cmac(sr, si, e2[b].x, e2[b].y, br[c][b], bi[c][b]);
+                b2s[c][t] = make_float2(sr, si);
+            }
+        }
+        if (wave == 3) {  // the line's columns (entry, band slot): exclusive prefix of the band counts (<= 4)
+            const int ne = ee - eb;  // <= m3 <= 64
+            const int nb = lane < ne ? (sk_nb[eb + lane] & 0xff) : 0;
+            const unsigned long long below = (1ull << lane) - 1ull;
+            int start = 0;
+#pragma unroll
+            for (int bit = 0; bit < 3; bit++) start += __popcll(__ballot((nb >> bit) & 1) & below) << bit;
+            for (int m = 0; m < nb; m++) colk[lb][start + m] = (eb + lane) | (m << 16);
+            if (lane == 63) ncols[lb] = start + nb;
+        }
+        __syncthreads();  // B2 and the column table of this line
+        const int ncol = ncols[lb];
+        if (ncol > 0) {  // uniform
+            for (int col0 = 0; col0 < ncol; col0 += 16) {  // uniform
+                if (tqn + 16 > GQM_TQ) flush();
+                const int j = col0 + (lane & 15);
+                int cid = 0, q3 = 0;
+                float ur = 0.f, ui = 0.f;
+                if (j < ncol) {
+                    cid = colk[lb][j];
+                    const int kk = cid & 0xffff, m = cid >> 16;
+                    const int meta = sk_nb[kk];
+                    q3 = (meta >> 16) & 63;
+                    const cplx<float> v = U[(int64_t)NW * NW * sk_kq[kk] + ai_ + NW * (((meta >> 8) & 0xff) + m)];
+                    ur = v.re;
+                    ui = v.im;
+                }
+                // t = sum B2 conj(U) e3 from three real products (Gauss):
+                //   re = b_r w_r - b_i w_i,  im = (b_r + b_i)(w_r + w_i) - b_r w_r - b_i w_i
+                f32x4 prr = {0.f, 0.f, 0.f, 0.f}, pii = prr, pss = prr;
+#pragma unroll
+                for (int h = 0; h < NE / 2; h++) {
+                    const float4 ev = ph3[q3 * (NE / 2) + h];
+#pragma unroll
+                    for (int s = 0; s < 2; s++) {
+                        const int c = 2 * h + s;
+                        if (c < N) {
+                            const float ex = s ? ev.z : ev.x, ey = s ? ev.w : ev.y;
+                            const float wr = ur * ex + ui * ey, wi = ur * ey - ui * ex;
+                            // A operand: this lane's B2 value, read per tile (the barrier below orders it)
+                            const float2 av = aok ? b2s[c][ai_ + NW * anu] : make_float2(0.f, 0.f);
+                            prr = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, wr, prr, 0, 0, 0);
+                            pii = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, wi, pii, 0, 0, 0);
+                            pss = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x + av.y, wr + wi, pss, 0, 0, 0);
+                        }
+                    }
+                }
+                // this wave's partial: lane holds rows nu = 4 (lane / 16) + r, column lane % 16
+                red[tb][0][wave][lane] = prr - pii;
+                red[tb][1][wave][lane] = pss - prr - pii;
+                __syncthreads();
+                {
+                    const int nu = t >> 4, col = t & 15, src = col + 16 * (nu >> 2), r = nu & 3;
+                    const float *rf = reinterpret_cast<const float *>(&red[tb][0][0][0]);
+                    float sr = 0.f, si = 0.f;
+#pragma unroll
+                    for (int w = 0; w < 4; w++) {
+                        sr += rf[(w * 64 + src) * 4 + r];
+                        si += rf[((4 + w) * 64 + src) * 4 + r];
+                    }
+                    if (col0 + col < ncol) {
+                        if (nu < nm) tq[tqn + col][nu] = make_float2(sr, si);
+                        if (nu == 0) tqid[tqn + col] = cid;  // lanes 0..15 of wave 0 built column col0 + col
+                    }
+                }
+                tqn += min(16, ncol - col0);
+                tb ^= 1;
+            }
+        }
+        lb ^= 1;
+    }
+    flush();
+}
+
 }  // namespace ephip
 
 extern "C" int ephip_kq_line_offsets(int32_t nunits, const int32_t *count, const int32_t *col_index, int64_t cs,
@@ -888,9 +1115,25 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
     if (nunits == 0) return EPHIP_OK;
     EPHIP_REQUIRE(rmap && A && line_off && col_index && ikq && U && bmin && bmax && uph && omega && g2,
                   "gridopt_kq_rotate: null device pointer");
+    dim3 g((unsigned)qgrid[0], (unsigned)nunits);
+    if (ephip_ft_get_variant(6) == 1) {  // stage 3 and the k+q-window rotation on MFMAs
+        using KM = decltype(&gridopt_kq_rotate_mfma_kernel<3>);
+#define GQM_ROW(c) {gridopt_kq_rotate_mfma_kernel<3, c>, gridopt_kq_rotate_mfma_kernel<4, c>, \
+                    gridopt_kq_rotate_mfma_kernel<5, c>, gridopt_kq_rotate_mfma_kernel<6, c>, \
+                    gridopt_kq_rotate_mfma_kernel<7, c>, gridopt_kq_rotate_mfma_kernel<8, c>}
+        static const KM mtable[2][6] = {GQM_ROW(0), GQM_ROW(12)};
+#undef GQM_ROW
+        const int ne = (np + 1) & ~1;
+        const size_t lds = (size_t)(qgrid[1] + qgrid[2]) * (ne / 2) * 16 + 2 * (size_t)qgrid[1] * qgrid[2] * 4;
+        mtable[nm == 12 ? 1 : 0][np - 3]<<<g, 256, lds, as_stream(stream)>>>(
+            nm, n1, p1min, p2min, p3min, rmap, qgrid[0], qgrid[1], qgrid[2], static_cast<const cplx<float> *>(A), lda,
+            ldr, line_off, col_index, ikq, cs, static_cast<const cplx<float> *>(U), bmin, bmax,
+            static_cast<const cplx<float> *>(uph), omega, nbw, g2);
+        EPHIP_LAUNCHED();
+        return EPHIP_OK;
+    }
     const int nd2 = nw * nm;
     const unsigned threads = (unsigned)(((nd2 + 63) / 64) * 64);
-    dim3 g((unsigned)qgrid[0], (unsigned)nunits);
     using K = decltype(&gridopt_kq_rotate_kernel<3, 2>);
 #define GQ_ROW(nb, c) {gridopt_kq_rotate_kernel<3, nb, c>, gridopt_kq_rotate_kernel<4, nb, c>, \
                        gridopt_kq_rotate_kernel<5, nb, c>, gridopt_kq_rotate_kernel<6, nb, c>, \

@@ -639,7 +639,13 @@ class OuterKLoop:
         if self.gridopt_q is not None:
             gq = self.gridopt_q
             n2n3 = int((gq.rmap.view(gq.np, gq.n1, 8) >= 0).any(1).sum().item())  # (c, b) columns present
-            f_kq = 8.0 * nd2 * (self.nrp * gq.m[0] * nu + n2n3 * gq.nlines * nu + gq.np * kept) + rot
+            from . import tune as T
+            if T.get_variant(T.KIND_GRIDOPT_KQ) == 1:
+                # stage 3 and the k+q-window rotation as one contraction over (i, c) per window band slot
+                f_kq = (8.0 * nd2 * (self.nrp * gq.m[0] * nu + n2n3 * gq.nlines * nu + gq.np * slots)
+                        + 8.0 * nm * nm * slots)
+            else:
+                f_kq = 8.0 * nd2 * (self.nrp * gq.m[0] * nu + n2n3 * gq.nlines * nu + gq.np * kept) + rot
         else:
             f_kq = 8.0 * nd2 * self.nrp * kept + (rot if self.fused else 0.0)
         return f_rr, f_kq
@@ -650,7 +656,9 @@ class OuterKLoop:
         from . import tune as T
         n = min(self.nk_batch, self.st.ksurv.shape[0])
         if self.gridopt is not None and self.gridopt_q is not None:
-            return {}  # both contractions staged: no direct Fourier kernel to pick
+            # both contractions staged: the staged kR->kq kernel's form (vector ALUs or MFMAs)
+            return {"gridopt_kq": T.tune_fourier(T.KIND_GRIDOPT_KQ, lambda: self.run_kbatch(0, n), reps=reps,
+                                                 variants=T.GRIDOPT_KQ_VARIANTS)}
         if self.packed:
             # the fused RR->kR + projection kernel has its own variants; tile shapes apply to the unfused one
             if self.gridopt is not None:

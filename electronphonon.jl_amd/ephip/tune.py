@@ -12,6 +12,10 @@ from __future__ import annotations
 from . import _lib as L
 
 KIND_FP32_WIDE, KIND_FP32_NARROW, KIND_FP64, KIND_PACKED_BF16, KIND_FUSED_ROT, KIND_FUSED_PROJ = 0, 1, 2, 3, 4, 5
+# the staged (GridOpt) kR->kq kernel: stage 3 and the k+q-window rotation on the vector ALUs or on MFMAs
+KIND_GRIDOPT_KQ = 6
+GRIDOPT_KQ_VARIANTS = (0, 1)
+GRIDOPT_KQ_VARIANT_NAMES = {0: "VALU stage 3 + DPP window sums", 1: "MFMA stage 3 + window rotation"}
 VARIANTS = (0, 1, 2)
 VARIANT_NAMES = {0: "single-buffered KC16", 1: "double-buffered KC16", 2: "double-buffered KC8"}
 PACKED_VARIANTS = (0, 1, 2, 3, 4, 5, 6, 7)

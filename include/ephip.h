@@ -336,7 +336,8 @@ int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t nbw, int32
  * 9, 10 = as 7, 8 with the epilogue's column metadata read before the main loop and the U_kq / u_ph
  * values of the next column block fetched ahead, 11, 12 = as 9, 10 with the main loop's global loads
  * issued two stages ahead. kind 5 = the fused RR->kR + projection kernel; variant 0 = loads one stage
- * ahead, 1 = two stages ahead.
+ * ahead, 1 = two stages ahead. kind 6 = the staged kR->kq kernel (ephip_gridopt_kq_rotate); variant 0 =
+ * stage 3 and the k+q-window rotation on the vector ALUs, 1 = as one contraction on FP32 MFMAs.
  * Process-wide setting (no counterpart in the reference, whose tuning constants are hard-coded:
  * src/wannier_to_bloch_batched.jl:397-406). Host-side, synchronous. */
 int ephip_ft_set_variant(int32_t kind, int32_t variant);

@@ -21,6 +21,17 @@ def cfg1():
     return m, g
 
 
+@pytest.fixture(params=[0, 1], ids=["kq-valu", "kq-mfma"])
+def kq_variant(request):
+    """The staged kR->kq kernel form (tuner kind 6): 0 = stage 3 and the k+q-window rotation on the vector
+    ALUs, 1 = on FP32 MFMAs."""
+    from ephip import tune
+    old = tune.get_variant(tune.KIND_GRIDOPT_KQ)
+    tune.set_variant(tune.KIND_GRIDOPT_KQ, request.param)
+    yield request.param
+    tune.set_variant(tune.KIND_GRIDOPT_KQ, old)
+
+
 def _oracle_states(m, g):
     xk = M.kpoints_grid(g["kgrid"])
     E, U = O.heev_batched(O.fourier(m.ham_r, m.irvec_h, xk))
@@ -265,7 +276,7 @@ def test_outer_k_gridopt(backend, model16, cfg1, which, nb, mfma):
 
 
 @pytest.mark.parametrize("nq,window,nb", [(4, 0.5, 40), (8, 0.5, 150), (4, 0.8, 40), (8, 1.5, 64), (8, 0.15, 40)])
-def test_outer_k_gridopt_kq(backend, model16, nq, window, nb):
+def test_outer_k_gridopt_kq(backend, model16, nq, window, nb, kq_variant):
     """kR->kq as the staged (GridOpt) transform over the uniform q grid with the k+q/phonon rotations and
     g2 fused (FP32): kept-q lists and g2 agree with the direct FP32 Fourier + rotation kernels (g2 to
     1e-5) and with the oracle (1e-4), for nbw = 1 (one column through the two-column kernel), 2, 3 and 4
@@ -298,7 +309,7 @@ def test_outer_k_gridopt_kq(backend, model16, nq, window, nb):
 
 
 @pytest.mark.parametrize("box_g", [5])
-def test_outer_k_gridopt_box_faces(backend, box_g):
+def test_outer_k_gridopt_box_faces(backend, box_g, kq_variant):
     """The staged kernels on 5^3 e-ph R boxes (other instantiations of the stage-2/3 and kR->kq kernels
     than the 3^3 and 7^3 boxes elsewhere): unit operators and g2 agree with the direct FP32 kernels to
     1e-5 and g2 with the oracle to 1e-4."""
@@ -333,7 +344,7 @@ def test_outer_k_gridopt_box_faces(backend, box_g):
 
 
 @pytest.mark.parametrize("natoms,window", [(3, 0.5), (3, 0.8), (5, 0.5), (5, 0.8)])
-def test_outer_k_gridopt_mode_counts(backend, natoms, window):
+def test_outer_k_gridopt_mode_counts(backend, natoms, window, kq_variant):
     """The staged kernels at mode counts other than 12 (nm 9 and 15: the kR->kq kernel's run-time mode
     loop, the MFMA projection over nm blocks of 16 rows; nm 15 fills 240 of 256 threads), window widths
     2 (column pairs) and 3 (odd path): unit operators and g2 agree with the direct FP32 kernels to 1e-5
@@ -371,7 +382,7 @@ def test_outer_k_gridopt_mode_counts(backend, natoms, window):
 
 
 @pytest.mark.parametrize("window", [0.5, 0.8])
-def test_outer_k_gridopt_kq_long_lines(backend, model16, window):
+def test_outer_k_gridopt_kq_long_lines(backend, model16, window, kq_variant):
     """Staged kR->kq on a q grid whose lines hold more kept q than one chunk of the kernel (32): k and q
     grids (2, 4, 40), so the kernel splits lines; g2 agrees with the direct FP32 kernels and the oracle."""
     from ephip import _lib as L
