@@ -610,9 +610,56 @@ constexpr int GQ_M3_MAX = 64, GQ_SLAB_MAX = 1024;
 // (zero past the k+q window), the layout of ephip_eph_kq_rotate_packed_bf16x3.
 constexpr int GQ_CHUNK = 32, GQ_M2_MAX = 64;
 
-template <int N, int NBW, int NMC = 0>  // NMC: compile-time mode count (12), or 0 for nm <= 16 at run time
+// Stage 1 of the staged kR->kq kernels for the row `row` of a unit operator (abase = the unit's first
+// element): B1[c][b] = sum_a e1[a] abase[row + ldr Rp(a, b, c)], a < n1, b, c < N.
+// Dense boxes (every Rp of the N^3 box present, Rp = s1 a + s2 b + s3 c, n1 = N): the operator is
+// addressed arithmetically (uniform offset + the thread's row), fully unrolled, so the loads are not
+// chained behind index-map reads and can be issued ahead of their use. Otherwise the index map rmap
+// (b + 8 (a + n1 c) -> Rp or -1) is read per (a, c), one face row of loads in flight.
+template <int N, bool DENSE>
+__device__ __forceinline__ void kq_stage1(float (&br)[N][N], float (&bi)[N][N], const cplx<float> *__restrict__ abase,
+                                          int row, int64_t ldr, int n1, const int *__restrict__ rmap,
+                                          const float2 *ph1, int s1, int s2, int s3) {
+#pragma unroll
+    for (int c = 0; c < N; c++)
+#pragma unroll
+        for (int b = 0; b < N; b++) br[c][b] = bi[c][b] = 0.f;
+    if constexpr (DENSE) {
+#pragma unroll
+        for (int c = 0; c < N; c++)
+#pragma unroll
+            for (int a = 0; a < N; a++) {
+                const float2 p = ph1[a];
+#pragma unroll
+                for (int b = 0; b < N; b++) {
+                    const cplx<float> *col = abase + ldr * (int64_t)(s1 * a + s2 * b + s3 * c);  // uniform
+                    const cplx<float> v = col[row];
+                    cmac(br[c][b], bi[c][b], p.x, p.y, v.re, v.im);
+                }
+            }
+    } else {
+        const cplx<float> *a_u = abase + row;
+#pragma unroll
+        for (int c = 0; c < N; c++)
+            for (int a = 0; a < n1; a++) {
+                const float2 p = ph1[a];
+                cplx<float> v[N];
+#pragma unroll
+                for (int b = 0; b < N; b++) {  // branch-free (absent R: column 0 loaded, zero used)
+                    const int ir = rmap[b + 8 * (a + n1 * c)];  // uniform; the N entries are contiguous
+                    v[b] = a_u[ldr * max(ir, 0)];
+                    if (ir < 0) v[b] = cplx<float>{0.f, 0.f};
+                }
+#pragma unroll
+                for (int b = 0; b < N; b++) cmac(br[c][b], bi[c][b], p.x, p.y, v[b].re, v[b].im);
+            }
+    }
+}
+
+template <int N, int NBW, int NMC = 0, bool DENSE = false>  // NMC: compile-time mode count (12), or 0 for nm <= 16
 __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
-    int nm, int n1, int p1min, int p2min, int p3min, const int *__restrict__ rmap, int m1, int m2, int m3,
+    int nm, int n1, int p1min, int p2min, int p3min, const int *__restrict__ rmap, int s1, int s2, int s3, int m1,
+    int m2, int m3,
     const cplx<float> *__restrict__ A, int64_t lda, int64_t ldr, const int *__restrict__ line_off,
     const int *__restrict__ col_index, const int *__restrict__ ikq, int64_t cs, const cplx<float> *__restrict__ U,
     const int64_t *__restrict__ bmin, const int64_t *__restrict__ bmax, const cplx<float> *__restrict__ uph,
@@ -670,29 +717,9 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
         const int nb = b0 > 0 ? (int)(b1 - b0 + 1) : 0;
         sk_nb[kk] = min(nb, nbw) | ((b0 > 0 ? (int)b0 - 1 : 0) << 8) | ((iq % m3) << 16) | (((iq / m3) % m2) << 22);
     }
-    // stage 1: one row of the face at a time (N independent loads in flight)
+    // stage 1
     float br[N][N], bi[N][N];
-#pragma unroll
-    for (int c = 0; c < N; c++)
-#pragma unroll
-        for (int b = 0; b < N; b++) br[c][b] = bi[c][b] = 0.f;
-    const cplx<float> *a_u = A + (int64_t)u * lda + (rv ? t : 0);
-#pragma unroll
-    for (int c = 0; c < N; c++)
-        for (int a = 0; a < n1; a++) {
-            const float2 p = ph1[a];
-            cplx<float> v[N];
-#pragma unroll
-            for (int b = 0; b < N; b++) {  // branch-free (absent R: column 0 loaded, zero used)
-                const int ir = rmap[b + 8 * (a + n1 * c)];  // uniform; the N entries are contiguous
-                v[b] = a_u[ldr * max(ir, 0)];
-                if (ir < 0) v[b] = cplx<float>{0.f, 0.f};
-            }
-#pragma unroll
-            for (int b = 0; b < N; b++) {
-                cmac(br[c][b], bi[c][b], p.x, p.y, v[b].re, v[b].im);
-            }
-        }
+    kq_stage1<N, DENSE>(br, bi, A + (int64_t)u * lda, rv ? t : 0, ldr, n1, rmap, ph1, s1, s2, s3);
     const int lane = t & 63;
     int e0 = 0, lc = 0;  // chunk start in the slab tables and the line that holds it
     while (e0 < nslab) {  // uniform
@@ -866,13 +893,17 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
 // thread (threads t < 16 nm, coalesced loads); the stage-2 sums B2 of a line pass through LDS into the
 // MFMA operand layout (lane: nu = lane % 16, i = 4 wave + lane / 16). The four waves' partial tiles are
 // summed through LDS into t, and the phonon rotation and g2 run on up to GQM_TQ collected columns at a
-// time. Output: the layout of gridopt_kq_rotate_kernel (zeros past the k+q window).
-constexpr int GQM_TQ = 64;
+// time. The U_kq columns of a line's first GQM_UC columns are gathered into LDS by the wave without
+// stage-2 rows (nm <= 12) while the others form the stage-2 sums. LDS arrays read across the 16 lanes of
+// an MFMA row use odd pitches (b2s: nm + 1 or 17 per i, tq and ucol: 17 per column), so those reads and
+// the writes that fill them are free of bank conflicts. Output: the layout of gridopt_kq_rotate_kernel
+// (zeros past the k+q window).
+constexpr int GQM_TQ = 48, GQM_UC = 32;
 
-template <int N, int NMC = 0>  // NMC: compile-time mode count (12), or 0 for nm <= 16 at run time
+template <int N, int NMC = 0, bool DENSE = false>  // NMC: compile-time mode count (12), or 0 for nm <= 16
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 : 2))) void
 gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min, const int *__restrict__ rmap,
-                              int m1, int m2, int m3, const cplx<float> *__restrict__ A, int64_t lda, int64_t ldr,
+                              int s1, int s2, int s3, int m1, int m2, int m3, const cplx<float> *__restrict__ A, int64_t lda, int64_t ldr,
                               const int *__restrict__ line_off, const int *__restrict__ col_index,
                               const int *__restrict__ ikq, int64_t cs, const cplx<float> *__restrict__ U,
                               const int64_t *__restrict__ bmin, const int64_t *__restrict__ bmax,
@@ -880,7 +911,7 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
                               float *__restrict__ g2) {
     constexpr int NW = 16;
     constexpr int NE = (N + 1) & ~1;
-    constexpr int ROWS = NMC > 0 ? NW * NMC : 256;
+    constexpr int BP = NMC > 0 ? NMC + 1 : 17;  // b2s pitch per i (odd)
     const int nm = NMC > 0 ? NMC : nm_rt;
     // dynamic: q2 and q3 phase rows (m2, m3 rows of NE / 2 float4), the slab's k+q states and metadata words
     extern __shared__ float4 gqm_dyn[];
@@ -890,12 +921,13 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
     int *sk_nb = sk_kq + m2 * m3;
     __shared__ float2 ph1[GO_N1_MAX];
     __shared__ int slo[GQ_M2_MAX + 1];
-    __shared__ float2 b2s[N][ROWS];          // stage-2 sums of the current line, [c][G]
+    __shared__ float2 b2s[N][NW * BP];       // stage-2 sums of the current line, [c][nu + BP i]
     __shared__ f32x4 red[2][2][4][64];       // the waves' partial tiles (real, imaginary), two buffers
-    __shared__ int colk[2][GQ_M3_MAX * 4];   // the line's columns: slab entry | band slot << 16
+    __shared__ int colk[2][GQ_M3_MAX * 4];   // the line's columns: entry | band slot << 12 | q3 << 16
     __shared__ int ncols[2];
-    __shared__ float2 tq[GQM_TQ][NW];        // t of the collected columns, [column][nu]
+    __shared__ float2 tq[GQM_TQ * 17];       // t of the collected columns, [nu + 17 column]
     __shared__ int tqid[GQM_TQ];
+    __shared__ float2 ucol[GQM_UC * 17];     // U_kq[i, band] of the line's first columns, [i + 17 column]
     const int q1 = blockIdx.x, u = blockIdx.y;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const bool rv = t < NW * nm;
@@ -931,42 +963,24 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
         float *o = g2 + row * nbw * nm;
         for (int e = nb * nm; e < nbw * nm; e++) o[e] = 0.f;  // slots past the k+q window
     }
-    // stage 1 (rows G = t): one row of the face at a time (N independent loads in flight)
+    __syncthreads();  // the slab tables, read by every wave from the first line on
+    // stage 1 (rows G = t)
     float br[N][N], bi[N][N];
-#pragma unroll
-    for (int c = 0; c < N; c++)
-#pragma unroll
-        for (int b = 0; b < N; b++) br[c][b] = bi[c][b] = 0.f;
-    const cplx<float> *a_u = A + (int64_t)u * lda + (rv ? t : 0);
-#pragma unroll
-    for (int c = 0; c < N; c++)
-        for (int a = 0; a < n1; a++) {
-            const float2 p = ph1[a];
-            cplx<float> v[N];
-#pragma unroll
-            for (int b = 0; b < N; b++) {
-                const int ir = rmap[b + 8 * (a + n1 * c)];
-                v[b] = a_u[ldr * max(ir, 0)];
-                if (ir < 0) v[b] = cplx<float>{0.f, 0.f};
-            }
-#pragma unroll
-            for (int b = 0; b < N; b++) cmac(br[c][b], bi[c][b], p.x, p.y, v[b].re, v[b].im);
-        }
+    kq_stage1<N, DENSE>(br, bi, A + (int64_t)u * lda, rv ? t : 0, ldr, n1, rmap, ph1, s1, s2, s3);
     // the phonon rotation and g2 of the collected columns: ep[nu'] = sum_nu t[nu] uph(q)[nu, nu']
     int tqn = 0;
     auto flush = [&]() {
         __syncthreads();  // t of the collected columns written
         for (int e = t; e < tqn * nm; e += blockDim.x) {
             const int col = e / nm, nu2 = e - col * nm;
-            const int cid = tqid[col], kk = cid & 0xffff, m = cid >> 16;
-            const int meta = sk_nb[kk];
-            const int iq = (q1 * m2 + ((meta >> 22) & 63)) * m3 + ((meta >> 16) & 63);
+            const int cid = tqid[col], kk = cid & 0xfff, m = (cid >> 12) & 15;
+            const int iq = (q1 * m2 + ((sk_nb[kk] >> 22) & 63)) * m3 + ((cid >> 16) & 63);
             const cplx<float> *up = uph + (int64_t)nm * nm * iq + nm * nu2;
             float er = 0.f, ei = 0.f;
             if constexpr (NMC > 0) {
 #pragma unroll
                 for (int n = 0; n < NMC; n++) {
-                    const float2 tv = tq[col][n];
+                    const float2 tv = tq[n + 17 * col];
                     const cplx<float> p = up[n];
                     cmac(er, ei, tv.x, tv.y, p.re, p.im);
                 }
@@ -975,7 +989,7 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
                 for (int n = 0; n < NW; n++) {
                     const int nn = min(n, nm - 1);
                     const float on = n < nm ? 1.f : 0.f;
-                    const float2 tv = tq[col][nn];
+                    const float2 tv = tq[nn + 17 * col];
                     const cplx<float> p = up[nn];
                     cmac(er, ei, on * tv.x, on * tv.y, p.re, p.im);
                 }
@@ -1006,18 +1020,28 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
                 float sr = 0.f, si = 0.f;
 #pragma unroll
                 for (int b = 0; b < N; b++) cmac(sr, si, e2[b].x, e2[b].y, br[c][b], bi[c][b]);
-                b2s[c][t] = make_float2(sr, si);
+                b2s[c][(t >> 4) + BP * (t & 15)] = make_float2(sr, si);
             }
         }
         if (wave == 3) {  // the line's columns (entry, band slot): exclusive prefix of the band counts (<= 4)
             const int ne = ee - eb;  // <= m3 <= 64
-            const int nb = lane < ne ? (sk_nb[eb + lane] & 0xff) : 0;
+            const int meta = lane < ne ? sk_nb[eb + lane] : 0;
+            const int nb = meta & 0xff;
             const unsigned long long below = (1ull << lane) - 1ull;
             int start = 0;
 #pragma unroll
             for (int bit = 0; bit < 3; bit++) start += __popcll(__ballot((nb >> bit) & 1) & below) << bit;
-            for (int m = 0; m < nb; m++) colk[lb][start + m] = (eb + lane) | (m << 16);
+            for (int m = 0; m < nb; m++) colk[lb][start + m] = (eb + lane) | (m << 12) | (meta & (63 << 16));
             if (lane == 63) ncols[lb] = start + nb;
+            // U_kq columns of the first GQM_UC columns: lane = (column % 4, i), 16 columns per pass
+            const int nc = min(__shfl(start + nb, 63, 64), GQM_UC);
+            wave_lds_sync();  // this wave's column table written
+            for (int j = lane >> 4; j < nc; j += 4) {
+                const int cid = colk[lb][j], kk = cid & 0xfff, m = (cid >> 12) & 15;
+                const int mlo = (sk_nb[kk] >> 8) & 0xff;
+                const cplx<float> v = U[(int64_t)NW * NW * sk_kq[kk] + (lane & 15) + NW * (mlo + m)];
+                ucol[(lane & 15) + 17 * j] = make_float2(v.re, v.im);
+            }
         }
         __syncthreads();  // B2 and the column table of this line
         const int ncol = ncols[lb];
@@ -1029,12 +1053,17 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
                 float ur = 0.f, ui = 0.f;
                 if (j < ncol) {
                     cid = colk[lb][j];
-                    const int kk = cid & 0xffff, m = cid >> 16;
-                    const int meta = sk_nb[kk];
-                    q3 = (meta >> 16) & 63;
-                    const cplx<float> v = U[(int64_t)NW * NW * sk_kq[kk] + ai_ + NW * (((meta >> 8) & 0xff) + m)];
-                    ur = v.re;
-                    ui = v.im;
+                    q3 = (cid >> 16) & 63;
+                    if (j < GQM_UC) {
+                        const float2 v = ucol[ai_ + 17 * j];
+                        ur = v.x;
+                        ui = v.y;
+                    } else {
+                        const int kk = cid & 0xfff, m = (cid >> 12) & 15;
+                        const cplx<float> v = U[(int64_t)NW * NW * sk_kq[kk] + ai_ + NW * (((sk_nb[kk] >> 8) & 0xff) + m)];
+                        ur = v.re;
+                        ui = v.im;
+                    }
                 }
                 // t = sum B2 conj(U) e3 from three real products (Gauss):
                 //   re = b_r w_r - b_i w_i,  im = (b_r + b_i)(w_r + w_i) - b_r w_r - b_i w_i
@@ -1049,7 +1078,7 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
                             const float ex = s ? ev.z : ev.x, ey = s ? ev.w : ev.y;
                             const float wr = ur * ex + ui * ey, wi = ur * ey - ui * ex;
                             // A operand: this lane's B2 value, read per tile (the barrier below orders it)
-                            const float2 av = aok ? b2s[c][ai_ + NW * anu] : make_float2(0.f, 0.f);
+                            const float2 av = aok ? b2s[c][anu + BP * ai_] : make_float2(0.f, 0.f);
                             prr = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, wr, prr, 0, 0, 0);
                             pii = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, wi, pii, 0, 0, 0);
                             pss = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x + av.y, wr + wi, pss, 0, 0, 0);
@@ -1070,7 +1099,7 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
                         si += rf[((4 + w) * 64 + src) * 4 + r];
                     }
                     if (col0 + col < ncol) {
-                        if (nu < nm) tq[tqn + col][nu] = make_float2(sr, si);
+                        if (nu < nm) tq[nu + 17 * (tqn + col)] = make_float2(sr, si);
                         if (nu == 0) tqid[tqn + col] = cid;  // lanes 0..15 of wave 0 built column col0 + col
                     }
                 }
@@ -1100,7 +1129,8 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
                                        const int32_t *qgrid, const void *A, int64_t lda, int64_t ldr,
                                        const int32_t *line_off, const int32_t *col_index, const int32_t *ikq,
                                        int64_t cs, const void *U, const int64_t *bmin, const int64_t *bmax,
-                                       const void *uph, const double *omega, float *g2, void *stream) {
+                                       const void *uph, const double *omega, const int32_t *dense_strides,
+                                       float *g2, void *stream) {
     EPHIP_REQUIRE(nw == 16 && nm >= 1 && nm <= 16, "gridopt_kq_rotate: nw must be 16 and nm <= 16 (got %d, %d)", nw,
                   nm);
     EPHIP_REQUIRE(nbw >= 1 && nbw <= 4, "gridopt_kq_rotate: nbw=%d outside 1..4", nbw);
@@ -1116,35 +1146,45 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
     EPHIP_REQUIRE(rmap && A && line_off && col_index && ikq && U && bmin && bmax && uph && omega && g2,
                   "gridopt_kq_rotate: null device pointer");
     dim3 g((unsigned)qgrid[0], (unsigned)nunits);
+    // dense box: every Rp of the np^3 box present at index s1 a + s2 b + s3 c (the caller's check)
+    const bool dense = dense_strides != nullptr && n1 == np;
+    const int s1 = dense ? dense_strides[0] : 0, s2 = dense ? dense_strides[1] : 0, s3 = dense ? dense_strides[2] : 0;
+    if (dense)
+        EPHIP_REQUIRE(s1 >= 0 && s2 >= 0 && s3 >= 0 && (int64_t)(s1 + s2 + s3) * (np - 1) < ((int64_t)1 << 31),
+                      "gridopt_kq_rotate: bad dense-box strides");
     if (ephip_ft_get_variant(6) == 1) {  // stage 3 and the k+q-window rotation on MFMAs
         using KM = decltype(&gridopt_kq_rotate_mfma_kernel<3>);
-#define GQM_ROW(c) {gridopt_kq_rotate_mfma_kernel<3, c>, gridopt_kq_rotate_mfma_kernel<4, c>, \
-                    gridopt_kq_rotate_mfma_kernel<5, c>, gridopt_kq_rotate_mfma_kernel<6, c>, \
-                    gridopt_kq_rotate_mfma_kernel<7, c>, gridopt_kq_rotate_mfma_kernel<8, c>}
-        static const KM mtable[2][6] = {GQM_ROW(0), GQM_ROW(12)};
+#define GQM_ROW(c, d) {gridopt_kq_rotate_mfma_kernel<3, c, d>, gridopt_kq_rotate_mfma_kernel<4, c, d>, \
+                       gridopt_kq_rotate_mfma_kernel<5, c, d>, gridopt_kq_rotate_mfma_kernel<6, c, d>, \
+                       gridopt_kq_rotate_mfma_kernel<7, c, d>, gridopt_kq_rotate_mfma_kernel<8, c, d>}
+        static const KM mtable[2][2][6] = {{GQM_ROW(0, false), GQM_ROW(12, false)},
+                                           {GQM_ROW(0, true), GQM_ROW(12, true)}};
 #undef GQM_ROW
         const int ne = (np + 1) & ~1;
         const size_t lds = (size_t)(qgrid[1] + qgrid[2]) * (ne / 2) * 16 + 2 * (size_t)qgrid[1] * qgrid[2] * 4;
-        mtable[nm == 12 ? 1 : 0][np - 3]<<<g, 256, lds, as_stream(stream)>>>(
-            nm, n1, p1min, p2min, p3min, rmap, qgrid[0], qgrid[1], qgrid[2], static_cast<const cplx<float> *>(A), lda,
-            ldr, line_off, col_index, ikq, cs, static_cast<const cplx<float> *>(U), bmin, bmax,
-            static_cast<const cplx<float> *>(uph), omega, nbw, g2);
+        mtable[dense ? 1 : 0][nm == 12 ? 1 : 0][np - 3]<<<g, 256, lds, as_stream(stream)>>>(
+            nm, n1, p1min, p2min, p3min, rmap, s1, s2, s3, qgrid[0], qgrid[1], qgrid[2],
+            static_cast<const cplx<float> *>(A), lda, ldr, line_off, col_index, ikq, cs,
+            static_cast<const cplx<float> *>(U), bmin, bmax, static_cast<const cplx<float> *>(uph), omega, nbw, g2);
         EPHIP_LAUNCHED();
         return EPHIP_OK;
     }
     const int nd2 = nw * nm;
     const unsigned threads = (unsigned)(((nd2 + 63) / 64) * 64);
     using K = decltype(&gridopt_kq_rotate_kernel<3, 2>);
-#define GQ_ROW(nb, c) {gridopt_kq_rotate_kernel<3, nb, c>, gridopt_kq_rotate_kernel<4, nb, c>, \
-                       gridopt_kq_rotate_kernel<5, nb, c>, gridopt_kq_rotate_kernel<6, nb, c>, \
-                       gridopt_kq_rotate_kernel<7, nb, c>, gridopt_kq_rotate_kernel<8, nb, c>}
-    static const K table[2][3][6] = {{GQ_ROW(2, 0), GQ_ROW(3, 0), GQ_ROW(4, 0)},
-                                     {GQ_ROW(2, 12), GQ_ROW(3, 12), GQ_ROW(4, 12)}};
+#define GQ_ROW(nb, c, d) {gridopt_kq_rotate_kernel<3, nb, c, d>, gridopt_kq_rotate_kernel<4, nb, c, d>, \
+                          gridopt_kq_rotate_kernel<5, nb, c, d>, gridopt_kq_rotate_kernel<6, nb, c, d>, \
+                          gridopt_kq_rotate_kernel<7, nb, c, d>, gridopt_kq_rotate_kernel<8, nb, c, d>}
+    static const K table[2][2][3][6] = {
+        {{GQ_ROW(2, 0, false), GQ_ROW(3, 0, false), GQ_ROW(4, 0, false)},
+         {GQ_ROW(2, 12, false), GQ_ROW(3, 12, false), GQ_ROW(4, 12, false)}},
+        {{GQ_ROW(2, 0, true), GQ_ROW(3, 0, true), GQ_ROW(4, 0, true)},
+         {GQ_ROW(2, 12, true), GQ_ROW(3, 12, true), GQ_ROW(4, 12, true)}}};
 #undef GQ_ROW
-    const K kern = table[nm == 12 ? 1 : 0][nbw <= 2 ? 0 : nbw - 2][np - 3];
-    kern<<<g, threads, 0, as_stream(stream)>>>(nm, n1, p1min, p2min, p3min, rmap, qgrid[0], qgrid[1], qgrid[2],
-                                                static_cast<const cplx<float> *>(A), lda, ldr, line_off, col_index,
-                                                ikq, cs, static_cast<const cplx<float> *>(U), bmin, bmax,
+    const K kern = table[dense ? 1 : 0][nm == 12 ? 1 : 0][nbw <= 2 ? 0 : nbw - 2][np - 3];
+    kern<<<g, threads, 0, as_stream(stream)>>>(nm, n1, p1min, p2min, p3min, rmap, s1, s2, s3, qgrid[0], qgrid[1],
+                                                qgrid[2], static_cast<const cplx<float> *>(A), lda, ldr, line_off,
+                                                col_index, ikq, cs, static_cast<const cplx<float> *>(U), bmin, bmax,
                                                 static_cast<const cplx<float> *>(uph), omega, nbw, g2);
     EPHIP_LAUNCHED();
     return EPHIP_OK;

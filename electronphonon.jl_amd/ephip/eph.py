@@ -823,6 +823,15 @@ class _GridOptKQ:
         rmap = -np.ones((self.np, self.n1, 8), np.int32)  # index b + 8 (a + n1 c)
         rmap[r[:, 2] - lo[2], r[:, 0] - lo[0], r[:, 1] - lo[1]] = np.arange(len(r), dtype=np.int32)
         self.rmap = torch.from_numpy(rmap.ravel()).to(dev)
+        import ctypes
+        # dense box (n1 = np, every Rp present, index affine in (a, b, c)): the kernel skips the index map
+        self.dense = None
+        box = rmap[:, :, :self.np]  # [c, a, b]
+        if self.n1 == self.np and (box >= 0).all() and self.np > 1:
+            s1, s2, s3 = int(box[0, 1, 0] - box[0, 0, 0]), int(box[0, 0, 1] - box[0, 0, 0]), int(box[1, 0, 0] - box[0, 0, 0])
+            c, a, b = np.meshgrid(*(np.arange(self.np),) * 3, indexing="ij")
+            if box[0, 0, 0] == 0 and min(s1, s2, s3) >= 0 and np.array_equal(box, s1 * a + s2 * b + s3 * c):
+                self.dense = (ctypes.c_int32 * 3)(s1, s2, s3)
         self.m = [int(x) for x in m]
         self.nlines = self.m[0] * self.m[1]
         import ctypes
@@ -837,7 +846,8 @@ class _GridOptKQ:
                self.pmin[2], L.ptr(self.rmap), C.addressof(self.qgrid_c), L.ptr(A), A.stride(0),
                st.nw * st.nm, L.ptr(loop.line_off), L.ptr(loop.col_index), L.ptr(loop.ikq), loop.cs,
                L.ptr(st.el.U), L.ptr(st.el.filt.band_min), L.ptr(st.el.filt.band_max), L.ptr(loop.uph),
-               L.ptr(loop.omega), L.ptr(loop.g2), stream)
+               L.ptr(loop.omega), C.addressof(self.dense) if self.dense is not None else None, L.ptr(loop.g2),
+               stream)
 
 
 def kept_q_bound(ksurv, kgrid, qgrid) -> int:

@@ -290,15 +290,17 @@ int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp, int32_t n
  * (p1min + a, p2min + b, p3min + c), a < n1 <= 32, b, c < np (3..8); rmap (int32, 8*n1*np, index
  * b + 8*(a + n1*c), rows padded to 8) gives the Rp index or -1. line_off (int32, nunits x (m1*m2 + 1)) holds each unit's
  * first kept-list position per q line, from ephip_kq_line_offsets (lower bounds in the sorted kept
- * lists of ephip_kq_compact). nw = 16, nm <= 16, nbw <= 4, m2, m3 <= 64, m2*m3 <= 1024. */
+ * lists of ephip_kq_compact). nw = 16, nm <= 16, nbw <= 4, m2, m3 <= 64, m2*m3 <= 1024. dense_strides: host
+ * int32[3] (s1, s2, s3) when the box is dense (n1 = np and every Rp present, Rp index = s1 a + s2 b + s3 c;
+ * the operator is then addressed without rmap), or NULL. */
 int ephip_kq_line_offsets(int32_t nunits, const int32_t *count, const int32_t *col_index, int64_t cs, int32_t m3,
                           int32_t nlines, int32_t *line_off, void *stream);
 int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int32_t nunits, int32_t n1, int32_t np,
                             int32_t p1min, int32_t p2min, int32_t p3min, const int32_t *rmap, const int32_t *qgrid,
                             const void *A, int64_t lda, int64_t ldr, const int32_t *line_off,
                             const int32_t *col_index, const int32_t *ikq, int64_t cs, const void *U,
-                            const int64_t *bmin, const int64_t *bmax, const void *uph, const double *omega, float *g2,
-                            void *stream);
+                            const int64_t *bmin, const int64_t *bmax, const void *uph, const double *omega,
+                            const int32_t *dense_strides, float *g2, void *stream);
 
 /* kR->kq Fourier over Rp fused with the k+q-window and phonon rotations and g2 (the
  * get_eph_kR_to_kq_batched! + eph_apply_rotations! + g2 sequence of src/wannier_to_bloch_batched.jl:
