@@ -48,11 +48,19 @@ TRAFFIC_BYTES_CFG2 = {
     ("bf16x3", "RR->kR", 384, "proj0"): (1827474 * 2 + 200604) * 1024,  # RR->kR fused with the projection
 }
 # kernel names by (stage, variant) for the split-BF16 fast path (tuner kinds 3 and 4)
+def _staged(v) -> bool:
+    """Variant label of a staged (GridOpt) contraction."""
+    return isinstance(v, str) and v.startswith("gridopt")
+
+
 KERNEL_NAMES = {
     ("RR->kR", "gridopt"): "gridopt_stage1_kernel + gridopt_kR_project_mfma_kernel: staged (GridOpt) RR->kR over "
                            "Re, stages 2/3 with the k-side window projection on FP32 MFMAs",
     ("kR->kq", "gridopt"): "gridopt_kq_rotate_kernel: staged (GridOpt) kR->kq over the q grid with the k+q/phonon "
                            "rotations and g2, FP32 VALU",
+    ("kR->kq", "gridopt_mfma"): "gridopt_kq_rotate_mfma_kernel: staged (GridOpt) kR->kq over the q grid, stages 1-2 "
+                                "on the VALU, stage 3 and the k+q-window rotation as one FP32 MFMA contraction, "
+                                "phonon rotation and g2",
     ("RR->kR", "proj0"): "ft_bf16p8_proj_kernel: eight-wave 192x128 packed split-BF16 RR->kR Fourier over Re "
                          "fused with the k-side window projection",
     ("RR->kR", "proj1"): "ft_bf16p8_proj_kernel<deep>: eight-wave 192x128 packed split-BF16 RR->kR Fourier over "
@@ -73,50 +81,65 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(O, m, st, epmat_c64, nk_sample, kgrid, min_seconds):
-    """Time the FP64 oracle (C restatement of the reference's CPU outer-q path, OpenMP) on whole q
-    slices of the workload: for q = 0, then q spread over the grid, RR->Rq for the q and the Rq->kq
-    pairs of the first nk_sample surviving k (all of them when nk_sample < 0), until min_seconds of
-    CPU work are done. Returns the measured pairs/s (no extrapolation when nk_sample < 0)."""
+def cpu_baseline(O, m, st, epmat_c64, nk_sample, kgrid, min_seconds, direct_seconds=3.0):
+    """Time the reference's CPU path as its own benchmarks run it: the outer-q loop in GridOpt mode
+    (run_eph_over_q_and_k with fourier_mode = "gridopt", benchmark/bench_bte_gpu.jl:58-60), restated in
+    FP64 C with OpenMP (oracle/ephor.c ephor_outer_q_gridopt; not the reference itself). Whole q slices in
+    grid order from q = 0 (the order the reference's loop takes, which gives its GridOpt caches their
+    reuse), each RR->Rq + the Rq->kq pairs of every surviving k, until min_seconds of CPU work; pairs/s on
+    the metric's numerator (K_surv per q). Beside it, the direct-sum restatement (ephor_outer_q_block, the
+    "normal" Fourier mode) on q slices spread over the grid for direct_seconds, as an extra field."""
     ksurv = st.ksurv.cpu().numpy()
     nks = len(ksurv)
     ik = ksurv if nk_sample < 0 else ksurv[:nk_sample]
     xk = st.el.xk.cpu().numpy()
-    nq = st.ph.xq.shape[0]
+    xq_all = st.ph.xq.cpu().numpy()
+    nq = xq_all.shape[0]
     bmin = st.el.filt.band_min.cpu().numpy()
     bmax = st.el.filt.band_max.cpu().numpy()
     U = st.el.U.cpu().numpy().reshape(xk.shape[0], -1).astype(np.complex128)
+    uph_all = st.ph.u.cpu().numpy().astype(np.complex128).reshape(nq, -1)
+    om_all = st.ph.omega.cpu().numpy().astype(np.float64)
     ep128 = epmat_c64.astype(np.complex128)
+    # GridOpt mode: blocks of consecutive q
+    t_go, q_go, pairs_go, blk = 0.0, 0, 0, 4
+    while q_go == 0 or (t_go < min_seconds and q_go < nq):
+        iqs = np.arange(q_go, min(q_go + blk, nq), dtype=np.int64)
+        t0 = time.perf_counter()
+        n, _, _, _ = O.outer_q_gridopt(m.nw, m.nmodes, ep128, m.irvec_el, m.irvec_ep, xq_all, uph_all, om_all, iqs,
+                                       xk, kgrid, U, bmin, bmax, ik, st.nbw)
+        t_go += time.perf_counter() - t0
+        q_go += len(iqs)
+        pairs_go += n
+    value = q_go * nks / t_go if len(ik) == nks else q_go * len(ik) / t_go
+    # direct-sum ("normal" Fourier mode) restatement, q spread over the grid
     t_rr = t_kq = 0.0
-    pairs = 0
     qs = []
-    while not qs or t_rr + t_kq < min_seconds:
+    while not qs or t_rr + t_kq < direct_seconds:
         iq = (len(qs) * 4999) % nq
         if iq in qs:
             break
         qs.append(iq)
-        xq = st.ph.xq[iq].cpu().numpy()
+        xq = xq_all[iq]
         ikq = O.hash_xk(xk[ik] + xq, kgrid)
-        uph = st.ph.u[iq].cpu().numpy().astype(np.complex128).ravel()
         t0 = time.perf_counter()
-        eRpq = O.eph_RR_to_Rq(m.nw, m.nmodes, ep128, m.irvec_el, m.irvec_ep, xq, uph)
+        eRpq = O.eph_RR_to_Rq(m.nw, m.nmodes, ep128, m.irvec_el, m.irvec_ep, xq, uph_all[iq])
         t1 = time.perf_counter()
         O.outer_q_block(m.nw, m.nmodes, eRpq, m.irvec_el, xk, U, bmin, bmax, ik, ikq, st.nbw)
         t2 = time.perf_counter()
         t_rr += t1 - t0
         t_kq += t2 - t1
-        pairs += len(ik)
-    if len(ik) == nks:
-        value = pairs / (t_rr + t_kq)
-        how = "measured"
-    else:   # extrapolate each q slice to all surviving k
-        value = len(qs) * nks / (t_rr + t_kq * nks / len(ik))
-        how = f"Rq->kq extrapolated from {len(ik)} to {nks} k per q"
+    direct = len(qs) * len(ik) / (t_rr + t_kq)
     return {"value": value, "unit": "pairs/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"FP64 C oracle (restatement of the reference CPU outer-q path, OpenMP): {len(qs)} q "
-                      f"slice(s) (q = {qs[:4]}{'...' if len(qs) > 4 else ''}), each RR->Rq + Rq->kq for "
-                      f"{len(ik)} of {nks} surviving k; RR->Rq {t_rr:.2f} s, Rq->kq {t_kq:.2f} s; {how}",
-            "seconds": t_rr + t_kq}
+            "sample": f"FP64 C restatement (not the reference) of the reference CPU outer-q loop in GridOpt mode "
+                      f"(fourier_mode=\"gridopt\", OpenMP, {O.num_threads()} threads): q 0..{q_go - 1} in grid "
+                      f"order, each RR->Rq + Rq->kq for {len(ik)} of {nks} surviving k ({pairs_go} pairs with a "
+                      f"window band at k+q computed); {t_go:.2f} s; measured",
+            "seconds": t_go,
+            "direct_sum": {"value": direct, "unit": "pairs/s", "seconds": t_rr + t_kq,
+                           "sample": f"direct phase sums (fourier_mode=\"normal\"): {len(qs)} q slice(s) "
+                                     f"{qs[:4]}{'...' if len(qs) > 4 else ''}, RR->Rq {t_rr:.2f} s, "
+                                     f"Rq->kq {t_kq:.2f} s"}}
 
 
 def oracle_check(m, st, epmat, grids, w, payload, nk_sample=32, nq_sample=2):
@@ -386,7 +409,8 @@ def main():
                                              f"proj{T.get_variant(T.KIND_FUSED_PROJ)}"
                                              if getattr(loop, "fused_proj", False) else
                                              T.get_variant(T.KIND_PACKED_BF16)),
-                               variant_krkq="gridopt" if staged_kq else T.get_variant(T.KIND_FUSED_ROT))
+                               variant_krkq=(("gridopt_mfma" if T.get_variant(T.KIND_GRIDOPT_KQ) == 1 else "gridopt")
+                                             if staged_kq else T.get_variant(T.KIND_FUSED_ROT)))
         run_as = ("fp32 staged" if staged_rr and staged_kq else
                   "+".join(sorted({"fp32 staged" if st_ else args.precision for st_ in (staged_rr, staged_kq)})))
         log(f"[bench] fast outer-k ({run_as}): {results['fast']['value']:.3e} pairs/s; RR->kR "
@@ -413,14 +437,14 @@ def main():
         var = {"RR->kR": r.get("variant_rrkr"), "kR->kq": r.get("variant_krkq")}
         stage = "RR->kR" if r["rrkr_avg_ms"] >= r["krkq_avg_ms"] else "kR->kq"
         if stage == "RR->kR":
-            if var[stage] == "gridopt" or bf:
+            if _staged(var[stage]) or bf:
                 name = KERNEL_NAMES.get((stage, var[stage])) or (
                     f"ft_bf16p_kernel (packed split-BF16, variant {var[stage]}), RR->kR Fourier over Re")
             else:
                 name = "ft_mfma_kernel<float,128>, RR->kR Fourier over Re"
             dom = (name + " (k batch)", r["rrkr_tflops"], r["rrkr_avg_ms"], r["rrkr_flop_per_launch"], stage)
         else:
-            if var[stage] == "gridopt" or r.get("fused_rotation"):
+            if _staged(var[stage]) or r.get("fused_rotation"):
                 name = KERNEL_NAMES.get((stage, var[stage])) or (
                     f"ft_bf16p_rot_kernel (variant {var[stage]}): packed split-BF16 kR->kq Fourier over Rp fused "
                     "with the k+q/phonon rotations and g2")
@@ -430,9 +454,9 @@ def main():
                    stage)
         # arithmetic of the two contractions as run: the staged (GridOpt) kernels are FP32 whatever the
         # precision argument
-        kinds = {"fp32" if var[k] == "gridopt" else args.precision for k in var}
+        kinds = {"fp32" if _staged(var[k]) else args.precision for k in var}
         path_dtype = "+".join(sorted(kinds))
-        dom_fp32 = var[dom[4]] == "gridopt" or args.precision == "fp32"
+        dom_fp32 = _staged(var[dom[4]]) or args.precision == "fp32"
         path = "outer-k, k-side window projection, k+q window skip"
         pairs_step = args.kbatch_k * nq_total
     else:
@@ -443,6 +467,8 @@ def main():
         pairs_step = nks * args.qtile
         dom_fp32 = True
     peak = PEAK["fp32" if ("fast" not in results or dom_fp32) else args.precision]
+    # the dominant kernel runs on the vector ALUs only (staged kR->kq in its VALU form, staged RR->kR stage 1)
+    valu_dom = "fast" in results and var[dom[4]] == "gridopt" and dom[4] == "kR->kq"
     out = {
         "metric": METRIC,
         "value": value,
@@ -461,22 +487,24 @@ def main():
                    "path": path, "K_surv": nks, "pairs_per_step": pairs_step,
                    "pre_filter_pairs": nk_total * nq_total,
                    "plain_outer_q_fp32_pairs_per_s": results.get("plain", {}).get("value")},
-        "roofline": {"bound": ("valu" if "fast" in results and var[dom[4]] == "gridopt" else "mfma"),
+        "roofline": {"bound": ("valu" if valu_dom else "mfma"),
                      "achieved": dom[1], "peak": peak, "unit": "TFLOP/s",
                      "frac": dom[1] / peak,
                      "compute": ("FP32 VALU, no MFMA: 157.3 TFLOP/s is MI355X's published FP32 peak, which counts "
                                  "packed FP32 (not used here); the unpacked vector peak is 78.6 TFLOP/s"
-                                 if "fast" in results and var[dom[4]] == "gridopt" else
+                                 if valu_dom else
+                                 "FP32 MFMA for stage 3 and the k+q-window contraction; stages 1-2, the phonon "
+                                 "rotation and g2 on the FP32 VALU (no packed FP32); flops of all stages counted"
+                                 if "fast" in results and var[dom[4]] == "gridopt_mfma" else
                                  "MFMA (split-BF16 flops counted once)" if "fast" in results and not dom_fp32
                                  else "MFMA FP32"),
-                     "traffic": (TRAFFIC_BYTES_CFG2.get(("fp32" if var[dom[4]] == "gridopt" else args.precision,
+                     "traffic": (TRAFFIC_BYTES_CFG2.get(("fp32" if _staged(var[dom[4]]) else args.precision,
                                                          dom[4], args.kbatch_k, var[dom[4]]))
                                  if ("fast" in results and args.config == "cfg2"
                                      and (dom[4] != "kR->kq" or r.get("fused_rotation"))) else None),
                      "kernel": dom[0], "avg_launch_ms": dom[2], "flop_per_launch": dom[3],
                      # FP32 VALU kernels: the same achieved rate against the unpacked vector peak
-                     "frac_unpacked_vector_peak": (dom[1] / 78.6 if "fast" in results and var[dom[4]] == "gridopt"
-                                                   else None)},
+                     "frac_unpacked_vector_peak": (dom[1] / 78.6 if valu_dom else None)},
         "cpu_baseline": cpu,
         "check": check,
         "detail": results,

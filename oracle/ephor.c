@@ -624,3 +624,235 @@ int ephor_num_threads(void) {
     return 1;
 #endif
 }
+
+/* ---------------------------------------------------------------------------------------------
+ * GridOpt staged Fourier transform (src/wannier/GridOpt.jl:5-139, queried as in get_fourier! of the
+ * GridOpt interpolator, src/wannier/WannierInterpolator.jl:142-163): for consecutive queries that share
+ * k1 (and k2) the phase sum runs one axis at a time on cached partial sums,
+ *   set23 when k1 changes: op23[:, (r2, r3)] = sum_r1 e^{2 pi i k1 r1} op[:, R]        (GridOpt.jl:71-107)
+ *   set3  when k2 changes: op3[:, r3]        = sum_r2 e^{2 pi i k2 r2} op23[:, (r2, r3)] (:110-131)
+ *   get3  every query:     out               = sum_r3 e^{2 pi i k3 r3} op3[:, r3]        (:133-139)
+ * The R vectors are grouped by (r2, r3) and by r3 through a sort (the reference assumes irvec sorted by
+ * (r3, r2, r1)); the change tests use the reference's tolerance sqrt(eps)/100. One instance per thread
+ * (the reference keeps one interpolator per thread); ephor_gridopt_reset drops the caches (a new
+ * operator). The rows of set23/set3 are split over OpenMP threads when `threads` is set (the reference's
+ * `threads` option, used for the e-ph operator).
+ * ------------------------------------------------------------------------------------------- */
+typedef struct {
+    int nr, nr23, nr3, threads;
+    int64_t ndata;
+    int32_t *perm;   /* R index by sorted position, (r3, r2, r1) order */
+    int32_t *r1;     /* r1 at sorted position */
+    int32_t *rng23;  /* (r2, r3) group g holds sorted positions rng23[g] .. rng23[g + 1] - 1 */
+    int32_t *r2g;    /* r2 of group g */
+    int32_t *rng3;   /* r3 group h holds (r2, r3) groups rng3[h] .. rng3[h + 1] - 1 */
+    int32_t *r3h;    /* r3 of group h */
+    double k1, k2;
+    zc *op23, *op3;
+} ephor_gridopt;
+
+static const int32_t *go_sort_irvec;
+static int go_cmp(const void *a, const void *b) {
+    const int32_t *ra = go_sort_irvec + 3 * *(const int32_t *)a, *rb = go_sort_irvec + 3 * *(const int32_t *)b;
+    for (int d = 2; d >= 0; d--)
+        if (ra[d] != rb[d]) return ra[d] < rb[d] ? -1 : 1;
+    return 0;
+}
+
+void ephor_gridopt_reset(ephor_gridopt *g) { g->k1 = g->k2 = NAN; }
+
+ephor_gridopt *ephor_gridopt_new(int nr, const int32_t *irvec, int64_t ndata, int threads) {
+    ephor_gridopt *g = (ephor_gridopt *)calloc(1, sizeof(ephor_gridopt));
+    g->nr = nr;
+    g->ndata = ndata;
+    g->threads = threads;
+    g->perm = (int32_t *)malloc(sizeof(int32_t) * nr);
+    g->r1 = (int32_t *)malloc(sizeof(int32_t) * nr);
+    g->rng23 = (int32_t *)malloc(sizeof(int32_t) * (nr + 1));
+    g->r2g = (int32_t *)malloc(sizeof(int32_t) * nr);
+    g->rng3 = (int32_t *)malloc(sizeof(int32_t) * (nr + 1));
+    g->r3h = (int32_t *)malloc(sizeof(int32_t) * nr);
+    for (int i = 0; i < nr; i++) g->perm[i] = i;
+    go_sort_irvec = irvec; /* single-threaded set-up */
+    qsort(g->perm, nr, sizeof(int32_t), go_cmp);
+    int n23 = 0, n3 = 0;
+    for (int i = 0; i < nr; i++) {
+        const int32_t *r = irvec + 3 * g->perm[i];
+        g->r1[i] = r[0];
+        if (i == 0 || r[1] != irvec[3 * g->perm[i - 1] + 1] || r[2] != irvec[3 * g->perm[i - 1] + 2]) {
+            if (n23 == 0 || r[2] != g->r3h[n3 - 1]) {
+                g->rng3[n3] = n23;
+                g->r3h[n3++] = r[2];
+            }
+            g->rng23[n23] = i;
+            g->r2g[n23++] = r[1];
+        }
+    }
+    g->rng23[n23] = nr;
+    g->rng3[n3] = n23;
+    g->nr23 = n23;
+    g->nr3 = n3;
+    g->op23 = (zc *)malloc(sizeof(zc) * ndata * n23);
+    g->op3 = (zc *)malloc(sizeof(zc) * ndata * n3);
+    ephor_gridopt_reset(g);
+    return g;
+}
+
+void ephor_gridopt_free(ephor_gridopt *g) {
+    if (!g) return;
+    free(g->perm); free(g->r1); free(g->rng23); free(g->r2g); free(g->rng3); free(g->r3h);
+    free(g->op23); free(g->op3);
+    free(g);
+}
+
+int ephor_gridopt_nr23(const ephor_gridopt *g) { return g->nr23; }
+int ephor_gridopt_nr3(const ephor_gridopt *g) { return g->nr3; }
+
+/* out[d] = sum_R e^{2 pi i xk.R} op[d + ldop R] through the cached stages. */
+void ephor_gridopt_get(ephor_gridopt *g, const zc *op, int64_t ldop, const double *xk, zc *out) {
+    const double tol = sqrt(DBL_EPSILON) / 100.0;
+    const int64_t nd = g->ndata;
+    if (!(fabs(xk[0] - g->k1) <= tol)) {
+        g->k1 = xk[0];
+        g->k2 = NAN;
+        zc *ph = (zc *)malloc(sizeof(zc) * g->nr);
+        for (int i = 0; i < g->nr; i++) ph[i] = cis2pi(xk[0] * g->r1[i]);
+#pragma omp parallel for schedule(static) if (g->threads)
+        for (int64_t d = 0; d < nd; d++)
+            for (int s = 0; s < g->nr23; s++) {
+                zc acc = 0.0;
+                for (int i = g->rng23[s]; i < g->rng23[s + 1]; i++) acc += ph[i] * op[d + ldop * g->perm[i]];
+                g->op23[d + nd * s] = acc;
+            }
+        free(ph);
+    }
+    if (!(fabs(xk[1] - g->k2) <= tol)) {
+        g->k2 = xk[1];
+        zc *ph = (zc *)malloc(sizeof(zc) * g->nr23);
+        for (int s = 0; s < g->nr23; s++) ph[s] = cis2pi(xk[1] * g->r2g[s]);
+#pragma omp parallel for schedule(static) if (g->threads)
+        for (int64_t d = 0; d < nd; d++)
+            for (int h = 0; h < g->nr3; h++) {
+                zc acc = 0.0;
+                for (int s = g->rng3[h]; s < g->rng3[h + 1]; s++) acc += ph[s] * g->op23[d + nd * s];
+                g->op3[d + nd * h] = acc;
+            }
+        free(ph);
+    }
+    zc ph3[64];
+    zc *ph = g->nr3 <= 64 ? ph3 : (zc *)malloc(sizeof(zc) * g->nr3);
+    for (int h = 0; h < g->nr3; h++) ph[h] = cis2pi(xk[2] * g->r3h[h]);
+    for (int64_t d = 0; d < nd; d++) {
+        zc acc = 0.0;
+        for (int h = 0; h < g->nr3; h++) acc += ph[h] * g->op3[d + nd * h];
+        out[d] = acc;
+    }
+    if (ph != ph3) free(ph);
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * The reference's CPU outer-q loop in GridOpt mode (src/calculator/run_eph_over_q_and_k.jl:289-390 with
+ * fourier_mode = "gridopt", the CPU setting of benchmark/bench_bte_gpu.jl:58-60), for the q of
+ * iq_list in order (grid order gives the GridOpt caches their reuse):
+ *   per q: g(Re, q) = get_eph_RR_to_Rq! through a GridOpt instance over Rp (rows threaded), then the
+ *          phonon rotation per Re;
+ *   per k of the surviving list (threads take contiguous chunks, one GridOpt instance over Re each, as
+ *          the reference's per-thread interpolators): k+q from the grid hash; skipped when k+q has no
+ *          window band (the xk_to_ik miss of :356-357 on the filtered k+q set); g(k, q) from the cached
+ *          stages; ep = U_kq^H g U_k on the window bands; g2 = |ep|^2 / (2 omega).
+ * epmat: "ph" layout (nw^2 nm nr_el) x nr_ep. xk: (3, nk) of the full fine grid, ks: surviving k (grid
+ * order). Returns the number of computed (k, q) pairs; g2sum[q] = sum of g2 over that q's pairs (a
+ * checksum), and, when ep_out/ik_out are given, the pairs of q = iq_list[0] are written for checking
+ * (ep as in ephor_outer_q_block, at most nout pairs, ik_out -1 past the last).
+ * ------------------------------------------------------------------------------------------- */
+int64_t ephor_outer_q_gridopt(int nw, int nm, int nr_el, int nr_ep, const zc *epmat, const int32_t *irvec_el,
+                              const int32_t *irvec_ep, const double *xq_all, const zc *uph_all,
+                              const double *omega_all, int nq_list, const int64_t *iq_list, const double *xk,
+                              const int32_t *kgrid, const zc *U, const int64_t *bmin, const int64_t *bmax,
+                              int64_t nks, const int64_t *ks, int nbw, double *g2sum, int64_t nout,
+                              int64_t *ik_out, zc *ep_out) {
+    const int64_t nb = (int64_t)nw * nw, nd = nb * nm, ndp = nd * nr_el;
+    zc *gq = (zc *)malloc(sizeof(zc) * ndp);
+    zc *eRpq = (zc *)malloc(sizeof(zc) * ndp);
+    ephor_gridopt *gp = ephor_gridopt_new(nr_ep, irvec_ep, ndp, 1);
+    int nthr = ephor_num_threads();
+    ephor_gridopt **ge = (ephor_gridopt **)malloc(sizeof(void *) * nthr);
+    for (int t = 0; t < nthr; t++) ge[t] = ephor_gridopt_new(nr_el, irvec_el, nd, 0);
+    int64_t npairs = 0, nrec = 0;
+    for (int64_t i = 0; i < nout; i++) ik_out[i] = -1;
+    for (int qi = 0; qi < nq_list; qi++) {
+        const int64_t iq = iq_list[qi];
+        const double *xq = xq_all + 3 * iq;
+        const zc *uph = uph_all + (int64_t)nm * nm * iq;
+        const double *om = omega_all + (int64_t)nm * iq;
+        ephor_gridopt_get(gp, epmat, ndp, xq, gq);
+#pragma omp parallel for schedule(static)
+        for (int re = 0; re < nr_el; re++)
+            for (int nu2 = 0; nu2 < nm; nu2++)
+                for (int64_t b = 0; b < nb; b++) {
+                    zc s = 0.0;
+                    for (int nu = 0; nu < nm; nu++) s += gq[b + nb * (nu + (int64_t)nm * re)] * uph[nu + nm * nu2];
+                    eRpq[b + nb * (nu2 + (int64_t)nm * re)] = s;
+                }
+        double qsum = 0.0;
+        int64_t qpairs = 0;
+#pragma omp parallel num_threads(nthr) reduction(+ : qsum, qpairs)
+        {
+            int tid = 0, nt = 1;
+#ifdef _OPENMP
+            tid = omp_get_thread_num();
+            nt = omp_get_num_threads();
+#endif
+            ephor_gridopt *g = ge[tid];
+            ephor_gridopt_reset(g); /* a new operator g(Re, q) */
+            zc *gk = (zc *)malloc(sizeof(zc) * nd);
+            zc *ep = (zc *)malloc(sizeof(zc) * nb * nm);
+            const int64_t c0 = nks * tid / nt, c1 = nks * (tid + 1) / nt;
+            for (int64_t c = c0; c < c1; c++) {
+                const int64_t ik = ks[c];
+                double xkq[3] = {xk[3 * ik] + xq[0], xk[3 * ik + 1] + xq[1], xk[3 * ik + 2] + xq[2]};
+                int64_t ikq;
+                const double zero_shift[3] = {0.0, 0.0, 0.0};
+                ephor_hash_xk(1, xkq, kgrid, zero_shift, &ikq);
+                if (ikq < 0 || bmin[ik] <= 0 || bmin[ikq] <= 0) continue;
+                ephor_gridopt_get(g, eRpq, nd, xk + 3 * ik, gk);
+                const int nlo = (int)bmin[ik] - 1, nbk = (int)bmax[ik] - nlo;
+                const int mlo = (int)bmin[ikq] - 1, nbkq = (int)bmax[ikq] - mlo;
+                rotate_rqkq(nw, nm, gk, U + nb * ik + (int64_t)nw * nlo, nbk, U + nb * ikq + (int64_t)nw * mlo,
+                            nbkq, ep);
+                for (int nu = 0; nu < nm; nu++) {
+                    const double inv = 1.0 / (2.0 * om[nu]);
+                    for (int mn = 0; mn < nbk * nbkq; mn++) {
+                        const zc v = ep[mn + (int64_t)nbkq * nbk * nu];
+                        qsum += (creal(v) * creal(v) + cimag(v) * cimag(v)) * inv;
+                    }
+                }
+                qpairs++;
+                if (qi == 0 && ep_out) {
+                    int64_t slot;
+#pragma omp atomic capture
+                    slot = nrec++;
+                    if (slot < nout) {
+                        ik_out[slot] = ik;
+                        zc *o = ep_out + (int64_t)nbw * nbw * nm * slot;
+                        for (int64_t i = 0; i < (int64_t)nbw * nbw * nm; i++) o[i] = 0.0;
+                        for (int nu = 0; nu < nm; nu++)
+                            for (int n = 0; n < nbk; n++)
+                                for (int m = 0; m < nbkq; m++)
+                                    o[m + nbw * (n + (int64_t)nbw * nu)] = ep[m + nbkq * (n + (int64_t)nbk * nu)];
+                    }
+                }
+            }
+            free(gk);
+            free(ep);
+        }
+        g2sum[qi] = qsum;
+        npairs += qpairs;
+    }
+    for (int t = 0; t < nthr; t++) ephor_gridopt_free(ge[t]);
+    free(ge);
+    ephor_gridopt_free(gp);
+    free(gq);
+    free(eRpq);
+    return npairs;
+}

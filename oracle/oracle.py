@@ -60,12 +60,22 @@ def lib():
                                        _dp, _dp, _i32, _i32, _d, _i64, _dp, _dp],
             "ephor_bte_increments": [_i32, _d, _d, _d, _d, _d, _d, _d, _i32, _d, _dp],
             "ephor_num_threads": [],
+            "ephor_gridopt_new": [_i32, _ip, _i64, _i32],
+            "ephor_gridopt_free": [C.c_void_p],
+            "ephor_gridopt_reset": [C.c_void_p],
+            "ephor_gridopt_get": [C.c_void_p, _zp, _i64, _dp, _zp],
+            "ephor_gridopt_nr23": [C.c_void_p],
+            "ephor_gridopt_nr3": [C.c_void_p],
+            "ephor_outer_q_gridopt": [_i32, _i32, _i32, _i32, _zp, _ip, _ip, _dp, _zp, _dp, _i32, _lp, _dp, _ip,
+                                      _zp, _lp, _lp, _i64, _lp, _i32, _dp, _i64, _lp, C.c_void_p],
         }
         for name, args in sig.items():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = C.c_int if name in ("ephor_heev", "ephor_heev_batched", "ephor_ph_eigen",
-                                            "ephor_num_threads") else None
+                                            "ephor_num_threads", "ephor_gridopt_nr23", "ephor_gridopt_nr3") else None
+        L.ephor_gridopt_new.restype = C.c_void_p
+        L.ephor_outer_q_gridopt.restype = C.c_int64
         _LIB = L
     return _LIB
 
@@ -240,3 +250,49 @@ def bte_accumulate_q(nm, nbw, ik, ikq, g2v, bmin, bmax, off, e_state, omega_q, w
 
 def num_threads() -> int:
     return lib().ephor_num_threads()
+
+
+class GridOpt:
+    """The GridOpt staged transform of one operator (ephor_gridopt_*): per-k queries reuse the k1-plane
+    and (k1, k2)-line partial sums while k1 (k2) stays the same."""
+
+    def __init__(self, op_r, irvec, threads=False):
+        self.op = _c(op_r, np.complex128)  # (nr, ld)
+        self.irvec = _c(irvec, np.int32)
+        self.nd = self.op.shape[1]
+        self.h = lib().ephor_gridopt_new(self.irvec.shape[0], self.irvec, self.nd, int(threads))
+
+    @property
+    def groups(self):
+        return lib().ephor_gridopt_nr23(self.h), lib().ephor_gridopt_nr3(self.h)
+
+    def get(self, xk) -> np.ndarray:
+        out = np.empty(self.nd, np.complex128)
+        lib().ephor_gridopt_get(self.h, self.op, self.nd, _c(xk, np.float64), out)
+        return out
+
+    def reset(self):
+        lib().ephor_gridopt_reset(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ephor_gridopt_free(self.h)
+            self.h = None
+
+
+def outer_q_gridopt(nw, nm, epmat_ph, irvec_el, irvec_ep, xq, uph_colmajor, omega, iq_list, xk_grid, kgrid, U,
+                    bmin, bmax, ksurv, nbw, nout=0):
+    """The reference's CPU outer-q loop in GridOpt mode over the q of iq_list (see ephor_outer_q_gridopt).
+    uph_colmajor (nq, nm*nm) column-major blocks, omega (nq, nm). Returns (pairs, g2sum per q, ik of the
+    recorded pairs of iq_list[0], their ep (nout, nm, nbw, nbw) C-order)."""
+    iq_list = _c(iq_list, np.int64)
+    g2sum = np.empty(len(iq_list))
+    ik_out = np.empty(max(nout, 1), np.int64)
+    ep_out = np.empty((max(nout, 1), nm, nbw, nbw), np.complex128)
+    n = lib().ephor_outer_q_gridopt(nw, nm, len(irvec_el), len(irvec_ep), _c(epmat_ph, np.complex128),
+                                    _c(irvec_el, np.int32), _c(irvec_ep, np.int32), _c(xq, np.float64),
+                                    _c(uph_colmajor, np.complex128), _c(omega, np.float64), len(iq_list), iq_list,
+                                    _c(xk_grid, np.float64), _c(kgrid, np.int32), _c(U, np.complex128),
+                                    _c(bmin, np.int64), _c(bmax, np.int64), len(ksurv), _c(ksurv, np.int64), nbw,
+                                    g2sum, nout, ik_out, ep_out.ctypes.data if nout > 0 else None)
+    return int(n), g2sum, ik_out[:nout], ep_out[:nout]
