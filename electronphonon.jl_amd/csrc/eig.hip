@@ -17,6 +17,8 @@
 // eigenvalues and orthonormal vectors also for exactly and nearly degenerate bands.
 #include "common.hpp"
 
+#include <mutex>
+
 namespace ephip {
 
 struct c32 {
@@ -522,4 +524,60 @@ extern "C" int ephip_eigvals_batched(int32_t n, int64_t nb, const void *H64, dou
     EPHIP_REQUIRE(H64 && E64 && work, "eigvals_batched: null device pointer");
     char *w = static_cast<char *>(work);
     return eigen_refined_run(n, nb, H64, E64, w, reinterpret_cast<float *>(w + 8LL * n * n * nb), info, stream);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Allocation-free eigenvalue entry: the workspace of ephip_eigvals_batched (FP32 eigenvectors, then FP32
+// eigenvalues) lives in a file-scope device array instead of a caller buffer. Each GPU the library runs
+// on holds its own copy until the process ends; its address on a device is looked up once, with that
+// device current, and kept. Calls that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------------------------
+namespace ephip {
+
+constexpr size_t EIG_SCRATCH_BYTES = size_t(16) << 20;  // 16 MiB per GPU
+constexpr int EIG_SCRATCH_DEVICES = 64;
+
+static __device__ __align__(256) char scratch[EIG_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[EIG_SCRATCH_DEVICES];
+
+// Device address of this GPU's scratch array (the host-side name is not a device address).
+static int scratch_address(void **out) {
+    int dev = -1;
+    EPHIP_TRY_HIP(hipGetDevice(&dev));
+    EPHIP_REQUIRE(dev >= 0 && dev < EIG_SCRATCH_DEVICES, "eigvals_batched_noalloc: device %d outside 0..%d", dev,
+                  EIG_SCRATCH_DEVICES - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        EPHIP_TRY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        EPHIP_REQUIRE(p != nullptr, "eigvals_batched_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return EPHIP_OK;
+}
+
+}  // namespace ephip
+
+extern "C" int ephip_eigvals_batched_noalloc(int32_t n, int64_t nb, const void *H64, double *E64, int32_t *info,
+                                             void *stream) {
+    int st = heev_launch_check(n, nb);
+    if (st) return st;
+    EPHIP_REQUIRE(n <= 31, "eigvals_batched_noalloc: n=%d > 31 (FP64 refinement limit)", n);
+    if (nb == 0) return EPHIP_OK;
+    const int64_t per = eigvals_work_bytes_per_matrix(n);
+    const int64_t cap = (int64_t)(EIG_SCRATCH_BYTES / (size_t)per);
+    EPHIP_REQUIRE(nb <= cap,
+                  "eigvals_batched_noalloc: %lld matrices of n=%d need %lld bytes of workspace; the %zu-byte scratch "
+                  "holds %lld (call in chunks, or use ephip_eigvals_batched with a caller workspace)",
+                  (long long)nb, n, (long long)(nb * per), EIG_SCRATCH_BYTES, (long long)cap);
+    EPHIP_REQUIRE(H64 && E64, "eigvals_batched_noalloc: null device pointer");
+    void *w = nullptr;
+    st = scratch_address(&w);
+    if (st) return st;
+    char *wc = static_cast<char *>(w);
+    // every entry of the workspace is written by the FP32 solve before the refinement reads it: no clearing
+    return eigen_refined_run(n, nb, H64, E64, wc, reinterpret_cast<float *>(wc + 8LL * n * n * nb), info, stream);
 }

@@ -399,6 +399,23 @@ def eigvals_batched_refined(H64):
     return E64
 
 
+def eigvals_batched_noalloc(H64, E64, info=None):
+    """ephip_eigvals_batched_noalloc: the eigenvalues of ``eigvals_batched_refined`` written into the
+    caller's float64 (nk, n) tensor ``E64`` (and int32 ``info`` per matrix, optional) with no device
+    allocation: the workspace is the library's 16 MiB per-GPU scratch array, so nk <= 16 MiB / (8 n^2 + 4 n).
+    Launch-only (no synchronisation, no convergence check)."""
+    torch = _torch()
+    if H64.dtype != torch.complex128 or E64.dtype != torch.float64:
+        raise L.EphipError("eigvals_batched_noalloc needs complex128 matrices and a float64 output")
+    if not (H64.is_contiguous() and E64.is_contiguous()):
+        raise L.EphipError("eigvals_batched_noalloc needs contiguous tensors")
+    nb, n = H64.shape[0], H64.shape[1]
+    if tuple(E64.shape) != (nb, n) or (info is not None and (info.dtype != torch.int32 or info.numel() != nb)):
+        raise L.EphipError("eigvals_batched_noalloc: E64 must be (nk, n), info int32 (nk)")
+    L.call("ephip_eigvals_batched_noalloc", n, nb, L.ptr(H64), L.ptr(E64), L.ptr(info) if info is not None else None,
+           L.stream_handle())
+
+
 # ------------------------------------------------------------------------------------------------
 # Window filter (src/filter.jl:19, 40-100, 195-225)
 # ------------------------------------------------------------------------------------------------

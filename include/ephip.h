@@ -103,6 +103,14 @@ int ephip_eigen_batched(int32_t n, int64_t nb, const void *H64, double *E64, voi
 int ephip_eigvals_batched(int32_t n, int64_t nb, const void *H64, double *E64, void *work, int32_t *info,
                           void *stream);
 
+/* eigvals_batched without a workspace argument and without any allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same results as ephip_eigvals_batched; its workspace is a
+ * file-scope device array of 16 MiB per GPU (held until the process ends), so nb <= floor(16 MiB /
+ * (8 n^2 + 4 n)) (7943 matrices at n = 16); larger nb returns EPHIP_ERR_ARG. Calls that overlap on one GPU
+ * share the array and race: serialise them (one stream, or events). n <= 31. */
+int ephip_eigvals_batched_noalloc(int32_t n, int64_t nb, const void *H64, double *E64, int32_t *info, void *stream);
+
 /* _filter_kpoints / inside_window -- src/filter.jl:19, :40-100 (device branch :46-72).
  * E: double (nw x nk) ascending. keep[k] = any band in [wmin, wmax]; bmin/bmax: 1-based
  * searchsortedfirst(E, wmin) / searchsortedlast(E, wmax), 0 when keep = 0; nbelow[k] = bmin - 1

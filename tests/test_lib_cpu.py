@@ -128,3 +128,15 @@ def test_eigen_entries_validate_arguments():
         L.call("ephip_eigvals_batched", 16, 4, 1, 1, None, None, None)
     L.call("ephip_eigvals_batched", 16, 0, None, None, None, None, None)
     L.call("ephip_eigen_batched", 8, 0, None, None, None, None, None, None)
+
+
+def test_eigvals_noalloc_validates_before_any_device_call():
+    """The allocation-free entry rejects sizes beyond its scratch array (16 MiB per GPU) and bad arguments
+    before touching the device; nb = 0 is a no-op."""
+    with pytest.raises(L.EphipError, match="scratch"):
+        L.call("ephip_eigvals_batched_noalloc", 16, 7944, 1, 1, None, None)
+    with pytest.raises(L.EphipError, match="n=32 > 31"):
+        L.call("ephip_eigvals_batched_noalloc", 32, 4, 1, 1, None, None)
+    with pytest.raises(L.EphipError, match="null device pointer"):
+        L.call("ephip_eigvals_batched_noalloc", 16, 7943, None, None, None, None)
+    L.call("ephip_eigvals_batched_noalloc", 16, 0, None, None, None, None)
