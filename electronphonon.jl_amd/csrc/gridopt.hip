@@ -977,7 +977,16 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
             const int iq = (q1 * m2 + ((sk_nb[kk] >> 22) & 63)) * m3 + ((cid >> 16) & 63);
             const cplx<float> *up = uph + (int64_t)nm * nm * iq + nm * nu2;
             float er = 0.f, ei = 0.f;
-            if constexpr (NMC > 0) {
+            if constexpr (NMC > 0 && NMC % 2 == 0) {  // the u_ph column as 16-byte loads (nm^2 iq + nm nu2 even)
+                const float4 *up4 = reinterpret_cast<const float4 *>(up);
+#pragma unroll
+                for (int h = 0; h < NMC / 2; h++) {
+                    const float4 p = up4[h];
+                    const float2 t0 = tq[2 * h + 17 * col], t1 = tq[2 * h + 1 + 17 * col];
+                    cmac(er, ei, t0.x, t0.y, p.x, p.y);
+                    cmac(er, ei, t1.x, t1.y, p.z, p.w);
+                }
+            } else if constexpr (NMC > 0) {
 #pragma unroll
                 for (int n = 0; n < NMC; n++) {
                     const float2 tv = tq[n + 17 * col];
