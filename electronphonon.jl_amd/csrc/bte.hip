@@ -125,6 +125,59 @@ __global__ void bte_accumulate_kernel(int nunits, int64_t cs, int nbw, int nm, c
     }
 }
 
+// bte_window_accumulate! with the reference's arguments (ext/ElectronPhononCUDAExt.jl:323-372, generic
+// src/boltzmann/boltzmann_calculator.jl:336-372): one thread per (m, n, j) of a q batch. i = imap_i[n]
+// (outer state, 1-based, 0 = outside the window), f = imap_f[m + ld_f (ikqs[j] - 1)] (inner state); per
+// temperature the increments of every mode with w >= w_cut are summed; S_out[(i - 1) + ld_so iT] gets an
+// atomic add, S_in[(i - i0 - 1) + ld_si1 ((f - 1) + ld_si2 iT)] the plain write (each (i, f) hit once).
+// g2vals (nbkq, nbk, nm, nq) and w (nm, nq) in R (float or double); everything else FP64 / int64.
+template <typename R>
+__global__ void bte_window_accumulate_kernel(double *__restrict__ S_out, int64_t ld_so, double *__restrict__ S_in,
+                                             int64_t ld_si1, int64_t ld_si2, const R *__restrict__ g2vals,
+                                             const R *__restrict__ wq, const int64_t *__restrict__ imap_i,
+                                             const int64_t *__restrict__ imap_f, int64_t ld_f, int64_t nk_f,
+                                             const int64_t *__restrict__ ikqs, const double *__restrict__ e_i,
+                                             const double *__restrict__ e_f, const double *__restrict__ wf, int nT,
+                                             const double *__restrict__ mus, const double *__restrict__ Ts,
+                                             const double *__restrict__ etas, int method, int smearing, double w_cut,
+                                             int nbkq, int nbk, int nm, int64_t nq, int64_t i0,
+                                             uint32_t *__restrict__ dropped) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nbkq * nbk * nq) return;
+    const int m = (int)(e % nbkq);
+    const int n = (int)((e / nbkq) % nbk);
+    const int64_t j = e / ((int64_t)nbkq * nbk);
+    const int64_t i = imap_i[n];
+    if (i <= 0) return;
+    const int64_t kq = ikqs[j];
+    if (kq < 1 || kq > nk_f || i - i0 < 1 || i - i0 > ld_si1) {
+        if (dropped) atomicAdd(dropped, 1u);
+        return;
+    }
+    const int64_t f = imap_f[m + ld_f * (kq - 1)];
+    if (f <= 0) return;
+    if (f > ld_si2) {
+        if (dropped) atomicAdd(dropped, 1u);
+        return;
+    }
+    const double ek = e_i[i - 1], ekq = e_f[f - 1], wtq = wf[f - 1];
+    const R *gr = g2vals + m + (int64_t)nbkq * (n + (int64_t)nbk * nm * j);
+    const int64_t nbb = (int64_t)nbkq * nbk;
+    for (int t = 0; t < nT; t++) {
+        double so = 0.0, si = 0.0;
+        for (int nu = 0; nu < nm; nu++) {
+            const double w = (double)wq[nu + (int64_t)nm * j];
+            if (w < w_cut) continue;
+            double a, b;
+            bte_increments(method, ek, ekq, w, (double)gr[nbb * nu], wtq, mus[t], Ts[t], smearing, etas[t], a, b);
+            so += a;
+            si += b;
+        }
+        atomicAdd(&S_out[(i - 1) + ld_so * t], so);
+        S_in[(i - i0 - 1) + ld_si1 * ((f - 1) + ld_si2 * t)] = si;
+    }
+}
+
 // Diagonal band velocities: v[s][a] = Re(u^H dH_a u) for state s = (k = state_k[s], band = state_band[s]);
 // dH: per state-k column, 3 operators of nw*nw (c64), column index state_col[s].
 __global__ void band_velocity_kernel(int nw, int64_t nstates, const int64_t *__restrict__ state_col,
@@ -324,6 +377,38 @@ extern "C" int ephip_bte_spmv(int32_t nunits, int32_t nbw, int32_t nT, int32_t i
     EPHIP_REQUIRE(rowptr && row_code && unit_state_index && S_in && x && y, "bte_spmv: null device pointer");
     bte_spmv_kernel<<<(unsigned)((nunits + 3) / 4), 256, 0, as_stream(stream)>>>(nunits, nbw, nT, iT, rowptr, row_code,
                                                                                 unit_state_index, S_in, x, y);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_bte_window_accumulate(int32_t fp64, double *S_out, int64_t ld_so, double *S_in, int64_t ld_si1,
+                                           int64_t ld_si2, const void *g2vals, const void *wq, const int64_t *imap_i,
+                                           const int64_t *imap_f, int64_t ld_f, int64_t nk_f, const int64_t *ikqs,
+                                           const double *e_i, const double *e_f, const double *wf, int32_t nT,
+                                           const double *mus, const double *Ts, const double *etas, int32_t method,
+                                           int32_t smearing, double w_cut, int32_t nbandkq, int32_t nbandk, int32_t nm,
+                                           int64_t nq, int64_t i0, uint32_t *dropped, void *stream) {
+    EPHIP_REQUIRE(nbandkq >= 0 && nbandk >= 0 && nm >= 0 && nq >= 0 && nT >= 1 && i0 >= 0 && nk_f >= 0,
+                  "bte_window_accumulate: bad sizes");
+    EPHIP_REQUIRE(ld_f >= (nbandkq > 0 ? nbandkq : 1) && ld_si1 >= 1 && ld_si2 >= 1 && ld_so >= 1,
+                  "bte_window_accumulate: bad leading dimensions");
+    EPHIP_REQUIRE(method >= 1 && method <= 6 && (smearing == 1 || smearing == 2),
+                  "bte_window_accumulate: bad method/smearing");
+    const int64_t ne = (int64_t)nbandkq * nbandk * nq;
+    if (ne == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(S_out && S_in && g2vals && wq && imap_i && imap_f && ikqs && e_i && e_f && wf && mus && Ts && etas,
+                  "bte_window_accumulate: null device pointer");
+    EPHIP_REQUIRE(ne <= 0x7fffffffLL * 256, "bte_window_accumulate: too many entries");
+    if (fp64)
+        bte_window_accumulate_kernel<double><<<nbk(ne), 256, 0, as_stream(stream)>>>(
+            S_out, ld_so, S_in, ld_si1, ld_si2, static_cast<const double *>(g2vals), static_cast<const double *>(wq),
+            imap_i, imap_f, ld_f, nk_f, ikqs, e_i, e_f, wf, nT, mus, Ts, etas, method, smearing, w_cut, nbandkq, nbandk, nm, nq,
+            i0, dropped);
+    else
+        bte_window_accumulate_kernel<float><<<nbk(ne), 256, 0, as_stream(stream)>>>(
+            S_out, ld_so, S_in, ld_si1, ld_si2, static_cast<const float *>(g2vals), static_cast<const float *>(wq),
+            imap_i, imap_f, ld_f, nk_f, ikqs, e_i, e_f, wf, nT, mus, Ts, etas, method, smearing, w_cut, nbandkq, nbandk, nm, nq,
+            i0, dropped);
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

@@ -325,6 +325,78 @@ __global__ void window_scatter_kernel(T *__restrict__ g2_out, T *__restrict__ w_
     w_out[lin] = w[nu + (int64_t)nm * iq];
 }
 
+// eph_apply_rotations! (ext/ElectronPhononCUDAExt.jl:177-201): one thread per (m, n, nu', q) of ep
+// (nbkq, nbk, nm, nq), dense column-major: ep = sum_nu (sum_iw conj(ukq[iw, m, q]) g[iw, n, nu, q]) uph[nu, nu', q],
+// g2 = |ep|^2 / (2 w[nu', q]) when g2 is given. g is read through its four strides (elements), so strided
+// views are taken as they are; ukq (nw, nbkq, nq), uph (nm, nm, nq) and w (nm, nq) dense.
+template <typename V, typename R>
+__global__ void apply_rotations_kernel(int nw, int nbkq, int nbk, int nm, int64_t nq, V *__restrict__ ep,
+                                       const V *__restrict__ g, int64_t s0, int64_t s1, int64_t s2, int64_t s3,
+                                       const V *__restrict__ ukq, const V *__restrict__ uph, R *__restrict__ g2,
+                                       const R *__restrict__ w) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nbkq * nbk * nm * nq) return;
+    const int m = (int)(e % nbkq);
+    const int n = (int)((e / nbkq) % nbk);
+    const int nu2 = (int)((e / ((int64_t)nbkq * nbk)) % nm);
+    const int64_t q = e / ((int64_t)nbkq * nbk * nm);
+    const V *uk = ukq + (int64_t)nw * nbkq * q + (int64_t)nw * m;
+    const V *up = uph + (int64_t)nm * nm * q + (int64_t)nm * nu2;
+    const V *gq = g + s1 * n + s3 * q;
+    R er = 0, ei = 0;
+    for (int nu = 0; nu < nm; nu++) {
+        R tr = 0, ti = 0;
+        for (int iw = 0; iw < nw; iw++) {
+            const V a = uk[iw], b = gq[s0 * iw + s2 * nu];
+            tr += a.x * b.x + a.y * b.y;
+            ti += a.x * b.y - a.y * b.x;
+        }
+        const V p = up[nu];
+        er += tr * p.x - ti * p.y;
+        ei += tr * p.y + ti * p.x;
+    }
+    V out;
+    out.x = er;
+    out.y = ei;
+    ep[e] = out;
+    if (g2) g2[e] = (er * er + ei * ei) / (2 * w[nu2 + (int64_t)nm * q]);
+}
+
+// eph_apply_rotations_rqkq! (ext/ElectronPhononCUDAExt.jl:241-258, generic src/wannier_to_bloch_batched.jl:
+// 360-377): one thread per (m, n, nu, k) of ep (nbkq, nbk, nm, nk), dense column-major:
+// ep = sum_jw (sum_iw conj(ukq[iw, m, k]) g[iw, jw, nu, k]) uk[jw, n, k]; g (nw, nw, nm, nk), uk (nw, nbk, nk),
+// ukq (nw, nbkq, nk) dense.
+template <typename V, typename R>
+__global__ void apply_rotations_rqkq_kernel(int nw, int nbkq, int nbk, int nm, int64_t nk, V *__restrict__ ep,
+                                            const V *__restrict__ g, const V *__restrict__ uk,
+                                            const V *__restrict__ ukq) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nbkq * nbk * nm * nk) return;
+    const int m = (int)(e % nbkq);
+    const int n = (int)((e / nbkq) % nbk);
+    const int nu = (int)((e / ((int64_t)nbkq * nbk)) % nm);
+    const int64_t k = e / ((int64_t)nbkq * nbk * nm);
+    const V *ul = ukq + (int64_t)nw * nbkq * k + (int64_t)nw * m;
+    const V *ur = uk + (int64_t)nw * nbk * k + (int64_t)nw * n;
+    const V *gk = g + (int64_t)nw * nw * (nu + (int64_t)nm * k);
+    R er = 0, ei = 0;
+    for (int jw = 0; jw < nw; jw++) {
+        R tr = 0, ti = 0;
+        for (int iw = 0; iw < nw; iw++) {
+            const V a = ul[iw], b = gk[iw + (int64_t)nw * jw];
+            tr += a.x * b.x + a.y * b.y;
+            ti += a.x * b.y - a.y * b.x;
+        }
+        const V p = ur[jw];
+        er += tr * p.x - ti * p.y;
+        ei += tr * p.y + ti * p.x;
+    }
+    V out;
+    out.x = er;
+    out.y = ei;
+    ep[e] = out;
+}
+
 }  // namespace ephip
 
 static bool valid_trans(int32_t t) { return t == 'N' || t == 'T' || t == 'C'; }
@@ -379,6 +451,51 @@ extern "C" int ephip_eph_window_scatter(int32_t fp64, void *g2_out, void *w_out,
             static_cast<float *>(g2_out), static_cast<float *>(w_out), static_cast<const float *>(g2vals), imap_i,
             imap_f, ld_f, nk_f, ikqs, static_cast<const float *>(w), nbkq, nbk, nm, nq, ni_stride, i0, n_out,
             dropped);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_eph_apply_rotations(int32_t fp64, int32_t nw, int32_t nbkq, int32_t nbk, int32_t nm, int64_t nq,
+                                         void *ep, const void *g, int64_t gs0, int64_t gs1, int64_t gs2, int64_t gs3,
+                                         const void *ukqs, const void *uphs, void *g2_out, const void *wq,
+                                         void *stream) {
+    EPHIP_REQUIRE(nw >= 1 && nbkq >= 0 && nbk >= 0 && nm >= 0 && nq >= 0, "eph_apply_rotations: bad sizes");
+    EPHIP_REQUIRE(nbkq <= nw && nbk <= nw, "eph_apply_rotations: band counts above nw");
+    const int64_t ne = (int64_t)nbkq * nbk * nm * nq;
+    if (ne == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(ep && g && ukqs && uphs && (!g2_out || wq), "eph_apply_rotations: null device pointer");
+    EPHIP_REQUIRE(ne <= 0x7fffffffLL * 256, "eph_apply_rotations: too many entries");
+    if (fp64)
+        apply_rotations_kernel<double2, double><<<blocks_for(ne), 256, 0, as_stream(stream)>>>(
+            nw, nbkq, nbk, nm, nq, static_cast<double2 *>(ep), static_cast<const double2 *>(g), gs0, gs1, gs2, gs3,
+            static_cast<const double2 *>(ukqs), static_cast<const double2 *>(uphs), static_cast<double *>(g2_out),
+            static_cast<const double *>(wq));
+    else
+        apply_rotations_kernel<float2, float><<<blocks_for(ne), 256, 0, as_stream(stream)>>>(
+            nw, nbkq, nbk, nm, nq, static_cast<float2 *>(ep), static_cast<const float2 *>(g), gs0, gs1, gs2, gs3,
+            static_cast<const float2 *>(ukqs), static_cast<const float2 *>(uphs), static_cast<float *>(g2_out),
+            static_cast<const float *>(wq));
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_eph_apply_rotations_rqkq(int32_t fp64, int32_t nw, int32_t nbkq, int32_t nbk, int32_t nm,
+                                              int64_t nk, void *ep, const void *g, const void *uks, const void *ukqs,
+                                              void *stream) {
+    EPHIP_REQUIRE(nw >= 1 && nbkq >= 0 && nbk >= 0 && nm >= 0 && nk >= 0, "eph_apply_rotations_rqkq: bad sizes");
+    EPHIP_REQUIRE(nbkq <= nw && nbk <= nw, "eph_apply_rotations_rqkq: band counts above nw");
+    const int64_t ne = (int64_t)nbkq * nbk * nm * nk;
+    if (ne == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(ep && g && uks && ukqs, "eph_apply_rotations_rqkq: null device pointer");
+    EPHIP_REQUIRE(ne <= 0x7fffffffLL * 256, "eph_apply_rotations_rqkq: too many entries");
+    if (fp64)
+        apply_rotations_rqkq_kernel<double2, double><<<blocks_for(ne), 256, 0, as_stream(stream)>>>(
+            nw, nbkq, nbk, nm, nk, static_cast<double2 *>(ep), static_cast<const double2 *>(g),
+            static_cast<const double2 *>(uks), static_cast<const double2 *>(ukqs));
+    else
+        apply_rotations_rqkq_kernel<float2, float><<<blocks_for(ne), 256, 0, as_stream(stream)>>>(
+            nw, nbkq, nbk, nm, nk, static_cast<float2 *>(ep), static_cast<const float2 *>(g),
+            static_cast<const float2 *>(uks), static_cast<const float2 *>(ukqs));
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

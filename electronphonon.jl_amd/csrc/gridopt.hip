@@ -663,7 +663,7 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
     const cplx<float> *__restrict__ A, int64_t lda, int64_t ldr, const int *__restrict__ line_off,
     const int *__restrict__ col_index, const int *__restrict__ ikq, int64_t cs, const cplx<float> *__restrict__ U,
     const int64_t *__restrict__ bmin, const int64_t *__restrict__ bmax, const cplx<float> *__restrict__ uph,
-    const double *__restrict__ omega, int nbw, float *__restrict__ g2) {
+    const double *__restrict__ omega, int nbw, float *__restrict__ g2, float2 *__restrict__ epo) {
     constexpr int NW = 16;
     // q1, all q2, all q3 phases; the q2/q3 rows padded to an even count and read as 16-byte pairs
     constexpr int NE = (N + 1) & ~1;
@@ -838,9 +838,10 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
             for (int m = 0; m < nbw; m++) {  // uniform
                 for (int e = t; e < nk * NMC; e += blockDim.x) {
                     const int kk = e / NMC, nu2 = e % NMC;
-                    float* out = g2 + (row0 + slab0 + ks + kk) * nbw * NMC + m * NMC + nu2;
+                    const int64_t oi = (row0 + slab0 + ks + kk) * nbw * NMC + m * NMC + nu2;
                     if (m >= (sk_nb[ks + kk] & 0xff)) {  // past the k+q window: zero
-                        *out = 0.f;
+                        g2[oi] = 0.f;
+                        if (epo) epo[oi] = make_float2(0.f, 0.f);
                         continue;
                     }
                     const int iq = sk_iq[ks + kk];
@@ -853,7 +854,8 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
                         cmac(er, ei, tv.x, tv.y, p.re, p.im);
                     }
                     const float w2 = 2.f * static_cast<float>(omega[(int64_t)NMC * iq + nu2]);
-                    *out = (er * er + ei * ei) / w2;
+                    g2[oi] = (er * er + ei * ei) / w2;
+                    if (epo) epo[oi] = make_float2(er, ei);
                 }
             }
         } else {
@@ -862,6 +864,7 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
                 if constexpr (!PAIRS) {  // past the k+q window: zero (its t may not have been formed)
                     if (m >= (sk_nb[ks + kk] & 0xff)) {
                         g2[(row0 + slab0 + ks + kk) * nbw * nm + rr] = 0.f;
+                        if (epo) epo[(row0 + slab0 + ks + kk) * nbw * nm + rr] = make_float2(0.f, 0.f);
                         continue;
                     }
                 }
@@ -878,6 +881,7 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
                 }
                 const float w2 = 2.f * static_cast<float>(omega[(int64_t)nm * iq + nu2]);
                 g2[(row0 + slab0 + ks + kk) * nbw * nm + rr] = (er * er + ei * ei) / w2;
+                if (epo) epo[(row0 + slab0 + ks + kk) * nbw * nm + rr] = make_float2(er, ei);
             }
         }
     }
@@ -908,7 +912,7 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
                               const int *__restrict__ ikq, int64_t cs, const cplx<float> *__restrict__ U,
                               const int64_t *__restrict__ bmin, const int64_t *__restrict__ bmax,
                               const cplx<float> *__restrict__ uph, const double *__restrict__ omega, int nbw,
-                              float *__restrict__ g2) {
+                              float *__restrict__ g2, float2 *__restrict__ epo) {
     constexpr int NW = 16;
     constexpr int NE = (N + 1) & ~1;
     constexpr int BP = NMC > 0 ? NMC + 1 : 17;  // b2s pitch per i (odd)
@@ -962,6 +966,8 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
         sk_nb[kk] = nb | ((b0 > 0 ? (int)b0 - 1 : 0) << 8) | ((iq % m3) << 16) | (((iq / m3) % m2) << 22);
         float *o = g2 + row * nbw * nm;
         for (int e = nb * nm; e < nbw * nm; e++) o[e] = 0.f;  // slots past the k+q window
+        if (epo)
+            for (int e = nb * nm; e < nbw * nm; e++) epo[row * nbw * nm + e] = make_float2(0.f, 0.f);
     }
     __syncthreads();  // the slab tables, read by every wave from the first line on
     // stage 1 (rows G = t)
@@ -1005,6 +1011,7 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
             }
             const float w2 = 2.f * static_cast<float>(omega[(int64_t)nm * iq + nu2]);
             g2[(row0 + slab0 + kk) * nbw * nm + m * nm + nu2] = (er * er + ei * ei) / w2;
+            if (epo) epo[(row0 + slab0 + kk) * nbw * nm + m * nm + nu2] = make_float2(er, ei);
         }
         __syncthreads();  // the collected columns consumed
         tqn = 0;
@@ -1139,7 +1146,7 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
                                        const int32_t *line_off, const int32_t *col_index, const int32_t *ikq,
                                        int64_t cs, const void *U, const int64_t *bmin, const int64_t *bmax,
                                        const void *uph, const double *omega, const int32_t *dense_strides,
-                                       float *g2, void *stream) {
+                                       float *g2, void *ep, void *stream) {
     EPHIP_REQUIRE(nw == 16 && nm >= 1 && nm <= 16, "gridopt_kq_rotate: nw must be 16 and nm <= 16 (got %d, %d)", nw,
                   nm);
     EPHIP_REQUIRE(nbw >= 1 && nbw <= 4, "gridopt_kq_rotate: nbw=%d outside 1..4", nbw);
@@ -1174,7 +1181,8 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
         mtable[dense ? 1 : 0][nm == 12 ? 1 : 0][np - 3]<<<g, 256, lds, as_stream(stream)>>>(
             nm, n1, p1min, p2min, p3min, rmap, s1, s2, s3, qgrid[0], qgrid[1], qgrid[2],
             static_cast<const cplx<float> *>(A), lda, ldr, line_off, col_index, ikq, cs,
-            static_cast<const cplx<float> *>(U), bmin, bmax, static_cast<const cplx<float> *>(uph), omega, nbw, g2);
+            static_cast<const cplx<float> *>(U), bmin, bmax, static_cast<const cplx<float> *>(uph), omega, nbw, g2,
+            static_cast<float2 *>(ep));
         EPHIP_LAUNCHED();
         return EPHIP_OK;
     }
@@ -1194,7 +1202,8 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
     kern<<<g, threads, 0, as_stream(stream)>>>(nm, n1, p1min, p2min, p3min, rmap, s1, s2, s3, qgrid[0], qgrid[1],
                                                 qgrid[2], static_cast<const cplx<float> *>(A), lda, ldr, line_off,
                                                 col_index, ikq, cs, static_cast<const cplx<float> *>(U), bmin, bmax,
-                                                static_cast<const cplx<float> *>(uph), omega, nbw, g2);
+                                                static_cast<const cplx<float> *>(uph), omega, nbw, g2,
+                                                static_cast<float2 *>(ep));
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

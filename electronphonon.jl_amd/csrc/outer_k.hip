@@ -260,6 +260,46 @@ __global__ __launch_bounds__(256) void rotate_kq_g2_lds_kernel(int nw, int nm, i
     }
 }
 
+// One k of an outer-k payload in the layout of EPDataQBatched (src/EPData.jl:66-73, built by
+// src/calculator/run_eph_over_k_and_kq.jl:843-876): eps / g2s (nw, nbandk, nm, nq) over the full k+q band
+// range and the k-side window [ibandk_offset + 1, ibandk_offset + nbandk], for the kept columns
+// j0 .. j0 + nq - 1 of the k; zero outside both windows. The k's units (one per in-window band, band0 + 1 ..
+// band0 + nunits) are consecutive payload rows of stride cs columns; unit row columns hold (slot, nu')
+// for k+q band bmin - 1 + slot. wqs (nm, nq) = omega of each column's q; ikqs (nq) = state_pos of each
+// column's k+q state (1-based position in the filtered state list).
+__global__ void units_to_qbatched_kernel(int nw, int nm, int nbw, int nbandk, int64_t cs,
+                                         const int *__restrict__ col_index, const int *__restrict__ ikq, int j0, int nq,
+                                         int nunits, int band0, int ibandk_offset, const float2 *__restrict__ ep,
+                                         const float *__restrict__ g2, const int64_t *__restrict__ bmin,
+                                         const int64_t *__restrict__ bmax, const double *__restrict__ omega,
+                                         const int64_t *__restrict__ state_pos, float2 *__restrict__ eps,
+                                         float *__restrict__ g2s, double *__restrict__ wqs, int64_t *__restrict__ ikqs) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nw * nbandk * nm * nq) return;
+    const int m = (int)(e % nw);
+    const int n = (int)((e / nw) % nbandk);
+    const int nu2 = (int)((e / ((int64_t)nw * nbandk)) % nm);
+    const int j = (int)(e / ((int64_t)nw * nbandk * nm));
+    const int kq = ikq[j0 + j];
+    const int iq = col_index[j0 + j];
+    const int64_t b0 = bmin[kq];
+    const int nbq = b0 > 0 ? min((int)(bmax[kq] - b0 + 1), nbw) : 0;
+    const int slot = m - (int)(b0 - 1), unit = ibandk_offset + n - band0;
+    float2 v = make_float2(0.f, 0.f);
+    float w = 0.f;
+    if (b0 > 0 && slot >= 0 && slot < nbq && unit >= 0 && unit < nunits) {
+        const int64_t src = ((int64_t)unit * cs + j0 + j) * nbw * nm + (int64_t)slot * nm + nu2;
+        if (ep) v = ep[src];
+        w = g2[src];
+    }
+    if (eps) eps[e] = v;
+    g2s[e] = w;
+    if (m == 0 && n == 0) {
+        wqs[nu2 + (int64_t)nm * j] = omega[(int64_t)nm * iq + nu2];
+        if (nu2 == 0) ikqs[j] = state_pos[kq];
+    }
+}
+
 }  // namespace ephip
 
 using namespace ephip;
@@ -335,6 +375,28 @@ extern "C" int ephip_eph_rotate_kq_g2(int32_t nw, int32_t nm, int32_t nbw, int32
                                                                   static_cast<const c32 *>(uph), omega, g2,
                                                                   static_cast<c32 *>(ep));
     }
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_units_to_qbatched(int32_t nw, int32_t nm, int32_t nbw, int32_t nbandk, int64_t cs,
+                                       const int32_t *col_index, const int32_t *ikq, int32_t j0, int32_t nq,
+                                       int32_t nunits, int32_t band0, int32_t ibandk_offset, const void *ep,
+                                       const float *g2, const int64_t *bmin, const int64_t *bmax, const double *omega,
+                                       const int64_t *state_pos, void *eps, float *g2s, double *wqs, int64_t *ikqs,
+                                       void *stream) {
+    EPHIP_REQUIRE(nw >= 1 && nm >= 1 && nbw >= 1 && nbandk >= 1 && nbandk <= nw && cs >= 0 && j0 >= 0 && nq >= 0 &&
+                      nunits >= 1 && nunits <= nbandk && band0 >= 0 && ibandk_offset >= 0 &&
+                      ibandk_offset + nbandk <= nw && band0 >= ibandk_offset &&
+                      band0 + nunits <= ibandk_offset + nbandk && (int64_t)j0 + nq <= cs,
+                  "units_to_qbatched: bad sizes or window");
+    if (nq == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(col_index && ikq && g2 && bmin && bmax && omega && state_pos && g2s && wqs && ikqs,
+                  "units_to_qbatched: null device pointer");
+    const int64_t ne = (int64_t)nw * nbandk * nm * nq;
+    units_to_qbatched_kernel<<<nblocks(ne), 256, 0, as_stream(stream)>>>(
+        nw, nm, nbw, nbandk, cs, col_index, ikq, j0, nq, nunits, band0, ibandk_offset,
+        static_cast<const float2 *>(ep), g2, bmin, bmax, omega, state_pos, static_cast<float2 *>(eps), g2s, wqs, ikqs);
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

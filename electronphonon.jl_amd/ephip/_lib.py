@@ -57,6 +57,14 @@ SIGNATURES = {
     "ephip_el_velocity_direct_batched": [_i32, _i64, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _i64, _vp],
     "ephip_batched_gemm": [_i32, _i32, _i32, _i32, _i32, _i32, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp,
                            _i64, _i64, _vp],
+    "ephip_eph_apply_rotations": [_i32, _i32, _i32, _i32, _i32, _i64, _vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp,
+                                  _vp, _vp, _vp],
+    "ephip_eph_apply_rotations_rqkq": [_i32, _i32, _i32, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp],
+    "ephip_bte_window_accumulate": [_i32, _vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp,
+                                    _vp, _i32, _vp, _vp, _vp, _i32, _i32, _d, _i32, _i32, _i32, _i64, _i64, _vp,
+                                    _vp],
+    "ephip_units_to_qbatched": [_i32, _i32, _i32, _i32, _i64, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp,
+                                _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ephip_eph_window_scatter": [_i32, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _i64, _vp, _vp, _i32, _i32, _i32,
                                  _i64, _i64, _i64, _vp, _vp],
     "ephip_bte_spmv": [_i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
@@ -75,7 +83,7 @@ SIGNATURES = {
     "ephip_ft_pack_phases": [_i32, _vp, _i64, _vp, _vp, _vp],
     "ephip_kq_line_offsets": [_i32, _vp, _vp, _i64, _i32, _i32, _vp, _vp],
     "ephip_gridopt_kq_rotate": [_i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i64, _i64,
-                                _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+                                _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ephip_eph_RR_to_kR_project_packed_bf16x3": [_i32, _i32, _i32, _i32, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp,
                                                  _i32, _vp, _i64, _vp],
     "ephip_fourier_packed_bf16x3": [_i64, _i32, _vp, _i32, _vp, _i64, _i32, _vp, _vp, _i64, _vp, _i64, _i64, _vp],

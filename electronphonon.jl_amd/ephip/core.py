@@ -162,6 +162,90 @@ def eph_window_scatter(g2_out, w_out, g2vals, imap_i_col, imap_f, ikqs, w, nband
                                f"range: check i0={i0}, ni_stride={ni_stride} and ikqs)")
 
 
+def _cplx_pair(*xs):
+    torch = _torch()
+    dt = xs[0].dtype
+    if dt not in (torch.complex64, torch.complex128) or any(x.dtype != dt for x in xs):
+        raise L.EphipError("complex64 or complex128 operands, all of one type")
+    return int(dt == torch.complex128), (torch.float64 if dt == torch.complex128 else torch.float32)
+
+
+def eph_apply_rotations(ep, g, ukqs, u_phs, g2_out=None, wq=None):
+    """eph_apply_rotations! (src/wannier_to_bloch_batched.jl:442-455; CUDA method ext/ElectronPhononCUDAExt.jl:
+    177-201) through ephip_eph_apply_rotations. Arrays in the reference's column-major index order held as
+    torch tensors with the axes reversed: ep (nq, nm, nbk, nbkq) contiguous; g (nq, nm, nbk, nw) any strided
+    view (its strides are passed); ukqs (nq, nbkq, nw), u_phs (nq, nm, nm) contiguous; optional g2_out
+    (nq, nm, nbk, nbkq) and wq (nq, nm) real of the matching precision. Launch only."""
+    fp64, rdt = _cplx_pair(ep, g, ukqs, u_phs)
+    nq, nm, nbk, nbkq = ep.shape
+    nw = g.shape[3]
+    if tuple(g.shape) != (nq, nm, nbk, nw) or tuple(ukqs.shape) != (nq, nbkq, nw) or \
+            tuple(u_phs.shape) != (nq, nm, nm):
+        raise L.EphipError("eph_apply_rotations: shapes do not match ep (nq, nm, nbk, nbkq)")
+    if not (ep.is_contiguous() and ukqs.is_contiguous() and u_phs.is_contiguous()):
+        raise L.EphipError("eph_apply_rotations: ep, ukqs and u_phs must be dense")
+    if g2_out is not None:
+        if wq is None or g2_out.dtype != rdt or wq.dtype != rdt or tuple(g2_out.shape) != tuple(ep.shape) or \
+                tuple(wq.shape) != (nq, nm) or not (g2_out.is_contiguous() and wq.is_contiguous()):
+            raise L.EphipError("eph_apply_rotations: g2_out (as ep) and wq (nq, nm) dense, real of ep's precision")
+    st = g.stride()
+    L.call("ephip_eph_apply_rotations", fp64, nw, nbkq, nbk, nm, nq, L.ptr(ep), L.ptr(g), st[3], st[2], st[1], st[0],
+           L.ptr(ukqs), L.ptr(u_phs), L.ptr(g2_out) if g2_out is not None else None,
+           L.ptr(wq) if g2_out is not None else None, L.stream_handle())
+    return ep
+
+
+def eph_apply_rotations_rqkq(ep, g, uks, ukqs):
+    """eph_apply_rotations_rqkq! (src/wannier_to_bloch_batched.jl:360-377; CUDA method ext/ElectronPhononCUDAExt.jl:
+    241-258) through ephip_eph_apply_rotations_rqkq: ep (nk, nm, nbk, nbkq), g (nk, nm, nw, nw) (the reference's
+    (nw, nw, nm, nk), iw fastest), uks (nk, nbk, nw), ukqs (nk, nbkq, nw), all contiguous. Launch only."""
+    fp64, _ = _cplx_pair(ep, g, uks, ukqs)
+    nk, nm, nbk, nbkq = ep.shape
+    nw = uks.shape[2]
+    if tuple(g.shape) != (nk, nm, nw, nw) or tuple(uks.shape) != (nk, nbk, nw) or tuple(ukqs.shape) != (nk, nbkq, nw):
+        raise L.EphipError("eph_apply_rotations_rqkq: shapes do not match ep (nk, nm, nbk, nbkq)")
+    if not all(x.is_contiguous() for x in (ep, g, uks, ukqs)):
+        raise L.EphipError("eph_apply_rotations_rqkq: dense operands")
+    L.call("ephip_eph_apply_rotations_rqkq", fp64, nw, nbkq, nbk, nm, nk, L.ptr(ep), L.ptr(g), L.ptr(uks),
+           L.ptr(ukqs), L.stream_handle())
+    return ep
+
+
+def bte_window_accumulate(S_out, S_in, g2vals, wqmat, imap_i_at_k, imap_f, ikqs, e_i, e_f, wf, mus, Ts, etas,
+                          method: int, w_cut: float, i0: int, smearing: int = 1, check: bool = True):
+    """bte_window_accumulate! (src/boltzmann/boltzmann_calculator.jl:336-372; CUDA method
+    ext/ElectronPhononCUDAExt.jl:323-372) through ephip_bte_window_accumulate. Column-major arrays held with
+    the axes reversed: S_out (nT, ni) float64, S_in (nT, nf, ni_tile) float64 (the tile of outer rows
+    i0 + 1 .. i0 + ni_tile), g2vals (nq, nm, nbk, nbkq) and wqmat (nq, nm) float32 or float64, imap_i_at_k
+    (nbk) int64, imap_f (nk, ld) int64, ikqs (nq) int64 (1-based), e_i, e_f, wf float64 per state; mus, Ts,
+    etas (nT) float64; smearing 1 = Gaussian, 2 = Lorentzian. With ``check`` the count of entries that fell
+    outside the outputs is read back (a synchronisation) and a non-zero count raises EphipError."""
+    torch = _torch()
+    if g2vals.dtype not in (torch.float32, torch.float64) or wqmat.dtype != g2vals.dtype:
+        raise L.EphipError("bte_window_accumulate: g2vals and wqmat float32 or float64 of one type")
+    f64 = (S_out, S_in, e_i, e_f, wf, mus, Ts, etas)
+    if not all(x.dtype == torch.float64 and x.is_contiguous() for x in f64):
+        raise L.EphipError("bte_window_accumulate: contiguous float64 S_out, S_in, energies, weights, mus, Ts, etas")
+    if not all(x.dtype == torch.int64 and x.is_contiguous() for x in (imap_i_at_k, imap_f, ikqs)) or imap_f.dim() != 2:
+        raise L.EphipError("bte_window_accumulate: contiguous int64 maps, imap_f 2-d (nk, ld)")
+    nq, nm, nbk, nbkq = g2vals.shape
+    nT = mus.numel()
+    nk_f, ld_f = imap_f.shape
+    if S_out.dim() != 2 or S_in.dim() != 3 or S_out.shape[0] != nT or S_in.shape[0] != nT or \
+            tuple(wqmat.shape) != (nq, nm) or ikqs.numel() != nq or imap_i_at_k.numel() != nbk or ld_f < nbkq:
+        raise L.EphipError("bte_window_accumulate: shapes do not match g2vals (nq, nm, nbk, nbkq)")
+    dropped = torch.zeros(1, dtype=torch.int32, device=g2vals.device) if check else None
+    L.call("ephip_bte_window_accumulate", int(g2vals.dtype == torch.float64), L.ptr(S_out), S_out.shape[1],
+           L.ptr(S_in), S_in.shape[2], S_in.shape[1], L.ptr(g2vals.contiguous()), L.ptr(wqmat.contiguous()),
+           L.ptr(imap_i_at_k), L.ptr(imap_f), ld_f, nk_f, L.ptr(ikqs), L.ptr(e_i), L.ptr(e_f), L.ptr(wf), nT,
+           L.ptr(mus), L.ptr(Ts), L.ptr(etas), int(method), int(smearing), float(w_cut), nbkq, nbk, nm, nq, int(i0),
+           L.ptr(dropped), L.stream_handle())
+    if check:
+        n = int(dropped.item())
+        if n:
+            raise L.EphipError(f"bte_window_accumulate: {n} entries fell outside S_in or the k+q range (i0={i0})")
+
+
 # ------------------------------------------------------------------------------------------------
 # Wannier objects and the phase-sum Fourier transform
 # ------------------------------------------------------------------------------------------------

@@ -355,6 +355,83 @@ class EPDataUnits:
     ikq: object
     g2: object
     nbw: int
+    ep: object = None   # complex64 (units, cs, nbw * nm): the rotated elements behind g2, when requested
+
+
+@dataclasses.dataclass
+class EPDataQBatched:
+    """The reference's batched outer-k payload EPDataQBatched (src/EPData.jl:66-73): one outer k with a
+    batch of k+q. Column-major arrays of the reference held as torch tensors with the axes reversed:
+    eps (nq, nm, nbandk, nw) complex64 = eps[m, n, nu, j] over the full k+q band range m and the k-side
+    window (physical band ibandk_offset + n, 1-based n), zero outside both energy windows; g2s the same
+    shape, float32 (|ep|^2 / (2 omega)); wqs (nq, nm) float64 = omega of each column's q; ik the 1-based
+    position of k in the filtered state list; ikqs (nq) int64, the 1-based positions of the k+q states
+    (k+q with no window band are not listed: their elements are exactly zero)."""
+    eps: object
+    g2s: object
+    wqs: object
+    ik: int
+    ikqs: object
+    ibandk_offset: int
+
+
+class QBatchedAdapter(AbstractCalculator):
+    """Runs calculators written against EPDataQBatched on the outer-k loop's batch payloads: for each k of
+    a batch (its units are consecutive) and tiles of at most nq_batch kept k+q, ephip_units_to_qbatched lays
+    the k's g2 (and eps, when the loop was built with with_ep=True) out as the reference's payload
+    (src/calculator/run_eph_over_k_and_kq.jl:843-876: full band at k+q, the nbandk_max-wide k-side window
+    at offset clamp(first band - 1, 0, nw - nbandk_max)), then calls calc.run(payload) for each calculator."""
+
+    def __init__(self, st, calculators, nq_batch: int = 4096):
+        torch = _torch()
+        self.st = st
+        self.calculators = list(calculators)
+        self.nq_batch = int(nq_batch)
+        dev = st.el.xk.device
+        self.nbandk = int(st.nbw)
+        nk_f = st.el.xk.shape[0]
+        pos = np.zeros(nk_f, dtype=np.int64)
+        pos[st.ksurv.cpu().numpy()] = np.arange(1, st.ksurv.shape[0] + 1)
+        self.pos_host = pos
+        self.state_pos = torch.from_numpy(pos).to(dev)
+        self.bmin_host = st.el.filt.band_min.cpu().numpy()
+        nw, nm, nb = st.nw, st.nm, self.nq_batch
+        self.eps = torch.empty((nb, nm, self.nbandk, nw), dtype=torch.complex64, device=dev)
+        self.g2s = torch.empty((nb, nm, self.nbandk, nw), dtype=torch.float32, device=dev)
+        self.wqs = torch.empty((nb, nm), dtype=torch.float64, device=dev)
+        self.ikqs = torch.empty(nb, dtype=torch.int64, device=dev)
+
+    def run(self, p: EPDataUnits):
+        st = self.st
+        nw, nm = st.nw, st.nm
+        ustate = p.unit_state.cpu().numpy()
+        count = p.count.cpu().numpy()
+        cs = p.g2.shape[1]
+        pos = self.pos_host
+        stream = L.stream_handle()
+        u = 0
+        while u < len(ustate):
+            k = int(ustate[u])
+            ub = u
+            while ub + 1 < len(ustate) and ustate[ub + 1] == k:
+                ub += 1
+            nunits = ub - u + 1
+            band0 = int(self.bmin_host[k]) - 1
+            off = min(max(band0, 0), nw - self.nbandk)
+            c = int(count[u])
+            for j0 in range(0, c, self.nq_batch):
+                nq = min(self.nq_batch, c - j0)
+                L.call("ephip_units_to_qbatched", nw, nm, p.nbw, self.nbandk, cs, L.ptr(p.col_index[u]),
+                       L.ptr(p.ikq[u]), j0, nq, nunits, band0, off,
+                       L.ptr(p.ep[u]) if p.ep is not None else None, L.ptr(p.g2[u]), L.ptr(st.el.filt.band_min),
+                       L.ptr(st.el.filt.band_max), L.ptr(st.ph.omega), L.ptr(self.state_pos),
+                       L.ptr(self.eps) if p.ep is not None else None, L.ptr(self.g2s), L.ptr(self.wqs),
+                       L.ptr(self.ikqs), stream)
+                payload = EPDataQBatched(self.eps[:nq] if p.ep is not None else None, self.g2s[:nq], self.wqs[:nq],
+                                         int(pos[k]), self.ikqs[:nq], off)
+                for calc in self.calculators:
+                    calc.run(payload)
+            u = ub + 1
 
 
 class OuterKLoop:
@@ -370,7 +447,7 @@ class OuterKLoop:
 
     def __init__(self, st: OuterQSetup, nk_batch: int | None = 128, precision: int = L.PREC_FP32,
                  nk_batch_cap: int = 512, fuse_rotation: bool = True, fuse_projection: bool = True,
-                 gridopt: bool = True, gridopt_mfma: bool = True):
+                 gridopt: bool = True, gridopt_mfma: bool = True, with_ep: bool = False):
         torch = _torch()
         self.st = st
         if nk_batch is None:  # memory-adaptive width (plan_batch), the e-ph tensor already resident
@@ -465,6 +542,11 @@ class OuterKLoop:
         if self.gridopt_q is not None:
             self.line_off = torch.empty((max_units, self.gridopt_q.nlines + 1), dtype=torch.int32, device=dev)
         self.g2 = torch.empty((max_units, self.cs, st.nbw * nm), dtype=torch.float32, device=dev)
+        # the complex elements behind g2 (the EPDataQBatched eps), from the staged or the unfused rotation kernels
+        if with_ep and self.fused:
+            raise ValueError("with_ep: the fused split-BF16 kR->kq kernel writes g2 only; use gridopt=True or "
+                             "fuse_rotation=False")
+        self.ep = torch.empty((max_units, self.cs, st.nbw * nm), dtype=torch.complex64, device=dev) if with_ep else None
         if self.packed and self.gridopt_q is None:
             self.apk_units = torch.empty(max_units * core.ft_stages(self.nrp) * nd2 * 64, dtype=torch.bfloat16,
                                          device=dev)
@@ -607,9 +689,10 @@ class OuterKLoop:
             L.call("ephip_eph_rotate_kq_g2", nw, nm, st.nbw, nu, self.cs, L.ptr(self.count),
                    L.ptr(self.col_index), L.ptr(self.ikq), L.ptr(self.Gq), nd2, self.cs * nd2, L.ptr(st.el.U),
                    L.ptr(st.el.filt.band_min), L.ptr(st.el.filt.band_max), L.ptr(self.uph), L.ptr(self.omega),
-                   L.ptr(self.g2), None, stream)
+                   L.ptr(self.g2), L.ptr(self.ep) if self.ep is not None else None, stream)
         payload = EPDataUnits((i0, i0 + n), self.unit_state[u0:u1], self.unit_band[u0:u1], self.count[:nu],
-                              self.col_index[:nu], self.ikq[:nu], self.g2[:nu], st.nbw)
+                              self.col_index[:nu], self.ikq[:nu], self.g2[:nu], st.nbw,
+                              self.ep[:nu] if self.ep is not None else None)
         for c in calculators:
             c.run(payload)
         return payload
@@ -847,7 +930,7 @@ class _GridOptKQ:
                st.nw * st.nm, L.ptr(loop.line_off), L.ptr(loop.col_index), L.ptr(loop.ikq), loop.cs,
                L.ptr(st.el.U), L.ptr(st.el.filt.band_min), L.ptr(st.el.filt.band_max), L.ptr(loop.uph),
                L.ptr(loop.omega), C.addressof(self.dense) if self.dense is not None else None, L.ptr(loop.g2),
-               stream)
+               L.ptr(loop.ep) if loop.ep is not None else None, stream)
 
 
 def kept_q_bound(ksurv, kgrid, qgrid) -> int:

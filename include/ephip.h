@@ -300,7 +300,8 @@ int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp, int32_t n
  * first kept-list position per q line, from ephip_kq_line_offsets (lower bounds in the sorted kept
  * lists of ephip_kq_compact). nw = 16, nm <= 16, nbw <= 4, m2, m3 <= 64, m2*m3 <= 1024. dense_strides: host
  * int32[3] (s1, s2, s3) when the box is dense (n1 = np and every Rp present, Rp index = s1 a + s2 b + s3 c;
- * the operator is then addressed without rmap), or NULL. */
+ * the operator is then addressed without rmap), or NULL. ep: NULL, or complex64 in the layout of g2 (one
+ * complex per g2 entry): the rotated matrix elements themselves (zero past the k+q window). */
 int ephip_kq_line_offsets(int32_t nunits, const int32_t *count, const int32_t *col_index, int64_t cs, int32_t m3,
                           int32_t nlines, int32_t *line_off, void *stream);
 int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int32_t nunits, int32_t n1, int32_t np,
@@ -308,7 +309,7 @@ int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int32_t nunits,
                             const void *A, int64_t lda, int64_t ldr, const int32_t *line_off,
                             const int32_t *col_index, const int32_t *ikq, int64_t cs, const void *U,
                             const int64_t *bmin, const int64_t *bmax, const void *uph, const double *omega,
-                            const int32_t *dense_strides, float *g2, void *stream);
+                            const int32_t *dense_strides, float *g2, void *ep, void *stream);
 
 /* kR->kq Fourier over Rp fused with the k+q-window and phonon rotations and g2 (the
  * get_eph_kR_to_kq_batched! + eph_apply_rotations! + g2 sequence of src/wannier_to_bloch_batched.jl:
@@ -451,6 +452,57 @@ int ephip_eph_window_scatter(int32_t fp64, void *g2_out, void *w_out, int64_t n_
                              const int64_t *imap_i, const int64_t *imap_f, int64_t ld_f, int64_t nk_f,
                              const int64_t *ikqs, const void *w, int32_t nbkq, int32_t nbk, int32_t nm, int64_t nq,
                              int64_t ni_stride, int64_t i0, uint32_t *dropped, void *stream);
+
+/* eph_apply_rotations!(ep, g, ukqs, u_phs, tmp; g2_out, wq) with the reference's arguments
+ * (src/wannier_to_bloch_batched.jl:442-455; CUDA method ext/ElectronPhononCUDAExt.jl:177-201):
+ * ep[m, n, nu', q] = sum_nu (sum_iw conj(ukqs[iw, m, q]) g[iw, n, nu, q]) u_phs[nu, nu', q], and when g2_out is
+ * given g2_out[m, n, nu', q] = |ep|^2 / (2 wq[nu', q]). ep (nbkq, nbk, nm, nq) and g2_out dense column-major;
+ * g (nw, nbk, nm, nq) read through its element strides gs0..gs3 (a strided device view is taken as it is);
+ * ukqs (nw, nbkq, nq), u_phs (nm, nm, nq), wq (nm, nq) dense. Complex128 / float64 when fp64 = 1, else
+ * complex64 / float32. The reference's tmp is not needed. */
+int ephip_eph_apply_rotations(int32_t fp64, int32_t nw, int32_t nbkq, int32_t nbk, int32_t nm, int64_t nq, void *ep,
+                              const void *g, int64_t gs0, int64_t gs1, int64_t gs2, int64_t gs3, const void *ukqs,
+                              const void *uphs, void *g2_out, const void *wq, void *stream);
+
+/* eph_apply_rotations_rqkq!(ep, g, uks, ukqs, tmp, uk_rep) with the reference's arguments
+ * (src/wannier_to_bloch_batched.jl:360-377; CUDA method ext/ElectronPhononCUDAExt.jl:241-258):
+ * ep[m, n, nu, k] = sum_{iw, jw} conj(ukqs[iw, m, k]) g[iw, jw, nu, k] uks[jw, n, k]; ep (nbkq, nbk, nm, nk),
+ * g (nw, nw, nm, nk), uks (nw, nbk, nk), ukqs (nw, nbkq, nk), all dense column-major, complex128 when
+ * fp64 = 1 else complex64. The reference's tmp and uk_rep scratch are not needed. */
+int ephip_eph_apply_rotations_rqkq(int32_t fp64, int32_t nw, int32_t nbkq, int32_t nbk, int32_t nm, int64_t nk,
+                                   void *ep, const void *g, const void *uks, const void *ukqs, void *stream);
+
+/* bte_window_accumulate! with the reference's arguments (src/boltzmann/boltzmann_calculator.jl:336-372; CUDA
+ * method ext/ElectronPhononCUDAExt.jl:323-372): for each (m, n, j) of a q batch, i = imap_i[n] and
+ * f = imap_f[m + ld_f (ikqs[j] - 1)] (int64, 1-based, 0 = outside the window); per temperature iT (nT of them:
+ * mus, Ts, etas) the increments of src/boltzmann/bte_scattering_core.jl:33-84 (method 1..6, smearing 1 =
+ * Gaussian, 2 = Lorentzian) of the modes with wq >= w_cut are summed, S_out[(i - 1) + ld_so iT] += sum_out
+ * (atomic) and S_in[(i - i0 - 1) + ld_si1 ((f - 1) + ld_si2 iT)] = sum_in (the tile of outer rows i0 + 1 ..
+ * i0 + ld_si1). g2vals (nbkq, nbk, nm, nq) and wq (nm, nq) float64 when fp64 = 1 else float32; e_i, e_f, wf
+ * (per-final-state weight), S_out, S_in float64. k+q indices outside 1..nk_f and rows outside the tile are not
+ * written and add 1 to *dropped (uint32, may be NULL). */
+int ephip_bte_window_accumulate(int32_t fp64, double *S_out, int64_t ld_so, double *S_in, int64_t ld_si1,
+                                int64_t ld_si2, const void *g2vals, const void *wq, const int64_t *imap_i,
+                                const int64_t *imap_f, int64_t ld_f, int64_t nk_f, const int64_t *ikqs,
+                                const double *e_i, const double *e_f, const double *wf, int32_t nT, const double *mus,
+                                const double *Ts, const double *etas, int32_t method, int32_t smearing, double w_cut,
+                                int32_t nbandkq, int32_t nbandk, int32_t nm, int64_t nq, int64_t i0, uint32_t *dropped,
+                                void *stream);
+
+/* One k of an outer-k batch payload in the layout of EPDataQBatched (src/EPData.jl:66-73, as the batched
+ * outer-k loop builds it, src/calculator/run_eph_over_k_and_kq.jl:843-876): eps (complex64, may be NULL) and
+ * g2s (float) (nw, nbandk, nm, nq) column-major over the full k+q band range m and the k-side window
+ * n = ibandk_offset + 1 .. ibandk_offset + nbandk, for the kept columns j0 .. j0 + nq - 1 of the k; zero
+ * outside the k and k+q windows. col_index / ikq: the kept q-grid indices and k+q states of the k's first
+ * unit row; the k's nunits units (bands band0 + 1 .. band0 + nunits, 0-based band0) are consecutive rows of
+ * cs columns of ep (complex64, may be NULL) / g2 (the staged kernels' (slot, nu') per column). wqs (nm, nq)
+ * float64 = omega (nm x nq_grid) at each column's q; ikqs (nq) int64 = state_pos[k+q state] (1-based
+ * position in the filtered state list). */
+int ephip_units_to_qbatched(int32_t nw, int32_t nm, int32_t nbw, int32_t nbandk, int64_t cs, const int32_t *col_index,
+                            const int32_t *ikq, int32_t j0, int32_t nq, int32_t nunits, int32_t band0,
+                            int32_t ibandk_offset, const void *ep, const float *g2, const int64_t *bmin,
+                            const int64_t *bmax, const double *omega, const int64_t *state_pos, void *eps, float *g2s,
+                            double *wqs, int64_t *ikqs, void *stream);
 
 /* Full rotated operator of nk k-points: out[k ld_out + a nw^2 + m + nw n] = (U_k^H M_a(k) U_k)[m, n] in FP64
  * for the three directions a, with M (complex128, column k of ld_m >= 3 nw^2 entries, direction slowest)
