@@ -249,6 +249,13 @@ def main():
                     help="run the transport measurement (conductivity, FP32 vs split-BF16) instead of the e-ph bench")
     ap.add_argument("--transport-eta", type=float, default=0.01, help="Gaussian smearing of the transport run (eV)")
     ap.add_argument("--transport-iter", type=int, default=200)
+    ap.add_argument("--surrogate", dest="surrogate", action="store_true", default=None,
+                    help="learned pre-filter leg: train the band-energy network on 24^3 + 20000 random k, prune the "
+                         "fine grid, check zero misses, time the exact and pruned states (default: on for cfg5)")
+    ap.add_argument("--no-surrogate", dest="surrogate", action="store_false")
+    ap.add_argument("--bf16-leg", dest="bf16_leg", action="store_true", default=None,
+                    help="also time the split-BF16 MFMA fast path (default: on for cfg5)")
+    ap.add_argument("--no-bf16-leg", dest="bf16_leg", action="store_false")
     ap.add_argument("--no-check", dest="check", action="store_false", default=True,
                     help="skip the oracle check (full-grid filter and sampled |g|^2 of the last timed batch)")
     args = ap.parse_args()
@@ -417,6 +424,91 @@ def main():
             f"{results['fast']['rrkr_tflops']:.1f} TFLOP/s ({results['fast']['rrkr_avg_ms']:.2f} ms), kR->kq "
             f"{results['fast']['krkq_tflops']:.1f} TFLOP/s ({results['fast']['krkq_avg_ms']:.2f} ms)")
 
+    # ---- split-BF16 MFMA fast path (packed operands, fused projection / rotations), a few steps
+    if args.bf16_leg is None:
+        args.bf16_leg = args.config == "cfg5"
+    if args.bf16_leg and args.path in ("fast", "both"):
+        del loop
+        torch.cuda.empty_cache()
+        lb = eph.OuterKLoop(st, nk_batch=384, precision=L.PREC_BF16X3, gridopt=False)
+        if args.tune:
+            lb.tune()
+        nb_steps = max(2, args.steps // 2)
+        for i0 in (0, 384):
+            lb.run_kbatch(i0, 384)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s_ in range(nb_steps):
+            lb.run_kbatch(((2 + s_) * 384) % max(1, nks - 384), 384)
+        torch.cuda.synchronize()
+        el_b = time.perf_counter() - t0
+        results["fast_bf16"] = dict(value=384 * nq_grid * nb_steps / el_b, ms_per_step=1e3 * el_b / nb_steps,
+                                    steps=nb_steps, k_batch=384,
+                                    kernels="ft_bf16p8_proj_kernel (RR->kR + projection), ft_bf16p8_rot_kernel "
+                                            "(kR->kq + rotations + g2), split-BF16 on MFMA")
+        log(f"[bench] fast outer-k split-BF16 MFMA: {results['fast_bf16']['value']:.3e} pairs/s")
+        del lb
+        torch.cuda.empty_cache()
+    # ---- learned pre-filter: the band-energy network ahead of the exact states of the fine grid
+    if args.surrogate is None:
+        args.surrogate = args.config == "cfg5"
+    if args.surrogate:
+        from ephip import surrogate as S
+        ham = core.WannierObject(m.irvec_h, m.ham_r).to_device(backend, dtype=torch.complex128)
+        rng = np.random.default_rng(5)
+        xtr = torch.from_numpy(np.vstack([M.kpoints_grid((24, 24, 24)), rng.random((20000, 3))])).to(
+            backend.torch_device)
+        t0 = time.perf_counter()
+        st_tr = eph.compute_electron_states(ham, xtr, None)
+        sur = S.BandSurrogate(m.nw, backend.torch_device, nf2=32, hidden=128, seed=1)
+        sur.train(xtr, st_tr.E, steps=3000)
+        e32, e16 = sur.calibrate(xtr, st_tr.E, window=(-w, w))
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        xf = st.el.xk
+        t0 = time.perf_counter()
+        ex = eph.compute_electron_states(ham, xf, (-w, w))
+        torch.cuda.synchronize()
+        t_exact = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        pr, cand = eph.compute_electron_states_pruned(ham, xf, (-w, w), sur)
+        torch.cuda.synchronize()
+        t_pruned = time.perf_counter() - t0
+        ke, kp = ex.filt.keep.bool(), pr.filt.keep.bool()
+        misses = int((ke & ~kp).sum())
+        same = bool(torch.equal(ke, kp) and torch.equal(ex.filt.band_min, pr.filt.band_min)
+                    and torch.equal(ex.filt.band_max, pr.filt.band_max))
+        results["surrogate"] = dict(
+            train_seconds=t_train, train_points=int(xtr.shape[0]), max_err_fp32_eV=e32, max_err_bf16_eV=e16,
+            window_distance_err_eV=sur.margin_error, margin_eV=sur.margin, candidates=int(cand.numel()),
+            grid_points=int(xf.shape[0]), exact_survivors=int(ke.sum()), misses=misses, selection_identical=same,
+            states_exact_ms=1e3 * t_exact, states_pruned_ms=1e3 * t_pruned)
+        log(f"[bench] surrogate: margin {sur.margin:.3f} eV, candidates {cand.numel()} of {xf.shape[0]}, misses "
+            f"{misses}; states exact {1e3 * t_exact:.0f} ms, pruned {1e3 * t_pruned:.0f} ms")
+        del ex, pr, ham, st_tr
+        torch.cuda.empty_cache()
+    # ---- whole-job rates (states of the fine grid + every k batch over the full q grid)
+    job = None
+    if "fast" in results and "plain" in results:
+        job_pairs = nks * nq_total
+        t_states = results.get("surrogate", {}).get("states_exact_ms", None)
+        t_states = (t_states or 0.0) * 1e-3
+        n_batches = -(-nks // args.kbatch_k)
+        job = {"pairs": job_pairs, "states_seconds": t_states,
+               "fast_fp32_staged": job_pairs / (t_states + n_batches * results["fast"]["ms_per_step"] * 1e-3),
+               "plain_fp32": job_pairs / (t_states + job_pairs / results["plain"]["value"])}
+        if "fast_bf16" in results:
+            t_st = t_states
+            if "surrogate" in results and results["surrogate"]["misses"] == 0:
+                t_st = results["surrogate"]["states_pruned_ms"] * 1e-3
+            job["fast_bf16_batched_surrogate"] = job_pairs / (t_st + -(-nks // 384) *
+                                                              results["fast_bf16"]["ms_per_step"] * 1e-3)
+            job["ratio_bf16_batched_surrogate_vs_plain"] = job["fast_bf16_batched_surrogate"] / job["plain_fp32"]
+        job["ratio_fast_vs_plain"] = job["fast_fp32_staged"] / job["plain_fp32"]
+        log(f"[bench] whole job: fast {job['fast_fp32_staged']:.3e}, plain {job['plain_fp32']:.3e} pairs/s"
+            + (f", split-BF16 + batched + surrogate {job['fast_bf16_batched_surrogate']:.3e}"
+               if "fast_bf16_batched_surrogate" in job else ""))
+
     cpu = None
     if args.cpu_sample != 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -507,6 +599,7 @@ def main():
                      "frac_unpacked_vector_peak": (dom[1] / 78.6 if valu_dom else None)},
         "cpu_baseline": cpu,
         "check": check,
+        "job": job,
         "detail": results,
     }
     print(json.dumps(out))

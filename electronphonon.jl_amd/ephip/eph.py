@@ -784,7 +784,7 @@ class _GridOptKR:
         lo, hi = r.min(0), r.max(0)
         n = hi - lo + 1
         kg = np.asarray(st.kgrid, dtype=np.int64)
-        if (len(ks) == 0 or n[0] > cls.N1_MAX or n[1] > cls.N23_MAX or n[2] > cls.N23_MAX or kg[2] > cls.KLINE_MAX
+        if (len(ks) == 0 or n[0] > cls.N1_MAX or n[1] > cls.N23_MAX or n[2] > cls.N23_MAX
                 or st.nbw > cls.NBW_MAX):
             return None
         return cls(st, ks, nd, r, lo, n, kg, mfma)
@@ -806,6 +806,11 @@ class _GridOptKR:
         i1, i2 = ks // (kg[1] * kg[2]), (ks // kg[2]) % kg[1]
         change = np.flatnonzero((np.diff(i1) != 0) | (np.diff(i2) != 0)) + 1
         starts = np.concatenate([[0], change, [len(ks)]]).astype(np.int64)
+        # a (k1, k2) line with more than KLINE_MAX surviving k (k3 grids above 64) is cut into pieces of at
+        # most KLINE_MAX k; each piece forms its own stage-2 sums (the same k2 phases)
+        if np.diff(starts).max() > self.KLINE_MAX:
+            pieces = [np.arange(a, b, self.KLINE_MAX) for a, b in zip(starts[:-1], starts[1:])]
+            starts = np.concatenate(pieces + [[len(ks)]]).astype(np.int64)
         self.line_start_host = starts
         self.line_start = torch.from_numpy(starts.astype(np.int32)).to(dev)
         self.line_plane = i1[starts[:-1]]

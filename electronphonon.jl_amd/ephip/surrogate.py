@@ -213,11 +213,27 @@ class BandSurrogate:
                L.ptr(y), L.ptr(keep), L.stream_handle())
         return keep, y
 
-    def calibrate(self, xk, E, factor: float = 2.0) -> tuple[float, float]:
-        """Max |error| of the FP32 and BF16 predictions over (xk, E); margin = factor x the BF16 error."""
+    @staticmethod
+    def window_distance(E, window):
+        """Distance (n,) of the band nearest to the energy window [wmin, wmax] (0 inside it)."""
+        lo, hi = float(window[0]), float(window[1])
+        return ((lo - E).clamp(min=0.0) + (E - hi).clamp(min=0.0)).min(dim=1).values
+
+    def calibrate(self, xk, E, factor: float = 2.0, window=None) -> tuple[float, float]:
+        """Max |error| of the FP32 and BF16 predictions over (xk, E); margin = factor x the BF16 error.
+        With ``window``, the error that sets the margin is the one the prune decision uses: of the distance
+        of the nearest predicted band to the window (a point is pruned when that distance exceeds the
+        margin, so a point with an exact band in the window is kept whenever the BF16 error of the
+        distance is below the margin); bands far from the window no longer widen it."""
         a = self.forward(xk)
         e32 = float((a["y"][:, :self.nw].double() - E).abs().max())
         _, y = self.infer_prune(xk, (0.0, 0.0), 0.0, want_y=True)
-        e16 = float((y[:, :self.nw].double() - E).abs().max())
-        self.margin = factor * e16
+        yb = y[:, :self.nw].double()
+        e16 = float((yb - E).abs().max())
+        if window is None:
+            self.margin = factor * e16
+        else:
+            ed = float((self.window_distance(yb, window) - self.window_distance(E, window)).abs().max())
+            self.margin = factor * ed
+            self.margin_error = ed
         return e32, e16

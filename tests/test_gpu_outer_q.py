@@ -415,6 +415,37 @@ def test_outer_k_gridopt_kq_long_lines(backend, model16, window, kq_variant):
     assert rel < 1e-4, rel
 
 
+@pytest.mark.parametrize("mfma", [True, False])
+def test_outer_k_gridopt_long_k_lines(backend, model16, mfma):
+    """Staged RR->kR on a k grid whose (k1, k2) lines hold more surviving k than the projection kernels take
+    per line (64; k grid (2, 2, 100), the shape of k3 grids above 64 such as 96^3): the lines are cut into
+    pieces of at most 64 k; unit operators agree with the direct FP32 kernels to 1e-5 and g2 with the
+    oracle to 1e-4."""
+    from ephip import _lib as L
+    from ephip import eph
+    m, g = model16
+    g = dict(g, kgrid=(2, 2, 100), qgrid=(2, 2, 4), window=0.5)
+    w = g["window"]
+    st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
+    ks = st.ksurv.cpu().numpy()
+    per_line = np.bincount(ks // 100)
+    assert per_line.max() > 64, per_line            # some (k1, k2) line outgrows one piece
+    nks = len(ks)
+    nb = 150
+    staged = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32, gridopt_mfma=mfma)
+    direct = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32, gridopt=False)
+    assert staged.gridopt is not None and staged.gridopt.max_line_k <= 64
+    rs = eph.UnitRecorder()
+    for i0 in (0, max(0, nks - nb)):
+        staged.run_kbatch(i0, nb, [rs])
+        direct.run_kbatch(i0, nb)
+        nu = int(staged.k_unit_start[min(i0 + nb, nks)] - staged.k_unit_start[i0])
+        a, b = staged.A[:nu].cpu().numpy(), direct.A[:nu].cpu().numpy()
+        assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b), (i0, np.linalg.norm(a - b) / np.linalg.norm(b))
+    rel = _outer_k_vs_oracle(m, g, st, rs, qstep=3)
+    assert rel < 1e-4, rel
+
+
 @pytest.mark.parametrize("gridopt", [True, False])
 def test_outer_k_pipelined(backend, model16, gridopt):
     """Batches run on two streams (RR->kR of the next batch beside kR->kq of the current one, double-

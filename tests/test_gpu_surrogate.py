@@ -92,3 +92,50 @@ def test_surrogate_train_bf16_and_zero_miss_prune(backend):
     assert np.array_equal(pruned.filt.band_min.cpu().numpy(), exact.filt.band_min.cpu().numpy())
     assert np.array_equal(pruned.filt.band_max.cpu().numpy(), exact.filt.band_max.cpu().numpy())
     assert cand.numel() < xf.shape[0]  # the network removes exact solves
+
+
+def test_surrogate_cfg5_model_zero_miss_prune(backend):
+    """The learned form of the workload as specified: cfg4's model (on-site energies in [-3, 3] eV, hopping
+    decay 1.0, 16 bands) trained on the exact eigenvalues of a 24^3 grid + 20000 random k (3000 Adam steps),
+    margin = 2 x the BF16 error of the decision quantity (distance of the nearest band to the +-0.3 eV
+    window), then the 96^3 fine grid (884 736 k) pruned and checked bit for bit against the exact filter
+    (keep flags and band ranges): zero misses. Prints what the pruning saves; on this model the network's
+    error (about 1 eV) is far above the typical distance of the nearest band to the window, so it keeps
+    nearly every point, which the printed figures record."""
+    import time
+    from ephip import core, eph
+    from ephip import surrogate as S
+    m, g = M.make_config_model("cfg5", with_g=False)
+    assert g["kgrid"] == (96, 96, 96) and m.nw == 16
+    ham = core.WannierObject(m.irvec_h, m.ham_r).to_device(backend, dtype=torch.complex128)
+    rng = np.random.default_rng(5)
+    xtr = torch.from_numpy(np.vstack([M.kpoints_grid((24, 24, 24)), rng.random((20000, 3))])).cuda()
+    st = eph.compute_electron_states(ham, xtr, None)
+    s = S.BandSurrogate(m.nw, backend.torch_device, nf2=32, hidden=128, seed=1)
+    losses = s.train(xtr, st.E, steps=3000)
+    assert losses[-1][1] < 0.1 * losses[0][1]
+    w = g["window"]
+    e32, e16 = s.calibrate(xtr, st.E, window=(-w, w))
+    xf = torch.from_numpy(M.kpoints_grid(g["kgrid"])).cuda()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    exact = eph.compute_electron_states(ham, xf, (-w, w))
+    torch.cuda.synchronize()
+    t_exact = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pruned, cand = eph.compute_electron_states_pruned(ham, xf, (-w, w), s)
+    torch.cuda.synchronize()
+    t_pruned = time.perf_counter() - t0
+    ke = exact.filt.keep.cpu().numpy().astype(bool)
+    kp = pruned.filt.keep.cpu().numpy().astype(bool)
+    d = S.BandSurrogate.window_distance(exact.E, (-w, w)).cpu().numpy()
+    print(f"cfg5 surrogate: max err bf16 {e16:.3f} eV, distance err {s.margin_error:.3f} eV, margin {s.margin:.3f} eV;"
+          f" candidates {cand.numel()} of {xf.shape[0]}, exact survivors {ke.sum()}, nearest-band distance median "
+          f"{np.median(d):.3f} eV, 90% {np.quantile(d, 0.9):.3f} eV; states exact {1e3 * t_exact:.0f} ms, pruned "
+          f"{1e3 * t_pruned:.0f} ms")
+    assert int((ke & ~kp).sum()) == 0
+    assert np.array_equal(kp, ke)
+    assert np.array_equal(pruned.filt.band_min.cpu().numpy(), exact.filt.band_min.cpu().numpy())
+    assert np.array_equal(pruned.filt.band_max.cpu().numpy(), exact.filt.band_max.cpu().numpy())
+    # every point the network keeps is either an exact survivor or within the margin of the window
+    assert cand.numel() >= ke.sum()
