@@ -354,6 +354,96 @@ class BoltzmannCalculator:
         return y
 
 
+class BoltzmannQBatchedCalculator:
+    """The reference's device BoltzmannCalculator on EPDataQBatched payloads
+    (src/boltzmann/boltzmann_calculator.jl:213-386 with the CUDA method of bte_window_accumulate!,
+    ext/ElectronPhononCUDAExt.jl:323-372): S_out (nT x n) accumulates on the device; S_in is a
+    TiledDeviceOutput (n_i, n_f, nT) tiled over the outer states: begin_batch zeroes the batch's tile
+    (calculator_begin!), every k payload accumulates into it through ephip_bte_window_accumulate with
+    tile-local rows, end_batch streams the tile into the host dense S_in (calculator_end!) in block
+    mode. The residency is decided on the first batch from the free device memory (force_block /
+    free_bytes override it, as TiledDeviceOutput's); in full mode the whole S_in stays on the device and
+    is downloaded once by postprocess. Feed it through eph.QBatchedAdapter. Host dense S_in: n^2 nT
+    float64 (small grids, as in the reference)."""
+
+    def __init__(self, params: ElectronTransportParams, smearing=("Gaussian", 0.1), occupation_method: int = 5,
+                 omega_cutoff: float = core.OMEGA_ACOUSTIC_EV, force_block=None, free_bytes=None,
+                 headroom: float = 1.2):
+        kind, eta = smearing
+        if kind not in SMEARING_KIND or not 1 <= int(occupation_method) <= 6:
+            raise ValueError("smearing kind must be Gaussian or Lorentzian and occupation_method in 1:6")
+        self.params = params
+        self.kind = SMEARING_KIND[kind]
+        self.etas = np.broadcast_to(np.asarray(eta, dtype=np.float64), params.Tlist.shape).copy()
+        self.method = int(occupation_method)
+        self.omega_cutoff = float(omega_cutoff)
+        self.force_block, self.free_bytes, self.headroom = force_block, free_bytes, headroom
+
+    def setup(self, loop):
+        from .tiled import TiledDeviceOutput
+        torch = _torch()
+        st = loop.st
+        self.loop = loop
+        self.bs = bs = band_states(st.el)
+        if np.any(np.isnan(self.params.mulist)):
+            bte_compute_mu(self.params, bs)
+        dev = st.el.E.device
+        self.nT = nT = len(self.params.Tlist)
+        n = bs.n
+        self.S_out = torch.zeros((nT, n), dtype=torch.float64, device=dev)
+        self.mus = torch.as_tensor(self.params.mulist, device=dev)
+        self.Ts = torch.as_tensor(self.params.Tlist, device=dev)
+        self.etas_d = torch.as_tensor(self.etas, device=dev)
+        nq = st.ph.xq.shape[0]
+        self.wf = torch.full((n,), 1.0 / nq, dtype=torch.float64, device=dev)   # per-final-state weight
+        # state map of the filtered k list: imap[p, band] = 1-based state of band (0-based) at surviving k p
+        ks = st.ksurv.cpu().numpy()
+        off = bs.state_offset.cpu().numpy()
+        bmin = st.el.filt.band_min.cpu().numpy()
+        bmax = st.el.filt.band_max.cpu().numpy()
+        imap = np.zeros((len(ks), st.nw), dtype=np.int64)   # (nks, nw) = column-major (nw, nks)
+        for p_, k in enumerate(ks):
+            nb = int(bmax[k] - bmin[k] + 1)
+            imap[p_, bmin[k] - 1:bmax[k]] = off[k] + 1 + np.arange(nb)
+        self.imap = torch.from_numpy(imap).to(dev)
+        self.tiled = TiledDeviceOutput((n, n, nT), 0, (off, ks), dtype=torch.float64, device=dev,
+                                       force_block=self.force_block, headroom=self.headroom,
+                                       free_bytes=self.free_bytes)
+        self.S_in = np.zeros((nT, n, n))                    # host dense, [iT, f, i] = reference Si[iT][i, f]
+        return self
+
+    def begin_batch(self, k_range, n_batch_max):
+        self.tiled.tile_begin(int(k_range[0]), int(k_range[1]) - 1, int(n_batch_max))
+
+    def run(self, p):
+        from . import core as C_
+        t = self.tiled
+        nbk = p.g2s.shape[2]
+        imap_i = self.imap[p.ik - 1, p.ibandk_offset:p.ibandk_offset + nbk].contiguous()
+        # FP64 increments from the payload's g2 and its FP64 frequencies (the entry takes both in one type)
+        g2 = p.g2s.to(self.S_out.dtype)
+        C_.bte_window_accumulate(self.S_out, t.device_array(0), g2, p.wqs.contiguous(), imap_i, self.imap, p.ikqs,
+                                 self.bs.es, self.bs.es, self.wf, self.mus, self.Ts, self.etas_d, self.method,
+                                 self.omega_cutoff, t.tile_offset(), smearing=self.kind, check=False)
+
+    def end_batch(self):
+        t = self.tiled
+        if t.is_block() and t.tile_length() > 0:
+            t.tile_download()
+            i0, ni = t.tile_offset(), t.tile_length()
+            self.S_in[:, :, i0:i0 + ni] = t.host_array(0)[:, :, :ni].numpy()
+
+    def postprocess(self):
+        t = self.tiled
+        if t.is_allocated() and not t.is_block():
+            self.S_in[:] = t.device_array(0).cpu().numpy()
+        t.tile_free()
+
+    def scattering_in_dense(self, iT: int = 0) -> np.ndarray:
+        """Host dense S_in (n x n) [i, f] for one temperature."""
+        return self.S_in[iT].T.copy()
+
+
 def run_boltzmann(loop, calc: BoltzmannCalculator, k_start: int = 0, k_stop: int | None = None):
     """setup -> every outer-k batch -> postprocess (the run_eph_over_k_and_kq calculator lifecycle)."""
     calc.setup(loop)

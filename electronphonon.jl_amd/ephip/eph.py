@@ -382,11 +382,12 @@ class QBatchedAdapter(AbstractCalculator):
     (src/calculator/run_eph_over_k_and_kq.jl:843-876: full band at k+q, the nbandk_max-wide k-side window
     at offset clamp(first band - 1, 0, nw - nbandk_max)), then calls calc.run(payload) for each calculator."""
 
-    def __init__(self, st, calculators, nq_batch: int = 4096):
+    def __init__(self, st, calculators, nq_batch: int = 4096, n_batch_max: int | None = None):
         torch = _torch()
         self.st = st
         self.calculators = list(calculators)
         self.nq_batch = int(nq_batch)
+        self.n_batch_max = n_batch_max   # the loop's k batch (sizes calculators' outer-k tiles)
         dev = st.el.xk.device
         self.nbandk = int(st.nbw)
         nk_f = st.el.xk.shape[0]
@@ -409,6 +410,10 @@ class QBatchedAdapter(AbstractCalculator):
         cs = p.g2.shape[1]
         pos = self.pos_host
         stream = L.stream_handle()
+        nbm = self.n_batch_max or (p.k_range[1] - p.k_range[0])
+        for calc in self.calculators:   # calculator_begin!(calc, OuterIterationBatch(), ctx)
+            if hasattr(calc, "begin_batch"):
+                calc.begin_batch(p.k_range, nbm)
         u = 0
         while u < len(ustate):
             k = int(ustate[u])
@@ -432,6 +437,9 @@ class QBatchedAdapter(AbstractCalculator):
                 for calc in self.calculators:
                     calc.run(payload)
             u = ub + 1
+        for calc in self.calculators:   # calculator_end!(calc, OuterIterationBatch(), ctx)
+            if hasattr(calc, "end_batch"):
+                calc.end_batch()
 
 
 class OuterKLoop:

@@ -263,3 +263,34 @@ def test_velocity_matrices_berry_connection(backend):
                 den = max(den, float(np.abs(ref).max()))
         print(f"nw {nw}: Berry-connection velocity vs tau-gauge derivative, max rel {num / den:.2e}")
         assert num / den <= 1e-6, num / den
+
+
+@pytest.mark.parametrize("mode", ["block", "full", "auto-small-memory"])
+def test_qbatched_boltzmann_tiled_output(transport, mode):
+    """The reference's device BoltzmannCalculator shape: EPDataQBatched payloads (eph.QBatchedAdapter) into
+    ephip_bte_window_accumulate with S_in held in a TiledDeviceOutput (block mode: one outer-k tile per
+    batch streamed to the host; full mode: the whole matrix on the device; auto with a stub free-bytes
+    value below the full size: block). S_out and the dense S_in agree with the compressed-row calculator
+    (pinned against the oracle above) to 1e-12 and with the oracle to 1e-5."""
+    from ephip import _lib as L
+    from ephip import bte, eph
+    t = transport
+    st = t["st"]
+    loop = eph.OuterKLoop(st, nk_batch=64, precision=L.PREC_FP32)
+    params = bte.ElectronTransportParams(Tlist=T_LIST, nlist=[0.1, 0.1], volume=1.0, mulist=MU_LIST)
+    kw = {"block": dict(force_block=True), "full": dict(force_block=False),
+          "auto-small-memory": dict(free_bytes=lambda: 1000)}[mode]
+    qc = bte.BoltzmannQBatchedCalculator(params, smearing=("Gaussian", ETA), occupation_method=METHOD, **kw)
+    qc.setup(loop)
+    adapter = eph.QBatchedAdapter(st, [qc], nq_batch=100, n_batch_max=64)
+    loop.run(calculators=[adapter])
+    qc.postprocess()
+    assert qc.tiled.ni_cap == 0  # freed
+    calc = t["calc"]
+    so_q, so_c = qc.S_out.cpu().numpy(), calc.S_out.cpu().numpy()
+    assert np.abs(so_q - so_c).max() <= 1e-12 * np.abs(so_c).max()
+    for iT in range(len(T_LIST)):
+        a, b = qc.scattering_in_dense(iT), calc.scattering_in_dense(iT)
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+        assert _rel(a, t["Si"][iT]) < 1e-5
+    assert _rel(so_q, t["So"]) < 1e-5

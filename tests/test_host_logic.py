@@ -58,3 +58,54 @@ def test_kept_q_bound_brute_force(kgrid, qgrid):
         h = (j[:, 0] * n[1] + j[:, 1]) * n[2] + j[:, 2]
         best = max(best, int(keep[h].sum()))
     assert eph.kept_q_bound(surv, kgrid, qgrid) == best
+
+
+def test_tiled_device_output_residency_and_tiles():
+    """TiledDeviceOutput (src/calculator/tiled_device_output.jl:32-151) on the host, with a stub free-bytes
+    value as the reference tests its residency logic (test/test_gpu.jl:994-1062): block mode when
+    headroom x full bytes exceed the free bytes (and full mode otherwise), the tile capacity equal to the
+    largest k batch's state count (brute force), every batch's offset / length, the zeroing of the
+    active tile region, the download, and force_block overriding the decision."""
+    import warnings
+
+    import numpy as np
+    import torch
+    from ephip import tiled
+    assert tiled.residency_use_block(1000, 900, 1.2) and not tiled.residency_use_block(1100, 900, 1.2)
+    rng = np.random.default_rng(7)
+    nk_grid = 60
+    nb = rng.integers(0, 4, nk_grid)                       # states per grid k (0 = no window band)
+    off = np.concatenate([[0], np.cumsum(nb)])
+    ks = np.flatnonzero(nb > 0)
+    n, nT, bmax = int(off[-1]), 2, 7
+    dims = (n, n, nT)
+    full = 8 * n * n * nT
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        t = tiled.TiledDeviceOutput(dims, 0, (off, ks), dtype=torch.float64, device="cpu",
+                                    free_bytes=lambda: full, headroom=1.2)
+        t.tile_begin(0, bmax - 1, bmax)
+        assert t.is_block() and any("block residency" in str(x.message) for x in w)
+    cap = max(int(off[ks[min(a + bmax, len(ks)) - 1] + 1] - off[ks[a]]) for a in range(0, len(ks), bmax))
+    assert t.ni_cap == cap and t.tile_stride() == cap
+    assert tuple(t.device_array(0).shape) == (nT, n, cap)
+    for a in range(0, len(ks), bmax):
+        b = min(a + bmax, len(ks)) - 1
+        t.device_array(0).fill_(7.0)
+        t.tile_begin(a, b, bmax)
+        i0, ni = int(off[ks[a]]), int(off[ks[b] + 1] - off[ks[a]])
+        assert (t.tile_offset(), t.tile_length()) == (i0, ni)
+        d = t.device_array(0)
+        assert float(d[:, :, :ni].abs().max()) == 0.0 and (ni == cap or float(d[:, :, ni:].min()) == 7.0)
+        d[:, :, :ni] = float(a + 1)
+        t.tile_download()
+        assert float(t.host_array(0)[:, :, :ni].min()) == float(a + 1)
+    t.tile_free()
+    assert not t.is_allocated()
+    t2 = tiled.TiledDeviceOutput(dims, 0, (off, ks), device="cpu", free_bytes=lambda: 2 * full)
+    t2.tile_begin(0, bmax - 1, bmax)
+    assert not t2.is_block() and t2.tile_offset() == 0 and tuple(t2.device_array(0).shape) == (nT, n, n)
+    assert float(t2.device_array(0).abs().max()) == 0.0
+    t3 = tiled.TiledDeviceOutput(dims, 0, (off, ks), device="cpu", force_block=True, free_bytes=lambda: 10 * full)
+    t3.tile_begin(0, bmax - 1, bmax)
+    assert t3.is_block()
