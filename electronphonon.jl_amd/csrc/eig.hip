@@ -2,7 +2,9 @@
 //
 // Replaces the reference's eigen_batched / eigvals_batched (src/wannier_to_bloch_batched.jl:30-65,
 // CUDA method ext/ElectronPhononCUDAExt.jl:80-115): ascending eigenvalues, orthonormal eigenvectors,
-// no degeneracy gauge fixing (the reference's batched path applies none either).
+// no degeneracy gauge fixing (the reference's batched path applies none either). The per-k path's
+// degenerate-band gauge fixing (solve_eigen_el!, src/diagonalize.jl:47-85) is a separate kernel at the end
+// of this file, applied to the eigenpairs afterwards.
 //
 // Method (FP32, all in LDS + registers of one wave):
 //   1. Householder reduction of the Hermitian matrix to tridiagonal form, one reflector per column;
@@ -309,6 +311,96 @@ __device__ void lds_gemm_cn(int n, const c64 *A, const c64 *B, c64 *out, int lan
     }
 }
 
+// Parallel cyclic Jacobi on the n x n Hermitian M (LDS, leading dimension n, n <= 32): the n/2 disjoint
+// rotations of a round-robin round are applied in parallel (columns: lane = row; rows: lane = column) and
+// accumulated into the n columns of Q (qrows x n, leading dimension qrows, qrows <= 64). Stops when the
+// strict upper triangle's Frobenius norm (the part the rotations remove; rounding leaves M Hermitian only
+// to a few ulps, and unitary rotations cannot remove that anti-Hermitian residue) is <= tol/sqrt(2) x the
+// Frobenius norm, or after 12 sweeps. A pivot below 1e-20 of its diagonal scale is left alone: rotating it
+// would only shrink it further, towards the subnormal range where its phase can no longer be normalised
+// and the rotation stops being unitary. rot: 16 entries of LDS.
+__device__ void wave_jacobi(int n, c64 *M, c64 *Q, int qrows, JacobiRot *rot, int lane, double tol) {
+    const int nn = n * n;
+    const int N = n + (n & 1);
+    const int npairs = N / 2;
+    for (int sweep = 0; sweep < 12; sweep++) {
+        double off = 0.0, tot = 0.0;
+        for (int e = lane; e < nn; e += 64) {
+            const double a2 = M[e].re * M[e].re + M[e].im * M[e].im;
+            tot += a2;
+            if (e % n < e / n) off += 2.0 * a2;
+        }
+        off = wave_sum(off);
+        tot = wave_sum(tot);
+        if (off <= tol * tol * tot) break;
+        for (int rd = 0; rd < N - 1; rd++) {
+            if (lane < npairs) {
+                int a, c;
+                if (lane == 0) {
+                    a = N - 1;
+                    c = rd;
+                } else {
+                    a = (rd + lane) % (N - 1);
+                    c = (rd - lane + (N - 1)) % (N - 1);
+                }
+                JacobiRot g;
+                g.p = min(a, c);
+                g.q = max(a, c);
+                g.gpp = c64{1.0, 0.0};
+                g.gpq = c64{0.0, 0.0};
+                g.gqp = c64{0.0, 0.0};
+                g.gqq = c64{1.0, 0.0};
+                if (g.q < n) {
+                    const c64 mpq = M[g.p + n * g.q];
+                    const double amp = hypot(mpq.re, mpq.im);
+                    const double scale = fabs(M[g.p + n * g.p].re) + fabs(M[g.q + n * g.q].re);
+                    if (amp > 1e-20 * scale && amp > 1e-280) {
+                        const double er = mpq.re / amp, ei = mpq.im / amp;
+                        const double tau = (M[g.q + n * g.q].re - M[g.p + n * g.p].re) / (2.0 * amp);
+                        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        const double cs = 1.0 / sqrt(1.0 + t * t), sn = t * cs;
+                        g.gpp = c64{cs, 0.0};
+                        g.gpq = c64{sn, 0.0};
+                        g.gqp = c64{-sn * er, sn * ei};
+                        g.gqq = c64{cs * er, -cs * ei};
+                    }
+                } else {
+                    g.q = g.p;  // dummy partner of an odd n: identity on a single index
+                }
+                rot[lane] = g;
+            }
+            wave_lds_sync();
+            if (lane < n || lane < qrows) {  // M <- M G, Q <- Q G (lane = row)
+                for (int k = 0; k < npairs; k++) {
+                    const JacobiRot g = rot[k];
+                    if (g.p == g.q) continue;
+                    if (lane < n) {
+                        const c64 mp = M[lane + n * g.p], mq = M[lane + n * g.q];
+                        M[lane + n * g.p] = cadd(cmul(mp, g.gpp), cmul(mq, g.gqp));
+                        M[lane + n * g.q] = cadd(cmul(mp, g.gpq), cmul(mq, g.gqq));
+                    }
+                    if (lane < qrows) {
+                        const c64 qp = Q[lane + qrows * g.p], qq = Q[lane + qrows * g.q];
+                        Q[lane + qrows * g.p] = cadd(cmul(qp, g.gpp), cmul(qq, g.gqp));
+                        Q[lane + qrows * g.q] = cadd(cmul(qp, g.gpq), cmul(qq, g.gqq));
+                    }
+                }
+            }
+            wave_lds_sync();
+            if (lane < n) {  // M <- G^H M (lane = column)
+                for (int k = 0; k < npairs; k++) {
+                    const JacobiRot g = rot[k];
+                    if (g.p == g.q) continue;
+                    const c64 mp = M[g.p + n * lane], mq = M[g.q + n * lane];
+                    M[g.p + n * lane] = cadd(cmulc(g.gpp, mp), cmulc(g.gqp, mq));
+                    M[g.q + n * lane] = cadd(cmulc(g.gpq, mp), cmulc(g.gqq, mq));
+                }
+            }
+            wave_lds_sync();
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void heev_refine_kernel(int n, int64_t nb, const c64 *__restrict__ H,
                                                           c32 *__restrict__ U, double *__restrict__ E64,
                                                           double tol) {
@@ -348,79 +440,7 @@ __global__ __launch_bounds__(256) void heev_refine_kernel(int n, int64_t nb, con
     lds_gemm_cn(n, Q, W, M, lane);
     wave_lds_sync();
     // 3. parallel cyclic Jacobi
-    const int N = n + (n & 1);
-    const int npairs = N / 2;
-    for (int sweep = 0; sweep < 12; sweep++) {
-        double off = 0.0, tot = 0.0;
-        for (int e = lane; e < nn; e += 64) {
-            const double a2 = M[e].re * M[e].re + M[e].im * M[e].im;
-            tot += a2;
-            if (e % n != e / n) off += a2;
-        }
-        off = wave_sum(off);
-        tot = wave_sum(tot);
-        if (off <= tol * tol * tot) break;
-        for (int rd = 0; rd < N - 1; rd++) {
-            if (lane < npairs) {
-                int a, c;
-                if (lane == 0) {
-                    a = N - 1;
-                    c = rd;
-                } else {
-                    a = (rd + lane) % (N - 1);
-                    c = (rd - lane + (N - 1)) % (N - 1);
-                }
-                JacobiRot g;
-                g.p = min(a, c);
-                g.q = max(a, c);
-                g.gpp = c64{1.0, 0.0};
-                g.gpq = c64{0.0, 0.0};
-                g.gqp = c64{0.0, 0.0};
-                g.gqq = c64{1.0, 0.0};
-                if (g.q < n) {
-                    const c64 mpq = M[g.p + n * g.q];
-                    const double amp = sqrt(mpq.re * mpq.re + mpq.im * mpq.im);
-                    if (amp > 0.0) {
-                        const double er = mpq.re / amp, ei = mpq.im / amp;
-                        const double tau = (M[g.q + n * g.q].re - M[g.p + n * g.p].re) / (2.0 * amp);
-                        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                        const double cs = 1.0 / sqrt(1.0 + t * t), sn = t * cs;
-                        g.gpp = c64{cs, 0.0};
-                        g.gpq = c64{sn, 0.0};
-                        g.gqp = c64{-sn * er, sn * ei};
-                        g.gqq = c64{cs * er, -cs * ei};
-                    }
-                } else {
-                    g.q = g.p;  // dummy partner of an odd n: identity on a single index
-                }
-                rot[lane] = g;
-            }
-            wave_lds_sync();
-            if (lane < n) {  // M <- M G, Q <- Q G (lane = row)
-                for (int k = 0; k < npairs; k++) {
-                    const JacobiRot g = rot[k];
-                    if (g.p == g.q) continue;
-                    const c64 mp = M[lane + n * g.p], mq = M[lane + n * g.q];
-                    M[lane + n * g.p] = cadd(cmul(mp, g.gpp), cmul(mq, g.gqp));
-                    M[lane + n * g.q] = cadd(cmul(mp, g.gpq), cmul(mq, g.gqq));
-                    const c64 qp = Q[lane + n * g.p], qq = Q[lane + n * g.q];
-                    Q[lane + n * g.p] = cadd(cmul(qp, g.gpp), cmul(qq, g.gqp));
-                    Q[lane + n * g.q] = cadd(cmul(qp, g.gpq), cmul(qq, g.gqq));
-                }
-            }
-            wave_lds_sync();
-            if (lane < n) {  // M <- G^H M (lane = column)
-                for (int k = 0; k < npairs; k++) {
-                    const JacobiRot g = rot[k];
-                    if (g.p == g.q) continue;
-                    const c64 mp = M[g.p + n * lane], mq = M[g.q + n * lane];
-                    M[g.p + n * lane] = cadd(cmulc(g.gpp, mp), cmulc(g.gqp, mq));
-                    M[g.q + n * lane] = cadd(cmulc(g.gpq, mp), cmulc(g.gqq, mq));
-                }
-            }
-            wave_lds_sync();
-        }
-    }
+    wave_jacobi(n, M, Q, n, rot, lane, tol);
     // 4. ascending order
     double d = 0.0;
     if (lane < n) d = M[lane + n * lane].re;
@@ -437,6 +457,96 @@ __global__ __launch_bounds__(256) void heev_refine_kernel(int n, int64_t nb, con
     for (int e = lane; e < nn; e += 64) {
         const int r = e % n, j = e / n;
         U[b * nn + r + (int64_t)n * rank_s[j]] = c32{static_cast<float>(Q[e].re), static_cast<float>(Q[e].im)};
+    }
+}
+
+
+// ---------------------------------------------------------------------------------------------
+// Degenerate-band gauge fixing of the reference's per-k eigensolve (solve_eigen_el!,
+// src/diagonalize.jl:47-85), one wave per matrix, after the eigenpairs are known. Bands are grouped as
+// the reference groups them: a group starts at band a and runs while |E_i - E_a| <= cutoff. In each group
+// of m > 1 bands the fixed Hermitian perturbation P (src/diagonalize.jl:11-18; the matrix of EPW's
+// hamwan2bloch) is projected, M = U_g^H P U_g, M is diagonalised by the FP64 Jacobi and U_g <- U_g V with
+// the columns in ascending order of M's eigenvalues. The gauge inside each group is then fixed by H and P
+// alone (up to one phase per column), whatever gauge the solver returned. Matrices without a pair closer
+// than the cutoff are not touched. LDS (one wave per workgroup): U_g | W = P U_g | M (n*n c64 each),
+// rotation table, ranks.
+// ---------------------------------------------------------------------------------------------
+__host__ __device__ constexpr int gauge_lds_bytes(int n) { return 3 * n * n * 16 + 16 * (int)sizeof(JacobiRot) + 4 * n; }
+
+__device__ __forceinline__ c64 degeneracy_perturbation(int r, int c) {  // P[r, c], 0-based row r, column c
+    return c64{0.25644832 + 0.005 * (double)(r + c + 2), 0.0125 * (double)(r - c)};
+}
+
+__device__ void gauge_fix_group(int n, int a, int m, c32 *__restrict__ Ub, c64 *Ug, c64 *W, c64 *M,
+                                JacobiRot *rot, int *rank_s, int lane) {
+    for (int t = lane; t < n * m; t += 64) {
+        const c32 u = Ub[t % n + n * (a + t / n)];
+        Ug[t] = c64{u.re, u.im};
+    }
+    wave_lds_sync();
+    for (int t = lane; t < n * m; t += 64) {  // W = P U_g
+        const int r = t % n, c = t / n;
+        c64 acc{0.0, 0.0};
+        for (int k = 0; k < n; k++) acc = cadd(acc, cmul(degeneracy_perturbation(r, k), Ug[k + n * c]));
+        W[t] = acc;
+    }
+    wave_lds_sync();
+    for (int t = lane; t < m * m; t += 64) {  // M = U_g^H W, upper triangle, mirrored: exactly Hermitian
+        const int i = t % m, j = t / m;
+        if (i > j) continue;
+        c64 acc{0.0, 0.0};
+        for (int r = 0; r < n; r++) acc = cadd(acc, cmulc(Ug[r + n * i], W[r + n * j]));
+        if (i == j) acc.im = 0.0;
+        M[t] = acc;
+        M[j + m * i] = c64{acc.re, -acc.im};
+    }
+    wave_lds_sync();
+    wave_jacobi(m, M, Ug, n, rot, lane, 1e-15);
+    const double d = lane < m ? M[lane + m * lane].re : 0.0;
+    int rank = 0;
+    for (int k = 0; k < m; k++) {
+        const double dk = rdl64(d, k);
+        rank += (dk < d || (dk == d && k < lane)) ? 1 : 0;
+    }
+    if (lane < m) rank_s[lane] = rank;
+    wave_lds_sync();
+    for (int t = lane; t < n * m; t += 64) {
+        const int r = t % n, c = t / n;
+        Ub[r + n * (a + rank_s[c])] = c32{static_cast<float>(Ug[t].re), static_cast<float>(Ug[t].im)};
+    }
+    wave_lds_sync();
+}
+
+__global__ __launch_bounds__(64) void degenerate_gauge_kernel(int n, int64_t nb, const double *__restrict__ E64,
+                                                              c32 *__restrict__ U, double cutoff) {
+    extern __shared__ double lds64[];
+    const int lane = threadIdx.x;
+    const int64_t b = blockIdx.x;
+    const double e = lane < n ? E64[b * n + lane] : 0.0;
+    bool any = false;  // is_degenerate: some neighbouring pair closer than the cutoff
+    for (int i = 0; i + 1 < n && !any; i++) any = fabs(rdl64(e, i + 1) - rdl64(e, i)) < cutoff;
+    if (!any) return;
+    c64 *Ug = reinterpret_cast<c64 *>(lds64);
+    c64 *W = Ug + n * n;
+    c64 *M = W + n * n;
+    JacobiRot *rot = reinterpret_cast<JacobiRot *>(M + n * n);
+    int *rank_s = reinterpret_cast<int *>(rot + 16);
+    c32 *Ub = U + b * n * n;
+    double first = rdl64(e, 0);
+    int a = 0;
+    for (int i = 0; i + 1 < n; i++) {  // wave-uniform walk over the ascending eigenvalues
+        const double next = rdl64(e, i + 1);
+        int last;
+        if (fabs(first - next) > cutoff)
+            last = i;
+        else if (i == n - 2)
+            last = i + 1;
+        else
+            continue;
+        if (last > a) gauge_fix_group(n, a, last - a + 1, Ub, Ug, W, M, rot, rank_s, lane);
+        first = next;
+        a = i + 1;
     }
 }
 
@@ -488,6 +598,21 @@ extern "C" int ephip_heev_refine(int32_t n, int64_t nb, const void *H64, void *U
     const size_t lds = wpb * sizeof(double) * refine_lds_doubles(n);
     heev_refine_kernel<<<(unsigned)((nb + wpb - 1) / wpb), 64 * wpb, lds, as_stream(stream)>>>(
         n, nb, static_cast<const c64 *>(H64), static_cast<c32 *>(U), E64, tol > 0.0 ? tol : 1e-15);
+    EPHIP_LAUNCHED();
+    return EPHIP_OK;
+}
+
+extern "C" int ephip_degenerate_gauge_fix(int32_t n, int64_t nb, const double *E64, void *U, double cutoff,
+                                          void *stream) {
+    int st = heev_launch_check(n, nb);
+    if (st) return st;
+    EPHIP_REQUIRE(n <= 31, "degenerate_gauge_fix: n=%d > 31", n);
+    EPHIP_REQUIRE(cutoff >= 0.0, "degenerate_gauge_fix: negative cutoff");
+    EPHIP_REQUIRE(nb <= 0x7fffffffLL, "degenerate_gauge_fix: batch too large");
+    if (nb == 0) return EPHIP_OK;
+    EPHIP_REQUIRE(E64 && U, "degenerate_gauge_fix: null device pointer");
+    degenerate_gauge_kernel<<<(unsigned)nb, 64, gauge_lds_bytes(n), as_stream(stream)>>>(
+        n, nb, E64, static_cast<c32 *>(U), cutoff);
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

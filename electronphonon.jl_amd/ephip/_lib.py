@@ -27,6 +27,7 @@ SIGNATURES = {
     "ephip_heev_c32": [_i32, _i64, _vp, _i32, _vp, _vp, _vp, _vp],
     "ephip_heev_refine": [_i32, _i64, _vp, _vp, _vp, _vp, _d, _vp],
     "ephip_eigen_batched": [_i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
+    "ephip_degenerate_gauge_fix": [_i32, _i64, _vp, _vp, _d, _vp],
     "ephip_eigvals_batched": [_i32, _i64, _vp, _vp, _vp, _vp, _vp],
     "ephip_eigvals_batched_noalloc": [_i32, _i64, _vp, _vp, _vp, _vp],
     "ephip_window_filter": [_i32, _i64, _vp, _d, _d, _vp, _vp, _vp, _vp, _vp],

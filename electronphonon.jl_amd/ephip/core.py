@@ -464,6 +464,20 @@ def eigen_batched_refined(H64, tol: float = 0.0):
     return E64, U
 
 
+def degenerate_gauge_fix(E64, U, cutoff: float = ELECTRON_DEGEN_CUTOFF_EV):
+    """In place: the degenerate-band gauge fixing of solve_eigen_el! (src/diagonalize.jl:47-85) on the
+    eigenpairs E64 (nk, n) float64 ascending, U (nk, n, n) complex64 column-major blocks, through
+    ephip_degenerate_gauge_fix. Returns U."""
+    torch = _torch()
+    if E64.dtype != torch.float64 or U.dtype != torch.complex64:
+        raise L.EphipError("degenerate_gauge_fix needs float64 E and complex64 U")
+    nb, n = E64.shape
+    if tuple(U.shape) != (nb, n, n) or not (E64.is_contiguous() and U.is_contiguous()):
+        raise L.EphipError(f"degenerate_gauge_fix: shapes {tuple(E64.shape)}, {tuple(U.shape)} (contiguous) expected")
+    L.call("ephip_degenerate_gauge_fix", n, nb, L.ptr(E64), L.ptr(U), float(cutoff), L.stream_handle())
+    return U
+
+
 def eigvals_batched_refined(H64):
     """eigvals_batched (src/wannier_to_bloch_batched.jl:30-41) at FP64 accuracy through the C-ABI entry
     ephip_eigvals_batched: E64 (nk, n) float64 ascending of the complex128 stack ``H64``; the FP32

@@ -39,10 +39,15 @@ class ElectronStates:
         return self.E.shape[1]
 
 
-def compute_electron_states(ham_dev: WannierObject, xk_dev, window=None, chunk: int = 32768) -> ElectronStates:
+def compute_electron_states(ham_dev: WannierObject, xk_dev, window=None, chunk: int = 32768,
+                            degen_cutoff: float | None = core.ELECTRON_DEGEN_CUTOFF_EV) -> ElectronStates:
     """H(R) -> H(k) in FP64 on the matrix cores, FP32 eigensolve, FP64 refinement, window filter.
     Device counterpart of compute_electron_states + filter_electron_states
-    (src/compute_states.jl:135-178, src/filter.jl:46-72)."""
+    (src/compute_states.jl:135-178, src/filter.jl:46-72). degen_cutoff: the eigenvectors of bands closer
+    than this get the per-k path's degenerate gauge fixing (get_el_eigen! -> solve_eigen_el!,
+    src/diagonalize.jl:47-85), so per-band quantities at degenerate k (|g|^2, band velocities) are those of
+    the reference's CPU path and do not depend on the solver; None keeps the batched path's raw gauge
+    (eigen_batched applies no fixing)."""
     torch = _torch()
     nk = xk_dev.shape[0]
     nw = int(round(math.sqrt(ham_dev.ndata)))
@@ -57,6 +62,8 @@ def compute_electron_states(ham_dev: WannierObject, xk_dev, window=None, chunk: 
         h = H64[: e - s]
         core.fourier_batched(ham_dev, xk_dev[s:e], out=h)
         Ec, Uc = core.eigen_batched_refined(h)
+        if degen_cutoff is not None:
+            core.degenerate_gauge_fix(Ec, Uc, degen_cutoff)
         E[s:e] = Ec
         U[s:e] = Uc
     filt = core.window_filter(E, window[0], window[1]) if window is not None else None

@@ -179,19 +179,24 @@ def hermitian_eph(ep: np.ndarray, irvec: np.ndarray, nw: int, nm: int) -> np.nda
     the matrix element of a Hermitian perturbation: g_ij,nu(Re, Rp) = conj(g_ji,nu(-Re, Rp - Re)),
     which makes g_mn,nu(k, q) = conj(g_nm,nu(k + q, -q)) in the Wannier gauge. Pairs whose partner
     Rp - Re falls outside the box are set to zero. Returns a new array of ep's dtype."""
+    import concurrent.futures as cf
     nr = irvec.shape[0]
     index = {tuple(r): i for i, r in enumerate(irvec.tolist())}
     neg = np.array([index[tuple(-x for x in r)] for r in irvec.tolist()])
     # column layout (numpy C order of the Julia column (i, j, nu, Re)): (Re, nu, j, i)
     g = ep.reshape(nr, nr, nm, nw, nw)
     out = np.zeros_like(g)
-    for ip in range(nr):
+
+    def column(ip):  # writes out[ip] only (numpy releases the GIL on these block copies)
         part = [index.get(tuple(irvec[ip] - irvec[ie])) for ie in range(nr)]
         ok = np.array([p is not None for p in part])
         ies = np.flatnonzero(ok)
         ips = np.array([part[ie] for ie in ies], dtype=np.int64)
         partner = g[ips, neg[ies]]                      # (n, nm, j', i') at (-Re, Rp - Re)
         out[ip, ies] = 0.5 * (g[ip, ies] + np.conj(np.swapaxes(partner, -1, -2)))
+
+    with cf.ThreadPoolExecutor(max_workers=16) as ex:
+        list(ex.map(column, range(nr)))
     return out.reshape(ep.shape)
 
 

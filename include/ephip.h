@@ -88,6 +88,15 @@ int ephip_heev_c32(int32_t n, int64_t nb, const void *H, int32_t h_is_c64, void 
 int ephip_heev_refine(int32_t n, int64_t nb, const void *H64, void *U, const float *E32, double *E64,
                       double tol, void *stream);
 
+/* Degenerate-band gauge fixing of the per-k eigensolve solve_eigen_el! (src/diagonalize.jl:11-18, 47-85),
+ * in place on eigenpairs already computed (E64 double (n x nb) ascending, U c32 (n x n x nb)): bands are
+ * grouped as the reference groups them (a group starting at band a runs while |E_i - E_a| <= cutoff); in
+ * each group of m > 1 bands, U_g <- U_g V where V diagonalises U_g^H P U_g (P: the reference's fixed
+ * perturbation matrix) with ascending eigenvalues. Matrices with no neighbouring pair closer than cutoff
+ * are untouched. cutoff: the reference's electron_degen_cutoff is 0.01 meV (src/common/constants.jl:6),
+ * 1e-5 in eV. n <= 31. */
+int ephip_degenerate_gauge_fix(int32_t n, int64_t nb, const double *E64, void *U, double cutoff, void *stream);
+
 /* eigen_batched(Hk) -- src/wannier_to_bloch_batched.jl:43-65 (CUDA method ext/ElectronPhononCUDAExt.jl:101-115),
  * at the FP64 accuracy of the reference's zheev path: ephip_heev_c32 then ephip_heev_refine. H64: c64
  * (n x n x nb), upper triangle read, not modified. E64: double (n x nb) ascending; U: c32 (n x n x nb)

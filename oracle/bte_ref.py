@@ -2,6 +2,7 @@
 CPU leg, never by the product path).
 
 Restates, on the host, the reference's uniform-grid transport pipeline without symmetry:
+* electron states with the per-k eigensolve's degenerate-band gauge fixing (src/diagonalize.jl:47-85);
 * band velocities Re(u^H dH/dk u), dH/dk = sum_R i R_cart H(R) e^{2 pi i k.R}
   (src/wannier/WannierObject.jl:78-87, src/wannier_to_bloch.jl:70-81);
 * the scattering accumulation bte_window_accumulate! (src/boltzmann/boltzmann_calculator.jl:336-365)
@@ -37,9 +38,16 @@ def velocity_operator(irvec, ham_r, lattice=None):
     return np.concatenate([1j * rc[:, a:a + 1] * ham_r for a in range(3)], axis=1)
 
 
-def states(m, kgrid, window):
+# Degeneracy threshold of the per-k eigensolve's gauge fixing: 0.01 meV (src/common/constants.jl:6), in eV.
+DEGEN_CUTOFF = 1e-5
+
+
+def states(m, kgrid, window, degen_cutoff=DEGEN_CUTOFF):
+    """Electron states of the grid (src/compute_states.jl, CPU path): eigenpairs with the degenerate-band
+    gauge fixing of get_el_eigen! / solve_eigen_el! (src/diagonalize.jl:47-85), window filter, state
+    lists. degen_cutoff None: no gauge fixing."""
     xk = O.kpoints_grid(kgrid)
-    E, U = O.heev_batched(O.fourier(m.ham_r, m.irvec_h, xk))
+    E, U = O.heev_batched(O.fourier(m.ham_r, m.irvec_h, xk), degen_cutoff=degen_cutoff)
     keep, bmin, bmax, nbelow = O.window_filter(E, window[0], window[1])
     nb = np.where(bmin > 0, bmax - bmin + 1, 0).astype(np.int64)
     off = np.concatenate([[0], np.cumsum(nb)]).astype(np.int64)

@@ -261,6 +261,94 @@ int ephor_heev_batched(int n, int64_t nb, zc *A, double *w, zc *V) {
     return bad ? -1 : 0;
 }
 
+/* The fixed Hermitian perturbation of the degenerate-band gauge fixing, P[r, c] with 0-based row r and
+ * column c (src/diagonalize.jl:11-18, written there with 1-based indices; EPW's hamwan2bloch matrix). */
+zc ephor_degeneracy_perturbation(int r, int c) {
+    return (0.25644832 + 0.005 * (double)(r + c + 2)) + I * (0.0125 * (double)(r - c));
+}
+
+/* Degenerate-band groups of ascending eigenvalues w[0..n) as solve_eigen_el! forms them
+ * (src/diagonalize.jl:58-84): a group starting at band a runs while |w_i - w_a| <= cutoff. group[i] = the
+ * first band of band i's group. Returns the number of groups. */
+int ephor_degenerate_groups(int n, const double *w, double cutoff, int *group) {
+    if (n <= 0) return 0;
+    int a = 0, ngroups = 0;
+    double first = w[0];  /* bands left unassigned by the walk (a trailing band) are groups of their own */
+    for (int i = 0; i < n; i++) group[i] = i;
+    for (int i = 0; i + 1 < n; i++) {
+        int last;
+        if (fabs(first - w[i + 1]) > cutoff)
+            last = i;
+        else if (i == n - 2)
+            last = i + 1;
+        else
+            continue;
+        for (int j = a; j <= last; j++) group[j] = a;
+        first = w[i + 1];
+        a = i + 1;
+    }
+    for (int i = 0; i < n; i++) ngroups += group[i] == i;
+    return ngroups;
+}
+
+/* Degenerate-band gauge fixing of solve_eigen_el! (src/diagonalize.jl:47-85) on the eigenpairs of a stack
+ * (w: n x nb ascending, V: n x n x nb column-major, in place): for each group of m > 1 bands,
+ * M = V_g^H P V_g is diagonalised (ascending eigenvalues, ephor_heev) and V_g <- V_g X. Stacks with no
+ * neighbouring pair closer than the cutoff are untouched (is_degenerate, src/diagonalize.jl:20-27).
+ * Returns 0, or -1 if a projected eigensolve failed. */
+int ephor_degenerate_gauge_fix(int n, int64_t nb, const double *w, zc *V, double cutoff) {
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : bad)
+    for (int64_t b = 0; b < nb; b++) {
+        const double *wb = w + (int64_t)n * b;
+        zc *Vb = V + (int64_t)n * n * b;
+        int any = 0;
+        for (int i = 0; i + 1 < n; i++) any |= fabs(wb[i + 1] - wb[i]) < cutoff;
+        if (!any) continue;
+        int *group = (int *)malloc(sizeof(int) * n);
+        zc *W = (zc *)malloc(sizeof(zc) * n * n), *M = (zc *)malloc(sizeof(zc) * n * n);
+        zc *X = (zc *)malloc(sizeof(zc) * n * n), *Y = (zc *)malloc(sizeof(zc) * n * n);
+        double *wm = (double *)malloc(sizeof(double) * n);
+        ephor_degenerate_groups(n, wb, cutoff, group);
+        for (int a = 0; a < n;) {
+            int m = 1;
+            while (a + m < n && group[a + m] == a) m++;
+            if (m > 1) {
+                zc *Vg = Vb + (int64_t)n * a;
+                for (int c = 0; c < m; c++)      /* W = P V_g */
+                    for (int r = 0; r < n; r++) {
+                        zc acc = 0;
+                        for (int k = 0; k < n; k++) acc += ephor_degeneracy_perturbation(r, k) * Vg[k + (int64_t)n * c];
+                        W[r + n * c] = acc;
+                    }
+                for (int j = 0; j < m; j++)      /* M = V_g^H W */
+                    for (int i = 0; i < m; i++) {
+                        zc acc = 0;
+                        for (int r = 0; r < n; r++) acc += conj(Vg[r + (int64_t)n * i]) * W[r + n * j];
+                        M[i + m * j] = acc;
+                    }
+                bad |= ephor_heev(m, M, m, wm, X, m) != 0;
+                for (int c = 0; c < m; c++)      /* Y = V_g X */
+                    for (int r = 0; r < n; r++) {
+                        zc acc = 0;
+                        for (int k = 0; k < m; k++) acc += Vg[r + (int64_t)n * k] * X[k + m * c];
+                        Y[r + n * c] = acc;
+                    }
+                for (int c = 0; c < m; c++)
+                    for (int r = 0; r < n; r++) Vg[r + (int64_t)n * c] = Y[r + n * c];
+            }
+            a += m;
+        }
+        free(group);
+        free(W);
+        free(M);
+        free(X);
+        free(Y);
+        free(wm);
+    }
+    return bad ? -1 : 0;
+}
+
 /* Phonon eigenproblem at one q (src/wannier_to_bloch.jl:186-205, src/diagonalize.jl:104-127):
  * D(q)[i,j] /= sqrt(m_i m_j); eigen; omega = sign(w) sqrt(|w|); u[i,:] /= sqrt(m_i). */
 int ephor_ph_eigen(int nm, zc *D, const double *mass, double *omega, zc *U) {

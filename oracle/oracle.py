@@ -44,6 +44,8 @@ def lib():
             "ephor_fourier": [_i64, _i32, _zp, _i64, _ip, _i32, _dp, _zp, _i64],
             "ephor_heev": [_i32, _zp, _i32, _dp, C.c_void_p, _i32],
             "ephor_heev_batched": [_i32, _i64, _zp, _dp, C.c_void_p],
+            "ephor_degenerate_gauge_fix": [_i32, _i64, _dp, _zp, _d],
+            "ephor_degenerate_groups": [_i32, _dp, _d, C.c_void_p],
             "ephor_ph_eigen": [_i32, _zp, _dp, _dp, _zp],
             "ephor_window_filter": [_i32, _i64, _dp, _d, _d, _bp, _lp, _lp, _lp],
             "ephor_kpoints_grid": [_i32, _i32, _i32, _dp, _dp],
@@ -72,7 +74,8 @@ def lib():
         for name, args in sig.items():
             f = getattr(L, name)
             f.argtypes = args
-            f.restype = C.c_int if name in ("ephor_heev", "ephor_heev_batched", "ephor_ph_eigen",
+            f.restype = C.c_int if name in ("ephor_heev", "ephor_heev_batched", "ephor_degenerate_gauge_fix",
+                                            "ephor_degenerate_groups", "ephor_ph_eigen",
                                             "ephor_num_threads", "ephor_gridopt_nr23", "ephor_gridopt_nr3") else None
         L.ephor_gridopt_new.restype = C.c_void_p
         L.ephor_outer_q_gridopt.restype = C.c_int64
@@ -116,9 +119,10 @@ def heev(h, vectors=True):
     return (w, v.T.copy()) if vectors else w
 
 
-def heev_batched(hs, vectors=True):
+def heev_batched(hs, vectors=True, degen_cutoff=None):
     """hs: (nb, n*n) column-major blocks (Fourier output layout). Returns E (nb, n) and
-    U (nb, n*n) column-major eigenvector blocks."""
+    U (nb, n*n) column-major eigenvector blocks. degen_cutoff: apply the per-k path's degenerate-band
+    gauge fixing (degenerate_gauge_fix) to the eigenvectors."""
     hs = np.array(hs, dtype=np.complex128, order="C", copy=True)
     nb = hs.shape[0]
     n = int(round(np.sqrt(hs.shape[1])))
@@ -127,7 +131,30 @@ def heev_batched(hs, vectors=True):
     st = lib().ephor_heev_batched(n, nb, hs, w, v.ctypes.data if vectors else None)
     if st != 0:
         raise RuntimeError("ephor_heev_batched did not converge")
+    if vectors and degen_cutoff is not None:
+        degenerate_gauge_fix(w, v, degen_cutoff)
     return (w, v) if vectors else w
+
+
+def degenerate_groups(w, cutoff):
+    """First band of each band's degenerate group, for ascending eigenvalues w (n,), grouped as
+    solve_eigen_el! groups them (src/diagonalize.jl:58-84)."""
+    w = _c(w, np.float64)
+    g = np.empty(len(w), np.int32)
+    lib().ephor_degenerate_groups(len(w), w, float(cutoff), g.ctypes.data)
+    return g
+
+
+def degenerate_gauge_fix(E, U, cutoff):
+    """In place on E (nb, n) ascending and U (nb, n*n) complex128 column-major blocks (C-contiguous): the
+    degenerate-band gauge fixing of solve_eigen_el! (src/diagonalize.jl:47-85). Returns U."""
+    E = _c(E, np.float64)
+    if U.dtype != np.complex128 or not U.flags.c_contiguous:
+        raise ValueError("degenerate_gauge_fix needs a C-contiguous complex128 U")
+    nb, n = E.shape
+    if lib().ephor_degenerate_gauge_fix(n, nb, E, U, float(cutoff)) != 0:
+        raise RuntimeError("ephor_degenerate_gauge_fix: a projected eigensolve did not converge")
+    return U
 
 
 def ph_eigen(dq_colmajor, mass):

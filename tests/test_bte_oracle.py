@@ -175,3 +175,29 @@ def test_calculator_argument_checks():
         bte.BoltzmannCalculator(p, smearing=("Gaussian", 0.0))
     with pytest.raises(ValueError):
         bte.ElectronTransportParams(Tlist=[0.025, 0.03], nlist=[0.01], volume=1.0)
+
+
+def test_conductivity_drude_closed_form():
+    """bte_ref.conductivity on a parabolic band with a constant scattering rate (S_in = 0, so BTE = SERTA)
+    against the Drude law sigma = n tau / m* (e = hbar = 1), n = spin / V * w sum f the carrier density:
+    by parts, sum (-df/de) v_x^2 = (1 / m*) sum f for E = |k|^2 / (2 m*) once f vanishes at the zone edge,
+    which a uniform grid sums to spectral accuracy. Independent of the formula's factors: a missing spin,
+    volume or weight factor, a transposed index or a sign flip of -df/de fails it; the off-diagonal
+    entries vanish by symmetry."""
+    import bte_ref as B
+    n1 = 48
+    k = (np.arange(n1) - n1 // 2) / n1
+    kx, ky, kz = np.meshgrid(k, k, k, indexing="ij")
+    K = 2 * np.pi * np.stack([kx.ravel(), ky.ravel(), kz.ravel()], 1)   # Cartesian, cubic cell a = 1
+    for mstar, spin, vol in ((1.0, 2, 1.7), (0.7, 1, 1.0)):
+        E = (K ** 2).sum(1) / (2 * mstar)
+        v = K / mstar
+        mu, T, tau = 1.2, 0.3, 3.0
+        w = 1.0 / len(E)
+        S_out = np.full((1, len(E)), 1.0 / tau)
+        rhs = (B.dfermi(E - mu, T) / S_out[0])[:, None] * v
+        sig = B.conductivity(rhs, v, w, spin, vol)
+        n = spin / vol * w * B.occ((E - mu) / T).sum()
+        assert abs(sig[0, 0] / (n * tau / mstar) - 1) < 1e-5
+        assert np.allclose(np.diag(sig), sig[0, 0], rtol=1e-12)
+        assert np.abs(sig - np.diag(np.diag(sig))).max() < 1e-12 * sig[0, 0]

@@ -139,6 +139,62 @@ def test_heev_refine_fp64(backend, n, split):
         assert np.abs(h[b] @ V - V * E[b]).max() < 1e-6 * np.abs(wl).max()
 
 
+@pytest.mark.parametrize("n", [8, 16, 31])
+def test_degenerate_gauge_fix_parity(backend, n):
+    """ephip_degenerate_gauge_fix after the refined eigensolve against the oracle's solve_eigen_el! gauge
+    fixing (src/diagonalize.jl:47-85) on matrices with planted 2- and 3-fold groups, pairs split by 2e-6
+    (inside the 1e-5 cutoff), a chain cut by the walk and non-degenerate spectra: every column equals the
+    oracle's up to a phase (|overlap| >= 1 - 1e-6: FP32 vectors); matrices without a close pair are left
+    bit-for-bit unchanged."""
+    from ephip import core
+    rng = np.random.default_rng(100 + n)
+    cut = 1e-5
+    spectra = []
+    for b in range(96):
+        e = np.sort(rng.uniform(-2.0, 2.0, n))
+        kind = b % 4
+        if kind == 1:
+            i = int(rng.integers(0, n - 1))
+            e[i + 1] = e[i]
+        elif kind == 2 and n >= 6:
+            i = int(rng.integers(0, n - 5))
+            e[i + 1] = e[i + 2] = e[i]
+            e[i + 4] = e[i + 3] + 2e-6
+        elif kind == 3 and n >= 4:
+            i = int(rng.integers(0, n - 3))
+            e[i + 1], e[i + 2] = e[i] + 0.6e-5, e[i] + 1.2e-5
+        e = np.sort(e)
+        if kind == 0:   # keep non-degenerate spectra well separated from the cutoff
+            e += np.arange(n) * 1e-3
+        spectra.append(e)
+    hs = []
+    for e in spectra:
+        q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+        h = q @ np.diag(e) @ q.conj().T
+        hs.append(0.5 * (h + h.conj().T))
+    hcol = np.stack([h.ravel(order="F") for h in hs])
+    E_ref, U_ref = O.heev_batched(hcol, degen_cutoff=cut)
+    E, U = core.eigen_batched_refined(_dev(hcol.reshape(-1, n, n)))
+    U_raw = U.clone()
+    core.degenerate_gauge_fix(E, U, cut)
+    E, U, U_raw = E.cpu().numpy(), U.cpu().numpy(), U_raw.cpu().numpy()
+    assert np.abs(E - E_ref).max() < 1e-12
+    worst, fixed = 0.0, 0
+    for b in range(len(hs)):
+        close = bool((np.diff(E[b]) < cut).any())
+        if not close:
+            assert np.array_equal(U[b], U_raw[b])
+            continue
+        fixed += 1
+        assert np.bincount(O.degenerate_groups(E_ref[b], cut)).max() <= 3
+        V = U[b].reshape(-1).reshape(n, n, order="F").astype(np.complex128)
+        Vr = U_ref[b].reshape(n, n, order="F")
+        ov = np.abs(np.sum(V.conj() * Vr, axis=0))
+        worst = max(worst, float(np.abs(ov - 1).max()))
+    print(f"gauge fix n={n}: {fixed} of {len(hs)} matrices fixed, worst 1 - |overlap| {worst:.2e}")
+    assert fixed >= len(hs) // 2 and worst <= 1e-6
+
+
 # ------------------------------------------------------------------------------ states + filter
 def test_electron_states_and_filter_bit_exact(backend):
     from ephip import core, eph

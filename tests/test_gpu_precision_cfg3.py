@@ -130,3 +130,61 @@ def test_cfg3_g2(cfg3, prec_name):
     print(f"cfg3 |g|^2 {prec_name}: norm-wise relative error {rel:.2e} over {len(errs)} (k, q) pairs")
     assert len(errs) > 20
     assert rel <= 1e-4
+
+
+@pytest.fixture(scope="module")
+def cfg3_transport(backend):
+    """The cfg3 model's time-reversal variant (real H(R), Hermitian real g, complex128) on the reduced
+    equal 8^3 grids, its device setup and the FP64 oracle's transport at 300 K, 10 meV (computed once)."""
+    import bte_ref as B
+    from ephip import eph
+    m, g = M.make_config_model("cfg3", time_reversal=True, with_g=False)
+    ep = M.hermitian_eph(M.build_epmat(m, dtype=np.complex128), m.irvec_el, m.nw, m.nmodes)
+    m.epmat_ph = ep.real.astype(ep.dtype)
+    del ep
+    kgrid = qgrid = (8, 8, 8)
+    w = g["window"]
+    T, mu = np.array([0.025852]), np.array([0.0])
+    st = eph.setup_outer_q(m, kgrid, qgrid, (-w, w), backend)
+    s = B.states(m, kgrid, (-w, w))
+    v = B.velocities(m, s)
+    So, Si = B.scattering(m, s, qgrid, kgrid, T, mu, np.array([0.01]), method=5)
+    ss, sb, its = B.solve_bte(So, Si, s["es"], v, s["weight"], T, mu, max_iter=2000, rtol=1e-11)
+    return dict(m=m, st=st, kgrid=kgrid, T=T, mu=mu, s=s, So=So, ss=ss, sb=sb, its=its)
+
+
+@pytest.mark.parametrize("prec_name", ["fp32", "bf16x3"])
+def test_cfg3_transport_vs_fp64(backend, cfg3_transport, prec_name):
+    """Transport of the cfg3 model (nw 16, 4 atoms, H/D 9^3, g 7^3 x 7^3, planted two-fold degeneracies on
+    k3 = 0 and 1/2; time-reversal variant, real H(R) and Hermitian real g, so the linearised BTE
+    converges) at 300 K with 10 meV Gaussian smearing on a reduced equal k = q = 8^3 grid (+-0.3 eV,
+    occupation convention 5), against the FP64 oracle: SERTA and iterative-BTE conductivities and the
+    mobility within 1e-5 relative (the metric's bound), for the staged FP32 kernels and the split-BF16
+    MFMA kernels. On 8^3 the planted planes hold 2 of 8 k3 values: 25% of the k-points are two-fold
+    degenerate (the configuration asks for 20%)."""
+    from ephip import _lib as L
+    from ephip import bte, core, eph
+    c = cfg3_transport
+    m, st, s = c["m"], c["st"], c["s"]
+    prec = L.PREC_FP32 if prec_name == "fp32" else L.PREC_BF16X3
+    loop = eph.OuterKLoop(st, nk_batch=128, precision=prec, gridopt=prec_name == "fp32")
+    assert (loop.gridopt_q is not None) == (prec_name == "fp32")
+    params = bte.ElectronTransportParams(Tlist=c["T"], nlist=[0.1], volume=1.0, mulist=c["mu"])
+    calc = bte.BoltzmannCalculator(params, smearing=("Gaussian", 0.01), occupation_method=5)
+    calc = bte.run_boltzmann(loop, calc)
+    vel = core.WannierObject(m.irvec_h, bte.velocity_operator(m.irvec_h, m.ham_r).op_r).to_device(
+        backend, dtype=torch.complex128)
+    bte.compute_velocities(vel, st.el, calc.bs)
+    res = bte.solve_electron_bte(calc, max_iter=2000, rtol=1e-11)
+    ss, sb = c["ss"], c["sb"]
+    assert np.array_equal(calc.bs.ks.cpu().numpy(), s["ks"]) and (c["So"] > 0).all()
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    e_s, e_b = rel(res.sigma_serta[0], ss[0]), rel(res.sigma[0], sb[0])
+    mob = rel(bte.transport_mobility(res.sigma, params), bte.transport_mobility(sb, params))
+    kz = M.kpoints_grid(c["kgrid"])[:, 2]
+    frac = float(np.mean((kz == 0) | (kz == 0.5)))
+    print(f"cfg3 transport {prec_name}, 8^3, 300 K, 10 meV: {calc.bs.n} states, SERTA rel {e_s:.2e}, BTE rel "
+          f"{e_b:.2e} ({res.niter[0]} iterations, oracle {c['its'][0]}), mobility rel {mob:.2e}; planted-plane "
+          f"fraction {frac:.3f}")
+    assert res.converged[0] and frac >= 0.2
+    assert e_s <= 1e-5 and e_b <= 1e-5 and mob <= 1e-5

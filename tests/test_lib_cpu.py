@@ -130,6 +130,16 @@ def test_eigen_entries_validate_arguments():
     L.call("ephip_eigen_batched", 8, 0, None, None, None, None, None, None)
 
 
+def test_degenerate_gauge_fix_validates_arguments():
+    with pytest.raises(L.EphipError, match="n=32 > 31"):
+        L.call("ephip_degenerate_gauge_fix", 32, 4, 1, 1, 1e-5, None)
+    with pytest.raises(L.EphipError, match="negative cutoff"):
+        L.call("ephip_degenerate_gauge_fix", 8, 4, 1, 1, -1.0, None)
+    with pytest.raises(L.EphipError, match="null device pointer"):
+        L.call("ephip_degenerate_gauge_fix", 8, 4, None, None, 1e-5, None)
+    L.call("ephip_degenerate_gauge_fix", 8, 0, None, None, 1e-5, None)
+
+
 def test_eigvals_noalloc_validates_before_any_device_call():
     """The allocation-free entry rejects sizes beyond its scratch array (16 MiB per GPU) and bad arguments
     before touching the device; nb = 0 is a no-op."""

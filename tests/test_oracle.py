@@ -116,6 +116,85 @@ def test_heev_reads_upper_triangle():
     assert np.allclose(O.heev(b, vectors=False), np.linalg.eigvalsh(a), atol=1e-13)
 
 
+def test_degenerate_groups_follow_the_reference_walk():
+    """Groups are formed as solve_eigen_el! walks the ascending eigenvalues (src/diagonalize.jl:58-84):
+    a group runs while |E_i - E_first| <= cutoff (measured from the group's first band, so a chain of
+    close neighbours is cut), and a trailing band or group closes at the last band."""
+    c = 1e-5
+    assert O.degenerate_groups([0.0, 0.6e-5, 1.2e-5, 1.0, 1.0 + 2e-6, 2.0], c).tolist() == [0, 0, 2, 3, 3, 5]
+    assert O.degenerate_groups([0.0, 1.0, 1.0 + 1e-6], c).tolist() == [0, 1, 1]
+    assert O.degenerate_groups([0.0, 1.0, 2.0], c).tolist() == [0, 1, 2]
+    assert O.degenerate_groups([0.5, 0.5, 0.5, 0.5], c).tolist() == [0, 0, 0, 0]
+    assert O.degenerate_groups([0.5], c).tolist() == [0]
+
+
+def _planted_stack(rng, n, spectra):
+    """Hermitian matrices Q diag(e) Q^H with random unitary Q, one per spectrum; (nb, n*n) column-major."""
+    hs = []
+    for e in spectra:
+        q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+        h = q @ np.diag(e) @ q.conj().T
+        hs.append(0.5 * (h + h.conj().T))
+    return np.stack([h.ravel(order="F") for h in hs])
+
+
+def test_degenerate_gauge_fix_properties():
+    """The fixed gauge depends on H and P only: rotating the solver's eigenvectors by an arbitrary unitary
+    inside every degenerate group before the fix gives the same vectors up to one phase each (a transposed
+    or conjugated projection, or a fix applied to the wrong columns, breaks this); the fixed vectors stay
+    orthonormal eigenvectors; U_g^H P U_g is diagonal with ascending diagonal for each group (the order
+    syev returns; P is the reference's matrix, src/diagonalize.jl:11-18); non-degenerate stacks are left
+    bit-for-bit unchanged. P = A 1 1^T + B (r 1^T + 1 r^T) + i C (r 1^T - 1 r^T) has rank 2, so the method
+    fixes groups of up to 3 bands; in a larger group (the 8-fold stack) part of the gauge stays free, and
+    only the eigenvector properties are checked there."""
+    rng = np.random.default_rng(17)
+    n, c = 8, 1e-5
+    spectra = [[-1.0, 0.3, 0.3, 0.3, 0.7, 0.7 + 3e-6, 1.2, 1.2],
+               [-2.0, -1.0, 0.0, 0.1, 0.1, 0.9, 1.5, 2.5],
+               [-1.0, -0.5, 0.0, 0.5, 1.0, 1.5, 2.0, 2.5],
+               [0.2] * 8]
+    hs = _planted_stack(rng, n, spectra)
+    E, U0 = O.heev_batched(hs)
+    U1 = O.degenerate_gauge_fix(E, U0.copy(), c)
+    assert np.array_equal(U1[2], U0[2])
+    U2 = U0.copy()
+    for b in range(len(spectra)):
+        g = O.degenerate_groups(E[b], c)
+        V = U2[b].reshape(n, n, order="F")
+        for a in np.unique(g):
+            cols = np.flatnonzero(g == a)
+            if len(cols) > 1:
+                r, _ = np.linalg.qr(rng.standard_normal((len(cols),) * 2) + 1j * rng.standard_normal((len(cols),) * 2))
+                V[:, cols] = V[:, cols] @ r
+        U2[b] = V.ravel(order="F")
+    U2 = O.degenerate_gauge_fix(E, U2, c)
+    P = np.array([[(0.25644832 + 0.005 * (i + j + 2)) + 1j * 0.0125 * (i - j) for j in range(n)] for i in range(n)])
+    for b in range(len(spectra)):
+        h = hs[b].reshape(n, n, order="F")
+        V1, V2 = U1[b].reshape(n, n, order="F"), U2[b].reshape(n, n, order="F")
+        assert np.abs(V1.conj().T @ V1 - np.eye(n)).max() < 1e-13
+        assert np.abs(h @ V1 - V1 * E[b]).max() < 1e-5   # the 3e-6 pair is mixed by the fix
+        g = O.degenerate_groups(E[b], c)
+        if np.bincount(g).max() > 3:
+            continue
+        assert np.abs(np.abs(np.sum(V1.conj() * V2, axis=0)) - 1).max() < 1e-12
+        for a in np.unique(g):
+            cols = np.flatnonzero(g == a)
+            if len(cols) > 1:
+                m = V1[:, cols].conj().T @ P @ V1[:, cols]
+                assert np.abs(m - np.diag(np.diag(m))).max() < 1e-12
+                assert np.all(np.diff(np.diag(m).real) > 0)
+
+
+def test_heev_batched_gauge_fix_option():
+    rng = np.random.default_rng(18)
+    hs = _planted_stack(rng, 6, [[0.0, 0.0, 1.0, 2.0, 2.0, 3.0]])
+    E, U = O.heev_batched(hs, degen_cutoff=1e-5)
+    E0, U0 = O.heev_batched(hs)
+    assert np.array_equal(E, E0)
+    assert np.allclose(U, O.degenerate_gauge_fix(E0, U0, 1e-5), atol=0)
+
+
 def test_ph_eigen_closed_form_and_mass():
     # D diagonal with masses: omega_i = sign(d_i/m_i) sqrt(|d_i/m_i|), u = e_i / sqrt(m_i)
     d = np.array([4.0, -9.0, 1.0])
