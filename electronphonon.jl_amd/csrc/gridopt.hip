@@ -656,14 +656,34 @@ __device__ __forceinline__ void kq_stage1(float (&br)[N][N], float (&bi)[N][N], 
     }
 }
 
+// Workgroup order of the staged kR->kq kernels (grid m1 x nunits). Workgroups are dispatched round-robin
+// over the 8 XCDs in linear order (x fastest), so in the plain order (q1 = blockIdx.x, unit = blockIdx.y)
+// the m1 q1-workgroups of one unit spread over every XCD and each XCD's L2 fetches the unit's operator
+// rows. The grouped order gives XCD x one contiguous run of (unit, q1) pairs, unit-major, so one unit's
+// workgroups share one L2: linear id L = x + 8 n maps to run position n of XCD x, whose run starts after
+// the runs of XCDs 0 .. x-1 (T / 8 pairs each, one more for the first T % 8 XCDs).
+__device__ __forceinline__ void kq_block(int grouped, int &q1, int &u) {
+    if (!grouped) {
+        q1 = blockIdx.x;
+        u = blockIdx.y;
+        return;
+    }
+    const int64_t T = (int64_t)gridDim.x * gridDim.y;
+    const int64_t L = blockIdx.x + (int64_t)gridDim.x * blockIdx.y;
+    const int64_t x = L & 7, n = L >> 3, base = T >> 3, rem = T & 7;
+    const int64_t id = x * base + (x < rem ? x : rem) + n;
+    q1 = (int)(id % gridDim.x);
+    u = (int)(id / gridDim.x);
+}
+
 template <int N, int NBW, int NMC = 0, bool DENSE = false>  // NMC: compile-time mode count (12), or 0 for nm <= 16
-__global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 : 1))) void gridopt_kq_rotate_kernel(
     int nm, int n1, int p1min, int p2min, int p3min, const int *__restrict__ rmap, int s1, int s2, int s3, int m1,
     int m2, int m3,
     const cplx<float> *__restrict__ A, int64_t lda, int64_t ldr, const int *__restrict__ line_off,
     const int *__restrict__ col_index, const int *__restrict__ ikq, int64_t cs, const cplx<float> *__restrict__ U,
     const int64_t *__restrict__ bmin, const int64_t *__restrict__ bmax, const cplx<float> *__restrict__ uph,
-    const double *__restrict__ omega, int nbw, float *__restrict__ g2, float2 *__restrict__ epo) {
+    const double *__restrict__ omega, int nbw, float *__restrict__ g2, float2 *__restrict__ epo, int grouped) {
     constexpr int NW = 16;
     // q1, all q2, all q3 phases; the q2/q3 rows padded to an even count and read as 16-byte pairs
     constexpr int NE = (N + 1) & ~1;
@@ -681,7 +701,8 @@ __global__ __launch_bounds__(256) void gridopt_kq_rotate_kernel(
     constexpr int CHUNK = NBW >= 4 ? GQ_CHUNK / 2 : NBW == 3 ? GQ_CHUNK * 3 / 4 : GQ_CHUNK;
     __shared__ float2 ucol[CHUNK * NBW * NW];
     __shared__ float2 tq[CHUNK * NBW * NW];  // t[kk][m][nu], nu < nm <= 16
-    const int q1 = blockIdx.x, u = blockIdx.y;
+    int q1, u;
+    kq_block(grouped, q1, u);
     const int t = threadIdx.x;
     const int nd2 = NW * nm;
     const bool rv = t < nd2;
@@ -912,7 +933,7 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
                               const int *__restrict__ ikq, int64_t cs, const cplx<float> *__restrict__ U,
                               const int64_t *__restrict__ bmin, const int64_t *__restrict__ bmax,
                               const cplx<float> *__restrict__ uph, const double *__restrict__ omega, int nbw,
-                              float *__restrict__ g2, float2 *__restrict__ epo) {
+                              float *__restrict__ g2, float2 *__restrict__ epo, int grouped) {
     constexpr int NW = 16;
     constexpr int NE = (N + 1) & ~1;
     constexpr int BP = NMC > 0 ? NMC + 1 : 17;  // b2s pitch per i (odd)
@@ -932,7 +953,8 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
     __shared__ float2 tq[GQM_TQ * 17];       // t of the collected columns, [nu + 17 column]
     __shared__ int tqid[GQM_TQ];
     __shared__ float2 ucol[GQM_UC * 17];     // U_kq[i, band] of the line's first columns, [i + 17 column]
-    const int q1 = blockIdx.x, u = blockIdx.y;
+    int q1, u;
+    kq_block(grouped, q1, u);
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const bool rv = t < NW * nm;
     const int nlines = m1 * m2;
@@ -1168,6 +1190,7 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
     if (dense)
         EPHIP_REQUIRE(s1 >= 0 && s2 >= 0 && s3 >= 0 && (int64_t)(s1 + s2 + s3) * (np - 1) < ((int64_t)1 << 31),
                       "gridopt_kq_rotate: bad dense-box strides");
+    const int grouped = ephip_ft_get_variant(7) == 1;  // XCD-grouped workgroup order
     if (ephip_ft_get_variant(6) == 1) {  // stage 3 and the k+q-window rotation on MFMAs
         using KM = decltype(&gridopt_kq_rotate_mfma_kernel<3>);
 #define GQM_ROW(c, d) {gridopt_kq_rotate_mfma_kernel<3, c, d>, gridopt_kq_rotate_mfma_kernel<4, c, d>, \
@@ -1182,7 +1205,7 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
             nm, n1, p1min, p2min, p3min, rmap, s1, s2, s3, qgrid[0], qgrid[1], qgrid[2],
             static_cast<const cplx<float> *>(A), lda, ldr, line_off, col_index, ikq, cs,
             static_cast<const cplx<float> *>(U), bmin, bmax, static_cast<const cplx<float> *>(uph), omega, nbw, g2,
-            static_cast<float2 *>(ep));
+            static_cast<float2 *>(ep), grouped);
         EPHIP_LAUNCHED();
         return EPHIP_OK;
     }
@@ -1203,7 +1226,7 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
                                                 qgrid[2], static_cast<const cplx<float> *>(A), lda, ldr, line_off,
                                                 col_index, ikq, cs, static_cast<const cplx<float> *>(U), bmin, bmax,
                                                 static_cast<const cplx<float> *>(uph), omega, nbw, g2,
-                                                static_cast<float2 *>(ep));
+                                                static_cast<float2 *>(ep), grouped);
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

@@ -16,6 +16,9 @@ KIND_FP32_WIDE, KIND_FP32_NARROW, KIND_FP64, KIND_PACKED_BF16, KIND_FUSED_ROT, K
 KIND_GRIDOPT_KQ = 6
 GRIDOPT_KQ_VARIANTS = (0, 1)
 GRIDOPT_KQ_VARIANT_NAMES = {0: "VALU stage 3 + DPP window sums", 1: "MFMA stage 3 + window rotation"}
+KIND_KQ_ORDER = 7
+KQ_ORDER_VARIANTS = (0, 1)
+KQ_ORDER_VARIANT_NAMES = {0: "q1 fastest, unit by unit", 1: "XCD-grouped units"}
 VARIANTS = (0, 1, 2)
 VARIANT_NAMES = {0: "single-buffered KC16", 1: "double-buffered KC16", 2: "double-buffered KC8"}
 PACKED_VARIANTS = (0, 1, 2, 3, 4, 5, 6, 7)

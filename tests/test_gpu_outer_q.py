@@ -21,15 +21,17 @@ def cfg1():
     return m, g
 
 
-@pytest.fixture(params=[0, 1], ids=["kq-valu", "kq-mfma"])
+@pytest.fixture(params=[(0, 0), (1, 0), (1, 1)], ids=["kq-valu", "kq-mfma", "kq-mfma-xcd"])
 def kq_variant(request):
     """The staged kR->kq kernel form (tuner kind 6): 0 = stage 3 and the k+q-window rotation on the vector
-    ALUs, 1 = on FP32 MFMAs."""
+    ALUs, 1 = on FP32 MFMAs; and its workgroup order (kind 7): 0 = plain, 1 = XCD-grouped."""
     from ephip import tune
-    old = tune.get_variant(tune.KIND_GRIDOPT_KQ)
-    tune.set_variant(tune.KIND_GRIDOPT_KQ, request.param)
-    yield request.param
-    tune.set_variant(tune.KIND_GRIDOPT_KQ, old)
+    old = tune.get_variant(tune.KIND_GRIDOPT_KQ), tune.get_variant(tune.KIND_KQ_ORDER)
+    tune.set_variant(tune.KIND_GRIDOPT_KQ, request.param[0])
+    tune.set_variant(tune.KIND_KQ_ORDER, request.param[1])
+    yield request.param[0]
+    tune.set_variant(tune.KIND_GRIDOPT_KQ, old[0])
+    tune.set_variant(tune.KIND_KQ_ORDER, old[1])
 
 
 def _oracle_states(m, g):
