@@ -266,10 +266,13 @@ __global__ __launch_bounds__(256) void heev_c32_kernel(int n, int64_t nb, const 
 //      quadratic from a nearly diagonal start, and exact or close eigenvalue pairs need no special
 //      handling (a degenerate pair gets a well-defined 45-degree rotation).
 // Eigenvalues are diag(M) sorted ascending, eigenvectors the matching columns of Q (rounded to FP32).
-// LDS per wave: H | Q | W | M (n*n c64 each) + rotation table and ranks.
+// LDS per wave: H | Q | W | M (n columns of c64 each, column pitch ld = n | 1) + rotation table and
+// ranks. The odd pitch spreads the 16-byte column-strided accesses (lane = column) over the banks; with
+// ld = n = 16 every lane of such an access fell on the same banks.
 // ---------------------------------------------------------------------------------------------
 constexpr int REFINE_EXTRA_DOUBLES = 176;
-__host__ __device__ constexpr int refine_lds_doubles(int n) { return 8 * n * n + REFINE_EXTRA_DOUBLES; }
+__host__ __device__ constexpr int refine_ld(int n) { return n | 1; }
+__host__ __device__ constexpr int refine_lds_doubles(int n) { return 8 * n * refine_ld(n) + REFINE_EXTRA_DOUBLES; }
 
 __device__ __forceinline__ double rdl64(double v, int l) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
@@ -288,45 +291,48 @@ struct JacobiRot {
     c64 gpp, gpq, gqp, gqq;  // G = diag(1, e^{-i phi}) [[cs, sn], [-sn, cs]] on (p, q)
 };
 
-// out[r + n*j] = alpha * sum_k A[r + n*k] B[k + n*j] + beta * C[r + n*j] ; lanes over entries
-__device__ void lds_gemm_nn(int n, const c64 *A, const c64 *B, c64 *out, double alpha, double beta, const c64 *C,
-                            int lane) {
+// n x n matrices in LDS with column pitch ld (element (r, j) at r + ld*j); lanes over entries.
+// out = alpha * A B + beta * C
+__device__ void lds_gemm_nn(int n, int ld, const c64 *A, const c64 *B, c64 *out, double alpha, double beta,
+                            const c64 *C, int lane) {
     for (int e = lane; e < n * n; e += 64) {
         const int r = e % n, j = e / n;
         c64 s{0.0, 0.0};
-        for (int k = 0; k < n; k++) s = cadd(s, cmul(A[r + n * k], B[k + n * j]));
+        for (int k = 0; k < n; k++) s = cadd(s, cmul(A[r + ld * k], B[k + ld * j]));
         c64 v{alpha * s.re, alpha * s.im};
-        if (beta != 0.0) v = c64{v.re + beta * C[e].re, v.im + beta * C[e].im};
-        out[e] = v;
+        if (beta != 0.0) v = c64{v.re + beta * C[r + ld * j].re, v.im + beta * C[r + ld * j].im};
+        out[r + ld * j] = v;
     }
 }
 
-// out[i + n*j] = sum_r conj(A[r + n*i]) B[r + n*j]
-__device__ void lds_gemm_cn(int n, const c64 *A, const c64 *B, c64 *out, int lane) {
+// out = A^H B
+__device__ void lds_gemm_cn(int n, int ld, const c64 *A, const c64 *B, c64 *out, int lane) {
     for (int e = lane; e < n * n; e += 64) {
         const int i = e % n, j = e / n;
         c64 s{0.0, 0.0};
-        for (int r = 0; r < n; r++) s = cadd(s, cmulc(A[r + n * i], B[r + n * j]));
-        out[e] = s;
+        for (int r = 0; r < n; r++) s = cadd(s, cmulc(A[r + ld * i], B[r + ld * j]));
+        out[i + ld * j] = s;
     }
 }
 
-// Parallel cyclic Jacobi on the n x n Hermitian M (LDS, leading dimension n, n <= 32): the n/2 disjoint
+// Parallel cyclic Jacobi on the n x n Hermitian M (LDS, column pitch ldm, n <= 32): the n/2 disjoint
 // rotations of a round-robin round are applied in parallel (columns: lane = row; rows: lane = column) and
-// accumulated into the n columns of Q (qrows x n, leading dimension qrows, qrows <= 64). Stops when the
+// accumulated into the n columns of Q (qrows x n, column pitch ldq, qrows <= 64). Stops when the
 // strict upper triangle's Frobenius norm (the part the rotations remove; rounding leaves M Hermitian only
 // to a few ulps, and unitary rotations cannot remove that anti-Hermitian residue) is <= tol/sqrt(2) x the
 // Frobenius norm, or after 12 sweeps. A pivot below 1e-20 of its diagonal scale is left alone: rotating it
 // would only shrink it further, towards the subnormal range where its phase can no longer be normalised
 // and the rotation stops being unitary. rot: 16 entries of LDS.
-__device__ void wave_jacobi(int n, c64 *M, c64 *Q, int qrows, JacobiRot *rot, int lane, double tol) {
+__device__ void wave_jacobi(int n, c64 *M, int ldm, c64 *Q, int ldq, int qrows, JacobiRot *rot, int lane,
+                            double tol) {
     const int nn = n * n;
     const int N = n + (n & 1);
     const int npairs = N / 2;
     for (int sweep = 0; sweep < 12; sweep++) {
         double off = 0.0, tot = 0.0;
         for (int e = lane; e < nn; e += 64) {
-            const double a2 = M[e].re * M[e].re + M[e].im * M[e].im;
+            const c64 me = M[e % n + ldm * (e / n)];
+            const double a2 = me.re * me.re + me.im * me.im;
             tot += a2;
             if (e % n < e / n) off += 2.0 * a2;
         }
@@ -351,12 +357,12 @@ __device__ void wave_jacobi(int n, c64 *M, c64 *Q, int qrows, JacobiRot *rot, in
                 g.gqp = c64{0.0, 0.0};
                 g.gqq = c64{1.0, 0.0};
                 if (g.q < n) {
-                    const c64 mpq = M[g.p + n * g.q];
+                    const c64 mpq = M[g.p + ldm * g.q];
                     const double amp = hypot(mpq.re, mpq.im);
-                    const double scale = fabs(M[g.p + n * g.p].re) + fabs(M[g.q + n * g.q].re);
+                    const double scale = fabs(M[g.p + ldm * g.p].re) + fabs(M[g.q + ldm * g.q].re);
                     if (amp > 1e-20 * scale && amp > 1e-280) {
                         const double er = mpq.re / amp, ei = mpq.im / amp;
-                        const double tau = (M[g.q + n * g.q].re - M[g.p + n * g.p].re) / (2.0 * amp);
+                        const double tau = (M[g.q + ldm * g.q].re - M[g.p + ldm * g.p].re) / (2.0 * amp);
                         const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                         const double cs = 1.0 / sqrt(1.0 + t * t), sn = t * cs;
                         g.gpp = c64{cs, 0.0};
@@ -375,14 +381,14 @@ __device__ void wave_jacobi(int n, c64 *M, c64 *Q, int qrows, JacobiRot *rot, in
                     const JacobiRot g = rot[k];
                     if (g.p == g.q) continue;
                     if (lane < n) {
-                        const c64 mp = M[lane + n * g.p], mq = M[lane + n * g.q];
-                        M[lane + n * g.p] = cadd(cmul(mp, g.gpp), cmul(mq, g.gqp));
-                        M[lane + n * g.q] = cadd(cmul(mp, g.gpq), cmul(mq, g.gqq));
+                        const c64 mp = M[lane + ldm * g.p], mq = M[lane + ldm * g.q];
+                        M[lane + ldm * g.p] = cadd(cmul(mp, g.gpp), cmul(mq, g.gqp));
+                        M[lane + ldm * g.q] = cadd(cmul(mp, g.gpq), cmul(mq, g.gqq));
                     }
                     if (lane < qrows) {
-                        const c64 qp = Q[lane + qrows * g.p], qq = Q[lane + qrows * g.q];
-                        Q[lane + qrows * g.p] = cadd(cmul(qp, g.gpp), cmul(qq, g.gqp));
-                        Q[lane + qrows * g.q] = cadd(cmul(qp, g.gpq), cmul(qq, g.gqq));
+                        const c64 qp = Q[lane + ldq * g.p], qq = Q[lane + ldq * g.q];
+                        Q[lane + ldq * g.p] = cadd(cmul(qp, g.gpp), cmul(qq, g.gqp));
+                        Q[lane + ldq * g.q] = cadd(cmul(qp, g.gpq), cmul(qq, g.gqq));
                     }
                 }
             }
@@ -391,9 +397,9 @@ __device__ void wave_jacobi(int n, c64 *M, c64 *Q, int qrows, JacobiRot *rot, in
                 for (int k = 0; k < npairs; k++) {
                     const JacobiRot g = rot[k];
                     if (g.p == g.q) continue;
-                    const c64 mp = M[g.p + n * lane], mq = M[g.q + n * lane];
-                    M[g.p + n * lane] = cadd(cmulc(g.gpp, mp), cmulc(g.gqp, mq));
-                    M[g.q + n * lane] = cadd(cmulc(g.gpq, mp), cmulc(g.gqq, mq));
+                    const c64 mp = M[g.p + ldm * lane], mq = M[g.q + ldm * lane];
+                    M[g.p + ldm * lane] = cadd(cmulc(g.gpp, mp), cmulc(g.gqp, mq));
+                    M[g.q + ldm * lane] = cadd(cmulc(g.gpq, mp), cmulc(g.gqq, mq));
                 }
             }
             wave_lds_sync();
@@ -408,42 +414,42 @@ __global__ __launch_bounds__(256) void heev_refine_kernel(int n, int64_t nb, con
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t b = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
     if (b >= nb) return;
-    const int nn = n * n;
+    const int nn = n * n, ld = refine_ld(n), nl = n * ld;
     c64 *Hs = reinterpret_cast<c64 *>(lds64 + (int64_t)wave * refine_lds_doubles(n));
-    c64 *Q = Hs + nn;
-    c64 *W = Q + nn;
-    c64 *M = W + nn;
-    JacobiRot *rot = reinterpret_cast<JacobiRot *>(M + nn);  // 16 x 72 bytes
+    c64 *Q = Hs + nl;
+    c64 *W = Q + nl;
+    c64 *M = W + nl;
+    JacobiRot *rot = reinterpret_cast<JacobiRot *>(M + nl);  // 16 x 72 bytes
     int *rank_s = reinterpret_cast<int *>(rot + 16);           // n ints
     for (int e = lane; e < nn; e += 64) {
         const int i = e % n, j = e / n;
         c64 h = (i <= j) ? H[b * nn + i + (int64_t)n * j] : H[b * nn + j + (int64_t)n * i];
         if (i > j) h.im = -h.im;
         if (i == j) h.im = 0.0;
-        Hs[e] = h;
+        Hs[i + ld * j] = h;
         const c32 u = U[b * nn + e];
-        Q[e] = c64{u.re, u.im};
+        Q[i + ld * j] = c64{u.re, u.im};
     }
     wave_lds_sync();
     // 1. two Loewdin steps: S = Q^H Q (in W), M = 3/2 Q - 1/2 Q S, Q <- M
     for (int it = 0; it < 2; it++) {
-        lds_gemm_cn(n, Q, Q, W, lane);
+        lds_gemm_cn(n, ld, Q, Q, W, lane);
         wave_lds_sync();
-        lds_gemm_nn(n, Q, W, M, -0.5, 1.5, Q, lane);
+        lds_gemm_nn(n, ld, Q, W, M, -0.5, 1.5, Q, lane);
         wave_lds_sync();
-        for (int e = lane; e < nn; e += 64) Q[e] = M[e];
+        for (int e = lane; e < nn; e += 64) Q[e % n + ld * (e / n)] = M[e % n + ld * (e / n)];
         wave_lds_sync();
     }
     // 2. M = Q^H (H Q)
-    lds_gemm_nn(n, Hs, Q, W, 1.0, 0.0, nullptr, lane);
+    lds_gemm_nn(n, ld, Hs, Q, W, 1.0, 0.0, nullptr, lane);
     wave_lds_sync();
-    lds_gemm_cn(n, Q, W, M, lane);
+    lds_gemm_cn(n, ld, Q, W, M, lane);
     wave_lds_sync();
     // 3. parallel cyclic Jacobi
-    wave_jacobi(n, M, Q, n, rot, lane, tol);
+    wave_jacobi(n, M, ld, Q, ld, n, rot, lane, tol);
     // 4. ascending order
     double d = 0.0;
-    if (lane < n) d = M[lane + n * lane].re;
+    if (lane < n) d = M[lane + ld * lane].re;
     int rank = 0;
     for (int k = 0; k < n; k++) {
         const double dk = rdl64(d, k);
@@ -456,7 +462,8 @@ __global__ __launch_bounds__(256) void heev_refine_kernel(int n, int64_t nb, con
     wave_lds_sync();
     for (int e = lane; e < nn; e += 64) {
         const int r = e % n, j = e / n;
-        U[b * nn + r + (int64_t)n * rank_s[j]] = c32{static_cast<float>(Q[e].re), static_cast<float>(Q[e].im)};
+        const c64 q = Q[r + ld * j];
+        U[b * nn + r + (int64_t)n * rank_s[j]] = c32{static_cast<float>(q.re), static_cast<float>(q.im)};
     }
 }
 
@@ -502,7 +509,7 @@ __device__ void gauge_fix_group(int n, int a, int m, c32 *__restrict__ Ub, c64 *
         M[j + m * i] = c64{acc.re, -acc.im};
     }
     wave_lds_sync();
-    wave_jacobi(m, M, Ug, n, rot, lane, 1e-15);
+    wave_jacobi(m, M, m, Ug, n, n, rot, lane, 1e-15);
     const double d = lane < m ? M[lane + m * lane].re : 0.0;
     int rank = 0;
     for (int k = 0; k < m; k++) {
