@@ -257,6 +257,10 @@ def main():
     ap.add_argument("--bf16-leg", dest="bf16_leg", action="store_true", default=None,
                     help="also time the split-BF16 MFMA fast path (default: on for cfg5)")
     ap.add_argument("--no-bf16-leg", dest="bf16_leg", action="store_false")
+    ap.add_argument("--graph", dest="graph", action="store_true", default=True,
+                    help="also capture the timed k batches of the staged fast path as one HIP graph and time its "
+                         "replay (reported beside the eager steps, not as the value; default on)")
+    ap.add_argument("--no-graph", dest="graph", action="store_false")
     ap.add_argument("--no-check", dest="check", action="store_false", default=True,
                     help="skip the oracle check (full-grid filter and sampled |g|^2 of the last timed batch)")
     args = ap.parse_args()
@@ -419,6 +423,25 @@ def main():
                                              T.get_variant(T.KIND_PACKED_BF16)),
                                variant_krkq=(("gridopt_mfma" if T.get_variant(T.KIND_GRIDOPT_KQ) == 1 else "gridopt")
                                              if staged_kq else T.get_variant(T.KIND_FUSED_ROT)))
+        if args.graph and staged_rr and staged_kq and not args.pipeline:
+            # the same timed batches as one captured HIP graph: launch overhead and host work removed
+            try:
+                bg = loop.capture(timed, nks_steps)
+                bg.replay()
+                torch.cuda.synchronize()
+                t0g = time.perf_counter()
+                bg.replay()
+                torch.cuda.synchronize()
+                elg = time.perf_counter() - t0g
+                results["fast_graph"] = dict(value=nks_steps * nq_grid * len(timed) / elg,
+                                             ms_per_step=1e3 * elg / len(timed), steps=len(timed))
+                log(f"[bench] fast outer-k as one captured graph: {results['fast_graph']['value']:.3e} pairs/s "
+                    f"({results['fast_graph']['ms_per_step']:.2f} ms per batch, eager "
+                    f"{1e3 * el / args.steps:.2f} ms)")
+                del bg
+            except Exception as exc:   # capture refused (not a device fault): report, keep the eager numbers
+                results["fast_graph"] = dict(error=str(exc))
+                log(f"[bench] graph capture failed: {exc}")
         run_as = ("fp32 staged" if staged_rr and staged_kq else
                   "+".join(sorted({"fp32 staged" if st_ else args.precision for st_ in (staged_rr, staged_kq)})))
         log(f"[bench] fast outer-k ({run_as}): {results['fast']['value']:.3e} pairs/s; RR->kR "

@@ -449,6 +449,20 @@ class QBatchedAdapter(AbstractCalculator):
                 calc.end_batch()
 
 
+@dataclasses.dataclass
+class BatchGraph:
+    """A captured sequence of outer-k batches (OuterKLoop.capture): ``replay()`` launches the recorded
+    graph on the current stream. ``payloads`` are the batches' payload views into the loop's buffers as
+    the capture built them (after a replay, the last batch's is current)."""
+    graph: object
+    starts: list
+    n: int
+    payloads: list
+
+    def replay(self):
+        self.graph.replay()
+
+
 class OuterKLoop:
     """Outer-k e-ph loop on the device (the reference's _loop_eph_over_k_and_kq_batched,
     src/calculator/run_eph_over_k_and_kq.jl:584-895, with the k-side window projection of :657-659,
@@ -599,6 +613,35 @@ class OuterKLoop:
         stream = L.stream_handle()
         self._stage_rrkr(i0, n, u0, u1, self.A, stream)
         return self._stage_krkq(i0, n, u0, u1, self.A, calculators, stream)
+
+    def capture(self, starts, n: int, calculators=(), warmup: bool = True) -> "BatchGraph":
+        """Capture the device work of the k batches starting at the surviving-k positions ``starts`` (n k
+        each), with the consumers' device work, as one HIP graph on the current device (the loop of
+        src/calculator/run_eph_over_k_and_kq.jl:584-895 over those batches, launch for launch). Replaying
+        it redoes exactly those launches into the loop's buffers; nothing is read back between batches
+        (the staged path sizes every launch on the host from tables fixed at setup), so a capture sees the
+        whole batch sequence. The stage-1 plane cache is invalidated before and after the capture, so the
+        graph computes every k1 plane it reads and eager batches after it do not trust planes the graph
+        only recorded. Consumers must do device work only (no host copies, allocations or
+        synchronisation). ``warmup``: run the first batch eagerly first (first launches of each kernel and
+        the lazily allocated tables happen outside the capture)."""
+        torch = _torch()
+        starts = [int(i0) for i0 in starts]
+        if warmup and starts:
+            self.run_kbatch(starts[0], n)
+        if self.gridopt is not None:
+            self.gridopt.invalidate()
+        timer, self.ft_timer = self.ft_timer, None
+        graph = torch.cuda.CUDAGraph()
+        torch.cuda.synchronize()
+        try:
+            with torch.cuda.graph(graph):
+                payloads = [self.run_kbatch(i0, n, calculators) for i0 in starts]
+        finally:
+            self.ft_timer = timer
+            if self.gridopt is not None:
+                self.gridopt.invalidate()
+        return BatchGraph(graph, starts, n, payloads)
 
     def run_pipelined(self, starts, n: int, calculators=()):
         """Run the k batches starting at the surviving-k positions ``starts`` (n k each) on two streams of
@@ -841,6 +884,12 @@ class _GridOptKR:
         self.plane_x1 = {p: float(x1[starts[l]]) for p, l in zip(self.planes, first_line)}
         self.plane_pos = {p: i for i, p in enumerate(self.planes)}
         self.cache = []  # [group, {plane: buffer}, buffers], most recent last, at most two
+
+    def invalidate(self):
+        """Forget which k1 planes the cached stage-1 buffers hold (the buffers are kept for reuse): the next
+        batch recomputes every plane it needs."""
+        for entry in self.cache:
+            entry[0], entry[1] = None, {}
 
     def _plane_buffer(self, plane, epmat_el, stream):
         import ctypes as C
