@@ -392,7 +392,7 @@ __global__ __launch_bounds__(256) void ft_mfma_db_kernel(int64_t ndata, int nr, 
 // Kernel variant per contraction kind, set by the tuner (ephip_ft_set_variant). Kinds:
 // 0 = FP32 wide (nk > 64), 1 = FP32 narrow (nk <= 64), 2 = FP64. Variants: 0 = single-buffered,
 // 1 = double-buffered KC 16, 2 = double-buffered KC 8.
-static int g_ft_variant[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+static int g_ft_variant[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
 template <class T, int TN>
 static void launch_ft(int variant, dim3 g, hipStream_t s, int64_t ndata, int nr, const cplx<T> *op, int64_t ldop,
@@ -1864,15 +1864,15 @@ extern "C" int ephip_fourier_batched_c32(int64_t ndata, int32_t nr, const void *
 }
 
 extern "C" int ephip_ft_set_variant(int32_t kind, int32_t variant) {
-    static const int nvar[8] = {3, 3, 3, 8, 13, 2, 2, 2};
-    EPHIP_REQUIRE(kind >= 0 && kind < 8, "ft_set_variant: kind %d outside 0..7", kind);
+    static const int nvar[9] = {3, 3, 3, 8, 13, 2, 2, 2, 2};
+    EPHIP_REQUIRE(kind >= 0 && kind < 9, "ft_set_variant: kind %d outside 0..8", kind);
     EPHIP_REQUIRE(variant >= 0 && variant < nvar[kind], "ft_set_variant: variant %d outside the range of kind %d",
                   variant, kind);
     g_ft_variant[kind] = variant;
     return EPHIP_OK;
 }
 
-extern "C" int ephip_ft_get_variant(int32_t kind) { return (kind >= 0 && kind < 8) ? g_ft_variant[kind] : -1; }
+extern "C" int ephip_ft_get_variant(int32_t kind) { return (kind >= 0 && kind < 9) ? g_ft_variant[kind] : -1; }
 
 extern "C" int ephip_ft_pack_operand(int64_t ndata, int32_t nr, const void *op, int64_t ldop, int32_t nunits,
                                      int64_t op_stride, void *apk, void *stream) {

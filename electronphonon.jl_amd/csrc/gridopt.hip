@@ -656,6 +656,40 @@ __device__ __forceinline__ void kq_stage1(float (&br)[N][N], float (&bi)[N][N], 
     }
 }
 
+// Symmetric boxes (N = 2h + 1 entries along an axis at r = -h .. h, phases e[h + j] = e^{2 pi i q j / m}):
+//   sum_b e[b] B[b] = B[h] + sum_{j=1..h} ( cos_j (B[h+j] + B[h-j]) + i sin_j (B[h+j] - B[h-j]) ),
+// with (cos_j, sin_j) = e[h + j]: h real-times-complex pairs (4 FMAs each) instead of N complex
+// multiply-adds. kq_fold stores S_j = B[h+j] + B[h-j] at h + j and D_j = B[h+j] - B[h-j] at h - j.
+template <int N>
+__device__ __forceinline__ void kq_fold(float (&vr)[N], float (&vi)[N]) {
+    constexpr int H = N / 2;
+#pragma unroll
+    for (int j = 1; j <= H; j++) {
+        const float pr = vr[H + j], pi = vi[H + j], mr = vr[H - j], mi = vi[H - j];
+        vr[H + j] = pr + mr;
+        vi[H + j] = pi + mi;
+        vr[H - j] = pr - mr;
+        vi[H - j] = pi - mi;
+    }
+}
+
+// sum_b e[b] B[b] from the folded B (kq_fold) and the phases e[H + j] of the positive offsets
+template <int N>
+__device__ __forceinline__ void kq_folded_sum(float &sr, float &si, const float2 *e, const float (&vr)[N],
+                                              const float (&vi)[N]) {
+    constexpr int H = N / 2;
+    sr = vr[H];
+    si = vi[H];
+#pragma unroll
+    for (int j = 1; j <= H; j++) {
+        const float c = e[H + j].x, sn = e[H + j].y;
+        sr = fmaf(c, vr[H + j], sr);
+        sr = fmaf(-sn, vi[H - j], sr);
+        si = fmaf(c, vi[H + j], si);
+        si = fmaf(sn, vr[H - j], si);
+    }
+}
+
 // Workgroup order of the staged kR->kq kernels (grid m1 x nunits). Workgroups are dispatched round-robin
 // over the 8 XCDs in linear order (x fastest), so in the plain order (q1 = blockIdx.x, unit = blockIdx.y)
 // the m1 q1-workgroups of one unit spread over every XCD and each XCD's L2 fetches the unit's operator
@@ -676,7 +710,8 @@ __device__ __forceinline__ void kq_block(int grouped, int &q1, int &u) {
     u = (int)(id / gridDim.x);
 }
 
-template <int N, int NBW, int NMC = 0, bool DENSE = false>  // NMC: compile-time mode count (12), or 0 for nm <= 16
+// SYM: the Rp box spans -h .. h along axes 2 and 3 (N = 2h + 1): stages 2 and 3 in the folded form.
+template <int N, int NBW, int NMC = 0, bool DENSE = false, bool SYM = false>  // NMC: compile-time mode count (12), or 0
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 : 1))) void gridopt_kq_rotate_kernel(
     int nm, int n1, int p1min, int p2min, int p3min, const int *__restrict__ rmap, int s1, int s2, int s3, int m1,
     int m2, int m3,
@@ -741,6 +776,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
     // stage 1
     float br[N][N], bi[N][N];
     kq_stage1<N, DENSE>(br, bi, A + (int64_t)u * lda, rv ? t : 0, ldr, n1, rmap, ph1, s1, s2, s3);
+    if constexpr (SYM) {
+#pragma unroll
+        for (int c = 0; c < N; c++) kq_fold<N>(br[c], bi[c]);  // over b, for the stage-2 sums
+    }
     const int lane = t & 63;
     int e0 = 0, lc = 0;  // chunk start in the slab tables and the line that holds it
     while (e0 < nslab) {  // uniform
@@ -796,11 +835,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
 #pragma unroll
                 for (int c = 0; c < N; c++) {
                     float sr = 0.f, si = 0.f;
+                    if constexpr (SYM) {
+                        kq_folded_sum<N>(sr, si, e2, br[c], bi[c]);
+                    } else {
 #pragma unroll
-                    for (int b = 0; b < N; b++) cmac(sr, si, e2[b].x, e2[b].y, br[c][b], bi[c][b]);
+                        for (int b = 0; b < N; b++) cmac(sr, si, e2[b].x, e2[b].y, br[c][b], bi[c][b]);
+                    }
                     b2r[c] = sr;
                     b2i[c] = si;
                 }
+                if constexpr (SYM) kq_fold<N>(b2r, b2i);  // over c, for the stage-3 sums of the line
             }
             float2 e3[NE];
 #pragma unroll
@@ -810,8 +854,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
                 e3[2 * h + 1] = make_float2(v.z, v.w);
             }
             float gr = 0.f, gi = 0.f;
+            if constexpr (SYM) {
+                kq_folded_sum<N>(gr, gi, e3, b2r, b2i);
+            } else {
 #pragma unroll
-            for (int c = 0; c < N; c++) cmac(gr, gi, e3[c].x, e3[c].y, b2r[c], b2i[c]);
+                for (int c = 0; c < N; c++) cmac(gr, gi, e3[c].x, e3[c].y, b2r[c], b2i[c]);
+            }
             // conj(w) g summed over the 16 lanes of the row. Even widths: the column pairs, lanes 4s of
             // the row write value s of each pair (re, im of its first column, re, im of its second). Odd
             // widths: one column when the k+q window holds one band (uniform; most kept q), lanes 0 and 8
@@ -925,7 +973,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
 // (zeros past the k+q window).
 constexpr int GQM_TQ = 48, GQM_UC = 32;
 
-template <int N, int NMC = 0, bool DENSE = false>  // NMC: compile-time mode count (12), or 0 for nm <= 16
+// SYM: the Rp box spans -h .. h along axis 2 (N = 2h + 1): stage 2 in the folded form.
+template <int N, int NMC = 0, bool DENSE = false, bool SYM = false>  // NMC: compile-time mode count (12), or 0
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 : 2))) void
 gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min, const int *__restrict__ rmap,
                               int s1, int s2, int s3, int m1, int m2, int m3, const cplx<float> *__restrict__ A, int64_t lda, int64_t ldr,
@@ -995,6 +1044,10 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
     // stage 1 (rows G = t)
     float br[N][N], bi[N][N];
     kq_stage1<N, DENSE>(br, bi, A + (int64_t)u * lda, rv ? t : 0, ldr, n1, rmap, ph1, s1, s2, s3);
+    if constexpr (SYM) {
+#pragma unroll
+        for (int c = 0; c < N; c++) kq_fold<N>(br[c], bi[c]);  // over b, for the stage-2 sums
+    }
     // the phonon rotation and g2 of the collected columns: ep[nu'] = sum_nu t[nu] uph(q)[nu, nu']
     int tqn = 0;
     auto flush = [&]() {
@@ -1056,8 +1109,12 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
 #pragma unroll
             for (int c = 0; c < N; c++) {
                 float sr = 0.f, si = 0.f;
+                if constexpr (SYM) {
+                    kq_folded_sum<N>(sr, si, e2, br[c], bi[c]);
+                } else {
 #pragma unroll
-                for (int b = 0; b < N; b++) cmac(sr, si, e2[b].x, e2[b].y, br[c][b], bi[c][b]);
+                    for (int b = 0; b < N; b++) cmac(sr, si, e2[b].x, e2[b].y, br[c][b], bi[c][b]);
+                }
                 b2s[c][(t >> 4) + BP * (t & 15)] = make_float2(sr, si);
             }
         }
@@ -1191,6 +1248,8 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
         EPHIP_REQUIRE(s1 >= 0 && s2 >= 0 && s3 >= 0 && (int64_t)(s1 + s2 + s3) * (np - 1) < ((int64_t)1 << 31),
                       "gridopt_kq_rotate: bad dense-box strides");
     const int grouped = ephip_ft_get_variant(7) == 1;  // XCD-grouped workgroup order
+    // folded stage-2/3 sums when the Rp box spans -h .. h along axes 2 and 3 (odd np, tuner kind 8 = 0)
+    const bool sym = (np & 1) && p2min == -(np / 2) && p3min == -(np / 2) && ephip_ft_get_variant(8) == 0;
     if (ephip_ft_get_variant(6) == 1) {  // stage 3 and the k+q-window rotation on MFMAs
         using KM = decltype(&gridopt_kq_rotate_mfma_kernel<3>);
 #define GQM_ROW(c, d) {gridopt_kq_rotate_mfma_kernel<3, c, d>, gridopt_kq_rotate_mfma_kernel<4, c, d>, \
@@ -1199,9 +1258,16 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
         static const KM mtable[2][2][6] = {{GQM_ROW(0, false), GQM_ROW(12, false)},
                                            {GQM_ROW(0, true), GQM_ROW(12, true)}};
 #undef GQM_ROW
+#define GQMS_ROW(c, d) {gridopt_kq_rotate_mfma_kernel<3, c, d, true>, gridopt_kq_rotate_mfma_kernel<5, c, d, true>, \
+                        gridopt_kq_rotate_mfma_kernel<7, c, d, true>}
+        static const KM mstable[2][2][3] = {{GQMS_ROW(0, false), GQMS_ROW(12, false)},
+                                            {GQMS_ROW(0, true), GQMS_ROW(12, true)}};
+#undef GQMS_ROW
         const int ne = (np + 1) & ~1;
         const size_t lds = (size_t)(qgrid[1] + qgrid[2]) * (ne / 2) * 16 + 2 * (size_t)qgrid[1] * qgrid[2] * 4;
-        mtable[dense ? 1 : 0][nm == 12 ? 1 : 0][np - 3]<<<g, 256, lds, as_stream(stream)>>>(
+        const KM km = (sym && np <= 7) ? mstable[dense ? 1 : 0][nm == 12 ? 1 : 0][(np - 3) / 2]
+                                       : mtable[dense ? 1 : 0][nm == 12 ? 1 : 0][np - 3];
+        km<<<g, 256, lds, as_stream(stream)>>>(
             nm, n1, p1min, p2min, p3min, rmap, s1, s2, s3, qgrid[0], qgrid[1], qgrid[2],
             static_cast<const cplx<float> *>(A), lda, ldr, line_off, col_index, ikq, cs,
             static_cast<const cplx<float> *>(U), bmin, bmax, static_cast<const cplx<float> *>(uph), omega, nbw, g2,
@@ -1221,7 +1287,17 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
         {{GQ_ROW(2, 0, true), GQ_ROW(3, 0, true), GQ_ROW(4, 0, true)},
          {GQ_ROW(2, 12, true), GQ_ROW(3, 12, true), GQ_ROW(4, 12, true)}}};
 #undef GQ_ROW
-    const K kern = table[dense ? 1 : 0][nm == 12 ? 1 : 0][nbw <= 2 ? 0 : nbw - 2][np - 3];
+#define GQS_ROW(nb, c, d) {gridopt_kq_rotate_kernel<3, nb, c, d, true>, gridopt_kq_rotate_kernel<5, nb, c, d, true>, \
+                           gridopt_kq_rotate_kernel<7, nb, c, d, true>}
+    static const K stable[2][2][3][3] = {
+        {{GQS_ROW(2, 0, false), GQS_ROW(3, 0, false), GQS_ROW(4, 0, false)},
+         {GQS_ROW(2, 12, false), GQS_ROW(3, 12, false), GQS_ROW(4, 12, false)}},
+        {{GQS_ROW(2, 0, true), GQS_ROW(3, 0, true), GQS_ROW(4, 0, true)},
+         {GQS_ROW(2, 12, true), GQS_ROW(3, 12, true), GQS_ROW(4, 12, true)}}};
+#undef GQS_ROW
+    const K kern = (sym && np <= 7)
+                       ? stable[dense ? 1 : 0][nm == 12 ? 1 : 0][nbw <= 2 ? 0 : nbw - 2][(np - 3) / 2]
+                       : table[dense ? 1 : 0][nm == 12 ? 1 : 0][nbw <= 2 ? 0 : nbw - 2][np - 3];
     kern<<<g, threads, 0, as_stream(stream)>>>(nm, n1, p1min, p2min, p3min, rmap, s1, s2, s3, qgrid[0], qgrid[1],
                                                 qgrid[2], static_cast<const cplx<float> *>(A), lda, ldr, line_off,
                                                 col_index, ikq, cs, static_cast<const cplx<float> *>(U), bmin, bmax,

@@ -781,7 +781,17 @@ class OuterKLoop:
             gq = self.gridopt_q
             n2n3 = int((gq.rmap.view(gq.np, gq.n1, 8) >= 0).any(1).sum().item())  # (c, b) columns present
             from . import tune as T
-            if T.get_variant(T.KIND_GRIDOPT_KQ) == 1:
+            mfma_kq = T.get_variant(T.KIND_GRIDOPT_KQ) == 1
+            if gq.folded():
+                # stages 2 (and 3 in the VALU form) as pair sums over opposite offsets: h = np // 2 pairs of
+                # 4 FMAs (one complex multiply-add's worth) per sum, plus the folds (4 adds per pair)
+                h = gq.np // 2
+                f_fold = self.nrp * gq.m[0] * nu + 0.5 * gq.np * h * gq.m[0] * nu
+                if mfma_kq:
+                    f_kq = 8.0 * nd2 * (f_fold + gq.np * h * gq.nlines * nu + gq.np * slots) + 8.0 * nm * nm * slots
+                else:
+                    f_kq = 8.0 * nd2 * (f_fold + (gq.np * h + 0.5 * h) * gq.nlines * nu + h * kept) + rot
+            elif mfma_kq:
                 # stage 3 and the k+q-window rotation as one contraction over (i, c) per window band slot
                 f_kq = (8.0 * nd2 * (self.nrp * gq.m[0] * nu + n2n3 * gq.nlines * nu + gq.np * slots)
                         + 8.0 * nm * nm * slots)
@@ -988,6 +998,14 @@ class _GridOptKQ:
         self.nlines = self.m[0] * self.m[1]
         import ctypes
         self.qgrid_c = (ctypes.c_int32 * 3)(*self.m)
+
+    def folded(self) -> bool:
+        """The kernel's folded stage-2/3 sums apply: the Rp box spans -h .. h along axes 2 and 3 (odd np)
+        and tuner kind 8 is at its default."""
+        from . import tune as T
+        h = self.np // 2
+        return (self.np % 2 == 1 and self.pmin[1] == -h and self.pmin[2] == -h
+                and T.get_variant(T.KIND_KQ_FOLD) == 0)
 
     def run(self, loop, nu, A, stream):
         import ctypes as C

@@ -19,6 +19,9 @@ GRIDOPT_KQ_VARIANT_NAMES = {0: "VALU stage 3 + DPP window sums", 1: "MFMA stage 
 KIND_KQ_ORDER = 7
 KQ_ORDER_VARIANTS = (0, 1)
 KQ_ORDER_VARIANT_NAMES = {0: "q1 fastest, unit by unit", 1: "XCD-grouped units"}
+KIND_KQ_FOLD = 8
+KQ_FOLD_VARIANTS = (0, 1)
+KQ_FOLD_VARIANT_NAMES = {0: "folded stage-2/3 sums on symmetric boxes", 1: "general stage-2/3 sums"}
 VARIANTS = (0, 1, 2)
 VARIANT_NAMES = {0: "single-buffered KC16", 1: "double-buffered KC16", 2: "double-buffered KC8"}
 PACKED_VARIANTS = (0, 1, 2, 3, 4, 5, 6, 7)

@@ -359,7 +359,10 @@ int ephip_eph_kq_rotate_packed_bf16x3(int32_t nw, int32_t nm, int32_t nbw, int32
  * ahead, 1 = two stages ahead. kind 6 = the staged kR->kq kernel (ephip_gridopt_kq_rotate); variant 0 =
  * stage 3 and the k+q-window rotation on the vector ALUs, 1 = as one contraction on FP32 MFMAs. kind 7 =
  * the staged kR->kq kernel's workgroup order; variant 0 = q1 fastest, unit by unit, 1 = XCD-grouped (each
- * XCD takes one contiguous run of (unit, q1) pairs, so a unit's workgroups share one L2).
+ * XCD takes one contiguous run of (unit, q1) pairs, so a unit's workgroups share one L2). kind 8 = the staged
+ * kR->kq kernel's stage-2/3 sums; variant 0 = the folded form (pairs of opposite offsets, cos and sin
+ * factors) when the Rp box spans -h .. h along axes 2 and 3 (odd np), the general form otherwise;
+ * 1 = always the general form.
  * Process-wide setting (no counterpart in the reference, whose tuning constants are hard-coded:
  * src/wannier_to_bloch_batched.jl:397-406). Host-side, synchronous. */
 int ephip_ft_set_variant(int32_t kind, int32_t variant);

@@ -21,17 +21,20 @@ def cfg1():
     return m, g
 
 
-@pytest.fixture(params=[(0, 0), (1, 0), (1, 1)], ids=["kq-valu", "kq-mfma", "kq-mfma-xcd"])
+@pytest.fixture(params=[(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 0, 1), (1, 0, 1)],
+                ids=["kq-valu", "kq-mfma", "kq-mfma-xcd", "kq-valu-general", "kq-mfma-general"])
 def kq_variant(request):
     """The staged kR->kq kernel form (tuner kind 6): 0 = stage 3 and the k+q-window rotation on the vector
-    ALUs, 1 = on FP32 MFMAs; and its workgroup order (kind 7): 0 = plain, 1 = XCD-grouped."""
+    ALUs, 1 = on FP32 MFMAs; its workgroup order (kind 7): 0 = plain, 1 = XCD-grouped; and its stage-2/3
+    sums (kind 8): 0 = folded on symmetric Rp boxes, 1 = always the general form."""
     from ephip import tune
-    old = tune.get_variant(tune.KIND_GRIDOPT_KQ), tune.get_variant(tune.KIND_KQ_ORDER)
-    tune.set_variant(tune.KIND_GRIDOPT_KQ, request.param[0])
-    tune.set_variant(tune.KIND_KQ_ORDER, request.param[1])
+    kinds = (tune.KIND_GRIDOPT_KQ, tune.KIND_KQ_ORDER, tune.KIND_KQ_FOLD)
+    old = [tune.get_variant(k) for k in kinds]
+    for k, v in zip(kinds, request.param):
+        tune.set_variant(k, v)
     yield request.param[0]
-    tune.set_variant(tune.KIND_GRIDOPT_KQ, old[0])
-    tune.set_variant(tune.KIND_KQ_ORDER, old[1])
+    for k, v in zip(kinds, old):
+        tune.set_variant(k, v)
 
 
 def _oracle_states(m, g):
