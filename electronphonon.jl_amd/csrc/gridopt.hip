@@ -26,6 +26,40 @@ __device__ __forceinline__ void cmac(float &sr, float &si, float ar, float ai, f
     si = fmaf(ai, br, si);
 }
 
+// Symmetric boxes (N = 2h + 1 entries along an axis at r = -h .. h, phases e[h + j] = e^{2 pi i q j / m}):
+//   sum_b e[b] B[b] = B[h] + sum_{j=1..h} ( cos_j (B[h+j] + B[h-j]) + i sin_j (B[h+j] - B[h-j]) ),
+// with (cos_j, sin_j) = e[h + j]: h real-times-complex pairs (4 FMAs each) instead of N complex
+// multiply-adds. kq_fold stores S_j = B[h+j] + B[h-j] at h + j and D_j = B[h+j] - B[h-j] at h - j.
+template <int N>
+__device__ __forceinline__ void kq_fold(float (&vr)[N], float (&vi)[N]) {
+    constexpr int H = N / 2;
+#pragma unroll
+    for (int j = 1; j <= H; j++) {
+        const float pr = vr[H + j], pi = vi[H + j], mr = vr[H - j], mi = vi[H - j];
+        vr[H + j] = pr + mr;
+        vi[H + j] = pi + mi;
+        vr[H - j] = pr - mr;
+        vi[H - j] = pi - mi;
+    }
+}
+
+// sum_b e[b] B[b] from the folded B (kq_fold) and the phases e[H + j] of the positive offsets
+template <int N>
+__device__ __forceinline__ void kq_folded_sum(float &sr, float &si, const float2 *e, const float (&vr)[N],
+                                              const float (&vi)[N]) {
+    constexpr int H = N / 2;
+    sr = vr[H];
+    si = vi[H];
+#pragma unroll
+    for (int j = 1; j <= H; j++) {
+        const float c = e[H + j].x, sn = e[H + j].y;
+        sr = fmaf(c, vr[H + j], sr);
+        sr = fmaf(-sn, vi[H - j], sr);
+        si = fmaf(c, vi[H + j], si);
+        si = fmaf(sn, vr[H - j], si);
+    }
+}
+
 constexpr int GO_N1_MAX = 32;
 
 // Stage 1. One thread per operand row d; op1 rows are written in the j-fastest order
@@ -326,7 +360,8 @@ __global__ __launch_bounds__(256) void gridopt_kR_tables_kernel(int nw, int np, 
 // (gridopt_kR_tables_kernel) into LDS buffers that alternate between lines, so one barrier per line
 // separates a line's staging from the previous line's reads; partial tiles alternate likewise, one
 // barrier per tile.
-template <int N>
+// SYM: the Re box spans -h .. h along axis 2 (N = 2h + 1): the stage-2 sums in the folded form (kq_fold).
+template <int N, bool SYM = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 : 2))) void
 gridopt_kR_project_mfma_kernel(int64_t nd, const cplx<float> *__restrict__ op1, int64_t k0, int64_t k1, int l0,
                                int l1, int lb0, const int *__restrict__ line_start, int max_pairs,
@@ -353,6 +388,10 @@ gridopt_kR_project_mfma_kernel(int64_t nd, const cplx<float> *__restrict__ op1, 
             vr[c][b] = v.re;
             vi[c][b] = v.im;
         }
+    if constexpr (SYM) {
+#pragma unroll
+        for (int c = 0; c < N; c++) kq_fold<N>(vr[c], vi[c]);  // over b, for the stage-2 sums
+    }
     const int ku0 = kus[k0];
     int lb = 0;  // line buffer
     for (int l = l0; l < l1; l++) {
@@ -387,10 +426,14 @@ gridopt_kR_project_mfma_kernel(int64_t nd, const cplx<float> *__restrict__ op1, 
 #pragma unroll
         for (int c = 0; c < N; c++) {
             float sr = 0.f, si = 0.f;
+            if constexpr (SYM) {
+                kq_folded_sum<N>(sr, si, ph2[lb], vr[c], vi[c]);
+            } else {
 #pragma unroll
-            for (int b = 0; b < N; b++) {
-                const float2 p = ph2[lb][b];
-                cmac(sr, si, p.x, p.y, vr[c][b], vi[c][b]);
+                for (int b = 0; b < N; b++) {
+                    const float2 p = ph2[lb][b];
+                    cmac(sr, si, p.x, p.y, vr[c][b], vi[c][b]);
+                }
             }
             o2r[c] = sr;
             o2i[c] = si;
@@ -531,8 +574,8 @@ extern "C" int ephip_gridopt_kR_tables(int32_t nw, int32_t np, int32_t r2min, in
     return EPHIP_OK;
 }
 
-extern "C" int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp, int32_t np, const void *op1,
-                                             int64_t k0, int64_t k1, int32_t l0, int32_t l1, int32_t lb0,
+extern "C" int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp, int32_t np, int32_t r2min,
+                                             const void *op1, int64_t k0, int64_t k1, int32_t l0, int32_t l1, int32_t lb0,
                                              const int32_t *line_start, int32_t max_line_k,
                                              int32_t max_line_units, const int32_t *kus, const void *ucol,
                                              const void *ph2, const void *ph3, int32_t nbw, void *A, int64_t lda,
@@ -553,11 +596,15 @@ extern "C" int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp
     static const K kern[6] = {gridopt_kR_project_mfma_kernel<3>, gridopt_kR_project_mfma_kernel<4>,
                               gridopt_kR_project_mfma_kernel<5>, gridopt_kR_project_mfma_kernel<6>,
                               gridopt_kR_project_mfma_kernel<7>, gridopt_kR_project_mfma_kernel<8>};
+    static const K kern_sym[3] = {gridopt_kR_project_mfma_kernel<3, true>, gridopt_kR_project_mfma_kernel<5, true>,
+                                  gridopt_kR_project_mfma_kernel<7, true>};
+    // folded stage-2 sums when the Re box spans -h .. h along axis 2 (odd np, tuner kind 8 = 0)
+    const bool sym = (np & 1) && np <= 7 && r2min == -(np / 2) && ephip_ft_get_variant(8) == 0;
     EPHIP_REQUIRE(max_line_units >= 0 && max_line_units <= max_line_k * nbw,
                   "gridopt_kR_project_mfma: max_line_units %d > max_line_k * nbw", max_line_units);
     const int max_pairs = max_line_units > 0 ? max_line_units : max(max_line_k, 1) * nbw;
     const size_t lds = 2 * (size_t)max_pairs * nw * sizeof(float2);  // U columns of a line's pairs, two buffers
-    kern[np - 3]<<<(unsigned)(nd / 256), 256, lds, as_stream(stream)>>>(
+    (sym ? kern_sym[(np - 3) / 2] : kern[np - 3])<<<(unsigned)(nd / 256), 256, lds, as_stream(stream)>>>(
         nd, static_cast<const cplx<float> *>(op1), k0, k1, l0, l1, lb0, line_start, max_pairs, kus,
         static_cast<const cplx<float> *>(ucol), static_cast<const cplx<float> *>(ph2),
         static_cast<const cplx<float> *>(ph3), nbw, static_cast<cplx<float> *>(A), lda);
@@ -653,40 +700,6 @@ __device__ __forceinline__ void kq_stage1(float (&br)[N][N], float (&bi)[N][N], 
 #pragma unroll
                 for (int b = 0; b < N; b++) cmac(br[c][b], bi[c][b], p.x, p.y, v[b].re, v[b].im);
             }
-    }
-}
-
-// Symmetric boxes (N = 2h + 1 entries along an axis at r = -h .. h, phases e[h + j] = e^{2 pi i q j / m}):
-//   sum_b e[b] B[b] = B[h] + sum_{j=1..h} ( cos_j (B[h+j] + B[h-j]) + i sin_j (B[h+j] - B[h-j]) ),
-// with (cos_j, sin_j) = e[h + j]: h real-times-complex pairs (4 FMAs each) instead of N complex
-// multiply-adds. kq_fold stores S_j = B[h+j] + B[h-j] at h + j and D_j = B[h+j] - B[h-j] at h - j.
-template <int N>
-__device__ __forceinline__ void kq_fold(float (&vr)[N], float (&vi)[N]) {
-    constexpr int H = N / 2;
-#pragma unroll
-    for (int j = 1; j <= H; j++) {
-        const float pr = vr[H + j], pi = vi[H + j], mr = vr[H - j], mi = vi[H - j];
-        vr[H + j] = pr + mr;
-        vi[H + j] = pi + mi;
-        vr[H - j] = pr - mr;
-        vi[H - j] = pi - mi;
-    }
-}
-
-// sum_b e[b] B[b] from the folded B (kq_fold) and the phases e[H + j] of the positive offsets
-template <int N>
-__device__ __forceinline__ void kq_folded_sum(float &sr, float &si, const float2 *e, const float (&vr)[N],
-                                              const float (&vi)[N]) {
-    constexpr int H = N / 2;
-    sr = vr[H];
-    si = vi[H];
-#pragma unroll
-    for (int j = 1; j <= H; j++) {
-        const float c = e[H + j].x, sn = e[H + j].y;
-        sr = fmaf(c, vr[H + j], sr);
-        sr = fmaf(-sn, vi[H - j], sr);
-        si = fmaf(c, vi[H + j], si);
-        si = fmaf(sn, vr[H - j], si);
     }
 }
 

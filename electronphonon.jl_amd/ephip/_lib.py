@@ -79,7 +79,8 @@ SIGNATURES = {
                                  _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i64, _vp],
     "ephip_gridopt_kR_tables": [_i32, _i32, _i32, _i32, _vp, _i64, _i64, _i32, _i32, _vp, _i32, _vp, _vp, _vp, _vp,
                                 _vp, _vp, _vp],
-    "ephip_gridopt_kR_project_mfma": [_i32, _i32, _i32, _i32, _vp, _i64, _i64, _i32, _i32, _i32, _vp, _i32, _i32,
+    "ephip_gridopt_kR_project_mfma": [_i32, _i32, _i32, _i32, _i32, _vp, _i64, _i64, _i32, _i32, _i32, _vp, _i32,
+                                      _i32,
                                       _vp, _vp, _vp, _vp, _i32, _vp, _i64, _vp],
     "ephip_ft_pack_phases": [_i32, _vp, _i64, _vp, _vp, _vp],
     "ephip_kq_line_offsets": [_i32, _vp, _vp, _i64, _i32, _i32, _vp, _vp],

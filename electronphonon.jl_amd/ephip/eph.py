@@ -772,8 +772,11 @@ class OuterKLoop:
             go = self.gridopt
             planes = go.line_plane[go.line_of_k[i0:i0 + n]]
             per_plane = np.bincount(go.line_plane[go.line_of_k], minlength=int(planes.max()) + 1)
+            # stage 2 per line: n2 n3 complex multiply-adds, or in the folded form h pairs (one multiply-add's
+            # worth each) per c over the padded np x np face, the fold's adds counted once per workgroup
+            s2 = go.np * (go.np // 2) if go.folded() else go.n2 * go.n3
             f_rr = 8.0 * self.nd_el * (self.nre * float(np.sum(1.0 / per_plane[planes]))
-                                       + go.n2 * go.n3 * (int(go.line_of_k[i0 + n - 1]) - int(go.line_of_k[i0]) + 1)
+                                       + s2 * (int(go.line_of_k[i0 + n - 1]) - int(go.line_of_k[i0]) + 1)
                                        + go.n3 * n + nu)
         else:
             f_rr = 8.0 * self.nd_el * self.nre * n
@@ -895,6 +898,13 @@ class _GridOptKR:
         self.plane_pos = {p: i for i, p in enumerate(self.planes)}
         self.cache = []  # [group, {plane: buffer}, buffers], most recent last, at most two
 
+    def folded(self) -> bool:
+        """The MFMA projection kernel's folded stage-2 sums apply: the Re box spans -h .. h along axis 2
+        (odd np) and tuner kind 8 is at its default."""
+        from . import tune as T
+        return (self.mfma and self.np % 2 == 1 and self.np <= 7 and self.rmin[1] == -(self.np // 2)
+                and T.get_variant(T.KIND_KQ_FOLD) == 0)
+
     def invalidate(self):
         """Forget which k1 planes the cached stage-1 buffers hold (the buffers are kept for reuse): the next
         batch recomputes every plane it needs."""
@@ -948,7 +958,8 @@ class _GridOptKR:
                 l_end += 1
             buf = self._plane_buffer(int(plane), loop.epmat_el, stream)
             if self.mfma:
-                L.call("ephip_gridopt_kR_project_mfma", st.nw, st.nm, loop.nrp, self.np, L.ptr(buf), i0, i0 + n, l,
+                L.call("ephip_gridopt_kR_project_mfma", st.nw, st.nm, loop.nrp, self.np, self.rmin[1], L.ptr(buf),
+                       i0, i0 + n, l,
                        l_end, l_first, L.ptr(self.line_start), self.max_line_k, self.max_line_units,
                        L.ptr(loop.k_unit_start_dev),
                        L.ptr(ucol), L.ptr(ph2), L.ptr(ph3), loop.max_units_per_k, L.ptr(A), A.stride(0), stream)

@@ -287,12 +287,15 @@ int ephip_gridopt_kR_project(int32_t nw, int32_t nm, int32_t nrp, int32_t np, in
  * e^{2 pi i k2 (r2min + b)} of each line. ephip_gridopt_kR_project_mfma then covers the lines
  * lb0 <= l0 <= l < l1 of one k1 plane (as ephip_gridopt_kR_project; its units are consecutive, at most
  * nbw per k). max_line_units (<= max_line_k*nbw; 0 means max_line_k*nbw) bounds the units of any line and
- * sizes the kernel's LDS. Output as ephip_gridopt_kR_project. */
+ * sizes the kernel's LDS. r2min: the box's first r2 (the tables' r2min); when the box spans -h .. h along
+ * axis 2 (odd np) the stage-2 sums run in the folded form (pairs of opposite offsets; tuner kind 8).
+ * Output as ephip_gridopt_kR_project. */
 int ephip_gridopt_kR_tables(int32_t nw, int32_t np, int32_t r2min, int32_t r3min, const double *xk, int64_t k0,
                             int64_t k1, int32_t lb0, int32_t lb1, const int32_t *line_start, int32_t nunits,
                             const int64_t *ustate, const int32_t *uband, const void *U, void *ucol, void *ph2,
                             void *ph3, void *stream);
-int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp, int32_t np, const void *op1, int64_t k0,
+int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp, int32_t np, int32_t r2min, const void *op1,
+                                  int64_t k0,
                                   int64_t k1, int32_t l0, int32_t l1, int32_t lb0, const int32_t *line_start,
                                   int32_t max_line_k, int32_t max_line_units, const int32_t *kus, const void *ucol,
                                   const void *ph2, const void *ph3, int32_t nbw, void *A, int64_t lda,
