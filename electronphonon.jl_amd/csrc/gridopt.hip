@@ -703,6 +703,67 @@ __device__ __forceinline__ void kq_stage1(float (&br)[N][N], float (&bi)[N][N], 
     }
 }
 
+// Stage 1 for every q1 of a batch's units at once, ahead of the kR->kq kernel (cube boxes, n1 = N):
+//   b1[((u m1 + q1) N^2 + b + N c) nd2 + G] = sum_a e^{2 pi i q1 (p1min + a) / m1} A_u[G, Rp(a, b, c)]
+// (zero for absent Rp). One thread per (unit, (b, c), G), G fastest: the N operator values of its column
+// are loaded once and feed all m1 sums (folded over a when r1 spans -h .. h); the kR->kq workgroup of
+// (unit, q1) then reads its 75 KB slice instead of forming stage 1 from the unit's 527 KB operator.
+constexpr int GQ_PRE_M1_MAX = 64;
+template <int N, bool DENSE, bool SYM>
+__global__ __launch_bounds__(256) void gridopt_kq_stage1_all_kernel(int nunits, int nd2, int m1, int p1min,
+                                                                    const int *__restrict__ rmap, int s1, int s2,
+                                                                    int s3, const cplx<float> *__restrict__ A,
+                                                                    int64_t lda, int64_t ldr, float2 *__restrict__ b1) {
+    __shared__ float2 e1[GQ_PRE_M1_MAX * N];
+    for (int e = threadIdx.x; e < m1 * N; e += blockDim.x) {
+        float c, s;
+        cis2pi((double)(e / N) / m1 * (double)(p1min + e % N), c, s);
+        e1[e] = make_float2(c, s);
+    }
+    __syncthreads();
+    constexpr int NN = N * N;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nunits * NN * nd2) return;
+    const int g = (int)(e % nd2), bc = (int)((e / nd2) % NN);
+    const int64_t u = e / ((int64_t)nd2 * NN);
+    const int b = bc % N, c = bc / N;
+    float vr[N], vi[N];
+    const cplx<float> *au = A + u * lda + g;
+#pragma unroll
+    for (int a = 0; a < N; a++) {
+        const int ir = DENSE ? s1 * a + s2 * b + s3 * c : rmap[b + 8 * (a + N * c)];
+        const cplx<float> v = au[ldr * max(ir, 0)];
+        vr[a] = ir >= 0 ? v.re : 0.f;
+        vi[a] = ir >= 0 ? v.im : 0.f;
+    }
+    if constexpr (SYM) kq_fold<N>(vr, vi);
+    float2 *out = b1 + (u * m1 * NN + bc) * nd2 + g;
+    for (int q1 = 0; q1 < m1; q1++) {
+        float sr = 0.f, si = 0.f;
+        if constexpr (SYM) {
+            kq_folded_sum<N>(sr, si, e1 + q1 * N, vr, vi);
+        } else {
+#pragma unroll
+            for (int a = 0; a < N; a++) cmac(sr, si, e1[q1 * N + a].x, e1[q1 * N + a].y, vr[a], vi[a]);
+        }
+        out[(int64_t)q1 * NN * nd2] = make_float2(sr, si);
+    }
+}
+
+// The kR->kq workgroup's stage-1 sums from the pre-pass output (b1 slice of (unit, q1), rows G).
+template <int N>
+__device__ __forceinline__ void kq_stage1_load(float (&br)[N][N], float (&bi)[N][N], const float2 *__restrict__ b1u,
+                                               int row, int nd2) {
+#pragma unroll
+    for (int c = 0; c < N; c++)
+#pragma unroll
+        for (int b = 0; b < N; b++) {
+            const float2 v = b1u[(int64_t)(b + N * c) * nd2 + row];
+            br[c][b] = v.x;
+            bi[c][b] = v.y;
+        }
+}
+
 // Workgroup order of the staged kR->kq kernels (grid m1 x nunits). Workgroups are dispatched round-robin
 // over the 8 XCDs in linear order (x fastest), so in the plain order (q1 = blockIdx.x, unit = blockIdx.y)
 // the m1 q1-workgroups of one unit spread over every XCD and each XCD's L2 fetches the unit's operator
@@ -731,7 +792,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
     const cplx<float> *__restrict__ A, int64_t lda, int64_t ldr, const int *__restrict__ line_off,
     const int *__restrict__ col_index, const int *__restrict__ ikq, int64_t cs, const cplx<float> *__restrict__ U,
     const int64_t *__restrict__ bmin, const int64_t *__restrict__ bmax, const cplx<float> *__restrict__ uph,
-    const double *__restrict__ omega, int nbw, float *__restrict__ g2, float2 *__restrict__ epo, int grouped) {
+    const double *__restrict__ omega, int nbw, float *__restrict__ g2, float2 *__restrict__ epo, int grouped,
+    const float2 *__restrict__ b1) {
     constexpr int NW = 16;
     // q1, all q2, all q3 phases; the q2/q3 rows padded to an even count and read as 16-byte pairs
     constexpr int NE = (N + 1) & ~1;
@@ -788,7 +850,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
     }
     // stage 1
     float br[N][N], bi[N][N];
-    kq_stage1<N, DENSE>(br, bi, A + (int64_t)u * lda, rv ? t : 0, ldr, n1, rmap, ph1, s1, s2, s3);
+    if (b1)  // precomputed by the stage-1 pre-pass
+        kq_stage1_load<N>(br, bi, b1 + ((int64_t)u * m1 + q1) * N * N * nd2, rv ? t : 0, nd2);
+    else
+        kq_stage1<N, DENSE>(br, bi, A + (int64_t)u * lda, rv ? t : 0, ldr, n1, rmap, ph1, s1, s2, s3);
     if constexpr (SYM) {
 #pragma unroll
         for (int c = 0; c < N; c++) kq_fold<N>(br[c], bi[c]);  // over b, for the stage-2 sums
@@ -995,7 +1060,8 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
                               const int *__restrict__ ikq, int64_t cs, const cplx<float> *__restrict__ U,
                               const int64_t *__restrict__ bmin, const int64_t *__restrict__ bmax,
                               const cplx<float> *__restrict__ uph, const double *__restrict__ omega, int nbw,
-                              float *__restrict__ g2, float2 *__restrict__ epo, int grouped) {
+                              float *__restrict__ g2, float2 *__restrict__ epo, int grouped,
+                              const float2 *__restrict__ b1) {
     constexpr int NW = 16;
     constexpr int NE = (N + 1) & ~1;
     constexpr int BP = NMC > 0 ? NMC + 1 : 17;  // b2s pitch per i (odd)
@@ -1018,7 +1084,8 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
     int q1, u;
     kq_block(grouped, q1, u);
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const bool rv = t < NW * nm;
+    const int nd2 = NW * nm;
+    const bool rv = t < nd2;
     const int nlines = m1 * m2;
     const int *lo_u = line_off + (int64_t)u * (nlines + 1);
     const int64_t row0 = (int64_t)u * cs;
@@ -1056,7 +1123,10 @@ gridopt_kq_rotate_mfma_kernel(int nm_rt, int n1, int p1min, int p2min, int p3min
     __syncthreads();  // the slab tables, read by every wave from the first line on
     // stage 1 (rows G = t)
     float br[N][N], bi[N][N];
-    kq_stage1<N, DENSE>(br, bi, A + (int64_t)u * lda, rv ? t : 0, ldr, n1, rmap, ph1, s1, s2, s3);
+    if (b1)  // precomputed by the stage-1 pre-pass
+        kq_stage1_load<N>(br, bi, b1 + ((int64_t)u * m1 + q1) * N * N * nd2, rv ? t : 0, nd2);
+    else
+        kq_stage1<N, DENSE>(br, bi, A + (int64_t)u * lda, rv ? t : 0, ldr, n1, rmap, ph1, s1, s2, s3);
     if constexpr (SYM) {
 #pragma unroll
         for (int c = 0; c < N; c++) kq_fold<N>(br[c], bi[c]);  // over b, for the stage-2 sums
@@ -1238,7 +1308,7 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
                                        const int32_t *line_off, const int32_t *col_index, const int32_t *ikq,
                                        int64_t cs, const void *U, const int64_t *bmin, const int64_t *bmax,
                                        const void *uph, const double *omega, const int32_t *dense_strides,
-                                       float *g2, void *ep, void *stream) {
+                                       float *g2, void *ep, void *b1, void *stream) {
     EPHIP_REQUIRE(nw == 16 && nm >= 1 && nm <= 16, "gridopt_kq_rotate: nw must be 16 and nm <= 16 (got %d, %d)", nw,
                   nm);
     EPHIP_REQUIRE(nbw >= 1 && nbw <= 4, "gridopt_kq_rotate: nbw=%d outside 1..4", nbw);
@@ -1261,6 +1331,25 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
         EPHIP_REQUIRE(s1 >= 0 && s2 >= 0 && s3 >= 0 && (int64_t)(s1 + s2 + s3) * (np - 1) < ((int64_t)1 << 31),
                       "gridopt_kq_rotate: bad dense-box strides");
     const int grouped = ephip_ft_get_variant(7) == 1;  // XCD-grouped workgroup order
+    if (b1) {  // stage 1 of every (unit, q1) by the pre-pass into b1 (cube boxes, n1 = np, m1 <= 64)
+        EPHIP_REQUIRE(n1 == np && qgrid[0] <= GQ_PRE_M1_MAX && (np & 1),
+                      "gridopt_kq_rotate: the stage-1 pre-pass needs a cube box with odd np (n1 = np = %d) and "
+                      "m1 <= %d (got m1 = %d)", np, GQ_PRE_M1_MAX, qgrid[0]);
+        const bool sym1 = p1min == -(np / 2);
+        using KS = decltype(&gridopt_kq_stage1_all_kernel<3, false, false>);
+#define GQP_ROW(d, y) {gridopt_kq_stage1_all_kernel<3, d, y>, gridopt_kq_stage1_all_kernel<5, d, y>, \
+                       gridopt_kq_stage1_all_kernel<7, d, y>}
+        static const KS ptable[2][2][3] = {{GQP_ROW(false, false), GQP_ROW(false, true)},
+                                           {GQP_ROW(true, false), GQP_ROW(true, true)}};
+#undef GQP_ROW
+        EPHIP_REQUIRE(np <= 7, "gridopt_kq_rotate: the stage-1 pre-pass covers np <= 7 (got %d)", np);
+        const int64_t nthr = (int64_t)nunits * np * np * nw * nm;
+        ptable[dense ? 1 : 0][sym1 ? 1 : 0][(np - 3) / 2]<<<(unsigned)((nthr + 255) / 256), 256, 0,
+                                                             as_stream(stream)>>>(
+            nunits, nw * nm, qgrid[0], p1min, rmap, s1, s2, s3, static_cast<const cplx<float> *>(A), lda, ldr,
+            static_cast<float2 *>(b1));
+        EPHIP_LAUNCHED();
+    }
     // folded stage-2/3 sums when the Rp box spans -h .. h along axes 2 and 3 (odd np, tuner kind 8 = 0)
     const bool sym = (np & 1) && p2min == -(np / 2) && p3min == -(np / 2) && ephip_ft_get_variant(8) == 0;
     if (ephip_ft_get_variant(6) == 1) {  // stage 3 and the k+q-window rotation on MFMAs
@@ -1284,7 +1373,7 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
             nm, n1, p1min, p2min, p3min, rmap, s1, s2, s3, qgrid[0], qgrid[1], qgrid[2],
             static_cast<const cplx<float> *>(A), lda, ldr, line_off, col_index, ikq, cs,
             static_cast<const cplx<float> *>(U), bmin, bmax, static_cast<const cplx<float> *>(uph), omega, nbw, g2,
-            static_cast<float2 *>(ep), grouped);
+            static_cast<float2 *>(ep), grouped, static_cast<const float2 *>(b1));
         EPHIP_LAUNCHED();
         return EPHIP_OK;
     }
@@ -1315,7 +1404,7 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
                                                 qgrid[2], static_cast<const cplx<float> *>(A), lda, ldr, line_off,
                                                 col_index, ikq, cs, static_cast<const cplx<float> *>(U), bmin, bmax,
                                                 static_cast<const cplx<float> *>(uph), omega, nbw, g2,
-                                                static_cast<float2 *>(ep), grouped);
+                                                static_cast<float2 *>(ep), grouped, static_cast<const float2 *>(b1));
     EPHIP_LAUNCHED();
     return EPHIP_OK;
 }

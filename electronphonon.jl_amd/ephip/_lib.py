@@ -85,7 +85,7 @@ SIGNATURES = {
     "ephip_ft_pack_phases": [_i32, _vp, _i64, _vp, _vp, _vp],
     "ephip_kq_line_offsets": [_i32, _vp, _vp, _i64, _i32, _i32, _vp, _vp],
     "ephip_gridopt_kq_rotate": [_i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _i64, _i64,
-                                _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+                                _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ephip_eph_RR_to_kR_project_packed_bf16x3": [_i32, _i32, _i32, _i32, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp,
                                                  _i32, _vp, _i64, _vp],
     "ephip_fourier_packed_bf16x3": [_i64, _i32, _vp, _i32, _vp, _i64, _i32, _vp, _vp, _i64, _vp, _i64, _i64, _vp],

@@ -785,21 +785,26 @@ class OuterKLoop:
             n2n3 = int((gq.rmap.view(gq.np, gq.n1, 8) >= 0).any(1).sum().item())  # (c, b) columns present
             from . import tune as T
             mfma_kq = T.get_variant(T.KIND_GRIDOPT_KQ) == 1
+            # stage 1: nrp complex multiply-adds per (unit, q1) and row in the workgroup, or by the pre-pass
+            # np^2 sums of np terms (h pairs when r1 spans -h .. h) per (unit, q1) and row
+            s1 = self.nrp
+            if gq.prepass and gq.can_prepass():
+                s1 = gq.np * gq.np * ((gq.np // 2) if gq.prepass_folded() else gq.np)
             if gq.folded():
                 # stages 2 (and 3 in the VALU form) as pair sums over opposite offsets: h = np // 2 pairs of
                 # 4 FMAs (one complex multiply-add's worth) per sum, plus the folds (4 adds per pair)
                 h = gq.np // 2
-                f_fold = self.nrp * gq.m[0] * nu + 0.5 * gq.np * h * gq.m[0] * nu
+                f_fold = s1 * gq.m[0] * nu + 0.5 * gq.np * h * gq.m[0] * nu
                 if mfma_kq:
                     f_kq = 8.0 * nd2 * (f_fold + gq.np * h * gq.nlines * nu + gq.np * slots) + 8.0 * nm * nm * slots
                 else:
                     f_kq = 8.0 * nd2 * (f_fold + (gq.np * h + 0.5 * h) * gq.nlines * nu + h * kept) + rot
             elif mfma_kq:
                 # stage 3 and the k+q-window rotation as one contraction over (i, c) per window band slot
-                f_kq = (8.0 * nd2 * (self.nrp * gq.m[0] * nu + n2n3 * gq.nlines * nu + gq.np * slots)
+                f_kq = (8.0 * nd2 * (s1 * gq.m[0] * nu + n2n3 * gq.nlines * nu + gq.np * slots)
                         + 8.0 * nm * nm * slots)
             else:
-                f_kq = 8.0 * nd2 * (self.nrp * gq.m[0] * nu + n2n3 * gq.nlines * nu + gq.np * kept) + rot
+                f_kq = 8.0 * nd2 * (s1 * gq.m[0] * nu + n2n3 * gq.nlines * nu + gq.np * kept) + rot
         else:
             f_kq = 8.0 * nd2 * self.nrp * kept + (rot if self.fused else 0.0)
         return f_rr, f_kq
@@ -810,9 +815,30 @@ class OuterKLoop:
         from . import tune as T
         n = min(self.nk_batch, self.st.ksurv.shape[0])
         if self.gridopt is not None and self.gridopt_q is not None:
-            # both contractions staged: the staged kR->kq kernel's form (vector ALUs or MFMAs)
-            return {"gridopt_kq": T.tune_fourier(T.KIND_GRIDOPT_KQ, lambda: self.run_kbatch(0, n), reps=reps,
-                                                 variants=T.GRIDOPT_KQ_VARIANTS)}
+            # both contractions staged: the staged kR->kq kernel's form (vector ALUs or MFMAs), then where its
+            # stage 1 runs (in each workgroup or by the pre-pass)
+            res = {"gridopt_kq": T.tune_fourier(T.KIND_GRIDOPT_KQ, lambda: self.run_kbatch(0, n), reps=reps,
+                                                variants=T.GRIDOPT_KQ_VARIANTS)}
+            gq = self.gridopt_q
+            if gq.can_prepass():
+                import time
+                torch = _torch()
+                times = {}
+                for v in T.KQ_STAGE1_VARIANTS:
+                    gq.prepass = bool(v)
+                    self.run_kbatch(0, n)
+                    torch.cuda.synchronize()
+                    ts = []
+                    for _ in range(reps):
+                        t0 = time.perf_counter()
+                        self.run_kbatch(0, n)
+                        torch.cuda.synchronize()
+                        ts.append(1e3 * (time.perf_counter() - t0))
+                    times[v] = float(np.median(ts))
+                best = min(times, key=times.get)
+                gq.prepass = bool(best)
+                res["kq_stage1"] = {**times, "best": best}
+            return res
         if self.packed:
             # the fused RR->kR + projection kernel has its own variants; tile shapes apply to the unfused one
             if self.gridopt is not None:
@@ -977,6 +1003,8 @@ class _GridOptKQ:
     the unit operators' Rp set embedded in its bounding box, the q of each unit's kept list grouped
     into (q1, q2) lines; writes g2 in the layout of the fused Fourier/rotation kernel."""
 
+    prepass_default = False  # where stage 1 runs in new instances (in each workgroup, or by the pre-pass)
+
     @classmethod
     def build(cls, st):
         r = np.asarray(st.epmat.irvec_dev().cpu().numpy(), dtype=np.int64)
@@ -1007,6 +1035,7 @@ class _GridOptKQ:
                 self.dense = (ctypes.c_int32 * 3)(s1, s2, s3)
         self.m = [int(x) for x in m]
         self.nlines = self.m[0] * self.m[1]
+        self.prepass = _GridOptKQ.prepass_default  # stage 1 by the pre-pass (OuterKLoop.tune picks per workload)
         import ctypes
         self.qgrid_c = (ctypes.c_int32 * 3)(*self.m)
 
@@ -1018,9 +1047,23 @@ class _GridOptKQ:
         return (self.np % 2 == 1 and self.pmin[1] == -h and self.pmin[2] == -h
                 and T.get_variant(T.KIND_KQ_FOLD) == 0)
 
+    def can_prepass(self) -> bool:
+        """The stage-1 pre-pass applies: a cube box with odd np <= 7 (n1 = np) and m1 <= 64."""
+        return self.n1 == self.np and self.np % 2 == 1 and self.np <= 7 and self.m[0] <= 64
+
+    def prepass_folded(self) -> bool:
+        return self.pmin[0] == -(self.np // 2)
+
     def run(self, loop, nu, A, stream):
         import ctypes as C
+        torch = _torch()
         st = loop.st
+        b1 = None
+        if self.prepass and self.can_prepass():  # stage 1 of every (unit, q1) by the pre-pass (workspace b1)
+            need = loop.max_units * self.m[0] * self.np * self.np * st.nw * st.nm
+            if getattr(self, "b1", None) is None or self.b1.numel() < need:
+                self.b1 = torch.empty(need, dtype=torch.complex64, device=A.device)
+            b1 = self.b1
         L.call("ephip_kq_line_offsets", nu, L.ptr(loop.count), L.ptr(loop.col_index), loop.cs, self.m[2],
                self.nlines, L.ptr(loop.line_off), stream)
         L.call("ephip_gridopt_kq_rotate", st.nw, st.nm, st.nbw, nu, self.n1, self.np, self.pmin[0], self.pmin[1],
@@ -1028,7 +1071,7 @@ class _GridOptKQ:
                st.nw * st.nm, L.ptr(loop.line_off), L.ptr(loop.col_index), L.ptr(loop.ikq), loop.cs,
                L.ptr(st.el.U), L.ptr(st.el.filt.band_min), L.ptr(st.el.filt.band_max), L.ptr(loop.uph),
                L.ptr(loop.omega), C.addressof(self.dense) if self.dense is not None else None, L.ptr(loop.g2),
-               L.ptr(loop.ep) if loop.ep is not None else None, stream)
+               L.ptr(loop.ep) if loop.ep is not None else None, L.ptr(b1) if b1 is not None else None, stream)
 
 
 def kept_q_bound(ksurv, kgrid, qgrid) -> int:

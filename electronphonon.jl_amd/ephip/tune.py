@@ -22,6 +22,9 @@ KQ_ORDER_VARIANT_NAMES = {0: "q1 fastest, unit by unit", 1: "XCD-grouped units"}
 KIND_KQ_FOLD = 8
 KQ_FOLD_VARIANTS = (0, 1)
 KQ_FOLD_VARIANT_NAMES = {0: "folded stage-2/3 sums on symmetric boxes", 1: "general stage-2/3 sums"}
+# host-side choice (no library kind): the staged kR->kq stage 1 in each workgroup, or by the pre-pass
+KQ_STAGE1_VARIANTS = (0, 1)
+KQ_STAGE1_VARIANT_NAMES = {0: "stage 1 per (unit, q1) workgroup", 1: "stage-1 pre-pass into a workspace"}
 VARIANTS = (0, 1, 2)
 VARIANT_NAMES = {0: "single-buffered KC16", 1: "double-buffered KC16", 2: "double-buffered KC8"}
 PACKED_VARIANTS = (0, 1, 2, 3, 4, 5, 6, 7)

@@ -313,7 +313,11 @@ int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp, int32_t n
  * lists of ephip_kq_compact). nw = 16, nm <= 16, nbw <= 4, m2, m3 <= 64, m2*m3 <= 1024. dense_strides: host
  * int32[3] (s1, s2, s3) when the box is dense (n1 = np and every Rp present, Rp index = s1 a + s2 b + s3 c;
  * the operator is then addressed without rmap), or NULL. ep: NULL, or complex64 in the layout of g2 (one
- * complex per g2 entry): the rotated matrix elements themselves (zero past the k+q window). */
+ * complex per g2 entry): the rotated matrix elements themselves (zero past the k+q window). b1: NULL (each
+ * (unit, q1) workgroup forms its stage-1 sums from the unit operator), or a complex64 workspace of
+ * nunits*m1*np*np*nw*nm elements: a pre-pass then forms stage 1 for every (unit, q1) at once (one read of
+ * each unit operator) and the workgroups read their slices; cube boxes only (n1 = np, odd np <= 7),
+ * m1 <= 64. */
 int ephip_kq_line_offsets(int32_t nunits, const int32_t *count, const int32_t *col_index, int64_t cs, int32_t m3,
                           int32_t nlines, int32_t *line_off, void *stream);
 int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int32_t nunits, int32_t n1, int32_t np,
@@ -321,7 +325,7 @@ int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int32_t nunits,
                             const void *A, int64_t lda, int64_t ldr, const int32_t *line_off,
                             const int32_t *col_index, const int32_t *ikq, int64_t cs, const void *U,
                             const int64_t *bmin, const int64_t *bmax, const void *uph, const double *omega,
-                            const int32_t *dense_strides, float *g2, void *ep, void *stream);
+                            const int32_t *dense_strides, float *g2, void *ep, void *b1, void *stream);
 
 /* kR->kq Fourier over Rp fused with the k+q-window and phonon rotations and g2 (the
  * get_eph_kR_to_kq_batched! + eph_apply_rotations! + g2 sequence of src/wannier_to_bloch_batched.jl:

@@ -21,20 +21,26 @@ def cfg1():
     return m, g
 
 
-@pytest.fixture(params=[(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 0, 1), (1, 0, 1)],
-                ids=["kq-valu", "kq-mfma", "kq-mfma-xcd", "kq-valu-general", "kq-mfma-general"])
+@pytest.fixture(params=[(0, 0, 0, 0), (1, 0, 0, 0), (1, 1, 0, 0), (0, 0, 1, 0), (1, 0, 1, 0), (0, 0, 0, 1),
+                        (1, 0, 0, 1), (0, 0, 1, 1)],
+                ids=["kq-valu", "kq-mfma", "kq-mfma-xcd", "kq-valu-general", "kq-mfma-general", "kq-valu-pre",
+                     "kq-mfma-pre", "kq-valu-general-pre"])
 def kq_variant(request):
     """The staged kR->kq kernel form (tuner kind 6): 0 = stage 3 and the k+q-window rotation on the vector
-    ALUs, 1 = on FP32 MFMAs; its workgroup order (kind 7): 0 = plain, 1 = XCD-grouped; and its stage-2/3
-    sums (kind 8): 0 = folded on symmetric Rp boxes, 1 = always the general form."""
-    from ephip import tune
+    ALUs, 1 = on FP32 MFMAs; its workgroup order (kind 7): 0 = plain, 1 = XCD-grouped; its stage-2/3 sums
+    (kind 8): 0 = folded on symmetric Rp boxes, 1 = always the general form; and where stage 1 runs: 0 = in
+    each workgroup, 1 = by the pre-pass (cube boxes; others keep the in-kernel stage 1)."""
+    from ephip import eph, tune
     kinds = (tune.KIND_GRIDOPT_KQ, tune.KIND_KQ_ORDER, tune.KIND_KQ_FOLD)
     old = [tune.get_variant(k) for k in kinds]
+    old_pre = eph._GridOptKQ.prepass_default
     for k, v in zip(kinds, request.param):
         tune.set_variant(k, v)
+    eph._GridOptKQ.prepass_default = bool(request.param[3])
     yield request.param[0]
     for k, v in zip(kinds, old):
         tune.set_variant(k, v)
+    eph._GridOptKQ.prepass_default = old_pre
 
 
 def _oracle_states(m, g):
