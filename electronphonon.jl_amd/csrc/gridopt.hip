@@ -810,7 +810,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
     // (cfg4, window width 3: 12.6 instead of 15.7 ms per 1152-k batch)
     constexpr int CHUNK = NBW >= 4 ? GQ_CHUNK / 2 : NBW == 3 ? GQ_CHUNK * 3 / 4 : GQ_CHUNK;
     __shared__ float2 ucol[CHUNK * NBW * NW];
-    __shared__ float2 tq[CHUNK * NBW * NW];  // t[kk][m][nu], nu < nm <= 16
+    __shared__ __attribute__((aligned(16))) float2 tq[CHUNK * NBW * NW];  // t[kk][m][nu], nu < nm <= 16
     int q1, u;
     kq_block(grouped, q1, u);
     const int t = threadIdx.x;
@@ -994,11 +994,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
                     const int iq = sk_iq[ks + kk];
                     const cplx<float> *up = uph + (int64_t)NMC * NMC * iq + NMC * nu2;
                     float er = 0.f, ei = 0.f;
+                    if constexpr (NMC % 2 == 0) {  // u_ph column and t row as 16-byte loads (even offsets)
+                        const float4 *up4 = reinterpret_cast<const float4 *>(up);
+                        const float4 *tq4 = reinterpret_cast<const float4 *>(tq + (kk * NBW + m) * NW);
 #pragma unroll
-                    for (int n = 0; n < NMC; n++) {  // constant offsets from one base address
-                        const float2 tv = tq[(kk * NBW + m) * NW + n];
-                        const cplx<float> p = up[n];
-                        cmac(er, ei, tv.x, tv.y, p.re, p.im);
+                        for (int h = 0; h < NMC / 2; h++) {
+                            const float4 p = up4[h], tv = tq4[h];
+                            cmac(er, ei, tv.x, tv.y, p.x, p.y);
+                            cmac(er, ei, tv.z, tv.w, p.z, p.w);
+                        }
+                    } else {
+#pragma unroll
+                        for (int n = 0; n < NMC; n++) {  // constant offsets from one base address
+                            const float2 tv = tq[(kk * NBW + m) * NW + n];
+                            const cplx<float> p = up[n];
+                            cmac(er, ei, tv.x, tv.y, p.re, p.im);
+                        }
                     }
                     const float w2 = 2.f * static_cast<float>(omega[(int64_t)NMC * iq + nu2]);
                     g2[oi] = (er * er + ei * ei) / w2;
