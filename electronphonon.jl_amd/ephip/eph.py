@@ -837,6 +837,8 @@ class OuterKLoop:
                     times[v] = float(np.median(ts))
                 best = min(times, key=times.get)
                 gq.prepass = bool(best)
+                if not gq.prepass:
+                    gq.b1 = None  # the trial's workspace is not kept
                 res["kq_stage1"] = {**times, "best": best}
             return res
         if self.packed:

@@ -84,6 +84,7 @@ def g2_parity(m, epmat_ph, kgrid, qgrid, got, bmin, bmax, nbw: int, cutoff: floa
     ep128 = np.asarray(epmat_ph, dtype=np.complex128)
     num = den = 0.0
     maxrel = 0.0
+    dmax = rmax = 0.0
     npairs = 0
     for q, per_k in got.items():
         if not per_k:
@@ -108,6 +109,10 @@ def g2_parity(m, epmat_ph, kgrid, qgrid, got, bmin, bmax, nbw: int, cutoff: floa
             den += float((r ** 2).sum())
             if np.abs(r).max() > 0:
                 maxrel = max(maxrel, float(np.abs(gg - r).max() / np.abs(r).max()))
+            dmax = max(dmax, float(np.abs(gg - r).max(initial=0.0)))
+            rmax = max(rmax, float(np.abs(r).max(initial=0.0)))
             npairs += 1
+    # rel_err: RMS (norm-wise) relative error over all sampled entries; max_rel_err: max|d| / max|ref| over
+    # all of them; max_block_rel_err: the worst (k, q) block's max|d| / max|ref|
     return {"pairs": npairs, "rel_err": float(np.sqrt(num / den)) if den > 0 else float("nan"),
-            "max_block_rel_err": maxrel}
+            "max_rel_err": dmax / rmax if rmax > 0 else float("nan"), "max_block_rel_err": maxrel}
