@@ -109,3 +109,17 @@ def test_tiled_device_output_residency_and_tiles():
     t3 = tiled.TiledDeviceOutput(dims, 0, (off, ks), device="cpu", force_block=True, free_bytes=lambda: 10 * full)
     t3.tile_begin(0, bmax - 1, bmax)
     assert t3.is_block()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU only: --gpus N > 1 exits with status 2 and says so, before any device work."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=root)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]
