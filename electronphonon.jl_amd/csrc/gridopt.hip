@@ -806,6 +806,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
     // an even column count is summed in pairs fed in lane order (dpp_row_sum4_pairs); an odd one with
     // dpp_row_sum4 (padding it to pairs costs more in staging than the selects it saves)
     constexpr bool PAIRS = NBW % 2 == 0;
+    // on the symmetric box the lane parts are arranged once per workgroup instead of per column, by bit 2
+    // of i for the pair sums and by bit 3 for the one-column sums of odd widths
+    constexpr bool SWAP = SYM;
+    constexpr int SWAP_BIT = PAIRS ? 4 : 8;
     // chunk size: GQ_CHUNK kept q, fewer for wide windows so that four workgroups still fit a CU's LDS
     // (cfg4, window width 3: 12.6 instead of 15.7 ms per 1152-k batch)
     constexpr int CHUNK = NBW >= 4 ? GQ_CHUNK / 2 : NBW == 3 ? GQ_CHUNK * 3 / 4 : GQ_CHUNK;
@@ -854,9 +858,44 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
         kq_stage1_load<N>(br, bi, b1 + ((int64_t)u * m1 + q1) * N * N * nd2, rv ? t : 0, nd2);
     else
         kq_stage1<N, DENSE>(br, bi, A + (int64_t)u * lda, rv ? t : 0, ldr, n1, rmap, ph1, s1, s2, s3);
+    if constexpr (SWAP) {
+        // rows with bit SWAP_BIT of i set keep i conj(B1) mirrored over b and c. The q2 and q3 phases of
+        // the symmetric box satisfy e[N - 1 - x] = conj(e[x]), so their stage-2/3 sums give i conj(g) =
+        // (g_i, g_r): the parts of g in the order the pair and one-column sums take them for these lanes
+        // (with conj(U) staged for their rows), and the per-column lane selects drop out of the loop.
+        if (i & SWAP_BIT) {
+            float tr[N][N], ti[N][N];
+#pragma unroll
+            for (int c = 0; c < N; c++)
+#pragma unroll
+                for (int b = 0; b < N; b++) {
+                    tr[c][b] = bi[N - 1 - c][N - 1 - b];
+                    ti[c][b] = br[N - 1 - c][N - 1 - b];
+                }
+#pragma unroll
+            for (int c = 0; c < N; c++)
+#pragma unroll
+                for (int b = 0; b < N; b++) {
+                    br[c][b] = tr[c][b];
+                    bi[c][b] = ti[c][b];
+                }
+        }
+    }
     if constexpr (SYM) {
 #pragma unroll
         for (int c = 0; c < N; c++) kq_fold<N>(br[c], bi[c]);  // over b, for the stage-2 sums
+        // over c, for the stage-3 sums: stage 2 acts on each c alone, so its sums come out folded
+        constexpr int H = N / 2;
+#pragma unroll
+        for (int b = 0; b < N; b++)
+#pragma unroll
+            for (int j = 1; j <= H; j++) {
+                const float pr = br[H + j][b], pi = bi[H + j][b], mr = br[H - j][b], mi = bi[H - j][b];
+                br[H + j][b] = pr + mr;
+                bi[H + j][b] = pi + mi;
+                br[H - j][b] = pr - mr;
+                bi[H - j][b] = pi - mi;
+            }
     }
     const int lane = t & 63;
     int e0 = 0, lc = 0;  // chunk start in the slab tables and the line that holds it
@@ -875,25 +914,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
             float2 w = make_float2(0.f, 0.f);
             if (m < nbk) {
                 const cplx<float> v = U[(int64_t)NW * NW * sk_kq[ks + kk] + ii + NW * (mlo + m)];
-                w = make_float2(v.re, v.im);
+                w = make_float2(v.re, (SWAP && (ii & SWAP_BIT)) ? -v.im : v.im);
             }
-            ucol[e] = w;
+            if constexpr (PAIRS) {  // per pair, the reading lane's own column first (bit 3 of i picks it)
+                ucol[(kk * NBW + (m & ~1) + ((m & 1) ^ ((ii >> 3) & 1))) * NW + ii] = w;
+            } else {
+                ucol[e] = w;
+            }
         }
         __syncthreads();  // U columns staged
         // lane kk keeps entry kk's metadata word (nk <= CHUNK < 64); the loop takes it with readlane
         const int meta = lane < nk ? sk_nb[ks + lane] : 0;
         float b2r[N], b2i[N];
         int cur = -1;
-        // the columns of each pair in lane order: lanes with bit 3 set take the pair's second column as
-        // their own (see dpp_row_sum4_pairs); read before the stage-2/3 sums that hide the latency
+        // the columns of each pair in lane order (staged so): lanes with bit 3 set take the pair's second
+        // column as their own (see dpp_row_sum4_pairs); read before the stage-2/3 sums that hide the latency
         const int h3 = (i >> 3) & 1;
         for (int kk = 0; kk < nk; kk++) {
             float2 wo[(NBW + 1) / 2], wx[(NBW + 1) / 2], w[NBW];
             if constexpr (PAIRS) {
 #pragma unroll
                 for (int h = 0; h < NBW / 2; h++) {
-                    wo[h] = ucol[(kk * NBW + 2 * h + h3) * NW + i];
-                    wx[h] = ucol[(kk * NBW + 2 * h + 1 - h3) * NW + i];
+                    wo[h] = ucol[(kk * NBW + 2 * h) * NW + i];
+                    wx[h] = ucol[(kk * NBW + 2 * h + 1) * NW + i];
                 }
             } else {
 #pragma unroll
@@ -919,10 +962,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
 #pragma unroll
                         for (int b = 0; b < N; b++) cmac(sr, si, e2[b].x, e2[b].y, br[c][b], bi[c][b]);
                     }
-                    b2r[c] = sr;
+                    b2r[c] = sr;  // folded over c on symmetric boxes (the stage-1 sums are)
                     b2i[c] = si;
                 }
-                if constexpr (SYM) kq_fold<N>(b2r, b2i);  // over c, for the stage-3 sums of the line
             }
             float2 e3[NE];
 #pragma unroll
@@ -946,27 +988,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
             float *tqf = reinterpret_cast<float *>(tq);
             if constexpr (!PAIRS) {
                 if ((w23 & 0xff) <= 1) {
-                    const float p1 = h3 ? gi : gr, p2 = h3 ? -gr : gi, q1 = h3 ? gr : gi, q2 = h3 ? gi : -gr;
-                    const float x = dpp_row_sum2(w[0], p1, p2, q1, q2);
+                    // the parts of g by bit 3 as in the pair sums (bit 2), held so with SWAP
+                    const float p1 = SWAP ? gr : h3 ? gi : gr, p2 = SWAP ? gi : h3 ? -gr : gi;
+                    const float q1 = SWAP ? gi : h3 ? gr : gi, q2 = SWAP ? -gr : h3 ? gi : -gr;
+                    float x = dpp_row_sum2(w[0], p1, p2, q1, q2);
+                    asm volatile("" : "+v"(x));  // all lanes form it, as below
                     if (rv && (i & 7) == 0) tqf[(kk * NBW * NW + nu) * 2 + h3] = x;
                     continue;
                 }
             }
             if constexpr (PAIRS) {
+                // the parts of g by bit 2 of i: (p1, p2) = (gr, gi) or (gi, -gr), (q1, q2) the other; with
+                // SWAP the lanes already hold (p1, p2) and their conj(U) makes (q1, q2) = (p2, -p1)
                 const bool h2 = (i & 4) != 0;
-                const float gp1 = h2 ? gi : gr, gp2 = h2 ? -gr : gi, gq1 = h2 ? gr : gi, gq2 = h2 ? gi : -gr;
+                const float gp1 = SWAP ? gr : h2 ? gi : gr, gp2 = SWAP ? gi : h2 ? -gr : gi;
+                const float gq1 = SWAP ? gi : h2 ? gr : gi, gq2 = SWAP ? -gr : h2 ? gi : -gr;
 #pragma unroll
                 for (int h = 0; h < NBW / 2; h++) {
-                    const float x = dpp_row_sum4_pairs(wo[h], wx[h], gp1, gp2, gq1, gq2);
+                    float x = dpp_row_sum4_pairs(wo[h], wx[h], gp1, gp2, gq1, gq2);
+                    // formed by every lane here, so that its last DPP addition is not moved into the store's
+                    // branch (where it takes a separate DPP move and addition)
+                    asm volatile("" : "+v"(x));
                     const int vi = 4 * h + (i >> 2);  // value index: column vi / 2, real (0) or imaginary (1)
                     if (rv && (i & 3) == 0) tqf[((kk * NBW + (vi >> 1)) * NW + nu) * 2 + (vi & 1)] = x;
                 }
             } else {
                 float v[2 * NBW];
+                // g and U as they are (SWAP rows with bit 3 of i hold (g_i, g_r) and conj(U))
+                const float tr = SWAP && h3 ? gi : gr, ti = SWAP && h3 ? gr : gi;
 #pragma unroll
                 for (int m = 0; m < NBW; m++) {
-                    v[2 * m] = w[m].x * gr + w[m].y * gi;
-                    v[2 * m + 1] = w[m].x * gi - w[m].y * gr;
+                    const float wy = SWAP && h3 ? -w[m].y : w[m].y;
+                    v[2 * m] = w[m].x * tr + wy * ti;
+                    v[2 * m + 1] = w[m].x * ti - wy * tr;
                 }
 #pragma unroll
                 for (int g0 = 0; g0 < 2 * NBW; g0 += 4) {
