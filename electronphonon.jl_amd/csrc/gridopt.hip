@@ -215,6 +215,18 @@ __device__ __forceinline__ float dpp_row_sum2(float2 w, float p1, float p2, floa
     return x;
 }
 
+// The same with the parts split by lane bit 2, as in dpp_row_sum4_pairs: lanes with bit 2 clear end with
+// the row sum of re(conj(w) g), lanes with bit 2 set with im.
+__device__ __forceinline__ float dpp_row_sum2_bit2(float2 w, float p1, float p2, float q1, float q2) {
+    float x = w.x * p1 + w.y * p2;
+    const float y = w.x * q1 + w.y * q2;
+    x += dpp_take<DPP_HALF_MIRROR>(y);  // partner 7 - l in the half row keeps the other part
+    x += dpp_take<DPP_QUAD_2301>(x);
+    x += dpp_take<DPP_QUAD_1032>(x);
+    x += dpp_take<DPP_ROR8>(x);
+    return x;
+}
+
 template <int N, int NW>
 __global__ __launch_bounds__(256) void gridopt_kR_project_kernel(
     int64_t nd, int r2min, int r3min, const cplx<float> *__restrict__ op1,
@@ -810,11 +822,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
     // of i for the pair sums and by bit 3 for the one-column sums of odd widths
     constexpr bool SWAP = SYM;
     constexpr int SWAP_BIT = PAIRS ? 4 : 8;
+    // width 2 on the symmetric box: a kept q whose k+q window holds one band (most) takes the one-column
+    // sum, its U column staged in both slots of the pair
+    constexpr bool ONE2 = SWAP && NBW == 2;
     // chunk size: GQ_CHUNK kept q, fewer for wide windows so that four workgroups still fit a CU's LDS
     // (cfg4, window width 3: 12.6 instead of 15.7 ms per 1152-k batch)
     constexpr int CHUNK = NBW >= 4 ? GQ_CHUNK / 2 : NBW == 3 ? GQ_CHUNK * 3 / 4 : GQ_CHUNK;
     __shared__ float2 ucol[CHUNK * NBW * NW];
     __shared__ __attribute__((aligned(16))) float2 tq[CHUNK * NBW * NW];  // t[kk][m][nu], nu < nm <= 16
+    // where the lanes that write no t of a column store (per lane), so that the column loop has no branch
+    // around its stores
+    __shared__ float tq_sink[256];
     int q1, u;
     kq_block(grouped, q1, u);
     const int t = threadIdx.x;
@@ -912,8 +930,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
             const int kk = e / (NBW * NW), m = (e / NW) % NBW, ii = e % NW;
             const int nbm = sk_nb[ks + kk], nbk = nbm & 0xff, mlo = (nbm >> 8) & 0xff;
             float2 w = make_float2(0.f, 0.f);
-            if (m < nbk) {
-                const cplx<float> v = U[(int64_t)NW * NW * sk_kq[ks + kk] + ii + NW * (mlo + m)];
+            const int ms = (ONE2 && nbk == 1) ? 0 : m;
+            if (ms < nbk) {
+                const cplx<float> v = U[(int64_t)NW * NW * sk_kq[ks + kk] + ii + NW * (mlo + ms)];
                 w = make_float2(v.re, (SWAP && (ii & SWAP_BIT)) ? -v.im : v.im);
             }
             if constexpr (PAIRS) {  // per pair, the reading lane's own column first (bit 3 of i picks it)
@@ -944,7 +963,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
             }
             const int w23 = __builtin_amdgcn_readlane(meta, kk);
             const int q2 = (w23 >> 22) & 63, q3 = (w23 >> 16) & 63;
-            if (q2 != cur) {  // uniform
+            if (__builtin_expect(q2 != cur, 0)) {  // uniform; the next line (rare) out of the loop's path
                 cur = q2;
                 float2 e2[NE];
 #pragma unroll
@@ -983,9 +1002,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
             // conj(w) g summed over the 16 lanes of the row. Even widths: the column pairs, lanes 4s of
             // the row write value s of each pair (re, im of its first column, re, im of its second). Odd
             // widths: one column when the k+q window holds one band (uniform; most kept q), lanes 0 and 8
-            // of the row writing its re and im, else dpp_row_sum4 over the columns (a one-column path
-            // for even widths measured slower than the pair sums it skips)
+            // of the row writing its re and im, else dpp_row_sum4 over the columns. Width 2 on the
+            // symmetric box: one column when the window holds one band, lanes 0 and 4 writing re and im
             float *tqf = reinterpret_cast<float *>(tq);
+            if constexpr (ONE2) {
+                if ((w23 & 0xff) <= 1) {  // uniform
+                    float x = dpp_row_sum2_bit2(wo[0], gr, gi, gi, -gr);
+                    asm volatile("" : "+v"(x));  // all lanes form it, as below
+                    *(rv && (i & 11) == 0 ? &tqf[(kk * NBW * NW + nu) * 2 + ((i >> 2) & 1)] : &tq_sink[t]) = x;
+                    continue;
+                }
+            }
             if constexpr (!PAIRS) {
                 if ((w23 & 0xff) <= 1) {
                     // the parts of g by bit 3 as in the pair sums (bit 2), held so with SWAP
@@ -993,7 +1020,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
                     const float q1 = SWAP ? gi : h3 ? gr : gi, q2 = SWAP ? -gr : h3 ? gi : -gr;
                     float x = dpp_row_sum2(w[0], p1, p2, q1, q2);
                     asm volatile("" : "+v"(x));  // all lanes form it, as below
-                    if (rv && (i & 7) == 0) tqf[(kk * NBW * NW + nu) * 2 + h3] = x;
+                    *(rv && (i & 7) == 0 ? &tqf[(kk * NBW * NW + nu) * 2 + h3] : &tq_sink[t]) = x;
                     continue;
                 }
             }
@@ -1010,7 +1037,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
                     // branch (where it takes a separate DPP move and addition)
                     asm volatile("" : "+v"(x));
                     const int vi = 4 * h + (i >> 2);  // value index: column vi / 2, real (0) or imaginary (1)
-                    if (rv && (i & 3) == 0) tqf[((kk * NBW + (vi >> 1)) * NW + nu) * 2 + (vi & 1)] = x;
+                    *(rv && (i & 3) == 0 ? &tqf[((kk * NBW + (vi >> 1)) * NW + nu) * 2 + (vi & 1)] : &tq_sink[t]) = x;
                 }
             } else {
                 float v[2 * NBW];
@@ -1073,7 +1100,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
         } else {
             for (int e = t; e < nk * nbw * nm; e += blockDim.x) {
                 const int kk = e / (nbw * nm), rr = e % (nbw * nm), m = rr / nm, nu2 = rr % nm;
-                if constexpr (!PAIRS) {  // past the k+q window: zero (its t may not have been formed)
+                if constexpr (!PAIRS || ONE2) {  // past the k+q window: zero (its t may not have been formed)
                     if (m >= (sk_nb[ks + kk] & 0xff)) {
                         g2[(row0 + slab0 + ks + kk) * nbw * nm + rr] = 0.f;
                         if (epo) epo[(row0 + slab0 + ks + kk) * nbw * nm + rr] = make_float2(0.f, 0.f);
