@@ -792,13 +792,15 @@ class OuterKLoop:
                 s1 = gq.np * gq.np * ((gq.np // 2) if gq.prepass_folded() else gq.np)
             if gq.folded():
                 # stages 2 (and 3 in the VALU form) as pair sums over opposite offsets: h = np // 2 pairs of
-                # 4 FMAs (one complex multiply-add's worth) per sum, plus the folds (4 adds per pair)
+                # 4 FMAs (one complex multiply-add's worth) per sum, plus the folds (4 adds per pair) of the
+                # stage-1 sums once per workgroup: over b, and in the VALU form also over c (ahead of stage 2)
                 h = gq.np // 2
-                f_fold = s1 * gq.m[0] * nu + 0.5 * gq.np * h * gq.m[0] * nu
+                fold = 0.5 * gq.np * h * gq.m[0] * nu
                 if mfma_kq:
-                    f_kq = 8.0 * nd2 * (f_fold + gq.np * h * gq.nlines * nu + gq.np * slots) + 8.0 * nm * nm * slots
+                    f_kq = (8.0 * nd2 * (s1 * gq.m[0] * nu + fold + gq.np * h * gq.nlines * nu + gq.np * slots)
+                            + 8.0 * nm * nm * slots)
                 else:
-                    f_kq = 8.0 * nd2 * (f_fold + (gq.np * h + 0.5 * h) * gq.nlines * nu + h * kept) + rot
+                    f_kq = 8.0 * nd2 * (s1 * gq.m[0] * nu + 2 * fold + gq.np * h * gq.nlines * nu + h * kept) + rot
             elif mfma_kq:
                 # stage 3 and the k+q-window rotation as one contraction over (i, c) per window band slot
                 f_kq = (8.0 * nd2 * (s1 * gq.m[0] * nu + n2n3 * gq.nlines * nu + gq.np * slots)
