@@ -5,6 +5,7 @@
   Gaussian smearing 100 meV, occupation convention 5, two temperatures): in-window state set
   (exact), band velocities, S_out, S_in, and the SERTA and iterative-BTE conductivities, all within
   1e-5 relative (Frobenius norm) of the oracle; mobility = sigma / n inherits the bound;
+* the one-call driver (run_transport) from the model and grids alone, mobilities against the oracle;
 * the compressed scattering-in product against the dense matrix, and independence of the k batching;
 * the same quantities on the default nw-16 e-ph path (matrix-core RR->kR projection, staged kR->kq)
   for a 4-atom time-reversal-symmetric model on 6^3 grids, window widths 2 and 3.
@@ -131,6 +132,28 @@ def test_conductivity_and_mobility(transport):
     mob = bte.transport_mobility(res.sigma, p)
     ref = bte.transport_mobility(t["sb"], p)
     assert _rel(mob, ref) <= 1e-5
+
+
+def test_run_transport_driver(transport, backend):
+    """The one-call driver (bte.run_transport: states and window filter, phonons, the outer-k e-ph loop
+    into a BoltzmannCalculator, velocities, SERTA and the iterative BTE) from the model and grids alone
+    gives the oracle's SERTA and BTE mobilities within 1e-5 relative (cfg1 shape)."""
+    from ephip import _lib as L
+    from ephip import bte
+    t = transport
+    m, g = t["m"], t["g"]
+    w = g["window"]
+    params = bte.ElectronTransportParams(Tlist=T_LIST, nlist=[0.1, 0.1], volume=1.0, mulist=MU_LIST)
+    run = bte.run_transport(m, g["kgrid"], g["qgrid"], (-w, w), params, backend, precision=L.PREC_FP32,
+                            smearing=("Gaussian", ETA), occupation_method=METHOD, nk_batch=64, max_iter=600,
+                            rtol=1e-10)
+    assert set(run.seconds) == {"setup", "eph_and_scattering", "velocities", "solve"}
+    assert all(run.result.converged)
+    ref_serta = bte.transport_mobility(t["ss"], params)
+    ref_bte = bte.transport_mobility(t["sb"], params)
+    e_s, e_b = _rel(run.mobility_serta, ref_serta), _rel(run.mobility, ref_bte)
+    print(f"run_transport mobility rel err: SERTA {e_s:.2e}, BTE {e_b:.2e}")
+    assert e_s <= 1e-5 and e_b <= 1e-5
 
 
 def test_spmv_matches_dense(transport):
