@@ -47,17 +47,19 @@ __global__ void project_kR_kernel(int nw, int nm, int nrp, int nunits, const c32
 
 // Per unit: the q of the list whose k+q (on the k grid) has an in-window band, in q order.
 // col_index[u*cs + c] = q, ikq[u*cs + c] = state index of k+q, count[u] = number kept.
-// One workgroup of 256 threads per unit; positions from wave ballots + a 4-wave prefix. The wave counts
-// alternate between two LDS slots, so one barrier per 256 q separates their writes from their reads
-// (a slot is rewritten two iterations later, after the next barrier). Grid coordinates in int32
-// (|rint(x n)| < 2^31 for the reduced coordinates of a k-grid the size check admits).
+// One workgroup of 256 threads per unit; positions from wave ballots + a 4-wave prefix. A pass takes
+// KQC_CHUNKS chunks of 256 q (their loads in flight together) and one barrier; the wave counts alternate
+// between two LDS slots, so that barrier separates their writes from their reads (a slot is rewritten two
+// passes later, after the next barrier). Grid coordinates in int32 (|rint(x n)| < 2^31 for the reduced
+// coordinates of a k-grid the size check admits).
+constexpr int KQC_CHUNKS = 4;
 __global__ __launch_bounds__(256) void kq_compact_kernel(int nunits, const int64_t *__restrict__ ustate,
                                                         const double *__restrict__ xk, int nq,
                                                         const double *__restrict__ xq, int n1, int n2, int n3,
                                                         const int64_t *__restrict__ bmin, int64_t cs,
                                                         int *__restrict__ col_index, int *__restrict__ ikq,
                                                         int *__restrict__ count) {
-    __shared__ int wave_cnt[2][4];
+    __shared__ int wave_cnt[2][KQC_CHUNKS][4];
     const int u = blockIdx.x;
     if (u >= nunits) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -67,26 +69,40 @@ __global__ __launch_bounds__(256) void kq_compact_kernel(int nunits, const int64
         const int c = (int)rint(x * n) % n;
         return c < 0 ? c + n : c;
     };
+    const unsigned long long below = (1ull << lane) - 1ull;
     int base = 0, slot = 0;
-    for (int c0 = 0; c0 < nq; c0 += 256) {
-        const int c = c0 + tid;
-        bool keep = false;
-        int h = 0;
-        if (c < nq) {
-            h = (red(k0 + xq[3 * c], n1) * n2 + red(k1 + xq[3 * c + 1], n2)) * n3 + red(k2 + xq[3 * c + 2], n3);
-            keep = bmin[h] > 0;
+    for (int c0 = 0; c0 < nq; c0 += 256 * KQC_CHUNKS) {
+        bool keep[KQC_CHUNKS];
+        int h[KQC_CHUNKS];
+        unsigned long long ball[KQC_CHUNKS];
+#pragma unroll
+        for (int q = 0; q < KQC_CHUNKS; q++) {
+            const int c = c0 + 256 * q + tid;
+            keep[q] = false;
+            h[q] = 0;
+            if (c < nq) {
+                h[q] = (red(k0 + xq[3 * c], n1) * n2 + red(k1 + xq[3 * c + 1], n2)) * n3 +
+                       red(k2 + xq[3 * c + 2], n3);
+                keep[q] = bmin[h[q]] > 0;
+            }
         }
-        const unsigned long long ball = __ballot(keep);
-        const int before = __popcll(ball & ((1ull << lane) - 1ull));
-        if (lane == 0) wave_cnt[slot][wave] = __popcll(ball);
+#pragma unroll
+        for (int q = 0; q < KQC_CHUNKS; q++) {
+            ball[q] = __ballot(keep[q]);
+            if (lane == 0) wave_cnt[slot][q][wave] = __popcll(ball[q]);
+        }
         __syncthreads();
-        int off = base;
-        for (int w = 0; w < wave; w++) off += wave_cnt[slot][w];
-        if (keep) {
-            col_index[u * cs + off + before] = c;
-            ikq[u * cs + off + before] = h;
+#pragma unroll
+        for (int q = 0; q < KQC_CHUNKS; q++) {
+            int off = base;
+            for (int w = 0; w < wave; w++) off += wave_cnt[slot][q][w];
+            if (keep[q]) {
+                const int64_t o = u * cs + off + __popcll(ball[q] & below);
+                col_index[o] = c0 + 256 * q + tid;
+                ikq[o] = h[q];
+            }
+            base += wave_cnt[slot][q][0] + wave_cnt[slot][q][1] + wave_cnt[slot][q][2] + wave_cnt[slot][q][3];
         }
-        base += wave_cnt[slot][0] + wave_cnt[slot][1] + wave_cnt[slot][2] + wave_cnt[slot][3];
         slot ^= 1;
     }
     if (tid == 0) count[u] = base;
