@@ -150,3 +150,35 @@ def test_eigvals_noalloc_validates_before_any_device_call():
     with pytest.raises(L.EphipError, match="null device pointer"):
         L.call("ephip_eigvals_batched_noalloc", 16, 7943, None, None, None, None)
     L.call("ephip_eigvals_batched_noalloc", 16, 0, None, None, None, None)
+
+
+def test_staged_entries_validate_arguments():
+    """The staged (GridOpt) entries reject shapes their kernels do not cover before any device call, and
+    treat empty work as a no-op."""
+    q = (C.c_int32 * 3)(24, 24, 24)
+    q_long = (C.c_int32 * 3)(24, 80, 24)      # n2 > 64
+    q_slab = (C.c_int32 * 3)(24, 40, 40)      # n2 n3 > 1024
+    qp, qlp, qsp = (C.cast(a, C.c_void_p).value for a in (q, q_long, q_slab))
+
+    def kq(nw=16, nm=12, nbw=2, nunits=1, n1=7, np_=7, qgrid=qp):
+        L.call("ephip_gridopt_kq_rotate", nw, nm, nbw, nunits, n1, np_, -3, -3, -3, None, qgrid, None,
+               16 * 12 * 343, 16 * 12, None, None, None, 576, None, None, None, None, None, None, None, None,
+               None, None)
+
+    for kw, msg in [(dict(nw=8), "nw must be 16"), (dict(nm=17), "nw must be 16"), (dict(nbw=5), "outside 1..4"),
+                    (dict(np_=9), "Rp box"), (dict(n1=33), "Rp box"), (dict(qgrid=qlp), "q grid"),
+                    (dict(qgrid=qsp), "q grid"), (dict(qgrid=None), "q grid"),
+                    (dict(), "null device pointer")]:
+        with pytest.raises(L.EphipError, match=msg):
+            kq(**kw)
+    kq(nunits=0)
+
+    def proj(nw=16, nbw=2, np_=7, max_line_k=48, k0=0, k1=8):
+        L.call("ephip_gridopt_kR_project_mfma", nw, 12, 343, np_, -3, None, k0, k1, 0, 4, 0, None, max_line_k, 0,
+               None, None, None, None, nbw, None, 16 * 12 * 343, None)
+
+    for kw, msg in [(dict(nw=8), "nw must be 16"), (dict(nbw=5), "bad sizes"), (dict(np_=2), "padded box face"),
+                    (dict(max_line_k=65), "k on a line"), (dict(), "null device pointer")]:
+        with pytest.raises(L.EphipError, match=msg):
+            proj(**kw)
+    proj(k1=0)
