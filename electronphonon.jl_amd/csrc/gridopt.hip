@@ -797,7 +797,8 @@ __device__ __forceinline__ void kq_block(int grouped, int &q1, int &u) {
 }
 
 // SYM: the Rp box spans -h .. h along axes 2 and 3 (N = 2h + 1): stages 2 and 3 in the folded form.
-template <int N, int NBW, int NMC = 0, bool DENSE = false, bool SYM = false>  // NMC: compile-time mode count (12), or 0
+// SEG: the slab's kept q may exceed GQ_SLAB_MAX (n2 n3 > GQ_SLAB_MAX): taken in segments of whole q lines
+template <int N, int NBW, int NMC = 0, bool DENSE = false, bool SYM = false, bool SEG = false>  // NMC: compile-time mode count (12), or 0
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 : 1))) void gridopt_kq_rotate_kernel(
     int nm, int n1, int p1min, int p2min, int p3min, const int *__restrict__ rmap, int s1, int s2, int s3, int m1,
     int m2, int m3,
@@ -859,17 +860,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
         reinterpret_cast<float2 *>(ph3)[e] = make_float2(c, s);
     }
     __syncthreads();
-    const int slab0 = slo[0], nslab = slo[m2] - slab0;  // <= m2 m3 <= GQ_SLAB_MAX
-    for (int kk = t; kk < nslab; kk += blockDim.x) {
-        const int64_t row = row0 + slab0 + kk;
-        const int kq = ikq[row];
-        const int iq = col_index[row];
-        sk_kq[kk] = kq;
-        sk_iq[kk] = iq;
-        const int64_t b0 = bmin[kq], b1 = bmax[kq];
-        const int nb = b0 > 0 ? (int)(b1 - b0 + 1) : 0;
-        sk_nb[kk] = min(nb, nbw) | ((b0 > 0 ? (int)b0 - 1 : 0) << 8) | ((iq % m3) << 16) | (((iq / m3) % m2) << 22);
-    }
+    // The slab's kept q in segments of whole q lines with at most GQ_SLAB_MAX entries (a line holds at
+    // most m3 <= 64): lines [L0, L1) of the segment, its tables in LDS from entry slo[L0] on.
+    auto segment_end = [&](int L0) {  // uniform, kept in a scalar register
+        if constexpr (!SEG) return m2;  // the whole slab (at most GQ_SLAB_MAX entries)
+        int L1 = L0 + 1;
+        while (L1 < m2 && slo[L1 + 1] - slo[L0] <= GQ_SLAB_MAX) L1++;
+        return __builtin_amdgcn_readfirstlane(L1);
+    };
+    auto fill = [&](int L0, int L1) {
+        const int sb = slo[L0], n = slo[L1] - sb;
+        for (int kk = t; kk < n; kk += blockDim.x) {
+            const int64_t row = row0 + sb + kk;
+            const int kq = ikq[row];
+            const int iq = col_index[row];
+            sk_kq[kk] = kq;
+            sk_iq[kk] = iq;
+            const int64_t b0 = bmin[kq], b1 = bmax[kq];
+            const int nb = b0 > 0 ? (int)(b1 - b0 + 1) : 0;
+            sk_nb[kk] = min(nb, nbw) | ((b0 > 0 ? (int)b0 - 1 : 0) << 8) | ((iq % m3) << 16) |
+                        (((iq / m3) % m2) << 22);
+        }
+    };
+    int L0 = 0, L1 = segment_end(0);
+    fill(L0, L1);
     // stage 1
     float br[N][N], bi[N][N];
     if (b1)  // precomputed by the stage-1 pre-pass
@@ -916,213 +930,222 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
             }
     }
     const int lane = t & 63;
-    int e0 = 0, lc = 0;  // chunk start in the slab tables and the line that holds it
-    while (e0 < nslab) {  // uniform
-        while (slo[lc + 1] - slab0 <= e0) lc++;
-        // the chunk: whole lines up to CHUNK entries, or the next CHUNK entries of a longer line
-        const int e1 = min(e0 + CHUNK, nslab);
-        int cut = e1;
-        for (int l = lc + 1; l <= m2 && slo[l] - slab0 <= e1; l++) cut = slo[l] - slab0;
-        const int ks = e0, nk = cut - e0;  // 0 < nk <= CHUNK
-        e0 = cut;
-        __syncthreads();  // slab tables written; the previous chunk's tables no longer read
-        for (int e = t; e < nk * NBW * NW; e += blockDim.x) {
-            const int kk = e / (NBW * NW), m = (e / NW) % NBW, ii = e % NW;
-            const int nbm = sk_nb[ks + kk], nbk = nbm & 0xff, mlo = (nbm >> 8) & 0xff;
-            float2 w = make_float2(0.f, 0.f);
-            const int ms = (ONE2 && nbk == 1) ? 0 : m;
-            if (ms < nbk) {
-                const cplx<float> v = U[(int64_t)NW * NW * sk_kq[ks + kk] + ii + NW * (mlo + ms)];
-                w = make_float2(v.re, (SWAP && (ii & SWAP_BIT)) ? -v.im : v.im);
-            }
-            if constexpr (PAIRS) {  // per pair, the reading lane's own column first (bit 3 of i picks it)
-                ucol[(kk * NBW + (m & ~1) + ((m & 1) ^ ((ii >> 3) & 1))) * NW + ii] = w;
-            } else {
-                ucol[e] = w;
-            }
-        }
-        __syncthreads();  // U columns staged
-        // lane kk keeps entry kk's metadata word (nk <= CHUNK < 64); the loop takes it with readlane
-        const int meta = lane < nk ? sk_nb[ks + lane] : 0;
-        float b2r[N], b2i[N];
-        int cur = -1;
-        // the columns of each pair in lane order (staged so): lanes with bit 3 set take the pair's second
-        // column as their own (see dpp_row_sum4_pairs); read before the stage-2/3 sums that hide the latency
-        const int h3 = (i >> 3) & 1;
-        for (int kk = 0; kk < nk; kk++) {
-            float2 wo[(NBW + 1) / 2], wx[(NBW + 1) / 2], w[NBW];
-            if constexpr (PAIRS) {
-#pragma unroll
-                for (int h = 0; h < NBW / 2; h++) {
-                    wo[h] = ucol[(kk * NBW + 2 * h) * NW + i];
-                    wx[h] = ucol[(kk * NBW + 2 * h + 1) * NW + i];
+    for (;;) {  // segments (uniform)
+        // the segment's first slab entry and entry count (uniform, in scalar registers)
+        const int sb = __builtin_amdgcn_readfirstlane(slo[L0]), nslab = __builtin_amdgcn_readfirstlane(slo[L1] - sb);
+        int e0 = 0, lc = L0;  // chunk start in the segment's tables and the line that holds it
+        while (e0 < nslab) {  // uniform
+            while (slo[lc + 1] - sb <= e0) lc++;
+            // the chunk: whole lines up to CHUNK entries, or the next CHUNK entries of a longer line
+            const int e1 = min(e0 + CHUNK, nslab);
+            int cut = e1;
+            for (int l = lc + 1; l <= L1 && slo[l] - sb <= e1; l++) cut = slo[l] - sb;
+            const int ks = e0, nk = cut - e0;  // 0 < nk <= CHUNK
+            e0 = cut;
+            __syncthreads();  // segment tables written; the previous chunk's tables no longer read
+            for (int e = t; e < nk * NBW * NW; e += blockDim.x) {
+                const int kk = e / (NBW * NW), m = (e / NW) % NBW, ii = e % NW;
+                const int nbm = sk_nb[ks + kk], nbk = nbm & 0xff, mlo = (nbm >> 8) & 0xff;
+                float2 w = make_float2(0.f, 0.f);
+                const int ms = (ONE2 && nbk == 1) ? 0 : m;
+                if (ms < nbk) {
+                    const cplx<float> v = U[(int64_t)NW * NW * sk_kq[ks + kk] + ii + NW * (mlo + ms)];
+                    w = make_float2(v.re, (SWAP && (ii & SWAP_BIT)) ? -v.im : v.im);
                 }
-            } else {
-#pragma unroll
-                for (int m = 0; m < NBW; m++) w[m] = ucol[(kk * NBW + m) * NW + i];
+                if constexpr (PAIRS) {  // per pair, the reading lane's own column first (bit 3 of i picks it)
+                    ucol[(kk * NBW + (m & ~1) + ((m & 1) ^ ((ii >> 3) & 1))) * NW + ii] = w;
+                } else {
+                    ucol[e] = w;
+                }
             }
-            const int w23 = __builtin_amdgcn_readlane(meta, kk);
-            const int q2 = (w23 >> 22) & 63, q3 = (w23 >> 16) & 63;
-            if (__builtin_expect(q2 != cur, 0)) {  // uniform; the next line (rare) out of the loop's path
-                cur = q2;
-                float2 e2[NE];
+            __syncthreads();  // U columns staged
+            // lane kk keeps entry kk's metadata word (nk <= CHUNK < 64); the loop takes it with readlane
+            const int meta = lane < nk ? sk_nb[ks + lane] : 0;
+            float b2r[N], b2i[N];
+            int cur = -1;
+            // the columns of each pair in lane order (staged so): lanes with bit 3 set take the pair's second
+            // column as their own (see dpp_row_sum4_pairs); read before the stage-2/3 sums that hide the latency
+            const int h3 = (i >> 3) & 1;
+            for (int kk = 0; kk < nk; kk++) {
+                float2 wo[(NBW + 1) / 2], wx[(NBW + 1) / 2], w[NBW];
+                if constexpr (PAIRS) {
+#pragma unroll
+                    for (int h = 0; h < NBW / 2; h++) {
+                        wo[h] = ucol[(kk * NBW + 2 * h) * NW + i];
+                        wx[h] = ucol[(kk * NBW + 2 * h + 1) * NW + i];
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < NBW; m++) w[m] = ucol[(kk * NBW + m) * NW + i];
+                }
+                const int w23 = __builtin_amdgcn_readlane(meta, kk);
+                const int q2 = (w23 >> 22) & 63, q3 = (w23 >> 16) & 63;
+                if (__builtin_expect(q2 != cur, 0)) {  // uniform; the next line (rare) out of the loop's path
+                    cur = q2;
+                    float2 e2[NE];
+#pragma unroll
+                    for (int h = 0; h < NE / 2; h++) {
+                        const float4 v = ph2[q2 * (NE / 2) + h];
+                        e2[2 * h] = make_float2(v.x, v.y);
+                        e2[2 * h + 1] = make_float2(v.z, v.w);
+                    }
+#pragma unroll
+                    for (int c = 0; c < N; c++) {
+                        float sr = 0.f, si = 0.f;
+                        if constexpr (SYM) {
+                            kq_folded_sum<N>(sr, si, e2, br[c], bi[c]);
+                        } else {
+#pragma unroll
+                            for (int b = 0; b < N; b++) cmac(sr, si, e2[b].x, e2[b].y, br[c][b], bi[c][b]);
+                        }
+                        b2r[c] = sr;  // folded over c on symmetric boxes (the stage-1 sums are)
+                        b2i[c] = si;
+                    }
+                }
+                float2 e3[NE];
 #pragma unroll
                 for (int h = 0; h < NE / 2; h++) {
-                    const float4 v = ph2[q2 * (NE / 2) + h];
-                    e2[2 * h] = make_float2(v.x, v.y);
-                    e2[2 * h + 1] = make_float2(v.z, v.w);
+                    const float4 v = ph3[q3 * (NE / 2) + h];
+                    e3[2 * h] = make_float2(v.x, v.y);
+                    e3[2 * h + 1] = make_float2(v.z, v.w);
                 }
+                float gr = 0.f, gi = 0.f;
+                if constexpr (SYM) {
+                    kq_folded_sum<N>(gr, gi, e3, b2r, b2i);
+                } else {
 #pragma unroll
-                for (int c = 0; c < N; c++) {
-                    float sr = 0.f, si = 0.f;
-                    if constexpr (SYM) {
-                        kq_folded_sum<N>(sr, si, e2, br[c], bi[c]);
-                    } else {
-#pragma unroll
-                        for (int b = 0; b < N; b++) cmac(sr, si, e2[b].x, e2[b].y, br[c][b], bi[c][b]);
-                    }
-                    b2r[c] = sr;  // folded over c on symmetric boxes (the stage-1 sums are)
-                    b2i[c] = si;
+                    for (int c = 0; c < N; c++) cmac(gr, gi, e3[c].x, e3[c].y, b2r[c], b2i[c]);
                 }
-            }
-            float2 e3[NE];
-#pragma unroll
-            for (int h = 0; h < NE / 2; h++) {
-                const float4 v = ph3[q3 * (NE / 2) + h];
-                e3[2 * h] = make_float2(v.x, v.y);
-                e3[2 * h + 1] = make_float2(v.z, v.w);
-            }
-            float gr = 0.f, gi = 0.f;
-            if constexpr (SYM) {
-                kq_folded_sum<N>(gr, gi, e3, b2r, b2i);
-            } else {
-#pragma unroll
-                for (int c = 0; c < N; c++) cmac(gr, gi, e3[c].x, e3[c].y, b2r[c], b2i[c]);
-            }
-            // conj(w) g summed over the 16 lanes of the row. Even widths: the column pairs, lanes 4s of
-            // the row write value s of each pair (re, im of its first column, re, im of its second). Odd
-            // widths: one column when the k+q window holds one band (uniform; most kept q), lanes 0 and 8
-            // of the row writing its re and im, else dpp_row_sum4 over the columns. Width 2 on the
-            // symmetric box: one column when the window holds one band, lanes 0 and 4 writing re and im
-            float *tqf = reinterpret_cast<float *>(tq);
-            if constexpr (ONE2) {
-                if ((w23 & 0xff) <= 1) {  // uniform
-                    float x = dpp_row_sum2_bit2(wo[0], gr, gi, gi, -gr);
-                    asm volatile("" : "+v"(x));  // all lanes form it, as below
-                    *(rv && (i & 11) == 0 ? &tqf[(kk * NBW * NW + nu) * 2 + ((i >> 2) & 1)] : &tq_sink[t]) = x;
-                    continue;
-                }
-            }
-            if constexpr (!PAIRS) {
-                if ((w23 & 0xff) <= 1) {
-                    // the parts of g by bit 3 as in the pair sums (bit 2), held so with SWAP
-                    const float p1 = SWAP ? gr : h3 ? gi : gr, p2 = SWAP ? gi : h3 ? -gr : gi;
-                    const float q1 = SWAP ? gi : h3 ? gr : gi, q2 = SWAP ? -gr : h3 ? gi : -gr;
-                    float x = dpp_row_sum2(w[0], p1, p2, q1, q2);
-                    asm volatile("" : "+v"(x));  // all lanes form it, as below
-                    *(rv && (i & 7) == 0 ? &tqf[(kk * NBW * NW + nu) * 2 + h3] : &tq_sink[t]) = x;
-                    continue;
-                }
-            }
-            if constexpr (PAIRS) {
-                // the parts of g by bit 2 of i: (p1, p2) = (gr, gi) or (gi, -gr), (q1, q2) the other; with
-                // SWAP the lanes already hold (p1, p2) and their conj(U) makes (q1, q2) = (p2, -p1)
-                const bool h2 = (i & 4) != 0;
-                const float gp1 = SWAP ? gr : h2 ? gi : gr, gp2 = SWAP ? gi : h2 ? -gr : gi;
-                const float gq1 = SWAP ? gi : h2 ? gr : gi, gq2 = SWAP ? -gr : h2 ? gi : -gr;
-#pragma unroll
-                for (int h = 0; h < NBW / 2; h++) {
-                    float x = dpp_row_sum4_pairs(wo[h], wx[h], gp1, gp2, gq1, gq2);
-                    // formed by every lane here, so that its last DPP addition is not moved into the store's
-                    // branch (where it takes a separate DPP move and addition)
-                    asm volatile("" : "+v"(x));
-                    const int vi = 4 * h + (i >> 2);  // value index: column vi / 2, real (0) or imaginary (1)
-                    *(rv && (i & 3) == 0 ? &tqf[((kk * NBW + (vi >> 1)) * NW + nu) * 2 + (vi & 1)] : &tq_sink[t]) = x;
-                }
-            } else {
-                float v[2 * NBW];
-                // g and U as they are (SWAP rows with bit 3 of i hold (g_i, g_r) and conj(U))
-                const float tr = SWAP && h3 ? gi : gr, ti = SWAP && h3 ? gr : gi;
-#pragma unroll
-                for (int m = 0; m < NBW; m++) {
-                    const float wy = SWAP && h3 ? -w[m].y : w[m].y;
-                    v[2 * m] = w[m].x * tr + wy * ti;
-                    v[2 * m + 1] = w[m].x * ti - wy * tr;
-                }
-#pragma unroll
-                for (int g0 = 0; g0 < 2 * NBW; g0 += 4) {
-                    const float x = dpp_row_sum4(v[g0], v[g0 + 1], g0 + 2 < 2 * NBW ? v[g0 + 2] : 0.f,
-                                                 g0 + 3 < 2 * NBW ? v[g0 + 3] : 0.f);
-                    const int vi = g0 + (i >> 2);
-                    if (rv && (i & 3) == 0 && vi < 2 * NBW)
-                        tqf[((kk * NBW + (vi >> 1)) * NW + nu) * 2 + (vi & 1)] = x;
-                }
-            }
-        }
-        __syncthreads();  // t of every kept q of the chunk
-        if constexpr (NMC > 0) {
-            // entries (slot m, kk, mode) with the slot outermost: past the k+q window (most entries of the
-            // slots m >= 1) a whole wave skips the rotation and writes zeros; compile-time index arithmetic
-            for (int m = 0; m < nbw; m++) {  // uniform
-                for (int e = t; e < nk * NMC; e += blockDim.x) {
-                    const int kk = e / NMC, nu2 = e % NMC;
-                    const int64_t oi = (row0 + slab0 + ks + kk) * nbw * NMC + m * NMC + nu2;
-                    if (m >= (sk_nb[ks + kk] & 0xff)) {  // past the k+q window: zero
-                        g2[oi] = 0.f;
-                        if (epo) epo[oi] = make_float2(0.f, 0.f);
+                // conj(w) g summed over the 16 lanes of the row. Even widths: the column pairs, lanes 4s of
+                // the row write value s of each pair (re, im of its first column, re, im of its second). Odd
+                // widths: one column when the k+q window holds one band (uniform; most kept q), lanes 0 and 8
+                // of the row writing its re and im, else dpp_row_sum4 over the columns. Width 2 on the
+                // symmetric box: one column when the window holds one band, lanes 0 and 4 writing re and im
+                float *tqf = reinterpret_cast<float *>(tq);
+                if constexpr (ONE2) {
+                    if ((w23 & 0xff) <= 1) {  // uniform
+                        float x = dpp_row_sum2_bit2(wo[0], gr, gi, gi, -gr);
+                        asm volatile("" : "+v"(x));  // all lanes form it, as below
+                        *(rv && (i & 11) == 0 ? &tqf[(kk * NBW * NW + nu) * 2 + ((i >> 2) & 1)] : &tq_sink[t]) = x;
                         continue;
+                    }
+                }
+                if constexpr (!PAIRS) {
+                    if ((w23 & 0xff) <= 1) {
+                        // the parts of g by bit 3 as in the pair sums (bit 2), held so with SWAP
+                        const float p1 = SWAP ? gr : h3 ? gi : gr, p2 = SWAP ? gi : h3 ? -gr : gi;
+                        const float q1 = SWAP ? gi : h3 ? gr : gi, q2 = SWAP ? -gr : h3 ? gi : -gr;
+                        float x = dpp_row_sum2(w[0], p1, p2, q1, q2);
+                        asm volatile("" : "+v"(x));  // all lanes form it, as below
+                        *(rv && (i & 7) == 0 ? &tqf[(kk * NBW * NW + nu) * 2 + h3] : &tq_sink[t]) = x;
+                        continue;
+                    }
+                }
+                if constexpr (PAIRS) {
+                    // the parts of g by bit 2 of i: (p1, p2) = (gr, gi) or (gi, -gr), (q1, q2) the other; with
+                    // SWAP the lanes already hold (p1, p2) and their conj(U) makes (q1, q2) = (p2, -p1)
+                    const bool h2 = (i & 4) != 0;
+                    const float gp1 = SWAP ? gr : h2 ? gi : gr, gp2 = SWAP ? gi : h2 ? -gr : gi;
+                    const float gq1 = SWAP ? gi : h2 ? gr : gi, gq2 = SWAP ? -gr : h2 ? gi : -gr;
+#pragma unroll
+                    for (int h = 0; h < NBW / 2; h++) {
+                        float x = dpp_row_sum4_pairs(wo[h], wx[h], gp1, gp2, gq1, gq2);
+                        // formed by every lane here, so that its last DPP addition is not moved into the store's
+                        // branch (where it takes a separate DPP move and addition)
+                        asm volatile("" : "+v"(x));
+                        const int vi = 4 * h + (i >> 2);  // value index: column vi / 2, real (0) or imaginary (1)
+                        *(rv && (i & 3) == 0 ? &tqf[((kk * NBW + (vi >> 1)) * NW + nu) * 2 + (vi & 1)] : &tq_sink[t]) = x;
+                    }
+                } else {
+                    float v[2 * NBW];
+                    // g and U as they are (SWAP rows with bit 3 of i hold (g_i, g_r) and conj(U))
+                    const float tr = SWAP && h3 ? gi : gr, ti = SWAP && h3 ? gr : gi;
+#pragma unroll
+                    for (int m = 0; m < NBW; m++) {
+                        const float wy = SWAP && h3 ? -w[m].y : w[m].y;
+                        v[2 * m] = w[m].x * tr + wy * ti;
+                        v[2 * m + 1] = w[m].x * ti - wy * tr;
+                    }
+#pragma unroll
+                    for (int g0 = 0; g0 < 2 * NBW; g0 += 4) {
+                        const float x = dpp_row_sum4(v[g0], v[g0 + 1], g0 + 2 < 2 * NBW ? v[g0 + 2] : 0.f,
+                                                     g0 + 3 < 2 * NBW ? v[g0 + 3] : 0.f);
+                        const int vi = g0 + (i >> 2);
+                        if (rv && (i & 3) == 0 && vi < 2 * NBW)
+                            tqf[((kk * NBW + (vi >> 1)) * NW + nu) * 2 + (vi & 1)] = x;
+                    }
+                }
+            }
+            __syncthreads();  // t of every kept q of the chunk
+            if constexpr (NMC > 0) {
+                // entries (slot m, kk, mode) with the slot outermost: past the k+q window (most entries of the
+                // slots m >= 1) a whole wave skips the rotation and writes zeros; compile-time index arithmetic
+                for (int m = 0; m < nbw; m++) {  // uniform
+                    for (int e = t; e < nk * NMC; e += blockDim.x) {
+                        const int kk = e / NMC, nu2 = e % NMC;
+                        const int64_t oi = (row0 + sb + ks + kk) * nbw * NMC + m * NMC + nu2;
+                        if (m >= (sk_nb[ks + kk] & 0xff)) {  // past the k+q window: zero
+                            g2[oi] = 0.f;
+                            if (epo) epo[oi] = make_float2(0.f, 0.f);
+                            continue;
+                        }
+                        const int iq = sk_iq[ks + kk];
+                        const cplx<float> *up = uph + (int64_t)NMC * NMC * iq + NMC * nu2;
+                        float er = 0.f, ei = 0.f;
+                        if constexpr (NMC % 2 == 0) {  // u_ph column and t row as 16-byte loads (even offsets)
+                            const float4 *up4 = reinterpret_cast<const float4 *>(up);
+                            const float4 *tq4 = reinterpret_cast<const float4 *>(tq + (kk * NBW + m) * NW);
+#pragma unroll
+                            for (int h = 0; h < NMC / 2; h++) {
+                                const float4 p = up4[h], tv = tq4[h];
+                                cmac(er, ei, tv.x, tv.y, p.x, p.y);
+                                cmac(er, ei, tv.z, tv.w, p.z, p.w);
+                            }
+                        } else {
+#pragma unroll
+                            for (int n = 0; n < NMC; n++) {  // constant offsets from one base address
+                                const float2 tv = tq[(kk * NBW + m) * NW + n];
+                                const cplx<float> p = up[n];
+                                cmac(er, ei, tv.x, tv.y, p.re, p.im);
+                            }
+                        }
+                        const float w2 = 2.f * static_cast<float>(omega[(int64_t)NMC * iq + nu2]);
+                        g2[oi] = (er * er + ei * ei) / w2;
+                        if (epo) epo[oi] = make_float2(er, ei);
+                    }
+                }
+            } else {
+                for (int e = t; e < nk * nbw * nm; e += blockDim.x) {
+                    const int kk = e / (nbw * nm), rr = e % (nbw * nm), m = rr / nm, nu2 = rr % nm;
+                    if constexpr (!PAIRS || ONE2) {  // past the k+q window: zero (its t may not have been formed)
+                        if (m >= (sk_nb[ks + kk] & 0xff)) {
+                            g2[(row0 + sb + ks + kk) * nbw * nm + rr] = 0.f;
+                            if (epo) epo[(row0 + sb + ks + kk) * nbw * nm + rr] = make_float2(0.f, 0.f);
+                            continue;
+                        }
                     }
                     const int iq = sk_iq[ks + kk];
-                    const cplx<float> *up = uph + (int64_t)NMC * NMC * iq + NMC * nu2;
+                    const cplx<float> *up = uph + (int64_t)nm * nm * iq + nm * nu2;
                     float er = 0.f, ei = 0.f;
-                    if constexpr (NMC % 2 == 0) {  // u_ph column and t row as 16-byte loads (even offsets)
-                        const float4 *up4 = reinterpret_cast<const float4 *>(up);
-                        const float4 *tq4 = reinterpret_cast<const float4 *>(tq + (kk * NBW + m) * NW);
 #pragma unroll
-                        for (int h = 0; h < NMC / 2; h++) {
-                            const float4 p = up4[h], tv = tq4[h];
-                            cmac(er, ei, tv.x, tv.y, p.x, p.y);
-                            cmac(er, ei, tv.z, tv.w, p.z, p.w);
-                        }
-                    } else {
-#pragma unroll
-                        for (int n = 0; n < NMC; n++) {  // constant offsets from one base address
-                            const float2 tv = tq[(kk * NBW + m) * NW + n];
-                            const cplx<float> p = up[n];
-                            cmac(er, ei, tv.x, tv.y, p.re, p.im);
-                        }
+                    for (int n = 0; n < NW; n++) {  // nm <= NW: fixed trip count, loads issued together
+                        const int nn = min(n, nm - 1);
+                        const float on = n < nm ? 1.f : 0.f;
+                        const float2 tv = tq[(kk * NBW + m) * NW + nn];
+                        const cplx<float> p = up[nn];
+                        cmac(er, ei, on * tv.x, on * tv.y, p.re, p.im);
                     }
-                    const float w2 = 2.f * static_cast<float>(omega[(int64_t)NMC * iq + nu2]);
-                    g2[oi] = (er * er + ei * ei) / w2;
-                    if (epo) epo[oi] = make_float2(er, ei);
+                    const float w2 = 2.f * static_cast<float>(omega[(int64_t)nm * iq + nu2]);
+                    g2[(row0 + sb + ks + kk) * nbw * nm + rr] = (er * er + ei * ei) / w2;
+                    if (epo) epo[(row0 + sb + ks + kk) * nbw * nm + rr] = make_float2(er, ei);
                 }
-            }
-        } else {
-            for (int e = t; e < nk * nbw * nm; e += blockDim.x) {
-                const int kk = e / (nbw * nm), rr = e % (nbw * nm), m = rr / nm, nu2 = rr % nm;
-                if constexpr (!PAIRS || ONE2) {  // past the k+q window: zero (its t may not have been formed)
-                    if (m >= (sk_nb[ks + kk] & 0xff)) {
-                        g2[(row0 + slab0 + ks + kk) * nbw * nm + rr] = 0.f;
-                        if (epo) epo[(row0 + slab0 + ks + kk) * nbw * nm + rr] = make_float2(0.f, 0.f);
-                        continue;
-                    }
-                }
-                const int iq = sk_iq[ks + kk];
-                const cplx<float> *up = uph + (int64_t)nm * nm * iq + nm * nu2;
-                float er = 0.f, ei = 0.f;
-#pragma unroll
-                for (int n = 0; n < NW; n++) {  // nm <= NW: fixed trip count, loads issued together
-                    const int nn = min(n, nm - 1);
-                    const float on = n < nm ? 1.f : 0.f;
-                    const float2 tv = tq[(kk * NBW + m) * NW + nn];
-                    const cplx<float> p = up[nn];
-                    cmac(er, ei, on * tv.x, on * tv.y, p.re, p.im);
-                }
-                const float w2 = 2.f * static_cast<float>(omega[(int64_t)nm * iq + nu2]);
-                g2[(row0 + slab0 + ks + kk) * nbw * nm + rr] = (er * er + ei * ei) / w2;
-                if (epo) epo[(row0 + slab0 + ks + kk) * nbw * nm + rr] = make_float2(er, ei);
             }
         }
+        if (!SEG || L1 >= m2) break;
+        L0 = L1;
+        L1 = segment_end(L0);
+        __syncthreads();  // the last chunk's reads of the segment tables done
+        fill(L0, L1);
     }
 }
 
@@ -1407,9 +1430,8 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
     EPHIP_REQUIRE(n1 >= 1 && n1 <= 32 && np >= 3 && np <= 8, "gridopt_kq_rotate: Rp box n1=%d (<= 32), np=%d (3..8)",
                   n1, np);
     EPHIP_REQUIRE(qgrid && qgrid[0] >= 1 && qgrid[1] >= 1 && qgrid[2] >= 1 && qgrid[2] <= GQ_M3_MAX &&
-                      qgrid[1] <= GQ_M2_MAX && qgrid[0] <= 65535 && qgrid[1] * qgrid[2] <= GQ_SLAB_MAX,
-                  "gridopt_kq_rotate: q grid must be n1 n2 n3 with n2, n3 <= %d and n2 n3 <= %d", GQ_M3_MAX,
-                  GQ_SLAB_MAX);
+                      qgrid[1] <= GQ_M2_MAX && qgrid[0] <= 65535,
+                  "gridopt_kq_rotate: q grid must be n1 n2 n3 with n1 <= 65535 and n2, n3 <= %d", GQ_M3_MAX);
     EPHIP_REQUIRE(nunits >= 0 && nunits <= 65535 && ldr >= (int64_t)nw * nm && lda >= ldr && cs >= 0,
                   "gridopt_kq_rotate: bad sizes");
     if (nunits == 0) return EPHIP_OK;
@@ -1444,7 +1466,9 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
     }
     // folded stage-2/3 sums when the Rp box spans -h .. h along axes 2 and 3 (odd np, tuner kind 8 = 0)
     const bool sym = (np & 1) && p2min == -(np / 2) && p3min == -(np / 2) && ephip_ft_get_variant(8) == 0;
-    if (ephip_ft_get_variant(6) == 1) {  // stage 3 and the k+q-window rotation on MFMAs
+    // stage 3 and the k+q-window rotation on MFMAs (its slab tables in LDS at once: n2 n3 <= GQ_SLAB_MAX;
+    // the VALU form takes larger slabs in segments)
+    if (ephip_ft_get_variant(6) == 1 && qgrid[1] * qgrid[2] <= GQ_SLAB_MAX) {
         using KM = decltype(&gridopt_kq_rotate_mfma_kernel<3>);
 #define GQM_ROW(c, d) {gridopt_kq_rotate_mfma_kernel<3, c, d>, gridopt_kq_rotate_mfma_kernel<4, c, d>, \
                        gridopt_kq_rotate_mfma_kernel<5, c, d>, gridopt_kq_rotate_mfma_kernel<6, c, d>, \
@@ -1472,26 +1496,29 @@ extern "C" int ephip_gridopt_kq_rotate(int32_t nw, int32_t nm, int32_t nbw, int3
     const int nd2 = nw * nm;
     const unsigned threads = (unsigned)(((nd2 + 63) / 64) * 64);
     using K = decltype(&gridopt_kq_rotate_kernel<3, 2>);
-#define GQ_ROW(nb, c, d) {gridopt_kq_rotate_kernel<3, nb, c, d>, gridopt_kq_rotate_kernel<4, nb, c, d>, \
-                          gridopt_kq_rotate_kernel<5, nb, c, d>, gridopt_kq_rotate_kernel<6, nb, c, d>, \
-                          gridopt_kq_rotate_kernel<7, nb, c, d>, gridopt_kq_rotate_kernel<8, nb, c, d>}
-    static const K table[2][2][3][6] = {
-        {{GQ_ROW(2, 0, false), GQ_ROW(3, 0, false), GQ_ROW(4, 0, false)},
-         {GQ_ROW(2, 12, false), GQ_ROW(3, 12, false), GQ_ROW(4, 12, false)}},
-        {{GQ_ROW(2, 0, true), GQ_ROW(3, 0, true), GQ_ROW(4, 0, true)},
-         {GQ_ROW(2, 12, true), GQ_ROW(3, 12, true), GQ_ROW(4, 12, true)}}};
+#define GQ_ROW(nb, c, d, g) {gridopt_kq_rotate_kernel<3, nb, c, d, false, g>, gridopt_kq_rotate_kernel<4, nb, c, d, false, g>, \
+                             gridopt_kq_rotate_kernel<5, nb, c, d, false, g>, gridopt_kq_rotate_kernel<6, nb, c, d, false, g>, \
+                             gridopt_kq_rotate_kernel<7, nb, c, d, false, g>, gridopt_kq_rotate_kernel<8, nb, c, d, false, g>}
+#define GQ_TAB(g) {{{GQ_ROW(2, 0, false, g), GQ_ROW(3, 0, false, g), GQ_ROW(4, 0, false, g)}, \
+                    {GQ_ROW(2, 12, false, g), GQ_ROW(3, 12, false, g), GQ_ROW(4, 12, false, g)}}, \
+                   {{GQ_ROW(2, 0, true, g), GQ_ROW(3, 0, true, g), GQ_ROW(4, 0, true, g)}, \
+                    {GQ_ROW(2, 12, true, g), GQ_ROW(3, 12, true, g), GQ_ROW(4, 12, true, g)}}}
+    static const K table[2][2][2][3][6] = {GQ_TAB(false), GQ_TAB(true)};  // [seg][dense][nm 12][nbw][np]
+#undef GQ_TAB
 #undef GQ_ROW
-#define GQS_ROW(nb, c, d) {gridopt_kq_rotate_kernel<3, nb, c, d, true>, gridopt_kq_rotate_kernel<5, nb, c, d, true>, \
-                           gridopt_kq_rotate_kernel<7, nb, c, d, true>}
-    static const K stable[2][2][3][3] = {
-        {{GQS_ROW(2, 0, false), GQS_ROW(3, 0, false), GQS_ROW(4, 0, false)},
-         {GQS_ROW(2, 12, false), GQS_ROW(3, 12, false), GQS_ROW(4, 12, false)}},
-        {{GQS_ROW(2, 0, true), GQS_ROW(3, 0, true), GQS_ROW(4, 0, true)},
-         {GQS_ROW(2, 12, true), GQS_ROW(3, 12, true), GQS_ROW(4, 12, true)}}};
+#define GQS_ROW(nb, c, d, g) {gridopt_kq_rotate_kernel<3, nb, c, d, true, g>, gridopt_kq_rotate_kernel<5, nb, c, d, true, g>, \
+                              gridopt_kq_rotate_kernel<7, nb, c, d, true, g>}
+#define GQS_TAB(g) {{{GQS_ROW(2, 0, false, g), GQS_ROW(3, 0, false, g), GQS_ROW(4, 0, false, g)}, \
+                     {GQS_ROW(2, 12, false, g), GQS_ROW(3, 12, false, g), GQS_ROW(4, 12, false, g)}}, \
+                    {{GQS_ROW(2, 0, true, g), GQS_ROW(3, 0, true, g), GQS_ROW(4, 0, true, g)}, \
+                     {GQS_ROW(2, 12, true, g), GQS_ROW(3, 12, true, g), GQS_ROW(4, 12, true, g)}}}
+    static const K stable[2][2][2][3][3] = {GQS_TAB(false), GQS_TAB(true)};
+#undef GQS_TAB
 #undef GQS_ROW
+    const int seg = qgrid[1] * qgrid[2] > GQ_SLAB_MAX;  // the slab in segments of whole q lines
     const K kern = (sym && np <= 7)
-                       ? stable[dense ? 1 : 0][nm == 12 ? 1 : 0][nbw <= 2 ? 0 : nbw - 2][(np - 3) / 2]
-                       : table[dense ? 1 : 0][nm == 12 ? 1 : 0][nbw <= 2 ? 0 : nbw - 2][np - 3];
+                       ? stable[seg][dense ? 1 : 0][nm == 12 ? 1 : 0][nbw <= 2 ? 0 : nbw - 2][(np - 3) / 2]
+                       : table[seg][dense ? 1 : 0][nm == 12 ? 1 : 0][nbw <= 2 ? 0 : nbw - 2][np - 3];
     kern<<<g, threads, 0, as_stream(stream)>>>(nm, n1, p1min, p2min, p3min, rmap, s1, s2, s3, qgrid[0], qgrid[1],
                                                 qgrid[2], static_cast<const cplx<float> *>(A), lda, ldr, line_off,
                                                 col_index, ikq, cs, static_cast<const cplx<float> *>(U), bmin, bmax,

@@ -1015,7 +1015,7 @@ class _GridOptKQ:
         lo, hi = r.min(0), r.max(0)
         n = hi - lo + 1
         m = np.asarray(st.qgrid, dtype=np.int64)
-        if n[0] > 32 or n[1] > 8 or n[2] > 8 or m[1] > 64 or m[2] > 64 or m[1] * m[2] > 1024 or m[0] > 65535:
+        if n[0] > 32 or n[1] > 8 or n[2] > 8 or m[1] > 64 or m[2] > 64 or m[0] > 65535:
             return None
         return cls(st, r, lo, n, m)
 

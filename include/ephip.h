@@ -310,7 +310,9 @@ int ephip_gridopt_kR_project_mfma(int32_t nw, int32_t nm, int32_t nrp, int32_t n
  * (p1min + a, p2min + b, p3min + c), a < n1 <= 32, b, c < np (3..8); rmap (int32, 8*n1*np, index
  * b + 8*(a + n1*c), rows padded to 8) gives the Rp index or -1. line_off (int32, nunits x (m1*m2 + 1)) holds each unit's
  * first kept-list position per q line, from ephip_kq_line_offsets (lower bounds in the sorted kept
- * lists of ephip_kq_compact). nw = 16, nm <= 16, nbw <= 4, m2, m3 <= 64, m2*m3 <= 1024. dense_strides: host
+ * lists of ephip_kq_compact). nw = 16, nm <= 16, nbw <= 4, m1 <= 65535, m2, m3 <= 64 (the kept q of a (unit, q1)
+ * slab are taken in segments of whole q lines of at most 1024 entries; the MFMA form, tuner kind 6 = 1,
+ * runs for m2*m3 <= 1024 and the vector form otherwise). dense_strides: host
  * int32[3] (s1, s2, s3) when the box is dense (n1 = np and every Rp present, Rp index = s1 a + s2 b + s3 c;
  * the operator is then addressed without rmap), or NULL. ep: NULL, or complex64 in the layout of g2 (one
  * complex per g2 entry): the rotated matrix elements themselves (zero past the k+q window). b1: NULL (each

@@ -426,6 +426,42 @@ def test_outer_k_gridopt_kq_long_lines(backend, model16, window, kq_variant):
     assert rel < 1e-4, rel
 
 
+@pytest.mark.parametrize("window", [0.5, 0.8])
+def test_outer_k_gridopt_kq_segmented_slab(backend, model16, window):
+    """Staged kR->kq on a q grid whose (unit, q1) slabs hold more than the kernel's 1024-entry LDS tables
+    (n2 n3 = 48 x 48 = 2304, about 1500 and 2000 of them kept at these windows): the slab is taken in
+    segments of whole q lines; kept lists exact, g2 against the direct FP32 kernels (1e-5) and the oracle
+    (1e-4)."""
+    from ephip import _lib as L
+    from ephip import eph
+    m, g = model16
+    g = dict(g, kgrid=(2, 48, 48), qgrid=(2, 48, 48), window=window)
+    w = g["window"]
+    st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
+    nks = st.ksurv.shape[0]
+    nb = 24
+    staged = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32)
+    direct = eph.OuterKLoop(st, nk_batch=nb, precision=L.PREC_FP32, gridopt=False)
+    assert staged.gridopt_q is not None and staged.gridopt is not None
+    rs, rd = eph.UnitRecorder(), eph.UnitRecorder()
+    for i0 in (0, max(0, nks - nb)):
+        staged.run_kbatch(i0, nb, [rs])
+        direct.run_kbatch(i0, nb, [rd])
+    most = 0
+    num = den = 0.0
+    for bs, bd in zip(rs.batches, rd.batches):
+        for u in range(len(bs["cols"])):
+            cols = np.asarray(bs["cols"][u])
+            assert np.array_equal(cols, bd["cols"][u])
+            most = max(most, int(np.bincount(cols // (48 * 48)).max(initial=0)))
+            num = max(num, float(np.abs(bs["g2"][u] - bd["g2"][u]).max(initial=0.0)))
+            den = max(den, float(np.abs(bd["g2"][u]).max(initial=0.0)))
+    assert most > 1024, most  # some (unit, q1) slab spans more than one segment
+    assert den > 0 and num / den < 1e-5, num / den
+    rel = _outer_k_vs_oracle(m, g, st, rs, qstep=331)
+    assert rel < 1e-4, rel
+
+
 @pytest.mark.parametrize("mfma", [True, False])
 def test_outer_k_gridopt_long_k_lines(backend, model16, mfma):
     """Staged RR->kR on a k grid whose (k1, k2) lines hold more surviving k than the projection kernels take

@@ -157,7 +157,7 @@ def test_staged_entries_validate_arguments():
     treat empty work as a no-op."""
     q = (C.c_int32 * 3)(24, 24, 24)
     q_long = (C.c_int32 * 3)(24, 80, 24)      # n2 > 64
-    q_slab = (C.c_int32 * 3)(24, 40, 40)      # n2 n3 > 1024
+    q_slab = (C.c_int32 * 3)(24, 40, 40)      # n2 n3 > 1024: taken in segments, accepted
     qp, qlp, qsp = (C.cast(a, C.c_void_p).value for a in (q, q_long, q_slab))
 
     def kq(nw=16, nm=12, nbw=2, nunits=1, n1=7, np_=7, qgrid=qp):
@@ -167,7 +167,7 @@ def test_staged_entries_validate_arguments():
 
     for kw, msg in [(dict(nw=8), "nw must be 16"), (dict(nm=17), "nw must be 16"), (dict(nbw=5), "outside 1..4"),
                     (dict(np_=9), "Rp box"), (dict(n1=33), "Rp box"), (dict(qgrid=qlp), "q grid"),
-                    (dict(qgrid=qsp), "q grid"), (dict(qgrid=None), "q grid"),
+                    (dict(qgrid=qsp), "null device pointer"), (dict(qgrid=None), "q grid"),
                     (dict(), "null device pointer")]:
         with pytest.raises(L.EphipError, match=msg):
             kq(**kw)
