@@ -426,6 +426,26 @@ def test_outer_k_gridopt_kq_long_lines(backend, model16, window, kq_variant):
     assert rel < 1e-4, rel
 
 
+def test_outer_k_wide_window_takes_direct_kernels(backend, model16):
+    """A window wider than the staged kR->kq kernel covers (more than 4 bands at k+q: nbw 5 here) runs
+    the outer-k loop on the direct FP32 kernels; g2 against the oracle (1e-4)."""
+    from ephip import _lib as L
+    from ephip import eph
+    m, g = model16
+    g = dict(g, qgrid=(4, 4, 4), window=3.0)
+    w = g["window"]
+    st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
+    assert st.nbw > 4, st.nbw
+    loop = eph.OuterKLoop(st, nk_batch=64, precision=L.PREC_FP32)
+    assert loop.gridopt_q is None
+    rec = eph.UnitRecorder()
+    nks = st.ksurv.shape[0]
+    for i0 in (0, max(0, nks - 64)):
+        loop.run_kbatch(i0, 64, [rec])
+    rel = _outer_k_vs_oracle(m, g, st, rec, qstep=5)
+    assert rel < 1e-4, rel
+
+
 @pytest.mark.parametrize("window", [0.5, 0.8])
 def test_outer_k_gridopt_kq_segmented_slab(backend, model16, window):
     """Staged kR->kq on a q grid whose (unit, q1) slabs hold more than the kernel's 1024-entry LDS tables
