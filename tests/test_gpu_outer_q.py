@@ -446,6 +446,26 @@ def test_outer_k_wide_window_takes_direct_kernels(backend, model16):
     assert rel < 1e-4, rel
 
 
+def test_outer_k_many_modes_take_direct_kr_kq(backend):
+    """More phonon modes than the staged kR->kq kernel covers (6 atoms, 18 modes > 16): the RR->kR side
+    stays staged, kR->kq runs on the direct FP32 kernels; g2 against the oracle (1e-4)."""
+    from ephip import _lib as L
+    from ephip import eph
+    m = M.make_model(nw=16, natoms=6, box_hd=5, box_g=3, seed=3033, g_dtype=np.complex128)
+    g = {"kgrid": (6, 6, 6), "qgrid": (3, 3, 3), "window": 0.5}
+    w = g["window"]
+    st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
+    assert m.nmodes == 18
+    loop = eph.OuterKLoop(st, nk_batch=48, precision=L.PREC_FP32)
+    assert loop.gridopt_q is None
+    rec = eph.UnitRecorder()
+    nks = st.ksurv.shape[0]
+    for i0 in (0, max(0, nks - 48)):
+        loop.run_kbatch(i0, 48, [rec])
+    rel = _outer_k_vs_oracle(m, g, st, rec, qstep=4)
+    assert rel < 1e-4, rel
+
+
 @pytest.mark.parametrize("window", [0.5, 0.8])
 def test_outer_k_gridopt_kq_segmented_slab(backend, model16, window):
     """Staged kR->kq on a q grid whose (unit, q1) slabs hold more than the kernel's 1024-entry LDS tables
