@@ -446,6 +446,26 @@ def test_outer_k_wide_window_takes_direct_kernels(backend, model16):
     assert rel < 1e-4, rel
 
 
+@pytest.mark.parametrize("nw", [12, 20])
+def test_outer_k_other_band_counts(backend, nw):
+    """Wannier band counts other than 8 and 16 (which the staged transforms cover) run the outer-k loop on
+    the direct FP32 kernels; kept lists exact and g2 against the oracle (1e-4)."""
+    from ephip import _lib as L
+    from ephip import eph
+    m = M.make_model(nw=nw, natoms=2, box_hd=5, box_g=3, seed=3100 + nw, g_dtype=np.complex128)
+    g = {"kgrid": (6, 6, 6), "qgrid": (3, 3, 3), "window": 0.5}
+    w = g["window"]
+    st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
+    loop = eph.OuterKLoop(st, nk_batch=48, precision=L.PREC_FP32)
+    assert loop.gridopt is None and loop.gridopt_q is None
+    rec = eph.UnitRecorder()
+    nks = st.ksurv.shape[0]
+    for i0 in (0, max(0, nks - 48)):
+        loop.run_kbatch(i0, 48, [rec])
+    rel = _outer_k_vs_oracle(m, g, st, rec, qstep=4)
+    assert rel < 1e-4, rel
+
+
 def test_outer_k_many_modes_take_direct_kr_kq(backend):
     """More phonon modes than the staged kR->kq kernel covers (6 atoms, 18 modes > 16): the RR->kR side
     stays staged, kR->kq runs on the direct FP32 kernels; g2 against the oracle (1e-4)."""
