@@ -53,12 +53,10 @@ def _oracle_states(m, g):
     return xk, E, U, keep, bmin, bmax, xq, ph
 
 
-@pytest.mark.parametrize("prec_name", ["fp32", "bf16x3"])
-def test_outer_q_loop_g2_parity(backend, cfg1, prec_name):
-    from ephip import _lib as L
+def _outer_q_parity(backend, m, g, prec, iqs):
+    """Outer-q tiles of the q in iqs against the oracle's RR->Rq->kq: k lists exact, |g|^2 within 1e-4
+    (degenerate-averaged, max-norm relative)."""
     from ephip import eph
-    m, g = cfg1
-    prec = L.PREC_FP32 if prec_name == "fp32" else L.PREC_BF16X3
     w = g["window"]
     st = eph.setup_outer_q(m, g["kgrid"], g["qgrid"], (-w, w), backend)
     xk, E, U, keep, bmin, bmax, xq, ph = _oracle_states(m, g)
@@ -66,7 +64,6 @@ def test_outer_q_loop_g2_parity(backend, cfg1, prec_name):
     assert np.array_equal(st.ksurv.cpu().numpy(), np.flatnonzero(keep))
     loop = eph.OuterQLoop(st, nk_batch=100, nq_tile=7, precision=prec)
     rec = eph.G2Recorder()
-    iqs = [0, 1, 9, 77, 300, 511]
     for iq in iqs:
         loop.run_qtile(iq, 1, [rec])
     nw, nm, nbw = m.nw, m.nmodes, st.nbw
@@ -99,6 +96,23 @@ def test_outer_q_loop_g2_parity(backend, cfg1, prec_name):
     e = np.array(errs)
     rel = e[:, 0].max() / e[:, 1].max()
     assert rel < 1e-4, rel
+
+
+@pytest.mark.parametrize("prec_name", ["fp32", "bf16x3"])
+def test_outer_q_loop_g2_parity(backend, cfg1, prec_name):
+    from ephip import _lib as L
+    m, g = cfg1
+    prec = L.PREC_FP32 if prec_name == "fp32" else L.PREC_BF16X3
+    _outer_q_parity(backend, m, g, prec, [0, 1, 9, 77, 300, 511])
+
+
+@pytest.mark.parametrize("nw", [12, 20])
+def test_outer_q_loop_other_band_counts(backend, nw):
+    """The plain outer-q loop at Wannier band counts other than 8 and 16 (FP32), against the oracle."""
+    from ephip import _lib as L
+    m = M.make_model(nw=nw, natoms=2, box_hd=5, box_g=3, seed=3200 + nw, g_dtype=np.complex128)
+    g = {"kgrid": (6, 6, 6), "qgrid": (3, 3, 3), "window": 0.5}
+    _outer_q_parity(backend, m, g, L.PREC_FP32, [0, 1, 13, 26])
 
 
 def _outer_k_vs_oracle(m, g, st, rec, qstep=23):
