@@ -151,12 +151,26 @@ class Model:
                           self.g_decay, dtype or np.complex64)
 
 
-def plant_mirror_degeneracy(ham: np.ndarray, irvec: np.ndarray, nw: int) -> np.ndarray:
-    """Symmetrise H(R) (nr, nw*nw column-major blocks) under the antiunitary A = J K combined with the
-    mirror k3 -> -k3, J = i sigma_y (x) 1 on the two halves of the orbitals (J^2 = -1):
-    H(R1, R2, -R3) = J H(R)^T J^{-1}. The map is an involution that preserves H(-R) = H(R)^dagger; on the
-    planes k3 = 0 and k3 = 1/2 the operator A commutes with H(k) and squares to -1, so every band there
-    is exactly two-fold degenerate (Kramers pairs)."""
+# The planted splitting f(k) = sin(2 pi 2 k2) sin(2 pi 3 k3) vanishes where 4 k2 or 6 k3 is an integer
+# (k2 in {0, 1/4, 1/2, 3/4}, k3 in {0, 1/6, ..., 5/6}): 1 - (44/48)(42/48) = 19.8% of a 48^3 grid.
+PLANT_SPLIT_R = (2, 3)  # the splitting's harmonics along k2 and k3
+
+
+def planted_degenerate(xk: np.ndarray) -> np.ndarray:
+    """Mask of the k (crystal coordinates, (nk, 3)) where the planted pairs stay exactly degenerate."""
+    a, b = PLANT_SPLIT_R
+    return (np.abs(np.sin(2 * np.pi * a * xk[:, 1])) < 1e-9) | (np.abs(np.sin(2 * np.pi * b * xk[:, 2])) < 1e-9)
+
+
+def plant_pair_degeneracy(ham: np.ndarray, irvec: np.ndarray, nw: int, seed: int, split: float = 0.5) -> np.ndarray:
+    """Two-fold degeneracies on a chosen fraction of the k-points. With J = i sigma_y (x) 1 on the two
+    halves of the orbitals (real, J^2 = -1), H(R) -> (H(R) + J H(R)^T J^-1) / 2 makes the antiunitary
+    J K commute with every H(k), so every band is exactly two-fold degenerate at every k; Hermiticity,
+    and H(-R) = H(R)* when H(R) is real, are kept. A splitting f(k) S is then added, S = (S0 - J S0 J^-1)/2
+    for a random real symmetric S0 (so J S^T J^-1 = -S lifts the pairs) and f(k) = sin(2 pi 2 k2)
+    sin(2 pi 3 k3), a real Fourier series on R = (0, +-2, +-3) with coefficients -s2 s3 / 4: the pairs
+    stay exactly degenerate where f = 0 (``planted_degenerate``) and split elsewhere, with near-
+    degenerate clusters next to those planes. Needs |R2| >= 2, |R3| >= 3 in the box."""
     if nw % 2:
         raise ValueError("planted degeneracies need an even number of Wannier functions")
     n = nw // 2
@@ -166,11 +180,21 @@ def plant_mirror_degeneracy(ham: np.ndarray, irvec: np.ndarray, nw: int) -> np.n
     Jinv = -J
     index = {tuple(r): i for i, r in enumerate(irvec.tolist())}
     out = np.empty_like(ham)
-    for ir, r in enumerate(irvec.tolist()):
-        jm = index[(r[0], r[1], -r[2])]
+    for ir in range(irvec.shape[0]):
         h = ham[ir].reshape(nw, nw, order="F")
-        hm = ham[jm].reshape(nw, nw, order="F")
-        out[ir] = (0.5 * (h + J @ hm.T @ Jinv)).ravel(order="F")
+        out[ir] = (0.5 * (h + J @ h.T @ Jinv)).ravel(order="F")
+    rng = np.random.default_rng(seed + 7919)
+    s0 = rng.standard_normal((nw, nw))
+    s0 = 0.5 * (s0 + s0.T)
+    S = 0.5 * (s0 - J @ s0 @ Jinv)
+    S *= split / np.abs(np.linalg.eigvalsh(S)).max()
+    a, b = PLANT_SPLIT_R
+    for s2 in (1, -1):
+        for s3 in (1, -1):
+            ir = index.get((0, a * s2, b * s3))
+            if ir is None:
+                raise ValueError("the R box does not hold the planted splitting's harmonics")
+            out[ir] += (-0.25 * s2 * s3 * S).astype(out.dtype).ravel(order="F")
     return out
 
 
@@ -209,14 +233,14 @@ def make_model(nw: int, natoms: int, box_hd: int, box_g: int, seed: int,
     out in the linearised BTE. ``time_reversal`` keeps the real parts of H(R) and of the symmetrised
     g (so E(k) = E(-k), v(-k) = -v(k)); the BTE right-hand side is then odd in k and the fixed-point
     iteration converges. ``planted_degeneracy`` makes every band exactly two-fold degenerate on the
-    planes k3 = 0 and k3 = 1/2 (``plant_mirror_degeneracy``)."""
+    k-points of ``planted_degenerate`` (19.8% of a 48^3 grid; ``plant_pair_degeneracy``)."""
     hermitian_g = hermitian_g or time_reversal
     irvec_h = rbox(box_hd)
     ham = hermitian_operator(seed, nw, irvec_h, h_decay, onsite, key=1)
     if time_reversal:
         ham = ham.real.astype(ham.dtype)
     if planted_degeneracy:
-        ham = plant_mirror_degeneracy(ham, irvec_h, nw)
+        ham = plant_pair_degeneracy(ham, irvec_h, nw, seed)
     dyn = dynamical_matrix(seed, natoms, irvec_h).astype(np.complex128)
     nm = 3 * natoms
     irvec_g = rbox(box_g)
@@ -241,7 +265,7 @@ CONFIGS = {
                  g_dtype=np.complex128, kgrid=(8, 8, 8), qgrid=(8, 8, 8), window=1.0),
     "cfg2": dict(nw=16, natoms=4, box_hd=9, box_g=7, seed=1002, onsite=(-2.0, 2.0),
                  g_dtype=np.complex64, kgrid=(48, 48, 48), qgrid=(24, 24, 24), window=0.3),
-    # cfg2's model with exact two-fold degeneracies planted on the planes k3 = 0, 1/2 (precision form)
+    # cfg2's model with exact two-fold degeneracies planted on 19.8% of the k-points (precision form)
     "cfg3": dict(nw=16, natoms=4, box_hd=9, box_g=7, seed=1002, onsite=(-2.0, 2.0), planted_degeneracy=True,
                  g_dtype=np.complex64, kgrid=(48, 48, 48), qgrid=(24, 24, 24), window=0.3),
     "cfg4": dict(nw=16, natoms=4, box_hd=9, box_g=7, seed=1004, onsite=(-3.0, 3.0),

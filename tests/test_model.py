@@ -59,3 +59,27 @@ def test_eph_decay_envelope():
     env = np.exp(-(ne[None, :] + npn[:, None]) / 0.8)
     ratio = g / env
     assert 0.5 < ratio.mean() < 1.5 and ratio.std() / ratio.mean() < 0.5
+
+
+def test_planted_degeneracy_fraction_and_pairs():
+    """cfg3's planting: exact two-fold pairs on model.planted_degenerate (19.8% of a 48^3 grid: the planes
+    4 k2 in Z or 6 k3 in Z), split pairs elsewhere; H(R) stays Hermitian, and the time-reversal variant
+    keeps E(-k) = E(k)."""
+    for tr in (False, True):
+        m, g = M.make_config_model("cfg3", with_g=False, time_reversal=tr)
+        for i in range(len(m.irvec_h)):
+            j = int(np.flatnonzero((m.irvec_h == -m.irvec_h[i]).all(1))[0])
+            a = m.ham_r[i].reshape(m.nw, m.nw, order="F")
+            b = m.ham_r[j].reshape(m.nw, m.nw, order="F")
+            assert np.allclose(b, a.conj().T, atol=1e-14)
+        xk = M.kpoints_grid((1, 48, 48))
+        E, _ = O.heev_batched(O.fourier(m.ham_r, m.irvec_h, xk))
+        mask = M.planted_degenerate(xk)
+        assert abs(mask.mean() - (1 - (44 / 48) * (42 / 48))) < 1e-12   # 19.8%
+        split = np.abs(E[:, 1::2] - E[:, 0::2]).max(1)
+        assert split[mask].max() < 1e-11
+        assert split[~mask].min() > 1e-3
+        if tr:
+            Em, _ = O.heev_batched(O.fourier(m.ham_r, m.irvec_h, -xk))
+            assert np.abs(Em - E).max() < 1e-11
+    assert abs(M.planted_degenerate(M.kpoints_grid((48, 48, 48))).mean() - 0.1979) < 1e-3
