@@ -597,12 +597,16 @@ extern "C" int ephip_heev_refine(int32_t n, int64_t nb, const void *H64, void *U
     int st = heev_launch_check(n, nb);
     if (st) return st;
     if (nb == 0) return EPHIP_OK;
-    EPHIP_REQUIRE(n <= 31, "heev_refine: n=%d > 31 (the FP64 working set of one matrix exceeds 64 KiB of LDS)", n);
     EPHIP_REQUIRE(H64 && U && E64, "heev_refine: null device pointer");
     EPHIP_REQUIRE(tol >= 0.0, "heev_refine: negative tolerance");
     (void)E32;
     const int wpb = waves_per_block(sizeof(double) * refine_lds_doubles(n));
     const size_t lds = wpb * sizeof(double) * refine_lds_doubles(n);
+    if (lds > 64 * 1024) {  // n = 32: one matrix's FP64 working set (67 KiB) in a one-wave block; gfx950 has
+                            // 160 KiB of LDS per workgroup
+        EPHIP_TRY_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(heev_refine_kernel),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
     heev_refine_kernel<<<(unsigned)((nb + wpb - 1) / wpb), 64 * wpb, lds, as_stream(stream)>>>(
         n, nb, static_cast<const c64 *>(H64), static_cast<c32 *>(U), E64, tol > 0.0 ? tol : 1e-15);
     EPHIP_LAUNCHED();
@@ -613,7 +617,6 @@ extern "C" int ephip_degenerate_gauge_fix(int32_t n, int64_t nb, const double *E
                                           void *stream) {
     int st = heev_launch_check(n, nb);
     if (st) return st;
-    EPHIP_REQUIRE(n <= 31, "degenerate_gauge_fix: n=%d > 31", n);
     EPHIP_REQUIRE(cutoff >= 0.0, "degenerate_gauge_fix: negative cutoff");
     EPHIP_REQUIRE(nb <= 0x7fffffffLL, "degenerate_gauge_fix: batch too large");
     if (nb == 0) return EPHIP_OK;
@@ -641,7 +644,6 @@ extern "C" int ephip_eigen_batched(int32_t n, int64_t nb, const void *H64, doubl
                                    int32_t *info, void *stream) {
     int st = heev_launch_check(n, nb);
     if (st) return st;
-    EPHIP_REQUIRE(n <= 31, "eigen_batched: n=%d > 31 (FP64 refinement limit)", n);
     if (nb == 0) return EPHIP_OK;
     EPHIP_REQUIRE(H64 && E64 && U && work, "eigen_batched: null device pointer");
     return eigen_refined_run(n, nb, H64, E64, U, work, info, stream);
@@ -651,7 +653,6 @@ extern "C" int ephip_eigvals_batched(int32_t n, int64_t nb, const void *H64, dou
                                      void *stream) {
     int st = heev_launch_check(n, nb);
     if (st) return st;
-    EPHIP_REQUIRE(n <= 31, "eigvals_batched: n=%d > 31 (FP64 refinement limit)", n);
     if (nb == 0) return EPHIP_OK;
     EPHIP_REQUIRE(H64 && E64 && work, "eigvals_batched: null device pointer");
     char *w = static_cast<char *>(work);
@@ -697,7 +698,6 @@ extern "C" int ephip_eigvals_batched_noalloc(int32_t n, int64_t nb, const void *
                                              void *stream) {
     int st = heev_launch_check(n, nb);
     if (st) return st;
-    EPHIP_REQUIRE(n <= 31, "eigvals_batched_noalloc: n=%d > 31 (FP64 refinement limit)", n);
     if (nb == 0) return EPHIP_OK;
     const int64_t per = eigvals_work_bytes_per_matrix(n);
     const int64_t cap = (int64_t)(EIG_SCRATCH_BYTES / (size_t)per);

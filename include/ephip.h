@@ -82,7 +82,7 @@ int ephip_heev_c32(int32_t n, int64_t nb, const void *H, int32_t h_is_c64, void 
  * eigenvectors U are orthonormalised in FP64, H64 is projected onto them (M = Q^H H Q, unitarily similar
  * to H and nearly diagonal) and M is diagonalised by parallel cyclic Jacobi sweeps in FP64 until the
  * off-diagonal Frobenius norm is <= tol x the Frobenius norm (tol = 0: 1e-15). U (c32) is overwritten
- * with the refined vectors; E64: double (n x nb) ascending. n <= 31. E32 is not read. Gives the FP64
+ * with the refined vectors; E64: double (n x nb) ascending. n <= 32. E32 is not read. Gives the FP64
  * accuracy of the reference's zheev eigenvalues (src/common/AllocatedLAPACK.jl:44-108), also for
  * degenerate bands. */
 int ephip_heev_refine(int32_t n, int64_t nb, const void *H64, void *U, const float *E32, double *E64,
@@ -94,21 +94,21 @@ int ephip_heev_refine(int32_t n, int64_t nb, const void *H64, void *U, const flo
  * each group of m > 1 bands, U_g <- U_g V where V diagonalises U_g^H P U_g (P: the reference's fixed
  * perturbation matrix) with ascending eigenvalues. Matrices with no neighbouring pair closer than cutoff
  * are untouched. cutoff: the reference's electron_degen_cutoff is 0.01 meV (src/common/constants.jl:6),
- * 1e-5 in eV. n <= 31. */
+ * 1e-5 in eV. n <= 32. */
 int ephip_degenerate_gauge_fix(int32_t n, int64_t nb, const double *E64, void *U, double cutoff, void *stream);
 
 /* eigen_batched(Hk) -- src/wannier_to_bloch_batched.jl:43-65 (CUDA method ext/ElectronPhononCUDAExt.jl:101-115),
  * at the FP64 accuracy of the reference's zheev path: ephip_heev_c32 then ephip_heev_refine. H64: c64
  * (n x n x nb), upper triangle read, not modified. E64: double (n x nb) ascending; U: c32 (n x n x nb)
  * eigenvectors; work: float (n x nb) device workspace (the FP32 eigenvalues); info: int32 per matrix
- * (0 = the FP32 solve converged) or NULL. n <= 31. */
+ * (0 = the FP32 solve converged) or NULL. n <= 32. */
 int ephip_eigen_batched(int32_t n, int64_t nb, const void *H64, double *E64, void *U, float *work, int32_t *info,
                         void *stream);
 
 /* eigvals_batched(Hk) -- src/wannier_to_bloch_batched.jl:30-41 (CUDA method ext/ElectronPhononCUDAExt.jl:80-93):
  * eigenvalues only, E64 double (n x nb) ascending, FP64-refined as in ephip_eigen_batched. work: device
  * workspace of nb (8 n^2 + 4 n) bytes, 8-byte aligned (the FP32 eigenvectors the refinement needs, then the
- * FP32 eigenvalues). info as in ephip_eigen_batched. n <= 31. */
+ * FP32 eigenvalues). info as in ephip_eigen_batched. n <= 32. */
 int ephip_eigvals_batched(int32_t n, int64_t nb, const void *H64, double *E64, void *work, int32_t *info,
                           void *stream);
 
@@ -117,7 +117,7 @@ int ephip_eigvals_batched(int32_t n, int64_t nb, const void *H64, double *E64, v
  * loops, harnesses that forbid hipMalloc). Same results as ephip_eigvals_batched; its workspace is a
  * file-scope device array of 16 MiB per GPU (held until the process ends), so nb <= floor(16 MiB /
  * (8 n^2 + 4 n)) (7943 matrices at n = 16); larger nb returns EPHIP_ERR_ARG. Calls that overlap on one GPU
- * share the array and race: serialise them (one stream, or events). n <= 31. */
+ * share the array and race: serialise them (one stream, or events). n <= 32. */
 int ephip_eigvals_batched_noalloc(int32_t n, int64_t nb, const void *H64, double *E64, int32_t *info, void *stream);
 
 /* _filter_kpoints / inside_window -- src/filter.jl:19, :40-100 (device branch :46-72).

@@ -120,7 +120,7 @@ def test_heev_c32_parity(backend, n):
         assert np.abs(V.conj().T @ V - np.eye(n)).max() < 3e-7 * n
 
 
-@pytest.mark.parametrize("n,split", [(8, 0.0), (16, 0.0), (16, 3e-6), (12, 5e-5)])
+@pytest.mark.parametrize("n,split", [(8, 0.0), (16, 0.0), (16, 3e-6), (12, 5e-5), (32, 0.0), (32, 3e-6)])
 def test_heev_refine_fp64(backend, n, split):
     """FP32 solve + FP64 refinement (Loewdin orthonormalisation, projection, parallel Jacobi) reaches
     FP64-grade eigenvalues, also on exactly and nearly degenerate pairs; vectors are orthonormal to
@@ -139,7 +139,7 @@ def test_heev_refine_fp64(backend, n, split):
         assert np.abs(h[b] @ V - V * E[b]).max() < 1e-6 * np.abs(wl).max()
 
 
-@pytest.mark.parametrize("n", [8, 16, 31])
+@pytest.mark.parametrize("n", [8, 16, 31, 32])
 def test_degenerate_gauge_fix_parity(backend, n):
     """ephip_degenerate_gauge_fix after the refined eigensolve against the oracle's solve_eigen_el! gauge
     fixing (src/diagonalize.jl:47-85) on matrices with planted 2- and 3-fold groups, pairs split by 2e-6

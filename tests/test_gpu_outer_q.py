@@ -460,7 +460,7 @@ def test_outer_k_wide_window_takes_direct_kernels(backend, model16):
     assert rel < 1e-4, rel
 
 
-@pytest.mark.parametrize("nw", [12, 20])
+@pytest.mark.parametrize("nw", [12, 20, 32])
 def test_outer_k_other_band_counts(backend, nw):
     """Wannier band counts other than 8 and 16 (which the staged transforms cover) run the outer-k loop on
     the direct FP32 kernels; kept lists exact and g2 against the oracle (1e-4)."""

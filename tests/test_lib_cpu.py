@@ -122,8 +122,8 @@ def test_binding_arity_matches_header():
 
 
 def test_eigen_entries_validate_arguments():
-    with pytest.raises(L.EphipError, match="n=32 > 31"):
-        L.call("ephip_eigen_batched", 32, 4, 1, 1, 1, 1, None, None)
+    with pytest.raises(L.EphipError, match="n=33 outside 1..32"):
+        L.call("ephip_eigen_batched", 33, 4, 1, 1, 1, 1, None, None)
     with pytest.raises(L.EphipError, match="null device pointer"):
         L.call("ephip_eigvals_batched", 16, 4, 1, 1, None, None, None)
     L.call("ephip_eigvals_batched", 16, 0, None, None, None, None, None)
@@ -131,8 +131,8 @@ def test_eigen_entries_validate_arguments():
 
 
 def test_degenerate_gauge_fix_validates_arguments():
-    with pytest.raises(L.EphipError, match="n=32 > 31"):
-        L.call("ephip_degenerate_gauge_fix", 32, 4, 1, 1, 1e-5, None)
+    with pytest.raises(L.EphipError, match="n=33 outside 1..32"):
+        L.call("ephip_degenerate_gauge_fix", 33, 4, 1, 1, 1e-5, None)
     with pytest.raises(L.EphipError, match="negative cutoff"):
         L.call("ephip_degenerate_gauge_fix", 8, 4, 1, 1, -1.0, None)
     with pytest.raises(L.EphipError, match="null device pointer"):
@@ -145,8 +145,8 @@ def test_eigvals_noalloc_validates_before_any_device_call():
     before touching the device; nb = 0 is a no-op."""
     with pytest.raises(L.EphipError, match="scratch"):
         L.call("ephip_eigvals_batched_noalloc", 16, 7944, 1, 1, None, None)
-    with pytest.raises(L.EphipError, match="n=32 > 31"):
-        L.call("ephip_eigvals_batched_noalloc", 32, 4, 1, 1, None, None)
+    with pytest.raises(L.EphipError, match="n=33 outside 1..32"):
+        L.call("ephip_eigvals_batched_noalloc", 33, 4, 1, 1, None, None)
     with pytest.raises(L.EphipError, match="null device pointer"):
         L.call("ephip_eigvals_batched_noalloc", 16, 7943, None, None, None, None)
     L.call("ephip_eigvals_batched_noalloc", 16, 0, None, None, None, None)
