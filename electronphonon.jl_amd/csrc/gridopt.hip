@@ -830,10 +830,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
     // (cfg4, window width 3: 12.6 instead of 15.7 ms per 1152-k batch)
     constexpr int CHUNK = NBW >= 4 ? GQ_CHUNK / 2 : NBW == 3 ? GQ_CHUNK * 3 / 4 : GQ_CHUNK;
     __shared__ float2 ucol[CHUNK * NBW * NW];
-    __shared__ __attribute__((aligned(16))) float2 tq[CHUNK * NBW * NW];  // t[kk][m][nu], nu < nm <= 16
-    // where the lanes that write no t of a column store (per lane), so that the column loop has no branch
-    // around its stores
-    __shared__ float tq_sink[256];
+    // t[kk][m][nu], nu < nm <= 16. Every lane of a row ends its row sum holding one of the values the row
+    // writes, and the lanes holding one value store it together (the same word from 4 or 8 lanes), so the
+    // column loop stores with no branch and no lane select; the rows of t >= 16 nm (a partly filled last
+    // wave, nu < 16 as blocks hold at most 256 threads) store into slots nu >= nm, which nothing reads.
+    __shared__ __attribute__((aligned(16))) float2 tq[CHUNK * NBW * NW];
     int q1, u;
     kq_block(grouped, q1, u);
     const int t = threadIdx.x;
@@ -1016,17 +1017,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
 #pragma unroll
                     for (int c = 0; c < N; c++) cmac(gr, gi, e3[c].x, e3[c].y, b2r[c], b2i[c]);
                 }
-                // conj(w) g summed over the 16 lanes of the row. Even widths: the column pairs, lanes 4s of
-                // the row write value s of each pair (re, im of its first column, re, im of its second). Odd
-                // widths: one column when the k+q window holds one band (uniform; most kept q), lanes 0 and 8
-                // of the row writing its re and im, else dpp_row_sum4 over the columns. Width 2 on the
-                // symmetric box: one column when the window holds one band, lanes 0 and 4 writing re and im
+                // conj(w) g summed over the 16 lanes of the row. Even widths: the column pairs, lanes 4s .. 4s + 3
+                // of the row write value s of each pair (re, im of its first column, re, im of its second). Odd
+                // widths: one column when the k+q window holds one band (uniform; most kept q), lanes 0 .. 7 and
+                // 8 .. 15 of the row writing its re and im, else dpp_row_sum4 over the columns. Width 2 on the
+                // symmetric box: one column when the window holds one band, the lanes with bit 2 of i clear
+                // writing re and those with it set im
                 float *tqf = reinterpret_cast<float *>(tq);
                 if constexpr (ONE2) {
                     if ((w23 & 0xff) <= 1) {  // uniform
-                        float x = dpp_row_sum2_bit2(wo[0], gr, gi, gi, -gr);
-                        asm volatile("" : "+v"(x));  // all lanes form it, as below
-                        *(rv && (i & 11) == 0 ? &tqf[(kk * NBW * NW + nu) * 2 + ((i >> 2) & 1)] : &tq_sink[t]) = x;
+                        const float x = dpp_row_sum2_bit2(wo[0], gr, gi, gi, -gr);
+                        tqf[(kk * NBW * NW + nu) * 2 + ((i >> 2) & 1)] = x;
                         continue;
                     }
                 }
@@ -1035,9 +1036,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
                         // the parts of g by bit 3 as in the pair sums (bit 2), held so with SWAP
                         const float p1 = SWAP ? gr : h3 ? gi : gr, p2 = SWAP ? gi : h3 ? -gr : gi;
                         const float q1 = SWAP ? gi : h3 ? gr : gi, q2 = SWAP ? -gr : h3 ? gi : -gr;
-                        float x = dpp_row_sum2(w[0], p1, p2, q1, q2);
-                        asm volatile("" : "+v"(x));  // all lanes form it, as below
-                        *(rv && (i & 7) == 0 ? &tqf[(kk * NBW * NW + nu) * 2 + h3] : &tq_sink[t]) = x;
+                        const float x = dpp_row_sum2(w[0], p1, p2, q1, q2);
+                        tqf[(kk * NBW * NW + nu) * 2 + h3] = x;
                         continue;
                     }
                 }
@@ -1049,12 +1049,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
                     const float gq1 = SWAP ? gi : h2 ? gr : gi, gq2 = SWAP ? -gr : h2 ? gi : -gr;
 #pragma unroll
                     for (int h = 0; h < NBW / 2; h++) {
-                        float x = dpp_row_sum4_pairs(wo[h], wx[h], gp1, gp2, gq1, gq2);
-                        // formed by every lane here, so that its last DPP addition is not moved into the store's
-                        // branch (where it takes a separate DPP move and addition)
-                        asm volatile("" : "+v"(x));
+                        const float x = dpp_row_sum4_pairs(wo[h], wx[h], gp1, gp2, gq1, gq2);
                         const int vi = 4 * h + (i >> 2);  // value index: column vi / 2, real (0) or imaginary (1)
-                        *(rv && (i & 3) == 0 ? &tqf[((kk * NBW + (vi >> 1)) * NW + nu) * 2 + (vi & 1)] : &tq_sink[t]) = x;
+                        tqf[((kk * NBW + (vi >> 1)) * NW + nu) * 2 + (vi & 1)] = x;
                     }
                 } else {
                     float v[2 * NBW];

@@ -10,9 +10,10 @@ on random data so that a wrong partner lane, sign or mirror fails:
 * the lanes that keep the imaginary part hold i conj(B1) mirrored over b and c, which makes their stage
   sums i conj(g) = (g_i, g_r), and read conj(w): their w.x p1 + w.y p2 is then Im(conj(w) g) and
   w.x p2 - w.y p1 is Re(conj(w) g), the same formulas as the other lanes with (p1, p2) = (g_r, g_i);
-* the DPP networks (row rotation by 8, half-row mirror, quad permutations) leave the row sums where the
-  kernel stores them: lanes 0 and 4 (one column, parts by bit 2), lanes 0 and 8 (one column, parts by
-  bit 3), lanes 4s (two columns, value s).
+* the DPP networks (row rotation by 8, half-row mirror, quad permutations) leave the row sums in every
+  lane, where the kernel stores them from: lanes with bit 2 clear / set hold re / im (one column, parts by
+  bit 2), the same by bit 3 (one column, parts by bit 3), lanes 4s .. 4s + 3 value s (two columns); the
+  lanes holding one value store it together.
 """
 import numpy as np
 import pytest
@@ -119,8 +120,8 @@ QUAD_1032 = lambda l: l ^ 1                           # noqa: E731
 
 
 def test_one_column_row_sum_parts_by_bit2():
-    """dpp_row_sum2_bit2 with the lane arrangement by bit 2 (width 2, one band at k+q): lanes 0 and 4
-    end with re and im of sum_i conj(w_i) g_i."""
+    """dpp_row_sum2_bit2 with the lane arrangement by bit 2 (width 2, one band at k+q): the lanes with
+    bit 2 clear end with re, those with it set with im of sum_i conj(w_i) g_i."""
     g, w = _cplx(16), _cplx(16)
     target = np.sum(np.conj(w) * g)
     swap = (np.arange(16) & 4) != 0
@@ -129,15 +130,15 @@ def test_one_column_row_sum_parts_by_bit2():
     x = x + _take(x, QUAD_2301)
     x = x + _take(x, QUAD_1032)
     x = x + _take(x, ROR8)
-    assert np.isclose(x[0], target.real) and np.isclose(x[4], target.imag)
+    assert np.allclose(x, np.where(np.arange(16) & 4, target.imag, target.real))
     # a partner that keeps the same part (row rotation first, as for bit 3) would not sum the row
     y = x0 + _take(x1, ROR8)
     assert not np.isclose(y[0] + y[1] + y[2] + y[3], target.real)
 
 
 def test_one_column_row_sum_parts_by_bit3():
-    """dpp_row_sum2 with the lane arrangement by bit 3 (odd widths, one band at k+q): lanes 0 and 8 end
-    with re and im."""
+    """dpp_row_sum2 with the lane arrangement by bit 3 (odd widths, one band at k+q): lanes 0 .. 7 end
+    with re, lanes 8 .. 15 with im."""
     g, w = _cplx(16), _cplx(16)
     target = np.sum(np.conj(w) * g)
     swap = (np.arange(16) & 8) != 0
@@ -146,13 +147,13 @@ def test_one_column_row_sum_parts_by_bit3():
     x = x + _take(x, HALF_MIRROR)
     x = x + _take(x, QUAD_2301)
     x = x + _take(x, QUAD_1032)
-    assert np.isclose(x[0], target.real) and np.isclose(x[8], target.imag)
+    assert np.allclose(x, np.where(np.arange(16) & 8, target.imag, target.real))
 
 
 def test_pair_row_sums():
     """dpp_row_sum4_pairs: two columns a, b; lanes with bit 3 set take b as their own column (staged in
     that order), lanes with bit 2 set keep the imaginary part (arranged as above). Lanes 4s end with
-    value s of (re a, im a, re b, im b)."""
+    value s of (re a, im a, re b, im b) in lanes 4s .. 4s + 3."""
     g, wa, wb = _cplx(16), _cplx(16), _cplx(16)
     want = [np.sum(np.conj(wa) * g).real, np.sum(np.conj(wa) * g).imag,
             np.sum(np.conj(wb) * g).real, np.sum(np.conj(wb) * g).imag]
@@ -170,4 +171,4 @@ def test_pair_row_sums():
     x = x + _take(x, QUAD_2301)
     x = x + _take(x, QUAD_1032)
     for s in range(4):
-        assert np.isclose(x[4 * s], want[s]), s
+        assert np.allclose(x[4 * s:4 * s + 4], want[s]), s
