@@ -668,6 +668,10 @@ constexpr int GQ_M3_MAX = 64, GQ_SLAB_MAX = 1024;
 // t[m, nu] uph(q)[nu, nu'], g2 = |ep|^2 / (2 omega(q)[nu']) at row u cs + (kept position), slots m < nbw
 // (zero past the k+q window), the layout of ephip_eph_kq_rotate_packed_bf16x3.
 constexpr int GQ_CHUNK = 32, GQ_M2_MAX = 64;
+// U_kq loads per thread issued together when a chunk's window columns are staged (measured at 4 and 6: 6 at
+// window width 2, 2.99 against 3.13 ms per cfg2 batch; 4 at width 3, 7.88 against 8.07 ms at cfg4)
+template <int NBW>
+constexpr int gq_stage() { return NBW == 2 ? 6 : 4; }
 
 // Stage 1 of the staged kR->kq kernels for the row `row` of a unit operator (abase = the unit's first
 // element): B1[c][b] = sum_a e1[a] abase[row + ldr Rp(a, b, c)], a < n1, b, c < N.
@@ -944,19 +948,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 7 ? 3 
             const int ks = e0, nk = cut - e0;  // 0 < nk <= CHUNK
             e0 = cut;
             __syncthreads();  // segment tables written; the previous chunk's tables no longer read
-            for (int e = t; e < nk * NBW * NW; e += blockDim.x) {
-                const int kk = e / (NBW * NW), m = (e / NW) % NBW, ii = e % NW;
-                const int nbm = sk_nb[ks + kk], nbk = nbm & 0xff, mlo = (nbm >> 8) & 0xff;
-                float2 w = make_float2(0.f, 0.f);
-                const int ms = (ONE2 && nbk == 1) ? 0 : m;
-                if (ms < nbk) {
-                    const cplx<float> v = U[(int64_t)NW * NW * sk_kq[ks + kk] + ii + NW * (mlo + ms)];
-                    w = make_float2(v.re, (SWAP && (ii & SWAP_BIT)) ? -v.im : v.im);
+            // STAGE loads in flight per thread: every load of a round is issued (its index clamped to a
+            // written table entry, U[0] where no band is read) before the first is waited for
+            constexpr int STAGE = gq_stage<NBW>();
+            const int nstage = nk * NBW * NW;
+            for (int e0 = t; e0 < nstage; e0 += STAGE * blockDim.x) {
+                cplx<float> v[STAGE];
+                bool ok[STAGE];
+#pragma unroll
+                for (int r = 0; r < STAGE; r++) {
+                    const int e = e0 + r * blockDim.x;
+                    const int kk = min(e / (NBW * NW), nk - 1), m = (e / NW) % NBW, ii = e % NW;
+                    const int nbm = sk_nb[ks + kk], nbk = nbm & 0xff, mlo = (nbm >> 8) & 0xff;
+                    const int ms = (ONE2 && nbk == 1) ? 0 : m;
+                    ok[r] = e < nstage && ms < nbk;
+                    v[r] = U[ok[r] ? (int64_t)NW * NW * sk_kq[ks + kk] + ii + NW * (mlo + ms) : 0];
                 }
-                if constexpr (PAIRS) {  // per pair, the reading lane's own column first (bit 3 of i picks it)
-                    ucol[(kk * NBW + (m & ~1) + ((m & 1) ^ ((ii >> 3) & 1))) * NW + ii] = w;
-                } else {
-                    ucol[e] = w;
+#pragma unroll
+                for (int r = 0; r < STAGE; r++) {
+                    const int e = e0 + r * blockDim.x;
+                    if (e >= nstage) break;
+                    const int kk = e / (NBW * NW), m = (e / NW) % NBW, ii = e % NW;
+                    const float2 w = ok[r] ? make_float2(v[r].re, (SWAP && (ii & SWAP_BIT)) ? -v[r].im : v[r].im)
+                                           : make_float2(0.f, 0.f);
+                    if constexpr (PAIRS) {  // per pair, the reading lane's own column first (bit 3 of i picks it)
+                        ucol[(kk * NBW + (m & ~1) + ((m & 1) ^ ((ii >> 3) & 1))) * NW + ii] = w;
+                    } else {
+                        ucol[e] = w;
+                    }
                 }
             }
             __syncthreads();  // U columns staged
