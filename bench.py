@@ -40,7 +40,7 @@ PEAK = {"fp32": 157.3, "bf16x3": 2500.0}  # TFLOP/s, MI355X dense MFMA (fp32, bf
 # profiles/cfg2_fast_bf16_mfma_hbm_counters.md). Counters cannot be read
 # inside the timed run; a variant or batch without a measurement reports traffic null.
 TRAFFIC_BYTES_CFG2 = {
-    ("fp32", "kR->kq", 1152, "gridopt"): (3389105 * 2 + 629216) * 1024,  # staged kR->kq + rotations + g2 (VALU)
+    ("fp32", "kR->kq", 1152, "gridopt"): (3376891 * 2 + 638069) * 1024,  # staged kR->kq + rotations + g2 (VALU)
     ("fp32", "kR->kq", 1152, "gridopt_mfma"): (3536000 * 2 + 819800) * 1024,  # the same, stage 3 on MFMAs
     ("fp32", "RR->kR", 1152, "gridopt"): (204717 * 2 + 274026) * 1024,   # staged RR->kR projection launch (MFMA)
     ("fp32", "kR->kq", 768, "gridopt"): (2186567 * 2 + 414734) * 1024,
