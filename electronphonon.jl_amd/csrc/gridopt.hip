@@ -413,20 +413,50 @@ gridopt_kR_project_mfma_kernel(int64_t nd, const cplx<float> *__restrict__ op1, 
         // the line's units, consecutive (at most nbw per k, so at most max_pairs)
         const int u_first = kus[kb] - ku0, np_l = min(kus[ke] - kus[kb], max_pairs);
         float2 *ucl = uc + (size_t)lb * max_pairs * NW;
-        if (t < N) {
-            const cplx<float> v = ph2g[(l - lb0) * N + t];
-            ph2[lb][t] = make_float2(v.re, v.im);
+        // the line's table loads for the first round of each table issued together (one memory round trip
+        // per line for up to blockDim q3 phases and 2 blockDim U entries), then written to LDS; the unfolded
+        // N = 7 form, at its register bound (12 to 20 bytes of spills so), loads them table by table
+        const int n3 = nkl * N, nuc = np_l * NW, bd = blockDim.x;
+        int kus0 = 0, kus1 = 0;
+        int e3 = t, eu = t;  // where the remaining rounds of the q3 phases and U entries start
+        if constexpr (SYM || N != 7) {
+            cplx<float> a2{0.f, 0.f}, a3{0.f, 0.f}, au0{0.f, 0.f}, au1{0.f, 0.f};
+            if (t < N) a2 = ph2g[(l - lb0) * N + t];
+            if (t < n3) a3 = ph3g[(kb - k0) * N + t];
+            if (t < nuc) au0 = ucol[(int64_t)u_first * NW + t];
+            if (t + bd < nuc) au1 = ucol[(int64_t)u_first * NW + t + bd];
+            if (t < 64 && t < nkl) {
+                kus0 = kus[kb + t];
+                kus1 = kus[kb + t + 1];
+            }
+            if (t < N) ph2[lb][t] = make_float2(a2.re, a2.im);
+            if (t < n3) ph3[lb][t] = make_float2(a3.re, a3.im);
+            if (t < nuc) ucl[t] = make_float2(au0.re, au0.im);
+            if (t + bd < nuc) ucl[t + bd] = make_float2(au1.re, au1.im);
+            e3 = t + bd;
+            eu = t + 2 * bd;
+        } else {
+            if (t < N) {
+                const cplx<float> v = ph2g[(l - lb0) * N + t];
+                ph2[lb][t] = make_float2(v.re, v.im);
+            }
         }
-        for (int e = t; e < nkl * N; e += blockDim.x) {
+        for (int e = e3; e < n3; e += bd) {
             const cplx<float> v = ph3g[(kb - k0) * N + e];
             ph3[lb][e] = make_float2(v.re, v.im);
         }
-        for (int e = t; e < np_l * NW; e += blockDim.x) {
+        for (int e = eu; e < nuc; e += bd) {
             const cplx<float> v = ucol[(int64_t)u_first * NW + e];
             ucl[e] = make_float2(v.re, v.im);
         }
+        if constexpr (!(SYM || N != 7)) {
+            if (t < 64 && t < nkl) {
+                kus0 = kus[kb + t];
+                kus1 = kus[kb + t + 1];
+            }
+        }
         if (t < 64) {  // pair -> k: exclusive prefix of the band counts (<= 4) over the line's k
-            const int nbk = t < nkl ? min(kus[kb + t + 1] - kus[kb + t], nbw) : 0;
+            const int nbk = t < nkl ? min(kus1 - kus0, nbw) : 0;
             const unsigned long long below = (1ull << t) - 1ull;
             int start = 0;
 #pragma unroll
